@@ -1,0 +1,153 @@
+/*
+ * luce-render on MI355X: the C ABI that replaces luce-gpu plus the embedded
+ * SPIR-V/MSL kernels under luce-render's API. Plain pointers and sizes only.
+ *
+ * Status codes follow src/render/module.lucb:60-63: 0 ok, 400 invalid,
+ * 401 limit, 402 stopped; LR_UNAVAILABLE (503) when no GPU can be opened
+ * (luce-gpu's `gpu.unavailable`, which tests/gpu/main.lucb:18-22 turns into a
+ * skip). The message of the last failure on the calling thread is read with
+ * lr_last_error.
+ *
+ * Ownership: host arrays are borrowed for the call (uploads copy them);
+ * handles own their device memory; destroying a NULL handle is a no-op
+ * (device.lucb:119-121 destroys default handles the same way). A scene must
+ * outlive the renderers that use it. Calls on one handle are not re-entrant.
+ */
+#ifndef LUCE_RENDER_HIP_H
+#define LUCE_RENDER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LR_OK 0
+#define LR_INVALID 400
+#define LR_LIMIT 401
+#define LR_STOPPED 402
+#define LR_UNAVAILABLE 503
+#define LR_DEVICE_ERROR 500
+
+typedef struct lr_device lr_device;
+typedef struct lr_gpu_scene lr_gpu_scene;
+
+/* src/render/bvh.lucb:23-27: the 32-byte std430 node { vec3 lo; u32 first;
+ * vec3 hi; u32 count; }. Interior: count 0, children at first and first + 1;
+ * leaf: `count` triangles from leaf-order slot `first`. */
+typedef struct lr_bvh_node {
+    float lo[3];
+    uint32_t first;
+    float hi[3];
+    uint32_t count;
+} lr_bvh_node;
+
+/* The compiled scene's arrays (src/render/scene.lucb:31-99) as the host hands
+ * them over after build_bvh: positions xyz per point, three point indices per
+ * triangle in leaf order, each triangle's polygon, its shading words
+ * (3 octahedral normals + material), its six uv floats, and the binary BVH. */
+typedef struct lr_scene_arrays {
+    const float *positions;
+    size_t points;
+    const uint32_t *indices;
+    const uint32_t *faces;
+    const uint32_t *shading; /* 4 per triangle; may be NULL */
+    const float *uvs;        /* 6 per triangle; may be NULL */
+    size_t triangles;
+    const lr_bvh_node *nodes;
+    size_t node_count;
+} lr_scene_arrays;
+
+/* Replaces gpu.Device.open (luce-gpu, used at tests/gpu/main.lucb:18). */
+int lr_device_open(int ordinal, lr_device **out);
+void lr_device_close(lr_device *device);
+
+/* The message of the calling thread's last failure; returns its length. */
+size_t lr_last_error(char *buffer, size_t capacity);
+
+/* Replaces build_bvh (src/render/bvh.lucb:66-217): the binned-SAH binary BVH
+ * over `triangles` triangles, the same tree and leaf order the reference
+ * builds. `nodes` holds at least max(1, 2 * triangles) entries; `order`
+ * receives each leaf slot's original triangle; `threads` 0 picks the host's
+ * cores. The caller reorders its per-triangle arrays by `order`. */
+int lr_bvh_build(const float *positions, size_t points, const uint32_t *indices, size_t triangles,
+                 int threads, lr_bvh_node *nodes, size_t *node_count, uint32_t *order);
+
+/* Replaces GpuScene.upload (src/render/device.lucb:41-89). The binary BVH is
+ * repacked on upload into the traversal layouts (width 2, 4 and 8, 64/128/256
+ * byte nodes) and a leaf-order triangle array (p0, e1, e2). */
+int lr_scene_upload(lr_device *device, const lr_scene_arrays *arrays, lr_gpu_scene **out);
+/* GpuScene.destroy (device.lucb:108). */
+void lr_scene_destroy(lr_gpu_scene *scene);
+
+/* Replaces probe (src/render/device.lucb:138-160): one pixel-centre ray per
+ * pixel of the 16-float frame (camera.lucb:41-47), width x height results row
+ * by row from the top: polygon (0xffffffff on a miss) and distance along the
+ * frame's unnormalized direction. `count` must equal width * height. */
+int lr_probe(lr_device *device, const lr_gpu_scene *scene, const float frame[16], uint32_t width,
+             uint32_t height, uint32_t *face_out, float *t_out, size_t count);
+
+/* Traversal configuration: BVH width (2, 4, 8), workgroup size (64, 128, 256)
+ * and LDS stack entries per lane (24, 32, 48). No reference counterpart: the
+ * reference fixes width 2, 64-wide groups and a 32-entry stack (bvh.glsl:49). */
+typedef struct lr_trace_config {
+    uint32_t width;
+    uint32_t group;
+    uint32_t stack;
+} lr_trace_config;
+int lr_scene_set_trace_config(lr_gpu_scene *scene, const lr_trace_config *config);
+int lr_scene_get_trace_config(const lr_gpu_scene *scene, lr_trace_config *config);
+
+/* The probe with device-resident output (for timing and for callers that keep
+ * results on the GPU): hits[i] = {polygon, t bits}, as probe.comp:33 writes
+ * them. `stream` is a hipStream_t (NULL: the default stream); nothing waits. */
+int lr_probe_device(const lr_gpu_scene *scene, const float frame[16], uint32_t width, uint32_t height,
+                    uint32_t *hits_device, void *stream);
+
+/* Closest hits of n device-resident rays, 8 floats each (origin xyz, t_max,
+ * direction xyz, unused), into device hits[i] = {polygon, t bits}. The ray
+ * semantics are trace_closest's (bvh.glsl:51-90). */
+int lr_trace_closest_device(const lr_gpu_scene *scene, const float *rays_device, size_t n,
+                            uint32_t *hits_device, void *stream);
+
+/* Whether anything blocks each of n device-resident rays before its t_max
+ * (trace_any, bvh.glsl:93-122): occluded[i] = 1 or 0. */
+int lr_trace_any_device(const lr_gpu_scene *scene, const float *rays_device, size_t n,
+                        uint32_t *occluded_device, void *stream);
+
+/* Scene statistics for roofline accounting: node counts per width, bytes of
+ * each layout, triangle bytes, and the stack bound of each width. */
+typedef struct lr_scene_info {
+    size_t triangles;
+    size_t binary_nodes;
+    size_t wide_nodes[3];     /* widths 2, 4, 8 */
+    size_t wide_bytes[3];
+    size_t triangle_bytes;
+    uint32_t stack_bound[3];  /* deepest stack any ray can need, per width */
+} lr_scene_info;
+int lr_scene_info_get(const lr_gpu_scene *scene, lr_scene_info *info);
+
+/* Device buffers and timing: the part of luce-gpu (gpu.Buffer.allocate /
+ * create / read / upload, Compute.wait and timestamps; used at
+ * src/render/device.lucb:137-156 and integrator.lucb:848-885) that callers of
+ * the device-resident entry points need. Copies run on the device's stream
+ * and complete before returning. */
+int lr_buffer_alloc(lr_device *device, size_t bytes, void **out);
+void lr_buffer_free(lr_device *device, void *buffer);
+int lr_buffer_write(lr_device *device, void *dst_device, const void *src_host, size_t bytes);
+int lr_buffer_read(lr_device *device, void *dst_host, const void *src_device, size_t bytes);
+/* The device's stream (a hipStream_t) and a wait for all work on it. */
+void *lr_device_stream(lr_device *device);
+int lr_device_synchronize(lr_device *device);
+/* HIP events for timing work on a stream. */
+int lr_event_create(void **event);
+int lr_event_record(void *event, void *stream);
+int lr_event_elapsed(void *start, void *stop, float *ms);
+void lr_event_destroy(void *event);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,302 @@
+// The C ABI's device, scene and tracing entry points (include/luce_render_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "lr_internal.h"
+#include "lr_scene.h"
+#include "trace_kernels.h"
+
+namespace lr {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int fail(int code, const std::string &message) {
+    g_error = message;
+    return code;
+}
+
+int hip_fail(int status, const char *what) {
+    return fail(LR_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString((hipError_t)status));
+}
+
+}  // namespace lr
+
+using namespace lr;
+
+int lr_gpu_scene::ensure_spill(uint64_t threads, uint32_t stack) {
+    uint32_t bound = info.stack_bound[config_width_index()];
+    uint64_t depth = bound > stack ? bound - stack : 0;
+    uint64_t need = depth * threads;
+    if (need <= spill_entries) return 0;
+    if (spill) (void)hipFree(spill);
+    spill = nullptr;
+    spill_entries = 0;
+    hipError_t e = hipMalloc(&spill, need * 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(spill)");
+    spill_entries = need;
+    return 0;
+}
+
+int lr_gpu_scene::config_width_index() const { return config.width == 2 ? 0 : config.width == 4 ? 1 : 2; }
+
+lr::DeviceScene lr_gpu_scene::device_scene() const {
+    DeviceScene sc{};
+    int wi = config_width_index();
+    sc.nodes = static_cast<const uint8_t *>(wide[wi]);
+    sc.tris = static_cast<const float4 *>(tris);
+    sc.faces = static_cast<const uint32_t *>(faces);
+    std::memcpy(sc.root_lo, root_lo, 12);
+    std::memcpy(sc.root_hi, root_hi, 12);
+    sc.empty = info.triangles == 0 ? 1u : 0u;
+    sc.spill = static_cast<uint32_t *>(spill);
+    sc.spill_stride = 0;
+    sc.spill_depth = 0;
+    return sc;
+}
+
+const lr::TraceVariant &lr_gpu_scene::variant() const {
+    return trace_variants()[variant_index(config.width, config.stack, config.group)];
+}
+
+lr_gpu_scene::~lr_gpu_scene() {
+    for (void *p : {wide[0], wide[1], wide[2], tris, faces, shading, uvs, spill})
+        if (p) (void)hipFree(p);
+}
+
+
+namespace {
+
+template <class T> int upload(void **dst, const T *src, size_t count, const char *what) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 64);  // an empty array still gets a word (device.lucb:125-128)
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e != hipSuccess) return hip_fail(e, what);
+    if (count) {
+        e = hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, what);
+    } else {
+        e = hipMemset(*dst, 0, bytes);
+        if (e != hipSuccess) return hip_fail(e, what);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t lr_last_error(char *buffer, size_t capacity) {
+    const std::string &m = g_error;
+    if (buffer && capacity) {
+        size_t n = std::min(capacity - 1, m.size());
+        std::memcpy(buffer, m.data(), n);
+        buffer[n] = 0;
+    }
+    return m.size();
+}
+
+int lr_device_open(int ordinal, lr_device **out) {
+    if (!out) return fail(LR_INVALID, "lr_device_open needs an output handle");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(LR_UNAVAILABLE, "no GPU");
+    if (ordinal < 0 || ordinal >= count) return fail(LR_INVALID, "no GPU of that ordinal");
+    hipError_t e = hipSetDevice(ordinal);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    auto *d = new lr_device();
+    d->ordinal = ordinal;
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete d;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ordinal) == hipSuccess) d->compute_units = prop.multiProcessorCount;
+    *out = d;
+    return 0;
+}
+
+void lr_device_close(lr_device *device) {
+    if (!device) return;
+    (void)hipSetDevice(device->ordinal);
+    (void)hipStreamSynchronize(device->stream);
+    (void)hipStreamDestroy(device->stream);
+    delete device;
+}
+
+int lr_bvh_build(const float *positions, size_t points, const uint32_t *indices, size_t triangles, int threads,
+                 lr_bvh_node *nodes, size_t *node_count, uint32_t *order) {
+    if (!nodes || !node_count || (triangles && (!positions || !indices || !order)))
+        return fail(LR_INVALID, "lr_bvh_build needs positions, indices, nodes and order");
+    if (points > 4294967295ull) return fail(LR_LIMIT, "a scene holds at most 2^32 points");
+    if (triangles >= (1ull << 27)) return fail(LR_LIMIT, "a scene holds at most 2^27 triangles");
+    for (size_t i = 0; i < triangles * 3; i++)
+        if (indices[i] >= points) return fail(LR_INVALID, "a triangle names a point the scene lacks");
+    *node_count = build_binary_bvh(positions, indices, triangles, threads, nodes, order);
+    return 0;
+}
+
+int lr_scene_upload(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **out) {
+    if (!device || !a || !out) return fail(LR_INVALID, "lr_scene_upload needs a device, arrays and a handle");
+    *out = nullptr;
+    if (a->triangles && (!a->positions || !a->indices || !a->faces))
+        return fail(LR_INVALID, "a scene with triangles needs positions, indices and faces");
+    if (!a->nodes || a->node_count == 0) return fail(LR_INVALID, "a scene needs its BVH (one node at least)");
+    if (a->triangles >= (1ull << 27)) return fail(LR_LIMIT, "a scene holds at most 2^27 triangles");
+    if (a->triangles && a->node_count > 2 * a->triangles) return fail(LR_INVALID, "more BVH nodes than 2 per triangle");
+    // The nodes must describe a tree over exactly these triangles.
+    for (size_t i = 0; i < a->node_count; i++) {
+        const lr_bvh_node &n = a->nodes[i];
+        if (a->triangles == 0) break;
+        if (n.count == 0) {
+            if ((size_t)n.first + 1 >= a->node_count || n.first == 0) return fail(LR_INVALID, "a BVH node names a child past the array");
+        } else if ((size_t)n.first + n.count > a->triangles || n.count > 16) {
+            return fail(LR_INVALID, "a BVH leaf names triangles past the array or more than 16");
+        }
+    }
+    for (size_t i = 0; i < a->triangles * 3; i++)
+        if (a->indices[i] >= a->points) return fail(LR_INVALID, "a triangle names a point the scene lacks");
+    (void)hipSetDevice(device->ordinal);
+    std::unique_ptr<lr_gpu_scene> s(new lr_gpu_scene());
+    s->device = device;
+    s->info.triangles = a->triangles;
+    s->info.binary_nodes = a->node_count;
+    std::memcpy(s->root_lo, a->nodes[0].lo, 12);
+    std::memcpy(s->root_hi, a->nodes[0].hi, 12);
+    // Leaf-order triangles as p0, e1 = p1 - p0, e2 = p2 - p0 (f32, as
+    // triangle_hit computes them, bvh.glsl:28-30).
+    std::vector<float> tri(std::max<size_t>(a->triangles, 1) * 12, 0.0f);
+    for (size_t t = 0; t < a->triangles; t++) {
+        const float *p0 = a->positions + (size_t)a->indices[t * 3] * 3;
+        const float *p1 = a->positions + (size_t)a->indices[t * 3 + 1] * 3;
+        const float *p2 = a->positions + (size_t)a->indices[t * 3 + 2] * 3;
+        float *o = &tri[t * 12];
+        o[0] = p0[0];
+        o[1] = p0[1];
+        o[2] = p0[2];
+        o[3] = p1[0] - p0[0];
+        o[4] = p1[1] - p0[1];
+        o[5] = p1[2] - p0[2];
+        o[6] = p2[0] - p0[0];
+        o[7] = p2[1] - p0[1];
+        o[8] = p2[2] - p0[2];
+    }
+    int rc;
+    if ((rc = upload(&s->tris, tri.data(), a->triangles * 12, "upload triangles"))) return rc;
+    if ((rc = upload(&s->faces, a->faces, a->triangles, "upload faces"))) return rc;
+    if (a->shading && (rc = upload(&s->shading, a->shading, a->triangles * 4, "upload shading"))) return rc;
+    if (a->uvs && (rc = upload(&s->uvs, a->uvs, a->triangles * 6, "upload uvs"))) return rc;
+    const int widths[3] = {2, 4, 8};
+    for (int wi = 0; wi < 3; wi++) {
+        WideLayout layout = collapse_wide(a->nodes, a->node_count, widths[wi]);
+        s->info.wide_nodes[wi] = layout.nodes;
+        s->info.wide_bytes[wi] = layout.bytes.size();
+        s->info.stack_bound[wi] = layout.stack_bound;
+        if ((rc = upload(&s->wide[wi], layout.bytes.data(), layout.bytes.size(), "upload wide nodes"))) return rc;
+    }
+    s->info.triangle_bytes = a->triangles * 48;
+    // Default traversal: width 4, 32-entry LDS stack, 128-wide groups.
+    s->config = lr_trace_config{4, 128, 32};
+    *out = s.release();
+    return 0;
+}
+
+void lr_scene_destroy(lr_gpu_scene *scene) {
+    if (!scene) return;
+    (void)hipSetDevice(scene->device->ordinal);
+    (void)hipStreamSynchronize(scene->device->stream);
+    delete scene;
+}
+
+int lr_scene_set_trace_config(lr_gpu_scene *scene, const lr_trace_config *config) {
+    if (!scene || !config) return fail(LR_INVALID, "lr_scene_set_trace_config needs a scene and a config");
+    if (variant_index(config->width, config->stack, config->group) < 0)
+        return fail(LR_INVALID, "traversal widths are 2, 4, 8; stacks 24, 32, 48; groups 64, 128, 256");
+    scene->config = *config;
+    return 0;
+}
+
+int lr_scene_get_trace_config(const lr_gpu_scene *scene, lr_trace_config *config) {
+    if (!scene || !config) return fail(LR_INVALID, "lr_scene_get_trace_config needs a scene and a config");
+    *config = scene->config;
+    return 0;
+}
+
+int lr_scene_info_get(const lr_gpu_scene *scene, lr_scene_info *info) {
+    if (!scene || !info) return fail(LR_INVALID, "lr_scene_info_get needs a scene");
+    *info = scene->info;
+    return 0;
+}
+
+int lr_probe_device(const lr_gpu_scene *scene_c, const float frame[16], uint32_t width, uint32_t height,
+                    uint32_t *hits_device, void *stream) {
+    lr_gpu_scene *scene = const_cast<lr_gpu_scene *>(scene_c);
+    if (!scene || !frame || !hits_device) return fail(LR_INVALID, "lr_probe_device needs a scene, a frame and output");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(LR_INVALID, "an image is 1..16384 pixels on a side");
+    const TraceVariant &v = scene->variant();
+    int rc = scene->ensure_spill(launch_threads(probe_items(width, height), scene->config.group), scene->config.stack);
+    if (rc) return rc;
+    DeviceScene sc = scene->device_scene();
+    int e = v.probe(sc, frame, width, height, reinterpret_cast<uint2 *>(hits_device), (hipStream_t)stream);
+    if (e) return hip_fail(e, "probe launch");
+    return 0;
+}
+
+int lr_probe(lr_device *device, const lr_gpu_scene *scene, const float frame[16], uint32_t width, uint32_t height,
+             uint32_t *face_out, float *t_out, size_t count) {
+    if (!device || !scene || !frame || !face_out || !t_out) return fail(LR_INVALID, "lr_probe needs its arguments");
+    if ((size_t)width * height != count) return fail(LR_INVALID, "a probe needs one result per pixel");  // device.lucb:139-140
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(LR_INVALID, "an image is 1..16384 pixels on a side");
+    (void)hipSetDevice(device->ordinal);
+    void *hits = nullptr;
+    hipError_t e = hipMalloc(&hits, count * 8);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(probe results)");
+    int rc = lr_probe_device(scene, frame, width, height, static_cast<uint32_t *>(hits), device->stream);
+    std::vector<uint32_t> host(count * 2);
+    if (!rc) {
+        e = hipMemcpyAsync(host.data(), hits, count * 8, hipMemcpyDeviceToHost, device->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(device->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "probe readback");
+    }
+    (void)hipFree(hits);
+    if (rc) return rc;
+    for (size_t i = 0; i < count; i++) {
+        face_out[i] = host[i * 2];
+        std::memcpy(&t_out[i], &host[i * 2 + 1], 4);
+    }
+    return 0;
+}
+
+int lr_trace_closest_device(const lr_gpu_scene *scene_c, const float *rays, size_t n, uint32_t *hits, void *stream) {
+    lr_gpu_scene *scene = const_cast<lr_gpu_scene *>(scene_c);
+    if (!scene || (n && (!rays || !hits))) return fail(LR_INVALID, "lr_trace_closest_device needs its arguments");
+    if (n > 0xffffff00ull) return fail(LR_LIMIT, "at most 2^32 - 256 rays a call");
+    int rc = scene->ensure_spill(launch_threads(n, scene->config.group), scene->config.stack);
+    if (rc) return rc;
+    int e = scene->variant().closest(scene->device_scene(), reinterpret_cast<const float4 *>(rays), (uint32_t)n,
+                                     reinterpret_cast<uint2 *>(hits), (hipStream_t)stream);
+    if (e) return hip_fail(e, "closest launch");
+    return 0;
+}
+
+int lr_trace_any_device(const lr_gpu_scene *scene_c, const float *rays, size_t n, uint32_t *occluded, void *stream) {
+    lr_gpu_scene *scene = const_cast<lr_gpu_scene *>(scene_c);
+    if (!scene || (n && (!rays || !occluded))) return fail(LR_INVALID, "lr_trace_any_device needs its arguments");
+    if (n > 0xffffff00ull) return fail(LR_LIMIT, "at most 2^32 - 256 rays a call");
+    int rc = scene->ensure_spill(launch_threads(n, scene->config.group), scene->config.stack);
+    if (rc) return rc;
+    int e = scene->variant().any(scene->device_scene(), reinterpret_cast<const float4 *>(rays), (uint32_t)n, occluded,
+                                 (hipStream_t)stream);
+    if (e) return hip_fail(e, "any-hit launch");
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+// Device-side BVH traversal over the wide layouts (bvh_wide.cpp), shared by
+// the probe, ray-batch, shadow and integrator kernels.
+//
+// Semantics follow shaders/bvh.glsl:
+//   * the slab test box_entry (bvh.glsl:17-25) with the entry clamped at 0 and
+//     the exit at the current best t, a child taken when enter <= leave;
+//   * Moller-Trumbore triangle_hit (bvh.glsl:27-47): |det| < 1e-20 rejected,
+//     u outside [0,1], v < 0 or u + v > 1 rejected, t <= 0 or t >= best
+//     rejected (the first triangle found at a given t keeps it);
+//   * axis-parallel directions replaced by +-1e-20 before inverting
+//     (bvh.glsl:55-56);
+//   * closest hit: hit children visited nearest first (ties: lower child
+//     index), the others pushed farthest first (bvh.glsl:73-84); with width 2
+//     this is the reference's visiting order exactly;
+//   * any hit: children taken in child order, the first triangle hit ends it
+//     (bvh.glsl:93-122).
+// Unlike bvh.glsl:79 no stack entry is ever dropped: entries past the LDS
+// short stack spill to a global area sized by the tree's exact stack bound.
+//
+// All arithmetic is f32 in the reference's operation order; the library is
+// compiled with -ffp-contract=off so no multiply-add is fused, and divisions
+// and square roots are IEEE correctly rounded (HIP's default).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lr {
+
+struct DeviceScene {
+    const uint8_t *nodes;    // wide nodes, node_bytes each
+    const float4 *tris;      // 3 float4 per leaf slot: p0.xyz e1.x | e1.yz e2.xy | e2.z - - -
+    const uint32_t *faces;   // polygon per leaf slot
+    float root_lo[3];
+    float root_hi[3];
+    uint32_t empty;          // an empty tree: every ray misses
+    uint32_t *spill;         // overflow stack entries, [depth - S][thread]
+    uint32_t spill_stride;   // threads the launch covers
+    uint32_t spill_depth;    // entries per thread the spill area holds
+};
+
+struct Hit {
+    float t;
+    uint32_t slot;  // leaf-order triangle, 0xffffffff when missed
+    float u, v;
+};
+
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kLeafFlag = 0x80000000u;
+
+__device__ __forceinline__ uint32_t leaf_entry(uint32_t first, uint32_t count) {
+    return kLeafFlag | ((count - 1u) << 27) | first;
+}
+
+__device__ __forceinline__ float safe_inverse(float d) {
+    float s = d;
+    if (fabsf(d) < 1e-20f) {
+        float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+        s = sg * 1e-20f + (d == 0.0f ? 1.0f : 0.0f) * 1e-20f;
+    }
+    return 1.0f / s;
+}
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    float s = ax * bx;
+    s = s + ay * by;
+    return s + az * bz;
+}
+
+// box_entry (bvh.glsl:17-25); -1 when missed.
+__device__ __forceinline__ float box_entry(float lox, float loy, float loz, float hix, float hiy, float hiz,
+                                           float ox, float oy, float oz, float ix, float iy, float iz, float tmax) {
+    float ax = (lox - ox) * ix, ay = (loy - oy) * iy, az = (loz - oz) * iz;
+    float bx = (hix - ox) * ix, by = (hiy - oy) * iy, bz = (hiz - oz) * iz;
+    float enter = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+    float leave = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+    return enter <= leave ? enter : -1.0f;
+}
+
+// triangle_hit (bvh.glsl:27-47) on a repacked triangle.
+__device__ __forceinline__ bool triangle_hit(const float4 *tris, uint32_t slot, float ox, float oy, float oz,
+                                             float dx, float dy, float dz, Hit &hit) {
+    const float4 a = tris[(size_t)slot * 3];
+    const float4 b = tris[(size_t)slot * 3 + 1];
+    const float4 c = tris[(size_t)slot * 3 + 2];
+    float e1x = a.w, e1y = b.x, e1z = b.y;
+    float e2x = b.z, e2y = b.w, e2z = c.x;
+    float px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
+    float det = dot3(e1x, e1y, e1z, px, py, pz);
+    if (fabsf(det) < 1e-20f) return false;
+    float inv = 1.0f / det;
+    float sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
+    float u = dot3(sx, sy, sz, px, py, pz) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+    float v = dot3(dx, dy, dz, qx, qy, qz) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = dot3(e2x, e2y, e2z, qx, qy, qz) * inv;
+    if (t <= 0.0f || t >= hit.t) return false;
+    hit.t = t;
+    hit.slot = slot;
+    hit.u = u;
+    hit.v = v;
+    return true;
+}
+
+template <int W> struct NodeView {
+    static constexpr int kQuads = (W == 2) ? 4 : (W == 4 ? 8 : 16);
+    float4 raw[kQuads];
+    __device__ __forceinline__ void load(const uint8_t *nodes, uint32_t index) {
+        const float4 *p = reinterpret_cast<const float4 *>(nodes + (size_t)index * (kQuads * 16));
+#pragma unroll
+        for (int q = 0; q < kQuads; q++) raw[q] = p[q];
+    }
+    __device__ __forceinline__ float f(int i) const {
+        const float *x = reinterpret_cast<const float *>(raw);
+        return x[i];
+    }
+    __device__ __forceinline__ uint32_t u(int i) const {
+        const uint32_t *x = reinterpret_cast<const uint32_t *>(raw);
+        return x[i];
+    }
+    __device__ __forceinline__ float lo_x(int c) const { return f(c); }
+    __device__ __forceinline__ float hi_x(int c) const { return f(W + c); }
+    __device__ __forceinline__ float lo_y(int c) const { return f(2 * W + c); }
+    __device__ __forceinline__ float hi_y(int c) const { return f(3 * W + c); }
+    __device__ __forceinline__ float lo_z(int c) const { return f(4 * W + c); }
+    __device__ __forceinline__ float hi_z(int c) const { return f(5 * W + c); }
+    __device__ __forceinline__ uint32_t child(int c) const { return u(6 * W + c); }
+    __device__ __forceinline__ uint32_t count(int c) const { return (u(7 * W + c / 4) >> (8 * (c % 4))) & 0xffu; }
+};
+
+// The per-lane stack: S entries in LDS (lane-interleaved, `stride` apart),
+// the rest in the global spill area.
+template <int S> struct LaneStack {
+    uint32_t *lds;
+    uint32_t *spill;
+    uint32_t spill_stride;
+    int sp;
+    __device__ __forceinline__ void push(uint32_t e, int stride) {
+        if (sp < S) lds[sp * stride] = e;
+        else spill[(size_t)(sp - S) * spill_stride] = e;
+        sp++;
+    }
+    __device__ __forceinline__ uint32_t pop(int stride) {
+        sp--;
+        return sp < S ? lds[sp * stride] : spill[(size_t)(sp - S) * spill_stride];
+    }
+};
+
+// trace_closest (bvh.glsl:51-90) over width-W nodes.
+template <int W, int S>
+__device__ __forceinline__ Hit trace_closest(const DeviceScene &sc, float ox, float oy, float oz, float dx, float dy,
+                                             float dz, float tmax, uint32_t *lds, int stride, uint32_t thread) {
+    Hit hit{tmax, kNone, 0.0f, 0.0f};
+    float ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+    if (sc.empty || sc.root_lo[0] > sc.root_hi[0] ||
+        box_entry(sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2], ox, oy, oz,
+                  ix, iy, iz, hit.t) < 0.0f)
+        return hit;
+    LaneStack<S> stack{lds, sc.spill + thread, sc.spill_stride, 0};
+    uint32_t cur = 0;
+    for (;;) {
+        if (cur & kLeafFlag) {
+            uint32_t first = cur & 0x07ffffffu, count = ((cur >> 27) & 0xfu) + 1u;
+            for (uint32_t i = 0; i < count; i++) triangle_hit(sc.tris, first + i, ox, oy, oz, dx, dy, dz, hit);
+        } else {
+            NodeView<W> n;
+            n.load(sc.nodes, cur);
+            float dist[W];
+            uint32_t ent[W];
+            int key[W];
+            int hits = 0;
+#pragma unroll
+            for (int c = 0; c < W; c++) {
+                uint32_t ch = n.child(c);
+                uint32_t cnt = n.count(c);
+                float e = -1.0f;
+                if (ch != kNone)
+                    e = box_entry(n.lo_x(c), n.lo_y(c), n.lo_z(c), n.hi_x(c), n.hi_y(c), n.hi_z(c), ox, oy, oz, ix, iy,
+                                  iz, hit.t);
+                bool take = e >= 0.0f;
+                hits += take ? 1 : 0;
+                dist[c] = take ? e : __builtin_huge_valf();
+                ent[c] = cnt ? leaf_entry(ch, cnt) : ch;
+                key[c] = c;
+            }
+            if (hits > 0) {
+                // Sort by (entry distance, child index): a stable order.
+#pragma unroll
+                for (int i = 1; i < W; i++) {
+#pragma unroll
+                    for (int j = W - 1; j >= i; j--) {
+                        bool sw = dist[j] < dist[j - 1] || (dist[j] == dist[j - 1] && key[j] < key[j - 1]);
+                        float td = dist[j];
+                        uint32_t te = ent[j];
+                        int tk = key[j];
+                        dist[j] = sw ? dist[j - 1] : dist[j];
+                        ent[j] = sw ? ent[j - 1] : ent[j];
+                        key[j] = sw ? key[j - 1] : key[j];
+                        dist[j - 1] = sw ? td : dist[j - 1];
+                        ent[j - 1] = sw ? te : ent[j - 1];
+                        key[j - 1] = sw ? tk : key[j - 1];
+                    }
+                }
+#pragma unroll
+                for (int i = W - 1; i >= 1; i--)
+                    if (i < hits) stack.push(ent[i], stride);
+                cur = ent[0];
+                continue;
+            }
+        }
+        if (stack.sp == 0) break;
+        cur = stack.pop(stride);
+    }
+    return hit;
+}
+
+// trace_any (bvh.glsl:93-122): whether anything blocks the ray before tmax.
+template <int W, int S>
+__device__ __forceinline__ bool trace_any(const DeviceScene &sc, float ox, float oy, float oz, float dx, float dy,
+                                          float dz, float tmax, uint32_t *lds, int stride, uint32_t thread) {
+    Hit hit{tmax, kNone, 0.0f, 0.0f};
+    float ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+    if (sc.empty || sc.root_lo[0] > sc.root_hi[0] ||
+        box_entry(sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2], ox, oy, oz,
+                  ix, iy, iz, tmax) < 0.0f)
+        return false;
+    LaneStack<S> stack{lds, sc.spill + thread, sc.spill_stride, 0};
+    uint32_t cur = 0;
+    for (;;) {
+        if (cur & kLeafFlag) {
+            uint32_t first = cur & 0x07ffffffu, count = ((cur >> 27) & 0xfu) + 1u;
+            for (uint32_t i = 0; i < count; i++)
+                if (triangle_hit(sc.tris, first + i, ox, oy, oz, dx, dy, dz, hit)) return true;
+        } else {
+            NodeView<W> n;
+            n.load(sc.nodes, cur);
+            uint32_t ent[W];
+            bool take[W];
+            int hits = 0;
+#pragma unroll
+            for (int c = 0; c < W; c++) {
+                uint32_t ch = n.child(c);
+                uint32_t cnt = n.count(c);
+                take[c] = ch != kNone && box_entry(n.lo_x(c), n.lo_y(c), n.lo_z(c), n.hi_x(c), n.hi_y(c), n.hi_z(c), ox,
+                                                   oy, oz, ix, iy, iz, tmax) >= 0.0f;
+                hits += take[c] ? 1 : 0;
+                ent[c] = cnt ? leaf_entry(ch, cnt) : ch;
+            }
+            if (hits > 0) {
+                // Children in order: the first taken is visited, the others
+                // pushed so that they pop in child order.
+                uint32_t next = kNone;
+#pragma unroll
+                for (int c = W - 1; c >= 0; c--) {
+                    if (!take[c]) continue;
+                    if (next != kNone) stack.push(next, stride);
+                    next = ent[c];
+                }
+                cur = next;
+                continue;
+            }
+        }
+        if (stack.sp == 0) break;
+        cur = stack.pop(stride);
+    }
+    return false;
+}
+
+}  // namespace lr

@@ -1,0 +1,170 @@
+// Tracing kernels: the primary-ray probe (shaders/probe.comp), closest hits of
+// ray batches and any-hit shadow tests, each instantiated per BVH width W,
+// LDS stack depth S and workgroup size G so the autotuner can pick a variant.
+#include "trace_kernels.h"
+
+// Kernels receive the scene by value; each launcher stamps the spill stride
+// (the threads it launches) before launching.
+
+namespace lr {
+
+struct Frame {
+    float v[16];
+};
+
+// probe.comp:26-30: the pixel-centre ray, f32 in the reference's order.
+__device__ __forceinline__ void probe_ray(const Frame &f, uint32_t x, uint32_t y, float &ox, float &oy, float &oz,
+                                          float &dx, float &dy, float &dz) {
+    float fx = (float)x + 0.5f, fy = (float)y + 0.5f;
+    float filmx = f.v[4] + fx * f.v[8];
+    filmx = filmx + fy * f.v[12];
+    float filmy = f.v[5] + fx * f.v[9];
+    filmy = filmy + fy * f.v[13];
+    float filmz = f.v[6] + fx * f.v[10];
+    filmz = filmz + fy * f.v[14];
+    if (f.v[3] > 0.5f) {
+        ox = f.v[0] + filmx;
+        oy = f.v[1] + filmy;
+        oz = f.v[2] + filmz;
+        float cx = f.v[9] * f.v[14] - f.v[10] * f.v[13];
+        float cy = f.v[10] * f.v[12] - f.v[8] * f.v[14];
+        float cz = f.v[8] * f.v[13] - f.v[9] * f.v[12];
+        float l = sqrtf(dot3(cx, cy, cz, cx, cy, cz));
+        dx = cx / l;
+        dy = cy / l;
+        dz = cz / l;
+    } else {
+        ox = f.v[0];
+        oy = f.v[1];
+        oz = f.v[2];
+        dx = filmx;
+        dy = filmy;
+        dz = filmz;
+    }
+}
+
+// Workgroups go to the 8 XCDs round-robin; renumber them so neighbouring
+// tiles (which share BVH nodes) land on the same XCD and its L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t block, uint32_t blocks) {
+    uint32_t per = blocks / 8u;
+    if (per == 0u || block >= per * 8u) return block;
+    return (block % 8u) * per + block / 8u;
+}
+
+template <int W, int S, int G>
+__global__ __launch_bounds__(G) void probe_kernel(DeviceScene sc, Frame f, uint32_t width, uint32_t height,
+                                                  uint2 *out) {
+    __shared__ uint32_t lds[S * G];
+    uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t gid = block * G + threadIdx.x;
+    // 8x8 pixel tiles per 64 lanes, tiles row-major.
+    uint32_t tile = gid >> 6, lane = gid & 63u;
+    uint32_t tiles_x = (width + 7u) / 8u;
+    uint32_t x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
+    if (x >= width || y >= height) return;
+    float ox, oy, oz, dx, dy, dz;
+    probe_ray(f, x, y, ox, oy, oz, dx, dy, dz);
+    Hit h = trace_closest<W, S>(sc, ox, oy, oz, dx, dy, dz, 3.0e38f, lds + threadIdx.x, G, blockIdx.x * G + threadIdx.x);
+    uint32_t face = h.slot == kNone ? kNone : sc.faces[h.slot];
+    out[(size_t)y * width + x] = make_uint2(face, __float_as_uint(h.t));
+}
+
+template <int W, int S, int G>
+__global__ __launch_bounds__(G) void closest_kernel(DeviceScene sc, const float4 *rays, uint32_t n, uint2 *out) {
+    __shared__ uint32_t lds[S * G];
+    uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t i = block * G + threadIdx.x;
+    if (i >= n) return;
+    float4 a = rays[(size_t)i * 2], b = rays[(size_t)i * 2 + 1];
+    Hit h = trace_closest<W, S>(sc, a.x, a.y, a.z, b.x, b.y, b.z, a.w, lds + threadIdx.x, G, blockIdx.x * G + threadIdx.x);
+    uint32_t face = h.slot == kNone ? kNone : sc.faces[h.slot];
+    out[i] = make_uint2(face, __float_as_uint(h.t));
+}
+
+template <int W, int S, int G>
+__global__ __launch_bounds__(G) void any_kernel(DeviceScene sc, const float4 *rays, uint32_t n, uint32_t *out) {
+    __shared__ uint32_t lds[S * G];
+    uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t i = block * G + threadIdx.x;
+    if (i >= n) return;
+    float4 a = rays[(size_t)i * 2], b = rays[(size_t)i * 2 + 1];
+    out[i] = trace_any<W, S>(sc, a.x, a.y, a.z, b.x, b.y, b.z, a.w, lds + threadIdx.x, G, blockIdx.x * G + threadIdx.x)
+                 ? 1u
+                 : 0u;
+}
+
+namespace {
+
+template <int W, int S, int G> int launch_probe_t(const DeviceScene &scene, const float *frame, uint32_t w, uint32_t h,
+                                                   uint2 *out, hipStream_t stream) {
+    Frame f;
+    for (int k = 0; k < 16; k++) f.v[k] = frame[k];
+    uint32_t blocks = (uint32_t)(launch_threads(probe_items(w, h), G) / G);
+    if (blocks == 0) return 0;
+    DeviceScene sc = scene;
+    sc.spill_stride = blocks * G;
+    hipLaunchKernelGGL((probe_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, f, w, h, out);
+    return (int)hipGetLastError();
+}
+
+template <int W, int S, int G> int launch_closest_t(const DeviceScene &scene, const float4 *rays, uint32_t n,
+                                                     uint2 *out, hipStream_t stream) {
+    if (n == 0) return 0;
+    uint32_t blocks = (n + G - 1) / G;
+    DeviceScene sc = scene;
+    sc.spill_stride = blocks * G;
+    hipLaunchKernelGGL((closest_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, rays, n, out);
+    return (int)hipGetLastError();
+}
+
+template <int W, int S, int G> int launch_any_t(const DeviceScene &scene, const float4 *rays, uint32_t n, uint32_t *out,
+                                                 hipStream_t stream) {
+    if (n == 0) return 0;
+    uint32_t blocks = (n + G - 1) / G;
+    DeviceScene sc = scene;
+    sc.spill_stride = blocks * G;
+    hipLaunchKernelGGL((any_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, rays, n, out);
+    return (int)hipGetLastError();
+}
+
+template <int W, int S, int G> void fill(TraceVariant &v) {
+    v.probe = &launch_probe_t<W, S, G>;
+    v.closest = &launch_closest_t<W, S, G>;
+    v.any = &launch_any_t<W, S, G>;
+}
+
+template <int W, int S> void fill_g(TraceVariant *table, int wi, int si) {
+    fill<W, S, 64>(table[(wi * 3 + si) * 3 + 0]);
+    fill<W, S, 128>(table[(wi * 3 + si) * 3 + 1]);
+    fill<W, S, 256>(table[(wi * 3 + si) * 3 + 2]);
+}
+
+template <int W> void fill_s(TraceVariant *table, int wi) {
+    fill_g<W, 24>(table, wi, 0);
+    fill_g<W, 32>(table, wi, 1);
+    fill_g<W, 48>(table, wi, 2);
+}
+
+}  // namespace
+
+const TraceVariant *trace_variants() {
+    static TraceVariant table[27];
+    static bool ready = false;
+    if (!ready) {
+        fill_s<2>(table, 0);
+        fill_s<4>(table, 1);
+        fill_s<8>(table, 2);
+        ready = true;
+    }
+    return table;
+}
+
+int variant_index(uint32_t width, uint32_t stack, uint32_t group) {
+    int wi = width == 2 ? 0 : width == 4 ? 1 : width == 8 ? 2 : -1;
+    int si = stack == 24 ? 0 : stack == 32 ? 1 : stack == 48 ? 2 : -1;
+    int gi = group == 64 ? 0 : group == 128 ? 1 : group == 256 ? 2 : -1;
+    if (wi < 0 || si < 0 || gi < 0) return -1;
+    return (wi * 3 + si) * 3 + gi;
+}
+
+}  // namespace lr

@@ -1,0 +1,120 @@
+"""ctypes binding of liblrhip.so (include/luce_render_hip.h).
+
+The library is built in-tree by luce-render_amd/build.py. Loading fails loudly
+when it is missing: there is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liblrhip.so"
+
+OK, INVALID, LIMIT, STOPPED, DEVICE_ERROR, UNAVAILABLE = 0, 400, 401, 402, 500, 503
+
+
+class LuceError(Exception):
+    """A failed call: `code` as src/render/module.lucb:60-63 names them."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class GpuUnavailable(LuceError):
+    pass
+
+
+class BvhNode(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_float * 3), ("first", ctypes.c_uint32),
+                ("hi", ctypes.c_float * 3), ("count", ctypes.c_uint32)]
+
+
+class SceneArrays(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("points", ctypes.c_size_t),
+                ("indices", ctypes.c_void_p), ("faces", ctypes.c_void_p),
+                ("shading", ctypes.c_void_p), ("uvs", ctypes.c_void_p),
+                ("triangles", ctypes.c_size_t), ("nodes", ctypes.c_void_p),
+                ("node_count", ctypes.c_size_t)]
+
+
+class TraceConfig(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("group", ctypes.c_uint32), ("stack", ctypes.c_uint32)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("triangles", ctypes.c_size_t), ("binary_nodes", ctypes.c_size_t),
+                ("wide_nodes", ctypes.c_size_t * 3), ("wide_bytes", ctypes.c_size_t * 3),
+                ("triangle_bytes", ctypes.c_size_t), ("stack_bound", ctypes.c_uint32 * 3)]
+
+
+_P = ctypes.c_void_p
+_SIGNATURES = {
+    "lr_device_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "lr_device_close": (None, [_P]),
+    "lr_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
+    "lr_bvh_build": (ctypes.c_int, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.c_int, _P,
+                                    ctypes.POINTER(ctypes.c_size_t), _P]),
+    "lr_scene_upload": (ctypes.c_int, [_P, ctypes.POINTER(SceneArrays), ctypes.POINTER(_P)]),
+    "lr_scene_destroy": (None, [_P]),
+    "lr_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, ctypes.c_size_t]),
+    "lr_scene_set_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
+    "lr_scene_get_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
+    "lr_probe_device": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P]),
+    "lr_trace_closest_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
+    "lr_trace_any_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
+    "lr_scene_info_get": (ctypes.c_int, [_P, ctypes.POINTER(SceneInfo)]),
+    "lr_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "lr_buffer_free": (None, [_P, _P]),
+    "lr_buffer_write": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "lr_buffer_read": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "lr_device_stream": (_P, [_P]),
+    "lr_device_synchronize": (ctypes.c_int, [_P]),
+    "lr_event_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "lr_event_record": (ctypes.c_int, [_P, _P]),
+    "lr_event_elapsed": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
+    "lr_event_destroy": (None, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is not built: run python luce-render_amd/build.py")
+        handle = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | ctypes.RTLD_GLOBAL)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_names():
+    return list(_SIGNATURES)
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    lib().lr_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(code):
+    if code == OK:
+        return
+    message = last_error()
+    if code == UNAVAILABLE:
+        raise GpuUnavailable(code, message)
+    raise LuceError(code, message)
+
+
+def ptr(array):
+    """The data pointer of a numpy array (None for None)."""
+    if array is None:
+        return None
+    return array.ctypes.data

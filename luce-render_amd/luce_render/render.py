@@ -1,0 +1,298 @@
+"""luce-render's host API over the HIP library: Device, GpuScene.upload,
+ray_frame / look_frame, probe (src/render/device.lucb, camera.lucb).
+
+    from luce_render import render
+    device = render.Device.open()
+    scene = render.compile_scene(set)
+    on_gpu = render.GpuScene.upload(device, scene)
+    frame = render.ray_frame(scene.cameras[0], 256, 256)
+    probes = render.probe(device, on_gpu, frame)
+"""
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import INVALID, GpuUnavailable, LuceError, TraceConfig, SceneInfo
+from .scene import (Camera, CompiledScene, GeometrySet, Instance, Light, Mesh, Stopping, build_bvh,
+                    compile_scene, octahedral, decode_octahedral)
+
+__all__ = ["Device", "GpuScene", "RayFrame", "ray_frame", "look_frame", "probe", "Probe", "compile_scene",
+           "build_bvh", "GeometrySet", "Mesh", "Camera", "Light", "Instance", "Stopping", "CompiledScene",
+           "LuceError", "GpuUnavailable", "octahedral", "decode_octahedral"]
+
+
+class Device:
+    """gpu.Device: one GPU and its stream."""
+
+    def __init__(self, handle, ordinal):
+        self.handle = handle
+        self.ordinal = ordinal
+
+    @staticmethod
+    def open(ordinal=0):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().lr_device_open(int(ordinal), ctypes.byref(h)))
+        return Device(h, ordinal)
+
+    def destroy(self):
+        if self.handle:
+            _lib.lib().lr_device_close(self.handle)
+            self.handle = None
+
+
+class GpuScene:
+    """GpuScene (device.lucb:16-37): the scene's buffers on a device."""
+
+    def __init__(self, handle, device, triangles):
+        self.handle = handle
+        self.device = device
+        self.triangles = triangles
+
+    @staticmethod
+    def upload(device, scene: CompiledScene, accelerate=True):
+        """GpuScene.upload (device.lucb:41-89); `accelerate` is accepted for the
+        reference's signature (there is no hardware ray tracing here)."""
+        arrays = _lib.SceneArrays()
+        arrays.positions = _lib.ptr(scene.positions)
+        arrays.points = scene.point_count()
+        arrays.indices = _lib.ptr(scene.indices)
+        arrays.faces = _lib.ptr(scene.faces)
+        arrays.shading = _lib.ptr(scene.shading)
+        arrays.uvs = _lib.ptr(scene.uvs)
+        arrays.triangles = scene.triangle_count()
+        arrays.nodes = _lib.ptr(scene.nodes)
+        arrays.node_count = len(scene.nodes)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().lr_scene_upload(device.handle, ctypes.byref(arrays), ctypes.byref(h)))
+        return GpuScene(h, device, scene.triangle_count())
+
+    def set_trace_config(self, width=4, group=128, stack=32):
+        cfg = TraceConfig(width, group, stack)
+        _lib.check(_lib.lib().lr_scene_set_trace_config(self.handle, ctypes.byref(cfg)))
+
+    def trace_config(self):
+        cfg = TraceConfig()
+        _lib.check(_lib.lib().lr_scene_get_trace_config(self.handle, ctypes.byref(cfg)))
+        return (cfg.width, cfg.group, cfg.stack)
+
+    def info(self):
+        info = SceneInfo()
+        _lib.check(_lib.lib().lr_scene_info_get(self.handle, ctypes.byref(info)))
+        return {
+            "triangles": info.triangles, "binary_nodes": info.binary_nodes,
+            "wide_nodes": list(info.wide_nodes), "wide_bytes": list(info.wide_bytes),
+            "triangle_bytes": info.triangle_bytes, "stack_bound": list(info.stack_bound),
+        }
+
+    def destroy(self):
+        if self.handle:
+            _lib.lib().lr_scene_destroy(self.handle)
+            self.handle = None
+
+
+@dataclass
+class RayFrame:
+    """RayFrame (camera.lucb:26-55), f64."""
+    position: np.ndarray
+    corner: np.ndarray
+    du: np.ndarray
+    dv: np.ndarray
+    forward: np.ndarray
+    orthographic: bool
+    width: int
+    height: int
+    lens_radius: float = 0.0
+    focus_distance: float = 1.0
+
+    def pack(self):
+        """The 16 floats the kernels read (camera.lucb:41-47)."""
+        v = [*self.position, 1.0 if self.orthographic else 0.0, *self.corner, 0.0,
+             *self.du, float(self.width), *self.dv, float(self.height)]
+        return np.array(v, dtype=np.float64).astype(np.float32)
+
+    def ray(self, x, y):
+        """Origin and unnormalized direction through (x, y) (camera.lucb:51-55)."""
+        film = self.corner + self.du * x + self.dv * y
+        if self.orthographic:
+            return self.position + film, self.forward.copy()
+        return self.position.copy(), film
+
+
+def _normalized(v):
+    v = np.asarray(v, dtype=np.float64)
+    n = math.sqrt(float(v @ v))
+    if not n > 0.0:
+        raise LuceError(INVALID, "a zero vector has no direction")
+    return v / n
+
+
+def ray_frame(camera: Camera, width, height):
+    """ray_frame (camera.lucb:57-76): film fit horizontally, looking down -Z."""
+    if not isinstance(camera, Camera):
+        raise LuceError(INVALID, "a ray frame needs a camera")
+    if width == 0 or height == 0:
+        raise LuceError(INVALID, "an image needs pixels")
+    m = np.asarray(camera.matrix, np.float64)
+    right, up, back = _normalized(m[0, :3]), _normalized(m[1, :3]), _normalized(m[2, :3])
+    forward = -back
+    aspect = height / width
+    ortho = camera.projection == "orthographic"
+    half_width = 0.05 * camera.horizontal_aperture if ortho else 0.5 * camera.horizontal_aperture / camera.focal_length
+    half_height = half_width * aspect
+    corner = right * -half_width + up * half_height + (np.zeros(3) if ortho else forward)
+    du = right * (2.0 * half_width / width)
+    dv = up * (-2.0 * half_height / height)
+    lens = 0.0005 * camera.focal_length / camera.f_stop if camera.f_stop > 0.0 and not ortho else 0.0
+    return RayFrame(m[3, :3].copy(), corner, du, dv, forward, ortho, int(width), int(height), lens,
+                    camera.focus_distance)
+
+
+def look_frame(position, forward, up, tan_half_height, width, height, orthographic=False):
+    """look_frame (camera.lucb:84-95)."""
+    if width == 0 or height == 0:
+        raise LuceError(INVALID, "an image needs pixels")
+    if not (0.0 < tan_half_height < 1e6):
+        raise LuceError(INVALID, "a view needs a field of view between 0 and 180 degrees, or a height")
+    ahead = _normalized(forward)
+    right = _normalized(np.cross(ahead, np.asarray(up, np.float64)))
+    above = np.cross(right, ahead)
+    half_height = tan_half_height
+    half_width = half_height * width / height
+    corner = right * -half_width + above * half_height + (np.zeros(3) if orthographic else ahead)
+    du = right * (2.0 * half_width / width)
+    dv = above * (-2.0 * half_height / height)
+    return RayFrame(np.asarray(position, np.float64), corner, du, dv, ahead, bool(orthographic), int(width),
+                    int(height), 0.0, 1.0)
+
+
+@dataclass
+class Probe:
+    """One pixel of a probe (device.lucb:132-134): polygon (None: missed) and
+    distance along the frame's unnormalized direction."""
+    face: object
+    distance: float
+
+
+def probe_arrays(device, scene: GpuScene, frame: RayFrame):
+    """The probe as arrays: faces u32 (0xffffffff on a miss) and distances f32,
+    row by row from the top."""
+    count = frame.width * frame.height
+    faces = np.empty(count, np.uint32)
+    dist = np.empty(count, np.float32)
+    packed = frame.pack()
+    _lib.check(_lib.lib().lr_probe(device.handle, scene.handle, _lib.ptr(packed), frame.width, frame.height,
+                                   _lib.ptr(faces), _lib.ptr(dist), count))
+    return faces, dist
+
+
+def probe(device, scene: GpuScene, frame: RayFrame, out=None):
+    """probe (device.lucb:138-160): one Probe per pixel into `out` (a list of
+    width*height entries) or a new list."""
+    count = frame.width * frame.height
+    if out is not None and len(out) != count:
+        raise LuceError(INVALID, "a probe needs one result per pixel")
+    faces, dist = probe_arrays(device, scene, frame)
+    result = out if out is not None else [None] * count
+    for i in range(count):
+        f = int(faces[i])
+        result[i] = Probe(None if f == 0xFFFFFFFF else f, float(dist[i]))
+    return result
+
+
+class Buffer:
+    """gpu.Buffer: device memory on a Device (allocate / create / read / upload)."""
+
+    def __init__(self, device, nbytes):
+        self.device = device
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().lr_buffer_alloc(device.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    @staticmethod
+    def create(device, array):
+        array = np.ascontiguousarray(array)
+        b = Buffer(device, array.nbytes)
+        b.upload(array)
+        return b
+
+    def upload(self, array, offset=0):
+        array = np.ascontiguousarray(array)
+        if offset + array.nbytes > self.nbytes:
+            raise LuceError(INVALID, "an upload past the buffer's end")
+        _lib.check(_lib.lib().lr_buffer_write(self.device.handle, self.ptr.value + offset, _lib.ptr(array),
+                                              array.nbytes))
+
+    def read(self, dtype, count=None, offset=0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dtype.itemsize
+        out = np.empty(count, dtype)
+        if offset + out.nbytes > self.nbytes:
+            raise LuceError(INVALID, "a read past the buffer's end")
+        _lib.check(_lib.lib().lr_buffer_read(self.device.handle, _lib.ptr(out), self.ptr.value + offset, out.nbytes))
+        return out
+
+    def destroy(self):
+        if self.ptr:
+            _lib.lib().lr_buffer_free(self.device.handle, self.ptr)
+            self.ptr = None
+
+
+def device_stream(device):
+    return _lib.lib().lr_device_stream(device.handle)
+
+
+def synchronize(device):
+    _lib.check(_lib.lib().lr_device_synchronize(device.handle))
+
+
+def probe_device(scene: GpuScene, frame: RayFrame, hits: Buffer, stream=None):
+    """The probe into a device buffer of width*height {polygon, t bits} pairs."""
+    packed = frame.pack()
+    if hits.nbytes < frame.width * frame.height * 8:
+        raise LuceError(INVALID, "the probe's output buffer is too small")
+    _lib.check(_lib.lib().lr_probe_device(scene.handle, _lib.ptr(packed), frame.width, frame.height, hits.ptr,
+                                          stream))
+
+
+def trace_closest_device(scene: GpuScene, rays: Buffer, n, hits: Buffer, stream=None):
+    """Closest hits of n rays (8 floats each: origin, t_max, direction, -)."""
+    if rays.nbytes < n * 32 or hits.nbytes < n * 8:
+        raise LuceError(INVALID, "ray or hit buffer too small")
+    _lib.check(_lib.lib().lr_trace_closest_device(scene.handle, rays.ptr, int(n), hits.ptr, stream))
+
+
+def trace_any_device(scene: GpuScene, rays: Buffer, n, occluded: Buffer, stream=None):
+    """Any-hit occlusion of n rays (8 floats each) before their t_max."""
+    if rays.nbytes < n * 32 or occluded.nbytes < n * 4:
+        raise LuceError(INVALID, "ray or output buffer too small")
+    _lib.check(_lib.lib().lr_trace_any_device(scene.handle, rays.ptr, int(n), occluded.ptr, stream))
+
+
+class Timer:
+    """A pair of HIP events on a stream: elapsed milliseconds between marks."""
+
+    def __init__(self):
+        self.start = ctypes.c_void_p()
+        self.stop = ctypes.c_void_p()
+        _lib.check(_lib.lib().lr_event_create(ctypes.byref(self.start)))
+        _lib.check(_lib.lib().lr_event_create(ctypes.byref(self.stop)))
+
+    def begin(self, stream):
+        _lib.check(_lib.lib().lr_event_record(self.start, stream))
+
+    def end(self, stream):
+        _lib.check(_lib.lib().lr_event_record(self.stop, stream))
+
+    def elapsed_ms(self):
+        ms = ctypes.c_float()
+        _lib.check(_lib.lib().lr_event_elapsed(self.start, self.stop, ctypes.byref(ms)))
+        return ms.value
+
+    def destroy(self):
+        _lib.lib().lr_event_destroy(self.start)
+        _lib.lib().lr_event_destroy(self.stop)

@@ -1,0 +1,273 @@
+"""Seeded synthetic scenes of the measured workloads.
+
+  cornell_probe_set      Cornell box (5 unit walls, 2 rotated boxes; 34
+                         triangles, 17 polygons) plus 10,000 random triangles
+                         (centres U[-0.4,0.4]^3, vertices N(0, 0.03) per axis,
+                         seed 1); pinhole camera down -Z; probed at 256x256.
+  cornell_render_set     the same box with Lambert walls (0.73 white, 0.65
+                         red/green bands) and a 0.25x0.25 rect light at
+                         y = 0.999, radiance 17; rendered at 512x512.
+  furnace_set            an albedo-0.8 sphere under a constant environment.
+  instanced_set          64 displaced icospheres (frequency 39: 30,420
+                         triangles each, ~1.95M unique) placed as 1,000 rigid
+                         instances in a 20^3 volume (30.4M triangles
+                         realized) with 256 rect/disk/sphere lights of
+                         log-uniform intensity in [1, 1000]; 1920x1080.
+
+Random numbers: SplitMix64-seeded xoshiro256** (Blackman and Vigna), f64
+uniforms from the top 53 bits, normals by Box-Muller (cosine branch).
+"""
+import math
+
+import numpy as np
+
+from .scene import Camera, GeometrySet, Instance, Light, Mesh
+
+MASK = (1 << 64) - 1
+
+
+class Xoshiro256:
+    """xoshiro256** seeded by SplitMix64."""
+
+    def __init__(self, seed):
+        s = seed & MASK
+        self.s = []
+        for _ in range(4):
+            s = (s + 0x9E3779B97F4A7C15) & MASK
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+            self.s.append(z ^ (z >> 31))
+
+    def next(self):
+        s = self.s
+        result = (((s[1] * 5) & MASK) << 7 | ((s[1] * 5) & MASK) >> 57) & MASK
+        result = (result * 9) & MASK
+        t = (s[1] << 17) & MASK
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = ((s[3] << 45) | (s[3] >> 19)) & MASK
+        return result
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+    def normal(self):
+        u1 = 1.0 - self.uniform()
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def placement(rotation=None, translation=(0.0, 0.0, 0.0), scale=1.0):
+    """A 4x4 placement, rows: x, y, z axes then translation."""
+    m = np.eye(4)
+    r = np.eye(3) if rotation is None else np.asarray(rotation, np.float64)
+    m[:3, :3] = r * scale
+    m[3, :3] = translation
+    return m
+
+
+def rotation_y(degrees):
+    a = math.radians(degrees)
+    c, s = math.cos(a), math.sin(a)
+    # rows are the images of the x, y, z axes
+    return np.array([[c, 0.0, -s], [0.0, 1.0, 0.0], [s, 0.0, c]])
+
+
+def quad_mesh(corners, normal, material=None):
+    """One polygon of two triangles over four corners (counter-clockwise seen
+    from `normal`)."""
+    pts = np.asarray(corners, np.float64)
+    tris = np.array([[0, 1, 2], [0, 2, 3]])
+    n = np.tile(np.asarray(normal, np.float64), (2, 3, 1))
+    return Mesh(pts, tris, np.array([0, 0]), 1, normals=n,
+                face_materials=None if material is None else [material])
+
+
+def box_mesh(size, material=None):
+    """An axis-aligned box centred at the origin: 6 polygons, 12 triangles,
+    flat outward normals."""
+    hx, hy, hz = (0.5 * s for s in size)
+    faces = [
+        ([[hx, -hy, -hz], [hx, hy, -hz], [hx, hy, hz], [hx, -hy, hz]], [1, 0, 0]),
+        ([[-hx, -hy, hz], [-hx, hy, hz], [-hx, hy, -hz], [-hx, -hy, -hz]], [-1, 0, 0]),
+        ([[-hx, hy, -hz], [-hx, hy, hz], [hx, hy, hz], [hx, hy, -hz]], [0, 1, 0]),
+        ([[-hx, -hy, hz], [-hx, -hy, -hz], [hx, -hy, -hz], [hx, -hy, hz]], [0, -1, 0]),
+        ([[-hx, -hy, hz], [hx, -hy, hz], [hx, hy, hz], [-hx, hy, hz]], [0, 0, 1]),
+        ([[hx, -hy, -hz], [-hx, -hy, -hz], [-hx, hy, -hz], [hx, hy, -hz]], [0, 0, -1]),
+    ]
+    pts, tris, tf, normals = [], [], [], []
+    for f, (c, n) in enumerate(faces):
+        b = len(pts)
+        pts.extend(c)
+        tris.extend([[b, b + 1, b + 2], [b, b + 2, b + 3]])
+        tf.extend([f, f])
+        normals.extend([[n, n, n], [n, n, n]])
+    mats = None if material is None else [material] * 6
+    return Mesh(np.array(pts), np.array(tris), np.array(tf), 6, normals=np.array(normals, np.float64),
+                face_materials=mats)
+
+
+def _cornell_geometry(white="white", red="red", green="green"):
+    """The box [-1,1]^3 open toward +z: floor, ceiling, back, left (red),
+    right (green), with two rotated boxes on the floor."""
+    walls = GeometrySet()
+    walls.meshes.append(quad_mesh([[-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]], [0, 1, 0], white))
+    walls.meshes.append(quad_mesh([[-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1]], [0, -1, 0], white))
+    walls.meshes.append(quad_mesh([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]], [0, 0, 1], white))
+    walls.meshes.append(quad_mesh([[-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]], [1, 0, 0], red))
+    walls.meshes.append(quad_mesh([[1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]], [-1, 0, 0], green))
+    short = GeometrySet(meshes=[box_mesh((0.6, 0.6, 0.6), white)])
+    tall = GeometrySet(meshes=[box_mesh((0.6, 1.2, 0.6), white)])
+    walls.instances.append(Instance(short, placement(rotation_y(-17.0), (0.33, -0.7, 0.25))))
+    walls.instances.append(Instance(tall, placement(rotation_y(20.0), (-0.33, -0.4, -0.3))))
+    return walls
+
+
+def cornell_camera(focal_length=30.0):
+    """A pinhole near (0, 0, 3) looking down -Z, 36 mm aperture. It sits a
+    little off the box's axis so that no pixel-centre ray runs exactly along
+    the edge where two walls meet."""
+    return Camera(placement(translation=(0.0123, 0.0217, 3.0)), "perspective", focal_length, 36.0)
+
+
+def random_triangles(count=10000, seed=1, spread=0.4, sigma=0.03):
+    """`count` triangles: centre U[-spread, spread]^3, each vertex the centre
+    plus N(0, sigma^2) per axis; one polygon each."""
+    rng = Xoshiro256(seed)
+    pts = np.empty((count * 3, 3))
+    for t in range(count):
+        c = [(2.0 * rng.uniform() - 1.0) * spread for _ in range(3)]
+        for k in range(3):
+            pts[t * 3 + k] = [c[a] + sigma * rng.normal() for a in range(3)]
+    tris = np.arange(count * 3).reshape(count, 3)
+    return Mesh(pts, tris, np.arange(count), count)
+
+
+def cornell_probe_set(random_count=10000, seed=1):
+    gset = _cornell_geometry()
+    if random_count:
+        gset.meshes.append(random_triangles(random_count, seed))
+    gset.cameras.append(cornell_camera())
+    return gset
+
+
+# ------------------------------------------------------------ instanced scene
+
+def icosphere(frequency):
+    """A geodesic sphere: each icosahedron face split into frequency^2
+    triangles on a barycentric grid, points projected to the unit sphere and
+    shared along edges (20 f^2 triangles, 10 f^2 + 2 points)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    f = frequency
+    keys = {}
+    points = []
+
+    def point(a, b, c, i, j):
+        # grid point i along a->b, j along a->c; edge points keyed by their
+        # integer position on the edge so both faces share them
+        k = f - i - j
+        weights = {a: k, b: i, c: j}
+        key = tuple(sorted((idx, w) for idx, w in weights.items() if w))
+        if key not in keys:
+            p = (v[a] * k + v[b] * i + v[c] * j) / f
+            keys[key] = len(points)
+            points.append(p / np.linalg.norm(p))
+        return keys[key]
+
+    tris = []
+    for a, b, c in faces:
+        for i in range(f):
+            for j in range(f - i):
+                p0 = point(a, b, c, i, j)
+                p1 = point(a, b, c, i + 1, j)
+                p2 = point(a, b, c, i, j + 1)
+                tris.append((p0, p1, p2))
+                if i + j < f - 1:
+                    p3 = point(a, b, c, i + 1, j + 1)
+                    tris.append((p1, p3, p2))
+    return np.array(points), np.array(tris, np.int64)
+
+
+def _hash3(ix, iy, iz, seed):
+    h = (ix.astype(np.int64) * 73856093) ^ (iy.astype(np.int64) * 19349663) ^ (iz.astype(np.int64) * 83492791) ^ seed
+    h = h & 0xFFFFFFFF
+    h = (h ^ (h >> 16)) * 0x45D9F3B & 0xFFFFFFFF
+    h = (h ^ (h >> 16)) * 0x45D9F3B & 0xFFFFFFFF
+    h = h ^ (h >> 16)
+    return (h & 0xFFFFFF) / float(0xFFFFFF) * 2.0 - 1.0
+
+
+def value_noise(p, seed, octaves=3, frequency=2.0):
+    """Seeded 3-D value noise (smoothstep trilinear lattice values in [-1, 1]),
+    `octaves` octaves of doubling frequency and halving weight, normalized to
+    [-1, 1]."""
+    total = np.zeros(len(p))
+    norm = 0.0
+    amp = 1.0
+    for o in range(octaves):
+        q = p * frequency * (2 ** o)
+        i = np.floor(q)
+        fr = q - i
+        s = fr * fr * (3.0 - 2.0 * fr)
+        acc = np.zeros(len(p))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    w = (s[:, 0] if dx else 1 - s[:, 0]) * (s[:, 1] if dy else 1 - s[:, 1]) * \
+                        (s[:, 2] if dz else 1 - s[:, 2])
+                    acc += w * _hash3(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, seed + 7919 * o)
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+    return total / norm
+
+
+def random_rotation(rng):
+    """A uniform random rotation (Shoemake's quaternion)."""
+    u1, u2, u3 = rng.uniform(), rng.uniform(), rng.uniform()
+    q = (math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+         math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3))
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w)],
+                     [2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w)],
+                     [2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced_set(meshes=64, instances=1000, frequency=39, radius=0.6, extent=20.0, lights=256, seed=3,
+                  amplitude=0.05):
+    """The large instanced scene: `meshes` displaced icospheres placed as
+    `instances` rigid instances in an `extent`^3 volume, `lights` lights."""
+    rng = Xoshiro256(seed)
+    base_points, base_tris = icosphere(frequency)
+    protos = []
+    for m in range(meshes):
+        d = value_noise(base_points * 1.0, seed * 1000 + m, 3, 2.0)
+        pts = base_points * (radius * (1.0 + amplitude * d))[:, None]
+        protos.append(GeometrySet(meshes=[Mesh(pts, base_tris, np.arange(len(base_tris)), len(base_tris),
+                                               face_materials=["surface"] * len(base_tris))],
+                                  materials={}))
+    gset = GeometrySet(materials={"surface": {"base_color": (0.7, 0.7, 0.7)}})
+    half = extent / 2.0
+    for i in range(instances):
+        rot = random_rotation(rng)
+        t = [(2.0 * rng.uniform() - 1.0) * half for _ in range(3)]
+        gset.instances.append(Instance(protos[i % meshes], placement(rot, t)))
+    kinds = ["rect", "disk", "sphere"]
+    for i in range(lights):
+        kind = kinds[i % 3]
+        t = [(2.0 * rng.uniform() - 1.0) * half for _ in range(3)]
+        intensity = math.exp(rng.uniform() * math.log(1000.0))
+        gset.lights.append(Light(kind, placement(random_rotation(rng), t), (1.0, 1.0, 1.0), intensity,
+                                 width=0.3, height=0.3, radius=0.15))
+    # Camera outside the volume looking down -Z at its centre.
+    gset.cameras.append(Camera(placement(translation=(0.0, 0.0, half + 14.0)), "perspective", 30.0, 36.0))
+    return gset

@@ -1,0 +1,163 @@
+"""Oracle (test infrastructure only): ctypes access to the C restatement in
+oracle_trace.c. Imported by tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg, never by the product.
+
+Build with `make -C oracle` (the library lands in oracle/_build/).
+"""
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+
+NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("first", "<u4"), ("hi", "<f4", 3), ("count", "<u4")])
+
+_P = ctypes.c_void_p
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        sig = {
+            "or_ray_frame": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, _P]),
+            "or_look_frame": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.c_int, _P]),
+            "or_pack_frame": (None, [_P, ctypes.c_uint32, ctypes.c_uint32, _P]),
+            "or_bvh_build": (ctypes.c_size_t, [_P, _P, ctypes.c_size_t, _P, _P, _P]),
+            "or_reorder_u32": (None, [_P, _P, _P, ctypes.c_size_t, ctypes.c_size_t]),
+            "or_check_bvh": (ctypes.c_int, [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t]),
+            "or_probe": (None, [_P, _P, _P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P]),
+            "or_trace_rays": (None, [_P, _P, _P, _P, _P, ctypes.c_size_t, ctypes.c_float, _P, _P, _P]),
+            "or_brute_force": (None, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_uint32, ctypes.c_uint32, _P,
+                                      ctypes.c_size_t, ctypes.c_int, _P, _P, _P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def ray_frame(matrix, orthographic, horizontal_aperture, focal_length, width, height, f_stop=0.0, focus=1.0):
+    """camera.lucb:57-76 in f64: a dict of the frame's parts."""
+    m = np.ascontiguousarray(np.asarray(matrix, np.float64)[:4, :3].reshape(12))
+    out = np.zeros(18)
+    rc = lib().or_ray_frame(_p(m), int(orthographic), horizontal_aperture, focal_length, f_stop, focus, width,
+                            height, _p(out))
+    if rc:
+        raise ValueError(rc)
+    return out
+
+
+def look_frame(position, forward, up, tan_half_height, width, height, orthographic=False):
+    pos = np.asarray(position, np.float64)
+    fw = np.asarray(forward, np.float64)
+    u = np.asarray(up, np.float64)
+    out = np.zeros(18)
+    rc = lib().or_look_frame(_p(pos), _p(fw), _p(u), tan_half_height, width, height, int(orthographic), _p(out))
+    if rc:
+        raise ValueError(rc)
+    return out
+
+
+def frame_ray(frame, x, y):
+    """camera.lucb:51-55 on an f64 frame from ray_frame/look_frame."""
+    film = frame[3:6] + frame[6:9] * x + frame[9:12] * y
+    if frame[17] > 0.5:
+        return frame[0:3] + film, frame[12:15].copy()
+    return frame[0:3].copy(), film
+
+
+def pack_frame(frame, width, height):
+    out = np.zeros(16, np.float32)
+    lib().or_pack_frame(_p(np.ascontiguousarray(frame)), width, height, _p(out))
+    return out
+
+
+def bvh_build(positions, indices):
+    """bvh.lucb:66-129: (nodes, order, max |f32 - f64| SAH cost gap)."""
+    positions = np.ascontiguousarray(positions, np.float32)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    n = len(indices)
+    nodes = np.zeros(max(1, 2 * n), NODE_DTYPE)
+    order = np.zeros(max(1, n), np.uint32)
+    gap = ctypes.c_double(0.0)
+    count = lib().or_bvh_build(_p(positions), _p(indices), n, _p(nodes), _p(order), ctypes.byref(gap))
+    return nodes[:count].copy(), order[:n].copy(), gap.value
+
+
+def check_bvh(nodes, positions, indices):
+    """tests.lucb:9-28: 0 when every invariant holds."""
+    return lib().or_check_bvh(_p(np.ascontiguousarray(nodes)), len(nodes), _p(np.ascontiguousarray(positions, np.float32)),
+                              _p(np.ascontiguousarray(indices, np.uint32)), len(indices))
+
+
+def probe(nodes, positions, indices, faces, frame16, width, height):
+    """probe.comp over the f32 mirror: faces, t, slots, counters
+    (interior visits, leaf visits, triangle tests, dropped stack entries)."""
+    n = width * height
+    f = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
+    s = np.zeros(n, np.uint32)
+    c = np.zeros(4, np.uint64)
+    lib().or_probe(_p(np.ascontiguousarray(nodes)), _p(np.ascontiguousarray(positions, np.float32)),
+                   _p(np.ascontiguousarray(indices, np.uint32)), _p(np.ascontiguousarray(faces, np.uint32)),
+                   _p(np.ascontiguousarray(frame16, np.float32)), width, height, _p(f), _p(t), _p(s), _p(c))
+    return f, t, s, c
+
+
+def trace_rays(nodes, positions, indices, faces, rays, t_max=3.0e38):
+    """trace_closest over rays (n, 6) f32 (origin, direction)."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = len(rays)
+    f = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
+    c = np.zeros(4, np.uint64)
+    lib().or_trace_rays(_p(np.ascontiguousarray(nodes)), _p(np.ascontiguousarray(positions, np.float32)),
+                        _p(np.ascontiguousarray(indices, np.uint32)), _p(np.ascontiguousarray(faces, np.uint32)),
+                        _p(rays), n, t_max, _p(f), _p(t), _p(c))
+    return f, t, c
+
+
+def brute_force(positions, indices, faces, frame=None, width=0, height=0, rays=None, threads=1):
+    """tests/gpu/main.lucb:82-98 in f64: (face or -1, distance, margin)."""
+    n = len(rays) if rays is not None else width * height
+    face = np.zeros(n, np.int64)
+    dist = np.zeros(n, np.float64)
+    margin = np.zeros(n, np.float64)
+    r = None if rays is None else np.ascontiguousarray(rays, np.float64)
+    fr = None if frame is None else np.ascontiguousarray(frame, np.float64)
+    lib().or_brute_force(_p(np.ascontiguousarray(positions, np.float32)), _p(np.ascontiguousarray(indices, np.uint32)),
+                         _p(np.ascontiguousarray(faces, np.uint32)), len(indices), _p(fr), width, height, _p(r),
+                         n if r is not None else 0, threads, _p(face), _p(dist), _p(margin))
+    return face, dist, margin
+
+
+def agrees(got_face, got_t, face, dist, margin):
+    """The reference's comparison (tests/gpu/main.lucb:61-78): same polygon
+    (or both missed) and |dt| <= 1e-4 t; disagreements count only where the
+    next polygon is more than 1e-3 farther (relative)."""
+    got_face = np.asarray(got_face, np.int64)
+    hit = face >= 0
+    missed = got_face == 0xFFFFFFFF
+    same = np.where(hit, (~missed) & (got_face == face) & (np.abs(got_t.astype(np.float64) - dist) <= 1e-4 * dist),
+                    missed)
+    differ = (~same) & (margin > 1e-3)
+    return same, differ

@@ -1,0 +1,157 @@
+"""GPU parity of the tracing kernels against the oracle: polygon ids and
+distance bits identical to the f32 mirror of bvh.glsl, and agreement with the
+f64 brute force under the reference's margin rule (tests/gpu/main.lucb:61-78)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+WIDTHS = (2, 4, 8)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def probe_scene(device, cornell_probe):
+    from luce_render import render
+    g = render.GpuScene.upload(device, cornell_probe)
+    yield g
+    g.destroy()
+
+
+def _mirror(sc, frame):
+    return O.probe(sc.nodes, sc.positions, sc.indices, sc.faces, frame.pack(), frame.width, frame.height)
+
+
+@pytest.mark.parametrize("width", WIDTHS)
+def test_probe_bit_exact_against_mirror(device, cornell_probe, probe_scene, width):
+    from luce_render import render
+    sc = cornell_probe
+    frame = render.ray_frame(sc.cameras[0], 256, 256)
+    probe_scene.set_trace_config(width, 128, 32)
+    say(f"probe 256x256 width {width}")
+    faces, t = render.probe_arrays(device, probe_scene, frame)
+    mf, mt, _, _ = _mirror(sc, frame)
+    assert np.array_equal(faces, mf), int((faces != mf).sum())
+    assert np.array_equal(t.view(np.uint32), mt.view(np.uint32))
+
+
+def test_probe_agrees_with_brute_force(device, cornell_probe, probe_scene):
+    from luce_render import render
+    sc = cornell_probe
+    frame = render.ray_frame(sc.cameras[0], 256, 256)
+    probe_scene.set_trace_config(4, 128, 32)
+    faces, t = render.probe_arrays(device, probe_scene, frame)
+    f64 = O.ray_frame(sc.cameras[0].matrix, False, 36.0, 30.0, 256, 256)
+    bf, bd, bm = O.brute_force(sc.positions, sc.indices, sc.faces, frame=f64, width=256, height=256, threads=8)
+    same, differ = O.agrees(faces, t, bf, bd, bm)
+    assert (bf >= 0).sum() > 256 * 256 // 2
+    assert differ.sum() == 0
+
+
+def test_probe_orthographic(device, cornell_probe, probe_scene):
+    from luce_render import render
+    sc = cornell_probe
+    frame = render.look_frame((0.031, 0.017, 4.0), (0.05, -0.1, -1.0), (0.0, 1.0, 0.0), 1.3, 96, 64, orthographic=True)
+    f64 = O.look_frame((0.031, 0.017, 4.0), (0.05, -0.1, -1.0), (0.0, 1.0, 0.0), 1.3, 96, 64, True)
+    mf, mt, _, _ = _mirror(sc, frame)
+    bf, bd, bm = O.brute_force(sc.positions, sc.indices, sc.faces, frame=f64, width=96, height=64, threads=8)
+    for width in WIDTHS:
+        probe_scene.set_trace_config(width, 64, 24)
+        say(f"probe orthographic width {width}")
+        faces, t = render.probe_arrays(device, probe_scene, frame)
+        assert np.array_equal(faces, mf) and np.array_equal(t.view(np.uint32), mt.view(np.uint32))
+        _, differ = O.agrees(faces, t, bf, bd, bm)
+        assert differ.sum() == 0
+
+
+def test_every_variant_agrees(device, cornell_probe, probe_scene):
+    from luce_render import render
+    sc = cornell_probe
+    frame = render.ray_frame(sc.cameras[0], 100, 75)
+    mf, mt, _, _ = _mirror(sc, frame)
+    for width in WIDTHS:
+        for group in (64, 128, 256):
+            for stack in (24, 32, 48):
+                probe_scene.set_trace_config(width, group, stack)
+                say(f"probe 100x75 variant w{width} g{group} s{stack}")
+                faces, t = render.probe_arrays(device, probe_scene, frame)
+                assert np.array_equal(faces, mf) and np.array_equal(t.view(np.uint32), mt.view(np.uint32)), \
+                    (width, group, stack)
+    probe_scene.set_trace_config(4, 128, 32)
+
+
+def _random_rays(n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-0.9, 0.9, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    # a few axis-parallel directions exercise the 1e-20 guard
+    d[:8] = [[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [0, 0.6, 0.8], [-0.0, 0, -1]]
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+def test_ray_batches_closest_and_any(device, cornell_probe, probe_scene):
+    from luce_render import render
+    sc = cornell_probe
+    n = 20000
+    o, d = _random_rays(n, 9)
+    tmax = np.where(np.arange(n) % 3 == 0, 3.0e38, 0.35).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7] = o, tmax, d
+    rb = render.Buffer.create(device, rays)
+    hb = render.Buffer(device, n * 8)
+    ob = render.Buffer(device, n * 4)
+    rays6 = np.concatenate([o, d], 1)
+    # the mirror's closest hit under each ray's own t_max
+    mf = np.zeros(n, np.uint32)
+    mt = np.zeros(n, np.float32)
+    for tm in (3.0e38, 0.35):
+        sel = tmax == np.float32(tm)
+        f_, t_, _ = O.trace_rays(sc.nodes, sc.positions, sc.indices, sc.faces, rays6[sel], tm)
+        mf[sel], mt[sel] = f_, t_
+    for width in WIDTHS:
+        probe_scene.set_trace_config(width, 128, 32)
+        say(f"ray batch closest/any width {width}")
+        render.trace_closest_device(probe_scene, rb, n, hb)
+        render.trace_any_device(probe_scene, rb, n, ob)
+        render.synchronize(device)
+        h = hb.read(np.uint32).reshape(n, 2)
+        assert np.array_equal(h[:, 0], mf)
+        assert np.array_equal(h[:, 1], mt.view(np.uint32))
+        occluded = ob.read(np.uint32)
+        # any-hit visits every leaf whose box passes at fixed t_max, so it is
+        # occluded exactly when some tested triangle lies before t_max
+        assert np.array_equal(occluded, (mf != 0xFFFFFFFF).astype(np.uint32))
+    for b in (rb, hb, ob):
+        b.destroy()
+    probe_scene.set_trace_config(4, 128, 32)
+
+
+def test_empty_and_tiny_scenes(device):
+    from luce_render import render
+    empty = render.compile_scene(render.GeometrySet())
+    g = render.GpuScene.upload(device, empty)
+    frame = render.look_frame((0, 0, 3), (0, 0, -1), (0, 1, 0), 0.5, 16, 8)
+    faces, t = render.probe_arrays(device, g, frame)
+    assert (faces == 0xFFFFFFFF).all() and (t == np.float32(3.0e38)).all()
+    g.destroy()
+    one = render.compile_scene(render.GeometrySet(meshes=[render.Mesh([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], [[0, 1, 2]])]))
+    g = render.GpuScene.upload(device, one)
+    faces, t = render.probe_arrays(device, g, frame)
+    mf, mt, _, _ = _mirror(one, frame)
+    assert np.array_equal(faces, mf) and np.array_equal(t.view(np.uint32), mt.view(np.uint32))
+    assert (faces == 0).sum() > 0
+    g.destroy()
+
+
+def test_probe_count_must_match(device, probe_scene):
+    from luce_render import render
+    frame = render.look_frame((0, 0, 3), (0, 0, -1), (0, 1, 0), 0.5, 16, 8)
+    with pytest.raises(render.LuceError) as e:
+        render.probe(device, probe_scene, frame, out=[None] * 10)
+    assert e.value.code == 400
