@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--instances", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--config", default="", help="width,group,stack to skip the sweep")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run the plain copy kernel over this many MiB (a known-traffic dispatch "
+                         "for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
@@ -169,7 +172,7 @@ def main():
     _, _, cb = O.trace_rays(sc.nodes, sc.positions, sc.indices, sc.faces, brays)
 
     def per_ray(c):
-        return (40.0 + 64.0 * c[0] + 36.0 * c[2]) / len(sample)
+        return 40.0 + (64.0 * c[0] + 36.0 * c[2]) / len(sample)
 
     bytes_p, bytes_b = per_ray(cp), per_ray(cb)
     # dominant kernel: the bounce batch (incoherent)
@@ -180,7 +183,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        m = 256
+        m = 1024
         sel = rng.choice(n, m, replace=False)
         ys2, xs2 = np.divmod(sel, WIDTH)
         d64 = frame.corner[None, :] + (xs2 + 0.5)[:, None] * frame.du[None, :] + (ys2 + 0.5)[:, None] * frame.dv[None, :]
@@ -221,6 +224,15 @@ def main():
                      "kernel_ms": round(dom_ms, 4)},
         "cpu_baseline": cpu,
     }
+    if args.copy_mib:
+        nbytes = args.copy_mib << 20
+        a, b = render.Buffer(device, nbytes), render.Buffer(device, nbytes)
+        for _ in range(3):
+            b.copy_from(a)
+        render.synchronize(device)
+        a.destroy()
+        b.destroy()
+        log(f"copy kernel: 3 dispatches of {nbytes} bytes read + {nbytes} written")
     print(json.dumps(out), flush=True)
     for b in (hits_p, hits_b, rays_b):
         b.destroy()

@@ -137,6 +137,10 @@ int lr_buffer_alloc(lr_device *device, size_t bytes, void **out);
 void lr_buffer_free(lr_device *device, void *buffer);
 int lr_buffer_write(lr_device *device, void *dst_device, const void *src_host, size_t bytes);
 int lr_buffer_read(lr_device *device, void *dst_host, const void *src_device, size_t bytes);
+/* A plain device-to-device copy kernel on the device's stream (16-byte
+ * aligned, whole 16-byte words; nothing waits): no reference counterpart, the
+ * known-traffic kernel hardware byte counters are scaled against. */
+int lr_buffer_copy(lr_device *device, void *dst_device, const void *src_device, size_t bytes);
 /* The device's stream (a hipStream_t) and a wait for all work on it. */
 void *lr_device_stream(lr_device *device);
 int lr_device_synchronize(lr_device *device);

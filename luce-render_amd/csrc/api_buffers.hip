@@ -4,6 +4,8 @@
 // callers of the tracing entry points need around them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lr_internal.h"
 #include "lr_scene.h"
 
@@ -82,3 +84,29 @@ void lr_event_destroy(void *event) {
 }
 
 }  // extern "C"
+
+namespace {
+
+// A plain device-to-device copy (16 bytes a lane, grid-stride): the reference
+// kernel of known traffic against which hardware byte counters are scaled.
+__global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int lr_buffer_copy(lr_device *device, void *dst, const void *src, size_t bytes) {
+    if (!device || (bytes && (!dst || !src))) return fail(LR_INVALID, "lr_buffer_copy needs its arguments");
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return fail(LR_INVALID, "lr_buffer_copy copies whole 16-byte words");
+    (void)hipSetDevice(device->ordinal);
+    size_t n = bytes / 16;
+    if (n == 0) return 0;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, device->stream, static_cast<const float4 *>(src),
+                       static_cast<float4 *>(dst), n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "copy launch");
+    return 0;
+}

@@ -68,6 +68,7 @@ _SIGNATURES = {
     "lr_buffer_free": (None, [_P, _P]),
     "lr_buffer_write": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "lr_buffer_read": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "lr_buffer_copy": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "lr_device_stream": (_P, [_P]),
     "lr_device_synchronize": (ctypes.c_int, [_P]),
     "lr_event_create": (ctypes.c_int, [ctypes.POINTER(_P)]),

@@ -236,6 +236,11 @@ class Buffer:
         _lib.check(_lib.lib().lr_buffer_read(self.device.handle, _lib.ptr(out), self.ptr.value + offset, out.nbytes))
         return out
 
+    def copy_from(self, other, nbytes=None):
+        """A device-side copy of `other`'s first nbytes (the copy kernel)."""
+        nbytes = min(self.nbytes, other.nbytes) if nbytes is None else nbytes
+        _lib.check(_lib.lib().lr_buffer_copy(self.device.handle, self.ptr, other.ptr, int(nbytes)))
+
     def destroy(self):
         if self.ptr:
             _lib.lib().lr_buffer_free(self.device.handle, self.ptr)
@@ -251,8 +256,12 @@ def synchronize(device):
 
 
 def probe_device(scene: GpuScene, frame: RayFrame, hits: Buffer, stream=None):
-    """The probe into a device buffer of width*height {polygon, t bits} pairs."""
+    """The probe into a device buffer of width*height {polygon, t bits} pairs,
+    on `stream` (default: the device's own stream, which Buffer reads and
+    synchronize() wait on)."""
     packed = frame.pack()
+    if stream is None:
+        stream = device_stream(scene.device)
     if hits.nbytes < frame.width * frame.height * 8:
         raise LuceError(INVALID, "the probe's output buffer is too small")
     _lib.check(_lib.lib().lr_probe_device(scene.handle, _lib.ptr(packed), frame.width, frame.height, hits.ptr,
@@ -263,6 +272,8 @@ def trace_closest_device(scene: GpuScene, rays: Buffer, n, hits: Buffer, stream=
     """Closest hits of n rays (8 floats each: origin, t_max, direction, -)."""
     if rays.nbytes < n * 32 or hits.nbytes < n * 8:
         raise LuceError(INVALID, "ray or hit buffer too small")
+    if stream is None:
+        stream = device_stream(scene.device)
     _lib.check(_lib.lib().lr_trace_closest_device(scene.handle, rays.ptr, int(n), hits.ptr, stream))
 
 
@@ -270,6 +281,8 @@ def trace_any_device(scene: GpuScene, rays: Buffer, n, occluded: Buffer, stream=
     """Any-hit occlusion of n rays (8 floats each) before their t_max."""
     if rays.nbytes < n * 32 or occluded.nbytes < n * 4:
         raise LuceError(INVALID, "ray or output buffer too small")
+    if stream is None:
+        stream = device_stream(scene.device)
     _lib.check(_lib.lib().lr_trace_any_device(scene.handle, rays.ptr, int(n), occluded.ptr, stream))
 
 
