@@ -33,6 +33,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (name<width,stack,group>) from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, or None."""
+    path = ROOT / "profiles" / "closest_hit_instanced_30M.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return int(parts[2])
+    return None
+
+
 def bounce_rays(sc, frame, faces_t, rng):
     """Cosine-distributed rays from each primary hit (8 floats a ray)."""
     n = WIDTH * HEIGHT
@@ -179,6 +192,7 @@ def main():
     dom_bytes, dom_ms, dom_name = (bytes_b * n, ms_b, "closest_kernel") if ms_b >= ms_p else (bytes_p * n, ms_p,
                                                                                               "probe_kernel")
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(f"{dom_name}<{best[0]},{best[2]},{best[1]}>")
 
     cpu = None
     if not args.no_cpu:
@@ -220,7 +234,7 @@ def main():
             "wide_bytes": info["wide_bytes"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,
                      "kernel_ms": round(dom_ms, 4)},
         "cpu_baseline": cpu,
     }
