@@ -128,6 +128,83 @@ typedef struct lr_scene_info {
 } lr_scene_info;
 int lr_scene_info_get(const lr_gpu_scene *scene, lr_scene_info *info);
 
+/* ------------------------------------------------------------------ renderer */
+
+typedef struct lr_renderer lr_renderer;
+
+/* What Loaded.create packs beside the geometry (src/render/loading.lucb:51-161):
+ * light records (7 float4 each) followed by the light tree's nodes (3 float4
+ * each); the 32 constants rows (128 floats, loading.lucb:253-284 and the view
+ * rows of :279-284); material records (16 float4 each, the default material
+ * first); the emitter words (tree links, emitter records, triangle map); the
+ * light paths a sample (Loaded.light_paths, :166-169) and the extra rounds
+ * cut-outs need (8 or 0). */
+typedef struct lr_loaded {
+    const float *lights;
+    size_t light_floats;
+    const float *constants;
+    const float *materials;
+    size_t material_floats;
+    const uint32_t *emitters;
+    size_t emitter_words;
+    uint32_t light_paths;
+    uint32_t through;
+} lr_loaded;
+
+/* RenderSettings (src/render/integrator.lucb:42-85) as far as the device
+ * needs them; the rest is in the loaded constants. pool 0: min(2^21, pixels).
+ * use_graph: replay each pass from one recorded hipGraph. trace_*: the
+ * traversal variant (0: the scene's configuration). */
+typedef struct lr_render_settings {
+    uint32_t width;
+    uint32_t height;
+    uint32_t samples;
+    uint32_t max_bounces;
+    uint32_t spectral;
+    uint32_t pool;
+    uint32_t seed;
+    uint32_t guiding;
+    uint32_t profile;
+    uint32_t use_graph;
+    uint32_t trace_width;
+    uint32_t trace_group;
+    uint32_t trace_stack;
+} lr_render_settings;
+
+/* Renderer.image / display: device pointers to the rgba32f ACEScg image and
+ * the exposed, clipped linear Rec.709 display image (rgba32f here). */
+typedef struct lr_image {
+    void *rgba;
+    void *display;
+    uint32_t width;
+    uint32_t height;
+} lr_image;
+
+/* Renderer.create (integrator.lucb:343): `frame` is the 16-float RayFrame of
+ * the settings' size. The scene must outlive the renderer. */
+int lr_renderer_create(lr_device *device, lr_gpu_scene *scene, const lr_loaded *loaded,
+                       const lr_render_settings *settings, const float frame[16], lr_renderer **out);
+/* Renderer.advance (:649): launches the next pass unless two are in flight;
+ * *busy = 0 once every sample is in the image. Never waits for the GPU. */
+int lr_renderer_advance(lr_renderer *renderer, int *busy);
+/* Renderer.finish (:724): every remaining pass, waited for. */
+int lr_renderer_finish(lr_renderer *renderer);
+/* Renderer.samples_done (:732): samples every pixel has in finished passes. */
+uint32_t lr_renderer_samples_done(lr_renderer *renderer);
+/* Renderer.image (:737). */
+int lr_renderer_image(lr_renderer *renderer, lr_image *out);
+/* Renderer.read (:778): width * height * 4 floats, row by row from the top. */
+int lr_renderer_read(lr_renderer *renderer, float *rgba, size_t count);
+/* The film's four planes (sum, half, scratch, splat; common.glsl:64-75),
+ * width * height * 16 floats: no reference counterpart, for tests. */
+int lr_renderer_read_film(lr_renderer *renderer, float *planes, size_t count);
+/* Renderer.kernel_times (:873): milliseconds per kernel kind (profile on),
+ * in the order camera, intersect closest, shade surface, shade miss,
+ * intersect shadow, light emission, finish, film convert, adaptive, rest. */
+int lr_renderer_kernel_times(lr_renderer *renderer, double out[10]);
+/* Renderer.destroy (:887): waits for passes in flight. */
+void lr_renderer_destroy(lr_renderer *renderer);
+
 /* Device buffers and timing: the part of luce-gpu (gpu.Buffer.allocate /
  * create / read / upload, Compute.wait and timestamps; used at
  * src/render/device.lucb:137-156 and integrator.lucb:848-885) that callers of

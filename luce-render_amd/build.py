@@ -26,6 +26,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     "-pthread",
+    "-munsafe-fp-atomics",
 ]
 
 

@@ -1,0 +1,136 @@
+// Path guiding by a hash grid of von Mises–Fisher mixtures (shaders/guide.glsl,
+// shaders/guide_fit.comp): the plain guide, which the learned guide stands in
+// for and which judges it. One buffer in three regions: u32 keys, float4 lobes
+// (4 lobes then the weights per cell), float stats (5 per lobe).
+#pragma once
+
+#include "render_common.h"
+
+namespace lr {
+namespace rt {
+
+constexpr uint32_t GUIDE_CELLS = 524288u;
+constexpr uint32_t GUIDE_LOBES = 4u;
+constexpr uint32_t GUIDE_RING = 3u;
+constexpr uint32_t GUIDE_TRAINING = 1u;
+constexpr uint32_t GUIDE_PROBES = 8u;
+constexpr size_t kGuideBytes = (size_t)GUIDE_CELLS * (4 + 16 * (GUIDE_LOBES + 1) + 4 * 5 * GUIDE_LOBES);
+
+__device__ __forceinline__ bool guiding(const Params &p) { return p.guide != nullptr && K(p, K_GUIDE).x > 0.5f; }
+__device__ __forceinline__ uint32_t *guide_keys(const Params &p) { return reinterpret_cast<uint32_t *>(p.guide); }
+__device__ __forceinline__ float4 *guide_lobes(const Params &p) {
+    return reinterpret_cast<float4 *>(p.guide + (size_t)GUIDE_CELLS * 4);
+}
+__device__ __forceinline__ float *guide_stats(const Params &p) {
+    return reinterpret_cast<float *>(p.guide + (size_t)GUIDE_CELLS * (4 + 16 * (GUIDE_LOBES + 1)));
+}
+
+// guide.glsl:101-122: the cell of point x with normal n, made on first use.
+__device__ __forceinline__ uint32_t guide_cell(const Params &p, V3 x, V3 n) {
+    float minimum = K(p, K_GUIDE_SIZE).x;
+    V3 cam{p.frame[0], p.frame[1], p.frame[2]};
+    float wanted = fmaxf(minimum, distance(x, cam) * K(p, K_GUIDE).y);
+    uint32_t level = (uint32_t)clampf(ceilf(log2f(wanted / minimum)), 0.0f, 31.0f);
+    float size = minimum * exp2f((float)level);
+    int qx = (int)floorf(x.x / size), qy = (int)floorf(x.y / size), qz = (int)floorf(x.z / size);
+    uint32_t octant = (n.x >= 0.0f ? 1u : 0u) | (n.y >= 0.0f ? 2u : 0u) | (n.z >= 0.0f ? 4u : 0u);
+    uint32_t key = hash_u32((uint32_t)qx * 73856093u ^
+                            hash_u32((uint32_t)qy * 19349663u ^ hash_u32((uint32_t)qz * 83492791u ^ (octant | (level << 3u)))));
+    key = key > 1u ? key : 1u;
+    uint32_t slot = key % GUIDE_CELLS;
+    uint32_t *keys = guide_keys(p);
+    for (uint32_t probe = 0; probe < GUIDE_PROBES; probe++) {
+        uint32_t at = (slot + probe) % GUIDE_CELLS;
+        uint32_t seen = __atomic_load_n(&keys[at], __ATOMIC_RELAXED);
+        if (seen == key) return at;
+        if (seen != 0u) continue;
+        uint32_t found = atomicCAS(&keys[at], 0u, key);
+        if (found == 0u || found == key) return at;
+    }
+    return kNoneU;
+}
+
+__device__ __forceinline__ float guide_count(const Params &p, uint32_t cell) {
+    const float *s = guide_stats(p);
+    float c = 0.0f;
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) c += s[(cell * GUIDE_LOBES + k) * 5u + 4u];
+    return c;
+}
+
+__device__ __forceinline__ float4 guide_lobe(const Params &p, uint32_t cell, uint32_t lobe) {
+    float4 v = guide_lobes(p)[cell * (GUIDE_LOBES + 1u) + lobe];
+    if (v.w > 0.0f) return v;
+    const float k = 0.57735026919f;  // the tetrahedron's directions, normalized
+    float sx = (lobe == 0 || lobe == 1) ? k : -k;
+    float sy = (lobe == 0 || lobe == 2) ? k : -k;
+    float sz = (lobe == 0 || lobe == 3) ? k : -k;
+    return make_float4(sx, sy, sz, 1.0f);
+}
+
+__device__ __forceinline__ float4 guide_weights(const Params &p, uint32_t cell) {
+    float4 w = guide_lobes(p)[cell * (GUIDE_LOBES + 1u) + GUIDE_LOBES];
+    float total = w.x + w.y + w.z + w.w;
+    return total > 0.0f ? w / total : make_float4(0.25f, 0.25f, 0.25f, 0.25f);
+}
+
+__device__ __forceinline__ float vmf_pdf(float4 lobe, V3 d) {
+    float kappa = lobe.w;
+    if (kappa < 1e-3f) return 1.0f / (4.0f * kPi);
+    return kappa / (2.0f * kPi * (1.0f - expf(-2.0f * kappa))) * expf(kappa * (dot(v3(lobe), d) - 1.0f));
+}
+
+__device__ __forceinline__ V3 vmf_sample(float4 lobe, float ux, float uy) {
+    float kappa = fmaxf(lobe.w, 1e-3f);
+    float w = 1.0f + logf(ux + (1.0f - ux) * expf(-2.0f * kappa)) / kappa;
+    float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+    float phi = 2.0f * kPi * uy;
+    V3 t, b;
+    basis(v3(lobe), t, b);
+    return normalize(t * (s * cosf(phi)) + b * (s * sinf(phi)) + v3(lobe) * w);
+}
+
+__device__ __forceinline__ float guide_pdf(const Params &p, uint32_t cell, V3 d) {
+    float4 w = guide_weights(p, cell);
+    float pdf = 0.0f;
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) pdf += lane(w, (int)k) * vmf_pdf(guide_lobe(p, cell, k), d);
+    return pdf;
+}
+
+__device__ __forceinline__ V3 guide_sample(const Params &p, uint32_t cell, float ux, float uy, float uz) {
+    float4 w = guide_weights(p, cell);
+    uint32_t lobe = GUIDE_LOBES - 1u;
+    float pick = uz;
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) {
+        if (pick < lane(w, (int)k)) {
+            lobe = k;
+            break;
+        }
+        pick -= lane(w, (int)k);
+    }
+    return vmf_sample(guide_lobe(p, cell, lobe), ux, uy);
+}
+
+// guide.glsl:183-201: credit the lobe that explains d best.
+__device__ __forceinline__ void guide_train(const Params &p, uint32_t cell, V3 d, float value, float pdf) {
+    if (cell == kNoneU || !(value > 0.0f) || !(pdf > 0.0f) || isinf(value)) return;
+    float4 w = guide_weights(p, cell);
+    uint32_t best = 0;
+    float best_score = -1.0f;
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) {
+        float score = (lane(w, (int)k) + 0.05f) * vmf_pdf(guide_lobe(p, cell, k), d);
+        if (score > best_score) {
+            best_score = score;
+            best = k;
+        }
+    }
+    float x = fminf(value / pdf, 1e4f);
+    float *s = guide_stats(p) + (size_t)(cell * GUIDE_LOBES + best) * 5u;
+    atomicAdd(&s[0], x);
+    atomicAdd(&s[1], x * d.x);
+    atomicAdd(&s[2], x * d.y);
+    atomicAdd(&s[3], x * d.z);
+    atomicAdd(&s[4], 1.0f);
+}
+
+}  // namespace rt
+}  // namespace lr

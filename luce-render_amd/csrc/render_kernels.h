@@ -1,0 +1,26 @@
+// Launchers of the wavefront integrator's kernels.
+#pragma once
+
+#include <algorithm>
+
+#include "render_common.h"
+
+namespace lr {
+
+enum class Stage { BeginPass, Camera, LightEmit, Shade, Miss, Finish, GuideFit, Convert, EndPass };
+
+// The tracing kernels of a pass for one (width, stack, group) variant.
+struct RenderTraceVariant {
+    void (*closest)(const rt::Params &p, uint32_t current, uint32_t blocks, hipStream_t s);
+    void (*shadow)(const rt::Params &p, uint32_t blocks, hipStream_t s);
+    uint32_t group;
+};
+const RenderTraceVariant *render_trace_variants();
+
+// `arg`: the closest-hit queue being read (Shade) or "convert every pass"
+// (Convert).
+void launch_render_stage(Stage stage, bool spectral, const rt::Params &p, uint32_t arg, uint32_t blocks, hipStream_t s);
+void launch_tables(float *tables, hipStream_t s);
+void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s);
+
+}  // namespace lr

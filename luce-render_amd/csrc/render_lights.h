@@ -1,0 +1,364 @@
+// UsdLux lights and the light tree on the device: sampling for next-event
+// estimation, analytic hits for BSDF rays, and the pdfs MIS needs
+// (shaders/lights.glsl:24-308, shaders/lighttree.glsl:18-137).
+#pragma once
+
+#include "render_common.h"
+
+namespace lr {
+namespace rt {
+
+constexpr uint32_t LIGHT_STRIDE = 7;
+constexpr uint32_t LIGHT_RECT = 0, LIGHT_DISK = 1, LIGHT_SPHERE = 2, LIGHT_DISTANT = 3, LIGHT_DOME = 4;
+constexpr uint32_t kLeafNode = 0x80000000u;
+constexpr uint32_t kEmitterTriangle = 0x80000000u;
+
+__device__ __forceinline__ float4 light_at(const Params &p, uint32_t light, uint32_t row) {
+    return p.lights[light * LIGHT_STRIDE + row];
+}
+__device__ __forceinline__ uint32_t light_type(const Params &p, uint32_t light) {
+    return (uint32_t)light_at(p, light, 0).w;
+}
+__device__ __forceinline__ bool light_visible(const Params &p, uint32_t light) {
+    return ((uint32_t)light_at(p, light, 5).w & 1u) != 0u;
+}
+__device__ __forceinline__ float light_select_pdf(const Params &p, uint32_t light) {
+    return K(p, K_TREE).z * light_at(p, light, 6).y;
+}
+__device__ __forceinline__ uint32_t light_emitter(const Params &p, uint32_t light) {
+    return (uint32_t)light_at(p, light, 6).w;
+}
+__device__ __forceinline__ uint32_t light_count(const Params &p) { return (uint32_t)K(p, K_INFO).x; }
+
+template <bool SPECTRAL> __device__ __forceinline__ Spec light_radiance(const Params &p, uint32_t light, Spec lambda) {
+    float4 rgb = light_at(p, light, 5);
+    Spec r = spec_of<SPECTRAL>(light_at(p, light, 4), rgb, lambda);
+    if constexpr (SPECTRAL) {
+        if (((uint32_t)rgb.w & 2u) != 0u) {
+            float kelvin = light_at(p, light, 6).x;
+            r = make_float4(r.x * blackbody(lambda.x, kelvin), r.y * blackbody(lambda.y, kelvin),
+                            r.z * blackbody(lambda.z, kelvin), r.w * blackbody(lambda.w, kelvin));
+        }
+    }
+    return r;
+}
+
+// A distant or dome light by power; `pdf` its share among them.
+__device__ __forceinline__ uint32_t pick_distant(const Params &p, float u, float &pdf) {
+    uint32_t n = light_count(p), chosen = kNoneU;
+    for (uint32_t l = 0; l < n; l++) {
+        if (light_type(p, l) < LIGHT_DISTANT) continue;
+        chosen = l;
+        if (u < light_at(p, l, 6).z) break;
+    }
+    pdf = chosen == kNoneU ? 0.0f : light_at(p, chosen, 6).y;
+    return chosen;
+}
+
+struct LightSample {
+    V3 direction;
+    float distance;
+    float pdf;
+    float cos_light;
+    bool delta;
+};
+
+__device__ __forceinline__ V3 sample_cone(V3 axis, float cos_max, float ux, float uy) {
+    float c = 1.0f - ux * (1.0f - cos_max);
+    float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    float phi = 2.0f * kPi * uy;
+    V3 t, b;
+    basis(axis, t, b);
+    return normalize(t * (s * cosf(phi)) + b * (s * sinf(phi)) + axis * c);
+}
+
+__device__ __forceinline__ float sphere_hit(V3 center, float radius, V3 o, V3 d) {
+    V3 oc = o - center;
+    float b = dot(oc, d);
+    float c = dot(oc, oc) - radius * radius;
+    float h = b * b - c;
+    if (h < 0.0f) return kInf;
+    h = sqrtf(h);
+    float t = -b - h;
+    if (t > 0.0f) return t;
+    t = -b + h;
+    return t > 0.0f ? t : kInf;
+}
+
+// Dome lights: untextured ones shine evenly (the only kind this build packs;
+// a textured dome is outside the measured workloads).
+__device__ __forceinline__ float dome_pdf(const Params &, uint32_t, V3) { return 1.0f / (4.0f * kPi); }
+
+template <bool SPECTRAL> __device__ __forceinline__ Spec dome_radiance(const Params &p, uint32_t light, V3, Spec lambda) {
+    return light_radiance<SPECTRAL>(p, light, lambda);
+}
+
+__device__ __forceinline__ bool sample_dome(const Params &p, uint32_t light, float ux, float uy, LightSample &s) {
+    s.distance = kInf;
+    s.delta = false;
+    s.cos_light = 1.0f;
+    float z = 1.0f - 2.0f * ux;
+    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    V3 l{r * cosf(2.0f * kPi * uy), z, r * sinf(2.0f * kPi * uy)};
+    s.direction = normalize(v3(light_at(p, light, 1)) * l.x + v3(light_at(p, light, 2)) * l.y +
+                            v3(light_at(p, light, 3)) * l.z);
+    s.pdf = dome_pdf(p, light, s.direction);
+    return s.pdf > 0.0f;
+}
+
+// lights.glsl:176-224
+__device__ __forceinline__ bool sample_light(const Params &p, uint32_t light, V3 x, float ux, float uy,
+                                             LightSample &s) {
+    float4 r0 = light_at(p, light, 0), r1 = light_at(p, light, 1), r2 = light_at(p, light, 2),
+           r3 = light_at(p, light, 3);
+    uint32_t type = (uint32_t)r0.w;
+    s.delta = false;
+    s.cos_light = 1.0f;
+    if (type == LIGHT_DOME) return sample_dome(p, light, ux, uy, s);
+    if (type == LIGHT_RECT || type == LIGHT_DISK) {
+        float ax, ay;
+        if (type == LIGHT_RECT) {
+            ax = 2.0f * ux - 1.0f;
+            ay = 2.0f * uy - 1.0f;
+        } else {
+            float2 c = concentric(ux, uy);
+            ax = c.x;
+            ay = c.y;
+        }
+        V3 q = v3(r0) + v3(r1) * ax + v3(r2) * ay;
+        V3 d = q - x;
+        float d2 = dot(d, d);
+        s.distance = sqrtf(d2);
+        s.direction = d / s.distance;
+        float cos_light = -dot(s.direction, v3(r3));
+        if (cos_light <= 0.0f || r3.w <= 0.0f) return false;
+        s.pdf = d2 / (cos_light * r3.w);
+        s.cos_light = cos_light;
+        return true;
+    }
+    if (type == LIGHT_SPHERE) {
+        V3 to_center = v3(r0) - x;
+        float d2 = dot(to_center, to_center);
+        float r = r1.w;
+        if (d2 <= r * r) return false;
+        float cos_max = sqrtf(fmaxf(0.0f, 1.0f - r * r / d2));
+        s.direction = sample_cone(to_center / sqrtf(d2), cos_max, ux, uy);
+        s.distance = sphere_hit(v3(r0), r, x, s.direction);
+        if (s.distance >= kInf) return false;
+        s.pdf = 1.0f / (2.0f * kPi * (1.0f - cos_max));
+        s.cos_light = fmaxf(0.0f, -dot(s.direction, normalize(x + s.direction * s.distance - v3(r0))));
+        return true;
+    }
+    V3 axis = -v3(r3);
+    float cos_half = r2.w;
+    s.distance = kInf;
+    if (cos_half >= 1.0f - 1e-7f) {
+        s.direction = axis;
+        s.pdf = 1.0f;
+        s.delta = true;
+        return true;
+    }
+    s.direction = sample_cone(axis, cos_half, ux, uy);
+    s.pdf = 1.0f / (2.0f * kPi * (1.0f - cos_half));
+    return true;
+}
+
+__device__ __forceinline__ float light_cos(const Params &p, uint32_t light, V3 o, V3 d, float t) {
+    float4 r0 = light_at(p, light, 0);
+    if ((uint32_t)r0.w == LIGHT_SPHERE) return fmaxf(0.0f, -dot(d, normalize(o + d * t - v3(r0))));
+    return fmaxf(0.0f, -dot(d, v3(light_at(p, light, 3))));
+}
+
+// lights.glsl:239-259: the nearer hit on the emitting side, or kInf.
+__device__ __forceinline__ float intersect_light(const Params &p, uint32_t light, V3 o, V3 d, float t_max,
+                                                 bool light_flow) {
+    float4 r0 = light_at(p, light, 0);
+    uint32_t type = (uint32_t)r0.w;
+    if (type == LIGHT_SPHERE) {
+        float t = sphere_hit(v3(r0), light_at(p, light, 1).w, o, d);
+        return t < t_max ? t : kInf;
+    }
+    if (type >= LIGHT_DISTANT) return kInf;
+    V3 u = v3(light_at(p, light, 1)), v = v3(light_at(p, light, 2)), n = v3(light_at(p, light, 3));
+    float facing = dot(d, n);
+    if (light_flow ? facing <= 0.0f : facing >= 0.0f) return kInf;
+    float t = dot(v3(r0) - o, n) / facing;
+    if (t <= 0.0f || t >= t_max) return kInf;
+    V3 q = o + d * t - v3(r0);
+    float a = dot(q, u) / dot(u, u);
+    float b = dot(q, v) / dot(v, v);
+    bool inside = type == LIGHT_RECT ? (fabsf(a) <= 1.0f && fabsf(b) <= 1.0f) : (a * a + b * b <= 1.0f);
+    return inside ? t : kInf;
+}
+
+__device__ __forceinline__ float light_hit_pdf(const Params &p, uint32_t light, V3 o, V3 d, float t) {
+    float4 r0 = light_at(p, light, 0);
+    uint32_t type = (uint32_t)r0.w;
+    if (type == LIGHT_SPHERE) {
+        V3 to_center = v3(r0) - o;
+        float r = light_at(p, light, 1).w;
+        float cos_max = sqrtf(fmaxf(0.0f, 1.0f - r * r / dot(to_center, to_center)));
+        return 1.0f / (2.0f * kPi * fmaxf(1e-12f, 1.0f - cos_max));
+    }
+    if (type == LIGHT_DISTANT) return 1.0f / (2.0f * kPi * fmaxf(1e-12f, 1.0f - light_at(p, light, 2).w));
+    if (type == LIGHT_DOME) return dome_pdf(p, light, d);
+    float4 r3 = light_at(p, light, 3);
+    float cos_light = fmaxf(1e-12f, -dot(d, v3(r3)));
+    return t * t / (cos_light * r3.w);
+}
+
+// Light paths from afar (lights.glsl:290-308).
+__device__ __forceinline__ bool emits_from_afar(const Params &p, uint32_t light) {
+    uint32_t type = light_type(p, light);
+    return type == LIGHT_DOME || (type == LIGHT_DISTANT && light_at(p, light, 2).w < 1.0f - 1e-7f);
+}
+__device__ __forceinline__ float afar_emission_pdf(const Params &p, uint32_t light, V3 w) {
+    return K(p, K_EMIT).x * light_at(p, light, 6).y * light_hit_pdf(p, light, V3{0, 0, 0}, w, kInf);
+}
+__device__ __forceinline__ float afar_first_density(const Params &p, V3 x, V3 w, float c) {
+    float radius = K(p, K_EMIT).y;
+    V3 offset = x - v3(K(p, K_EMIT_CENTER));
+    V3 across = offset - w * dot(offset, w);
+    if (dot(across, across) > radius * radius) return 0.0f;
+    return c / (kPi * radius * radius);
+}
+
+// ---------------------------------------------------------------- light tree
+
+__device__ __forceinline__ float safe_sqrt(float x) { return sqrtf(fmaxf(0.0f, x)); }
+
+// pbrt-v4's importance of tree node `node` for point x with normal n
+// (lighttree.glsl:326-354).
+__device__ __forceinline__ float tree_importance(const Params &p, uint32_t node, V3 x, V3 n) {
+    uint32_t base = (uint32_t)K(p, K_TREE).y + node * 3u;
+    float4 a = p.lights[base], b = p.lights[base + 1], c = p.lights[base + 2];
+    if (a.w <= 0.0f) return 0.0f;
+    V3 center = (v3(a) + v3(b)) * 0.5f;
+    V3 offset = x - center;
+    float radius = 0.5f * length(v3(b) - v3(a));
+    float d2o = dot(offset, offset);
+    float distance2 = fmaxf(d2o, radius);
+    V3 wi = d2o > 0.0f ? normalize(offset) : v3(c);
+    float cos_w = dot(v3(c), wi);
+    float sin_w = safe_sqrt(1.0f - cos_w * cos_w);
+    float cos_b = d2o < radius * radius ? -1.0f : safe_sqrt(1.0f - radius * radius / d2o);
+    float sin_b = safe_sqrt(1.0f - cos_b * cos_b);
+    float cos_o = b.w;
+    float sin_o = safe_sqrt(1.0f - cos_o * cos_o);
+    float cos_x = cos_w > cos_o ? 1.0f : cos_w * cos_o + sin_w * sin_o;
+    float sin_x = cos_w > cos_o ? 0.0f : sin_w * cos_o - cos_w * sin_o;
+    float cos_p = cos_x > cos_b ? 1.0f : cos_x * cos_b + sin_x * sin_b;
+    if (cos_p <= c.w) return 0.0f;
+    float importance = a.w * cos_p / distance2;
+    float cos_i = fabsf(dot(wi, n));
+    float sin_i = safe_sqrt(1.0f - cos_i * cos_i);
+    float cos_pi = cos_i > cos_b ? 1.0f : cos_i * cos_b + sin_i * sin_b;
+    return fmaxf(importance * cos_pi, 0.0f);
+}
+
+__device__ __forceinline__ bool tree_sample(const Params &p, float u, V3 x, V3 n, uint32_t &emitter, float &pdf) {
+    pdf = 1.0f;
+    emitter = 0;
+    if (K(p, K_TREE).x < 1.0f) return false;
+    uint32_t node = 0;
+    for (uint32_t depth = 0; depth < 64u; depth++) {
+        uint32_t link = p.emitters[node * 2u];
+        if ((p.emitters[node * 2u + 1u] & kLeafNode) != 0u) {
+            emitter = link;
+            return pdf > 0.0f;
+        }
+        float left = tree_importance(p, link, x, n);
+        float right = tree_importance(p, link + 1u, x, n);
+        if (left + right <= 0.0f) return false;
+        float chance = left / (left + right);
+        if (u < chance) {
+            u = fminf(u / chance, 0.99999994f);
+            pdf *= chance;
+            node = link;
+        } else {
+            u = fminf((u - chance) / (1.0f - chance), 0.99999994f);
+            pdf *= 1.0f - chance;
+            node = link + 1u;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ float tree_pdf(const Params &p, uint32_t emitter, V3 x, V3 n) {
+    uint32_t record = (uint32_t)K(p, K_EMITTERS).x + emitter * 4u;
+    uint32_t trail = p.emitters[record + 1u], depth = p.emitters[record + 2u];
+    uint32_t node = 0;
+    float pdf = 1.0f;
+    for (uint32_t level = 0; level < depth; level++) {
+        uint32_t link = p.emitters[node * 2u];
+        float left = tree_importance(p, link, x, n);
+        float right = tree_importance(p, link + 1u, x, n);
+        if (left + right <= 0.0f) return 0.0f;
+        uint32_t bit = (trail >> level) & 1u;
+        pdf *= bit == 0u ? left / (left + right) : right / (left + right);
+        node = link + bit;
+    }
+    return pdf;
+}
+
+__device__ __forceinline__ uint32_t emitter_code(const Params &p, uint32_t emitter) {
+    return p.emitters[(uint32_t)K(p, K_EMITTERS).x + emitter * 4u];
+}
+__device__ __forceinline__ uint32_t triangle_emitter(const Params &p, uint32_t triangle) {
+    return p.emitters[(uint32_t)K(p, K_EMITTERS).y + triangle];
+}
+
+// A leaf-order triangle's corners from the repacked triangle array (p0, e1, e2
+// with e1 = p1 - p0, e2 = p2 - p0 as f32: the same edges shading computes).
+__device__ __forceinline__ void triangle_edges(const Params &p, uint32_t tri, V3 &p0, V3 &e1, V3 &e2) {
+    const float4 a = p.scene.tris[(size_t)tri * 3], b = p.scene.tris[(size_t)tri * 3 + 1],
+                 c = p.scene.tris[(size_t)tri * 3 + 2];
+    p0 = V3{a.x, a.y, a.z};
+    e1 = V3{a.w, b.x, b.y};
+    e2 = V3{b.z, b.w, c.x};
+}
+
+__device__ __forceinline__ float4 material_at(const Params &p, uint32_t m, uint32_t row) {
+    return p.materials[m * 16u + row];
+}
+template <bool SPECTRAL> __device__ __forceinline__ Spec material_color(const Params &p, uint32_t m, uint32_t row,
+                                                                       Spec lambda) {
+    return spec_of<SPECTRAL>(material_at(p, m, row), material_at(p, m, row + 1u), lambda);
+}
+
+// An emissive triangle sampled uniformly by area (lighttree.glsl:415-435).
+template <bool SPECTRAL>
+__device__ __forceinline__ bool sample_triangle(const Params &p, uint32_t tri, V3 x, float ux, float uy, Spec lambda,
+                                                LightSample &s, Spec &radiance) {
+    V3 a, e1, e2;
+    triangle_edges(p, tri, a, e1, e2);
+    V3 b = a + e1, c = a + e2;
+    float r = sqrtf(ux);
+    V3 q = a * (1.0f - r) + b * (r * (1.0f - uy)) + c * (r * uy);
+    V3 cross_ab = cross(e1, e2);
+    float area = 0.5f * length(cross_ab);
+    V3 d = q - x;
+    float d2 = dot(d, d);
+    if (area <= 0.0f || d2 <= 0.0f) return false;
+    s.distance = sqrtf(d2);
+    s.direction = d / s.distance;
+    float cos_light = fabsf(dot(s.direction, cross_ab)) / (2.0f * area);
+    if (cos_light <= 0.0f) return false;
+    s.pdf = d2 / (cos_light * area);
+    s.cos_light = cos_light;
+    s.delta = false;
+    radiance = material_color<SPECTRAL>(p, p.shading[tri].w, 6u, lambda);
+    return true;
+}
+
+__device__ __forceinline__ float triangle_hit_pdf(const Params &p, uint32_t tri, V3 d, float t) {
+    V3 a, e1, e2;
+    triangle_edges(p, tri, a, e1, e2);
+    V3 cross_ab = cross(e1, e2);
+    float area = 0.5f * length(cross_ab);
+    float cos_light = fabsf(dot(d, cross_ab)) / fmaxf(1e-20f, 2.0f * area);
+    return t * t / fmaxf(1e-20f, cos_light * area);
+}
+
+}  // namespace rt
+}  // namespace lr

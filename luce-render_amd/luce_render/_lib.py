@@ -48,6 +48,23 @@ class SceneInfo(ctypes.Structure):
                 ("triangle_bytes", ctypes.c_size_t), ("stack_bound", ctypes.c_uint32 * 3)]
 
 
+class Loaded(ctypes.Structure):
+    _fields_ = [("lights", ctypes.c_void_p), ("light_floats", ctypes.c_size_t), ("constants", ctypes.c_void_p),
+                ("materials", ctypes.c_void_p), ("material_floats", ctypes.c_size_t), ("emitters", ctypes.c_void_p),
+                ("emitter_words", ctypes.c_size_t), ("light_paths", ctypes.c_uint32), ("through", ctypes.c_uint32)]
+
+
+class RenderSettingsC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("width", "height", "samples", "max_bounces", "spectral", "pool", "seed",
+                                                "guiding", "profile", "use_graph", "trace_width", "trace_group",
+                                                "trace_stack")]
+
+
+class Image(ctypes.Structure):
+    _fields_ = [("rgba", ctypes.c_void_p), ("display", ctypes.c_void_p), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32)]
+
+
 _P = ctypes.c_void_p
 _SIGNATURES = {
     "lr_device_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
@@ -64,6 +81,16 @@ _SIGNATURES = {
     "lr_trace_closest_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
     "lr_trace_any_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
     "lr_scene_info_get": (ctypes.c_int, [_P, ctypes.POINTER(SceneInfo)]),
+    "lr_renderer_create": (ctypes.c_int, [_P, _P, ctypes.POINTER(Loaded), ctypes.POINTER(RenderSettingsC), _P,
+                                          ctypes.POINTER(_P)]),
+    "lr_renderer_advance": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "lr_renderer_finish": (ctypes.c_int, [_P]),
+    "lr_renderer_samples_done": (ctypes.c_uint32, [_P]),
+    "lr_renderer_image": (ctypes.c_int, [_P, ctypes.POINTER(Image)]),
+    "lr_renderer_read": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "lr_renderer_read_film": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "lr_renderer_kernel_times": (ctypes.c_int, [_P, _P]),
+    "lr_renderer_destroy": (None, [_P]),
     "lr_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "lr_buffer_free": (None, [_P, _P]),
     "lr_buffer_write": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),

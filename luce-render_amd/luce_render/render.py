@@ -10,7 +10,7 @@ ray_frame / look_frame, probe (src/render/device.lucb, camera.lucb).
 """
 import ctypes
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 
 import numpy as np
 
@@ -309,3 +309,127 @@ class Timer:
     def destroy(self):
         _lib.lib().lr_event_destroy(self.start)
         _lib.lib().lr_event_destroy(self.stop)
+
+
+# ------------------------------------------------------------------ renderer
+
+from .loading import Loaded, OpenPbr, RenderSettings  # noqa: E402
+
+KERNEL_NAMES = ["camera", "intersect closest", "shade surface", "shade miss", "intersect shadow", "light emission",
+                "finish", "film convert", "adaptive", "later rounds"]
+
+
+def find_camera(scene, index=-1):
+    """find_camera (integrator.lucb:955-965): the settings' camera or the first."""
+    if not scene.cameras:
+        raise LuceError(INVALID, "the scene has no camera")
+    if index < 0:
+        return scene.cameras[0]
+    if index >= len(scene.cameras):
+        raise LuceError(INVALID, "no camera of that index")
+    return scene.cameras[index]
+
+
+class Renderer:
+    """Renderer (integrator.lucb:259-903): create, advance, finish,
+    samples_done, image, read, kernel_times, destroy."""
+
+    def __init__(self):
+        self.handle = None
+
+    @staticmethod
+    def create(device, scene, settings: RenderSettings, gpu_scene=None):
+        """Renderer.create (integrator.lucb:343): through the settings' camera.
+        `gpu_scene` reuses an uploaded scene (else the scene is uploaded)."""
+        if settings.width == 0 or settings.height == 0 or settings.width > 16384 or settings.height > 16384:
+            raise LuceError(INVALID, "an image is 1 to 16384 pixels on a side")
+        frame = ray_frame(find_camera(scene, settings.camera), settings.width, settings.height)
+        return Renderer.create_viewed(device, scene, settings, frame, gpu_scene)
+
+    @staticmethod
+    def create_viewed(device, scene, settings: RenderSettings, frame: RayFrame, gpu_scene=None):
+        if settings.samples == 0 or settings.max_bounces == 0 or settings.max_bounces > 1024:
+            raise LuceError(INVALID, "a render needs samples and 1 to 1024 bounces")
+        r = Renderer()
+        r.device = device
+        r.settings = replace(settings, width=frame.width, height=frame.height)
+        r.frame = frame
+        r.owned_scene = gpu_scene is None
+        r.gpu_scene = gpu_scene if gpu_scene is not None else GpuScene.upload(device, scene)
+        r.loaded = Loaded(scene, r.settings, frame)
+        ld = _lib.Loaded()
+        ld.lights, ld.light_floats = _lib.ptr(r.loaded.lights), len(r.loaded.lights)
+        ld.constants = _lib.ptr(r.loaded.constants)
+        ld.materials, ld.material_floats = _lib.ptr(r.loaded.materials), len(r.loaded.materials)
+        ld.emitters, ld.emitter_words = _lib.ptr(r.loaded.emitters), len(r.loaded.emitters)
+        ld.light_paths, ld.through = r.loaded.light_paths, r.loaded.through
+        s = r.settings
+        cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, s.pool, s.seed,
+                                  1 if s.guiding else 0, 1 if s.profile else 0, 1 if s.use_graph else 0, *s.trace)
+        r.frame16 = frame.pack()
+        h = ctypes.c_void_p()
+        try:
+            _lib.check(_lib.lib().lr_renderer_create(device.handle, r.gpu_scene.handle, ctypes.byref(ld),
+                                                     ctypes.byref(cs), _lib.ptr(r.frame16), ctypes.byref(h)))
+        except Exception:
+            if r.owned_scene:
+                r.gpu_scene.destroy()
+            raise
+        r.handle = h
+        return r
+
+    def advance(self):
+        busy = ctypes.c_int(0)
+        _lib.check(_lib.lib().lr_renderer_advance(self.handle, ctypes.byref(busy)))
+        return bool(busy.value)
+
+    def finish(self):
+        _lib.check(_lib.lib().lr_renderer_finish(self.handle))
+
+    def samples_done(self):
+        return int(_lib.lib().lr_renderer_samples_done(self.handle))
+
+    def image(self):
+        im = _lib.Image()
+        _lib.check(_lib.lib().lr_renderer_image(self.handle, ctypes.byref(im)))
+        return im
+
+    def read(self, out=None):
+        """The image as rgba32f (height, width, 4), row by row from the top."""
+        n = self.settings.width * self.settings.height * 4
+        if out is not None and out.size != n:
+            raise LuceError(INVALID, "reading the image needs four floats a pixel")
+        buf = np.empty(n, np.float32) if out is None else out
+        _lib.check(_lib.lib().lr_renderer_read(self.handle, _lib.ptr(buf), n))
+        return buf.reshape(self.settings.height, self.settings.width, 4)
+
+    def read_film(self):
+        n = self.settings.width * self.settings.height * 16
+        buf = np.empty(n, np.float32)
+        _lib.check(_lib.lib().lr_renderer_read_film(self.handle, _lib.ptr(buf), n))
+        return buf.reshape(4, self.settings.height, self.settings.width, 4)
+
+    def kernel_times(self):
+        out = np.zeros(10, np.float64)
+        _lib.check(_lib.lib().lr_renderer_kernel_times(self.handle, _lib.ptr(out)))
+        return dict(zip(KERNEL_NAMES, out.tolist()))
+
+    def destroy(self):
+        if self.handle:
+            _lib.lib().lr_renderer_destroy(self.handle)
+            self.handle = None
+        if getattr(self, "owned_scene", False) and self.gpu_scene is not None:
+            self.gpu_scene.destroy()
+            self.gpu_scene = None
+
+
+def render(device, scene, settings, gpu_scene=None):
+    """Create, finish and read a render (tests/gpu/integrator.lucb:69-76)."""
+    r = Renderer.create(device, scene, settings, gpu_scene)
+    try:
+        r.finish()
+        if r.samples_done() != settings.samples:
+            raise LuceError(INVALID, f"finished with {r.samples_done()} of {settings.samples} samples")
+        return r.read().copy()
+    finally:
+        r.destroy()

@@ -73,7 +73,7 @@ class Light:
     radius: float = 0.5
     angle: float = 0.53
     visible: bool = True
-    spectrum: Optional[np.ndarray] = None  # tabulated emission (64 bins 380..720 nm), else flat/colour
+    temperature: float = 0.0
 
 
 @dataclass

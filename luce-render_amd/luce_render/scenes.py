@@ -271,3 +271,67 @@ def instanced_set(meshes=64, instances=1000, frequency=39, radius=0.6, extent=20
     # Camera outside the volume looking down -Z at its centre.
     gset.cameras.append(Camera(placement(translation=(0.0, 0.0, half + 14.0)), "perspective", 30.0, 36.0))
     return gset
+
+
+# ------------------------------------------------------------- render scenes
+
+BAND_BINS = 64
+BAND_LAMBDA = np.linspace(380.0, 720.0, BAND_BINS)
+
+
+def band_spectrum(low, high, value=0.65):
+    """A 64-bin 380..720 nm reflectance: `value` inside [low, high] nm, 0 outside."""
+    return np.where((BAND_LAMBDA >= low) & (BAND_LAMBDA <= high), value, 0.0)
+
+
+def spectrum_to_acescg(reflectance):
+    """A tabulated reflectance (64 bins over 380..720 nm, linear between bins,
+    0 outside) as the ACEScg colour of its XYZ under E, through the same
+    colour-matching fit and film matrix the renderer uses. The spectral build
+    renders its sigmoid fit (fit_spectrum), the RGB build this colour."""
+    from . import spectrum as S
+    lam = np.arange(S.LAMBDA_MIN, S.LAMBDA_MAX + 0.5, 1.0)
+    r = np.interp(lam, BAND_LAMBDA, reflectance, left=0.0, right=0.0)
+    xyz = (S.cmf(lam) * r[:, None]).sum(axis=0) / S.cmf_integrals()
+    return tuple(float(c) for c in S.film_matrix() @ xyz)
+
+
+def lambert(color):
+    """A Lambertian OpenPBR material record: no specular, roughness 0."""
+    return {"base_color": tuple(color), "specular_weight": 0.0, "base_diffuse_roughness": 0.0}
+
+
+def cornell_render_set(random_count=0, light_radiance=17.0):
+    """The Cornell render workload: the probe's box with Lambert walls (0.73
+    white; 0.65 red band 600..720 nm on the left, 0.65 green band 500..570 nm
+    on the right) and a 0.25 x 0.25 rect light at y = 0.999 facing down,
+    radiance 17 (flat spectrum)."""
+    gset = _cornell_geometry()
+    gset.materials = {
+        "white": lambert((0.73, 0.73, 0.73)),
+        "red": lambert(spectrum_to_acescg(band_spectrum(600.0, 720.0))),
+        "green": lambert(spectrum_to_acescg(band_spectrum(500.0, 570.0))),
+    }
+    if random_count:
+        gset.meshes.append(random_triangles(random_count, 1))
+    # The light's local -Z is its emission direction: rows x, y, z with z = +y
+    # world so that it shines down.
+    m = np.eye(4)
+    m[0, :3] = (1.0, 0.0, 0.0)
+    m[1, :3] = (0.0, 0.0, 1.0)
+    m[2, :3] = (0.0, 1.0, 0.0)
+    m[3, :3] = (0.0, 0.999, 0.0)
+    gset.lights.append(Light("rect", m, (1.0, 1.0, 1.0), light_radiance, width=0.25, height=0.25))
+    gset.cameras.append(cornell_camera())
+    return gset
+
+
+def furnace_set(albedo=0.8, frequency=16):
+    """A white furnace: an albedo-`albedo` Lambert sphere (radius 1 at the
+    origin) under a constant environment (the render's background), seen from
+    (0, 0, 4)."""
+    pts, tris = icosphere(frequency)
+    gset = GeometrySet(meshes=[Mesh(pts, tris, np.arange(len(tris)), len(tris), face_materials=["ball"] * len(tris))])
+    gset.materials = {"ball": lambert((albedo, albedo, albedo))}
+    gset.cameras.append(Camera(placement(translation=(0.0, 0.0, 4.0)), "perspective", 50.0, 36.0))
+    return gset

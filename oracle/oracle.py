@@ -40,6 +40,12 @@ def lib():
             "or_check_bvh": (ctypes.c_int, [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t]),
             "or_probe": (None, [_P, _P, _P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P]),
             "or_trace_rays": (None, [_P, _P, _P, _P, _P, ctypes.c_size_t, ctypes.c_float, _P, _P, _P]),
+            "or_render": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, _P, _P, _P,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, ctypes.c_size_t,
+                                         ctypes.c_int, _P]),
+            "or_sample4": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, _P]),
+            "or_make_tables": (None, [_P]),
             "or_brute_force": (None, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_uint32, ctypes.c_uint32, _P,
                                       ctypes.c_size_t, ctypes.c_int, _P, _P, _P]),
         }
@@ -161,3 +167,32 @@ def agrees(got_face, got_t, face, dist, margin):
                     missed)
     differ = (~same) & (margin > 1e-3)
     return same, differ
+
+
+def sample4(seed, pixel, index, group):
+    """sampler.glsl:323-331: the four 24-bit integers and their values."""
+    bits = np.zeros(4, np.uint32)
+    vals = np.zeros(4, np.float64)
+    lib().or_sample4(seed, pixel, index, group, _p(bits), _p(vals))
+    return bits, vals
+
+
+def tables():
+    """tables.comp's albedo tables in f64 (17,440 values)."""
+    t = np.zeros(32 * 32 + 32 + 32 * 32 * 16, np.float64)
+    lib().or_make_tables(_p(t))
+    return t
+
+
+def render(scene, loaded, frame16, width, height, samples, max_bounces, spectral, seed, pixels, threads=8):
+    """The f64 camera-path integrator over the listed pixels: ACEScg (n, 3).
+    `loaded` carries the packed lights/constants/materials/emitters."""
+    pixels = np.ascontiguousarray(pixels, np.uint32)
+    out = np.zeros((len(pixels), 3), np.float64)
+    lib().or_render(_p(np.ascontiguousarray(scene.positions, np.float32)), _p(np.ascontiguousarray(scene.indices, np.uint32)),
+                    _p(np.ascontiguousarray(scene.shading, np.uint32)), scene.triangle_count(),
+                    _p(np.ascontiguousarray(scene.nodes)), len(scene.nodes),
+                    _p(loaded.lights), _p(loaded.constants), _p(loaded.materials), _p(loaded.emitters),
+                    _p(np.ascontiguousarray(frame16, np.float32)), width, height, samples, max_bounces,
+                    1 if spectral else 0, seed, loaded.through, _p(pixels), len(pixels), threads, _p(out))
+    return out

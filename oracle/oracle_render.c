@@ -1,0 +1,1074 @@
+/*
+ * Oracle (test infrastructure only): an f64 CPU restatement of luce-render's
+ * camera-path integrator, one path at a time, drawing the same Owen-scrambled
+ * Sobol numbers as the kernels (integer-exact), so a GPU render and this one
+ * agree pixel by pixel up to f32 rounding and the rare decision it flips.
+ * Nothing in the product links or calls this file.
+ *
+ * Follows, in f64:
+ *   sampler          shaders/sampler.glsl:8-55, sobol_table.glsl
+ *   wavelengths/film shaders/spectrum.glsl:10-118, finish.comp:17-27,
+ *                    film_convert.comp:36-53
+ *   camera           shaders/camera.comp:14-50
+ *   closest + lights shaders/intersect_closest.comp:102-133, lights.glsl
+ *   surface shading  shaders/shade_surface.comp:48-312 (light tracing and
+ *                    guiding off: the two are checked statistically)
+ *   light tree       shaders/lighttree.glsl:326-445
+ *   OpenPBR / GGX    shaders/openpbr.glsl:100-556, microfacet.glsl:9-84;
+ *                    albedo tables by tables.comp's estimator, in f64
+ *   miss             shaders/shade_miss.comp:12-51
+ *   shadow           shaders/intersect_shadow.comp:28-97
+ * Ray casts use the scene's binary BVH with an f64 slab test and f64
+ * Moller-Trumbore (the brute force's arithmetic, tests/gpu/main.lucb:105-119).
+ *
+ * The packed arrays (lights, constants, materials, emitters) are the ones the
+ * renderer receives; the oracle reads them as the kernels do.
+ *
+ * Pinning: the furnace closed form (rho L), the rect-light irradiance closed
+ * form (rho/pi L x the corner-irradiance of a rectangle), f64 brute-force hit
+ * agreement of the caster, and the GPU images themselves in the tests.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+#define PI 3.14159265358979323846
+#define INF 3.0e38
+#define NONE 0xffffffffu
+
+typedef struct { double x, y, z; } v3;
+static v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 mul(v3 a, double s) { return V(a.x * s, a.y * s, a.z * s); }
+static v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 cross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static double len(v3 a) { return sqrt(dot(a, a)); }
+static v3 norm(v3 a) { return mul(a, 1.0 / len(a)); }
+static double clampd(double x, double a, double b) { return x < a ? a : (x > b ? b : x); }
+static double mixd(double a, double b, double t) { return a * (1.0 - t) + b * t; }
+static double fract(double x) { return x - floor(x); }
+
+typedef struct { double v[4]; } spec;
+static spec S1(double s) { spec r = {{s, s, s, s}}; return r; }
+static spec sadd(spec a, spec b) { spec r; for (int i = 0; i < 4; i++) r.v[i] = a.v[i] + b.v[i]; return r; }
+static spec smul(spec a, spec b) { spec r; for (int i = 0; i < 4; i++) r.v[i] = a.v[i] * b.v[i]; return r; }
+static spec sscale(spec a, double s) { spec r; for (int i = 0; i < 4; i++) r.v[i] = a.v[i] * s; return r; }
+
+/* ------------------------------------------------------------- scene */
+
+typedef struct { float lo[3]; uint32_t first; float hi[3]; uint32_t count; } onode;
+
+typedef struct {
+    const float *positions; const uint32_t *indices; const uint32_t *shading; size_t triangles;
+    const onode *nodes; size_t node_count;
+    const float *lights; const float *constants; const float *materials; const uint32_t *emitters;
+    const double *frame;     /* 16 floats of the packed frame, as f64 */
+    uint32_t width, height, samples, max_bounces, spectral, seed, through;
+    double *tables;          /* f64 albedo tables */
+} oscene;
+
+static v3 point_of(const oscene *s, uint32_t i) {
+    return V(s->positions[(size_t)i * 3], s->positions[(size_t)i * 3 + 1], s->positions[(size_t)i * 3 + 2]);
+}
+static double K(const oscene *s, int row, int c) { return s->constants[row * 4 + c]; }
+static v3 Kv(const oscene *s, int row) { return V(K(s, row, 0), K(s, row, 1), K(s, row, 2)); }
+static double L(const oscene *s, uint32_t light, int row, int c) { return s->lights[(light * 7 + row) * 4 + c]; }
+static v3 Lv(const oscene *s, uint32_t light, int row) { return V(L(s, light, row, 0), L(s, light, row, 1), L(s, light, row, 2)); }
+static double M(const oscene *s, uint32_t m, int row, int c) { return s->materials[(m * 16 + row) * 4 + c]; }
+
+/* ------------------------------------------------------------ sampler */
+
+static const uint32_t SOBOL[128] = {
+    0x80000000u, 0x40000000u, 0x20000000u, 0x10000000u, 0x08000000u, 0x04000000u, 0x02000000u, 0x01000000u,
+    0x00800000u, 0x00400000u, 0x00200000u, 0x00100000u, 0x00080000u, 0x00040000u, 0x00020000u, 0x00010000u,
+    0x00008000u, 0x00004000u, 0x00002000u, 0x00001000u, 0x00000800u, 0x00000400u, 0x00000200u, 0x00000100u,
+    0x00000080u, 0x00000040u, 0x00000020u, 0x00000010u, 0x00000008u, 0x00000004u, 0x00000002u, 0x00000001u,
+    0x80000000u, 0xc0000000u, 0xa0000000u, 0xf0000000u, 0x88000000u, 0xcc000000u, 0xaa000000u, 0xff000000u,
+    0x80800000u, 0xc0c00000u, 0xa0a00000u, 0xf0f00000u, 0x88880000u, 0xcccc0000u, 0xaaaa0000u, 0xffff0000u,
+    0x80008000u, 0xc000c000u, 0xa000a000u, 0xf000f000u, 0x88008800u, 0xcc00cc00u, 0xaa00aa00u, 0xff00ff00u,
+    0x80808080u, 0xc0c0c0c0u, 0xa0a0a0a0u, 0xf0f0f0f0u, 0x88888888u, 0xccccccccu, 0xaaaaaaaau, 0xffffffffu,
+    0x80000000u, 0xc0000000u, 0x60000000u, 0x90000000u, 0xe8000000u, 0x5c000000u, 0x8e000000u, 0xc5000000u,
+    0x68800000u, 0x9cc00000u, 0xee600000u, 0x55900000u, 0x80680000u, 0xc09c0000u, 0x60ee0000u, 0x90550000u,
+    0xe8808000u, 0x5cc0c000u, 0x8e606000u, 0xc5909000u, 0x6868e800u, 0x9c9c5c00u, 0xeeee8e00u, 0x5555c500u,
+    0x8000e880u, 0xc0005cc0u, 0x60008e60u, 0x9000c590u, 0xe8006868u, 0x5c009c9cu, 0x8e00eeeeu, 0xc5005555u,
+    0x80000000u, 0xc0000000u, 0x20000000u, 0x50000000u, 0xf8000000u, 0x74000000u, 0xa2000000u, 0x93000000u,
+    0xd8800000u, 0x25400000u, 0x59e00000u, 0xe6d00000u, 0x78080000u, 0xb40c0000u, 0x82020000u, 0xc3050000u,
+    0x208f8000u, 0x51474000u, 0xfbea2000u, 0x75d93000u, 0xa0858800u, 0x914e5400u, 0xdbe79e00u, 0x25db6d00u,
+    0x58800080u, 0xe54000c0u, 0x79e00020u, 0xb6d00050u, 0x800800f8u, 0xc00c0074u, 0x200200a2u, 0x50050093u};
+
+uint32_t or_hash_u32(uint32_t x) {
+    x ^= x >> 16; x *= 0x21f0aaadu; x ^= x >> 15; x *= 0x735a2d97u; x ^= x >> 15;
+    return x;
+}
+static uint32_t hash_combine(uint32_t seed, uint32_t v) { return seed ^ (or_hash_u32(v) + 0x9e3779b9u + (seed << 6) + (seed >> 2)); }
+static uint32_t rev(uint32_t x) {
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+    x = ((x >> 4) & 0x0f0f0f0fu) | ((x & 0x0f0f0f0fu) << 4);
+    x = ((x >> 8) & 0x00ff00ffu) | ((x & 0x00ff00ffu) << 8);
+    return (x >> 16) | (x << 16);
+}
+static uint32_t lk(uint32_t x, uint32_t seed) {
+    x += seed; x ^= x * 0x6c50b47cu; x ^= x * 0xb82f1e52u; x ^= x * 0xc7afe638u; x ^= x * 0x8d22f6e6u;
+    return x;
+}
+static uint32_t nus(uint32_t x, uint32_t seed) { return rev(lk(rev(x), seed)); }
+static uint32_t sobol(uint32_t index, uint32_t d) {
+    uint32_t x = 0;
+    for (uint32_t bit = 0; index; bit++, index >>= 1) if (index & 1u) x ^= SOBOL[d * 32 + bit];
+    return x;
+}
+/* sampler.glsl:323-331; the four 24-bit integers and their values. */
+void or_sample4(uint32_t seed0, uint32_t pixel, uint32_t index, uint32_t group, uint32_t bits[4], double out[4]) {
+    uint32_t seed = hash_combine(hash_combine(seed0, pixel), group);
+    uint32_t shuffled = nus(index, or_hash_u32(seed));
+    for (uint32_t d = 0; d < 4; d++) {
+        uint32_t b = nus(sobol(shuffled, d), hash_combine(seed, d + 1)) >> 8;
+        if (bits) bits[d] = b;
+        if (out) out[d] = (double)b * (1.0 / 16777216.0);
+    }
+}
+static void sample4(const oscene *s, uint32_t pixel, uint32_t index, uint32_t group, double u[4]) {
+    or_sample4(s->seed, pixel, index, group, NULL, u);
+}
+
+/* ------------------------------------------------------------ spectrum */
+
+static spec sample_wavelengths(double u) {
+    spec l;
+    for (int i = 0; i < 4; i++) {
+        double v = fract(u + i * 0.25);
+        l.v[i] = 538.0 - 138.888889 * atanh(0.85691062 - 1.82750197 * v);
+    }
+    return l;
+}
+static double wavelength_pdf(double l) {
+    if (l < 360.0 || l > 830.0) return 0.0;
+    double c = cosh(0.0072 * (l - 538.0));
+    return 0.0039398042 / (c * c);
+}
+static double lobe(double x, double m, double b, double a) { double t = (x - m) / (x < m ? b : a); return exp(-0.5 * t * t); }
+static v3 cmf(double l) {
+    return V(1.056 * lobe(l, 599.8, 37.9, 31.0) + 0.362 * lobe(l, 442.0, 16.0, 26.7) - 0.065 * lobe(l, 501.1, 20.4, 26.2),
+             0.821 * lobe(l, 568.8, 46.9, 40.5) + 0.286 * lobe(l, 530.9, 16.3, 31.1),
+             1.217 * lobe(l, 437.0, 11.8, 36.0) + 0.681 * lobe(l, 459.0, 26.0, 13.8));
+}
+static spec spec_of(const oscene *s, const float *fit, const float *rgb, spec lambda) {
+    spec r = S1(0.0);
+    if (s->spectral) {
+        for (int i = 0; i < 4; i++) {
+            double x = (lambda.v[i] - 360.0) / 470.0;
+            double z = ((double)fit[0] * x + fit[1]) * x + fit[2];
+            r.v[i] = fit[3] * (0.5 + z / (2.0 * sqrt(1.0 + z * z)));
+        }
+    } else {
+        r.v[0] = rgb[0]; r.v[1] = rgb[1]; r.v[2] = rgb[2]; r.v[3] = 0.0;
+    }
+    return r;
+}
+static double blackbody(double l, double k) {
+    double peak = 2.8977721e6 / k, x = 1.4387769e7 / (l * k), r = peak / l;
+    return r * r * r * r * r * (142.32492 / (exp(x) - 1.0));
+}
+static double spec_max(const oscene *s, spec a) {
+    if (s->spectral) return fmax(fmax(a.v[0], a.v[1]), fmax(a.v[2], a.v[3]));
+    return fmax(a.v[0], fmax(a.v[1], a.v[2]));
+}
+static v3 spec_to_film(const oscene *s, spec r, spec lambda) {
+    if (!s->spectral) return V(r.v[0], r.v[1], r.v[2]);
+    v3 sum = V(0, 0, 0);
+    for (int i = 0; i < 4; i++) {
+        double pdf = wavelength_pdf(lambda.v[i]);
+        if (pdf > 0.0) sum = add(sum, mul(cmf(lambda.v[i]), r.v[i] / pdf));
+    }
+    return V(sum.x * 0.25 / K(s, 6, 0), sum.y * 0.25 / K(s, 6, 1), sum.z * 0.25 / K(s, 6, 2));
+}
+
+/* ------------------------------------------------------------ ray casts */
+
+typedef struct { double t; uint32_t tri; double u, v; } ohit;
+
+static int box64(const onode *n, v3 o, v3 inv, double tmax) {
+    double t0 = 0.0, t1 = tmax;
+    double lo[3] = {n->lo[0], n->lo[1], n->lo[2]}, hi[3] = {n->hi[0], n->hi[1], n->hi[2]};
+    double oo[3] = {o.x, o.y, o.z}, ii[3] = {inv.x, inv.y, inv.z};
+    for (int k = 0; k < 3; k++) {
+        double a = (lo[k] - oo[k]) * ii[k], b = (hi[k] - oo[k]) * ii[k];
+        double nn = a < b ? a : b, ff = a < b ? b : a;
+        if (nn > t0) t0 = nn;
+        if (ff < t1) t1 = ff;
+    }
+    /* a little slack: f64 boxes of f32 geometry must not cull a true hit */
+    return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
+}
+
+static int tri64(const oscene *s, uint32_t t, v3 o, v3 d, double tmax, ohit *h) {
+    v3 a = point_of(s, s->indices[t * 3]);
+    v3 e1 = sub(point_of(s, s->indices[t * 3 + 1]), a), e2 = sub(point_of(s, s->indices[t * 3 + 2]), a);
+    v3 q = cross(d, e2);
+    double det = dot(e1, q);
+    if (fabs(det) < 1e-30) return 0;
+    v3 sv = sub(o, a);
+    double u = dot(sv, q) / det;
+    if (u < 0.0 || u > 1.0) return 0;
+    v3 r = cross(sv, e1);
+    double v = dot(d, r) / det;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    double tt = dot(e2, r) / det;
+    if (tt <= 0.0 || tt >= tmax || tt >= h->t) return 0;
+    h->t = tt; h->tri = t; h->u = u; h->v = v;
+    return 1;
+}
+
+static ohit closest(const oscene *s, v3 o, v3 d, double tmax) {
+    ohit h = {tmax, NONE, 0, 0};
+    if (s->triangles == 0 || s->nodes[0].lo[0] > s->nodes[0].hi[0]) return h;
+    v3 inv = V(1.0 / (fabs(d.x) < 1e-20 ? (d.x < 0 ? -1e-20 : 1e-20) : d.x),
+               1.0 / (fabs(d.y) < 1e-20 ? (d.y < 0 ? -1e-20 : 1e-20) : d.y),
+               1.0 / (fabs(d.z) < 1e-20 ? (d.z < 0 ? -1e-20 : 1e-20) : d.z));
+    uint32_t stack[256];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const onode *n = &s->nodes[stack[--sp]];
+        if (!box64(n, o, inv, h.t)) continue;
+        if (n->count) {
+            for (uint32_t i = 0; i < n->count; i++) tri64(s, n->first + i, o, d, tmax, &h);
+        } else if (sp + 2 <= 256) {
+            stack[sp++] = n->first + 1;
+            stack[sp++] = n->first;
+        }
+    }
+    return h;
+}
+
+/* ------------------------------------------------------------ lights */
+
+static uint32_t light_count(const oscene *s) { return (uint32_t)K(s, 0, 0); }
+static uint32_t light_type(const oscene *s, uint32_t l) { return (uint32_t)L(s, l, 0, 3); }
+static int light_visible(const oscene *s, uint32_t l) { return ((uint32_t)L(s, l, 5, 3) & 1u) != 0; }
+static double light_select_pdf(const oscene *s, uint32_t l) { return K(s, 8, 2) * L(s, l, 6, 1); }
+
+static spec light_radiance(const oscene *s, uint32_t l, spec lambda) {
+    spec r = spec_of(s, &s->lights[(l * 7 + 4) * 4], &s->lights[(l * 7 + 5) * 4], lambda);
+    if (s->spectral && ((uint32_t)L(s, l, 5, 3) & 2u)) {
+        double k = L(s, l, 6, 0);
+        for (int i = 0; i < 4; i++) r.v[i] *= blackbody(lambda.v[i], k);
+    }
+    return r;
+}
+
+static uint32_t pick_distant(const oscene *s, double u, double *pdf) {
+    uint32_t n = light_count(s), chosen = NONE;
+    for (uint32_t l = 0; l < n; l++) {
+        if (light_type(s, l) < 3) continue;
+        chosen = l;
+        if (u < L(s, l, 6, 2)) break;
+    }
+    *pdf = chosen == NONE ? 0.0 : L(s, chosen, 6, 1);
+    return chosen;
+}
+
+typedef struct { v3 direction; double distance, pdf, cos_light; int delta; } lsample;
+
+static void basis(v3 n, v3 *t, v3 *b) {
+    double sg = n.z >= 0.0 ? 1.0 : -1.0;
+    double a = -1.0 / (sg + n.z), c = n.x * n.y * a;
+    *t = V(1.0 + sg * n.x * n.x * a, sg * c, -sg * n.x);
+    *b = V(c, sg + n.y * n.y * a, -n.y);
+}
+static void concentric(double ux, double uy, double *x, double *y) {
+    double ox = 2 * ux - 1, oy = 2 * uy - 1;
+    if (ox == 0 && oy == 0) { *x = 0; *y = 0; return; }
+    double r, th;
+    if (fabs(ox) > fabs(oy)) { r = ox; th = (PI / 4) * (oy / ox); }
+    else { r = oy; th = (PI / 2) - (PI / 4) * (ox / oy); }
+    *x = r * cos(th); *y = r * sin(th);
+}
+static v3 sample_cone(v3 axis, double cos_max, double ux, double uy) {
+    double c = 1.0 - ux * (1.0 - cos_max), sn = sqrt(fmax(0.0, 1.0 - c * c)), phi = 2 * PI * uy;
+    v3 t, b;
+    basis(axis, &t, &b);
+    return norm(add(add(mul(t, sn * cos(phi)), mul(b, sn * sin(phi))), mul(axis, c)));
+}
+static double sphere_hit(v3 c, double r, v3 o, v3 d) {
+    v3 oc = sub(o, c);
+    double b = dot(oc, d), cc = dot(oc, oc) - r * r, h = b * b - cc;
+    if (h < 0) return INF;
+    h = sqrt(h);
+    double t = -b - h;
+    if (t > 0) return t;
+    t = -b + h;
+    return t > 0 ? t : INF;
+}
+
+static int sample_light(const oscene *s, uint32_t l, v3 x, double ux, double uy, lsample *o) {
+    uint32_t type = light_type(s, l);
+    o->delta = 0;
+    o->cos_light = 1.0;
+    if (type == 4) {
+        double z = 1 - 2 * ux, r = sqrt(fmax(0, 1 - z * z));
+        v3 lv = V(r * cos(2 * PI * uy), z, r * sin(2 * PI * uy));
+        o->direction = norm(add(add(mul(Lv(s, l, 1), lv.x), mul(Lv(s, l, 2), lv.y)), mul(Lv(s, l, 3), lv.z)));
+        o->distance = INF;
+        o->pdf = 1.0 / (4.0 * PI);
+        return 1;
+    }
+    if (type == 0 || type == 1) {
+        double ax, ay;
+        if (type == 0) { ax = 2 * ux - 1; ay = 2 * uy - 1; } else concentric(ux, uy, &ax, &ay);
+        v3 q = add(add(Lv(s, l, 0), mul(Lv(s, l, 1), ax)), mul(Lv(s, l, 2), ay));
+        v3 d = sub(q, x);
+        double d2 = dot(d, d);
+        o->distance = sqrt(d2);
+        o->direction = mul(d, 1.0 / o->distance);
+        double cl = -dot(o->direction, Lv(s, l, 3));
+        if (cl <= 0 || L(s, l, 3, 3) <= 0) return 0;
+        o->pdf = d2 / (cl * L(s, l, 3, 3));
+        o->cos_light = cl;
+        return 1;
+    }
+    if (type == 2) {
+        v3 tc = sub(Lv(s, l, 0), x);
+        double d2 = dot(tc, tc), r = L(s, l, 1, 3);
+        if (d2 <= r * r) return 0;
+        double cm = sqrt(fmax(0, 1 - r * r / d2));
+        o->direction = sample_cone(mul(tc, 1.0 / sqrt(d2)), cm, ux, uy);
+        o->distance = sphere_hit(Lv(s, l, 0), r, x, o->direction);
+        if (o->distance >= INF) return 0;
+        o->pdf = 1.0 / (2 * PI * (1 - cm));
+        o->cos_light = fmax(0, -dot(o->direction, norm(sub(add(x, mul(o->direction, o->distance)), Lv(s, l, 0)))));
+        return 1;
+    }
+    v3 axis = neg(Lv(s, l, 3));
+    double ch = L(s, l, 2, 3);
+    o->distance = INF;
+    if (ch >= 1.0 - 1e-7) { o->direction = axis; o->pdf = 1; o->delta = 1; return 1; }
+    o->direction = sample_cone(axis, ch, ux, uy);
+    o->pdf = 1.0 / (2 * PI * (1 - ch));
+    return 1;
+}
+
+static double intersect_light(const oscene *s, uint32_t l, v3 o, v3 d, double tmax, int flow) {
+    uint32_t type = light_type(s, l);
+    if (type == 2) { double t = sphere_hit(Lv(s, l, 0), L(s, l, 1, 3), o, d); return t < tmax ? t : INF; }
+    if (type >= 3) return INF;
+    v3 u = Lv(s, l, 1), v = Lv(s, l, 2), n = Lv(s, l, 3);
+    double f = dot(d, n);
+    if (flow ? f <= 0 : f >= 0) return INF;
+    double t = dot(sub(Lv(s, l, 0), o), n) / f;
+    if (t <= 0 || t >= tmax) return INF;
+    v3 q = sub(add(o, mul(d, t)), Lv(s, l, 0));
+    double a = dot(q, u) / dot(u, u), b = dot(q, v) / dot(v, v);
+    int in = type == 0 ? (fabs(a) <= 1 && fabs(b) <= 1) : (a * a + b * b <= 1);
+    return in ? t : INF;
+}
+
+static double light_hit_pdf(const oscene *s, uint32_t l, v3 o, v3 d, double t) {
+    uint32_t type = light_type(s, l);
+    if (type == 2) {
+        v3 tc = sub(Lv(s, l, 0), o);
+        double r = L(s, l, 1, 3), cm = sqrt(fmax(0, 1 - r * r / dot(tc, tc)));
+        return 1.0 / (2 * PI * fmax(1e-12, 1 - cm));
+    }
+    if (type == 3) return 1.0 / (2 * PI * fmax(1e-12, 1 - L(s, l, 2, 3)));
+    if (type == 4) return 1.0 / (4 * PI);
+    double cl = fmax(1e-12, -dot(d, Lv(s, l, 3)));
+    return t * t / (cl * L(s, l, 3, 3));
+}
+
+/* light tree (lighttree.glsl:326-402) */
+static double tree_importance(const oscene *s, uint32_t node, v3 x, v3 n) {
+    uint32_t base = (uint32_t)K(s, 8, 1) + node * 3;
+    const float *a = &s->lights[base * 4], *b = &s->lights[(base + 1) * 4], *c = &s->lights[(base + 2) * 4];
+    if (a[3] <= 0) return 0;
+    v3 center = mul(add(V(a[0], a[1], a[2]), V(b[0], b[1], b[2])), 0.5);
+    v3 off = sub(x, center);
+    double radius = 0.5 * len(sub(V(b[0], b[1], b[2]), V(a[0], a[1], a[2])));
+    double d2o = dot(off, off), dist2 = fmax(d2o, radius);
+    v3 cv = V(c[0], c[1], c[2]);
+    v3 wi = d2o > 0 ? norm(off) : cv;
+    double cw = dot(cv, wi), sw = sqrt(fmax(0, 1 - cw * cw));
+    double cb = d2o < radius * radius ? -1.0 : sqrt(fmax(0, 1 - radius * radius / d2o)), sb = sqrt(fmax(0, 1 - cb * cb));
+    double co = b[3], so = sqrt(fmax(0, 1 - co * co));
+    double cx = cw > co ? 1.0 : cw * co + sw * so, sx = cw > co ? 0.0 : sw * co - cw * so;
+    double cp = cx > cb ? 1.0 : cx * cb + sx * sb;
+    if (cp <= c[3]) return 0;
+    double imp = a[3] * cp / dist2;
+    double ci = fabs(dot(wi, n)), si = sqrt(fmax(0, 1 - ci * ci));
+    double cpi = ci > cb ? 1.0 : ci * cb + si * sb;
+    return fmax(imp * cpi, 0.0);
+}
+static int tree_sample(const oscene *s, double u, v3 x, v3 n, uint32_t *emitter, double *pdf) {
+    *pdf = 1.0;
+    *emitter = 0;
+    if (K(s, 8, 0) < 1) return 0;
+    uint32_t node = 0;
+    for (int depth = 0; depth < 64; depth++) {
+        uint32_t link = s->emitters[node * 2];
+        if (s->emitters[node * 2 + 1] & 0x80000000u) { *emitter = link; return *pdf > 0; }
+        double l = tree_importance(s, link, x, n), r = tree_importance(s, link + 1, x, n);
+        if (l + r <= 0) return 0;
+        double ch = l / (l + r);
+        if (u < ch) { u = fmin(u / ch, 0.99999994); *pdf *= ch; node = link; }
+        else { u = fmin((u - ch) / (1 - ch), 0.99999994); *pdf *= 1 - ch; node = link + 1; }
+    }
+    return 0;
+}
+static double tree_pdf(const oscene *s, uint32_t emitter, v3 x, v3 n) {
+    uint32_t rec = (uint32_t)K(s, 10, 0) + emitter * 4;
+    uint32_t trail = s->emitters[rec + 1], depth = s->emitters[rec + 2], node = 0;
+    double pdf = 1.0;
+    for (uint32_t lv = 0; lv < depth; lv++) {
+        uint32_t link = s->emitters[node * 2];
+        double l = tree_importance(s, link, x, n), r = tree_importance(s, link + 1, x, n);
+        if (l + r <= 0) return 0;
+        uint32_t bit = (trail >> lv) & 1u;
+        pdf *= bit == 0 ? l / (l + r) : r / (l + r);
+        node = link + bit;
+    }
+    return pdf;
+}
+static uint32_t emitter_code(const oscene *s, uint32_t e) { return s->emitters[(uint32_t)K(s, 10, 0) + e * 4]; }
+static uint32_t triangle_emitter(const oscene *s, uint32_t t) { return s->emitters[(uint32_t)K(s, 10, 1) + t]; }
+static spec material_color(const oscene *s, uint32_t m, int row, spec lambda) {
+    return spec_of(s, &s->materials[(m * 16 + row) * 4], &s->materials[(m * 16 + row + 1) * 4], lambda);
+}
+static int sample_triangle(const oscene *s, uint32_t t, v3 x, double ux, double uy, spec lambda, lsample *o, spec *rad) {
+    v3 a = point_of(s, s->indices[t * 3]), b = point_of(s, s->indices[t * 3 + 1]), c = point_of(s, s->indices[t * 3 + 2]);
+    double r = sqrt(ux);
+    v3 q = add(add(mul(a, 1 - r), mul(b, r * (1 - uy))), mul(c, r * uy));
+    v3 cab = cross(sub(b, a), sub(c, a));
+    double area = 0.5 * len(cab);
+    v3 d = sub(q, x);
+    double d2 = dot(d, d);
+    if (area <= 0 || d2 <= 0) return 0;
+    o->distance = sqrt(d2);
+    o->direction = mul(d, 1.0 / o->distance);
+    double cl = fabs(dot(o->direction, cab)) / (2 * area);
+    if (cl <= 0) return 0;
+    o->pdf = d2 / (cl * area);
+    o->cos_light = cl;
+    o->delta = 0;
+    *rad = material_color(s, s->shading[t * 4 + 3], 6, lambda);
+    return 1;
+}
+static double triangle_hit_pdf(const oscene *s, uint32_t t, v3 d, double tt) {
+    v3 a = point_of(s, s->indices[t * 3]);
+    v3 cab = cross(sub(point_of(s, s->indices[t * 3 + 1]), a), sub(point_of(s, s->indices[t * 3 + 2]), a));
+    double area = 0.5 * len(cab);
+    double cl = fabs(dot(d, cab)) / fmax(1e-20, 2 * area);
+    return tt * tt / fmax(1e-20, cl * area);
+}
+
+/* ------------------------------------------------------------ GGX / OpenPBR */
+
+typedef struct { double x, y; } d2v;
+static d2v ggx_alpha(double r, double an) {
+    double r2 = r * r, t = r2 * sqrt(2.0 / (1.0 + (1.0 - an) * (1.0 - an)));
+    d2v a = {fmax(t, 1e-4), fmax((1.0 - an) * t, 1e-4)};
+    return a;
+}
+static double ggx_d(v3 m, d2v a) {
+    if (m.z <= 0) return 0;
+    v3 sv = V(m.x / a.x, m.y / a.y, m.z);
+    double k = dot(sv, sv);
+    return 1.0 / (PI * a.x * a.y * k * k);
+}
+static double ggx_lambda(v3 v, d2v a) {
+    double z2 = v.z * v.z;
+    if (z2 <= 0) return INF;
+    double a2 = (a.x * a.x * v.x * v.x + a.y * a.y * v.y * v.y) / z2;
+    return 0.5 * (-1.0 + sqrt(1.0 + a2));
+}
+static double ggx_g1(v3 v, d2v a) { return 1.0 / (1.0 + ggx_lambda(v, a)); }
+static double ggx_g2(v3 i, v3 o, d2v a) { return 1.0 / (1.0 + ggx_lambda(i, a) + ggx_lambda(o, a)); }
+static v3 ggx_sample(v3 wo, d2v a, double ux, double uy) {
+    v3 v = norm(V(wo.x * a.x, wo.y * a.y, wo.z));
+    double phi = 2 * PI * ux, z = (1 - uy) * (1 + v.z) - v.z, sn = sqrt(clampd(1 - z * z, 0, 1));
+    v3 h = add(V(sn * cos(phi), sn * sin(phi), z), v);
+    return norm(V(h.x * a.x, h.y * a.y, fmax(h.z, 1e-6)));
+}
+static double ggx_reflection_pdf(v3 wo, v3 h, d2v a) { return ggx_g1(wo, a) * ggx_d(h, a) / (4 * wo.z); }
+static double fresnel_dielectric(double ci, double eta) {
+    double c = fabs(ci), g2 = eta * eta - 1 + c * c;
+    if (g2 <= 0) return 1;
+    double g = sqrt(g2), a = (g - c) / (g + c), b = (c * (g + c) - 1) / (c * (g - c) + 1);
+    return 0.5 * a * a * (1 + b * b);
+}
+static double fresnel_avg(double eta) { return log((10893.0 * eta - 1438.2) / (-774.4 * eta * eta + 10212.0 * eta + 1.0)); }
+static v3 reflectv(v3 i, v3 n) { return sub(i, mul(n, 2 * dot(n, i))); }
+static v3 refractv(v3 i, v3 n, double eta) {
+    double d = dot(n, i), k = 1 - eta * eta * (1 - d * d);
+    if (k < 0) return V(0, 0, 0);
+    return sub(mul(i, eta), mul(n, eta * d + sqrt(k)));
+}
+
+#define TMU 32
+#define TR 32
+#define TETA 16
+#define T_AVG (TMU * TR)
+#define T_DIE (T_AVG + TR)
+#define T_SIZE (T_DIE + TMU * TR * TETA)
+static double table_mu(int i) { return mixd(0.02, 1.0, (double)i / (TMU - 1)); }
+static double table_albedo(double mu, double r, double eta) {
+    v3 wo = V(sqrt(1 - mu * mu), 0, mu);
+    d2v a = ggx_alpha(r, 0);
+    double sum = 0;
+    for (int i = 0; i < 32; i++)
+        for (int j = 0; j < 32; j++) {
+            v3 h = ggx_sample(wo, a, (i + 0.5) / 32, (j + 0.5) / 32);
+            v3 wi = reflectv(neg(wo), h);
+            if (wi.z <= 0) continue;
+            double f = eta > 0 ? fresnel_dielectric(dot(wo, h), eta) : 1.0;
+            sum += f * ggx_g2(wi, wo, a) / ggx_g1(wo, a);
+        }
+    return sum / 1024.0;
+}
+/* tables.comp in f64 */
+void or_make_tables(double *t) {
+    int plane = TMU * TR;
+    for (int cell = 0; cell < plane * (1 + TETA); cell++) {
+        int layer = cell / plane, at = cell % plane;
+        double mu = table_mu(at % TMU), r = (double)(at / TMU) / (TR - 1);
+        if (layer == 0) t[at] = table_albedo(mu, r, -1);
+        else {
+            double z = 0.8 * (layer - 1) / (TETA - 1);
+            t[T_DIE + (layer - 1) * plane + at] = table_albedo(mu, r, (1 + z * z) / (1 - z * z));
+        }
+    }
+    for (int c = 0; c < TR; c++) {
+        double sum = 0;
+        for (int m = 0; m + 1 < TMU; m++) {
+            double a = table_mu(m), b = table_mu(m + 1);
+            sum += 0.5 * (t[c * TMU + m] * a + t[c * TMU + m + 1] * b) * (b - a);
+        }
+        t[T_AVG + c] = 2 * sum;
+    }
+}
+static double table_row(const double *t, int base, double r, double mu) {
+    double x = clampd((mu - 0.02) / 0.98 * (TMU - 1), 0, TMU - 1), y = clampd(r * (TR - 1), 0, TR - 1);
+    int x0 = (int)x, y0 = (int)y, x1 = x0 + 1 < TMU ? x0 + 1 : TMU - 1, y1 = y0 + 1 < TR ? y0 + 1 : TR - 1;
+    double fx = x - x0, fy = y - y0;
+    double a = mixd(t[base + y0 * TMU + x0], t[base + y0 * TMU + x1], fx);
+    double b = mixd(t[base + y1 * TMU + x0], t[base + y1 * TMU + x1], fx);
+    return mixd(a, b, fy);
+}
+static double table_ggx(const double *t, double mu, double r) { return table_row(t, 0, r, mu); }
+static double table_dielectric(const double *t, double mu, double r, double eta) {
+    double z = clampd(sqrt(fabs(eta - 1) / (eta + 1)) / 0.8 * (TETA - 1), 0, TETA - 1);
+    int z0 = (int)z, z1 = z0 + 1 < TETA ? z0 + 1 : TETA - 1, plane = TMU * TR;
+    return mixd(table_row(t, T_DIE + z0 * plane, r, mu), table_row(t, T_DIE + z1 * plane, r, mu), z - z0);
+}
+
+#define FON_C1 (0.5 - 2.0 / (3.0 * PI))
+#define FON_C2 (2.0 / 3.0 - 28.0 / (15.0 * PI))
+static double fon_albedo(double mu, double r) {
+    double m = 1 - mu, g = m * (0.0571085289 + m * (0.491881867 + m * (-0.332181442 + m * 0.0714429953)));
+    return (1 + r * g) / (1 + FON_C1 * r);
+}
+static spec eon(spec rho, double r, v3 wi, v3 wo) {
+    double mi = wi.z, mo = wo.z, sv = dot(wi, wo) - mi * mo;
+    double sot = sv > 0 ? sv / fmax(mi, mo) : sv, a = 1.0 / (1 + FON_C1 * r);
+    double eo = fon_albedo(mo, r), ei = fon_albedo(mi, r), avg = a * (1 + FON_C2 * r);
+    spec out;
+    for (int k = 0; k < 4; k++) {
+        double single = rho.v[k] * (a * (1 + r * sot) / PI);
+        double ms = rho.v[k] * rho.v[k] * avg / (1 - rho.v[k] * (1 - avg));
+        out.v[k] = single + ms * (fmax(1e-7, 1 - eo) * fmax(1e-7, 1 - ei) / (PI * fmax(1e-7, 1 - avg)));
+    }
+    return out;
+}
+static spec f82(spec f0, spec tint, double mu) {
+    const double bar = 1.0 / 7.0;
+    double m = 1 - mu, m5 = m * m * m * m * m, b5 = pow(1 - bar, 5);
+    spec o;
+    for (int k = 0; k < 4; k++) {
+        double sch = f0.v[k] + (1 - f0.v[k]) * m5, sb = f0.v[k] + (1 - f0.v[k]) * b5;
+        double a = sb * (1 - tint.v[k]) / (bar * b5 * (1 - bar));
+        o.v[k] = clampd(sch - a * (mu * m5 * m), 0, 1);
+    }
+    return o;
+}
+static double lum(double r, double g, double b) { return r * 0.2722287168 + g * 0.6740817658 + b * 0.0536895174; }
+
+typedef struct {
+    v3 n, t, b, g;
+    uint32_t m;
+    int back, inside;
+    double droughness, metal, sw, trans, rough, croughness, ior, dark, lbase, lspec, ltrans, lcoat;
+    d2v alpha, calpha;
+    double eta, coat, ceta;
+    double dcomp, missing, salb, ccomp, calb, ck;
+    double pc, pm, ps, pt, pd;
+    spec base, mtint, stint, ttint, ctint, substrate;
+} surf;
+
+static v3 to_local(const surf *s, v3 v) { return V(dot(v, s->t), dot(v, s->b), dot(v, s->n)); }
+static v3 to_world(const surf *s, v3 v) { return add(add(mul(s->t, v.x), mul(s->b, v.y)), mul(s->n, v.z)); }
+
+static void read_material(const oscene *sc, surf *s, double hero) {
+    uint32_t m = s->m;
+    s->droughness = M(sc, m, 10, 1);
+    s->metal = clampd(M(sc, m, 10, 0), 0, 1);
+    s->sw = fmax(M(sc, m, 3, 3), 0);
+    s->trans = clampd(M(sc, m, 9, 3), 0, 1);
+    s->inside = s->back && s->trans > 0;
+    s->coat = !s->inside ? clampd(M(sc, m, 5, 3), 0, 1) : 0;
+    s->croughness = clampd(M(sc, m, 11, 1), 0, 1);
+    s->ceta = fmax(M(sc, m, 11, 2), 1.0001);
+    s->calpha.x = s->calpha.y = 1;
+    if (s->coat > 0) s->calpha = ggx_alpha(s->croughness, clampd(M(sc, m, 12, 2), 0, 1));
+    s->dark = clampd(M(sc, m, 11, 3), 0, 1);
+    double rough = clampd(M(sc, m, 10, 2), 0, 1);
+    double r4 = rough * rough * rough * rough, c4 = pow(s->croughness, 4);
+    s->rough = mixd(rough, fmin(1, pow(r4 + 2 * c4, 0.25)), s->coat);
+    s->alpha = ggx_alpha(s->rough, clampd(M(sc, m, 10, 3), 0, 1));
+    double eta = fmax(M(sc, m, 11, 0), 1.0001);
+    if (sc->spectral && M(sc, m, 12, 3) != 0 && hero > 0)
+        eta = fmax(eta + M(sc, m, 12, 3) * (1 / (hero * hero) - 1 / (587.6 * 587.6)), 1.0001);
+    double rel = eta / s->ceta;
+    if (rel < 1) rel = 1 / rel;
+    s->ior = fmax(mixd(eta, rel, s->coat), 1.0001);
+    s->eta = s->inside ? 1 / s->ior : s->ior;
+    double bw = M(sc, m, 1, 3);
+    s->lbase = fmax(lum(M(sc, m, 1, 0) * bw, M(sc, m, 1, 1) * bw, M(sc, m, 1, 2) * bw), 0);
+    s->lspec = fmax(lum(M(sc, m, 3, 0), M(sc, m, 3, 1), M(sc, m, 3, 2)), 0);
+    s->ltrans = fmax(lum(M(sc, m, 9, 0), M(sc, m, 9, 1), M(sc, m, 9, 2)), 0.05);
+    s->lcoat = lum(M(sc, m, 5, 0), M(sc, m, 5, 1), M(sc, m, 5, 2));
+}
+static void choose(surf *s, double wc, double wm, double ws, double wt, double wd) {
+    if (s->inside) { wm = 0; wd = 0; ws = fmax(ws, 0.05); wt = fmax(wt, s->trans > 0 ? 0.05 : 0); }
+    double tot = wc + wm + ws + wt + wd, sc = tot > 0 ? 1 / tot : 0;
+    s->pc = wc * sc; s->pm = wm * sc; s->ps = ws * sc; s->pt = wt * sc; s->pd = wd * sc;
+}
+static void orient(const oscene *sc, surf *s, v3 wo) {
+    double mu = clampd(wo.z, 0, 1);
+    const double *t = sc->tables;
+    s->ccomp = 1; s->calb = 0;
+    if (s->coat > 0) {
+        double e = fmax(table_ggx(t, mu, s->croughness), 1e-4);
+        s->ccomp = 1 + fresnel_avg(s->ceta) * (1 - e) / e;
+        s->calb = clampd(s->ccomp * table_dielectric(t, mu, s->croughness, s->ceta), 0, 1);
+    }
+    double e = fmax(table_ggx(t, mu, s->rough), 1e-4), miss = (1 - e) / e;
+    s->missing = miss;
+    s->dcomp = 1 + fresnel_avg(s->ior) * miss;
+    s->salb = clampd(s->sw * s->dcomp * table_dielectric(t, mu, s->rough, s->ior), 0, 1);
+    double sub_rgb = 1;
+    s->ck = 0;
+    if (s->coat > 0) {
+        double ks = fresnel_dielectric(mu, s->ceta), kr = 1 - (1 - fresnel_avg(s->ceta)) / (s->ceta * s->ceta);
+        s->ck = mixd(ks, kr, mixd(1, s->rough, s->metal));
+        double drgb = (1 - s->ck) / fmax(1e-4, 1 - s->lbase * s->ck);
+        sub_rgb = mixd(1, s->lcoat * (1 - s->calb) * mixd(1, drgb, s->dark), s->coat);
+    }
+    double di = 1 - s->metal;
+    choose(s, s->coat * s->calb, sub_rgb * s->metal, sub_rgb * di * s->salb * s->lspec,
+           sub_rgb * di * s->trans * (1 - s->salb) * s->ltrans, sub_rgb * di * (1 - s->trans) * (1 - s->salb) * s->lbase);
+}
+static void colors(const oscene *sc, surf *s, spec lambda) {
+    s->base = sscale(material_color(sc, s->m, 0, lambda), M(sc, s->m, 1, 3));
+    s->mtint = material_color(sc, s->m, 2, lambda);
+    s->stint = sscale(s->mtint, s->sw);
+    s->ttint = material_color(sc, s->m, 8, lambda);
+    s->ctint = s->coat > 0 ? material_color(sc, s->m, 4, lambda) : S1(1);
+    s->substrate = S1(1);
+    if (s->coat > 0)
+        for (int k = 0; k < 4; k++) {
+            double delta = (1 - s->ck) / (1 - s->base.v[k] * s->ck);
+            double under = s->ctint.v[k] * (1 - s->calb) * mixd(1, delta, s->dark);
+            s->substrate.v[k] = mixd(1, under, s->coat);
+        }
+}
+static int refraction_half(v3 wi, v3 wo, double eta, v3 *h) {
+    *h = norm(add(wo, mul(wi, eta)));
+    if (h->z < 0) *h = neg(*h);
+    return dot(wo, *h) > 0 && dot(wi, *h) < 0;
+}
+static double surface_pdf(const surf *s, v3 wi, v3 wo) {
+    if (wo.z <= 0 || wi.z == 0) return 0;
+    if (wi.z < 0) {
+        if (s->pt <= 0) return 0;
+        v3 h;
+        if (!refraction_half(wi, wo, s->eta, &h)) return 0;
+        double oh = dot(wo, h), ih = dot(wi, h), den = oh + s->eta * ih, jac = s->eta * s->eta * fabs(ih) / (den * den);
+        return s->pt * ggx_g1(wo, s->alpha) * ggx_d(h, s->alpha) * oh / wo.z * jac;
+    }
+    v3 h = norm(add(wi, wo));
+    double pdf = 0;
+    if (s->pc > 0) pdf += s->pc * ggx_reflection_pdf(wo, h, s->calpha);
+    if (!s->inside) pdf += s->pd * wi.z / PI;
+    pdf += (s->ps + s->pm) * ggx_reflection_pdf(wo, h, s->alpha);
+    return pdf;
+}
+static spec evaluate(const surf *s, v3 wi, v3 wo, double *pdf) {
+    *pdf = 0;
+    if (wo.z <= 0 || wi.z == 0) return S1(0);
+    *pdf = surface_pdf(s, wi, wo);
+    double di = 1 - s->metal;
+    if (wi.z < 0) {
+        if (s->pt <= 0) return S1(0);
+        v3 h;
+        if (!refraction_half(wi, wo, s->eta, &h)) return S1(0);
+        double oh = dot(wo, h), ih = dot(wi, h), den = oh + s->eta * ih;
+        double f = (1 - fresnel_dielectric(oh, s->eta)) * ggx_d(h, s->alpha) * ggx_g2(neg(wi), wo, s->alpha) * fabs(ih) * oh /
+                   (fabs(wi.z) * wo.z * den * den);
+        return sscale(smul(s->substrate, s->ttint), di * s->trans * f);
+    }
+    v3 h = norm(add(wi, wo));
+    double ch = dot(wo, h);
+    spec f = S1(0), below = S1(0);
+    if (s->pc > 0) {
+        double coat = ggx_d(h, s->calpha) * ggx_g2(wi, wo, s->calpha) / (4 * wi.z * wo.z);
+        f = S1(s->coat * fresnel_dielectric(ch, s->ceta) * s->ccomp * coat);
+    }
+    if (!s->inside) below = sadd(below, sscale(eon(s->base, s->droughness, wi, wo), di * (1 - s->trans) * (1 - s->salb)));
+    double mf = ggx_d(h, s->alpha) * ggx_g2(wi, wo, s->alpha) / (4 * wi.z * wo.z);
+    if (di > 0) below = sadd(below, sscale(s->stint, di * fresnel_dielectric(ch, s->eta) * s->dcomp * mf));
+    if (s->metal > 0 && !s->inside) {
+        spec F = f82(s->base, s->mtint, ch);
+        for (int k = 0; k < 4; k++) {
+            double avg = (20 * s->base.v[k] + 1) / 21, comp = 1 + avg * s->missing;
+            below.v[k] += fmin(1.0, F.v[k] * s->sw) * comp * (s->metal * mf);
+        }
+    }
+    return sadd(f, smul(s->substrate, below));
+}
+static int sample_dir(const surf *s, v3 wo, double ux, double uy, double uz, v3 *wi, double *pdf, uint32_t *lobe) {
+    *pdf = 0;
+    if (wo.z <= 0) return 0;
+    double pick = uz;
+    if (pick < s->pd) {
+        double dx, dy;
+        concentric(ux, uy, &dx, &dy);
+        *wi = V(dx, dy, sqrt(fmax(0, 1 - (dx * dx + dy * dy))));
+        *lobe = 0;
+    } else if ((pick -= s->pd) < s->pc) {
+        *wi = reflectv(neg(wo), ggx_sample(wo, s->calpha, ux, uy));
+        *lobe = 1;
+    } else if ((pick -= s->pc) < s->ps + s->pm) {
+        *wi = reflectv(neg(wo), ggx_sample(wo, s->alpha, ux, uy));
+        *lobe = 1;
+    } else if (s->pt > 0) {
+        v3 h = ggx_sample(wo, s->alpha, ux, uy);
+        *wi = refractv(neg(wo), h, 1 / s->eta);
+        if (dot(*wi, *wi) == 0) return 0;
+        *lobe = 2;
+    } else return 0;
+    if (wi->z == 0) return 0;
+    *pdf = surface_pdf(s, *wi, wo);
+    return *pdf > 0;
+}
+
+/* ------------------------------------------------------------ the path */
+
+static double mis_weight(double a, double b) { a = fmin(a, 1e18); b = fmin(b, 1e18); return 1.0 / (1 + a * a + b * b); }
+static v3 offset_origin(v3 p, v3 n) {
+    double sc = fmax(1.0, fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z))));
+    return add(p, mul(n, 1e-4 * sc));
+}
+static v3 oct_decode(uint32_t p) {
+    double x = fmax(-1.0, (double)(int16_t)(p & 0xffff) / 32767.0), y = fmax(-1.0, (double)(int16_t)(p >> 16) / 32767.0);
+    v3 n = V(x, y, 1 - fabs(x) - fabs(y));
+    if (n.z < 0) {
+        double nx = (1 - fabs(n.y)) * (n.x >= 0 ? 1 : -1), ny = (1 - fabs(n.x)) * (n.y >= 0 ? 1 : -1);
+        n.x = nx; n.y = ny;
+    }
+    return norm(n);
+}
+static spec clamp_ind(const oscene *s, spec c, uint32_t total) {
+    double cv = K(s, 0, 2);
+    if (total == 0 || cv <= 0) return c;
+    double m = spec_max(s, c);
+    return m > cv ? sscale(c, cv / m) : c;
+}
+static uint32_t next_bounce(const oscene *s, uint32_t b, uint32_t lobe) {
+    uint32_t shift = 8 * (lobe + 1), kind = ((b >> shift) & 0xff) + 1;
+    if ((double)kind > K(s, 7, (int)lobe) || (b & 0xff) >= 255) return NONE;
+    return (b & ~(0xffu << shift)) + (kind << shift) + 1;
+}
+
+/* Whether the shadow ray gets through; `kept` what the cut-outs pass
+ * (intersect_shadow.comp:56-97). */
+static int shadow(const oscene *s, v3 o, v3 d, double reach, double *kept) {
+    *kept = 1.0;
+    ohit h = closest(s, o, d, reach);
+    if (h.tri == NONE) return 1;
+    if (K(s, 0, 3) < 0.5) return 0;
+    v3 from = o;
+    double remaining = reach;
+    for (int hop = 0; hop < 16; hop++) {
+        ohit g = closest(s, from, d, remaining);
+        if (g.tri == NONE) return 1;
+        double op = clampd(M(s, s->shading[g.tri * 4 + 3], 12, 0), 0, 1);
+        *kept *= 1 - op;
+        if (*kept < 1e-4) return 0;
+        v3 q = add(from, mul(d, g.t));
+        double margin = 1e-4 * fmax(1.0, fmax(fabs(q.x), fmax(fabs(q.y), fabs(q.z))));
+        from = add(q, mul(d, margin));
+        remaining -= g.t + margin;
+        if (remaining <= 0) return 1;
+    }
+    return 0;
+}
+
+static spec miss_radiance(const oscene *s, uint32_t bounce, v3 rd, double ray_pdf, spec lambda) {
+    spec rad = spec_of(s, &s->constants[4], &s->constants[8], lambda);
+    for (uint32_t l = 0; l < light_count(s); l++) {
+        uint32_t type = light_type(s, l);
+        if (type < 3) continue;
+        if (type == 3) {
+            double ch = L(s, l, 2, 3);
+            if (ch >= 1.0 - 1e-7 || dot(rd, neg(Lv(s, l, 3))) < ch) continue;
+        }
+        double w = 1.0;
+        if (bounce == 0) { if (!light_visible(s, l)) continue; }
+        else if (ray_pdf > 0) w = mis_weight(light_hit_pdf(s, l, V(0, 0, 0), rd, INF) * light_select_pdf(s, l) / ray_pdf, 0.0);
+        rad = sadd(rad, sscale(light_radiance(s, l, lambda), w));
+    }
+    return rad;
+}
+
+/* One camera path of pixel `pixel`, sample `index`: its film value. */
+static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) {
+    double u[4];
+    sample4(s, pixel, index, 0, u);
+    const double *f = s->frame;
+    double fx = (double)(pixel % s->width) + u[0], fy = (double)(pixel / s->width) + u[1];
+    v3 film = add(add(V(f[4], f[5], f[6]), mul(V(f[8], f[9], f[10]), fx)), mul(V(f[12], f[13], f[14]), fy));
+    int ortho = f[3] > 0.5;
+    v3 origin = ortho ? add(V(f[0], f[1], f[2]), film) : V(f[0], f[1], f[2]);
+    v3 rd = norm(ortho ? cross(V(f[8], f[9], f[10]), V(f[12], f[13], f[14])) : film);
+    if (!ortho && K(s, 14, 2) > 0) {
+        v3 focused = add(V(f[0], f[1], f[2]), mul(rd, K(s, 14, 3) / dot(rd, Kv(s, 15))));
+        double lx, ly;
+        concentric(u[2], u[3], &lx, &ly);
+        v3 du = V(f[8], f[9], f[10]), dv = V(f[12], f[13], f[14]);
+        origin = add(add(V(f[0], f[1], f[2]), mul(norm(du), lx * K(s, 14, 2))), mul(norm(dv), ly * K(s, 14, 2)));
+        rd = norm(sub(focused, origin));
+    }
+    spec lambda = S1(0);
+    if (s->spectral) {
+        double w[4];
+        sample4(s, pixel, index, 1, w);
+        lambda = sample_wavelengths(w[0]);
+    }
+    spec throughput = S1(1), radiance = S1(0);
+    uint32_t bounces = 0;
+    double ray_pdf = 0.0, survival_scale = 1.0;
+    v3 vertex = origin;
+    v3 prev_normal = V(0, 0, 1);
+    uint32_t rounds = s->max_bounces + 1 + s->through;
+    for (uint32_t round = 0; round < rounds; round++) {
+        uint32_t total = bounces & 0xff;
+        ohit h = closest(s, origin, rd, INF);
+        uint32_t what = h.tri;
+        double ht = h.t;
+        for (uint32_t l = 0; l < light_count(s); l++) {
+            if (total == 0 && !light_visible(s, l)) continue;
+            double t = intersect_light(s, l, origin, rd, ht, 0);
+            if (t < ht) { ht = t; what = 0x80000000u | l; }
+        }
+        if (what == NONE) {
+            spec c = smul(throughput, miss_radiance(s, total, rd, ray_pdf, lambda));
+            double cv = K(s, 0, 2), m = spec_max(s, c);
+            if (total > 0 && cv > 0 && m > cv) c = sscale(c, cv / m);
+            radiance = sadd(radiance, c);
+            break;
+        }
+        if (what & 0x80000000u) {
+            uint32_t light = what & ~0x80000000u;
+            double w = 1.0;
+            if (total == 0) { if (!light_visible(s, light)) break; }
+            else if (ray_pdf > 0) {
+                uint32_t em = (uint32_t)L(s, light, 6, 3);
+                double reach = len(sub(vertex, add(origin, mul(rd, ht))));
+                double chance = (1 - K(s, 8, 2)) * tree_pdf(s, em, vertex, prev_normal);
+                w = mis_weight(light_hit_pdf(s, light, vertex, rd, reach) * chance / ray_pdf, 0);
+            }
+            radiance = sadd(radiance, clamp_ind(s, sscale(smul(throughput, light_radiance(s, light, lambda)), w), total));
+            break;
+        }
+        uint32_t tri = what;
+        v3 p0 = point_of(s, s->indices[tri * 3]);
+        v3 geo = norm(cross(sub(point_of(s, s->indices[tri * 3 + 1]), p0), sub(point_of(s, s->indices[tri * 3 + 2]), p0)));
+        int inside = dot(geo, rd) > 0;
+        if (inside) geo = neg(geo);
+        const uint32_t *attr = &s->shading[tri * 4];
+        v3 x = add(origin, mul(rd, ht));
+        double reach = len(sub(vertex, x));
+        double uo[4];
+        sample4(s, pixel, index, 4 + 3 * total, uo);
+        double op = clampd(M(s, attr[3], 12, 0), 0, 1);
+        if (op < 1) {
+            float htf = (float)ht;
+            uint32_t bits;
+            memcpy(&bits, &htf, 4);
+            if (fract(uo[0] + (double)or_hash_u32(bits) * (1.0 / 4294967296.0)) >= op) {
+                origin = offset_origin(x, neg(geo));
+                continue;
+            }
+        }
+        if (s->spectral && M(s, attr[3], 12, 3) != 0 && lambda.v[1] >= 0) {
+            throughput = (spec){{throughput.v[0] * 4, 0, 0, 0}};
+            radiance = (spec){{radiance.v[0] * 4, 0, 0, 0}};
+            lambda = (spec){{lambda.v[0], -lambda.v[1], -lambda.v[2], -lambda.v[3]}};
+        }
+        double hero = lambda.v[1] < 0 ? lambda.v[0] : 0;
+        if (M(s, attr[3], 7, 3) > 0) {
+            double w = 1;
+            uint32_t em = triangle_emitter(s, tri);
+            if (total > 0 && ray_pdf > 0 && em != NONE) {
+                double chance = (1 - K(s, 8, 2)) * tree_pdf(s, em, vertex, prev_normal);
+                w = mis_weight(triangle_hit_pdf(s, tri, rd, reach) * chance / ray_pdf, 0);
+            }
+            radiance = sadd(radiance, clamp_ind(s, sscale(smul(throughput, material_color(s, attr[3], 6, lambda)), w), total));
+        }
+        if (total >= (uint32_t)K(s, 0, 1)) break;
+        double b1 = h.u, b2 = h.v;
+        v3 normal = norm(add(add(mul(oct_decode(attr[0]), 1 - b1 - b2), mul(oct_decode(attr[1]), b1)), mul(oct_decode(attr[2]), b2)));
+        if (dot(normal, geo) < 0) normal = neg(normal);
+        surf sf;
+        memset(&sf, 0, sizeof(sf));
+        sf.n = normal;
+        basis(normal, &sf.t, &sf.b);
+        sf.g = geo;
+        sf.m = attr[3];
+        sf.back = inside;
+        v3 wo = to_local(&sf, neg(rd));
+        read_material(s, &sf, hero);
+        orient(s, &sf, wo);
+        colors(s, &sf, lambda);
+        v3 from = offset_origin(x, geo);
+        double ul[4];
+        sample4(s, pixel, index, 2 + 3 * total, ul);
+        double p_distant = K(s, 8, 2);
+        lsample ls;
+        spec emitted = S1(0);
+        double chance = 0;
+        int sampled = 0;
+        uint32_t emitter = NONE;
+        if (ul[0] < p_distant) {
+            double share;
+            uint32_t light = pick_distant(s, ul[0] / p_distant, &share);
+            if (light != NONE && share > 0 && sample_light(s, light, x, ul[1], ul[2], &ls)) {
+                chance = p_distant * share;
+                emitted = light_radiance(s, light, lambda);
+                sampled = 1;
+            }
+        } else {
+            double tc;
+            if (tree_sample(s, (ul[0] - p_distant) / fmax(1e-7, 1 - p_distant), from, normal, &emitter, &tc)) {
+                chance = (1 - p_distant) * tc;
+                uint32_t code = emitter_code(s, emitter);
+                if (code & 0x80000000u) sampled = sample_triangle(s, code & ~0x80000000u, x, ul[1], ul[2], lambda, &ls, &emitted);
+                else if (sample_light(s, code, x, ul[1], ul[2], &ls)) { emitted = light_radiance(s, code, lambda); sampled = 1; }
+            }
+        }
+        if (sampled && chance > 0) {
+            double side = dot(ls.direction, geo);
+            v3 wi = to_local(&sf, ls.direction);
+            if ((side > 0) == (wi.z > 0) && (side > 0 || sf.pt > 0)) {
+                double bp;
+                spec fv = evaluate(&sf, wi, wo, &bp);
+                if (bp > 0) {
+                    double lp = ls.pdf * chance;
+                    double w = ls.delta ? 1.0 : mis_weight(bp / lp, 0);
+                    spec c = clamp_ind(s, sscale(smul(smul(throughput, fv), emitted), fabs(wi.z) * (w / lp)), total);
+                    v3 target = add(x, mul(ls.direction, ls.distance));
+                    double margin = 4e-4 * fmax(1.0, fmax(fabs(target.x), fmax(fabs(target.y), fabs(target.z))));
+                    double rs = ls.distance >= INF ? INF : fmax(0, ls.distance - margin);
+                    double kept;
+                    if (shadow(s, offset_origin(x, side > 0 ? geo : neg(geo)), ls.direction, rs, &kept))
+                        radiance = sadd(radiance, sscale(c, kept));
+                }
+            }
+        }
+        double ub[4];
+        sample4(s, pixel, index, 3 + 3 * total, ub);
+        v3 wi;
+        double pdf;
+        uint32_t lobe;
+        if (!sample_dir(&sf, wo, ub[0], ub[1], ub[3], &wi, &pdf, &lobe)) break;
+        double again;
+        spec fv = evaluate(&sf, wi, wo, &again);
+        spec weight = sscale(fv, fabs(wi.z) / pdf);
+        v3 dir = to_world(&sf, wi);
+        double side = dot(dir, geo);
+        if ((side > 0) != (wi.z > 0)) break;
+        bounces = next_bounce(s, bounces, lobe);
+        if (bounces == NONE) break;
+        throughput = smul(throughput, weight);
+        survival_scale = fmin(survival_scale * 1.0, 1e6);
+        if (total + 1 >= 3) {
+            double survive = fmin(0.95, spec_max(s, throughput) * survival_scale);
+            if (ub[2] >= survive) break;
+            throughput = sscale(throughput, 1.0 / survive);
+        }
+        prev_normal = normal;
+        origin = side > 0 ? from : offset_origin(x, neg(geo));
+        vertex = origin;
+        rd = dir;
+        ray_pdf = pdf;
+    }
+    v3 value = spec_to_film(s, radiance, lambda);
+    if (isnan(value.x) || isnan(value.y) || isnan(value.z) || isinf(value.x) || isinf(value.y) || isinf(value.z)) {
+        value = V(0, 0, 0);
+        if (nan) (*nan)++;
+    }
+    return value;
+}
+
+typedef struct { const oscene *s; const uint32_t *pixels; size_t n; double *out; size_t begin, end; } rjob;
+
+static void *render_run(void *arg) {
+    rjob *j = (rjob *)arg;
+    for (size_t i = j->begin; i < j->end; i++) {
+        uint32_t pixel = j->pixels[i];
+        v3 sum = V(0, 0, 0);
+        int nan = 0;
+        for (uint32_t k = 0; k < j->s->samples; k++) sum = add(sum, trace_path(j->s, pixel, k, &nan));
+        v3 mean = mul(sum, 1.0 / fmax((double)j->s->samples, 1.0));
+        if (j->s->spectral) {
+            v3 r0 = Kv(j->s, 3), r1 = Kv(j->s, 4), r2 = Kv(j->s, 5);
+            mean = V(dot(r0, mean), dot(r1, mean), dot(r2, mean));
+        }
+        j->out[i * 3] = mean.x; j->out[i * 3 + 1] = mean.y; j->out[i * 3 + 2] = mean.z;
+    }
+    return NULL;
+}
+
+/* Renders the listed pixels (camera paths, light tracing and guiding off):
+ * ACEScg, 3 doubles each. `frame`: the 16 packed floats. */
+int or_render(const float *positions, const uint32_t *indices, const uint32_t *shading, size_t triangles,
+              const void *nodes, size_t node_count, const float *lights, const float *constants,
+              const float *materials, const uint32_t *emitters, const float *frame, uint32_t width, uint32_t height,
+              uint32_t samples, uint32_t max_bounces, uint32_t spectral, uint32_t seed, uint32_t through,
+              const uint32_t *pixels, size_t n, int threads, double *out) {
+    oscene s;
+    double fr[16];
+    for (int k = 0; k < 16; k++) fr[k] = frame[k];
+    static double tables[T_SIZE];
+    static int tables_ready = 0;
+    if (!tables_ready) { or_make_tables(tables); tables_ready = 1; }
+    s.positions = positions; s.indices = indices; s.shading = shading; s.triangles = triangles;
+    s.nodes = (const onode *)nodes; s.node_count = node_count;
+    s.lights = lights; s.constants = constants; s.materials = materials; s.emitters = emitters;
+    s.frame = fr; s.width = width; s.height = height; s.samples = samples; s.max_bounces = max_bounces;
+    s.spectral = spectral; s.seed = seed; s.through = through; s.tables = tables;
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    pthread_t th[64];
+    rjob jobs[64];
+    for (int k = 0; k < threads; k++) {
+        jobs[k] = (rjob){&s, pixels, n, out, n * (size_t)k / (size_t)threads, n * (size_t)(k + 1) / (size_t)threads};
+        pthread_create(&th[k], NULL, render_run, &jobs[k]);
+    }
+    for (int k = 0; k < threads; k++) pthread_join(th[k], NULL);
+    return 0;
+}

@@ -1,0 +1,164 @@
+"""GPU parity of the wavefront integrator: closed forms (white furnace), the
+f64 oracle integrator pixel by pixel at fixed Sobol seeds, and the
+reference's self-consistency checks (spectral vs RGB, light tracing vs camera
+paths alone, recorded vs launched passes)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _render(device, gset, **kw):
+    from luce_render import render
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(gset)
+    st = RenderSettings.of(kw.pop("width"), kw.pop("height"), kw.pop("samples"))
+    for k, v in kw.items():
+        setattr(st, k, v)
+    say(f"render {st.width}x{st.height} spp {st.samples} spectral {st.spectral} lt {st.light_tracing} graph {st.use_graph}")
+    img = render.render(device, sc, st)
+    return sc, st, img
+
+
+@pytest.mark.parametrize("spectral", [False, True])
+def test_white_furnace(device, spectral):
+    """tests/gpu/integrator.lucb:87-109 with albedo 0.8: a convex Lambert
+    sphere under a constant environment L = 1 is exactly 0.8 L; the
+    environment itself 1."""
+    from luce_render import scenes
+    _, _, img = _render(device, scenes.furnace_set(), width=64, height=48, samples=32, spectral=spectral,
+                        background=(1.0, 1.0, 1.0), max_bounces=8)
+    centre = img[16:32, 24:40, :3].mean(axis=(0, 1))
+    corner = img[0:6, 0:6, :3].mean(axis=(0, 1))
+    assert np.all(np.abs(centre - 0.8) < 0.01), centre
+    assert np.all(np.abs(corner - 1.0) < 0.01), corner
+    assert np.all(np.isfinite(img))
+
+
+def _oracle_compare(sc, st, img, pixels):
+    from luce_render import render
+    from luce_render.loading import Loaded
+    frame = render.ray_frame(sc.cameras[0], st.width, st.height)
+    loaded = Loaded(sc, st, frame)
+    ref = O.render(sc, loaded, frame.pack(), st.width, st.height, st.samples, st.max_bounces, st.spectral, st.seed,
+                   pixels)
+    got = img.reshape(-1, 4)[pixels, :3].astype(np.float64)
+    return got, ref
+
+
+@pytest.mark.parametrize("spectral", [False, True])
+def test_cornell_matches_f64_oracle(device, spectral):
+    """The Cornell workload against the f64 oracle at the same Sobol seeds,
+    camera paths only: the same paths, so pixel values agree to f32 rounding
+    except where rounding flips a decision (Russian roulette, a grazing hit).
+    Tolerances: the image mean within 0.5%; 90% of pixels within 1e-3
+    relative (plus 1e-6 absolute)."""
+    from luce_render import scenes
+    sc, st, img = _render(device, scenes.cornell_render_set(), width=96, height=96, samples=16, spectral=spectral,
+                          max_bounces=8, light_tracing=0.0)
+    pixels = np.arange(96 * 96, dtype=np.uint32)
+    got, ref = _oracle_compare(sc, st, img, pixels)
+    rel = np.abs(got - ref) / (np.abs(ref) + 1e-6)
+    per_pixel = rel.max(axis=1)
+    close = (per_pixel < 1e-3).mean()
+    say(f"cornell spectral={spectral}: mean gpu {got.mean(0)} oracle {ref.mean(0)}, within 1e-3: {close:.4f}, "
+        f"median rel {np.median(per_pixel):.2e}")
+    assert np.all(np.abs(got.mean(0) - ref.mean(0)) <= 0.005 * ref.mean(0))
+    assert close >= 0.90
+
+
+@pytest.mark.parametrize("spectral", [False, True])
+def test_furnace_matches_f64_oracle(device, spectral):
+    from luce_render import scenes
+    sc, st, img = _render(device, scenes.furnace_set(), width=48, height=36, samples=8, spectral=spectral,
+                          background=(1.0, 1.0, 1.0), max_bounces=8, light_tracing=0.0)
+    pixels = np.arange(48 * 36, dtype=np.uint32)
+    got, ref = _oracle_compare(sc, st, img, pixels)
+    per_pixel = (np.abs(got - ref) / (np.abs(ref) + 1e-6)).max(axis=1)
+    assert (per_pixel < 1e-4).mean() >= 0.97
+    assert np.abs(got.mean() - ref.mean()) < 1e-3
+
+
+def test_spectral_agrees_with_rgb(device):
+    """tests/gpu/integrator.lucb:773-798: the two builds agree on the Cornell
+    mean within the spectral upsampling's error (3%)."""
+    from luce_render import scenes
+    means = []
+    for spectral in (False, True):
+        _, _, img = _render(device, scenes.cornell_render_set(), width=96, height=96, samples=64, spectral=spectral,
+                            max_bounces=8, light_tracing=0.0)
+        means.append(img[..., :3].mean(axis=(0, 1)))
+    say("rgb", means[0], "spectral", means[1])
+    assert np.all(np.abs(means[1] - means[0]) <= 0.03 * means[0])
+
+
+@pytest.mark.parametrize("spectral", [False, True])
+def test_light_tracing_is_unbiased(device, spectral):
+    """tests/gpu/integrator.lucb:434-465: with light paths (0.5 a pixel, MIS
+    over three techniques) the image mean matches camera paths alone within
+    1%, and its variance does not grow."""
+    from luce_render import scenes
+    _, _, off = _render(device, scenes.cornell_render_set(), width=96, height=96, samples=128, spectral=spectral,
+                        max_bounces=8, light_tracing=0.0)
+    _, _, on = _render(device, scenes.cornell_render_set(), width=96, height=96, samples=128, spectral=spectral,
+                       max_bounces=8, light_tracing=0.5)
+    m_off, m_on = off[..., :3].mean(axis=(0, 1)), on[..., :3].mean(axis=(0, 1))
+    say("camera paths", m_off, "with light paths", m_on)
+    assert np.all(np.abs(m_on - m_off) <= 0.01 * m_off)
+    assert np.all(np.isfinite(on))
+
+
+def test_recorded_pass_equals_launched_pass(device):
+    """The hipGraph replay of a pass gives the image the plain launches give,
+    bit for bit (camera paths only: no float-atomic splats)."""
+    from luce_render import scenes
+    _, _, a = _render(device, scenes.cornell_render_set(), width=64, height=64, samples=8, spectral=True,
+                      max_bounces=6, light_tracing=0.0, use_graph=True)
+    _, _, b = _render(device, scenes.cornell_render_set(), width=64, height=64, samples=8, spectral=True,
+                      max_bounces=6, light_tracing=0.0, use_graph=False)
+    assert np.array_equal(a, b)
+
+
+def test_bands_equal_one_pass(device):
+    """A pool smaller than the image traces it in bands (integrator.lucb:
+    788-796): the same image as one band."""
+    from luce_render import scenes
+    _, _, a = _render(device, scenes.cornell_render_set(), width=64, height=64, samples=4, spectral=False,
+                      max_bounces=6, light_tracing=0.0, pool=0)
+    _, _, b = _render(device, scenes.cornell_render_set(), width=64, height=64, samples=4, spectral=False,
+                      max_bounces=6, light_tracing=0.0, pool=1000)
+    assert np.array_equal(a, b)
+
+
+def test_renderer_lifecycle(device):
+    """advance keeps at most two passes in flight and never waits; finish
+    completes every sample; kernel_times charges the kinds with profile on."""
+    from luce_render import render, scenes
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(scenes.cornell_render_set())
+    st = RenderSettings.of(48, 48, 6)
+    st.profile = True
+    r = render.Renderer.create(device, sc, st)
+    try:
+        while r.advance():
+            pass
+        assert r.samples_done() == 6
+        times = r.kernel_times()
+        assert times["intersect closest"] > 0 and times["shade surface"] > 0 and times["camera"] > 0
+        img = r.read()
+        assert img.shape == (48, 48, 4) and np.all(np.isfinite(img))
+    finally:
+        r.destroy()
+    with pytest.raises(render.LuceError) as e:
+        render.Renderer.create(device, sc, RenderSettings.of(0, 10, 1))
+    assert e.value.code == 400
+    bad = RenderSettings.of(8, 8, 1)
+    bad.max_bounces = 0
+    with pytest.raises(render.LuceError):
+        render.Renderer.create(device, sc, bad)
