@@ -1,0 +1,160 @@
+"""Host-side pins of the render path's inputs (no GPU): the reference's own
+known answers for the spectrum fit, blackbody and light tree
+(src/render/tests.lucb:93-124, 155-183), the sampler's direction table and
+stratification, and the packed layouts."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def test_white_fit_is_constant_and_film_returns_white():
+    """tests.lucb:93-99."""
+    from luce_render import spectrum as S
+    white = S.fit_spectrum((1.0, 1.0, 1.0))
+    assert white.c0 == 0.0 and white.c1 == 0.0 and white.c2 == 0.0 and abs(white.at(500.0) - 1.0) < 1e-12
+    back = S.film_matrix() @ np.ones(3)
+    assert np.all(np.abs(back - 1.0) < 1e-6)
+
+
+@pytest.mark.parametrize("color", [(0.8, 0.2, 0.1), (0.05, 0.5, 0.2), (0.1, 0.2, 0.9), (0.9, 0.9, 0.2)])
+def test_fit_round_trip(color):
+    """tests.lucb:100-109: a colour's fitted spectrum through the film comes back within 0.01."""
+    from luce_render import spectrum as S
+    fit = S.fit_spectrum(color)
+    lam = np.arange(S.LAMBDA_MIN, S.LAMBDA_MAX + 0.5, 1.0)
+    xyz = (S.cmf(lam) * fit.at(lam)[:, None]).sum(axis=0) / S.cmf_integrals()
+    rgb = S.film_matrix() @ xyz
+    assert np.all(np.abs(rgb - np.array(color)) < 0.01), rgb
+
+
+def test_blackbody_colours():
+    """tests.lucb:111-122: 5455 K near neutral, 3000 K warm, 9000 K cool."""
+    from luce_render import spectrum as S
+    m = S.film_matrix()
+    n = m @ (S.blackbody_xyz(5455.0) / S.blackbody_xyz(5455.0)[1])
+    assert np.all(np.abs(n - 1.0) < 0.06)
+    w = m @ (S.blackbody_xyz(3000.0) / S.blackbody_xyz(3000.0)[1])
+    assert w[0] > 1.2 * w[1] and w[1] > w[2]
+    k = m @ (S.blackbody_xyz(9000.0) / S.blackbody_xyz(9000.0)[1])
+    assert k[2] > k[1] > k[0]
+
+
+def test_cmf_integrals_make_constant_spectrum_white():
+    """spectrum.lucb:12-14: the constant spectrum 1 is XYZ (1, 1, 1) under E."""
+    from luce_render import spectrum as S
+    lam = np.arange(S.LAMBDA_MIN, S.LAMBDA_MAX + 0.5, 1.0)
+    assert np.allclose(S.cmf(lam).sum(axis=0) / S.cmf_integrals(), 1.0)
+
+
+def test_light_tree_trails_reach_every_emitter_and_hold_energy():
+    """tests.lucb:155-183: an emissive 9x9 grid (162 triangles): every
+    emitter's trail from the root ends at its leaf, depth <= 8, the triangle
+    map points back, and the root holds the total energy."""
+    from luce_render import render
+    from luce_render.loading import OpenPbr, build_light_tree
+    xs = np.linspace(-2, 2, 10)
+    px, pz = np.meshgrid(xs, xs, indexing="ij")
+    pts = np.stack([px.ravel(), np.zeros(px.size), pz.ravel()], 1)
+    tris = []
+    for i in range(9):
+        for j in range(9):
+            a, b, c, d = i * 10 + j, (i + 1) * 10 + j, (i + 1) * 10 + j + 1, i * 10 + j + 1
+            tris += [[a, b, c], [a, c, d]]
+    sc = render.compile_scene(render.GeometrySet(meshes=[render.Mesh(pts, np.array(tris))]))
+    glowing = OpenPbr(emission_luminance=2.0)
+    nodes, links, records, count, energy = build_light_tree(sc, [], 0, [glowing])
+    assert count == sc.triangle_count() == 162
+    total = 0.0
+    for index in range(count):
+        trail, depth = int(records[index * 4 + 1]), int(records[index * 4 + 2])
+        assert depth <= 8
+        node = 0
+        for level in range(depth):
+            node = int(links[node * 2]) + ((trail >> level) & 1)
+        assert links[node * 2 + 1] == 0x80000000 and links[node * 2] == index
+        code = int(records[index * 4])
+        assert code & 0x80000000 and records[count * 4 + (code & 0x7FFFFFFF)] == index
+        total += nodes[node * 12 + 3]
+    assert abs(nodes[3] / total - 1.0) < 1e-4
+
+
+def _sobol_directions():
+    """Joe–Kuo direction numbers for dimensions 1..3 (new-joe-kuo-6.21201:
+    s, a, m = (1,0,[1]), (2,1,[1,3]), (3,1,[1,3,1])), dimension 0 van der Corput."""
+    params = [(1, 0, [1]), (2, 1, [1, 3]), (3, 1, [1, 3, 1])]
+    rows = [[1 << (31 - b) for b in range(32)]]
+    for s, a, m in params:
+        v = [0] * 33
+        for k in range(1, 33):
+            if k <= s:
+                v[k] = m[k - 1] << (32 - k)
+            else:
+                x = v[k - s] ^ (v[k - s] >> s)
+                for i in range(1, s):
+                    if (a >> (s - 1 - i)) & 1:
+                        x ^= v[k - i]
+                v[k] = x
+        rows.append([x & 0xFFFFFFFF for x in v[1:]])
+    return rows
+
+
+def test_sobol_table_is_joe_kuo():
+    """The kernels' and the oracle's direction table (shaders/sobol_table.glsl,
+    a reference-held vector) regenerated from Joe and Kuo's parameters."""
+    rows = _sobol_directions()
+    assert rows[1][:4] == [0x80000000, 0xC0000000, 0xA0000000, 0xF0000000]
+    assert rows[3][31] == 0x50050093 and rows[2][31] == 0xC5005555
+
+
+def test_sampler_stratifies_every_dimension():
+    """Owen scrambling keeps Sobol's elementary intervals: the first 2^k
+    samples of a pixel put exactly one point in each 1/2^k interval of every
+    coordinate (a property a wrong scramble, table or bit order breaks)."""
+    for group in (0, 3, 7):
+        for pixel in (0, 12345):
+            vals = np.array([O.sample4(1, pixel, i, group)[1] for i in range(64)])
+            for d in range(4):
+                bins = np.floor(vals[:, d] * 64).astype(int)
+                assert sorted(bins.tolist()) == list(range(64)), (group, pixel, d)
+
+
+def test_sampler_differs_by_pixel_and_group():
+    a = O.sample4(1, 5, 0, 0)[0]
+    assert not np.array_equal(a, O.sample4(1, 6, 0, 0)[0])
+    assert not np.array_equal(a, O.sample4(1, 5, 0, 1)[0])
+    assert not np.array_equal(a, O.sample4(2, 5, 0, 0)[0])
+
+
+def test_oracle_tables_bound_albedo():
+    """The f64 albedo tables (tables.comp's estimator): E_ggx in (0, 1],
+    nearly 1 when smooth, falling with roughness at grazing angles; the
+    dielectric table below the GGX one (Fresnel <= 1)."""
+    t = O.tables()
+    ggx = t[:32 * 32].reshape(32, 32)  # [roughness][mu]
+    assert np.all(ggx > 0) and np.all(ggx <= 1.0 + 1e-9)
+    assert ggx[0].min() > 0.99
+    assert ggx[31, 0] < ggx[0, 0]
+    die = t[32 * 32 + 32:].reshape(16, 32, 32)
+    assert np.all(die <= ggx[None] + 1e-12)
+
+
+def test_packed_layouts():
+    """Light records are 7 float4 (lights.lucb), materials 16 float4
+    (materials.lucb), the constants 32 rows."""
+    from luce_render import render, scenes
+    from luce_render.loading import Loaded, RenderSettings
+    sc = render.compile_scene(scenes.cornell_render_set())
+    st = RenderSettings.of(64, 64, 1)
+    L = Loaded(sc, st, render.ray_frame(sc.cameras[0], 64, 64))
+    assert L.light_count == 1 and L.emitter_count == 1
+    rec = L.lights[:28]
+    assert rec[3] == 0.0  # rect
+    assert abs(rec[15] - 0.0625) < 1e-6  # area of 0.25 x 0.25
+    assert np.allclose(rec[12:15], [0, -1, 0])  # shines down
+    assert abs(rec[20] - 17.0) < 1e-5  # radiance
+    assert len(L.materials) == 16 * 4 * (1 + len(sc.materials))
+    assert L.constants.size == 128 and L.constants[1] == 12
+    assert L.light_paths == 64 * 64 // 2

@@ -57,7 +57,7 @@ def test_cornell_matches_f64_oracle(device, spectral):
     """The Cornell workload against the f64 oracle at the same Sobol seeds,
     camera paths only: the same paths, so pixel values agree to f32 rounding
     except where rounding flips a decision (Russian roulette, a grazing hit).
-    Tolerances: the image mean within 0.5%; 90% of pixels within 1e-3
+    Tolerances: the image mean within 0.5%; 99% of pixels within 1e-3
     relative (plus 1e-6 absolute)."""
     from luce_render import scenes
     sc, st, img = _render(device, scenes.cornell_render_set(), width=96, height=96, samples=16, spectral=spectral,
@@ -70,7 +70,7 @@ def test_cornell_matches_f64_oracle(device, spectral):
     say(f"cornell spectral={spectral}: mean gpu {got.mean(0)} oracle {ref.mean(0)}, within 1e-3: {close:.4f}, "
         f"median rel {np.median(per_pixel):.2e}")
     assert np.all(np.abs(got.mean(0) - ref.mean(0)) <= 0.005 * ref.mean(0))
-    assert close >= 0.90
+    assert close >= 0.99
 
 
 @pytest.mark.parametrize("spectral", [False, True])
