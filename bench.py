@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--instances", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--config", default="", help="width,group,stack to skip the sweep")
+    ap.add_argument("--render-spp", type=int, default=16,
+                    help="samples of the full path-traced render of the same scene (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the plain copy kernel over this many MiB (a known-traffic dispatch "
                          "for scaling hardware byte counters)")
@@ -133,20 +135,16 @@ def main():
         timer.destroy()
         return ms / k
 
-    # Traversal variant: a short sweep over BVH width (the autotuner's full
-    # sweep over width x group x stack lives in luce_render.autotune).
+    # Traversal variant: the autotuner's sweep over BVH width x workgroup x
+    # LDS stack depth on both batches (every variant returns the same hits).
     if args.config:
-        w, g, s = (int(x) for x in args.config.split(","))
-        best = (w, g, s)
+        w, g, s_ = (int(x) for x in args.config.split(","))
+        best = (w, g, s_)
+        sweep = None
     else:
-        results = {}
-        for w in (2, 4, 8):
-            for g in (64, 128, 256):
-                gpu.set_trace_config(w, g, 32)
-                step()
-                results[(w, g, 32)] = time_steps(2)
-        best = min(results, key=results.get)
-        log("sweep:", {f"w{k[0]}g{k[1]}s{k[2]}": round(v, 3) for k, v in results.items()})
+        from luce_render import autotune
+        best, sweep = autotune.tune(device, gpu, rays_b, n, hits=hits_b, frame=frame, repeats=2)
+        log("sweep (ms, probe + bounce):", {f"w{k[0]}g{k[1]}s{k[2]}": round(v, 3) for k, v in sweep.items()})
     gpu.set_trace_config(*best)
 
     for _ in range(args.warmup):
@@ -194,6 +192,38 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(f"{dom_name}<{best[0]},{best[2]},{best[1]}>")
 
+    # The full wavefront path tracer on the same scene: RGB, 12 bounces,
+    # 256 lights in the light tree, light tracing at the reference's default.
+    render_stats = None
+    if args.render_spp > 0:
+        from luce_render.loading import RenderSettings
+        st = RenderSettings.of(WIDTH, HEIGHT, args.render_spp)
+        st.spectral = False
+        st.max_bounces = 12
+        st.trace = best
+        renderer = render.Renderer.create(device, sc, st, gpu_scene=gpu)
+        warm = render.Renderer.create(device, sc, RenderSettings(WIDTH, HEIGHT, 1, spectral=False, trace=best),
+                                      gpu_scene=gpu)
+        warm.finish()
+        warm.destroy()
+        render.synchronize(device)
+        timer = render.Timer()
+        timer.begin(stream)
+        renderer.finish()
+        timer.end(stream)
+        ms = timer.elapsed_ms()
+        timer.destroy()
+        stats = renderer.stats()
+        img = renderer.read()
+        renderer.destroy()
+        rays = stats["closest_rays"] + stats["shadow_rays"]
+        render_stats = {"spp": args.render_spp, "ms_per_sample": round(ms / args.render_spp, 3),
+                        "samples_per_s": round(args.render_spp * WIDTH * HEIGHT / (ms * 1e-3), 1),
+                        "mrays_per_s_in_render": round(rays / (ms * 1e-3) / 1e6, 1),
+                        "closest_rays": stats["closest_rays"], "shadow_rays": stats["shadow_rays"],
+                        "finite": bool(np.all(np.isfinite(img)))}
+        log("render:", render_stats)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -227,10 +257,12 @@ def main():
                         f"{sc.triangle_count()}-triangle scene (1,000 instances of 64 displaced icospheres)",
             "rays_per_step": total_rays,
             "trace_config": {"width": best[0], "group": best[1], "stack": best[2]},
+            "autotune_ms": None if sweep is None else {f"{k[0]},{k[1]},{k[2]}": round(v, 3) for k, v in sweep.items()},
             "primary_mrays_s": round(n / (ms_p * 1e-3) / 1e6, 2),
             "bounce_mrays_s": round(n / (ms_b * 1e-3) / 1e6, 2),
             "bytes_per_ray": {"primary": round(bytes_p, 1), "bounce": round(bytes_b, 1)},
             "bvh_nodes": info["binary_nodes"],
+            "path_tracing": render_stats,
             "wide_bytes": info["wide_bytes"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

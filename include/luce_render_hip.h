@@ -202,6 +202,15 @@ int lr_renderer_read_film(lr_renderer *renderer, float *planes, size_t count);
  * in the order camera, intersect closest, shade surface, shade miss,
  * intersect shadow, light emission, finish, film convert, adaptive, rest. */
 int lr_renderer_kernel_times(lr_renderer *renderer, double out[10]);
+/* Counters of the work done so far (waits for passes in flight): no
+ * reference counterpart, for throughput reports. */
+typedef struct lr_render_stats {
+    uint32_t samples_done;
+    uint32_t light_samples;
+    uint64_t closest_rays;
+    uint64_t shadow_rays;
+} lr_render_stats;
+int lr_renderer_stats(lr_renderer *renderer, lr_render_stats *out);
 /* Renderer.destroy (:887): waits for passes in flight. */
 void lr_renderer_destroy(lr_renderer *renderer);
 

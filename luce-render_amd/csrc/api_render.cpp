@@ -374,6 +374,21 @@ int lr_renderer_kernel_times(lr_renderer *r, double out[10]) {
     return 0;
 }
 
+int lr_renderer_stats(lr_renderer *r, lr_render_stats *out) {
+    if (!r || !out) return fail(LR_INVALID, "lr_renderer_stats needs a renderer and an output");
+    int rc = r->retire(true);
+    if (rc) return rc;
+    rt::PassState ps;
+    hipError_t e = hipMemcpyAsync(&ps, r->params.pass, sizeof(ps), hipMemcpyDeviceToHost, r->device->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
+    if (e != hipSuccess) return hip_fail(e, "stats read");
+    out->samples_done = ps.samples_done;
+    out->light_samples = ps.light_samples;
+    out->closest_rays = ps.closest_rays;
+    out->shadow_rays = ps.shadow_rays;
+    return 0;
+}
+
 void lr_renderer_destroy(lr_renderer *r) {
     if (!r) return;
     (void)hipSetDevice(r->device->ordinal);

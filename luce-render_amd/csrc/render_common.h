@@ -77,6 +77,8 @@ struct PassState {
     uint32_t light_samples; // light-tracing samples taken so far (film_convert divides splats by them)
     uint32_t samples_done;  // whole samples finished
     uint32_t bands;         // bands a sample
+    unsigned long long closest_rays;  // rays the closest-hit kernels have traced
+    unsigned long long shadow_rays;   // shadow rays traced
 };
 
 // Launch parameters fixed for a view: pointers, sizes, the frame.

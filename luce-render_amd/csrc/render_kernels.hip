@@ -371,7 +371,10 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(Params p, uint32_t curren
     const PassState &ps = *p.pass;
     uint32_t queue = Q_CLOSEST + current;
     uint32_t count = p.queues[queue];
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.queues[Q_SHADOW] = 0u;  // consumed by the last bounce's shadow kernel
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.queues[Q_SHADOW] = 0u;  // consumed by the last bounce's shadow kernel
+        p.pass->closest_rays += count;
+    }
     uint32_t stride_threads = gridDim.x * G;
     DeviceScene sc = p.scene;
     sc.spill_stride = stride_threads;
@@ -910,7 +913,10 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
     State st(p);
     const PassState &ps = *p.pass;
     uint32_t count = p.queues[Q_SHADOW];
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.queues[Q_MISS] = 0u;  // consumed by this bounce's miss kernel
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.queues[Q_MISS] = 0u;  // consumed by this bounce's miss kernel
+        p.pass->shadow_rays += count;
+    }
     DeviceScene sc = p.scene;
     sc.spill_stride = gridDim.x * G;
     uint32_t nlights = light_count(p);

@@ -65,6 +65,11 @@ class Image(ctypes.Structure):
                 ("height", ctypes.c_uint32)]
 
 
+class RenderStats(ctypes.Structure):
+    _fields_ = [("samples_done", ctypes.c_uint32), ("light_samples", ctypes.c_uint32),
+                ("closest_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64)]
+
+
 _P = ctypes.c_void_p
 _SIGNATURES = {
     "lr_device_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
@@ -91,6 +96,7 @@ _SIGNATURES = {
     "lr_renderer_read_film": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "lr_renderer_kernel_times": (ctypes.c_int, [_P, _P]),
     "lr_renderer_destroy": (None, [_P]),
+    "lr_renderer_stats": (ctypes.c_int, [_P, ctypes.POINTER(RenderStats)]),
     "lr_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "lr_buffer_free": (None, [_P, _P]),
     "lr_buffer_write": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),

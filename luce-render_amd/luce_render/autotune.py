@@ -1,0 +1,73 @@
+"""The traversal autotuner: times every (BVH width, workgroup size, LDS stack
+depth) variant of the closest-hit kernel on a scene and a ray batch and
+picks the fastest. The variants return identical hits (their parity is
+tested), so the choice is by time alone.
+
+    best, table = autotune.tune(device, gpu_scene, rays, n)
+    gpu_scene.set_trace_config(*best)
+"""
+import json
+
+from . import render
+
+WIDTHS = (2, 4, 8)
+GROUPS = (64, 128, 256)
+STACKS = (24, 32, 48)
+
+
+def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks=STACKS, repeats=3, frame=None):
+    """Times trace_closest_device over `rays` (a Buffer of n rays), and the
+    probe of `frame` too when given, for each variant; returns the fastest
+    (width, group, stack) and {variant: ms per launch}."""
+    stream = render.device_stream(device)
+    own = hits is None
+    if own:
+        hits = render.Buffer(device, max(n, 1) * 8)
+    probe_hits = render.Buffer(device, frame.width * frame.height * 8) if frame is not None else None
+    table = {}
+    timer = render.Timer()
+    try:
+        for w in widths:
+            for g in groups:
+                for s in stacks:
+                    scene.set_trace_config(w, g, s)
+
+                    def once():
+                        render.trace_closest_device(scene, rays, n, hits, stream)
+                        if frame is not None:
+                            render.probe_device(scene, frame, probe_hits, stream)
+
+                    once()  # first launch: spill area and code load
+                    render.synchronize(device)
+                    timer.begin(stream)
+                    for _ in range(repeats):
+                        once()
+                    timer.end(stream)
+                    table[(w, g, s)] = timer.elapsed_ms() / repeats
+    finally:
+        timer.destroy()
+        if own:
+            hits.destroy()
+        if probe_hits is not None:
+            probe_hits.destroy()
+    best = min(table, key=table.get)
+    scene.set_trace_config(*best)
+    return best, table
+
+
+def save(path, best, table, key=""):
+    with open(path, "w") as f:
+        json.dump({"key": key, "best": list(best), "table": {f"{w},{g},{s}": ms for (w, g, s), ms in table.items()}},
+                  f, indent=1)
+
+
+def load(path, key=""):
+    """The cached choice for `key`, or None."""
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("key") != key:
+        return None
+    return tuple(d["best"])

@@ -414,6 +414,12 @@ class Renderer:
         _lib.check(_lib.lib().lr_renderer_kernel_times(self.handle, _lib.ptr(out)))
         return dict(zip(KERNEL_NAMES, out.tolist()))
 
+    def stats(self):
+        st = _lib.RenderStats()
+        _lib.check(_lib.lib().lr_renderer_stats(self.handle, ctypes.byref(st)))
+        return {"samples_done": st.samples_done, "light_samples": st.light_samples,
+                "closest_rays": st.closest_rays, "shadow_rays": st.shadow_rays}
+
     def destroy(self):
         if self.handle:
             _lib.lib().lr_renderer_destroy(self.handle)
