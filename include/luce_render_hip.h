@@ -214,6 +214,43 @@ int lr_renderer_stats(lr_renderer *renderer, lr_render_stats *out);
 /* Renderer.destroy (:887): waits for passes in flight. */
 void lr_renderer_destroy(lr_renderer *renderer);
 
+/* ------------------------------------------------------------------- serving */
+
+/* Many renders resident on one GPU, advanced together: each step runs one
+ * sample of every active request in shared kernel launches. Requests are
+ * admitted in arrival order once their device memory fits the budget. No
+ * reference counterpart (the reference renders one view at a time). */
+typedef struct lr_server lr_server;
+typedef struct lr_server_settings {
+    uint64_t budget_bytes;  /* device memory for requests' state, films and images */
+    uint32_t spectral;      /* every request's build */
+    uint32_t trace_width;   /* the traversal variant every request uses */
+    uint32_t trace_group;
+    uint32_t trace_stack;
+    uint32_t max_active;    /* 0: no limit beyond the budget */
+} lr_server_settings;
+typedef struct lr_request_status {
+    uint32_t state;         /* 0 waiting for admission, 1 rendering, 2 finished */
+    uint32_t samples_done;
+    uint32_t samples;
+    uint64_t bytes;         /* its device memory */
+} lr_request_status;
+int lr_server_create(lr_device *device, const lr_server_settings *settings, lr_server **out);
+/* Queues a render of `scene` (which must outlive the request); the loaded
+ * arrays are copied. 401 when the request alone exceeds the budget. */
+int lr_server_submit(lr_server *server, lr_gpu_scene *scene, const lr_loaded *loaded,
+                     const lr_render_settings *settings, const float frame[16], uint64_t *id);
+/* Admits what fits, runs one sample of every active request and waits for
+ * it; requests that reach their samples finish. */
+int lr_server_step(lr_server *server, uint32_t *active, uint32_t *pending);
+int lr_server_status(lr_server *server, uint64_t id, lr_request_status *out);
+/* A finished request's image (rgba32f, row by row from the top). */
+int lr_server_read(lr_server *server, uint64_t id, float *rgba, size_t count);
+int lr_server_stats(lr_server *server, uint64_t id, lr_render_stats *out);
+/* Frees a request: cancels it when waiting or rendering. */
+int lr_server_release(lr_server *server, uint64_t id);
+void lr_server_destroy(lr_server *server);
+
 /* Device buffers and timing: the part of luce-gpu (gpu.Buffer.allocate /
  * create / read / upload, Compute.wait and timestamps; used at
  * src/render/device.lucb:137-156 and integrator.lucb:848-885) that callers of

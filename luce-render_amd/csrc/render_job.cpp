@@ -1,0 +1,243 @@
+// A render's device state and the pass launch sequence
+// (src/render/integrator.lucb:417-438 allocate_view, :788-828 record).
+#include "render_job.h"
+
+#include <cstring>
+#include <memory>
+
+#include "render_bsdf.h"
+#include "render_guide.h"
+
+namespace lr {
+
+uint32_t grid_blocks(uint64_t items, uint32_t group) {
+    uint64_t b = (items + group - 1) / group;
+    if (b < 1) b = 1;
+    if (b > 65535) b = 65535;
+    return (uint32_t)b;
+}
+
+namespace {
+
+struct Shape {
+    uint32_t pixels, band, light_paths, slots, bands;
+};
+
+Shape shape_of(const lr_loaded *loaded, const lr_render_settings &st) {
+    Shape s;
+    s.pixels = st.width * st.height;
+    uint32_t pool = st.pool ? st.pool : 2097152u;  // integrator.lucb:96-98
+    s.band = pool < s.pixels ? pool : s.pixels;
+    s.light_paths = loaded->light_paths < s.band ? loaded->light_paths : s.band;
+    s.slots = s.band + s.light_paths;
+    s.bands = (s.pixels + s.band - 1) / s.band;
+    return s;
+}
+
+int width_index(uint32_t w) { return w == 2 ? 0 : w == 4 ? 1 : 2; }
+
+}  // namespace
+
+RenderJob::~RenderJob() {
+    for (void *b : buffers) (void)hipFree(b);
+}
+
+void *RenderJob::alloc(size_t n, int &rc) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, n ? n : 64);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "hipMalloc(render state)");
+        return nullptr;
+    }
+    buffers.push_back(p);
+    bytes += n ? n : 64;
+    return p;
+}
+
+size_t RenderJob::keep_image_only() {
+    std::vector<void *> kept;
+    for (void *b : buffers) {
+        if (b == (void *)params.image) kept.push_back(b);
+        else (void)hipFree(b);
+    }
+    buffers.swap(kept);
+    bytes = (size_t)params.pixels * 16;
+    return bytes;
+}
+
+size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, const lr_render_settings *st) {
+    Shape s = shape_of(loaded, *st);
+    size_t b = 0;
+    b += loaded->light_floats * 4 + 512 + loaded->material_floats * 4 + loaded->emitter_words * 4;
+    b += (size_t)rt::kStateV * s.slots * 16 + (size_t)rt::kStateU * s.slots * 4;
+    b += ((size_t)rt::kQueueBase + (size_t)rt::kQueues * s.slots) * 4;
+    b += (size_t)6 * s.pixels * 16;  // film (4 planes), image, display
+    if (st->guiding) b += rt::kGuideBytes;
+    uint32_t w = st->trace_width ? st->trace_width : scene->config.width;
+    uint32_t stack = st->trace_stack ? st->trace_stack : scene->config.stack;
+    uint32_t group = st->trace_group ? st->trace_group : scene->config.group;
+    uint32_t bound = scene->info.stack_bound[width_index(w)];
+    if (bound > stack) b += (size_t)(bound - stack) * grid_blocks(s.slots, group) * group * 4;
+    return b;
+}
+
+int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded, const lr_render_settings *settings_in,
+                      const float frame[16], float *shared_tables) {
+    if (!dev || !sc || !loaded || !settings_in || !frame) return fail(LR_INVALID, "a render needs its arguments");
+    const lr_render_settings &st = *settings_in;
+    // integrator.lucb:344, 376-377
+    if (st.width == 0 || st.height == 0 || st.width > 16384 || st.height > 16384)
+        return fail(LR_INVALID, "an image is 1 to 16384 pixels on a side");
+    if (st.samples == 0 || st.max_bounces == 0 || st.max_bounces > 1024)
+        return fail(LR_INVALID, "a render needs samples and 1 to 1024 bounces");
+    if ((uint32_t)frame[11] != st.width || (uint32_t)frame[15] != st.height)
+        return fail(LR_INVALID, "the frame's size is not the settings' image size");
+    if (!loaded->constants || !loaded->lights || !loaded->materials || !loaded->emitters || loaded->material_floats < 64)
+        return fail(LR_INVALID, "a render needs the loaded lights, constants, materials (the default one at least) and emitters");
+    if (!sc->shading) return fail(LR_INVALID, "rendering needs the scene's shading attributes");
+    device = dev;
+    scene = sc;
+    settings = st;
+    uint32_t w = st.trace_width ? st.trace_width : sc->config.width;
+    uint32_t stack = st.trace_stack ? st.trace_stack : sc->config.stack;
+    uint32_t group = st.trace_group ? st.trace_group : sc->config.group;
+    trace_variant = variant_index(w, stack, group);
+    if (trace_variant < 0) return fail(LR_INVALID, "traversal widths are 2, 4, 8; stacks 24, 32, 48; groups 64, 128, 256");
+    settings.trace_width = w;
+    settings.trace_stack = stack;
+    settings.trace_group = group;
+    Shape sh = shape_of(loaded, st);
+    slots = sh.slots;
+    bands = sh.bands;
+    rounds = st.max_bounces + 1 + loaded->through;
+    int rc = 0;
+    rt::Params &p = params;
+    std::memcpy(p.frame, frame, 64);
+    p.pool = slots;
+    p.seed = st.seed;
+    p.pixels = sh.pixels;
+    p.width = st.width;
+    p.height = st.height;
+    p.band_size = sh.band;
+    p.light_paths = sh.light_paths;
+    p.spectral = st.spectral;
+    hipStream_t s = dev->stream;
+    auto upload = [&](const void *src, size_t n) -> void * {
+        void *d = alloc(n, rc);
+        if (!d) return nullptr;
+        if (n) {
+            hipError_t e = hipMemcpyAsync(d, src, n, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "render upload");
+                return nullptr;
+            }
+        }
+        return d;
+    };
+    p.lights = (const float4 *)upload(loaded->lights, loaded->light_floats * 4);
+    p.constants = (const float4 *)upload(loaded->constants, 128 * 4);
+    p.materials = (const float4 *)upload(loaded->materials, loaded->material_floats * 4);
+    p.emitters = (const uint32_t *)upload(loaded->emitters, loaded->emitter_words * 4);
+    if (rc) return rc;
+    float *tables = shared_tables;
+    if (!tables) {
+        tables = (float *)alloc((size_t)rt::TABLE_SIZE * 4, rc);
+        if (rc) return rc;
+        launch_tables(tables, s);
+    }
+    p.tables = tables;
+    p.shading = (const uint4 *)sc->shading;
+    p.uvs = (const float *)sc->uvs;
+    p.state_v = (float4 *)alloc((size_t)rt::kStateV * slots * 16, rc);
+    p.state_u = (uint32_t *)alloc((size_t)rt::kStateU * slots * 4, rc);
+    size_t queue_words = rt::kQueueBase + (size_t)rt::kQueues * slots;
+    p.queues = (uint32_t *)alloc(queue_words * 4, rc);
+    p.film = (float4 *)alloc((size_t)4 * sh.pixels * 16, rc);
+    p.image = (float4 *)alloc((size_t)sh.pixels * 16, rc);
+    p.display = (float4 *)alloc((size_t)sh.pixels * 16, rc);
+    p.pass = (rt::PassState *)alloc(sizeof(rt::PassState), rc);
+    if (st.guiding) p.guide = (uint8_t *)alloc(rt::kGuideBytes, rc);
+    params_device = (rt::Params *)alloc(sizeof(rt::Params), rc);
+    if (rc) return rc;
+    // The traversal's spill area for the largest grid a pass launches, and
+    // the node layout of the chosen width.
+    p.scene = sc->device_scene();
+    p.scene.nodes = (const uint8_t *)sc->wide[width_index(w)];
+    uint32_t bound = sc->info.stack_bound[width_index(w)];
+    uint64_t depth = bound > stack ? bound - stack : 0;
+    uint64_t threads = (uint64_t)grid_blocks(slots, group) * group;
+    p.scene.spill = depth ? (uint32_t *)alloc(depth * threads * 4, rc) : nullptr;
+    p.scene.spill_stride = (uint32_t)threads;
+    p.scene.spill_depth = (uint32_t)depth;
+    if (rc) return rc;
+    launch_fill((uint32_t *)p.film, (size_t)4 * sh.pixels * 4, 0u, s);
+    launch_fill((uint32_t *)p.image, (size_t)sh.pixels * 4, 0u, s);
+    launch_fill(p.queues, queue_words, 0u, s);
+    launch_fill(p.state_u, (size_t)rt::kStateU * slots, 0u, s);
+    if (p.guide) launch_fill((uint32_t *)p.guide, rt::kGuideBytes / 4, 0u, s);
+    rt::PassState init{};
+    init.bands = bands;
+    hipError_t e = hipMemcpyAsync(p.pass, &init, sizeof(init), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(params_device, &p, sizeof(p), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "render set-up");
+    return 0;
+}
+
+void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params, hipStream_t s,
+                 std::vector<std::pair<int, hipEvent_t>> *marks) {
+    if (jobs.empty()) return;
+    uint32_t n = (uint32_t)jobs.size();
+    bool sp = jobs[0]->settings.spectral != 0;
+    const RenderTraceVariant &tv = render_trace_variants()[jobs[0]->trace_variant];
+    uint32_t rounds = 0, slots = 0, band = 0, lights = 0, pixels = 0;
+    bool guide = false;
+    for (RenderJob *j : jobs) {
+        rounds = std::max(rounds, j->rounds);
+        slots = std::max(slots, j->slots);
+        band = std::max(band, j->params.band_size);
+        lights = std::max(lights, j->params.light_paths);
+        pixels = std::max(pixels, j->params.pixels);
+        guide = guide || j->params.guide;
+    }
+    auto mark = [&](int kind) {
+        if (!marks) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        marks->push_back({kind, e});
+    };
+    launch_render_stage(Stage::BeginPass, sp, params, n, 0, 1, s);
+    mark(KRest);
+    launch_render_stage(Stage::Camera, sp, params, n, 0, grid_blocks(band, 256), s);
+    mark(KCamera);
+    if (lights) {
+        launch_render_stage(Stage::LightEmit, sp, params, n, 0, grid_blocks(lights, 256), s);
+        mark(KLight);
+    }
+    // One round more than bounces: the last BSDF rays still find lights
+    // (integrator.lucb:806-814).
+    for (uint32_t b = 0; b < rounds; b++) {
+        uint32_t cur = b % 2u;
+        tv.closest(params, n, cur, grid_blocks(slots, tv.group), s);
+        mark(KClosest);
+        launch_render_stage(Stage::Shade, sp, params, n, cur, grid_blocks(slots, 128), s);
+        mark(KSurface);
+        launch_render_stage(Stage::Miss, sp, params, n, 0, grid_blocks(slots, 256), s);
+        mark(KMiss);
+        tv.shadow(params, n, grid_blocks(slots, tv.group), s);
+        mark(KShadow);
+    }
+    launch_render_stage(Stage::Finish, sp, params, n, 0, grid_blocks(band, 256), s);
+    mark(KFinish);
+    if (guide) {
+        launch_render_stage(Stage::GuideFit, sp, params, n, 0, 2048, s);
+        mark(KRest);
+    }
+    launch_render_stage(Stage::Convert, sp, params, n, 0, grid_blocks(pixels, 256), s);
+    mark(KConvert);
+    launch_render_stage(Stage::EndPass, sp, params, n, 0, 1, s);
+    mark(KRest);
+}
+
+}  // namespace lr

@@ -1,0 +1,54 @@
+// One render's device state (a request), and the launch sequence of a pass
+// over any number of renders at once.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "lr_internal.h"
+#include "lr_scene.h"
+#include "render_kernels.h"
+
+namespace lr {
+
+// Everything a render keeps on the device: loaded arrays, path state, queues,
+// film, images, its pass state and the kernel parameters (a host copy and a
+// device copy, which a single render's launches read).
+struct RenderJob {
+    lr_device *device = nullptr;
+    lr_gpu_scene *scene = nullptr;
+    lr_render_settings settings{};
+    rt::Params params{};
+    rt::Params *params_device = nullptr;
+    uint32_t rounds = 0;
+    uint32_t slots = 0;
+    uint32_t bands = 1;
+    int trace_variant = -1;
+    size_t bytes = 0;  // device memory held
+    std::vector<void *> buffers;
+
+    ~RenderJob();
+    void *alloc(size_t bytes, int &rc);
+    // Validates, allocates and uploads (integrator.lucb:343-438); the film,
+    // queues and pass state start cleared.
+    int create(lr_device *device, lr_gpu_scene *scene, const lr_loaded *loaded, const lr_render_settings *settings,
+               const float frame[16], float *shared_tables);
+    // Frees everything but the finished image; returns the bytes still held.
+    size_t keep_image_only();
+    // Device bytes a render of these settings holds (its admission cost).
+    static size_t footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, const lr_render_settings *settings);
+};
+
+uint32_t grid_blocks(uint64_t items, uint32_t group);
+
+// Launches one pass of every render in `jobs` (sharing each launch: grid row
+// y is render y), their parameters at `params` (a device array in job
+// order). All jobs share the spectral mode and traversal variant. With
+// `marks`, an event after each stage, tagged by kernel kind.
+void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params, hipStream_t s,
+                 std::vector<std::pair<int, hipEvent_t>> *marks);
+
+enum KernelKind { KCamera, KClosest, KSurface, KMiss, KShadow, KLight, KFinish, KConvert, KAdaptive, KRest, KKinds };
+
+}  // namespace lr

@@ -11,15 +11,18 @@ enum class Stage { BeginPass, Camera, LightEmit, Shade, Miss, Finish, GuideFit, 
 
 // The tracing kernels of a pass for one (width, stack, group) variant.
 struct RenderTraceVariant {
-    void (*closest)(const rt::Params &p, uint32_t current, uint32_t blocks, hipStream_t s);
-    void (*shadow)(const rt::Params &p, uint32_t blocks, hipStream_t s);
+    // `p`: a device array of `n` requests' parameters (blockIdx.y picks one).
+    void (*closest)(const rt::Params *p, uint32_t n, uint32_t current, uint32_t blocks, hipStream_t s);
+    void (*shadow)(const rt::Params *p, uint32_t n, uint32_t blocks, hipStream_t s);
     uint32_t group;
 };
 const RenderTraceVariant *render_trace_variants();
 
+// `p`: a device array of `n` requests' parameters, one per grid row.
 // `arg`: the closest-hit queue being read (Shade) or "convert every pass"
 // (Convert).
-void launch_render_stage(Stage stage, bool spectral, const rt::Params &p, uint32_t arg, uint32_t blocks, hipStream_t s);
+void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks,
+                         hipStream_t s);
 void launch_tables(float *tables, hipStream_t s);
 void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s);
 

@@ -158,7 +158,8 @@ template <bool SPECTRAL> __device__ __forceinline__ Spec clamped(const Params &p
 
 // ------------------------------------------------------------ pass control
 
-__global__ void begin_pass_kernel(Params p) {
+__global__ void begin_pass_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
     PassState *s = p.pass;
     if (threadIdx.x < 16) p.queues[threadIdx.x] = 0u;
     if (threadIdx.x == 0) {
@@ -171,7 +172,8 @@ __global__ void begin_pass_kernel(Params p) {
     }
 }
 
-__global__ void end_pass_kernel(Params p) {
+__global__ void end_pass_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
     PassState *s = p.pass;
     if (threadIdx.x == 0) {
         s->band += 1u;
@@ -193,7 +195,8 @@ __global__ void end_pass_kernel(Params p) {
 
 // --------------------------------------------------------------- camera
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(Params p) {
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
     State st(p);
     const PassState &ps = *p.pass;
     uint32_t count = ps.band_count;
@@ -239,7 +242,8 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(Pa
 
 // ---------------------------------------------------------- light emission
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kernel(Params p) {
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
     State st(p);
     const PassState &ps = *p.pass;
     uint32_t count = ps.light_count;
@@ -365,7 +369,8 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kerne
 // ---------------------------------------------------------- closest hits
 
 template <int W, int S, int G>
-__global__ __launch_bounds__(G) void closest_kernel_rt(Params p, uint32_t current) {
+__global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint32_t current) {
+    const Params &p = plist[blockIdx.y];
     __shared__ uint32_t lds[S * G];
     State st(p);
     const PassState &ps = *p.pass;
@@ -375,9 +380,11 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(Params p, uint32_t curren
         p.queues[Q_SHADOW] = 0u;  // consumed by the last bounce's shadow kernel
         p.pass->closest_rays += count;
     }
-    uint32_t stride_threads = gridDim.x * G;
-    DeviceScene sc = p.scene;
-    sc.spill_stride = stride_threads;
+    // The spill area is laid out for the render's own thread count (every
+    // thread that traverses has an index below its queue length, which
+    // never exceeds the render's slots), not this launch's, which shared
+    // launches size for the largest render.
+    const DeviceScene &sc = p.scene;
     uint32_t nlights = light_count(p);
     FOR_ITEMS(count, at, active) {
         uint32_t path = active ? queue_path(p, queue, at) : 0u;
@@ -831,7 +838,8 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
     return out;
 }
 
-template <bool SPECTRAL> __global__ __launch_bounds__(128) void shade_kernel(Params p, uint32_t current) {
+template <bool SPECTRAL> __global__ __launch_bounds__(128) void shade_kernel(const Params *plist, uint32_t current) {
+    const Params &p = plist[blockIdx.y];
     const PassState &ps = *p.pass;
     uint32_t cameras = p.queues[Q_SURFACE];
     uint32_t lights = p.queues[Q_LIGHT];
@@ -856,7 +864,8 @@ template <bool SPECTRAL> __global__ __launch_bounds__(128) void shade_kernel(Par
 
 // ---------------------------------------------------------------- misses
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(Params p) {
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
     State st(p);
     const PassState &ps = *p.pass;
     uint32_t count = p.queues[Q_MISS];
@@ -908,7 +917,8 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(Para
 
 // ------------------------------------------------------------ shadow rays
 
-template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kernel(Params p) {
+template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
     __shared__ uint32_t lds[S * G];
     State st(p);
     const PassState &ps = *p.pass;
@@ -917,8 +927,7 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
         p.queues[Q_MISS] = 0u;  // consumed by this bounce's miss kernel
         p.pass->shadow_rays += count;
     }
-    DeviceScene sc = p.scene;
-    sc.spill_stride = gridDim.x * G;
+    const DeviceScene &sc = p.scene;  // spill laid out for the render's own threads
     uint32_t nlights = light_count(p);
     bool cutouts = K(p, K_INFO).w >= 0.5f;
     FOR_ITEMS(count, at, active) {
@@ -978,7 +987,8 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
 
 // ------------------------------------------------------------- the film
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(Params p) {
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
     State st(p);
     const PassState &ps = *p.pass;
     uint32_t count = ps.band_count;
@@ -1013,7 +1023,9 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(Pa
     }
 }
 
-__global__ __launch_bounds__(256) void guide_fit_kernel(Params p) {
+__global__ __launch_bounds__(256) void guide_fit_kernel(const Params *plist) {
+    const Params &p = plist[blockIdx.y];
+    if (!p.guide) return;
     for (uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x; cell < GUIDE_CELLS; cell += gridDim.x * blockDim.x) {
         if (guide_keys(p)[cell] == 0u) continue;
         float *s = guide_stats(p);
@@ -1051,7 +1063,8 @@ __device__ __forceinline__ V3 neutral(V3 c) {
 }
 
 // Only the sample's last band converts (integrator.lucb:672-676).
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void film_convert_kernel(Params p, uint32_t always) {
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void film_convert_kernel(const Params *plist, uint32_t always) {
+    const Params &p = plist[blockIdx.y];
     const PassState &ps = *p.pass;
     if (!always && ps.band + 1u != ps.bands) return;
     uint32_t samples = ps.light_samples;
@@ -1123,11 +1136,12 @@ __global__ __launch_bounds__(256) void fill_kernel(uint32_t *dst, size_t words, 
 
 namespace {
 
-template <int W, int S, int G> void launch_closest(const rt::Params &p, uint32_t cur, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((rt::closest_kernel_rt<W, S, G>), dim3(blocks), dim3(G), 0, s, p, cur);
+template <int W, int S, int G>
+void launch_closest(const rt::Params *p, uint32_t n, uint32_t cur, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL((rt::closest_kernel_rt<W, S, G>), dim3(blocks, n), dim3(G), 0, s, p, cur);
 }
-template <int W, int S, int G> void launch_shadow(const rt::Params &p, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((rt::shadow_kernel<W, S, G>), dim3(blocks), dim3(G), 0, s, p);
+template <int W, int S, int G> void launch_shadow(const rt::Params *p, uint32_t n, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL((rt::shadow_kernel<W, S, G>), dim3(blocks, n), dim3(G), 0, s, p);
 }
 
 template <int W, int S, int G> void fill_trace(RenderTraceVariant &v) {
@@ -1160,28 +1174,31 @@ const RenderTraceVariant *render_trace_variants() {
     return table;
 }
 
-template <bool SPECTRAL> static void launch_stage(Stage stage, const rt::Params &p, uint32_t arg, uint32_t blocks,
-                                                  hipStream_t s) {
+template <bool SPECTRAL>
+static void launch_stage(Stage stage, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks, hipStream_t s) {
+    dim3 g(blocks, n);
     switch (stage) {
-        case Stage::Camera: hipLaunchKernelGGL((rt::camera_kernel<SPECTRAL>), dim3(blocks), dim3(256), 0, s, p); break;
-        case Stage::LightEmit: hipLaunchKernelGGL((rt::light_emit_kernel<SPECTRAL>), dim3(blocks), dim3(256), 0, s, p); break;
-        case Stage::Shade: hipLaunchKernelGGL((rt::shade_kernel<SPECTRAL>), dim3(blocks), dim3(128), 0, s, p, arg); break;
-        case Stage::Miss: hipLaunchKernelGGL((rt::miss_kernel<SPECTRAL>), dim3(blocks), dim3(256), 0, s, p); break;
-        case Stage::Finish: hipLaunchKernelGGL((rt::finish_kernel<SPECTRAL>), dim3(blocks), dim3(256), 0, s, p); break;
-        case Stage::Convert: hipLaunchKernelGGL((rt::film_convert_kernel<SPECTRAL>), dim3(blocks), dim3(256), 0, s, p, arg); break;
+        case Stage::Camera: hipLaunchKernelGGL((rt::camera_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
+        case Stage::LightEmit: hipLaunchKernelGGL((rt::light_emit_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
+        case Stage::Shade: hipLaunchKernelGGL((rt::shade_kernel<SPECTRAL>), g, dim3(128), 0, s, p, arg); break;
+        case Stage::Miss: hipLaunchKernelGGL((rt::miss_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
+        case Stage::Finish: hipLaunchKernelGGL((rt::finish_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
+        case Stage::Convert: hipLaunchKernelGGL((rt::film_convert_kernel<SPECTRAL>), g, dim3(256), 0, s, p, arg); break;
         default: break;
     }
 }
 
-void launch_render_stage(Stage stage, bool spectral, const rt::Params &p, uint32_t arg, uint32_t blocks, hipStream_t s) {
+void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks,
+                         hipStream_t s) {
+    if (n == 0) return;
     switch (stage) {
-        case Stage::BeginPass: hipLaunchKernelGGL(rt::begin_pass_kernel, dim3(1), dim3(64), 0, s, p); return;
-        case Stage::EndPass: hipLaunchKernelGGL(rt::end_pass_kernel, dim3(1), dim3(64), 0, s, p); return;
-        case Stage::GuideFit: hipLaunchKernelGGL(rt::guide_fit_kernel, dim3(blocks), dim3(256), 0, s, p); return;
+        case Stage::BeginPass: hipLaunchKernelGGL(rt::begin_pass_kernel, dim3(1, n), dim3(64), 0, s, p); return;
+        case Stage::EndPass: hipLaunchKernelGGL(rt::end_pass_kernel, dim3(1, n), dim3(64), 0, s, p); return;
+        case Stage::GuideFit: hipLaunchKernelGGL(rt::guide_fit_kernel, dim3(blocks, n), dim3(256), 0, s, p); return;
         default: break;
     }
-    if (spectral) launch_stage<true>(stage, p, arg, blocks, s);
-    else launch_stage<false>(stage, p, arg, blocks, s);
+    if (spectral) launch_stage<true>(stage, p, n, arg, blocks, s);
+    else launch_stage<false>(stage, p, n, arg, blocks, s);
 }
 
 void launch_tables(float *tables, hipStream_t s) {

@@ -70,6 +70,16 @@ class RenderStats(ctypes.Structure):
                 ("closest_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64)]
 
 
+class ServerSettings(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_uint64), ("spectral", ctypes.c_uint32), ("trace_width", ctypes.c_uint32),
+                ("trace_group", ctypes.c_uint32), ("trace_stack", ctypes.c_uint32), ("max_active", ctypes.c_uint32)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint32), ("samples_done", ctypes.c_uint32), ("samples", ctypes.c_uint32),
+                ("bytes", ctypes.c_uint64)]
+
+
 _P = ctypes.c_void_p
 _SIGNATURES = {
     "lr_device_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
@@ -97,6 +107,15 @@ _SIGNATURES = {
     "lr_renderer_kernel_times": (ctypes.c_int, [_P, _P]),
     "lr_renderer_destroy": (None, [_P]),
     "lr_renderer_stats": (ctypes.c_int, [_P, ctypes.POINTER(RenderStats)]),
+    "lr_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerSettings), ctypes.POINTER(_P)]),
+    "lr_server_submit": (ctypes.c_int, [_P, _P, ctypes.POINTER(Loaded), ctypes.POINTER(RenderSettingsC), _P,
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "lr_server_step": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "lr_server_status": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]),
+    "lr_server_read": (ctypes.c_int, [_P, ctypes.c_uint64, _P, ctypes.c_size_t]),
+    "lr_server_stats": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(RenderStats)]),
+    "lr_server_release": (ctypes.c_int, [_P, ctypes.c_uint64]),
+    "lr_server_destroy": (None, [_P]),
     "lr_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "lr_buffer_free": (None, [_P, _P]),
     "lr_buffer_write": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),

@@ -147,6 +147,16 @@ def random_triangles(count=10000, seed=1, spread=0.4, sigma=0.03):
     return Mesh(pts, tris, np.arange(count), count)
 
 
+def random_triangles_fast(count, seed=1, spread=0.4, sigma=0.03):
+    """Large random-triangle clouds (hundreds of thousands to millions):
+    the same distribution as random_triangles, drawn with numpy's PCG64."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, (count, 1, 3))
+    pts = (c + sigma * rng.standard_normal((count, 3, 3))).reshape(-1, 3)
+    tris = np.arange(count * 3).reshape(count, 3)
+    return Mesh(pts, tris, np.arange(count), count)
+
+
 def cornell_probe_set(random_count=10000, seed=1):
     gset = _cornell_geometry()
     if random_count:
@@ -313,7 +323,10 @@ def cornell_render_set(random_count=0, light_radiance=17.0):
         "green": lambert(spectrum_to_acescg(band_spectrum(500.0, 570.0))),
     }
     if random_count:
-        gset.meshes.append(random_triangles(random_count, 1))
+        gset.meshes.append(random_triangles(random_count, 1) if random_count <= 20000
+                           else random_triangles_fast(random_count, 1))
+        gset.materials["random"] = lambert((0.5, 0.5, 0.5))
+        gset.meshes[-1].face_materials = ["random"] * random_count
     # The light's local -Z is its emission direction: rows x, y, z with z = +y
     # world so that it shines down.
     m = np.eye(4)
@@ -335,3 +348,11 @@ def furnace_set(albedo=0.8, frequency=16):
     gset.materials = {"ball": lambert((albedo, albedo, albedo))}
     gset.cameras.append(Camera(placement(translation=(0.0, 0.0, 4.0)), "perspective", 50.0, 36.0))
     return gset
+
+
+def serving_sets():
+    """The serving workload's four scenes: the Cornell box with 10,000
+    random triangles (the probe's geometry), the plain Cornell render, and
+    the Cornell render with 500,000 and 5,000,000 random triangles."""
+    return [cornell_render_set(random_count=10000), cornell_render_set(),
+            cornell_render_set(random_count=500000), cornell_render_set(random_count=5000000)]

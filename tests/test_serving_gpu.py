@@ -1,0 +1,121 @@
+"""The serving loop on the GPU: a request renders the same alone and in a
+mix of other requests (scenes, resolutions, sample targets) sharing every
+launch; admission respects the memory budget; cancellation frees slots."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def scenes(device):
+    from luce_render import render, scenes as S
+    out = []
+    for g in (S.cornell_render_set(), S.cornell_render_set(random_count=10000), S.furnace_set()):
+        sc = render.compile_scene(g)
+        out.append((render.GpuScene.upload(device, sc), sc))
+    yield out
+    for g, _ in out:
+        g.destroy()
+
+
+def _serve(device, scenes, jobs, lt, budget=8 << 30):
+    """jobs: [(scene index, w, h, spp)]; returns the images in order."""
+    from luce_render import serving
+    from luce_render.loading import RenderSettings
+    server = serving.Server(device, budget_bytes=budget)
+    try:
+        ids = []
+        for (k, w, h, spp) in jobs:
+            st = RenderSettings(w, h, spp, max_bounces=6, spectral=False, light_tracing=lt,
+                                background=(0.2, 0.2, 0.2) if k == 2 else (0.0, 0.0, 0.0))
+            ids.append(server.submit(scenes[k][0], scenes[k][1], st))
+        steps = 0
+        while server.busy() or steps == 0:
+            server.step()
+            steps += 1
+            assert steps < 1000
+        out = [server.read(i, w, h).copy() for i, (k, w, h, spp) in zip(ids, jobs)]
+        for i in ids:
+            server.release(i)
+        return out, steps
+    finally:
+        server.destroy()
+
+
+@pytest.mark.parametrize("lt", [0.0, 0.5])
+def test_alone_equals_mixed(device, scenes, lt):
+    target = (0, 96, 64, 12)
+    mix = [(1, 128, 128, 5), target, (2, 64, 48, 20), (0, 32, 32, 3), (1, 200, 100, 12)]
+    say(f"serving mix lt {lt}")
+    (alone,), _ = _serve(device, scenes, [target], lt)
+    mixed, steps = _serve(device, scenes, mix, lt)
+    got = mixed[1]
+    assert steps == 20
+    if lt == 0.0:
+        assert np.array_equal(alone, got)  # camera paths: the same bits
+    else:
+        # light paths splat with float atomics, whose order differs
+        rel = np.abs(alone - got).max() / (np.abs(alone).max() + 1e-12)
+        say("max relative difference", rel)
+        assert rel < 1e-4
+
+
+def test_budget_admits_in_order(device, scenes):
+    """A budget that holds one 96x64 request at a time still finishes them all,
+    with the same images."""
+    from luce_render import serving
+    from luce_render.loading import RenderSettings
+    jobs = [(0, 96, 64, 4), (1, 96, 64, 3), (0, 96, 64, 2)]
+    free, _ = _serve(device, scenes, jobs, 0.0)
+    # find one request's footprint, then a budget of one and a half of it
+    server = serving.Server(device, budget_bytes=8 << 30)
+    rid = server.submit(scenes[0][0], scenes[0][1], RenderSettings(96, 64, 1, max_bounces=6, spectral=False,
+                                                                     light_tracing=0.0))
+    need = server.status(rid)["bytes"]
+    server.release(rid)
+    server.destroy()
+    tight, steps = _serve(device, scenes, jobs, 0.0, budget=int(need * 1.5))
+    assert steps == 4 + 3 + 2
+    for a, b in zip(free, tight):
+        assert np.array_equal(a, b)
+
+
+def test_oversized_request_and_cancel(device, scenes):
+    from luce_render import render, serving
+    from luce_render.loading import RenderSettings
+    server = serving.Server(device, budget_bytes=1 << 20)
+    try:
+        with pytest.raises(render.LuceError) as e:
+            server.submit(scenes[0][0], scenes[0][1], RenderSettings(512, 512, 4, spectral=False))
+        assert e.value.code == 401
+    finally:
+        server.destroy()
+    server = serving.Server(device, budget_bytes=8 << 30)
+    try:
+        a = server.submit(scenes[0][0], scenes[0][1], RenderSettings(64, 64, 100, spectral=False, light_tracing=0.0))
+        b = server.submit(scenes[1][0], scenes[1][1], RenderSettings(64, 64, 2, spectral=False, light_tracing=0.0))
+        server.step()
+        server.release(a)  # cancel while rendering
+        while server.busy():
+            server.step()
+        assert server.status(b)["state"] == 2
+        with pytest.raises(render.LuceError):
+            server.status(a)
+    finally:
+        server.destroy()
+
+
+def test_poisson_serving_reports_latency(device, scenes):
+    """A short Poisson stream (40 requests a second) through the loop:
+    every request finishes and the report is consistent."""
+    from luce_render import serving
+    sched = serving.poisson_schedule(count=24, rate=40.0, scenes=3, seed=5, spp_range=(4, 16),
+                                     resolutions=[(64, 64), (128, 96)])
+    rep = serving.run(device, scenes, sched, max_bounces=4, server_kw={"budget_bytes": 8 << 30})
+    say(rep.p50_s, rep.p99_s, rep.rays_per_s, rep.peak_active)
+    assert rep.requests == 24 and rep.p99_s >= rep.p50_s > 0 and rep.rays > 0
