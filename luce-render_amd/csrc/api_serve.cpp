@@ -36,6 +36,7 @@ struct Request {
     size_t footprint = 0;
     std::unique_ptr<RenderJob> job;
     uint32_t samples = 0;
+    rt::PassState final_state{};  // its counters, kept once its state is freed
 };
 
 }  // namespace
@@ -175,6 +176,7 @@ int lr_server_step(lr_server *s, uint32_t *active_out, uint32_t *pending_out) {
                 // Finished: its image stays until read and released; the
                 // rest of its memory goes back to the budget.
                 r->state = 2;
+                (void)hipMemcpy(&r->final_state, r->job->params.pass, sizeof(rt::PassState), hipMemcpyDeviceToHost);
                 s->used -= r->footprint;
                 r->footprint = r->job->keep_image_only();
                 s->used += r->footprint;
@@ -220,10 +222,12 @@ int lr_server_stats(lr_server *s, uint64_t id, lr_render_stats *out) {
     if (!s || !out) return fail(LR_INVALID, "lr_server_stats needs a server and an output");
     auto it = s->requests.find(id);
     if (it == s->requests.end() || !it->second->job) return fail(LR_INVALID, "no such admitted request");
-    rt::PassState ps;
-    hipError_t e = hipMemcpyAsync(&ps, it->second->job->params.pass, sizeof(ps), hipMemcpyDeviceToHost, s->device->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->device->stream);
-    if (e != hipSuccess) return hip_fail(e, "request stats");
+    rt::PassState ps = it->second->final_state;
+    if (it->second->state != 2) {
+        hipError_t e = hipMemcpyAsync(&ps, it->second->job->params.pass, sizeof(ps), hipMemcpyDeviceToHost, s->device->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->device->stream);
+        if (e != hipSuccess) return hip_fail(e, "request stats");
+    }
     out->samples_done = ps.samples_done;
     out->light_samples = ps.light_samples;
     out->closest_rays = ps.closest_rays;
