@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--instances", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--config", default="", help="width,group,stack to skip the sweep")
+    ap.add_argument("--config", default="", help="width,group,stack[,refill] to skip the sweep")
     ap.add_argument("--render-spp", type=int, default=16,
                     help="samples of the full path-traced render of the same scene (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -138,13 +138,13 @@ def main():
     # Traversal variant: the autotuner's sweep over BVH width x workgroup x
     # LDS stack depth on both batches (every variant returns the same hits).
     if args.config:
-        w, g, s_ = (int(x) for x in args.config.split(","))
-        best = (w, g, s_)
+        best = tuple(int(x) for x in args.config.split(","))
+        best = best + (0,) * (4 - len(best))
         sweep = None
     else:
         from luce_render import autotune
         best, sweep = autotune.tune(device, gpu, rays_b, n, hits=hits_b, frame=frame, repeats=2)
-        log("sweep (ms, probe + bounce):", {f"w{k[0]}g{k[1]}s{k[2]}": round(v, 3) for k, v in sweep.items()})
+        log("sweep (ms, probe + bounce):", {f"w{k[0]}g{k[1]}s{k[2]}r{k[3]}": round(v, 3) for k, v in sweep.items()})
     gpu.set_trace_config(*best)
 
     for _ in range(args.warmup):
@@ -187,8 +187,11 @@ def main():
 
     bytes_p, bytes_b = per_ray(cp), per_ray(cb)
     # dominant kernel: the bounce batch (incoherent)
-    dom_bytes, dom_ms, dom_name = (bytes_b * n, ms_b, "closest_kernel") if ms_b >= ms_p else (bytes_p * n, ms_p,
-                                                                                              "probe_kernel")
+    suffix = "_stream_kernel" if best[3] > 0 else "_kernel"
+    if ms_b >= ms_p:
+        dom_bytes, dom_ms, dom_name = bytes_b * n, ms_b, "closest" + suffix
+    else:
+        dom_bytes, dom_ms, dom_name = bytes_p * n, ms_p, "probe" + suffix
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(f"{dom_name}<{best[0]},{best[2]},{best[1]}>")
 
@@ -200,9 +203,9 @@ def main():
         st = RenderSettings.of(WIDTH, HEIGHT, args.render_spp)
         st.spectral = False
         st.max_bounces = 12
-        st.trace = best
+        st.trace = tuple(best[:3])
         renderer = render.Renderer.create(device, sc, st, gpu_scene=gpu)
-        warm = render.Renderer.create(device, sc, RenderSettings(WIDTH, HEIGHT, 1, spectral=False, trace=best),
+        warm = render.Renderer.create(device, sc, RenderSettings(WIDTH, HEIGHT, 1, spectral=False, trace=tuple(best[:3])),
                                       gpu_scene=gpu)
         warm.finish()
         warm.destroy()
@@ -256,8 +259,8 @@ def main():
             "workload": "closest-hit, 1920x1080 primary + bounce-1 batches over the instanced "
                         f"{sc.triangle_count()}-triangle scene (1,000 instances of 64 displaced icospheres)",
             "rays_per_step": total_rays,
-            "trace_config": {"width": best[0], "group": best[1], "stack": best[2]},
-            "autotune_ms": None if sweep is None else {f"{k[0]},{k[1]},{k[2]}": round(v, 3) for k, v in sweep.items()},
+            "trace_config": {"width": best[0], "group": best[1], "stack": best[2], "refill": best[3]},
+            "autotune_ms": None if sweep is None else {",".join(map(str, k)): round(v, 3) for k, v in sweep.items()},
             "primary_mrays_s": round(n / (ms_p * 1e-3) / 1e6, 2),
             "bounce_mrays_s": round(n / (ms_b * 1e-3) / 1e6, 2),
             "bytes_per_ray": {"primary": round(bytes_p, 1), "bounce": round(bytes_b, 1)},

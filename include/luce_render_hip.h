@@ -88,13 +88,19 @@ void lr_scene_destroy(lr_gpu_scene *scene);
 int lr_probe(lr_device *device, const lr_gpu_scene *scene, const float frame[16], uint32_t width,
              uint32_t height, uint32_t *face_out, float *t_out, size_t count);
 
-/* Traversal configuration: BVH width (2, 4, 8), workgroup size (64, 128, 256)
- * and LDS stack entries per lane (24, 32, 48). No reference counterpart: the
- * reference fixes width 2, 64-wide groups and a 32-entry stack (bvh.glsl:49). */
+/* Traversal configuration: BVH width (2, 4, 8), workgroup size (64, 128, 256),
+ * LDS stack entries per lane (24, 32, 48) and the ray schedule of the ray-batch
+ * launches: refill 0 gives each thread one ray; refill k (1..64) runs
+ * persistent waves that take new rays from a shared counter once k of their
+ * lanes are idle (the probe keeps one ray per thread). Every configuration returns the same hits. No reference
+ * counterpart: the reference fixes width 2, 64-wide groups, a 32-entry stack
+ * (bvh.glsl:49) and one ray per invocation. Launches on one scene are ordered
+ * (one stream at a time): they share its spill area and ray counter. */
 typedef struct lr_trace_config {
     uint32_t width;
     uint32_t group;
     uint32_t stack;
+    uint32_t refill;
 } lr_trace_config;
 int lr_scene_set_trace_config(lr_gpu_scene *scene, const lr_trace_config *config);
 int lr_scene_get_trace_config(const lr_gpu_scene *scene, lr_trace_config *config);

@@ -21,10 +21,11 @@ struct lr_gpu_scene {
     void *uvs = nullptr;
     void *spill = nullptr;
     uint64_t spill_entries = 0;
+    uint32_t *work = nullptr;  // the persistent launches' two counters
     float root_lo[3] = {0, 0, 0};
     float root_hi[3] = {0, 0, 0};
     lr_scene_info info{};
-    lr_trace_config config{4, 128, 32};
+    lr_trace_config config{4, 64, 24, 32};
 
     ~lr_gpu_scene();
     int ensure_spill(uint64_t threads, uint32_t stack);

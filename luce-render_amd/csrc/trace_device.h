@@ -37,6 +37,7 @@ struct DeviceScene {
     uint32_t *spill;         // overflow stack entries, [depth - S][thread]
     uint32_t spill_stride;   // threads the launch covers
     uint32_t spill_depth;    // entries per thread the spill area holds
+    uint32_t *work;          // persistent launches: {rays handed out, waves finished}, zero between launches
 };
 
 struct Hit {
@@ -77,31 +78,32 @@ __device__ __forceinline__ float box_entry(float lox, float loy, float loz, floa
     return enter <= leave ? enter : -1.0f;
 }
 
-// triangle_hit (bvh.glsl:27-47) on a repacked triangle.
-__device__ __forceinline__ bool triangle_hit(const float4 *tris, uint32_t slot, float ox, float oy, float oz,
-                                             float dx, float dy, float dz, Hit &hit) {
-    const float4 a = tris[(size_t)slot * 3];
-    const float4 b = tris[(size_t)slot * 3 + 1];
-    const float4 c = tris[(size_t)slot * 3 + 2];
+// triangle_hit (bvh.glsl:27-47) on a repacked triangle: whether it is hit
+// before `best`, and where. Every quantity is
+// computed exactly as the reference computes it; its early exits become one
+// predicate over the same comparisons, so the loads and arithmetic run
+// straight through (no divergent branch, no load held back behind one).
+__device__ __forceinline__ bool triangle_test(float4 a, float4 b, float4 c, float ox, float oy, float oz, float dx,
+                                              float dy, float dz, float best, float &t_out, float &u_out,
+                                              float &v_out) {
     float e1x = a.w, e1y = b.x, e1z = b.y;
     float e2x = b.z, e2y = b.w, e2z = c.x;
     float px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
     float det = dot3(e1x, e1y, e1z, px, py, pz);
-    if (fabsf(det) < 1e-20f) return false;
     float inv = 1.0f / det;
     float sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
     float u = dot3(sx, sy, sz, px, py, pz) * inv;
-    if (u < 0.0f || u > 1.0f) return false;
     float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
     float v = dot3(dx, dy, dz, qx, qy, qz) * inv;
-    if (v < 0.0f || u + v > 1.0f) return false;
     float t = dot3(e2x, e2y, e2z, qx, qy, qz) * inv;
-    if (t <= 0.0f || t >= hit.t) return false;
-    hit.t = t;
-    hit.slot = slot;
-    hit.u = u;
-    hit.v = v;
-    return true;
+    bool reject = fabsf(det) < 1e-20f;
+    reject = reject || u < 0.0f || u > 1.0f;
+    reject = reject || v < 0.0f || u + v > 1.0f;
+    reject = reject || t <= 0.0f || t >= best;
+    t_out = t;
+    u_out = u;
+    v_out = v;
+    return !reject;
 }
 
 template <int W> struct NodeView {
@@ -148,22 +150,46 @@ template <int S> struct LaneStack {
     }
 };
 
-// trace_closest (bvh.glsl:51-90) over width-W nodes.
-template <int W, int S>
-__device__ __forceinline__ Hit trace_closest(const DeviceScene &sc, float ox, float oy, float oz, float dx, float dy,
-                                             float dz, float tmax, uint32_t *lds, int stride, uint32_t thread) {
-    Hit hit{tmax, kNone, 0.0f, 0.0f};
-    float ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
-    if (sc.empty || sc.root_lo[0] > sc.root_hi[0] ||
-        box_entry(sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2], ox, oy, oz,
-                  ix, iy, iz, hit.t) < 0.0f)
-        return hit;
-    LaneStack<S> stack{lds, sc.spill + thread, sc.spill_stride, 0};
-    uint32_t cur = 0;
-    for (;;) {
+// One ray's closest-hit walk (bvh.glsl:51-90) over width-W nodes, advanced one
+// node at a time, so a persistent wave can hand a finished lane a new ray.
+template <int W, int S> struct ClosestWalk {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+    Hit hit;
+    uint32_t cur;
+    LaneStack<S> stack;
+
+    // Sets the ray up; false when it misses the root box (or the tree is
+    // empty), and hit then holds the miss.
+    __device__ __forceinline__ bool begin(const DeviceScene &sc, float ox_, float oy_, float oz_, float dx_, float dy_,
+                                          float dz_, float tmax) {
+        ox = ox_, oy = oy_, oz = oz_, dx = dx_, dy = dy_, dz = dz_;
+        hit = Hit{tmax, kNone, 0.0f, 0.0f};
+        ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+        cur = 0;
+        stack.sp = 0;
+        return !(sc.empty || sc.root_lo[0] > sc.root_hi[0] ||
+                 box_entry(sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2],
+                           ox, oy, oz, ix, iy, iz, hit.t) < 0.0f);
+    }
+
+    // Visits the current node: a leaf's triangles, or an interior node's
+    // children (hit ones nearest first, ties by child index, the others
+    // pushed farthest first). False once the stack is empty.
+    __device__ __forceinline__ bool step(const DeviceScene &sc, int stride) {
         if (cur & kLeafFlag) {
-            uint32_t first = cur & 0x07ffffffu, count = ((cur >> 27) & 0xfu) + 1u;
-            for (uint32_t i = 0; i < count; i++) triangle_hit(sc.tris, first + i, ox, oy, oz, dx, dy, dz, hit);
+            // The leaf's triangles in order; the next one's loads are issued
+            // before the current one is tested.
+            uint32_t first = cur & 0x07ffffffu, last = first + ((cur >> 27) & 0xfu);
+            const float4 *tp = sc.tris + (size_t)first * 3;
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            for (uint32_t slot = first;; slot++) {
+                const float4 *np = sc.tris + (size_t)(slot < last ? slot + 1 : slot) * 3;
+                float4 na = np[0], nb = np[1], nc = np[2];
+                float t, u, v;
+                if (triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, hit.t, t, u, v)) hit = Hit{t, slot, u, v};
+                if (slot == last) break;
+                a = na, b = nb, c = nc;
+            }
         } else {
             NodeView<W> n;
             n.load(sc.nodes, cur);
@@ -175,11 +201,11 @@ __device__ __forceinline__ Hit trace_closest(const DeviceScene &sc, float ox, fl
             for (int c = 0; c < W; c++) {
                 uint32_t ch = n.child(c);
                 uint32_t cnt = n.count(c);
-                float e = -1.0f;
-                if (ch != kNone)
-                    e = box_entry(n.lo_x(c), n.lo_y(c), n.lo_z(c), n.hi_x(c), n.hi_y(c), n.hi_z(c), ox, oy, oz, ix, iy,
-                                  iz, hit.t);
-                bool take = e >= 0.0f;
+                // every slot's planes are tested (an empty slot holds zeros)
+                // and empty slots are masked afterwards
+                float e = box_entry(n.lo_x(c), n.lo_y(c), n.lo_z(c), n.hi_x(c), n.hi_y(c), n.hi_z(c), ox, oy, oz, ix,
+                                    iy, iz, hit.t);
+                bool take = ch != kNone && e >= 0.0f;
                 hits += take ? 1 : 0;
                 dist[c] = take ? e : __builtin_huge_valf();
                 ent[c] = cnt ? leaf_entry(ch, cnt) : ch;
@@ -207,32 +233,52 @@ __device__ __forceinline__ Hit trace_closest(const DeviceScene &sc, float ox, fl
                 for (int i = W - 1; i >= 1; i--)
                     if (i < hits) stack.push(ent[i], stride);
                 cur = ent[0];
-                continue;
+                return true;
             }
         }
-        if (stack.sp == 0) break;
+        if (stack.sp == 0) return false;
         cur = stack.pop(stride);
+        return true;
     }
-    return hit;
-}
+};
 
-// trace_any (bvh.glsl:93-122): whether anything blocks the ray before tmax.
-template <int W, int S>
-__device__ __forceinline__ bool trace_any(const DeviceScene &sc, float ox, float oy, float oz, float dx, float dy,
-                                          float dz, float tmax, uint32_t *lds, int stride, uint32_t thread) {
-    Hit hit{tmax, kNone, 0.0f, 0.0f};
-    float ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
-    if (sc.empty || sc.root_lo[0] > sc.root_hi[0] ||
-        box_entry(sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2], ox, oy, oz,
-                  ix, iy, iz, tmax) < 0.0f)
-        return false;
-    LaneStack<S> stack{lds, sc.spill + thread, sc.spill_stride, 0};
-    uint32_t cur = 0;
-    for (;;) {
+// One ray's any-hit walk (bvh.glsl:93-122), advanced one node at a time.
+template <int W, int S> struct AnyWalk {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz, tmax;
+    uint32_t cur;
+    bool blocked;
+    LaneStack<S> stack;
+
+    __device__ __forceinline__ bool begin(const DeviceScene &sc, float ox_, float oy_, float oz_, float dx_, float dy_,
+                                          float dz_, float tmax_) {
+        ox = ox_, oy = oy_, oz = oz_, dx = dx_, dy = dy_, dz = dz_, tmax = tmax_;
+        ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+        cur = 0;
+        blocked = false;
+        stack.sp = 0;
+        return !(sc.empty || sc.root_lo[0] > sc.root_hi[0] ||
+                 box_entry(sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2],
+                           ox, oy, oz, ix, iy, iz, tmax) < 0.0f);
+    }
+
+    // Children in child order: the first taken is visited, the others pushed
+    // so that they pop in child order. False once blocked or exhausted.
+    __device__ __forceinline__ bool step(const DeviceScene &sc, int stride) {
         if (cur & kLeafFlag) {
-            uint32_t first = cur & 0x07ffffffu, count = ((cur >> 27) & 0xfu) + 1u;
-            for (uint32_t i = 0; i < count; i++)
-                if (triangle_hit(sc.tris, first + i, ox, oy, oz, dx, dy, dz, hit)) return true;
+            uint32_t first = cur & 0x07ffffffu, last = first + ((cur >> 27) & 0xfu);
+            const float4 *tp = sc.tris + (size_t)first * 3;
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            for (uint32_t slot = first;; slot++) {
+                const float4 *np = sc.tris + (size_t)(slot < last ? slot + 1 : slot) * 3;
+                float4 na = np[0], nb = np[1], nc = np[2];
+                float t, u, v;
+                if (triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, tmax, t, u, v)) {
+                    blocked = true;
+                    return false;
+                }
+                if (slot == last) break;
+                a = na, b = nb, c = nc;
+            }
         } else {
             NodeView<W> n;
             n.load(sc.nodes, cur);
@@ -243,14 +289,13 @@ __device__ __forceinline__ bool trace_any(const DeviceScene &sc, float ox, float
             for (int c = 0; c < W; c++) {
                 uint32_t ch = n.child(c);
                 uint32_t cnt = n.count(c);
-                take[c] = ch != kNone && box_entry(n.lo_x(c), n.lo_y(c), n.lo_z(c), n.hi_x(c), n.hi_y(c), n.hi_z(c), ox,
-                                                   oy, oz, ix, iy, iz, tmax) >= 0.0f;
+                float e = box_entry(n.lo_x(c), n.lo_y(c), n.lo_z(c), n.hi_x(c), n.hi_y(c), n.hi_z(c), ox, oy, oz, ix,
+                                    iy, iz, tmax);
+                take[c] = ch != kNone && e >= 0.0f;
                 hits += take[c] ? 1 : 0;
                 ent[c] = cnt ? leaf_entry(ch, cnt) : ch;
             }
             if (hits > 0) {
-                // Children in order: the first taken is visited, the others
-                // pushed so that they pop in child order.
                 uint32_t next = kNone;
 #pragma unroll
                 for (int c = W - 1; c >= 0; c--) {
@@ -259,13 +304,37 @@ __device__ __forceinline__ bool trace_any(const DeviceScene &sc, float ox, float
                     next = ent[c];
                 }
                 cur = next;
-                continue;
+                return true;
             }
         }
-        if (stack.sp == 0) break;
+        if (stack.sp == 0) return false;
         cur = stack.pop(stride);
+        return true;
     }
-    return false;
+};
+
+// trace_closest (bvh.glsl:51-90): the walk run to its end.
+template <int W, int S>
+__device__ __forceinline__ Hit trace_closest(const DeviceScene &sc, float ox, float oy, float oz, float dx, float dy,
+                                             float dz, float tmax, uint32_t *lds, int stride, uint32_t thread) {
+    ClosestWalk<W, S> walk;
+    walk.stack = LaneStack<S>{lds, sc.spill + thread, sc.spill_stride, 0};
+    if (walk.begin(sc, ox, oy, oz, dx, dy, dz, tmax))
+        while (walk.step(sc, stride)) {
+        }
+    return walk.hit;
+}
+
+// trace_any (bvh.glsl:93-122): whether anything blocks the ray before tmax.
+template <int W, int S>
+__device__ __forceinline__ bool trace_any(const DeviceScene &sc, float ox, float oy, float oz, float dx, float dy,
+                                          float dz, float tmax, uint32_t *lds, int stride, uint32_t thread) {
+    AnyWalk<W, S> walk;
+    walk.stack = LaneStack<S>{lds, sc.spill + thread, sc.spill_stride, 0};
+    if (walk.begin(sc, ox, oy, oz, dx, dy, dz, tmax))
+        while (walk.step(sc, stride)) {
+        }
+    return walk.blocked;
 }
 
 }  // namespace lr

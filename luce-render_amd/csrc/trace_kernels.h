@@ -8,10 +8,15 @@ namespace lr {
 struct TraceVariant {
     // Each sets sc.spill_stride to the threads it launches; the caller has
     // sized sc.spill for them (launch_threads below).
+    // refill 0: one ray per thread; 1..64: persistent waves that take new
+    // rays once that many lanes are idle (needs sc.work). The probe always
+    // gives each thread one ray.
     int (*probe)(const DeviceScene &sc, const float *frame, uint32_t width, uint32_t height, uint2 *out,
-                 hipStream_t stream);
-    int (*closest)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint2 *out, hipStream_t stream);
-    int (*any)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint32_t *out, hipStream_t stream);
+                 uint32_t refill, hipStream_t stream);
+    int (*closest)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint2 *out, uint32_t refill,
+                   hipStream_t stream);
+    int (*any)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint32_t *out, uint32_t refill,
+               hipStream_t stream);
 };
 
 // 27 variants: index (width 2/4/8, stack 24/32/48, group 64/128/256).
@@ -20,7 +25,7 @@ int variant_index(uint32_t width, uint32_t stack, uint32_t group);
 
 // Threads a launch over `items` work items covers at group size `group`.
 inline uint64_t launch_threads(uint64_t items, uint32_t group) { return (items + group - 1) / group * group; }
-inline uint64_t probe_items(uint32_t width, uint32_t height) {
+__host__ __device__ inline uint64_t probe_items(uint32_t width, uint32_t height) {
     return (uint64_t)((width + 7) / 8) * ((height + 7) / 8) * 64;
 }
 

@@ -93,10 +93,126 @@ __global__ __launch_bounds__(G) void any_kernel(DeviceScene sc, const float4 *ra
                  : 0u;
 }
 
+// ---------------------------------------------------------------- persistent
+// Persistent launches: every wave takes rays from a shared counter and, once
+// at least `refill` of its lanes have finished their rays, hands those lanes
+// the next rays, so lanes do not idle behind the wave's slowest ray. Each
+// ray's walk is the one-ray-per-thread walk, node for node.
+
+template <int G, class Walk, class Source>
+__device__ __forceinline__ void stream_rays(const DeviceScene &sc, Walk &walk, Source &src, uint32_t n,
+                                            uint32_t refill) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t ray = kNone;
+    bool more = true;
+    for (;;) {
+        uint64_t idle = __ballot(ray == kNone);
+        uint32_t count = (uint32_t)__popcll(idle);
+        if (!more && count == 64u) break;
+        if (more && count >= refill) {
+            uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&sc.work[0], count);
+            base = __shfl(base, (int)leader);
+            more = (uint64_t)base + count < n;
+            if (ray == kNone) {
+                uint32_t r = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                if (r < n && src.begin(sc, walk, r)) ray = r;
+            }
+        }
+        if (ray != kNone && !walk.step(sc, G)) {
+            src.finish(sc, walk, ray);
+            ray = kNone;
+        }
+    }
+    // The last wave out clears the counters for the next launch.
+    if (lane == 0u) {
+        uint32_t waves = gridDim.x * (G / 64u);
+        if (atomicAdd(&sc.work[1], 1u) == waves - 1u) {
+            atomicExch(&sc.work[0], 0u);
+            atomicExch(&sc.work[1], 0u);
+        }
+    }
+}
+
+// Rays in the batch layout: (origin, t_max), (direction, -).
+struct BatchClosest {
+    const float4 *rays;
+    uint2 *out;
+    template <class Walk> __device__ __forceinline__ bool begin(const DeviceScene &sc, Walk &w, uint32_t r) {
+        float4 a = rays[(size_t)r * 2], b = rays[(size_t)r * 2 + 1];
+        if (w.begin(sc, a.x, a.y, a.z, b.x, b.y, b.z, a.w)) return true;
+        out[r] = make_uint2(kNone, __float_as_uint(a.w));
+        return false;
+    }
+    template <class Walk> __device__ __forceinline__ void finish(const DeviceScene &sc, Walk &w, uint32_t r) {
+        uint32_t face = w.hit.slot == kNone ? kNone : sc.faces[w.hit.slot];
+        out[r] = make_uint2(face, __float_as_uint(w.hit.t));
+    }
+};
+
+struct BatchAny {
+    const float4 *rays;
+    uint32_t *out;
+    template <class Walk> __device__ __forceinline__ bool begin(const DeviceScene &sc, Walk &w, uint32_t r) {
+        float4 a = rays[(size_t)r * 2], b = rays[(size_t)r * 2 + 1];
+        if (w.begin(sc, a.x, a.y, a.z, b.x, b.y, b.z, a.w)) return true;
+        out[r] = 0u;
+        return false;
+    }
+    template <class Walk> __device__ __forceinline__ void finish(const DeviceScene &, Walk &w, uint32_t r) {
+        out[r] = w.blocked ? 1u : 0u;
+    }
+};
+
+template <int W, int S, int G>
+__global__ __launch_bounds__(G) void closest_stream_kernel(DeviceScene sc, const float4 *rays, uint32_t n, uint2 *out,
+                                                           uint32_t refill) {
+    __shared__ uint32_t lds[S * G];
+    uint32_t thread = blockIdx.x * G + threadIdx.x;
+    ClosestWalk<W, S> walk;
+    walk.stack = LaneStack<S>{lds + threadIdx.x, sc.spill + thread, sc.spill_stride, 0};
+    BatchClosest src{rays, out};
+    stream_rays<G>(sc, walk, src, n, refill);
+}
+
+template <int W, int S, int G>
+__global__ __launch_bounds__(G) void any_stream_kernel(DeviceScene sc, const float4 *rays, uint32_t n, uint32_t *out,
+                                                       uint32_t refill) {
+    __shared__ uint32_t lds[S * G];
+    uint32_t thread = blockIdx.x * G + threadIdx.x;
+    AnyWalk<W, S> walk;
+    walk.stack = LaneStack<S>{lds + threadIdx.x, sc.spill + thread, sc.spill_stride, 0};
+    BatchAny src{rays, out};
+    stream_rays<G>(sc, walk, src, n, refill);
+}
+
 namespace {
 
+// Workgroups of a persistent launch: as many as are resident at once, and no
+// more than the items need.
+template <class K> uint32_t resident_blocks(K kernel, uint32_t group, uint64_t items) {
+    static int per_cu = -1;
+    static int cus = 0;
+    if (per_cu < 0) {
+        int dev = 0, b = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, (int)group, 0) != hipSuccess || b < 1) b = 1;
+        per_cu = b;
+    }
+    uint64_t need = (items + group - 1) / group;
+    uint64_t resident = (uint64_t)per_cu * (uint64_t)(cus > 0 ? cus : 1);
+    return (uint32_t)(need < resident ? need : resident);
+}
+
 template <int W, int S, int G> int launch_probe_t(const DeviceScene &scene, const float *frame, uint32_t w, uint32_t h,
-                                                   uint2 *out, hipStream_t stream) {
+                                                   uint2 *out, uint32_t refill, hipStream_t stream) {
+    // Primary rays keep one ray per thread whatever `refill` says: their 8x8
+    // tiles are coherent, and a shared counter would scatter them over the
+    // XCDs' separate L2s.
+    (void)refill;
     Frame f;
     for (int k = 0; k < 16; k++) f.v[k] = frame[k];
     uint32_t blocks = (uint32_t)(launch_threads(probe_items(w, h), G) / G);
@@ -108,8 +224,16 @@ template <int W, int S, int G> int launch_probe_t(const DeviceScene &scene, cons
 }
 
 template <int W, int S, int G> int launch_closest_t(const DeviceScene &scene, const float4 *rays, uint32_t n,
-                                                     uint2 *out, hipStream_t stream) {
+                                                     uint2 *out, uint32_t refill, hipStream_t stream) {
     if (n == 0) return 0;
+    if (refill && scene.work && n < 0x80000000u) {
+        uint32_t blocks = resident_blocks(closest_stream_kernel<W, S, G>, G, n);
+        DeviceScene sc = scene;
+        sc.spill_stride = blocks * G;
+        hipLaunchKernelGGL((closest_stream_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, rays, n, out,
+                           refill);
+        return (int)hipGetLastError();
+    }
     uint32_t blocks = (n + G - 1) / G;
     DeviceScene sc = scene;
     sc.spill_stride = blocks * G;
@@ -118,8 +242,15 @@ template <int W, int S, int G> int launch_closest_t(const DeviceScene &scene, co
 }
 
 template <int W, int S, int G> int launch_any_t(const DeviceScene &scene, const float4 *rays, uint32_t n, uint32_t *out,
-                                                 hipStream_t stream) {
+                                                 uint32_t refill, hipStream_t stream) {
     if (n == 0) return 0;
+    if (refill && scene.work && n < 0x80000000u) {
+        uint32_t blocks = resident_blocks(any_stream_kernel<W, S, G>, G, n);
+        DeviceScene sc = scene;
+        sc.spill_stride = blocks * G;
+        hipLaunchKernelGGL((any_stream_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, rays, n, out, refill);
+        return (int)hipGetLastError();
+    }
     uint32_t blocks = (n + G - 1) / G;
     DeviceScene sc = scene;
     sc.spill_stride = blocks * G;

@@ -39,7 +39,8 @@ class SceneArrays(ctypes.Structure):
 
 
 class TraceConfig(ctypes.Structure):
-    _fields_ = [("width", ctypes.c_uint32), ("group", ctypes.c_uint32), ("stack", ctypes.c_uint32)]
+    _fields_ = [("width", ctypes.c_uint32), ("group", ctypes.c_uint32), ("stack", ctypes.c_uint32),
+                ("refill", ctypes.c_uint32)]
 
 
 class SceneInfo(ctypes.Structure):

@@ -1,7 +1,7 @@
 """The traversal autotuner: times every (BVH width, workgroup size, LDS stack
-depth) variant of the closest-hit kernel on a scene and a ray batch and
-picks the fastest. The variants return identical hits (their parity is
-tested), so the choice is by time alone.
+depth, ray schedule) variant of the closest-hit kernel on a scene and a ray
+batch and picks the fastest. The variants return identical hits (their
+parity is tested), so the choice is by time alone.
 
     best, table = autotune.tune(device, gpu_scene, rays, n)
     gpu_scene.set_trace_config(*best)
@@ -13,12 +13,14 @@ from . import render
 WIDTHS = (2, 4, 8)
 GROUPS = (64, 128, 256)
 STACKS = (24, 32, 48)
+REFILLS = (0, 32, 48)  # 0: one ray per thread; k: persistent waves refilled at k idle lanes
 
 
-def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks=STACKS, repeats=3, frame=None):
+def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks=STACKS, refills=REFILLS, repeats=3,
+         frame=None):
     """Times trace_closest_device over `rays` (a Buffer of n rays), and the
     probe of `frame` too when given, for each variant; returns the fastest
-    (width, group, stack) and {variant: ms per launch}."""
+    (width, group, stack, refill) and {variant: ms per launch}."""
     stream = render.device_stream(device)
     own = hits is None
     if own:
@@ -30,20 +32,21 @@ def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks
         for w in widths:
             for g in groups:
                 for s in stacks:
-                    scene.set_trace_config(w, g, s)
+                    for r in refills:
+                        scene.set_trace_config(w, g, s, r)
 
-                    def once():
-                        render.trace_closest_device(scene, rays, n, hits, stream)
-                        if frame is not None:
-                            render.probe_device(scene, frame, probe_hits, stream)
+                        def once():
+                            render.trace_closest_device(scene, rays, n, hits, stream)
+                            if frame is not None:
+                                render.probe_device(scene, frame, probe_hits, stream)
 
-                    once()  # first launch: spill area and code load
-                    render.synchronize(device)
-                    timer.begin(stream)
-                    for _ in range(repeats):
-                        once()
-                    timer.end(stream)
-                    table[(w, g, s)] = timer.elapsed_ms() / repeats
+                        once()  # first launch: spill area and code load
+                        render.synchronize(device)
+                        timer.begin(stream)
+                        for _ in range(repeats):
+                            once()
+                        timer.end(stream)
+                        table[(w, g, s, r)] = timer.elapsed_ms() / repeats
     finally:
         timer.destroy()
         if own:
@@ -57,7 +60,7 @@ def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks
 
 def save(path, best, table, key=""):
     with open(path, "w") as f:
-        json.dump({"key": key, "best": list(best), "table": {f"{w},{g},{s}": ms for (w, g, s), ms in table.items()}},
+        json.dump({"key": key, "best": list(best), "table": {",".join(map(str, k)): ms for k, ms in table.items()}},
                   f, indent=1)
 
 

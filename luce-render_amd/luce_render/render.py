@@ -69,14 +69,17 @@ class GpuScene:
         _lib.check(_lib.lib().lr_scene_upload(device.handle, ctypes.byref(arrays), ctypes.byref(h)))
         return GpuScene(h, device, scene.triangle_count())
 
-    def set_trace_config(self, width=4, group=128, stack=32):
-        cfg = TraceConfig(width, group, stack)
+    def set_trace_config(self, width=4, group=64, stack=24, refill=32):
+        """BVH width, workgroup size, LDS stack entries per lane, and the ray
+        schedule: refill 0 gives each thread one ray, refill k runs persistent
+        waves that take new rays once k lanes are idle."""
+        cfg = TraceConfig(width, group, stack, refill)
         _lib.check(_lib.lib().lr_scene_set_trace_config(self.handle, ctypes.byref(cfg)))
 
     def trace_config(self):
         cfg = TraceConfig()
         _lib.check(_lib.lib().lr_scene_get_trace_config(self.handle, ctypes.byref(cfg)))
-        return (cfg.width, cfg.group, cfg.stack)
+        return (cfg.width, cfg.group, cfg.stack, cfg.refill)
 
     def info(self):
         info = SceneInfo()
@@ -226,6 +229,10 @@ class Buffer:
         _lib.check(_lib.lib().lr_buffer_write(self.device.handle, self.ptr.value + offset, _lib.ptr(array),
                                               array.nbytes))
 
+    def fill(self, byte):
+        """Sets every byte (an upload from the host)."""
+        self.upload(np.full(self.nbytes, byte & 0xFF, np.uint8))
+
     def read(self, dtype, count=None, offset=0):
         dtype = np.dtype(dtype)
         if count is None:
@@ -365,7 +372,8 @@ class Renderer:
         ld.light_paths, ld.through = r.loaded.light_paths, r.loaded.through
         s = r.settings
         cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, s.pool, s.seed,
-                                  1 if s.guiding else 0, 1 if s.profile else 0, 1 if s.use_graph else 0, *s.trace)
+                                  1 if s.guiding else 0, 1 if s.profile else 0, 1 if s.use_graph else 0,
+                                  *tuple(s.trace)[:3])
         r.frame16 = frame.pack()
         h = ctypes.c_void_p()
         try:

@@ -9,6 +9,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 WIDTHS = (2, 4, 8)
+REFILLS = (0, 1, 16, 64)  # 0: one ray per thread; k: persistent waves refilled at k idle lanes
 
 
 def say(*a):
@@ -115,20 +116,49 @@ def test_ray_batches_closest_and_any(device, cornell_probe, probe_scene):
         f_, t_, _ = O.trace_rays(sc.nodes, sc.positions, sc.indices, sc.faces, rays6[sel], tm)
         mf[sel], mt[sel] = f_, t_
     for width in WIDTHS:
-        probe_scene.set_trace_config(width, 128, 32)
-        say(f"ray batch closest/any width {width}")
-        render.trace_closest_device(probe_scene, rb, n, hb)
-        render.trace_any_device(probe_scene, rb, n, ob)
-        render.synchronize(device)
-        h = hb.read(np.uint32).reshape(n, 2)
-        assert np.array_equal(h[:, 0], mf)
-        assert np.array_equal(h[:, 1], mt.view(np.uint32))
-        occluded = ob.read(np.uint32)
-        # any-hit visits every leaf whose box passes at fixed t_max, so it is
-        # occluded exactly when some tested triangle lies before t_max
-        assert np.array_equal(occluded, (mf != 0xFFFFFFFF).astype(np.uint32))
+        for refill in REFILLS:
+            probe_scene.set_trace_config(width, 128, 32, refill)
+            say(f"ray batch closest/any width {width} refill {refill}")
+            hb.fill(0xAB)
+            ob.fill(0xAB)
+            render.trace_closest_device(probe_scene, rb, n, hb)
+            render.trace_any_device(probe_scene, rb, n, ob)
+            render.synchronize(device)
+            h = hb.read(np.uint32).reshape(n, 2)
+            assert np.array_equal(h[:, 0], mf)
+            assert np.array_equal(h[:, 1], mt.view(np.uint32))
+            occluded = ob.read(np.uint32)
+            # any-hit visits every leaf whose box passes at fixed t_max, so it
+            # is occluded exactly when some tested triangle lies before t_max
+            assert np.array_equal(occluded, (mf != 0xFFFFFFFF).astype(np.uint32))
     for b in (rb, hb, ob):
         b.destroy()
+    probe_scene.set_trace_config(4, 128, 32)
+
+
+def test_persistent_launches_back_to_back(device, cornell_probe, probe_scene):
+    """The persistent launches share one ray counter that the last wave of
+    each launch clears: launches in a row, of falling and rising sizes (down
+    to a single ray, fewer than a wave), each cover every ray exactly once."""
+    from luce_render import render
+    sc = cornell_probe
+    o, d = _random_rays(5000, 21)
+    rays = np.zeros((5000, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7] = o, 3.0e38, d
+    mf, mt, _ = O.trace_rays(sc.nodes, sc.positions, sc.indices, sc.faces, np.concatenate([o, d], 1))
+    rb = render.Buffer.create(device, rays)
+    hb = render.Buffer(device, 5000 * 8)
+    probe_scene.set_trace_config(4, 64, 24, 16)
+    for n in (5000, 1, 37, 5000, 64, 4999):
+        say(f"persistent closest batch of {n}")
+        hb.fill(0xAB)
+        render.trace_closest_device(probe_scene, rb, n, hb)
+        render.synchronize(device)
+        h = hb.read(np.uint32).reshape(5000, 2)
+        assert np.array_equal(h[:n, 0], mf[:n]) and np.array_equal(h[:n, 1], mt[:n].view(np.uint32)), n
+        assert (h[n:] == 0xABABABAB).all(), n
+    rb.destroy()
+    hb.destroy()
     probe_scene.set_trace_config(4, 128, 32)
 
 
@@ -137,15 +167,19 @@ def test_empty_and_tiny_scenes(device):
     empty = render.compile_scene(render.GeometrySet())
     g = render.GpuScene.upload(device, empty)
     frame = render.look_frame((0, 0, 3), (0, 0, -1), (0, 1, 0), 0.5, 16, 8)
-    faces, t = render.probe_arrays(device, g, frame)
-    assert (faces == 0xFFFFFFFF).all() and (t == np.float32(3.0e38)).all()
+    for refill in (0, 16):
+        g.set_trace_config(4, 128, 32, refill)
+        faces, t = render.probe_arrays(device, g, frame)
+        assert (faces == 0xFFFFFFFF).all() and (t == np.float32(3.0e38)).all()
     g.destroy()
     one = render.compile_scene(render.GeometrySet(meshes=[render.Mesh([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], [[0, 1, 2]])]))
     g = render.GpuScene.upload(device, one)
-    faces, t = render.probe_arrays(device, g, frame)
     mf, mt, _, _ = _mirror(one, frame)
-    assert np.array_equal(faces, mf) and np.array_equal(t.view(np.uint32), mt.view(np.uint32))
-    assert (faces == 0).sum() > 0
+    for refill in (0, 16):
+        g.set_trace_config(4, 128, 32, refill)
+        faces, t = render.probe_arrays(device, g, frame)
+        assert np.array_equal(faces, mf) and np.array_equal(t.view(np.uint32), mt.view(np.uint32))
+        assert (faces == 0).sum() > 0
     g.destroy()
 
 
