@@ -339,14 +339,55 @@ def cornell_render_set(random_count=0, light_radiance=17.0):
     return gset
 
 
-def furnace_set(albedo=0.8, frequency=16):
-    """A white furnace: an albedo-`albedo` Lambert sphere (radius 1 at the
-    origin) under a constant environment (the render's background), seen from
-    (0, 0, 4)."""
+def furnace_set(albedo=0.8, frequency=16, roughness=0.0):
+    """A white furnace: an albedo-`albedo` diffuse sphere (radius 1 at the
+    origin; EON roughness `roughness`, 0 being Lambert) under a constant
+    environment (the render's background), seen from (0, 0, 4)."""
     pts, tris = icosphere(frequency)
     gset = GeometrySet(meshes=[Mesh(pts, tris, np.arange(len(tris)), len(tris), face_materials=["ball"] * len(tris))])
-    gset.materials = {"ball": lambert((albedo, albedo, albedo))}
+    ball = lambert((albedo, albedo, albedo))
+    ball["base_diffuse_roughness"] = float(roughness)
+    gset.materials = {"ball": ball}
     gset.cameras.append(Camera(placement(translation=(0.0, 0.0, 4.0)), "perspective", 50.0, 36.0))
+    return gset
+
+
+def lamp_room_set():
+    """The guiding workload of tests/gpu/integrator.lucb:362-389: a closed
+    6 x 3 x 6 room (floor y = 0, ceiling y = 3, walls at x, z = +-3) lit only
+    by a 0.2 x 0.2 rect lamp at (0, 0.25, 0) shining up (radiance 200) inside
+    a shade of four 0.3-wide, 1-tall panels (y 0.2..1.2), so everything the
+    camera sees is lit by the small spot the lamp throws on the ceiling. The
+    surfaces carry no material (the render's default one applies). Camera at
+    (2.5, 1.5, 2.5) looking toward the room's centre, focal length 24."""
+    meshes = []
+    h = 3.0
+    # floor, ceiling, then the walls; normals face the room
+    meshes.append(quad_mesh([[-3, 0, 3], [3, 0, 3], [3, 0, -3], [-3, 0, -3]], [0, 1, 0]))
+    meshes.append(quad_mesh([[-3, h, -3], [3, h, -3], [3, h, 3], [-3, h, 3]], [0, -1, 0]))
+    meshes.append(quad_mesh([[3, 0, 3], [3, h, 3], [3, h, -3], [3, 0, -3]], [-1, 0, 0]))
+    meshes.append(quad_mesh([[-3, 0, -3], [-3, h, -3], [-3, h, 3], [-3, 0, 3]], [1, 0, 0]))
+    meshes.append(quad_mesh([[-3, 0, 3], [-3, h, 3], [3, h, 3], [3, 0, 3]], [0, 0, -1]))
+    meshes.append(quad_mesh([[3, 0, -3], [3, h, -3], [-3, h, -3], [-3, 0, -3]], [0, 0, 1]))
+    lo, hi, w = 0.2, 1.2, 0.15
+    meshes.append(quad_mesh([[w, lo, -w], [w, hi, -w], [w, hi, w], [w, lo, w]], [1, 0, 0]))
+    meshes.append(quad_mesh([[-w, lo, w], [-w, hi, w], [-w, hi, -w], [-w, lo, -w]], [-1, 0, 0]))
+    meshes.append(quad_mesh([[-w, lo, w], [w, lo, w], [w, hi, w], [-w, hi, w]], [0, 0, 1]))
+    meshes.append(quad_mesh([[w, lo, -w], [-w, lo, -w], [-w, hi, -w], [w, hi, -w]], [0, 0, -1]))
+    gset = GeometrySet(meshes=meshes)
+    # The light's local -Z is its emission direction: z = -y world, so it
+    # shines up.
+    m = np.eye(4)
+    m[0, :3] = (1.0, 0.0, 0.0)
+    m[1, :3] = (0.0, 0.0, 1.0)
+    m[2, :3] = (0.0, -1.0, 0.0)
+    m[3, :3] = (0.0, 0.25, 0.0)
+    gset.lights.append(Light("rect", m, (1.0, 1.0, 1.0), 200.0, width=0.2, height=0.2))
+    pitch, yaw = -0.15, 0.785
+    cx, sx, cy, sy = math.cos(pitch), math.sin(pitch), math.cos(yaw), math.sin(yaw)
+    rx = np.array([[1.0, 0.0, 0.0], [0.0, cx, sx], [0.0, -sx, cx]])
+    ry = np.array([[cy, 0.0, -sy], [0.0, 1.0, 0.0], [sy, 0.0, cy]])
+    gset.cameras.append(Camera(placement(rx @ ry, (2.5, 1.5, 2.5)), "perspective", 24.0, 36.0))
     return gset
 
 
