@@ -11,6 +11,10 @@ closest-hit rays, each 2,073,600 rays:
            are re-aimed at the volume), the incoherent second bounce.
 Inputs are resident in HBM before timing; HIP events on the device's stream
 bracket the timed steps. One JSON line is printed.
+
+Beside the metric, reported in `config`: the full path tracer on the same
+scene (path_tracing) and the serving loop under Poisson arrivals (serving:
+p50/p99 time to the target sample count, p99 time to 64 samples, rays/s).
 """
 import argparse
 import json
@@ -89,6 +93,8 @@ def main():
     ap.add_argument("--config", default="", help="width,group,stack[,refill] to skip the sweep")
     ap.add_argument("--render-spp", type=int, default=16,
                     help="samples of the full path-traced render of the same scene (0: skip)")
+    ap.add_argument("--serving", type=int, default=40,
+                    help="requests of the Poisson serving leg (40 requests/s over the four serving scenes; 0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the plain copy kernel over this many MiB (a known-traffic dispatch "
                          "for scaling hardware byte counters)")
@@ -227,6 +233,27 @@ def main():
                         "finite": bool(np.all(np.isfinite(img)))}
         log("render:", render_stats)
 
+    # The serving form: Poisson arrivals (40 requests/s) over the four serving
+    # scenes, resolutions 256^2..1920x1080, 16..256 samples, shared launches
+    # under a 200 GB budget; the schedule is a prefix of the 2,000-request one.
+    serving_stats = None
+    if args.serving > 0:
+        from luce_render import serving
+        served = []
+        for g in scenes.serving_sets():
+            c = render.compile_scene(g)
+            served.append((render.GpuScene.upload(device, c), c))
+        schedule = serving.poisson_schedule(count=args.serving, rate=40.0, scenes=len(served))
+        rep = serving.run(device, served, schedule, server_kw={"budget_bytes": 200 << 30}, max_bounces=8,
+                          light_tracing=0.5, log=log)
+        serving_stats = {"requests": rep.requests, "rate_per_s": 40.0, "p50_s": round(rep.p50_s, 3),
+                         "p99_s": round(rep.p99_s, 3), "p99_to_64spp_s": round(rep.p99_to_64spp_s, 3),
+                         "mean_s": round(rep.mean_s, 3), "wall_s": round(rep.wall_s, 2),
+                         "mrays_per_s": round(rep.rays_per_s / 1e6, 1), "peak_active": rep.peak_active}
+        log("serving:", serving_stats)
+        for g, _ in served:
+            g.destroy()
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -266,6 +293,7 @@ def main():
             "bytes_per_ray": {"primary": round(bytes_p, 1), "bounce": round(bytes_b, 1)},
             "bvh_nodes": info["binary_nodes"],
             "path_tracing": render_stats,
+            "serving": serving_stats,
             "wide_bytes": info["wide_bytes"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
