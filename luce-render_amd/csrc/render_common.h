@@ -136,20 +136,62 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
-// Appends `path` to `queue` for every lane with `active` set: one atomic per
-// wave (ballot, then each lane's rank among the active by mbcnt). Every lane
-// of the wave must call it.
-__device__ __forceinline__ void wave_push(const Params &p, uint32_t queue, uint32_t path, bool active) {
-    uint64_t mask = __ballot(active);
-    if (mask == 0) return;
-    uint32_t lane = lane_id();
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&p.queues[queue], (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-    if (active) p.queues[kQueueBase + (size_t)queue * p.pool + base + rank] = path;
-}
+// Queue pushes of one workgroup staged in LDS: each wave claims LDS slots with
+// one LDS atomic, and a flush claims the workgroup's run of every queue with
+// one global atomic per queue and copies the staged ids out in order. Global
+// atomics on one counter from every XCD serialize at the memory side, so one
+// per wave per push was the integrator's largest cost; staging several
+// iterations of a workgroup cuts them by that factor. Q queues of CAP ids.
+template <uint32_t Q, uint32_t CAP> struct StagedPushes {
+    uint32_t ids[Q][CAP];
+    uint32_t count[Q];
+    uint32_t base[Q];
+    uint32_t queue[Q];
+
+    // Every thread of the workgroup calls it before the loop.
+    __device__ __forceinline__ void init(const uint32_t (&queues)[Q]) {
+        if (threadIdx.x < Q) {
+            count[threadIdx.x] = 0u;
+            queue[threadIdx.x] = queues[threadIdx.x];
+        }
+        __syncthreads();
+    }
+
+    // Wave-wide: the active lanes' paths go to staged queue q.
+    __device__ __forceinline__ void push(uint32_t q, uint32_t path, bool active) {
+        uint64_t mask = __ballot(active);
+        if (mask == 0) return;
+        uint32_t lane = lane_id();
+        uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        uint32_t at = 0;
+        if (lane == leader) at = atomicAdd(&count[q], (uint32_t)__popcll(mask));
+        at = __shfl(at, (int)leader);
+        uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        if (active) ids[q][at + rank] = path;
+    }
+
+    // Workgroup-wide, once per loop iteration (block-uniform): writes the
+    // staged ids out when `last` or when another iteration might not fit.
+    __device__ __forceinline__ void sync(const Params &p, bool last) {
+        __syncthreads();
+        bool flush = last;
+        for (uint32_t q = 0; q < Q; q++) flush = flush || count[q] + blockDim.x > CAP;
+        if (!flush) return;
+        if (threadIdx.x < Q) {
+            uint32_t c = count[threadIdx.x];
+            base[threadIdx.x] = c ? atomicAdd(&p.queues[queue[threadIdx.x]], c) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t q = 0; q < Q; q++) {
+            uint32_t c = count[q];
+            uint32_t *dst = p.queues + kQueueBase + (size_t)queue[q] * p.pool + base[q];
+            for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) dst[i] = ids[q][i];
+        }
+        __syncthreads();
+        if (threadIdx.x < Q) count[threadIdx.x] = 0u;
+        __syncthreads();
+    }
+};
 
 __device__ __forceinline__ uint32_t queue_path(const Params &p, uint32_t queue, uint32_t at) {
     return p.queues[kQueueBase + (size_t)queue * p.pool + at];

@@ -17,6 +17,14 @@ uint32_t grid_blocks(uint64_t items, uint32_t group) {
     return (uint32_t)b;
 }
 
+uint32_t stage_blocks(uint64_t items, uint32_t group, int compute_units) {
+    // About 32 resident waves a compute unit; the integrator's loops are
+    // persistent, so more workgroups only queue behind these.
+    uint64_t cap = (uint64_t)(compute_units > 0 ? compute_units : 256) * 32u / (group / 64u);
+    uint32_t b = grid_blocks(items, group);
+    return b < cap ? b : (uint32_t)cap;
+}
+
 namespace {
 
 struct Shape {
@@ -169,6 +177,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     p.scene.spill = depth ? (uint32_t *)alloc(depth * threads * 4, rc) : nullptr;
     p.scene.spill_stride = (uint32_t)threads;
     p.scene.spill_depth = (uint32_t)depth;
+    p.scene.work = nullptr;  // the render's tracing launches give each thread one ray
     if (rc) return rc;
     launch_fill((uint32_t *)p.film, (size_t)4 * sh.pixels * 4, 0u, s);
     launch_fill((uint32_t *)p.image, (size_t)sh.pixels * 4, 0u, s);
@@ -209,10 +218,11 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     };
     launch_render_stage(Stage::BeginPass, sp, params, n, 0, 1, s);
     mark(KRest);
-    launch_render_stage(Stage::Camera, sp, params, n, 0, grid_blocks(band, 256), s);
+    int cus = jobs[0]->device->compute_units;
+    launch_render_stage(Stage::Camera, sp, params, n, 0, stage_blocks(band, 256, cus), s);
     mark(KCamera);
     if (lights) {
-        launch_render_stage(Stage::LightEmit, sp, params, n, 0, grid_blocks(lights, 256), s);
+        launch_render_stage(Stage::LightEmit, sp, params, n, 0, stage_blocks(lights, 256, cus), s);
         mark(KLight);
     }
     // One round more than bounces: the last BSDF rays still find lights
@@ -220,8 +230,10 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     for (uint32_t b = 0; b < rounds; b++) {
         uint32_t cur = b % 2u;
         tv.closest(params, n, cur, grid_blocks(slots, tv.group), s);
+        launch_render_stage(Stage::Route, sp, params, n, cur, stage_blocks(slots, 256, cus), s);
         mark(KClosest);
-        launch_render_stage(Stage::Shade, sp, params, n, cur, grid_blocks(slots, 128), s);
+        launch_render_stage(Stage::Shade, sp, params, n, cur, stage_blocks(band, 128, cus), s);
+        if (lights) launch_render_stage(Stage::ShadeLight, sp, params, n, cur, stage_blocks(lights, 128, cus), s);
         mark(KSurface);
         launch_render_stage(Stage::Miss, sp, params, n, 0, grid_blocks(slots, 256), s);
         mark(KMiss);

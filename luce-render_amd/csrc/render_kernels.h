@@ -7,7 +7,7 @@
 
 namespace lr {
 
-enum class Stage { BeginPass, Camera, LightEmit, Shade, Miss, Finish, GuideFit, Convert, EndPass };
+enum class Stage { BeginPass, Camera, LightEmit, Route, Shade, ShadeLight, Miss, Finish, GuideFit, Convert, EndPass };
 
 // The tracing kernels of a pass for one (width, stack, group) variant.
 struct RenderTraceVariant {
@@ -19,8 +19,8 @@ struct RenderTraceVariant {
 const RenderTraceVariant *render_trace_variants();
 
 // `p`: a device array of `n` requests' parameters, one per grid row.
-// `arg`: the closest-hit queue being read (Shade) or "convert every pass"
-// (Convert).
+// `arg`: the closest-hit queue being read (Route, Shade, ShadeLight) or
+// "convert every pass" (Convert).
 void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks,
                          hipStream_t s);
 void launch_tables(float *tables, hipStream_t s);

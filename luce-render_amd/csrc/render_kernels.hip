@@ -193,6 +193,16 @@ __global__ void end_pass_kernel(const Params *plist) {
         for (uint32_t i = _base + (threadIdx.x & 63u), _once = 1; _once; _once = 0)                         \
             for (bool active = i < (count); _once; _once = 0)
 
+// The same with workgroup-uniform trip counts, for loops whose body ends in a
+// workgroup-wide StagedPushes::sync.
+#define FOR_BLOCK_ITEMS(count, i, active)                                                                   \
+    for (uint32_t _base = blockIdx.x * blockDim.x; _base < (count); _base += gridDim.x * blockDim.x)        \
+        for (uint32_t i = _base + threadIdx.x, _once = 1; _once; _once = 0)                                 \
+            for (bool active = i < (count); _once; _once = 0)
+
+// Staged ids per queue: up to eight iterations of a workgroup between flushes.
+constexpr uint32_t kStageIterations = 8u;
+
 // --------------------------------------------------------------- camera
 
 template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(const Params *plist) {
@@ -200,7 +210,9 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(co
     State st(p);
     const PassState &ps = *p.pass;
     uint32_t count = ps.band_count;
-    FOR_ITEMS(count, path, active) {
+    __shared__ StagedPushes<1, 256 * kStageIterations> staged;
+    staged.init({Q_CLOSEST});
+    FOR_BLOCK_ITEMS(count, path, active) {
         bool push = false;
         if (active) {
             uint32_t pixel = ps.band_start + path;
@@ -236,8 +248,10 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(co
                 push = true;
             }
         }
-        wave_push(p, Q_CLOSEST, path, push);
+        staged.push(0, path, push);
+        staged.sync(p, false);
     }
+    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------- light emission
@@ -247,7 +261,9 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kerne
     State st(p);
     const PassState &ps = *p.pass;
     uint32_t count = ps.light_count;
-    FOR_ITEMS(count, i, active) {
+    __shared__ StagedPushes<1, 256 * kStageIterations> staged;
+    staged.init({Q_CLOSEST});
+    FOR_BLOCK_ITEMS(count, i, active) {
         bool push = false;
         uint32_t path = ps.band_count + i;
         if (active) {
@@ -362,18 +378,26 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kerne
                 }
             }
         }
-        wave_push(p, Q_CLOSEST, path, push);
+        staged.push(0, path, push);
+        staged.sync(p, false);
     }
+    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------- closest hits
 
+// The closest hits of one bounce's queue (intersect_closest.comp:45-87): the
+// BVH walk, then the analytic lights that the ray meets first. Results go to
+// SV_HIT / SU_HIT (kNoneU: a miss); route_kernel then sorts the paths into
+// the miss, surface and light queues, so this kernel needs no LDS beyond the
+// traversal stack and keeps its occupancy. One ray per thread: every
+// persistent schedule measured slower here, its refills stalling on the
+// queue-then-state loads.
 template <int W, int S, int G>
 __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint32_t current) {
     const Params &p = plist[blockIdx.y];
     __shared__ uint32_t lds[S * G];
     State st(p);
-    const PassState &ps = *p.pass;
     uint32_t queue = Q_CLOSEST + current;
     uint32_t count = p.queues[queue];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -386,38 +410,50 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint
     // launches size for the largest render.
     const DeviceScene &sc = p.scene;
     uint32_t nlights = light_count(p);
+    uint32_t cameras = p.pass->band_count;
     FOR_ITEMS(count, at, active) {
-        uint32_t path = active ? queue_path(p, queue, at) : 0u;
-        bool to_miss = false, to_surface = false, to_light = false;
-        if (active) {
-            float4 o4 = st.v(SV_ORIGIN, path), d4 = st.v(SV_DIRECTION, path);
-            V3 o = v3(o4), d = v3(d4);
-            Hit h = trace_closest<W, S>(sc, o.x, o.y, o.z, d.x, d.y, d.z, kInf, lds + threadIdx.x, G,
-                                        blockIdx.x * G + threadIdx.x);
-            uint32_t what = h.slot;
-            bool light_path = path >= ps.band_count;
-            bool camera_ray = bounce_total(st.u(SU_BOUNCE, path)) == 0u;
-            for (uint32_t l = 0; l < nlights; l++) {
-                if (camera_ray && !light_visible(p, l)) continue;
-                float t = intersect_light(p, l, o, d, h.t, light_path);
-                if (t < h.t) {
-                    h.t = t;
-                    what = kLightHit | l;
-                }
-            }
-            if (what == kNoneU) {
-                to_miss = true;
-            } else {
-                st.set_v(SV_HIT, path, make_float4(h.t, 0.0f, h.u, h.v));
-                st.set_u(SU_HIT, path, what);
-                to_light = light_path;
-                to_surface = !light_path;
+        if (!active) continue;
+        uint32_t path = queue_path(p, queue, at);
+        float4 o4 = st.v(SV_ORIGIN, path), d4 = st.v(SV_DIRECTION, path);
+        V3 o = v3(o4), d = v3(d4);
+        Hit h = trace_closest<W, S>(sc, o.x, o.y, o.z, d.x, d.y, d.z, kInf, lds + threadIdx.x, G,
+                                    blockIdx.x * G + threadIdx.x);
+        uint32_t what = h.slot;
+        bool light_path = path >= cameras;
+        bool camera_ray = bounce_total(st.u(SU_BOUNCE, path)) == 0u;
+        for (uint32_t l = 0; l < nlights; l++) {
+            if (camera_ray && !light_visible(p, l)) continue;
+            float t = intersect_light(p, l, o, d, h.t, light_path);
+            if (t < h.t) {
+                h.t = t;
+                what = kLightHit | l;
             }
         }
-        wave_push(p, Q_MISS, path, to_miss);
-        wave_push(p, Q_SURFACE, path, to_surface);
-        wave_push(p, Q_LIGHT, path, to_light);
+        if (what != kNoneU) st.set_v(SV_HIT, path, make_float4(h.t, 0.0f, h.u, h.v));
+        st.set_u(SU_HIT, path, what);
     }
+}
+
+// The paths of this bounce's closest queue to the miss, surface (camera
+// paths) and light (light paths) queues, by their hits.
+__global__ __launch_bounds__(256) void route_kernel(const Params *plist, uint32_t current) {
+    const Params &p = plist[blockIdx.y];
+    State st(p);
+    uint32_t queue = Q_CLOSEST + current;
+    uint32_t count = p.queues[queue];
+    uint32_t cameras = p.pass->band_count;
+    __shared__ StagedPushes<3, 256 * kStageIterations> staged;
+    staged.init({Q_MISS, Q_SURFACE, Q_LIGHT});
+    FOR_BLOCK_ITEMS(count, at, active) {
+        uint32_t path = active ? queue_path(p, queue, at) : 0u;
+        uint32_t what = active ? st.u(SU_HIT, path) : kNoneU;
+        bool light_path = path >= cameras;
+        staged.push(0, path, active && what == kNoneU);
+        staged.push(1, path, active && what != kNoneU && !light_path);
+        staged.push(2, path, active && what != kNoneU && light_path);
+        staged.sync(p, false);
+    }
+    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------- surface shading
@@ -838,28 +874,31 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
     return out;
 }
 
-template <bool SPECTRAL> __global__ __launch_bounds__(128) void shade_kernel(const Params *plist, uint32_t current) {
+// Camera paths' and light paths' hits are shaded by separate launches: each
+// needs fewer registers than the two together, so more waves fit.
+template <bool SPECTRAL, bool LIGHT>
+__global__ __launch_bounds__(128) void shade_kernel(const Params *plist, uint32_t current) {
     const Params &p = plist[blockIdx.y];
     const PassState &ps = *p.pass;
-    uint32_t cameras = p.queues[Q_SURFACE];
-    uint32_t lights = p.queues[Q_LIGHT];
-    uint32_t count = cameras + lights;
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.queues[Q_CLOSEST + current] = 0u;  // consumed by this bounce's closest
-    FOR_ITEMS(count, at, active) {
+    const uint32_t queue = LIGHT ? Q_LIGHT : Q_SURFACE;
+    uint32_t count = p.queues[queue];
+    // consumed by this bounce's closest-hit and routing kernels
+    if (!LIGHT && blockIdx.x == 0 && threadIdx.x == 0) p.queues[Q_CLOSEST + current] = 0u;
+    __shared__ StagedPushes<2, 128 * kStageIterations> staged;
+    staged.init({Q_CLOSEST + 1u - current, Q_SHADOW});
+    FOR_BLOCK_ITEMS(count, at, active) {
         Pushes pu;
         uint32_t path = 0;
         if (active) {
-            if (at < cameras) {
-                path = queue_path(p, Q_SURFACE, at);
-                pu = shade_camera<SPECTRAL>(p, ps, path);
-            } else {
-                path = queue_path(p, Q_LIGHT, at - cameras);
-                pu = shade_light<SPECTRAL>(p, ps, path);
-            }
+            path = queue_path(p, queue, at);
+            if constexpr (LIGHT) pu = shade_light<SPECTRAL>(p, ps, path);
+            else pu = shade_camera<SPECTRAL>(p, ps, path);
         }
-        wave_push(p, Q_CLOSEST + 1u - current, path, pu.closest);
-        wave_push(p, Q_SHADOW, path, pu.shadow);
+        staged.push(0, path, pu.closest);
+        staged.push(1, path, pu.shadow);
+        staged.sync(p, false);
     }
+    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------------- misses
@@ -917,6 +956,10 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(cons
 
 // ------------------------------------------------------------ shadow rays
 
+// Shadow rays (intersect_shadow.comp:51-92): any hit to the light; with
+// cut-outs present, a blocked ray walks through the surfaces it meets (at most
+// 16), keeping 1 - opacity of each; an unblocked ray delivers its radiance to
+// the camera path or, for a light path, splats it onto the film.
 template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist) {
     const Params &p = plist[blockIdx.y];
     __shared__ uint32_t lds[S * G];
@@ -950,9 +993,9 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
         if (blocked) {
             V3 from = v3(o);
             float remaining = o.w;
-            deliver = false;
             for (uint32_t hop = 0; hop < 16u; hop++) {
-                Hit h = trace_closest<W, S>(sc, from.x, from.y, from.z, d.x, d.y, d.z, remaining, lds + threadIdx.x, G, thread);
+                Hit h = trace_closest<W, S>(sc, from.x, from.y, from.z, d.x, d.y, d.z, remaining, lds + threadIdx.x, G,
+                                            thread);
                 if (h.slot == kNone) {
                     deliver = true;
                     break;
@@ -1180,7 +1223,9 @@ static void launch_stage(Stage stage, const rt::Params *p, uint32_t n, uint32_t 
     switch (stage) {
         case Stage::Camera: hipLaunchKernelGGL((rt::camera_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
         case Stage::LightEmit: hipLaunchKernelGGL((rt::light_emit_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
-        case Stage::Shade: hipLaunchKernelGGL((rt::shade_kernel<SPECTRAL>), g, dim3(128), 0, s, p, arg); break;
+        case Stage::Shade: hipLaunchKernelGGL((rt::shade_kernel<SPECTRAL, false>), g, dim3(128), 0, s, p, arg); break;
+        case Stage::ShadeLight: hipLaunchKernelGGL((rt::shade_kernel<SPECTRAL, true>), g, dim3(128), 0, s, p, arg); break;
+        case Stage::Route: hipLaunchKernelGGL(rt::route_kernel, g, dim3(256), 0, s, p, arg); break;
         case Stage::Miss: hipLaunchKernelGGL((rt::miss_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
         case Stage::Finish: hipLaunchKernelGGL((rt::finish_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
         case Stage::Convert: hipLaunchKernelGGL((rt::film_convert_kernel<SPECTRAL>), g, dim3(256), 0, s, p, arg); break;

@@ -337,4 +337,57 @@ __device__ __forceinline__ bool trace_any(const DeviceScene &sc, float ox, float
     return walk.blocked;
 }
 
+// Persistent launches: every wave takes rays from a shared counter and, once
+// at least `refill` of its lanes have finished their rays, hands those lanes
+// the next rays, so lanes do not idle behind the wave's slowest ray. Each
+// ray's walk is the one-ray-per-thread walk, node for node. The source's
+// begin(sc, walk, r) sets ray r up (false: it is already done and stored) and
+// finish(sc, walk, r) stores its result. Finishing waits for the refill, so
+// the lanes whose rays ended finish together rather than one or two at a
+// time. sc.work holds the counter pair, zero between launches: the last wave
+// of a launch (gridDim.x workgroups of G threads) clears it.
+template <int G, class Walk, class Source>
+__device__ __forceinline__ void stream_rays(const DeviceScene &sc, Walk &walk, Source &src, uint32_t n,
+                                            uint32_t refill) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t ray = kNone;
+    bool walking = false;
+    bool more = true;
+    for (;;) {
+        uint64_t idle = __ballot(!walking);
+        uint32_t count = (uint32_t)__popcll(idle);
+        if (count >= refill || (!more && count == 64u)) {
+            if (ray != kNone && !walking) {
+                src.finish(sc, walk, ray);
+                ray = kNone;
+            }
+            if (!more && count == 64u) break;
+            if (more) {
+                uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&sc.work[0], count);
+                base = __shfl(base, (int)leader);
+                more = (uint64_t)base + count < n;
+                if (!walking) {
+                    uint32_t r = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                    if (r < n && src.begin(sc, walk, r)) {
+                        ray = r;
+                        walking = true;
+                    }
+                }
+            }
+        }
+        if (walking) walking = walk.step(sc, G);
+    }
+    // The last wave out clears the counters for the next launch.
+    if (lane == 0u) {
+        uint32_t waves = gridDim.x * (G / 64u);
+        if (atomicAdd(&sc.work[1], 1u) == waves - 1u) {
+            atomicExch(&sc.work[0], 0u);
+            atomicExch(&sc.work[1], 0u);
+        }
+    }
+}
+
 }  // namespace lr

@@ -94,47 +94,6 @@ __global__ __launch_bounds__(G) void any_kernel(DeviceScene sc, const float4 *ra
 }
 
 // ---------------------------------------------------------------- persistent
-// Persistent launches: every wave takes rays from a shared counter and, once
-// at least `refill` of its lanes have finished their rays, hands those lanes
-// the next rays, so lanes do not idle behind the wave's slowest ray. Each
-// ray's walk is the one-ray-per-thread walk, node for node.
-
-template <int G, class Walk, class Source>
-__device__ __forceinline__ void stream_rays(const DeviceScene &sc, Walk &walk, Source &src, uint32_t n,
-                                            uint32_t refill) {
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t ray = kNone;
-    bool more = true;
-    for (;;) {
-        uint64_t idle = __ballot(ray == kNone);
-        uint32_t count = (uint32_t)__popcll(idle);
-        if (!more && count == 64u) break;
-        if (more && count >= refill) {
-            uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&sc.work[0], count);
-            base = __shfl(base, (int)leader);
-            more = (uint64_t)base + count < n;
-            if (ray == kNone) {
-                uint32_t r = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                if (r < n && src.begin(sc, walk, r)) ray = r;
-            }
-        }
-        if (ray != kNone && !walk.step(sc, G)) {
-            src.finish(sc, walk, ray);
-            ray = kNone;
-        }
-    }
-    // The last wave out clears the counters for the next launch.
-    if (lane == 0u) {
-        uint32_t waves = gridDim.x * (G / 64u);
-        if (atomicAdd(&sc.work[1], 1u) == waves - 1u) {
-            atomicExch(&sc.work[0], 0u);
-            atomicExch(&sc.work[1], 0u);
-        }
-    }
-}
 
 // Rays in the batch layout: (origin, t_max), (direction, -).
 struct BatchClosest {
