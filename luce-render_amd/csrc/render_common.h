@@ -105,6 +105,7 @@ struct Params {
     float4 *image;
     float4 *display;
     uint8_t *guide;        // guiding grid (keys, lobes, stats); null without guiding
+    const float4 *light_bounds;  // bounding spheres: one per group of kLightGroup lights, then one per light
     PassState *pass;
     DeviceScene scene;
 };

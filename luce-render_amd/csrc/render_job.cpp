@@ -77,6 +77,8 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     Shape s = shape_of(loaded, *st);
     size_t b = 0;
     b += loaded->light_floats * 4 + 512 + loaded->material_floats * 4 + loaded->emitter_words * 4;
+    uint32_t nlights = (uint32_t)loaded->constants[0];
+    b += ((size_t)(nlights + 7u) / 8u + nlights) * 16;
     b += (size_t)rt::kStateV * s.slots * 16 + (size_t)rt::kStateU * s.slots * 4;
     b += ((size_t)rt::kQueueBase + (size_t)rt::kQueues * s.slots) * 4;
     b += (size_t)6 * s.pixels * 16;  // film (4 planes), image, display
@@ -147,6 +149,14 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     p.materials = (const float4 *)upload(loaded->materials, loaded->material_floats * 4);
     p.emitters = (const uint32_t *)upload(loaded->emitters, loaded->emitter_words * 4);
     if (rc) return rc;
+    // Bounding spheres of the analytic lights (their count is the first
+    // constant), for the culled light loops.
+    uint32_t nlights = (uint32_t)loaded->constants[0];
+    if ((size_t)nlights * 28u > loaded->light_floats) return fail(LR_INVALID, "the light count exceeds the light records");
+    size_t bound_count = (nlights + 7u) / 8u + nlights;
+    p.light_bounds = (const float4 *)alloc(bound_count * 16, rc);
+    if (rc) return rc;
+    launch_light_bounds(p.lights, nlights, (float4 *)p.light_bounds, s);
     float *tables = shared_tables;
     if (!tables) {
         tables = (float *)alloc((size_t)rt::TABLE_SIZE * 4, rc);

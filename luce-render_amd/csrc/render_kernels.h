@@ -25,5 +25,7 @@ void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32
                          hipStream_t s);
 void launch_tables(float *tables, hipStream_t s);
 void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s);
+// Bounding spheres of `n` analytic lights for culling: ceil(n / 8) + n float4.
+void launch_light_bounds(const float4 *lights, uint32_t n, float4 *out, hipStream_t s);
 
 }  // namespace lr

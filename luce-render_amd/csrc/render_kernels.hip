@@ -408,7 +408,7 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint
     // thread that traverses has an index below its queue length, which
     // never exceeds the render's slots), not this launch's, which shared
     // launches size for the largest render.
-    const DeviceScene &sc = p.scene;
+    const DeviceScene sc = p.scene;  // a copy: registers, not reloads after every store
     uint32_t nlights = light_count(p);
     uint32_t cameras = p.pass->band_count;
     FOR_ITEMS(count, at, active) {
@@ -421,14 +421,7 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint
         uint32_t what = h.slot;
         bool light_path = path >= cameras;
         bool camera_ray = bounce_total(st.u(SU_BOUNCE, path)) == 0u;
-        for (uint32_t l = 0; l < nlights; l++) {
-            if (camera_ray && !light_visible(p, l)) continue;
-            float t = intersect_light(p, l, o, d, h.t, light_path);
-            if (t < h.t) {
-                h.t = t;
-                what = kLightHit | l;
-            }
-        }
+        if (nlights) nearest_light(p, o, d, camera_ray, light_path, h.t, what);
         if (what != kNoneU) st.set_v(SV_HIT, path, make_float4(h.t, 0.0f, h.u, h.v));
         st.set_u(SU_HIT, path, what);
     }
@@ -970,7 +963,7 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
         p.queues[Q_MISS] = 0u;  // consumed by this bounce's miss kernel
         p.pass->shadow_rays += count;
     }
-    const DeviceScene &sc = p.scene;  // spill laid out for the render's own threads
+    const DeviceScene sc = p.scene;  // spill laid out for the render's own threads; a register copy
     uint32_t nlights = light_count(p);
     bool cutouts = K(p, K_INFO).w >= 0.5f;
     FOR_ITEMS(count, at, active) {
@@ -979,12 +972,7 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
         float4 o = st.v(SV_SHADOW_ORIGIN, path);
         V3 d = v3(st.v(SV_SHADOW_DIRECTION, path));
         bool light_path = path >= ps.band_count;
-        if (light_path) {
-            bool seen = false;
-            for (uint32_t l = 0; l < nlights && !seen; l++)
-                seen = light_visible(p, l) && intersect_light(p, l, v3(o), d, o.w, true) < o.w;
-            if (seen) continue;
-        }
+        if (light_path && nlights && light_in_the_way(p, v3(o), d, o.w)) continue;
         uint32_t thread = blockIdx.x * G + threadIdx.x;
         bool blocked = trace_any<W, S>(sc, o.x, o.y, o.z, d.x, d.y, d.z, o.w, lds + threadIdx.x, G, thread);
         if (blocked && !cutouts) continue;
@@ -1168,6 +1156,46 @@ __global__ __launch_bounds__(64) void tables_kernel(float *t, uint32_t phase) {
 }
 
 // A fill kernel (graph-safe in place of memset nodes).
+// Bounding spheres of the analytic lights (see nearest_light): group g's
+// sphere at out[g], light l's at out[groups + l]. One thread per group.
+__global__ __launch_bounds__(64) void light_bounds_kernel(const float4 *lights, uint32_t n, float4 *out) {
+    uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= groups) return;
+    uint32_t first = g * kLightGroup, end = min(n, first + kLightGroup);
+    float4 sph[kLightGroup];
+    V3 centre{0, 0, 0};
+    uint32_t hittable = 0;
+    for (uint32_t l = first; l < end; l++) {
+        const float4 *r = lights + (size_t)l * 7u;
+        uint32_t type = (uint32_t)r[0].w;
+        float radius = -1.0f;
+        if (type == LIGHT_SPHERE) radius = r[1].w;
+        else if (type < LIGHT_DISTANT) radius = length(v3(r[1])) + length(v3(r[2]));
+        V3 c = v3(r[0]);
+        if (radius >= 0.0f) {
+            float scale = fmaxf(fabsf(c.x), fmaxf(fabsf(c.y), fabsf(c.z)));
+            radius = radius * 1.001f + 1e-5f * scale + 1e-30f;
+            centre = centre + c;
+            hittable++;
+        }
+        sph[l - first] = f4(c, radius);
+        out[groups + l] = sph[l - first];
+    }
+    float group_radius = -1.0f;
+    if (hittable) {
+        centre = centre * (1.0f / (float)hittable);
+        group_radius = 0.0f;
+        for (uint32_t l = first; l < end; l++) {
+            float4 s = sph[l - first];
+            if (s.w >= 0.0f) group_radius = fmaxf(group_radius, length(v3(s) - centre) + s.w);
+        }
+        float scale = fmaxf(fabsf(centre.x), fmaxf(fabsf(centre.y), fabsf(centre.z)));
+        group_radius = group_radius * 1.001f + 1e-5f * scale + 1e-30f;
+    }
+    out[g] = f4(centre, group_radius);
+}
+
 __global__ __launch_bounds__(256) void fill_kernel(uint32_t *dst, size_t words, uint32_t value) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = value;
@@ -1250,6 +1278,12 @@ void launch_tables(float *tables, hipStream_t s) {
     uint32_t cells = rt::TABLE_MU * rt::TABLE_ROUGHNESS * (1u + rt::TABLE_ETA);
     hipLaunchKernelGGL(rt::tables_kernel, dim3((cells + 63) / 64), dim3(64), 0, s, tables, 0u);
     hipLaunchKernelGGL(rt::tables_kernel, dim3(1), dim3(64), 0, s, tables, 1u);
+}
+
+void launch_light_bounds(const float4 *lights, uint32_t n, float4 *out, hipStream_t s) {
+    uint32_t groups = (n + rt::kLightGroup - 1u) / rt::kLightGroup;
+    if (groups == 0) return;
+    hipLaunchKernelGGL(rt::light_bounds_kernel, dim3((groups + 63) / 64), dim3(64), 0, s, lights, n, out);
 }
 
 void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s) {

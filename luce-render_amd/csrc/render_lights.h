@@ -191,6 +191,69 @@ __device__ __forceinline__ float intersect_light(const Params &p, uint32_t light
     return inside ? t : kInf;
 }
 
+// Culling for the analytic lights a ray may meet: a bounding sphere per light
+// and per group of kLightGroup consecutive lights (radius < 0: nothing to
+// hit), inflated so that f32 rounding never culls a light intersect_light
+// would return. Lights are still tested in index order, so the nearest light
+// and its ties come out exactly as from the plain loop.
+constexpr uint32_t kLightGroup = 8u;
+
+__device__ __forceinline__ bool sphere_may_hit(float4 s, V3 o, V3 d, float dd, float t_max) {
+    if (s.w < 0.0f) return false;
+    V3 w = v3(s) - o;
+    float b = dot(w, d);
+    float ww = dot(w, w);
+    float r2 = s.w * s.w * dd;
+    float slack = 1e-5f * ww * dd;
+    if (ww * dd - b * b > r2 + slack) return false;      // the line passes outside
+    if (b < 0.0f && b * b > r2 + slack) return false;    // wholly behind the origin
+    if (t_max < kInf) {
+        float e = b - t_max * dd;
+        float far = t_max * dd;
+        if (e > 0.0f && e * e > r2 + slack + 1e-5f * far * far) return false;  // wholly beyond t_max
+    }
+    return true;
+}
+
+// The nearest analytic light before `t` along the ray (lights.glsl:239-259
+// over every light, intersect_closest.comp:58-66): updates t and what.
+__device__ __forceinline__ void nearest_light(const Params &p, V3 o, V3 d, bool camera_ray, bool light_flow, float &t,
+                                              uint32_t &what) {
+    uint32_t n = light_count(p);
+    float dd = dot(d, d);
+    uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
+    for (uint32_t g = 0; g < groups; g++) {
+        if (!sphere_may_hit(p.light_bounds[g], o, d, dd, t)) continue;
+        uint32_t end = min(n, (g + 1u) * kLightGroup);
+        for (uint32_t l = g * kLightGroup; l < end; l++) {
+            if (!sphere_may_hit(p.light_bounds[groups + l], o, d, dd, t)) continue;
+            if (camera_ray && !light_visible(p, l)) continue;
+            float tl = intersect_light(p, l, o, d, t, light_flow);
+            if (tl < t) {
+                t = tl;
+                what = kLightHit | l;
+            }
+        }
+    }
+}
+
+// Whether a light path's connection to the camera passes through a visible
+// light before t_max (intersect_shadow.comp:55-60).
+__device__ __forceinline__ bool light_in_the_way(const Params &p, V3 o, V3 d, float t_max) {
+    uint32_t n = light_count(p);
+    float dd = dot(d, d);
+    uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
+    for (uint32_t g = 0; g < groups; g++) {
+        if (!sphere_may_hit(p.light_bounds[g], o, d, dd, t_max)) continue;
+        uint32_t end = min(n, (g + 1u) * kLightGroup);
+        for (uint32_t l = g * kLightGroup; l < end; l++) {
+            if (!sphere_may_hit(p.light_bounds[groups + l], o, d, dd, t_max)) continue;
+            if (light_visible(p, l) && intersect_light(p, l, o, d, t_max, true) < t_max) return true;
+        }
+    }
+    return false;
+}
+
 __device__ __forceinline__ float light_hit_pdf(const Params &p, uint32_t light, V3 o, V3 d, float t) {
     float4 r0 = light_at(p, light, 0);
     uint32_t type = (uint32_t)r0.w;

@@ -352,6 +352,24 @@ def furnace_set(albedo=0.8, frequency=16, roughness=0.0):
     return gset
 
 
+def many_lights_set(count=40, seed=5):
+    """The Cornell render box lit by `count` small analytic lights instead of
+    its rect light: rect, disk and sphere lights in turn (0.12 wide, radius
+    0.06) at random places and orientations inside the box, radiance
+    log-uniform in [1, 100], all visible to the camera. Exercises the closest
+    hit's loop over many lights and its culling."""
+    gset = cornell_render_set()
+    gset.lights = []
+    rng = Xoshiro256(seed)
+    kinds = ["rect", "disk", "sphere"]
+    for i in range(count):
+        t = [(2.0 * rng.uniform() - 1.0) * 0.8 for _ in range(3)]
+        intensity = math.exp(rng.uniform() * math.log(100.0))
+        gset.lights.append(Light(kinds[i % 3], placement(random_rotation(rng), t), (1.0, 1.0, 1.0), intensity,
+                                 width=0.12, height=0.12, radius=0.06))
+    return gset
+
+
 def lamp_room_set():
     """The guiding workload of tests/gpu/integrator.lucb:362-389: a closed
     6 x 3 x 6 room (floor y = 0, ceiling y = 3, walls at x, z = +-3) lit only

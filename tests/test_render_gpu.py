@@ -74,6 +74,25 @@ def test_cornell_matches_f64_oracle(device, spectral):
 
 
 @pytest.mark.parametrize("spectral", [False, True])
+def test_many_lights_match_f64_oracle(device, spectral):
+    """Forty rect, disk and sphere lights inside the Cornell box, seen
+    directly and through bounces: the closest hit's culled light loop must
+    find the same light (or triangle) as the oracle's plain loop over every
+    light. Same tolerances as the Cornell comparison."""
+    from luce_render import scenes
+    sc, st, img = _render(device, scenes.many_lights_set(), width=64, height=64, samples=4, spectral=spectral,
+                          max_bounces=4, light_tracing=0.0)
+    pixels = np.arange(64 * 64, dtype=np.uint32)
+    got, ref = _oracle_compare(sc, st, img, pixels)
+    rel = np.abs(got - ref) / (np.abs(ref) + 1e-6)
+    per_pixel = rel.max(axis=1)
+    close = (per_pixel < 1e-3).mean()
+    say(f"many lights spectral={spectral}: mean gpu {got.mean(0)} oracle {ref.mean(0)}, within 1e-3: {close:.4f}")
+    assert np.all(np.abs(got.mean(0) - ref.mean(0)) <= 0.005 * ref.mean(0))
+    assert close >= 0.99
+
+
+@pytest.mark.parametrize("spectral", [False, True])
 def test_furnace_matches_f64_oracle(device, spectral):
     from luce_render import scenes
     sc, st, img = _render(device, scenes.furnace_set(), width=48, height=36, samples=8, spectral=spectral,
