@@ -233,7 +233,7 @@ typedef struct lr_server_settings {
     uint32_t trace_width;   /* the traversal variant every request uses */
     uint32_t trace_group;
     uint32_t trace_stack;
-    uint32_t max_active;    /* 0: no limit beyond the budget */
+    uint32_t max_active;    /* 0: no limit beyond the budget (and 128, the rows of a shared launch) */
 } lr_server_settings;
 typedef struct lr_request_status {
     uint32_t state;         /* 0 waiting for admission, 1 rendering, 2 finished */

@@ -63,6 +63,7 @@ struct lr_server {
         while (!pending.empty()) {
             Request *r = requests[pending.front()].get();
             if (settings.max_active && active.size() >= settings.max_active) break;
+            if (active.size() >= rt::kMaxRows) break;  // rows a shared launch can carry
             if (used + r->footprint > settings.budget_bytes) break;
             r->job.reset(new RenderJob());
             int rc = r->job->create(device, r->scene, &r->loaded, &r->settings, r->frame, tables);

@@ -194,6 +194,70 @@ template <uint32_t Q, uint32_t CAP> struct StagedPushes {
     }
 };
 
+// A launch over several renders (rows of a device array of Params): every
+// workgroup first lays the rows' item counts end to end, each padded to whole
+// workgroups, so that one grid of workgroups sized for the machine covers all
+// rows and every workgroup iteration works for exactly one row. A shared
+// launch then costs no more workgroups than the work needs, however uneven
+// the rows are. At most kMaxRows rows.
+constexpr uint32_t kMaxRows = 128u;
+
+struct RowPlan {
+    uint32_t off[kMaxRows + 1];  // padded prefix: row r's items start at off[r]
+    uint32_t count[kMaxRows];
+
+    // Workgroup-wide: count_of(p) is a row's item count. Returns the padded total.
+    template <class F> __device__ __forceinline__ uint32_t make(const Params *plist, uint32_t n, F count_of) {
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            uint32_t c = count_of(plist[i]);
+            count[i] = c;
+            off[i + 1] = (c + blockDim.x - 1u) / blockDim.x * blockDim.x;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64u) {
+            // one wave scans the padded counts, 64 at a time
+            uint32_t lane = threadIdx.x, carry = 0;
+            for (uint32_t base = 0; base < n; base += 64u) {
+                uint32_t i = base + lane;
+                uint32_t v = i < n ? off[i + 1] : 0u;
+                for (uint32_t d = 1; d < 64u; d <<= 1) {
+                    uint32_t t = __shfl_up(v, d);
+                    if (lane >= d) v += t;
+                }
+                if (i < n) off[i + 1] = v + carry;
+                carry += __shfl(v, 63);
+            }
+            if (lane == 0) off[0] = 0u;
+        }
+        __syncthreads();
+        return off[n];
+    }
+    // The row whose padded range holds item g (workgroup-uniform).
+    __device__ __forceinline__ uint32_t row_of(uint32_t g, uint32_t n) const {
+        uint32_t lo = 0, hi = n;  // off[lo] <= g < off[hi]
+        while (hi - lo > 1u) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        return __builtin_amdgcn_readfirstlane(lo);
+    }
+};
+
+// The items of every row: g walks the padded ranges a workgroup at a time
+// (workgroup-uniform), r is g's row, `at` the item within the row, `active`
+// whether it exists. `last` tells whether the next iteration leaves row r.
+#define FOR_ROW_ITEMS(plan, n, total, g, r, at, active)                                                    \
+    for (uint32_t g = blockIdx.x * blockDim.x; g < (total); g += gridDim.x * blockDim.x)                  \
+        for (uint32_t r = (plan).row_of(g, n), _once = 1; _once; _once = 0)                                \
+            for (uint32_t at = g - (plan).off[r] + threadIdx.x; _once; _once = 0)                          \
+                for (bool active = at < (plan).count[r]; _once; _once = 0)
+
+__device__ __forceinline__ bool row_ends(const RowPlan &plan, uint32_t n, uint32_t total, uint32_t g, uint32_t r) {
+    uint32_t next = g + gridDim.x * blockDim.x;
+    return next >= total || plan.row_of(next, n) != r;
+}
+
 __device__ __forceinline__ uint32_t queue_path(const Params &p, uint32_t queue, uint32_t at) {
     return p.queues[kQueueBase + (size_t)queue * p.pool + at];
 }

@@ -17,13 +17,6 @@ uint32_t grid_blocks(uint64_t items, uint32_t group) {
     return (uint32_t)b;
 }
 
-uint32_t stage_blocks(uint64_t items, uint32_t group, int compute_units) {
-    // About 32 resident waves a compute unit; the integrator's loops are
-    // persistent, so more workgroups only queue behind these.
-    uint64_t cap = (uint64_t)(compute_units > 0 ? compute_units : 256) * 32u / (group / 64u);
-    uint32_t b = grid_blocks(items, group);
-    return b < cap ? b : (uint32_t)cap;
-}
 
 namespace {
 
@@ -209,16 +202,25 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     uint32_t n = (uint32_t)jobs.size();
     bool sp = jobs[0]->settings.spectral != 0;
     const RenderTraceVariant &tv = render_trace_variants()[jobs[0]->trace_variant];
-    uint32_t rounds = 0, slots = 0, band = 0, lights = 0, pixels = 0;
+    uint32_t rounds = 0;
+    uint64_t lights = 0;
     bool guide = false;
     for (RenderJob *j : jobs) {
         rounds = std::max(rounds, j->rounds);
-        slots = std::max(slots, j->slots);
-        band = std::max(band, j->params.band_size);
-        lights = std::max(lights, j->params.light_paths);
-        pixels = std::max(pixels, j->params.pixels);
+        lights += j->params.light_paths;
         guide = guide || j->params.guide;
     }
+    // Workgroups of a stage: enough for every request's largest item count
+    // (the pushing stages' launchers cut this down; their kernels loop).
+    auto blocks = [&](uint32_t group, auto items) {
+        uint64_t b = 0;
+        for (RenderJob *j : jobs) b += (items(j) + group - 1) / group;
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 0x7fffffffu));
+    };
+    auto band = [](RenderJob *j) -> uint64_t { return j->params.band_size; };
+    auto light_paths = [](RenderJob *j) -> uint64_t { return j->params.light_paths; };
+    auto slots = [](RenderJob *j) -> uint64_t { return j->slots; };
+    auto pixels = [](RenderJob *j) -> uint64_t { return j->params.pixels; };
     auto mark = [&](int kind) {
         if (!marks) return;
         hipEvent_t e;
@@ -228,35 +230,34 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     };
     launch_render_stage(Stage::BeginPass, sp, params, n, 0, 1, s);
     mark(KRest);
-    int cus = jobs[0]->device->compute_units;
-    launch_render_stage(Stage::Camera, sp, params, n, 0, stage_blocks(band, 256, cus), s);
+    launch_render_stage(Stage::Camera, sp, params, n, 0, blocks(256, band), s);
     mark(KCamera);
     if (lights) {
-        launch_render_stage(Stage::LightEmit, sp, params, n, 0, stage_blocks(lights, 256, cus), s);
+        launch_render_stage(Stage::LightEmit, sp, params, n, 0, blocks(256, light_paths), s);
         mark(KLight);
     }
     // One round more than bounces: the last BSDF rays still find lights
     // (integrator.lucb:806-814).
     for (uint32_t b = 0; b < rounds; b++) {
         uint32_t cur = b % 2u;
-        tv.closest(params, n, cur, grid_blocks(slots, tv.group), s);
-        launch_render_stage(Stage::Route, sp, params, n, cur, stage_blocks(slots, 256, cus), s);
+        tv.closest(params, n, cur, blocks(tv.group, slots), s);
+        launch_render_stage(Stage::Route, sp, params, n, cur, blocks(256, slots), s);
         mark(KClosest);
-        launch_render_stage(Stage::Shade, sp, params, n, cur, stage_blocks(band, 128, cus), s);
-        if (lights) launch_render_stage(Stage::ShadeLight, sp, params, n, cur, stage_blocks(lights, 128, cus), s);
+        launch_render_stage(Stage::Shade, sp, params, n, cur, blocks(128, band), s);
+        if (lights) launch_render_stage(Stage::ShadeLight, sp, params, n, cur, blocks(128, light_paths), s);
         mark(KSurface);
-        launch_render_stage(Stage::Miss, sp, params, n, 0, grid_blocks(slots, 256), s);
+        launch_render_stage(Stage::Miss, sp, params, n, 0, blocks(256, slots), s);
         mark(KMiss);
-        tv.shadow(params, n, grid_blocks(slots, tv.group), s);
+        tv.shadow(params, n, blocks(tv.group, slots), s);
         mark(KShadow);
     }
-    launch_render_stage(Stage::Finish, sp, params, n, 0, grid_blocks(band, 256), s);
+    launch_render_stage(Stage::Finish, sp, params, n, 0, blocks(256, band), s);
     mark(KFinish);
     if (guide) {
         launch_render_stage(Stage::GuideFit, sp, params, n, 0, 2048, s);
         mark(KRest);
     }
-    launch_render_stage(Stage::Convert, sp, params, n, 0, grid_blocks(pixels, 256), s);
+    launch_render_stage(Stage::Convert, sp, params, n, 0, blocks(256, pixels), s);
     mark(KConvert);
     launch_render_stage(Stage::EndPass, sp, params, n, 0, 1, s);
     mark(KRest);

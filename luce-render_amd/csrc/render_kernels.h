@@ -11,14 +11,16 @@ enum class Stage { BeginPass, Camera, LightEmit, Route, Shade, ShadeLight, Miss,
 
 // The tracing kernels of a pass for one (width, stack, group) variant.
 struct RenderTraceVariant {
-    // `p`: a device array of `n` requests' parameters (blockIdx.y picks one).
+    // `p`: a device array of `n` requests' parameters; `blocks` workgroups
+    // share their items.
     void (*closest)(const rt::Params *p, uint32_t n, uint32_t current, uint32_t blocks, hipStream_t s);
     void (*shadow)(const rt::Params *p, uint32_t n, uint32_t blocks, hipStream_t s);
     uint32_t group;
 };
 const RenderTraceVariant *render_trace_variants();
 
-// `p`: a device array of `n` requests' parameters, one per grid row.
+// `p`: a device array of `n` (at most rt::kMaxRows) requests' parameters;
+// `blocks` workgroups share every request's items (1-D grid).
 // `arg`: the closest-hit queue being read (Route, Shade, ShadeLight) or
 // "convert every pass" (Convert).
 void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks,

@@ -159,7 +159,7 @@ template <bool SPECTRAL> __device__ __forceinline__ Spec clamped(const Params &p
 // ------------------------------------------------------------ pass control
 
 __global__ void begin_pass_kernel(const Params *plist) {
-    const Params &p = plist[blockIdx.y];
+    const Params &p = plist[blockIdx.x];
     PassState *s = p.pass;
     if (threadIdx.x < 16) p.queues[threadIdx.x] = 0u;
     if (threadIdx.x == 0) {
@@ -173,7 +173,7 @@ __global__ void begin_pass_kernel(const Params *plist) {
 }
 
 __global__ void end_pass_kernel(const Params *plist) {
-    const Params &p = plist[blockIdx.y];
+    const Params &p = plist[blockIdx.x];
     PassState *s = p.pass;
     if (threadIdx.x == 0) {
         s->band += 1u;
@@ -205,14 +205,15 @@ constexpr uint32_t kStageIterations = 8u;
 
 // --------------------------------------------------------------- camera
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(const Params *plist) {
-    const Params &p = plist[blockIdx.y];
-    State st(p);
-    const PassState &ps = *p.pass;
-    uint32_t count = ps.band_count;
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(const Params *plist, uint32_t n) {
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->band_count; });
     __shared__ StagedPushes<1, 256 * kStageIterations> staged;
     staged.init({Q_CLOSEST});
-    FOR_BLOCK_ITEMS(count, path, active) {
+    FOR_ROW_ITEMS(plan, n, total, g, r, path, active) {
+        const Params &p = plist[r];
+        State st(p);
+        const PassState &ps = *p.pass;
         bool push = false;
         if (active) {
             uint32_t pixel = ps.band_start + path;
@@ -249,21 +250,21 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(co
             }
         }
         staged.push(0, path, push);
-        staged.sync(p, false);
+        staged.sync(p, row_ends(plan, n, total, g, r));
     }
-    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------- light emission
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kernel(const Params *plist) {
-    const Params &p = plist[blockIdx.y];
-    State st(p);
-    const PassState &ps = *p.pass;
-    uint32_t count = ps.light_count;
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kernel(const Params *plist, uint32_t n) {
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->light_count; });
     __shared__ StagedPushes<1, 256 * kStageIterations> staged;
     staged.init({Q_CLOSEST});
-    FOR_BLOCK_ITEMS(count, i, active) {
+    FOR_ROW_ITEMS(plan, n, total, g, r, i, active) {
+        const Params &p = plist[r];
+        State st(p);
+        const PassState &ps = *p.pass;
         bool push = false;
         uint32_t path = ps.band_count + i;
         if (active) {
@@ -379,9 +380,8 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kerne
             }
         }
         staged.push(0, path, push);
-        staged.sync(p, false);
+        staged.sync(p, row_ends(plan, n, total, g, r));
     }
-    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------- closest hits
@@ -394,30 +394,29 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kerne
 // persistent schedule measured slower here, its refills stalling on the
 // queue-then-state loads.
 template <int W, int S, int G>
-__global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint32_t current) {
-    const Params &p = plist[blockIdx.y];
+__global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint32_t n, uint32_t current) {
     __shared__ uint32_t lds[S * G];
-    State st(p);
-    uint32_t queue = Q_CLOSEST + current;
-    uint32_t count = p.queues[queue];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        p.queues[Q_SHADOW] = 0u;  // consumed by the last bounce's shadow kernel
-        p.pass->closest_rays += count;
-    }
-    // The spill area is laid out for the render's own thread count (every
-    // thread that traverses has an index below its queue length, which
-    // never exceeds the render's slots), not this launch's, which shared
-    // launches size for the largest render.
-    const DeviceScene sc = p.scene;  // a copy: registers, not reloads after every store
-    uint32_t nlights = light_count(p);
-    uint32_t cameras = p.pass->band_count;
-    FOR_ITEMS(count, at, active) {
+    __shared__ RowPlan plan;
+    const uint32_t queue = Q_CLOSEST + current;
+    uint32_t total = plan.make(plist, n, [queue](const Params &q) { return q.queues[queue]; });
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            plist[i].queues[Q_SHADOW] = 0u;  // consumed by the last bounce's shadow kernel
+            plist[i].pass->closest_rays += plan.count[i];
+        }
+    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
         if (!active) continue;
+        const Params &p = plist[r];
+        State st(p);
+        // The spill column is the item's index in its render's queue: below
+        // the render's slots, for which its spill area is laid out.
+        const DeviceScene sc = p.scene;  // a copy: registers, not reloads after every store
+        uint32_t nlights = light_count(p);
+        uint32_t cameras = p.pass->band_count;
         uint32_t path = queue_path(p, queue, at);
         float4 o4 = st.v(SV_ORIGIN, path), d4 = st.v(SV_DIRECTION, path);
         V3 o = v3(o4), d = v3(d4);
-        Hit h = trace_closest<W, S>(sc, o.x, o.y, o.z, d.x, d.y, d.z, kInf, lds + threadIdx.x, G,
-                                    blockIdx.x * G + threadIdx.x);
+        Hit h = trace_closest<W, S>(sc, o.x, o.y, o.z, d.x, d.y, d.z, kInf, lds + threadIdx.x, G, at);
         uint32_t what = h.slot;
         bool light_path = path >= cameras;
         bool camera_ray = bounce_total(st.u(SU_BOUNCE, path)) == 0u;
@@ -429,24 +428,24 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint
 
 // The paths of this bounce's closest queue to the miss, surface (camera
 // paths) and light (light paths) queues, by their hits.
-__global__ __launch_bounds__(256) void route_kernel(const Params *plist, uint32_t current) {
-    const Params &p = plist[blockIdx.y];
-    State st(p);
-    uint32_t queue = Q_CLOSEST + current;
-    uint32_t count = p.queues[queue];
-    uint32_t cameras = p.pass->band_count;
+__global__ __launch_bounds__(256) void route_kernel(const Params *plist, uint32_t n, uint32_t current) {
+    __shared__ RowPlan plan;
+    const uint32_t queue = Q_CLOSEST + current;
+    uint32_t total = plan.make(plist, n, [queue](const Params &q) { return q.queues[queue]; });
     __shared__ StagedPushes<3, 256 * kStageIterations> staged;
     staged.init({Q_MISS, Q_SURFACE, Q_LIGHT});
-    FOR_BLOCK_ITEMS(count, at, active) {
+    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
+        const Params &p = plist[r];
+        State st(p);
+        uint32_t cameras = p.pass->band_count;
         uint32_t path = active ? queue_path(p, queue, at) : 0u;
         uint32_t what = active ? st.u(SU_HIT, path) : kNoneU;
         bool light_path = path >= cameras;
         staged.push(0, path, active && what == kNoneU);
         staged.push(1, path, active && what != kNoneU && !light_path);
         staged.push(2, path, active && what != kNoneU && light_path);
-        staged.sync(p, false);
+        staged.sync(p, row_ends(plan, n, total, g, r));
     }
-    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------- surface shading
@@ -870,16 +869,18 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
 // Camera paths' and light paths' hits are shaded by separate launches: each
 // needs fewer registers than the two together, so more waves fit.
 template <bool SPECTRAL, bool LIGHT>
-__global__ __launch_bounds__(128) void shade_kernel(const Params *plist, uint32_t current) {
-    const Params &p = plist[blockIdx.y];
-    const PassState &ps = *p.pass;
+__global__ __launch_bounds__(128) void shade_kernel(const Params *plist, uint32_t n, uint32_t current) {
+    __shared__ RowPlan plan;
     const uint32_t queue = LIGHT ? Q_LIGHT : Q_SURFACE;
-    uint32_t count = p.queues[queue];
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[LIGHT ? Q_LIGHT : Q_SURFACE]; });
     // consumed by this bounce's closest-hit and routing kernels
-    if (!LIGHT && blockIdx.x == 0 && threadIdx.x == 0) p.queues[Q_CLOSEST + current] = 0u;
+    if (!LIGHT && blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) plist[i].queues[Q_CLOSEST + current] = 0u;
     __shared__ StagedPushes<2, 128 * kStageIterations> staged;
     staged.init({Q_CLOSEST + 1u - current, Q_SHADOW});
-    FOR_BLOCK_ITEMS(count, at, active) {
+    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
+        const Params &p = plist[r];
+        const PassState &ps = *p.pass;
         Pushes pu;
         uint32_t path = 0;
         if (active) {
@@ -889,25 +890,26 @@ __global__ __launch_bounds__(128) void shade_kernel(const Params *plist, uint32_
         }
         staged.push(0, path, pu.closest);
         staged.push(1, path, pu.shadow);
-        staged.sync(p, false);
+        staged.sync(p, row_ends(plan, n, total, g, r));
     }
-    staged.sync(p, true);
 }
 
 // ---------------------------------------------------------------- misses
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(const Params *plist) {
-    const Params &p = plist[blockIdx.y];
-    State st(p);
-    const PassState &ps = *p.pass;
-    uint32_t count = p.queues[Q_MISS];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        p.queues[Q_SURFACE] = 0u;  // consumed by this bounce's shading
-        p.queues[Q_LIGHT] = 0u;
-    }
-    uint32_t nlights = light_count(p);
-    FOR_ITEMS(count, at, active) {
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(const Params *plist, uint32_t n) {
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[Q_MISS]; });
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            plist[i].queues[Q_SURFACE] = 0u;  // consumed by this bounce's shading
+            plist[i].queues[Q_LIGHT] = 0u;
+        }
+    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
         if (!active) continue;
+        const Params &p = plist[r];
+        State st(p);
+        const PassState &ps = *p.pass;
+        uint32_t nlights = light_count(p);
         uint32_t path = queue_path(p, Q_MISS, at);
         if (path >= ps.band_count) continue;  // light paths that leave just end
         uint32_t bounce = bounce_total(st.u(SU_BOUNCE, path));
@@ -953,27 +955,30 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(cons
 // cut-outs present, a blocked ray walks through the surfaces it meets (at most
 // 16), keeping 1 - opacity of each; an unblocked ray delivers its radiance to
 // the camera path or, for a light path, splats it onto the film.
-template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist) {
-    const Params &p = plist[blockIdx.y];
+template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist, uint32_t n) {
     __shared__ uint32_t lds[S * G];
-    State st(p);
-    const PassState &ps = *p.pass;
-    uint32_t count = p.queues[Q_SHADOW];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        p.queues[Q_MISS] = 0u;  // consumed by this bounce's miss kernel
-        p.pass->shadow_rays += count;
-    }
-    const DeviceScene sc = p.scene;  // spill laid out for the render's own threads; a register copy
-    uint32_t nlights = light_count(p);
-    bool cutouts = K(p, K_INFO).w >= 0.5f;
-    FOR_ITEMS(count, at, active) {
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[Q_SHADOW]; });
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            plist[i].queues[Q_MISS] = 0u;  // consumed by this bounce's miss kernel
+            plist[i].pass->shadow_rays += plan.count[i];
+        }
+    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
         if (!active) continue;
+        const Params &p = plist[r];
+        State st(p);
+        const PassState &ps = *p.pass;
+        // spill column: the item's index in its render's queue (a register copy of the scene)
+        const DeviceScene sc = p.scene;
+        uint32_t nlights = light_count(p);
+        bool cutouts = K(p, K_INFO).w >= 0.5f;
         uint32_t path = queue_path(p, Q_SHADOW, at);
         float4 o = st.v(SV_SHADOW_ORIGIN, path);
         V3 d = v3(st.v(SV_SHADOW_DIRECTION, path));
         bool light_path = path >= ps.band_count;
         if (light_path && nlights && light_in_the_way(p, v3(o), d, o.w)) continue;
-        uint32_t thread = blockIdx.x * G + threadIdx.x;
+        uint32_t thread = at;
         bool blocked = trace_any<W, S>(sc, o.x, o.y, o.z, d.x, d.y, d.z, o.w, lds + threadIdx.x, G, thread);
         if (blocked && !cutouts) continue;
         float kept = 1.0f;
@@ -1018,14 +1023,15 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
 
 // ------------------------------------------------------------- the film
 
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(const Params *plist) {
-    const Params &p = plist[blockIdx.y];
-    State st(p);
-    const PassState &ps = *p.pass;
-    uint32_t count = ps.band_count;
-    bool guide_on = guiding(p);
-    FOR_ITEMS(count, path, active) {
+template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(const Params *plist, uint32_t n) {
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->band_count; });
+    FOR_ROW_ITEMS(plan, n, total, g, r, path, active) {
         if (!active) continue;
+        const Params &p = plist[r];
+        State st(p);
+        const PassState &ps = *p.pass;
+        bool guide_on = guiding(p);
         uint32_t pixel = st.u(SU_PIXEL, path);
         if (pixel == kNoneU) continue;
         Spec radiance = st.v(SV_RADIANCE, path), lambda = st.v(SV_LAMBDA, path);
@@ -1094,12 +1100,16 @@ __device__ __forceinline__ V3 neutral(V3 c) {
 }
 
 // Only the sample's last band converts (integrator.lucb:672-676).
-template <bool SPECTRAL> __global__ __launch_bounds__(256) void film_convert_kernel(const Params *plist, uint32_t always) {
-    const Params &p = plist[blockIdx.y];
-    const PassState &ps = *p.pass;
-    if (!always && ps.band + 1u != ps.bands) return;
-    uint32_t samples = ps.light_samples;
-    for (uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x; pixel < p.pixels; pixel += gridDim.x * blockDim.x) {
+template <bool SPECTRAL>
+__global__ __launch_bounds__(256) void film_convert_kernel(const Params *plist, uint32_t n, uint32_t always) {
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [always](const Params &q) {
+        return always || q.pass->band + 1u == q.pass->bands ? q.pixels : 0u;
+    });
+    FOR_ROW_ITEMS(plan, n, total, g, r, pixel, active) {
+        if (!active) continue;
+        const Params &p = plist[r];
+        uint32_t samples = p.pass->light_samples;
         float4 sum = p.film[pixel];
         V3 value = v3(sum) / fmaxf(sum.w, 1.0f);
         if (samples > 0u) value = value + v3(p.film[(size_t)FILM_SPLAT * p.pixels + pixel]) / (float)samples;
@@ -1207,12 +1217,27 @@ __global__ __launch_bounds__(256) void fill_kernel(uint32_t *dst, size_t words, 
 
 namespace {
 
+// Workgroups of the pushing stages: about 32 waves a compute unit, so that
+// each workgroup loops over several iterations between its staged flushes.
+// (The tracing kernels take one item per thread: measured faster than any
+// persistent grid, whose slowest workgroups set the launch's end.)
+uint32_t staged_blocks(uint32_t group, uint32_t wanted) {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    }
+    uint64_t cap = (uint64_t)cus * 32u / (group / 64u);
+    return wanted < cap ? wanted : (uint32_t)cap;
+}
+
 template <int W, int S, int G>
 void launch_closest(const rt::Params *p, uint32_t n, uint32_t cur, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((rt::closest_kernel_rt<W, S, G>), dim3(blocks, n), dim3(G), 0, s, p, cur);
+    hipLaunchKernelGGL((rt::closest_kernel_rt<W, S, G>), dim3(blocks), dim3(G), 0, s, p, n, cur);
 }
 template <int W, int S, int G> void launch_shadow(const rt::Params *p, uint32_t n, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((rt::shadow_kernel<W, S, G>), dim3(blocks, n), dim3(G), 0, s, p);
+    hipLaunchKernelGGL((rt::shadow_kernel<W, S, G>), dim3(blocks), dim3(G), 0, s, p, n);
 }
 
 template <int W, int S, int G> void fill_trace(RenderTraceVariant &v) {
@@ -1247,26 +1272,28 @@ const RenderTraceVariant *render_trace_variants() {
 
 template <bool SPECTRAL>
 static void launch_stage(Stage stage, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks, hipStream_t s) {
-    dim3 g(blocks, n);
+#define LR_STAGE(kernel, group, ...)                                                             \
+    hipLaunchKernelGGL(kernel, dim3(staged_blocks(group, blocks)), dim3(group), 0, s, __VA_ARGS__)
     switch (stage) {
-        case Stage::Camera: hipLaunchKernelGGL((rt::camera_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
-        case Stage::LightEmit: hipLaunchKernelGGL((rt::light_emit_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
-        case Stage::Shade: hipLaunchKernelGGL((rt::shade_kernel<SPECTRAL, false>), g, dim3(128), 0, s, p, arg); break;
-        case Stage::ShadeLight: hipLaunchKernelGGL((rt::shade_kernel<SPECTRAL, true>), g, dim3(128), 0, s, p, arg); break;
-        case Stage::Route: hipLaunchKernelGGL(rt::route_kernel, g, dim3(256), 0, s, p, arg); break;
-        case Stage::Miss: hipLaunchKernelGGL((rt::miss_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
-        case Stage::Finish: hipLaunchKernelGGL((rt::finish_kernel<SPECTRAL>), g, dim3(256), 0, s, p); break;
-        case Stage::Convert: hipLaunchKernelGGL((rt::film_convert_kernel<SPECTRAL>), g, dim3(256), 0, s, p, arg); break;
+        case Stage::Camera: LR_STAGE((rt::camera_kernel<SPECTRAL>), 256, p, n); break;
+        case Stage::LightEmit: LR_STAGE((rt::light_emit_kernel<SPECTRAL>), 256, p, n); break;
+        case Stage::Shade: LR_STAGE((rt::shade_kernel<SPECTRAL, false>), 128, p, n, arg); break;
+        case Stage::ShadeLight: LR_STAGE((rt::shade_kernel<SPECTRAL, true>), 128, p, n, arg); break;
+        case Stage::Route: LR_STAGE(rt::route_kernel, 256, p, n, arg); break;
+        case Stage::Miss: LR_STAGE((rt::miss_kernel<SPECTRAL>), 256, p, n); break;
+        case Stage::Finish: LR_STAGE((rt::finish_kernel<SPECTRAL>), 256, p, n); break;
+        case Stage::Convert: LR_STAGE((rt::film_convert_kernel<SPECTRAL>), 256, p, n, arg); break;
         default: break;
     }
+#undef LR_STAGE
 }
 
 void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks,
                          hipStream_t s) {
     if (n == 0) return;
     switch (stage) {
-        case Stage::BeginPass: hipLaunchKernelGGL(rt::begin_pass_kernel, dim3(1, n), dim3(64), 0, s, p); return;
-        case Stage::EndPass: hipLaunchKernelGGL(rt::end_pass_kernel, dim3(1, n), dim3(64), 0, s, p); return;
+        case Stage::BeginPass: hipLaunchKernelGGL(rt::begin_pass_kernel, dim3(n), dim3(64), 0, s, p); return;
+        case Stage::EndPass: hipLaunchKernelGGL(rt::end_pass_kernel, dim3(n), dim3(64), 0, s, p); return;
         case Stage::GuideFit: hipLaunchKernelGGL(rt::guide_fit_kernel, dim3(blocks, n), dim3(256), 0, s, p); return;
         default: break;
     }
