@@ -279,6 +279,33 @@ int lr_event_record(void *event, void *stream);
 int lr_event_elapsed(void *start, void *stop, float *ms);
 void lr_event_destroy(void *event);
 
+/* ------------------------------------------------------------------------
+ * The learned guide: an MLP (64 one-blob inputs of a surface point and its
+ * normal, three hidden layers of 64, 32 outputs) giving a mixture of four von
+ * Mises-Fisher lobes over incident directions, run on the matrix cores in
+ * FP16 with FP32 accumulation and trained by Adam on the loss
+ * -weight * log q(direction). No reference counterpart (the reference guides
+ * with its hash grid only); these entry points expose the network on its own
+ * (the renderer trains it internally when guiding is 2).
+ *   samples: 10 floats each (position xyz, normal xyz, direction xyz, weight)
+ *   points:  6 floats each (position xyz, normal xyz)
+ *   mixtures: 20 floats a point (lobe k: direction xyz, concentration at
+ *             4k..4k+3; the four lobe weights at 16..19) */
+typedef struct lr_guide_net lr_guide_net;
+int lr_guide_net_create(lr_device *device, const float lo[3], const float hi[3], uint32_t seed, lr_guide_net **out);
+void lr_guide_net_destroy(lr_guide_net *net);
+size_t lr_guide_net_param_count(void);
+/* Parameters: W0, W1, W2 (64 x 64), W3 (64 x 32) row = input, then biases
+ * b0, b1, b2 (64), b3 (32). Setting them clears the Adam moments. */
+int lr_guide_net_set_params(lr_guide_net *net, const float *params, size_t count);
+int lr_guide_net_get_params(lr_guide_net *net, float *params, size_t count);
+/* The mean gradient of the batch loss sum_s (w_s / sum w) (-log q(d_s | x_s))
+ * by every parameter, and the mean per-sample loss. */
+int lr_guide_net_gradient(lr_guide_net *net, const float *samples, size_t n, float *grad, size_t count, float *loss);
+/* One Adam step (beta 0.9 / 0.99) on the batch at learning rate `rate`. */
+int lr_guide_net_step(lr_guide_net *net, const float *samples, size_t n, float rate, float *loss);
+int lr_guide_net_mixtures(lr_guide_net *net, const float *points, size_t n, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,207 @@
+// The learned path guide: a small MLP (64 one-blob inputs -> 3 hidden layers
+// of 64 ReLU units -> 32 outputs) that maps a surface point and its normal to
+// a mixture of four von Mises-Fisher lobes over incident directions, the same
+// distribution family the grid guide (render_guide.h) stores per cell. The
+// matrix products run on the CDNA matrix cores (v_mfma_f32_16x16x16_f16: FP16
+// operands, FP32 accumulation), 16 points per product, activations kept in
+// registers between layers. No reference counterpart: the reference guides
+// with the grid only; this network is trained online from the same path
+// records the grid learns from, and judged against the grid.
+//
+// Fragment layouts of v_mfma_f32_16x16x16_f16 (64 lanes; lane = c + 16 g):
+//   A[i][k] (16x16): lane i + 16 (k / 4), element k % 4
+//   B[k][j] (16x16): lane j + 16 (k / 4), element k % 4
+//   D[i][j] (16x16): lane j + 16 (i / 4), element i % 4
+// A layer computes Z^T = W^T H^T over 16 points (columns): A = W^T tiles,
+// B = H^T, and D (Z^T) is already the next layer's B, so activations never
+// leave their lanes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lr {
+namespace net {
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t kIn = 64, kHidden = 64, kOut = 32, kLobes = 4, kLayers = 4;
+constexpr uint32_t kPosBins = 16, kNormalBins = 8;
+// Parameters (f32 master copy): W0, W1, W2 [64][64], W3 [64][32] (row = input,
+// column = output), then b0, b1, b2 [64], b3 [32].
+constexpr uint32_t kW3 = 3u * kIn * kHidden;
+constexpr uint32_t kBias = kW3 + kHidden * kOut;
+constexpr uint32_t kParams = kBias + 3u * kHidden + kOut;  // 14560
+// Tiles per layer (16x16 FP16 fragments): forward A = W^T (out tiles x in
+// steps), backward A = W (in tiles x out steps).
+constexpr uint32_t kTiles = 3u * 16u + 8u;  // 56
+__host__ __device__ constexpr uint32_t layer_out(uint32_t l) { return l == 3u ? kOut : kHidden; }
+__host__ __device__ constexpr uint32_t weight_offset(uint32_t l) { return l * kIn * kHidden; }
+__host__ __device__ constexpr uint32_t bias_offset(uint32_t l) { return kBias + l * kHidden; }
+__host__ __device__ constexpr uint32_t tile_base(uint32_t l) { return l * 16u; }
+
+// The network's device memory: master parameters, Adam moments, FP16
+// fragments for the forward and backward products, bounds and step state.
+struct NetBuffers {
+    float *params;       // kParams
+    float *m, *v;        // Adam moments
+    _Float16 *fwd;       // kTiles * 64 lanes * 4
+    _Float16 *bwd;       // kTiles * 64 lanes * 4
+    float *bounds;       // lo xyz, 1 / extent xyz
+    float *stats;        // [0] sum of sample weights of the current batch, [1] Adam step
+};
+
+// Index of W_l[in][out] in the forward fragments (A = W^T: row out, column in).
+__host__ __device__ inline uint32_t fwd_index(uint32_t l, uint32_t in, uint32_t out) {
+    uint32_t m = out / 16u, i = out % 16u, kk = in / 16u, g = (in % 16u) / 4u, r = in % 4u;
+    return ((tile_base(l) + m * 4u + kk) * 64u + i + 16u * g) * 4u + r;
+}
+// Index of W_l[in][out] in the backward fragments (A = W: row in, column out).
+__host__ __device__ inline uint32_t bwd_index(uint32_t l, uint32_t in, uint32_t out) {
+    uint32_t steps = layer_out(l) / 16u;
+    uint32_t m = in / 16u, i = in % 16u, kk = out / 16u, g = (out % 16u) / 4u, r = out % 4u;
+    return ((tile_base(l) + m * steps + kk) * 64u + i + 16u * g) * 4u + r;
+}
+
+__device__ __forceinline__ half4_t load_frag(const _Float16 *frags, uint32_t tile, uint32_t lane) {
+    return *reinterpret_cast<const half4_t *>(frags + ((size_t)tile * 64u + lane) * 4u);
+}
+
+__device__ __forceinline__ float4_t mfma(half4_t a, half4_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// One-blob encoding (Mueller et al. 2019): a value in [0, 1] spread over
+// `bins` Gaussian bumps of width 1 / bins.
+__device__ __forceinline__ float blob(float u, uint32_t bin, uint32_t bins) {
+    float d = (u - ((float)bin + 0.5f) / (float)bins) * (float)bins;
+    return __expf(-0.5f * d * d);
+}
+
+// Octahedral map of a unit vector to [0, 1]^2.
+__device__ __forceinline__ void octahedral_uv(float nx, float ny, float nz, float &u, float &v) {
+    float s = fabsf(nx) + fabsf(ny) + fabsf(nz);
+    s = s > 0.0f ? s : 1.0f;
+    float x = nx / s, y = ny / s;
+    if (nz < 0.0f) {
+        float ox = (1.0f - fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        float oy = (1.0f - fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = ox, y = oy;
+    }
+    u = 0.5f * x + 0.5f, v = 0.5f * y + 0.5f;
+}
+
+// The input fragments of one point (B operand, K-step kk: features
+// 16 kk + 4 g + r): kk 0..2 the position's x, y, z blobs, kk 3 the normal's
+// octahedral u (features 0..7) and v (8..15) blobs.
+__device__ __forceinline__ void encode(const float *bounds, float px, float py, float pz, float nx, float ny, float nz,
+                                       uint32_t g, half4_t h[4]) {
+    float u[3] = {(px - bounds[0]) * bounds[3], (py - bounds[1]) * bounds[4], (pz - bounds[2]) * bounds[5]};
+    float ou, ov;
+    octahedral_uv(nx, ny, nz, ou, ov);
+#pragma unroll
+    for (uint32_t kk = 0; kk < 3u; kk++) {
+        float x = fminf(fmaxf(u[kk], 0.0f), 1.0f);
+#pragma unroll
+        for (uint32_t r = 0; r < 4u; r++) h[kk][r] = (_Float16)blob(x, 4u * g + r, kPosBins);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 4u; r++) {
+        uint32_t f = 4u * g + r;
+        h[3][r] = (_Float16)(f < 8u ? blob(ou, f, kNormalBins) : blob(ov, f - 8u, kNormalBins));
+    }
+}
+
+// Activations of one layer for 16 points: 4 tiles of D (= next B), FP32.
+struct Act {
+    float4_t t[4];
+};
+
+__device__ __forceinline__ half4_t to_half(float4_t x) {
+    half4_t h;
+#pragma unroll
+    for (int r = 0; r < 4; r++) h[r] = (_Float16)x[r];
+    return h;
+}
+
+// Z^T = W_l^T H^T + b for 16 points; `in` the B fragments of H (4 steps).
+// Output tiles: layer_out(l) / 16.
+__device__ __forceinline__ void layer_forward(const _Float16 *fwd, const float *params, uint32_t l,
+                                              const half4_t in[4], uint32_t lane, float4_t out[4]) {
+    uint32_t g = lane / 16u;
+    uint32_t tiles = layer_out(l) / 16u;
+    for (uint32_t m = 0; m < tiles; m++) {
+        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t kk = 0; kk < 4u; kk++) acc = mfma(load_frag(fwd, tile_base(l) + m * 4u + kk, lane), in[kk], acc);
+        const float *b = params + bias_offset(l) + m * 16u + 4u * g;
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] += b[r];
+        out[m] = acc;
+    }
+}
+
+// The forward pass of 16 points from their input fragments: every layer's
+// pre-activations (z[0..2] hidden, out: the 32 outputs in 2 tiles).
+__device__ __forceinline__ void forward(const _Float16 *fwd, const float *params, const half4_t x[4], uint32_t lane,
+                                        Act z[3], float4_t out[2]) {
+    half4_t h[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) h[k] = x[k];
+    for (uint32_t l = 0; l < 3u; l++) {
+        layer_forward(fwd, params, l, h, lane, z[l].t);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float4_t a = z[l].t[k];
+#pragma unroll
+            for (int r = 0; r < 4; r++) a[r] = fmaxf(a[r], 0.0f);
+            h[k] = to_half(a);
+        }
+    }
+    float4_t o[4];
+    layer_forward(fwd, params, 3u, h, lane, o);
+    out[0] = o[0];
+    out[1] = o[1];
+}
+
+// ------------------------------------------------------------ the mixture
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kLogKappaLo = -4.0f, kLogKappaHi = 7.0f;
+
+// Lobe k from the outputs o[5k .. 5k+4]: direction (normalized o[5k..5k+2]),
+// concentration exp(clamp(o[5k+3])), weight softmax(o[5k+4]).
+struct Mixture {
+    float mu[kLobes][3];
+    float kappa[kLobes];
+    float w[kLobes];
+};
+
+__device__ __forceinline__ Mixture mixture(const float *o) {
+    Mixture m;
+    float top = -3.0e38f;
+    for (uint32_t k = 0; k < kLobes; k++) top = fmaxf(top, o[5 * k + 4]);
+    float total = 0.0f;
+    for (uint32_t k = 0; k < kLobes; k++) {
+        float x = o[5 * k], y = o[5 * k + 1], z = o[5 * k + 2];
+        float len = sqrtf(x * x + y * y + z * z);
+        if (len < 1e-6f) {
+            x = 0.0f, y = 0.0f, z = 1.0f, len = 1.0f;
+        }
+        m.mu[k][0] = x / len, m.mu[k][1] = y / len, m.mu[k][2] = z / len;
+        m.kappa[k] = __expf(fminf(fmaxf(o[5 * k + 3], kLogKappaLo), kLogKappaHi));
+        m.w[k] = __expf(o[5 * k + 4] - top);
+        total += m.w[k];
+    }
+    for (uint32_t k = 0; k < kLobes; k++) m.w[k] /= total;
+    return m;
+}
+
+// vMF density on the sphere: kappa / (2 pi (1 - e^-2kappa)) e^(kappa (mu.d - 1)).
+__device__ __forceinline__ float vmf(float kappa, float cos_mu) {
+    return kappa / (2.0f * kPi * -expm1f(-2.0f * kappa)) * __expf(kappa * (cos_mu - 1.0f));
+}
+
+}  // namespace net
+}  // namespace lr

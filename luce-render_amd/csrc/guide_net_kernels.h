@@ -1,0 +1,239 @@
+// Kernels of the learned guide (guide_net.h): inference of 16 points per
+// wave, and training steps (weight sum, gradients by backpropagation on the
+// matrix cores, Adam). Each is templated on where its points or samples come
+// from: host-provided arrays (the library's test entry points) or the
+// integrator's path records.
+#pragma once
+
+#include "guide_net.h"
+
+namespace lr {
+namespace net {
+
+// A training sample: a surface point, its normal, the direction a path left
+// it by and the weight value / pdf of that direction (the incident radiance
+// estimate over the sampling density).
+struct Sample {
+    float p[3], n[3], d[3], weight;
+    bool valid;
+};
+
+constexpr float kMaxSampleScale = 1024.0f;  // per-sample weight over the batch mean, at most
+constexpr float kAdamBeta1 = 0.9f, kAdamBeta2 = 0.99f, kAdamEps = 1e-8f;
+
+// Loss of one sample, -c log q(d), and its gradient by the 20 used outputs
+// (the other 12 get 0). o: the 32 outputs.
+__device__ inline float sample_loss(const float *o, const float *d, float c, float *grad) {
+    Mixture mx = mixture(o);
+    float dens[kLobes], cosk[kLobes];
+    float pdf = 0.0f;
+    for (uint32_t k = 0; k < kLobes; k++) {
+        cosk[k] = mx.mu[k][0] * d[0] + mx.mu[k][1] * d[1] + mx.mu[k][2] * d[2];
+        dens[k] = mx.w[k] * vmf(mx.kappa[k], cosk[k]);
+        pdf += dens[k];
+    }
+    for (uint32_t i = 0; i < kOut; i++) grad[i] = 0.0f;
+    if (!(pdf > 1e-30f) || c == 0.0f) return 0.0f;
+    for (uint32_t k = 0; k < kLobes; k++) {
+        float resp = dens[k] / pdf;
+        float kappa = mx.kappa[k];
+        grad[5 * k + 4] = -c * (resp - mx.w[k]);
+        float lk = o[5 * k + 3];
+        if (lk > kLogKappaLo && lk < kLogKappaHi) {
+            float t = 2.0f * kappa / expm1f(2.0f * kappa);
+            grad[5 * k + 3] = -c * resp * (1.0f - t + kappa * (cosk[k] - 1.0f));
+        }
+        float x = o[5 * k], y = o[5 * k + 1], z = o[5 * k + 2];
+        float len = sqrtf(x * x + y * y + z * z);
+        if (len >= 1e-6f) {
+            float s = -c * resp * kappa / len;
+            for (uint32_t a = 0; a < 3u; a++) grad[5 * k + a] = s * (d[a] - cosk[k] * mx.mu[k][a]);
+        }
+    }
+    return -c * logf(pdf);
+}
+
+// Workgroup-wide (any size): adds the valid samples' weights into stats[0]
+// and counts the Adam step in stats[1].
+template <class Source>
+__global__ __launch_bounds__(256) void weights_kernel(NetBuffers net, Source src, uint32_t n) {
+    __shared__ float part[256];
+    float sum = 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        Sample s = src.sample(i);
+        if (s.valid) sum += s.weight;
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2u; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&net.stats[0], part[0]);
+        if (blockIdx.x == 0) atomicAdd(&net.stats[1], 1.0f);
+    }
+}
+
+// One wave a workgroup. Each wave takes groups of 16 samples (stride: the
+// grid) and adds its gradient sum into its own row of `partials` (kParams + 1
+// floats: the last the loss), which must start at zero.
+template <class Source>
+__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials) {
+    __shared__ float hs[16][kIn];
+    __shared__ float ds[16][kHidden];
+    __shared__ float os[16][kOut];
+    const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
+    float *part = partials + (size_t)blockIdx.x * (kParams + 1u);
+    float mean = net.stats[0] / (float)(n > 0 ? n : 1);
+    float loss = 0.0f;
+    for (uint32_t base = blockIdx.x * 16u; base < n; base += gridDim.x * 16u) {
+        Sample s = src.sample(base + j < n ? base + j : n - 1u);
+        bool valid = base + j < n && s.valid && mean > 0.0f;
+        float c = valid ? fminf(s.weight / mean, kMaxSampleScale) : 0.0f;
+        half4_t x[4];
+        encode(net.bounds, s.p[0], s.p[1], s.p[2], s.n[0], s.n[1], s.n[2], g, x);
+        Act z[3];
+        float4_t o[2];
+        forward(net.fwd, net.params, x, lane, z, o);
+        // outputs to LDS; one lane per sample forms the loss gradient
+#pragma unroll
+        for (uint32_t m = 0; m < 2u; m++)
+#pragma unroll
+            for (uint32_t r = 0; r < 4u; r++) os[j][16u * m + 4u * g + r] = o[m][r];
+        __syncthreads();
+        if (g == 0u) {
+            float out[kOut], grad[kOut];
+            for (uint32_t i = 0; i < kOut; i++) out[i] = os[j][i];
+            loss += sample_loss(out, s.d, c, grad);
+            for (uint32_t i = 0; i < kOut; i++) os[j][i] = grad[i];
+        }
+        __syncthreads();
+        // dZ of the output layer, as tiles (D layout)
+        float4_t dz[4];
+#pragma unroll
+        for (uint32_t m = 0; m < 2u; m++)
+#pragma unroll
+            for (uint32_t r = 0; r < 4u; r++) dz[m][r] = os[j][16u * m + 4u * g + r];
+        for (int l = 3; l >= 0; l--) {
+            const uint32_t outs = layer_out((uint32_t)l), steps = outs / 16u;
+            // the layer's input H (relu of the previous pre-activation, or the encoding)
+#pragma unroll
+            for (uint32_t m = 0; m < 4u; m++)
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; r++)
+                    hs[j][16u * m + 4u * g + r] = l == 0 ? (float)x[m][r] : fmaxf(z[l - 1].t[m][r], 0.0f);
+            for (uint32_t m = 0; m < steps; m++)
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; r++) ds[j][16u * m + 4u * g + r] = dz[m][r];
+            __syncthreads();
+            // dW = H^T dZ: rows (inputs) by tile m, columns (outputs) by tile q,
+            // the 16 samples as the product's inner dimension
+            for (uint32_t m = 0; m < 4u; m++) {
+                half4_t a;
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; r++) a[r] = (_Float16)hs[4u * g + r][16u * m + j];
+                for (uint32_t q = 0; q < steps; q++) {
+                    half4_t b;
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; r++) b[r] = (_Float16)ds[4u * g + r][16u * q + j];
+                    float4_t w = mfma(a, b, float4_t{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; r++) {
+                        float *at = part + weight_offset((uint32_t)l) + (16u * m + 4u * g + r) * outs + 16u * q + j;
+                        *at += w[r];
+                    }
+                }
+            }
+            // db = sum of dZ over the samples
+            if (lane < outs) {
+                float sb = 0.0f;
+                for (uint32_t k = 0; k < 16u; k++) sb += ds[k][lane];
+                part[bias_offset((uint32_t)l) + lane] += sb;
+            }
+            if (l > 0) {
+                // dH^T = W dZ^T, then through the ReLU of layer l - 1
+                half4_t b[4];
+                for (uint32_t q = 0; q < steps; q++) b[q] = to_half(dz[q]);
+                float4_t dh[4];
+                for (uint32_t m = 0; m < 4u; m++) {
+                    float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                    for (uint32_t q = 0; q < steps; q++)
+                        acc = mfma(load_frag(net.bwd, tile_base((uint32_t)l) + m * steps + q, lane), b[q], acc);
+                    dh[m] = acc;
+                }
+#pragma unroll
+                for (uint32_t m = 0; m < 4u; m++)
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; r++) dz[m][r] = z[l - 1].t[m][r] > 0.0f ? dh[m][r] : 0.0f;
+            }
+            __syncthreads();
+        }
+    }
+    // the wave's loss
+    for (uint32_t off = 32u; off > 0; off >>= 1) loss += __shfl_xor(loss, (int)off);
+    if (lane == 0) part[kParams] += loss;
+}
+
+// Writes parameter i (f32) and its FP16 fragments.
+__device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w) {
+    net.params[i] = w;
+    if (i >= kBias) return;
+    uint32_t l = i < kW3 ? i / (kIn * kHidden) : 3u;
+    uint32_t local = i - weight_offset(l), outs = layer_out(l);
+    uint32_t in = local / outs, out = local % outs;
+    net.fwd[fwd_index(l, in, out)] = (_Float16)w;
+    net.bwd[bwd_index(l, in, out)] = (_Float16)w;
+}
+
+// One thread a parameter: the gradient is the sum of `waves` partial rows
+// over the samples; an Adam step at rate `rate` (0: only refresh the FP16
+// fragments from the master copy). Clears the batch's weight sum.
+__global__ __launch_bounds__(256) void adam_kernel(NetBuffers net, const float *partials, uint32_t waves,
+                                                   uint32_t samples, float rate) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kParams) {
+        float w = net.params[i];
+        if (rate > 0.0f) {
+            float grad = 0.0f;
+            for (uint32_t k = 0; k < waves; k++) grad += partials[(size_t)k * (kParams + 1u) + i];
+            grad /= (float)(samples > 0 ? samples : 1);
+            float t = net.stats[1];
+            float m = kAdamBeta1 * net.m[i] + (1.0f - kAdamBeta1) * grad;
+            float v = kAdamBeta2 * net.v[i] + (1.0f - kAdamBeta2) * grad * grad;
+            net.m[i] = m;
+            net.v[i] = v;
+            float mh = m / (1.0f - powf(kAdamBeta1, t)), vh = v / (1.0f - powf(kAdamBeta2, t));
+            w -= rate * mh / (sqrtf(vh) + kAdamEps);
+        }
+        store_param(net, i, w);
+    }
+    if (i == 0) net.stats[0] = 0.0f;
+}
+
+// Mixtures of points, 16 a wave: `dst.point(i)` gives point i's position and
+// normal, `dst.store(i, mixture)` takes the result.
+template <class Points>
+__global__ __launch_bounds__(64) void infer_kernel(NetBuffers net, Points pts, uint32_t n) {
+    __shared__ float os[16][kOut];
+    const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
+    for (uint32_t base = blockIdx.x * 16u; base < n; base += gridDim.x * 16u) {
+        float p[3], nn[3];
+        pts.point(base + j < n ? base + j : n - 1u, p, nn);
+        half4_t x[4];
+        encode(net.bounds, p[0], p[1], p[2], nn[0], nn[1], nn[2], g, x);
+        Act z[3];
+        float4_t o[2];
+        forward(net.fwd, net.params, x, lane, z, o);
+#pragma unroll
+        for (uint32_t m = 0; m < 2u; m++)
+#pragma unroll
+            for (uint32_t r = 0; r < 4u; r++) os[j][16u * m + 4u * g + r] = o[m][r];
+        __syncthreads();
+        if (g == 0u && base + j < n) pts.store(base + j, mixture(os[j]));
+        __syncthreads();
+    }
+}
+
+}  // namespace net
+}  // namespace lr
