@@ -160,7 +160,10 @@ typedef struct lr_loaded {
 /* RenderSettings (src/render/integrator.lucb:42-85) as far as the device
  * needs them; the rest is in the loaded constants. pool 0: min(2^21, pixels).
  * use_graph: replay each pass from one recorded hipGraph. trace_*: the
- * traversal variant (0: the scene's configuration). */
+ * traversal variant (0: the scene's configuration). guiding: 0 off, 1 the
+ * reference's hash grid of vMF mixtures, 2 the learned guide (an MLP on the
+ * matrix cores trained online during the first train_samples samples, 0: 8,
+ * with train_steps Adam steps after each, 0: 16; single renders only). */
 typedef struct lr_render_settings {
     uint32_t width;
     uint32_t height;
@@ -175,6 +178,8 @@ typedef struct lr_render_settings {
     uint32_t trace_width;
     uint32_t trace_group;
     uint32_t trace_stack;
+    uint32_t train_samples;
+    uint32_t train_steps;
 } lr_render_settings;
 
 /* Renderer.image / display: device pointers to the rgba32f ACEScg image and

@@ -127,6 +127,7 @@ int lr_server_submit(lr_server *s, lr_gpu_scene *scene, const lr_loaded *loaded,
         return fail(LR_INVALID, "a render needs samples and 1 to 1024 bounces");
     if ((settings->spectral != 0) != (s->settings.spectral != 0))
         return fail(LR_INVALID, "a server's requests share its spectral mode");
+    if (settings->guiding > 1) return fail(LR_INVALID, "served requests guide with the hash grid at most");
     std::unique_ptr<Request> r(new Request());
     r->scene = scene;
     r->settings = *settings;

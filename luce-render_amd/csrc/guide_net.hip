@@ -51,6 +51,11 @@ struct ArrayPoints {
     }
 };
 
+__global__ __launch_bounds__(256) void adam_kernel(NetBuffers net, float *partials, uint32_t waves, uint32_t samples,
+                                                   float rate) {
+    adam_body(net, partials, waves, samples, rate, false);
+}
+
 __global__ void zero_kernel(float *p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
@@ -105,6 +110,7 @@ int GuideNet::create(lr_device *dev, const float lo[3], const float hi[3], uint3
     hipError_t e = hipMemcpyAsync(b.bounds, bounds, 64, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(b.stats, stats, 64, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "learned guide set-up");
+    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, s, partials, (size_t)waves * (kParams + 1));
     return set_params(init.data());
 }
 
@@ -119,8 +125,7 @@ int GuideNet::set_params(const float *params) {
 }
 
 void GuideNet::refresh(hipStream_t s) {
-    hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, b, (const float *)partials, 0u, 0u,
-                       0.0f);
+    hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, b, partials, 0u, 0u, 0.0f);
 }
 
 }  // namespace lr
@@ -193,8 +198,8 @@ static int batch(lr_guide_net *net, const float *samples, size_t n, float rate, 
         e = hipMemcpyAsync(part.data(), g.partials, part.size() * 4, hipMemcpyDeviceToHost, s);
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, g.b, (const float *)g.partials,
-                           waves, (uint32_t)n, rate);
+        hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, g.b, g.partials, waves,
+                           (uint32_t)n, rate);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -53,10 +53,9 @@ __device__ inline float sample_loss(const float *o, const float *d, float c, flo
     return -c * logf(pdf);
 }
 
-// Workgroup-wide (any size): adds the valid samples' weights into stats[0]
-// and counts the Adam step in stats[1].
-template <class Source>
-__global__ __launch_bounds__(256) void weights_kernel(NetBuffers net, Source src, uint32_t n) {
+// Workgroup-wide (256 threads): adds the valid samples' weights into
+// stats[0] and counts the Adam step in stats[1].
+template <class Source> __device__ __forceinline__ void weights_body(NetBuffers &net, Source &src, uint32_t n) {
     __shared__ float part[256];
     float sum = 0.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -75,11 +74,16 @@ __global__ __launch_bounds__(256) void weights_kernel(NetBuffers net, Source src
     }
 }
 
+template <class Source>
+__global__ __launch_bounds__(256) void weights_kernel(NetBuffers net, Source src, uint32_t n) {
+    weights_body(net, src, n);
+}
+
 // One wave a workgroup. Each wave takes groups of 16 samples (stride: the
 // grid) and adds its gradient sum into its own row of `partials` (kParams + 1
 // floats: the last the loss), which must start at zero.
 template <class Source>
-__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials) {
+__device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t n, float *partials) {
     __shared__ float hs[16][kIn];
     __shared__ float ds[16][kHidden];
     __shared__ float os[16][kOut];
@@ -175,6 +179,11 @@ __global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, ui
     if (lane == 0) part[kParams] += loss;
 }
 
+template <class Source>
+__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials) {
+    grad_body(net, src, n, partials);
+}
+
 // Writes parameter i (f32) and its FP16 fragments.
 __device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w) {
     net.params[i] = w;
@@ -186,17 +195,25 @@ __device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w
     net.bwd[bwd_index(l, in, out)] = (_Float16)w;
 }
 
-// One thread a parameter: the gradient is the sum of `waves` partial rows
+// One thread a parameter (kParams + 1 threads when clearing): the gradient
+// is the sum of `waves` partial rows
 // over the samples; an Adam step at rate `rate` (0: only refresh the FP16
 // fragments from the master copy). Clears the batch's weight sum.
-__global__ __launch_bounds__(256) void adam_kernel(NetBuffers net, const float *partials, uint32_t waves,
-                                                   uint32_t samples, float rate) {
+// With `clear`, the partial rows read are zeroed for the next batch.
+__device__ __forceinline__ void adam_body(NetBuffers &net, float *partials, uint32_t waves, uint32_t samples,
+                                          float rate, bool clear) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear && i == kParams)
+        for (uint32_t k = 0; k < waves; k++) partials[(size_t)k * (kParams + 1u) + kParams] = 0.0f;
     if (i < kParams) {
         float w = net.params[i];
         if (rate > 0.0f) {
             float grad = 0.0f;
-            for (uint32_t k = 0; k < waves; k++) grad += partials[(size_t)k * (kParams + 1u) + i];
+            for (uint32_t k = 0; k < waves; k++) {
+                float *at = partials + (size_t)k * (kParams + 1u) + i;
+                grad += *at;
+                if (clear) *at = 0.0f;
+            }
             grad /= (float)(samples > 0 ? samples : 1);
             float t = net.stats[1];
             float m = kAdamBeta1 * net.m[i] + (1.0f - kAdamBeta1) * grad;
@@ -211,10 +228,10 @@ __global__ __launch_bounds__(256) void adam_kernel(NetBuffers net, const float *
     if (i == 0) net.stats[0] = 0.0f;
 }
 
+
 // Mixtures of points, 16 a wave: `dst.point(i)` gives point i's position and
 // normal, `dst.store(i, mixture)` takes the result.
-template <class Points>
-__global__ __launch_bounds__(64) void infer_kernel(NetBuffers net, Points pts, uint32_t n) {
+template <class Points> __device__ __forceinline__ void infer_body(const NetBuffers &net, Points &pts, uint32_t n) {
     __shared__ float os[16][kOut];
     const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
     for (uint32_t base = blockIdx.x * 16u; base < n; base += gridDim.x * 16u) {
@@ -233,6 +250,11 @@ __global__ __launch_bounds__(64) void infer_kernel(NetBuffers net, Points pts, u
         if (g == 0u && base + j < n) pts.store(base + j, mixture(os[j]));
         __syncthreads();
     }
+}
+
+template <class Points>
+__global__ __launch_bounds__(64) void infer_kernel(NetBuffers net, Points pts, uint32_t n) {
+    infer_body(net, pts, n);
 }
 
 }  // namespace net

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "guide_net.h"
 #include "trace_device.h"
 
 namespace lr {
@@ -51,10 +52,11 @@ constexpr int K_EMIT_CENTER = 22;
 // State fields (float4 and u32), as common.glsl:89-157.
 constexpr uint32_t SV_ORIGIN = 0, SV_DIRECTION = 1, SV_THROUGHPUT = 2, SV_RADIANCE = 3, SV_LAMBDA = 4,
                    SV_SHADOW_ORIGIN = 5, SV_SHADOW_DIRECTION = 6, SV_SHADOW_RADIANCE = 7, SV_HIT = 8, SV_VERTEX = 9,
-                   SV_GUIDE0 = 10, SV_MIS = 13;
+                   SV_GUIDE0 = 10, SV_MIS = 13,
+                   SV_GUIDE_POS0 = 14;  // learned guide: (point, octahedral geometric normal bits) per guide record
 constexpr uint32_t SU_PIXEL = 0, SU_BOUNCE = 1, SU_HIT = 2, SU_NORMAL = 3, SU_GUIDE_CELL0 = 4, SU_GUIDE_DIR0 = 7,
                    SU_SPLAT = 10;
-constexpr uint32_t kStateV = 14, kStateU = 11;
+constexpr uint32_t kStateV = 17, kStateU = 11;
 
 // Queues: counters [0..16), then each queue's paths from QUEUE_BASE.
 constexpr uint32_t Q_CLOSEST = 0, Q_SURFACE = 2, Q_MISS = 3, Q_SHADOW = 4, Q_LIGHT = 5;
@@ -106,6 +108,16 @@ struct Params {
     float4 *display;
     uint8_t *guide;        // guiding grid (keys, lobes, stats); null without guiding
     const float4 *light_bounds;  // bounding spheres: one per group of kLightGroup lights, then one per light
+    // The learned guide (guiding 2): the network, the per-slot mixtures the
+    // inference writes each bounce (rows of `pool` float4: four lobes, then
+    // the lobe weights), the training gradient rows and schedule.
+    net::NetBuffers net;
+    float4 *guide_mix;
+    float *net_partials;
+    uint32_t learned;
+    uint32_t train_samples;  // samples whose paths train it
+    uint32_t train_batch;    // records a training step draws
+    float train_rate;
     PassState *pass;
     DeviceScene scene;
 };

@@ -16,7 +16,11 @@ constexpr uint32_t GUIDE_TRAINING = 1u;
 constexpr uint32_t GUIDE_PROBES = 8u;
 constexpr size_t kGuideBytes = (size_t)GUIDE_CELLS * (4 + 16 * (GUIDE_LOBES + 1) + 4 * 5 * GUIDE_LOBES);
 
-__device__ __forceinline__ bool guiding(const Params &p) { return p.guide != nullptr && K(p, K_GUIDE).x > 0.5f; }
+__device__ __forceinline__ bool guiding(const Params &p) {
+    return (p.guide != nullptr || p.learned) && K(p, K_GUIDE).x > 0.5f;
+}
+// The learned guide has taken a training step (its mixtures are in use).
+__device__ __forceinline__ bool net_ready(const Params &p) { return p.learned && p.net.stats[1] >= 1.0f; }
 __device__ __forceinline__ uint32_t *guide_keys(const Params &p) { return reinterpret_cast<uint32_t *>(p.guide); }
 __device__ __forceinline__ float4 *guide_lobes(const Params &p) {
     return reinterpret_cast<float4 *>(p.guide + (size_t)GUIDE_CELLS * 4);
@@ -57,7 +61,10 @@ __device__ __forceinline__ float guide_count(const Params &p, uint32_t cell) {
     return c;
 }
 
+// With the learned guide, `cell` is the path's slot and the lobes are the
+// mixture inferred for its current vertex.
 __device__ __forceinline__ float4 guide_lobe(const Params &p, uint32_t cell, uint32_t lobe) {
+    if (p.learned) return p.guide_mix[(size_t)lobe * p.pool + cell];
     float4 v = guide_lobes(p)[cell * (GUIDE_LOBES + 1u) + lobe];
     if (v.w > 0.0f) return v;
     const float k = 0.57735026919f;  // the tetrahedron's directions, normalized
@@ -68,6 +75,7 @@ __device__ __forceinline__ float4 guide_lobe(const Params &p, uint32_t cell, uin
 }
 
 __device__ __forceinline__ float4 guide_weights(const Params &p, uint32_t cell) {
+    if (p.learned) return p.guide_mix[(size_t)GUIDE_LOBES * p.pool + cell];
     float4 w = guide_lobes(p)[cell * (GUIDE_LOBES + 1u) + GUIDE_LOBES];
     float total = w.x + w.y + w.z + w.w;
     return total > 0.0f ? w / total : make_float4(0.25f, 0.25f, 0.25f, 0.25f);

@@ -7,6 +7,7 @@
 
 #include "render_bsdf.h"
 #include "render_guide.h"
+#include "render_learned.h"
 
 namespace lr {
 
@@ -56,6 +57,7 @@ void *RenderJob::alloc(size_t n, int &rc) {
 }
 
 size_t RenderJob::keep_image_only() {
+    net.reset();
     std::vector<void *> kept;
     for (void *b : buffers) {
         if (b == (void *)params.image) kept.push_back(b);
@@ -75,7 +77,10 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     b += (size_t)rt::kStateV * s.slots * 16 + (size_t)rt::kStateU * s.slots * 4;
     b += ((size_t)rt::kQueueBase + (size_t)rt::kQueues * s.slots) * 4;
     b += (size_t)6 * s.pixels * 16;  // film (4 planes), image, display
-    if (st->guiding) b += rt::kGuideBytes;
+    if (st->guiding == 1) b += rt::kGuideBytes;
+    if (st->guiding == 2)
+        b += (size_t)5 * s.slots * 16 + (size_t)kNetWaves * (net::kParams + 1) * 4 + (size_t)net::kParams * 12 +
+             (size_t)net::kTiles * 64 * 16 + 128;
     uint32_t w = st->trace_width ? st->trace_width : scene->config.width;
     uint32_t stack = st->trace_stack ? st->trace_stack : scene->config.stack;
     uint32_t group = st->trace_group ? st->trace_group : scene->config.group;
@@ -93,6 +98,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
         return fail(LR_INVALID, "an image is 1 to 16384 pixels on a side");
     if (st.samples == 0 || st.max_bounces == 0 || st.max_bounces > 1024)
         return fail(LR_INVALID, "a render needs samples and 1 to 1024 bounces");
+    if (st.guiding > 2) return fail(LR_INVALID, "guiding is 0 (off), 1 (hash grid) or 2 (learned)");
     if ((uint32_t)frame[11] != st.width || (uint32_t)frame[15] != st.height)
         return fail(LR_INVALID, "the frame's size is not the settings' image size");
     if (!loaded->constants || !loaded->lights || !loaded->materials || !loaded->emitters || loaded->material_floats < 64)
@@ -167,7 +173,20 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     p.image = (float4 *)alloc((size_t)sh.pixels * 16, rc);
     p.display = (float4 *)alloc((size_t)sh.pixels * 16, rc);
     p.pass = (rt::PassState *)alloc(sizeof(rt::PassState), rc);
-    if (st.guiding) p.guide = (uint8_t *)alloc(rt::kGuideBytes, rc);
+    if (st.guiding == 1) p.guide = (uint8_t *)alloc(rt::kGuideBytes, rc);
+    if (st.guiding == 2) {
+        net.reset(new GuideNet());
+        int nrc = net->create(dev, sc->root_lo, sc->root_hi, st.seed, kNetWaves);
+        if (nrc) return nrc;
+        p.net = net->b;
+        p.net_partials = net->partials;
+        p.guide_mix = (float4 *)alloc((size_t)5 * slots * 16, rc);
+        p.learned = 1;
+        p.train_samples = st.train_samples ? st.train_samples : 8u;
+        p.train_batch = 8192u;
+        p.train_rate = 5e-3f;
+        train_steps = st.train_steps ? st.train_steps : 16u;
+    }
     params_device = (rt::Params *)alloc(sizeof(rt::Params), rc);
     if (rc) return rc;
     // The traversal's spill area for the largest grid a pass launches, and
@@ -187,6 +206,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     launch_fill(p.queues, queue_words, 0u, s);
     launch_fill(p.state_u, (size_t)rt::kStateU * slots, 0u, s);
     if (p.guide) launch_fill((uint32_t *)p.guide, rt::kGuideBytes / 4, 0u, s);
+    if (p.guide_mix) launch_fill((uint32_t *)p.guide_mix, (size_t)5 * slots * 4, 0u, s);
     rt::PassState init{};
     init.bands = bands;
     hipError_t e = hipMemcpyAsync(p.pass, &init, sizeof(init), hipMemcpyHostToDevice, s);
@@ -210,6 +230,8 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         lights += j->params.light_paths;
         guide = guide || j->params.guide;
     }
+    // The learned guide runs for single renders (the server refuses it).
+    bool learned = n == 1 && jobs[0]->params.learned;
     // Workgroups of a stage: enough for every request's largest item count
     // (the pushing stages' launchers cut this down; their kernels loop).
     auto blocks = [&](uint32_t group, auto items) {
@@ -243,6 +265,10 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         tv.closest(params, n, cur, blocks(tv.group, slots), s);
         launch_render_stage(Stage::Route, sp, params, n, cur, blocks(256, slots), s);
         mark(KClosest);
+        if (learned) {
+            launch_learned_infer(params, jobs[0]->params.band_size, s);
+            mark(KRest);
+        }
         launch_render_stage(Stage::Shade, sp, params, n, cur, blocks(128, band), s);
         if (lights) launch_render_stage(Stage::ShadeLight, sp, params, n, cur, blocks(128, light_paths), s);
         mark(KSurface);
@@ -253,6 +279,10 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     }
     launch_render_stage(Stage::Finish, sp, params, n, 0, blocks(256, band), s);
     mark(KFinish);
+    if (learned) {
+        launch_learned_train(params, sp, jobs[0]->train_steps, s);
+        mark(KRest);
+    }
     if (guide) {
         launch_render_stage(Stage::GuideFit, sp, params, n, 0, 2048, s);
         mark(KRest);

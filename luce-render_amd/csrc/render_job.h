@@ -4,8 +4,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
+#include "guide_net_host.h"
 #include "lr_internal.h"
 #include "lr_scene.h"
 #include "render_kernels.h"
@@ -27,6 +29,8 @@ struct RenderJob {
     int trace_variant = -1;
     size_t bytes = 0;  // device memory held
     std::vector<void *> buffers;
+    std::unique_ptr<GuideNet> net;  // the learned guide (guiding 2)
+    uint32_t train_steps = 0;
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);

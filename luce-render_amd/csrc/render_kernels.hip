@@ -565,10 +565,11 @@ template <bool SPECTRAL> __device__ Pushes shade_camera(const Params &p, const P
     float guided = 0.0f;
     bool guide_on = guiding(p);
     if (guide_on && total < GUIDE_RING) {
-        cell = guide_cell(p, x, geometric);
+        cell = p.learned ? path : guide_cell(p, x, geometric);
         bool suited = !surface.inside && surface.p_transmission <= 0.0f &&
                       (surface.p_diffuse > 0.05f || fminf(surface.alpha.x, surface.alpha.y) > 0.1f);
-        if (cell != kNoneU && suited && guide_count(p, cell) >= K(p, K_GUIDE).w) guided = K(p, K_GUIDE).z;
+        bool ready = cell != kNoneU && (p.learned ? net_ready(p) : guide_count(p, cell) >= K(p, K_GUIDE).w);
+        if (suited && ready) guided = K(p, K_GUIDE).z;
     }
 
     float4 u_light = sample4(p.seed, pixel, ps.sample, group_light(total));
@@ -683,8 +684,10 @@ template <bool SPECTRAL> __device__ Pushes shade_camera(const Params &p, const P
     st.set_u(SU_BOUNCE, path, bounces);
     st.set_u(SU_NORMAL, path, octahedral_encode(normal));
     if (guide_on && total < GUIDE_RING) {
-        if (cell == kNoneU) cell = guide_cell(p, x, geometric);
+        if (cell == kNoneU) cell = p.learned ? path : guide_cell(p, x, geometric);
         st.set_u(SU_GUIDE_CELL0 + total, path, cell);
+        if (p.learned)
+            st.set_v(SV_GUIDE_POS0 + total, path, make_float4(x.x, x.y, x.z, __uint_as_float(octahedral_encode(geometric))));
         st.set_u(SU_GUIDE_DIR0 + total, path, octahedral_encode(direction));
         float found = spec_luminance<SPECTRAL>(p, st.v(SV_RADIANCE, path), lambda) +
                       (pushed_shadow ? spec_luminance<SPECTRAL>(p, st.v(SV_SHADOW_RADIANCE, path), lambda) : 0.0f);
@@ -1046,7 +1049,7 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(co
             float4 h = half;
             half = make_float4(h.x + 2.0f * value.x, h.y + 2.0f * value.y, h.z + 2.0f * value.z, h.w);
         }
-        if (guide_on && hash_u32(pixel * 9781u + ps.sample) % GUIDE_TRAINING == 0u) {
+        if (guide_on && !p.learned && hash_u32(pixel * 9781u + ps.sample) % GUIDE_TRAINING == 0u) {
             float found = spec_luminance<SPECTRAL>(p, radiance, lambda);
             for (uint32_t k = 0; k < GUIDE_RING; k++) {
                 uint32_t cell = st.u(SU_GUIDE_CELL0 + k, path);

@@ -1,0 +1,20 @@
+// Launchers of the learned guide inside the integrator (render_learned.hip).
+#pragma once
+
+#include <algorithm>
+
+#include "render_common.h"
+
+namespace lr {
+
+// Gradient rows of a training step (one wave each).
+constexpr uint32_t kNetWaves = 256u;
+
+// Mixtures for this bounce's surface queue of the render at p[0] (`band`:
+// its camera paths, an upper bound of the queue).
+void launch_learned_infer(const rt::Params *p, uint32_t band, hipStream_t s);
+// `steps` Adam steps on the render's path records; the kernels do nothing
+// once the render is past its training samples.
+void launch_learned_train(const rt::Params *p, bool spectral, uint32_t steps, hipStream_t s);
+
+}  // namespace lr

@@ -329,7 +329,9 @@ class RenderSettings:
     noise_threshold: float = 0.0
     adaptive_minimum: int = 16
     profile: bool = False
-    guiding: bool = False
+    guiding: int = 0  # 0 off, 1 the hash grid of vMF mixtures, 2 the learned guide
+    train_samples: int = 0  # learned guide: samples that train it (0: 8)
+    train_steps: int = 0  # learned guide: Adam steps after each of them (0: 16)
     light_tracing: float = 0.5
     use_graph: bool = True
     trace: tuple = (0, 0, 0)  # traversal width, group, stack (0: the scene's)

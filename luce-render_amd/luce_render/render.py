@@ -372,8 +372,8 @@ class Renderer:
         ld.light_paths, ld.through = r.loaded.light_paths, r.loaded.through
         s = r.settings
         cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, s.pool, s.seed,
-                                  1 if s.guiding else 0, 1 if s.profile else 0, 1 if s.use_graph else 0,
-                                  *tuple(s.trace)[:3])
+                                  int(s.guiding), 1 if s.profile else 0, 1 if s.use_graph else 0,
+                                  *tuple(s.trace)[:3], s.train_samples, s.train_steps)
         r.frame16 = frame.pack()
         h = ctypes.c_void_p()
         try:

@@ -51,7 +51,7 @@ class Server:
         ld.light_paths, ld.through = loaded.light_paths, loaded.through
         s = settings
         cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, 0, s.seed,
-                                  1 if s.guiding else 0, 0, 0, 0, 0, 0)
+                                  int(s.guiding), 0, 0, 0, 0, 0, 0, 0)
         packed = frame.pack()
         rid = ctypes.c_uint64()
         _lib.check(_lib.lib().lr_server_submit(self.handle, gpu_scene.handle, ctypes.byref(ld), ctypes.byref(cs),
