@@ -83,6 +83,41 @@ def bounce_rays(sc, frame, faces_t, rng):
     return rays
 
 
+def guiding_leg(device, stream, render, gset, width, height, ref_spp, kw, spp=64):
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(gset)
+    gpu = render.GpuScene.upload(device, sc)
+
+    def run(samples, seed, guiding):
+        st = RenderSettings(width, height, samples, spectral=False, guiding=guiding, seed=seed, **kw)
+        r = render.Renderer.create(device, sc, st, gpu_scene=gpu)
+        t = render.Timer()
+        render.synchronize(device)
+        t.begin(stream)
+        r.finish()
+        t.end(stream)
+        ms = t.elapsed_ms()
+        t.destroy()
+        img = r.read()[:, :, :3].astype(np.float64)
+        r.destroy()
+        return ms, img
+
+    _, ref = run(ref_spp, 1, 0)
+    out = {"workload": f"{width}x{height}, {sc.triangle_count()} triangles, reference {ref_spp} spp plain",
+           "spp": spp}
+    plain_ms, plain_img = run(spp, 2, 0)
+    plain_mse = float(((plain_img - ref) ** 2).mean())
+    out["plain"] = {"ms": round(plain_ms, 2), "mse": plain_mse}
+    for name, mode in (("grid", 1), ("learned", 2)):
+        ms, img = run(spp, 2, mode)
+        mse = float(((img - ref) ** 2).mean())
+        needed = ms * (mse / plain_mse)
+        out[name] = {"ms": round(ms, 2), "mse": mse, "equal_time_mse": mse * ms / plain_ms,
+                     "time_to_plain_mse_ratio": round(needed / plain_ms, 3)}
+    gpu.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +130,8 @@ def main():
                     help="samples of the full path-traced render of the same scene (0: skip)")
     ap.add_argument("--serving", type=int, default=40,
                     help="requests of the Poisson serving leg (40 requests/s over the four serving scenes; 0: skip)")
+    ap.add_argument("--guiding", default="960x540:1024",
+                    help="WxH:reference-spp of the guiding leg on the glossy workload (empty: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the plain copy kernel over this many MiB (a known-traffic dispatch "
                          "for scaling hardware byte counters)")
@@ -254,6 +291,24 @@ def main():
         for g, _ in served:
             g.destroy()
 
+    # Path guiding: plain, the hash grid and the learned guide at 64 samples
+    # against a long plain reference; MSE, wall time, and the time each guided
+    # render would need to reach plain's 64-sample MSE (MSE falling as
+    # 1 / samples), over plain's 64-sample time.
+    guiding_stats = None
+    if args.guiding:
+        size, ref_spp = args.guiding.split(":")
+        gw, gh = (int(x) for x in size.split("x"))
+        from luce_render.loading import OpenPbr
+        guiding_stats = {}
+        for name, gset, w, h, rspp, kw in (
+                ("glossy", scenes.glossy_set(), gw, gh, int(ref_spp), dict(max_bounces=8, light_tracing=0.5)),
+                ("lamp_room", scenes.lamp_room_set(), 256, 192, 4096,
+                 dict(max_bounces=2, light_tracing=0.0,
+                      material=OpenPbr(base_color=(0.3, 0.3, 0.3), specular_weight=0.0)))):
+            guiding_stats[name] = guiding_leg(device, stream, render, gset, w, h, rspp, kw)
+            log(f"guiding {name}:", guiding_stats[name])
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -294,6 +349,7 @@ def main():
             "bvh_nodes": info["binary_nodes"],
             "path_tracing": render_stats,
             "serving": serving_stats,
+            "guiding": guiding_stats,
             "wide_bytes": info["wide_bytes"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

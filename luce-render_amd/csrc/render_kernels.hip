@@ -569,7 +569,11 @@ template <bool SPECTRAL> __device__ Pushes shade_camera(const Params &p, const P
         bool suited = !surface.inside && surface.p_transmission <= 0.0f &&
                       (surface.p_diffuse > 0.05f || fminf(surface.alpha.x, surface.alpha.y) > 0.1f);
         bool ready = cell != kNoneU && (p.learned ? net_ready(p) : guide_count(p, cell) >= K(p, K_GUIDE).w);
-        if (suited && ready) guided = K(p, K_GUIDE).z;
+        // The learned guide takes the diffuse lobe's share of the BSDF's
+        // choice (its mixtures model incident light, which a glossy lobe's
+        // own sampling already follows better); the grid a fixed share
+        // (loading.lucb:281).
+        if (suited && ready) guided = K(p, K_GUIDE).z * (p.learned ? surface.p_diffuse : 1.0f);
     }
 
     float4 u_light = sample4(p.seed, pixel, ps.sample, group_light(total));

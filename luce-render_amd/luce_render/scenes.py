@@ -370,6 +370,50 @@ def many_lights_set(count=40, seed=5):
     return gset
 
 
+def glossy_set(seed=7, soup=10000, light_radiance=30.0):
+    """The learned-guiding workload: the Cornell box and its two boxes with
+    `soup` random triangles (the probe's geometry), every surface a random
+    OpenPBR material (base colour U[0.2, 0.9]^3, metalness 1 with chance
+    0.3, specular roughness U[0.05, 0.3]; the triangles in 16 material
+    groups), lit only by 8 sphere lights of radius 0.02 just under the
+    ceiling, each hidden from the room below by a 0.16-wide occluder: the
+    room is lit by the bright patches the lights throw on the ceiling."""
+    rng = Xoshiro256(seed)
+
+    def material():
+        return {"base_color": tuple(0.2 + 0.7 * rng.uniform() for _ in range(3)),
+                "base_metalness": 1.0 if rng.uniform() < 0.3 else 0.0,
+                "specular_roughness": 0.05 + 0.25 * rng.uniform()}
+
+    names = ["floor", "ceiling", "back", "left", "right", "short", "tall"]
+    gset = GeometrySet()
+    gset.meshes.append(quad_mesh([[-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]], [0, 1, 0], "floor"))
+    gset.meshes.append(quad_mesh([[-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1]], [0, -1, 0], "ceiling"))
+    gset.meshes.append(quad_mesh([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]], [0, 0, 1], "back"))
+    gset.meshes.append(quad_mesh([[-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]], [1, 0, 0], "left"))
+    gset.meshes.append(quad_mesh([[1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]], [-1, 0, 0], "right"))
+    short = GeometrySet(meshes=[box_mesh((0.6, 0.6, 0.6), "short")])
+    tall = GeometrySet(meshes=[box_mesh((0.6, 1.2, 0.6), "tall")])
+    gset.instances.append(Instance(short, placement(rotation_y(-17.0), (0.33, -0.7, 0.25))))
+    gset.instances.append(Instance(tall, placement(rotation_y(20.0), (-0.33, -0.4, -0.3))))
+    gset.materials = {name: material() for name in names}
+    if soup:
+        tri = random_triangles(soup, 1) if soup <= 20000 else random_triangles_fast(soup, 1)
+        tri.face_materials = [f"soup{t % 16}" for t in range(soup)]
+        gset.meshes.append(tri)
+        for k in range(16):
+            gset.materials[f"soup{k}"] = material()
+    # lights below the ceiling, each with a downward-facing occluder under it
+    h = 0.08
+    for i, (x, z) in enumerate([(x, z) for x in (-0.6, -0.2, 0.2, 0.6) for z in (-0.4, 0.2)]):
+        gset.lights.append(Light("sphere", placement(translation=(x, 0.9, z)), (1.0, 1.0, 1.0), light_radiance,
+                                 radius=0.02))
+        gset.meshes.append(quad_mesh([[x - h, 0.86, z - h], [x + h, 0.86, z - h], [x + h, 0.86, z + h],
+                                      [x - h, 0.86, z + h]], [0, -1, 0], "ceiling"))
+    gset.cameras.append(cornell_camera())
+    return gset
+
+
 def lamp_room_set():
     """The guiding workload of tests/gpu/integrator.lucb:362-389: a closed
     6 x 3 x 6 room (floor y = 0, ceiling y = 3, walls at x, z = +-3) lit only
