@@ -6,7 +6,8 @@
 // / shadow, finish, guide fit, film convert, end. Its launches never change
 // (sample and band live on the device), so the pass is recorded once into a
 // hipGraph and replayed; up to two passes are in flight, as advance keeps them
-// (integrator.lucb:651-655).
+// (integrator.lucb:651-655). With the learned guide, passes within its
+// training samples replay a second graph that also holds the training steps.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -40,12 +41,14 @@ struct lr_renderer {
     uint32_t completed = 0;
     std::deque<Pass> in_flight;
     double kernel_ns[KKinds] = {0};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};  // without, with the training steps
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 
     ~lr_renderer() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (int k = 0; k < 2; k++) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        }
         for (auto &ps : in_flight) {
             (void)hipEventDestroy(ps.done);
             if (ps.start) (void)hipEventDestroy(ps.start);
@@ -53,9 +56,9 @@ struct lr_renderer {
         }
     }
 
-    void record(hipStream_t s, std::vector<std::pair<int, hipEvent_t>> *marks) {
+    void record(hipStream_t s, std::vector<std::pair<int, hipEvent_t>> *marks, bool train) {
         std::vector<RenderJob *> one{&job};
-        record_pass(one, job.params_device, s, marks);
+        record_pass(one, job.params_device, s, marks, train);
     }
 
     // Retire finished passes (integrator.lucb:622-638).
@@ -85,24 +88,26 @@ struct lr_renderer {
     int launch_pass() {
         hipStream_t s = device->stream;
         Pass ps;
+        bool train = job.params.learned && sample < job.params.train_samples;
         if (settings.profile) {
             (void)hipEventCreate(&ps.start);
             (void)hipEventRecord(ps.start, s);
-            record(s, &ps.marks);
+            record(s, &ps.marks, train);
         } else if (settings.use_graph) {
-            if (!exec) {
+            int k = train ? 1 : 0;
+            if (!exec[k]) {
                 hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-                record(s, nullptr);
-                e = hipStreamEndCapture(s, &graph);
+                record(s, nullptr, train);
+                e = hipStreamEndCapture(s, &graph[k]);
                 if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                e = hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0);
                 if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
             }
-            hipError_t e = hipGraphLaunch(exec, s);
+            hipError_t e = hipGraphLaunch(exec[k], s);
             if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
         } else {
-            record(s, nullptr);
+            record(s, nullptr, train);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "pass launch");

@@ -183,22 +183,24 @@ static int batch(lr_guide_net *net, const float *samples, size_t n, float rate, 
     hipError_t e = hipMalloc(&dev, n * 40);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(batch)");
     e = hipMemcpyAsync(dev, samples, n * 40, hipMemcpyHostToDevice, s);
-    uint32_t waves = (uint32_t)std::min<size_t>(g.waves, (n + 15) / 16);
+    uint32_t waves = (uint32_t)std::min<size_t>(256, (n + 15) / 16);
+    uint32_t rows = std::min(waves, g.waves);
     ArraySamples src{(const float *)dev};
     if (e == hipSuccess) {
-        size_t words = (size_t)waves * (kParams + 1);
+        size_t words = (size_t)rows * (kParams + 1);
         hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, s, g.partials, words);
         hipLaunchKernelGGL((weights_kernel<ArraySamples>), dim3(64), dim3(256), 0, s, g.b, src, (uint32_t)n);
-        hipLaunchKernelGGL((grad_kernel<ArraySamples>), dim3(waves), dim3(64), 0, s, g.b, src, (uint32_t)n, g.partials);
+        hipLaunchKernelGGL((grad_kernel<ArraySamples>), dim3(waves), dim3(64), 0, s, g.b, src, (uint32_t)n, g.partials,
+                           rows);
         e = hipGetLastError();
     }
     std::vector<float> part;
     if (e == hipSuccess && (grad || loss)) {
-        part.resize((size_t)waves * (kParams + 1));
+        part.resize((size_t)rows * (kParams + 1));
         e = hipMemcpyAsync(part.data(), g.partials, part.size() * 4, hipMemcpyDeviceToHost, s);
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, g.b, g.partials, waves,
+        hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, g.b, g.partials, rows,
                            (uint32_t)n, rate);
         e = hipGetLastError();
     }
@@ -208,13 +210,13 @@ static int batch(lr_guide_net *net, const float *samples, size_t n, float rate, 
     if (grad) {
         for (size_t i = 0; i < kParams; i++) {
             double sum = 0.0;
-            for (uint32_t w = 0; w < waves; w++) sum += part[(size_t)w * (kParams + 1) + i];
+            for (uint32_t w = 0; w < rows; w++) sum += part[(size_t)w * (kParams + 1) + i];
             grad[i] = (float)(sum / (double)n);
         }
     }
     if (loss) {
         double sum = 0.0;
-        for (uint32_t w = 0; w < waves; w++) sum += part[(size_t)w * (kParams + 1) + kParams];
+        for (uint32_t w = 0; w < rows; w++) sum += part[(size_t)w * (kParams + 1) + kParams];
         *loss = (float)(sum / (double)n);
     }
     return 0;

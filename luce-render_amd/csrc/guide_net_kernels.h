@@ -21,33 +21,33 @@ struct Sample {
 constexpr float kMaxSampleScale = 1024.0f;  // per-sample weight over the batch mean, at most
 constexpr float kAdamBeta1 = 0.9f, kAdamBeta2 = 0.99f, kAdamEps = 1e-8f;
 
-// Loss of one sample, -c log q(d), and its gradient by the 20 used outputs
-// (the other 12 get 0). o: the 32 outputs.
-__device__ inline float sample_loss(const float *o, const float *d, float c, float *grad) {
-    Mixture mx = mixture(o);
-    float dens[kLobes], cosk[kLobes];
+// Loss of one sample, -c log q(d); the 32 outputs in `io` are replaced by
+// the loss's gradient by them (the 12 unused get 0).
+__device__ inline float sample_loss(float *io, const float *d, float c) {
+    Mixture mx = mixture(io);
+    float dens[kLobes], cosk[kLobes], len[kLobes], lk[kLobes];
     float pdf = 0.0f;
     for (uint32_t k = 0; k < kLobes; k++) {
         cosk[k] = mx.mu[k][0] * d[0] + mx.mu[k][1] * d[1] + mx.mu[k][2] * d[2];
         dens[k] = mx.w[k] * vmf(mx.kappa[k], cosk[k]);
         pdf += dens[k];
+        float x = io[5 * k], y = io[5 * k + 1], z = io[5 * k + 2];
+        len[k] = sqrtf(x * x + y * y + z * z);
+        lk[k] = io[5 * k + 3];
     }
-    for (uint32_t i = 0; i < kOut; i++) grad[i] = 0.0f;
+    for (uint32_t i = 0; i < kOut; i++) io[i] = 0.0f;
     if (!(pdf > 1e-30f) || c == 0.0f) return 0.0f;
     for (uint32_t k = 0; k < kLobes; k++) {
         float resp = dens[k] / pdf;
         float kappa = mx.kappa[k];
-        grad[5 * k + 4] = -c * (resp - mx.w[k]);
-        float lk = o[5 * k + 3];
-        if (lk > kLogKappaLo && lk < kLogKappaHi) {
+        io[5 * k + 4] = -c * (resp - mx.w[k]);
+        if (lk[k] > kLogKappaLo && lk[k] < kLogKappaHi) {
             float t = 2.0f * kappa / expm1f(2.0f * kappa);
-            grad[5 * k + 3] = -c * resp * (1.0f - t + kappa * (cosk[k] - 1.0f));
+            io[5 * k + 3] = -c * resp * (1.0f - t + kappa * (cosk[k] - 1.0f));
         }
-        float x = o[5 * k], y = o[5 * k + 1], z = o[5 * k + 2];
-        float len = sqrtf(x * x + y * y + z * z);
-        if (len >= 1e-6f) {
-            float s = -c * resp * kappa / len;
-            for (uint32_t a = 0; a < 3u; a++) grad[5 * k + a] = s * (d[a] - cosk[k] * mx.mu[k][a]);
+        if (len[k] >= 1e-6f) {
+            float s = -c * resp * kappa / len[k];
+            for (uint32_t a = 0; a < 3u; a++) io[5 * k + a] = s * (d[a] - cosk[k] * mx.mu[k][a]);
         }
     }
     return -c * logf(pdf);
@@ -80,38 +80,58 @@ __global__ __launch_bounds__(256) void weights_kernel(NetBuffers net, Source src
 }
 
 // One wave a workgroup. Each wave takes groups of 16 samples (stride: the
-// grid) and adds its gradient sum into its own row of `partials` (kParams + 1
-// floats: the last the loss), which must start at zero.
+// grid) and adds its gradient sums into row blockIdx.x % rows of `partials`
+// (kParams + 1 floats: the last the loss), which must start at zero. The
+// adds are atomics whose results nobody waits for, so a wave never stalls on
+// them. Pre-activations wait in LDS for the backward pass, keeping the
+// registers to one layer's tiles.
 template <class Source>
-__device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t n, float *partials) {
-    __shared__ float hs[16][kIn];
-    __shared__ float ds[16][kHidden];
-    __shared__ float os[16][kOut];
+__device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t n, float *partials, uint32_t rows) {
+    __shared__ float xs[16][kIn];          // the encodings
+    __shared__ float zs[3][16][kHidden];   // hidden pre-activations
+    __shared__ float ds[16][kHidden];      // dZ of the layer being differentiated
+    __shared__ float os[16][kOut];         // outputs, then their gradients
     const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
-    float *part = partials + (size_t)blockIdx.x * (kParams + 1u);
+    float *part = partials + (size_t)(blockIdx.x % rows) * (kParams + 1u);
     float mean = net.stats[0] / (float)(n > 0 ? n : 1);
     float loss = 0.0f;
     for (uint32_t base = blockIdx.x * 16u; base < n; base += gridDim.x * 16u) {
         Sample s = src.sample(base + j < n ? base + j : n - 1u);
         bool valid = base + j < n && s.valid && mean > 0.0f;
         float c = valid ? fminf(s.weight / mean, kMaxSampleScale) : 0.0f;
-        half4_t x[4];
-        encode(net.bounds, s.p[0], s.p[1], s.p[2], s.n[0], s.n[1], s.n[2], g, x);
-        Act z[3];
-        float4_t o[2];
-        forward(net.fwd, net.params, x, lane, z, o);
-        // outputs to LDS; one lane per sample forms the loss gradient
+        half4_t h[4];
+        encode(net.bounds, s.p[0], s.p[1], s.p[2], s.n[0], s.n[1], s.n[2], g, h);
 #pragma unroll
-        for (uint32_t m = 0; m < 2u; m++)
+        for (uint32_t m = 0; m < 4u; m++)
 #pragma unroll
-            for (uint32_t r = 0; r < 4u; r++) os[j][16u * m + 4u * g + r] = o[m][r];
-        __syncthreads();
-        if (g == 0u) {
-            float out[kOut], grad[kOut];
-            for (uint32_t i = 0; i < kOut; i++) out[i] = os[j][i];
-            loss += sample_loss(out, s.d, c, grad);
-            for (uint32_t i = 0; i < kOut; i++) os[j][i] = grad[i];
+            for (uint32_t r = 0; r < 4u; r++) xs[j][16u * m + 4u * g + r] = (float)h[m][r];
+        // forward, layer by layer
+#pragma unroll 1
+        for (uint32_t l = 0; l < 3u; l++) {
+            float4_t z[4];
+            layer_forward(net.fwd, net.params, l, h, lane, z);
+#pragma unroll
+            for (uint32_t m = 0; m < 4u; m++) {
+                float4_t a = z[m];
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; r++) {
+                    zs[l][j][16u * m + 4u * g + r] = a[r];
+                    a[r] = fmaxf(a[r], 0.0f);
+                }
+                h[m] = to_half(a);
+            }
         }
+        {
+            float4_t o[4];
+            layer_forward(net.fwd, net.params, 3u, h, lane, o);
+#pragma unroll
+            for (uint32_t m = 0; m < 2u; m++)
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; r++) os[j][16u * m + 4u * g + r] = o[m][r];
+        }
+        __syncthreads();
+        // one lane per sample forms the loss gradient
+        if (g == 0u) loss += sample_loss(os[j], s.d, c);
         __syncthreads();
         // dZ of the output layer, as tiles (D layout)
         float4_t dz[4];
@@ -119,69 +139,69 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
         for (uint32_t m = 0; m < 2u; m++)
 #pragma unroll
             for (uint32_t r = 0; r < 4u; r++) dz[m][r] = os[j][16u * m + 4u * g + r];
+        dz[2] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        dz[3] = dz[2];
+#pragma unroll 1
         for (int l = 3; l >= 0; l--) {
             const uint32_t outs = layer_out((uint32_t)l), steps = outs / 16u;
-            // the layer's input H (relu of the previous pre-activation, or the encoding)
 #pragma unroll
             for (uint32_t m = 0; m < 4u; m++)
+                if (m < steps)
 #pragma unroll
-                for (uint32_t r = 0; r < 4u; r++)
-                    hs[j][16u * m + 4u * g + r] = l == 0 ? (float)x[m][r] : fmaxf(z[l - 1].t[m][r], 0.0f);
-            for (uint32_t m = 0; m < steps; m++)
-#pragma unroll
-                for (uint32_t r = 0; r < 4u; r++) ds[j][16u * m + 4u * g + r] = dz[m][r];
+                    for (uint32_t r = 0; r < 4u; r++) ds[j][16u * m + 4u * g + r] = dz[m][r];
             __syncthreads();
             // dW = H^T dZ: rows (inputs) by tile m, columns (outputs) by tile q,
-            // the 16 samples as the product's inner dimension
+            // the 16 samples as the product's inner dimension; H the layer's
+            // input (the encoding, or the ReLU of the previous pre-activation)
+#pragma unroll 1
             for (uint32_t m = 0; m < 4u; m++) {
                 half4_t a;
 #pragma unroll
-                for (uint32_t r = 0; r < 4u; r++) a[r] = (_Float16)hs[4u * g + r][16u * m + j];
+                for (uint32_t r = 0; r < 4u; r++)
+                    a[r] = (_Float16)(l == 0 ? xs[4u * g + r][16u * m + j] : fmaxf(zs[l - 1][4u * g + r][16u * m + j], 0.0f));
                 for (uint32_t q = 0; q < steps; q++) {
                     half4_t b;
 #pragma unroll
                     for (uint32_t r = 0; r < 4u; r++) b[r] = (_Float16)ds[4u * g + r][16u * q + j];
                     float4_t w = mfma(a, b, float4_t{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
-                    for (uint32_t r = 0; r < 4u; r++) {
-                        float *at = part + weight_offset((uint32_t)l) + (16u * m + 4u * g + r) * outs + 16u * q + j;
-                        *at += w[r];
-                    }
+                    for (uint32_t r = 0; r < 4u; r++)
+                        atomicAdd(part + weight_offset((uint32_t)l) + (16u * m + 4u * g + r) * outs + 16u * q + j, w[r]);
                 }
             }
             // db = sum of dZ over the samples
             if (lane < outs) {
                 float sb = 0.0f;
                 for (uint32_t k = 0; k < 16u; k++) sb += ds[k][lane];
-                part[bias_offset((uint32_t)l) + lane] += sb;
+                atomicAdd(&part[bias_offset((uint32_t)l) + lane], sb);
             }
             if (l > 0) {
                 // dH^T = W dZ^T, then through the ReLU of layer l - 1
                 half4_t b[4];
-                for (uint32_t q = 0; q < steps; q++) b[q] = to_half(dz[q]);
-                float4_t dh[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4u; q++) b[q] = to_half(dz[q]);
+#pragma unroll
                 for (uint32_t m = 0; m < 4u; m++) {
                     float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                    for (uint32_t q = 0; q < steps; q++)
-                        acc = mfma(load_frag(net.bwd, tile_base((uint32_t)l) + m * steps + q, lane), b[q], acc);
-                    dh[m] = acc;
+#pragma unroll
+                    for (uint32_t q = 0; q < 4u; q++)
+                        if (q < steps) acc = mfma(load_frag(net.bwd, tile_base((uint32_t)l) + m * steps + q, lane), b[q], acc);
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; r++) dz[m][r] = zs[l - 1][j][16u * m + 4u * g + r] > 0.0f ? acc[r] : 0.0f;
                 }
-#pragma unroll
-                for (uint32_t m = 0; m < 4u; m++)
-#pragma unroll
-                    for (uint32_t r = 0; r < 4u; r++) dz[m][r] = z[l - 1].t[m][r] > 0.0f ? dh[m][r] : 0.0f;
             }
             __syncthreads();
         }
     }
     // the wave's loss
     for (uint32_t off = 32u; off > 0; off >>= 1) loss += __shfl_xor(loss, (int)off);
-    if (lane == 0) part[kParams] += loss;
+    if (lane == 0) atomicAdd(&part[kParams], loss);
 }
 
 template <class Source>
-__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials) {
-    grad_body(net, src, n, partials);
+__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials,
+                                                  uint32_t rows) {
+    grad_body(net, src, n, partials, rows);
 }
 
 // Writes parameter i (f32) and its FP16 fragments.

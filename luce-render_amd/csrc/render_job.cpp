@@ -79,7 +79,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     b += (size_t)6 * s.pixels * 16;  // film (4 planes), image, display
     if (st->guiding == 1) b += rt::kGuideBytes;
     if (st->guiding == 2)
-        b += (size_t)5 * s.slots * 16 + (size_t)kNetWaves * (net::kParams + 1) * 4 + (size_t)net::kParams * 12 +
+        b += (size_t)5 * s.slots * 16 + (size_t)kNetRows * (net::kParams + 1) * 4 + (size_t)net::kParams * 12 +
              (size_t)net::kTiles * 64 * 16 + 128;
     uint32_t w = st->trace_width ? st->trace_width : scene->config.width;
     uint32_t stack = st->trace_stack ? st->trace_stack : scene->config.stack;
@@ -176,7 +176,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     if (st.guiding == 1) p.guide = (uint8_t *)alloc(rt::kGuideBytes, rc);
     if (st.guiding == 2) {
         net.reset(new GuideNet());
-        int nrc = net->create(dev, sc->root_lo, sc->root_hi, st.seed, kNetWaves);
+        int nrc = net->create(dev, sc->root_lo, sc->root_hi, st.seed, kNetRows);
         if (nrc) return nrc;
         p.net = net->b;
         p.net_partials = net->partials;
@@ -217,7 +217,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
 }
 
 void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params, hipStream_t s,
-                 std::vector<std::pair<int, hipEvent_t>> *marks) {
+                 std::vector<std::pair<int, hipEvent_t>> *marks, bool train) {
     if (jobs.empty()) return;
     uint32_t n = (uint32_t)jobs.size();
     bool sp = jobs[0]->settings.spectral != 0;
@@ -279,7 +279,7 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     }
     launch_render_stage(Stage::Finish, sp, params, n, 0, blocks(256, band), s);
     mark(KFinish);
-    if (learned) {
+    if (learned && train) {
         launch_learned_train(params, sp, jobs[0]->train_steps, s);
         mark(KRest);
     }

@@ -50,8 +50,10 @@ uint32_t grid_blocks(uint64_t items, uint32_t group);
 // y is render y), their parameters at `params` (a device array in job
 // order). All jobs share the spectral mode and traversal variant. With
 // `marks`, an event after each stage, tagged by kernel kind.
+// `train`: include the learned guide's training steps (a single render still
+// within its training samples; the kernels also check on the device).
 void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params, hipStream_t s,
-                 std::vector<std::pair<int, hipEvent_t>> *marks);
+                 std::vector<std::pair<int, hipEvent_t>> *marks, bool train = true);
 
 enum KernelKind { KCamera, KClosest, KSurface, KMiss, KShadow, KLight, KFinish, KConvert, KAdaptive, KRest, KKinds };
 

@@ -7,8 +7,10 @@
 
 namespace lr {
 
-// Gradient rows of a training step (one wave each).
+// Waves of a training step's gradient kernel, and the partial rows they add
+// into (wave w into row w % kNetRows).
 constexpr uint32_t kNetWaves = 256u;
+constexpr uint32_t kNetRows = 32u;
 
 // Mixtures for this bounce's surface queue of the render at p[0] (`band`:
 // its camera paths, an upper bound of the queue).

@@ -104,14 +104,14 @@ template <bool SPECTRAL> __global__ __launch_bounds__(64) void learned_grad_kern
     if (!training(p)) return;
     net::NetBuffers n = p.net;
     PathRecords<SPECTRAL> src{p, salt(p, step)};
-    net::grad_body(n, src, p.train_batch, p.net_partials);
+    net::grad_body(n, src, p.train_batch, p.net_partials, kNetRows);
 }
 
 __global__ __launch_bounds__(256) void learned_adam_kernel(const Params *plist) {
     const Params &p = plist[0];
     if (!training(p)) return;
     net::NetBuffers n = p.net;
-    net::adam_body(n, p.net_partials, kNetWaves, p.train_batch, p.train_rate, true);
+    net::adam_body(n, p.net_partials, kNetRows, p.train_batch, p.train_rate, true);
 }
 
 }  // namespace rt
