@@ -303,7 +303,7 @@ def main():
         guiding_stats = {}
         for name, gset, w, h, rspp, kw in (
                 ("glossy", scenes.glossy_set(), gw, gh, int(ref_spp), dict(max_bounces=8, light_tracing=0.5)),
-                ("lamp_room", scenes.lamp_room_set(), 256, 192, 4096,
+                ("lamp_room", scenes.lamp_room_set(), 960, 720, 1024,
                  dict(max_bounces=2, light_tracing=0.0,
                       material=OpenPbr(base_color=(0.3, 0.3, 0.3), specular_weight=0.0)))):
             guiding_stats[name] = guiding_leg(device, stream, render, gset, w, h, rspp, kw)
