@@ -223,6 +223,9 @@ typedef struct lr_render_stats {
 } lr_render_stats;
 int lr_renderer_stats(lr_renderer *renderer, lr_render_stats *out);
 /* Renderer.destroy (:887): waits for passes in flight. */
+/* Renderer.converged_pixels (integrator.lucb:759): pixels adaptive sampling
+ * has stopped (0 without a noise threshold); waits for passes in flight. */
+int lr_renderer_converged_pixels(lr_renderer *renderer, size_t *out);
 void lr_renderer_destroy(lr_renderer *renderer);
 
 /* ------------------------------------------------------------------- serving */

@@ -236,6 +236,25 @@ int lr_renderer_stats(lr_renderer *r, lr_render_stats *out) {
     return 0;
 }
 
+int lr_renderer_converged_pixels(lr_renderer *r, size_t *out) {
+    if (!r || !out) return fail(LR_INVALID, "lr_renderer_converged_pixels needs a renderer and an output");
+    int rc = r->retire(true);
+    if (rc) return rc;
+    hipStream_t s = r->device->stream;
+    uint32_t *counter = nullptr;
+    hipError_t e = hipMalloc(&counter, 64);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(converged count)");
+    launch_fill(counter, 1, 0u, s);
+    launch_converged(r->job.params_device, r->job.params.pixels, counter, s);
+    uint32_t count = 0;
+    e = hipMemcpyAsync(&count, counter, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(counter);
+    if (e != hipSuccess) return hip_fail(e, "converged count");
+    *out = count;
+    return 0;
+}
+
 void lr_renderer_destroy(lr_renderer *r) {
     if (!r) return;
     (void)hipSetDevice(r->device->ordinal);

@@ -119,6 +119,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     slots = sh.slots;
     bands = sh.bands;
     rounds = st.max_bounces + 1 + loaded->through;
+    adaptive = loaded->constants[44] > 0.0f;  // the noise threshold (loading.lucb:267)
     int rc = 0;
     rt::Params &p = params;
     std::memcpy(p.frame, frame, 64);
@@ -243,6 +244,8 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     auto light_paths = [](RenderJob *j) -> uint64_t { return j->params.light_paths; };
     auto slots = [](RenderJob *j) -> uint64_t { return j->slots; };
     auto pixels = [](RenderJob *j) -> uint64_t { return j->params.pixels; };
+    bool adaptive = false;
+    for (RenderJob *j : jobs) adaptive = adaptive || j->adaptive;
     auto mark = [&](int kind) {
         if (!marks) return;
         hipEvent_t e;
@@ -286,6 +289,11 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     if (guide) {
         launch_render_stage(Stage::GuideFit, sp, params, n, 0, 2048, s);
         mark(KRest);
+    }
+    if (adaptive) {
+        launch_render_stage(Stage::Adaptive, sp, params, n, 0, blocks(256, pixels), s);
+        launch_render_stage(Stage::Adaptive, sp, params, n, 1, blocks(256, pixels), s);
+        mark(KAdaptive);
     }
     launch_render_stage(Stage::Convert, sp, params, n, 0, blocks(256, pixels), s);
     mark(KConvert);

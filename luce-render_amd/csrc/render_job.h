@@ -31,6 +31,7 @@ struct RenderJob {
     std::vector<void *> buffers;
     std::unique_ptr<GuideNet> net;  // the learned guide (guiding 2)
     uint32_t train_steps = 0;
+    bool adaptive = false;  // a noise threshold is set: passes check convergence
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);

@@ -7,7 +7,7 @@
 
 namespace lr {
 
-enum class Stage { BeginPass, Camera, LightEmit, Route, Shade, ShadeLight, Miss, Finish, GuideFit, Convert, EndPass };
+enum class Stage { BeginPass, Camera, LightEmit, Route, Shade, ShadeLight, Miss, Finish, GuideFit, Adaptive, Convert, EndPass };
 
 // The tracing kernels of a pass for one (width, stack, group) variant.
 struct RenderTraceVariant {
@@ -21,12 +21,14 @@ const RenderTraceVariant *render_trace_variants();
 
 // `p`: a device array of `n` (at most rt::kMaxRows) requests' parameters;
 // `blocks` workgroups share every request's items (1-D grid).
-// `arg`: the closest-hit queue being read (Route, Shade, ShadeLight) or
-// "convert every pass" (Convert).
+// `arg`: the closest-hit queue being read (Route, Shade, ShadeLight), the
+// phase (Adaptive) or "convert every pass" (Convert).
 void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks,
                          hipStream_t s);
 void launch_tables(float *tables, hipStream_t s);
 void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s);
+// Adds the render at p[0]'s converged pixels into *out.
+void launch_converged(const rt::Params *p, uint32_t pixels, uint32_t *out, hipStream_t s);
 // Bounding spheres of `n` analytic lights for culling: ceil(n / 8) + n float4.
 void launch_light_bounds(const float4 *lights, uint32_t n, float4 *out, hipStream_t s);
 

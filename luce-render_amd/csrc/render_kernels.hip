@@ -1093,6 +1093,59 @@ __global__ __launch_bounds__(256) void guide_fit_kernel(const Params *plist) {
     }
 }
 
+// Adaptive sampling's convergence check (adaptive.comp:14-40; Dammertz et
+// al.): phase 0 gives each pixel its own verdict, |full - half| / (1e-4 +
+// sqrt(brightness)) below the noise threshold, in the scratch plane; phase 1
+// marks a pixel converged (FILM_HALF.w = 1, which the camera kernel skips)
+// when its whole 3x3 neighbourhood is. Runs after the last band of every 4th
+// sample from the adaptive minimum on (integrator.lucb:818-826).
+__device__ __forceinline__ bool adaptive_now(const Params &p) {
+    const PassState &ps = *p.pass;
+    uint32_t taken = ps.sample + 1u;
+    return K(p, K_ADAPTIVE).x > 0.0f && ps.band + 1u == ps.bands && (float)taken >= K(p, K_ADAPTIVE).y &&
+           taken % 4u == 0u;
+}
+
+__global__ __launch_bounds__(256) void adaptive_kernel(const Params *plist, uint32_t n, uint32_t phase) {
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return adaptive_now(q) ? q.pixels : 0u; });
+    FOR_ROW_ITEMS(plan, n, total, g, r, pixel, active) {
+        if (!active) continue;
+        const Params &p = plist[r];
+        if (phase == 0u) {
+            float4 sum = p.film[pixel], half = p.film[(size_t)FILM_HALF * p.pixels + pixel];
+            float scale = 1.0f / fmaxf(sum.w, 1.0f);
+            float difference = (fabsf(sum.x - half.x) + fabsf(sum.y - half.y) + fabsf(sum.z - half.z)) * scale;
+            float brightness = fmaxf(0.0f, (sum.x + sum.y + sum.z) * scale);
+            float error = difference / (0.0001f + sqrtf(brightness));
+            p.film[(size_t)FILM_SCRATCH * p.pixels + pixel] =
+                make_float4(error < K(p, K_ADAPTIVE).x ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            int w = (int)p.width, h = (int)p.height;
+            int x = (int)(pixel % p.width), y = (int)(pixel / p.width);
+            float converged = 1.0f;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    int nx = x + dx, ny = y + dy;
+                    if (nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+                    converged = fminf(converged, p.film[(size_t)FILM_SCRATCH * p.pixels + (size_t)ny * w + nx].x);
+                }
+            float4 &half = p.film[(size_t)FILM_HALF * p.pixels + pixel];
+            half.w = converged;
+        }
+    }
+}
+
+// Pixels marked converged, added into *out.
+__global__ __launch_bounds__(256) void converged_kernel(const Params *plist, uint32_t *out) {
+    const Params &p = plist[0];
+    uint32_t count = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.pixels; i += gridDim.x * blockDim.x)
+        count += p.film[(size_t)FILM_HALF * p.pixels + i].w > 0.5f ? 1u : 0u;
+    for (uint32_t off = 32u; off > 0; off >>= 1) count += __shfl_xor(count, (int)off);
+    if ((threadIdx.x & 63u) == 0u && count) atomicAdd(out, count);
+}
+
 // Khronos PBR Neutral (film_convert.comp:21-34).
 __device__ __forceinline__ V3 neutral(V3 c) {
     const float start = 0.8f - 0.04f, desaturation = 0.15f;
@@ -1292,6 +1345,7 @@ static void launch_stage(Stage stage, const rt::Params *p, uint32_t n, uint32_t 
         case Stage::Miss: LR_STAGE((rt::miss_kernel<SPECTRAL>), 256, p, n); break;
         case Stage::Finish: LR_STAGE((rt::finish_kernel<SPECTRAL>), 256, p, n); break;
         case Stage::Convert: LR_STAGE((rt::film_convert_kernel<SPECTRAL>), 256, p, n, arg); break;
+        case Stage::Adaptive: LR_STAGE(rt::adaptive_kernel, 256, p, n, arg); break;
         default: break;
     }
 #undef LR_STAGE
@@ -1314,6 +1368,11 @@ void launch_tables(float *tables, hipStream_t s) {
     uint32_t cells = rt::TABLE_MU * rt::TABLE_ROUGHNESS * (1u + rt::TABLE_ETA);
     hipLaunchKernelGGL(rt::tables_kernel, dim3((cells + 63) / 64), dim3(64), 0, s, tables, 0u);
     hipLaunchKernelGGL(rt::tables_kernel, dim3(1), dim3(64), 0, s, tables, 1u);
+}
+
+void launch_converged(const rt::Params *p, uint32_t pixels, uint32_t *out, hipStream_t s) {
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)pixels + 255) / 256, 4096);
+    hipLaunchKernelGGL(rt::converged_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, s, p, out);
 }
 
 void launch_light_bounds(const float4 *lights, uint32_t n, float4 *out, hipStream_t s) {

@@ -422,6 +422,12 @@ class Renderer:
         _lib.check(_lib.lib().lr_renderer_kernel_times(self.handle, _lib.ptr(out)))
         return dict(zip(KERNEL_NAMES, out.tolist()))
 
+    def converged_pixels(self):
+        """Renderer.converged_pixels (integrator.lucb:759)."""
+        out = ctypes.c_size_t()
+        _lib.check(_lib.lib().lr_renderer_converged_pixels(self.handle, ctypes.byref(out)))
+        return out.value
+
     def stats(self):
         st = _lib.RenderStats()
         _lib.check(_lib.lib().lr_renderer_stats(self.handle, ctypes.byref(st)))

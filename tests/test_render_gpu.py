@@ -181,3 +181,32 @@ def test_renderer_lifecycle(device):
     bad.max_bounces = 0
     with pytest.raises(render.LuceError):
         render.Renderer.create(device, sc, bad)
+
+
+def test_adaptive_sampling_stops_converged_pixels(device):
+    """adaptive.comp with a noise threshold: the furnace's constant background
+    has no variance, so its pixels converge at the first check (16 samples)
+    and stop sampling, while the image keeps its values (each pixel the mean
+    of the samples it took): background 1, sphere 0.8 (within 0.01), and
+    fewer rays than the same render without the threshold."""
+    from luce_render import render, scenes
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(scenes.furnace_set())
+    out = {}
+    for threshold in (0.0, 0.05):
+        st = RenderSettings(64, 48, 64, max_bounces=8, spectral=False, background=(1.0, 1.0, 1.0),
+                            noise_threshold=threshold, adaptive_minimum=16, light_tracing=0.0)
+        r = render.Renderer.create(device, sc, st)
+        try:
+            r.finish()
+            out[threshold] = (r.converged_pixels(), r.read().copy(), r.stats())
+        finally:
+            r.destroy()
+    conv0, _, stats0 = out[0.0]
+    conv, img, stats = out[0.05]
+    say(f"adaptive: converged {conv} of {64 * 48}, rays {stats['closest_rays']} vs {stats0['closest_rays']}")
+    assert conv0 == 0
+    assert conv > 64 * 48 // 4
+    assert stats["closest_rays"] < stats0["closest_rays"]
+    assert np.all(np.abs(img[0:6, 0:6, :3] - 1.0) < 0.01)
+    assert np.all(np.abs(img[16:32, 24:40, :3].mean(axis=(0, 1)) - 0.8) < 0.01)
