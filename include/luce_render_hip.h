@@ -223,6 +223,15 @@ typedef struct lr_render_stats {
 } lr_render_stats;
 int lr_renderer_stats(lr_renderer *renderer, lr_render_stats *out);
 /* Renderer.destroy (:887): waits for passes in flight. */
+/* Renderer.restart / reload / replace_scene (integrator.lucb:463-597): start
+ * over with new loaded arrays, settings and frame, and with `scene` in place of
+ * the renderer's when not NULL (which must then outlive the renderer). Passes
+ * in flight finish and count for nothing; the film, sample count and kernel
+ * times start over. The projection (perspective or orthographic) is kept.
+ * The renderer's device state is made anew (the image and display pointers
+ * change); on failure the renderer is left as it was. */
+int lr_renderer_reset(lr_renderer *renderer, lr_gpu_scene *scene, const lr_loaded *loaded,
+                      const lr_render_settings *settings, const float frame[16]);
 /* Renderer.converged_pixels (integrator.lucb:759): pixels adaptive sampling
  * has stopped (0 without a noise threshold); waits for passes in flight. */
 int lr_renderer_converged_pixels(lr_renderer *renderer, size_t *out);

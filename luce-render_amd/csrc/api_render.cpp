@@ -33,7 +33,7 @@ struct Pass {
 }  // namespace
 
 struct lr_renderer {
-    RenderJob job;
+    std::unique_ptr<RenderJob> job{new RenderJob()};
     lr_device *device = nullptr;
     lr_render_settings settings{};
     // host mirror of the device pass state
@@ -56,9 +56,18 @@ struct lr_renderer {
         }
     }
 
+    void drop_graphs() {
+        for (int k = 0; k < 2; k++) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            exec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+    }
+
     void record(hipStream_t s, std::vector<std::pair<int, hipEvent_t>> *marks, bool train) {
-        std::vector<RenderJob *> one{&job};
-        record_pass(one, job.params_device, s, marks, train);
+        std::vector<RenderJob *> one{job.get()};
+        record_pass(one, job->params_device, s, marks, train);
     }
 
     // Retire finished passes (integrator.lucb:622-638).
@@ -88,7 +97,7 @@ struct lr_renderer {
     int launch_pass() {
         hipStream_t s = device->stream;
         Pass ps;
-        bool train = job.params.learned && sample < job.params.train_samples;
+        bool train = job->params.learned && sample < job->params.train_samples;
         if (settings.profile) {
             (void)hipEventCreate(&ps.start);
             (void)hipEventRecord(ps.start, s);
@@ -112,7 +121,7 @@ struct lr_renderer {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "pass launch");
         band += 1;
-        if (band == job.bands) {
+        if (band == job->bands) {
             band = 0;
             sample += 1;
         }
@@ -133,10 +142,10 @@ int lr_renderer_create(lr_device *device, lr_gpu_scene *scene, const lr_loaded *
     *out = nullptr;
     (void)hipSetDevice(device->ordinal);
     std::unique_ptr<lr_renderer> r(new lr_renderer());
-    int rc = r->job.create(device, scene, loaded, settings, frame, nullptr);
+    int rc = r->job->create(device, scene, loaded, settings, frame, nullptr);
     if (rc) return rc;
     r->device = device;
-    r->settings = r->job.settings;
+    r->settings = r->job->settings;
     *out = r.release();
     return 0;
 }
@@ -183,8 +192,8 @@ uint32_t lr_renderer_samples_done(lr_renderer *r) {
 
 int lr_renderer_image(lr_renderer *r, lr_image *out) {
     if (!r || !out) return fail(LR_INVALID, "lr_renderer_image needs a renderer and an output");
-    out->rgba = r->job.params.image;
-    out->display = r->job.params.display;
+    out->rgba = r->job->params.image;
+    out->display = r->job->params.display;
     out->width = r->settings.width;
     out->height = r->settings.height;
     return 0;
@@ -196,7 +205,7 @@ int lr_renderer_read(lr_renderer *r, float *rgba, size_t count) {
     if (count != pixels * 4) return fail(LR_INVALID, "reading the image needs four floats a pixel");
     int rc = r->retire(true);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(rgba, r->job.params.image, pixels * 16, hipMemcpyDeviceToHost, r->device->stream);
+    hipError_t e = hipMemcpyAsync(rgba, r->job->params.image, pixels * 16, hipMemcpyDeviceToHost, r->device->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
     if (e != hipSuccess) return hip_fail(e, "image read");
     return 0;
@@ -208,7 +217,7 @@ int lr_renderer_read_film(lr_renderer *r, float *planes, size_t count) {
     if (count != pixels * 16) return fail(LR_INVALID, "the film is four planes of four floats a pixel");
     int rc = r->retire(true);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(planes, r->job.params.film, pixels * 64, hipMemcpyDeviceToHost, r->device->stream);
+    hipError_t e = hipMemcpyAsync(planes, r->job->params.film, pixels * 64, hipMemcpyDeviceToHost, r->device->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
     if (e != hipSuccess) return hip_fail(e, "film read");
     return 0;
@@ -226,13 +235,34 @@ int lr_renderer_stats(lr_renderer *r, lr_render_stats *out) {
     int rc = r->retire(true);
     if (rc) return rc;
     rt::PassState ps;
-    hipError_t e = hipMemcpyAsync(&ps, r->job.params.pass, sizeof(ps), hipMemcpyDeviceToHost, r->device->stream);
+    hipError_t e = hipMemcpyAsync(&ps, r->job->params.pass, sizeof(ps), hipMemcpyDeviceToHost, r->device->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
     if (e != hipSuccess) return hip_fail(e, "stats read");
     out->samples_done = ps.samples_done;
     out->light_samples = ps.light_samples;
     out->closest_rays = ps.closest_rays;
     out->shadow_rays = ps.shadow_rays;
+    return 0;
+}
+
+int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *loaded, const lr_render_settings *settings,
+                      const float frame[16]) {
+    if (!r || !loaded || !settings || !frame) return fail(LR_INVALID, "lr_renderer_reset needs its arguments");
+    if ((frame[3] > 0.5f) != (r->job->params.frame[3] > 0.5f)) return fail(LR_INVALID, "a restart keeps the view's projection");
+    (void)hipSetDevice(r->device->ordinal);
+    // Passes in flight finish and count for nothing.
+    int rc = r->retire(true);
+    if (rc) return rc;
+    // The new state is made before the old goes, so a failure leaves the
+    // renderer as it was.
+    std::unique_ptr<RenderJob> job(new RenderJob());
+    rc = job->create(r->device, scene ? scene : r->job->scene, loaded, settings, frame, nullptr);
+    if (rc) return rc;
+    r->drop_graphs();
+    r->job.swap(job);
+    r->settings = r->job->settings;
+    r->sample = r->band = r->completed = 0;
+    for (double &k : r->kernel_ns) k = 0.0;
     return 0;
 }
 
@@ -245,7 +275,7 @@ int lr_renderer_converged_pixels(lr_renderer *r, size_t *out) {
     hipError_t e = hipMalloc(&counter, 64);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(converged count)");
     launch_fill(counter, 1, 0u, s);
-    launch_converged(r->job.params_device, r->job.params.pixels, counter, s);
+    launch_converged(r->job->params_device, r->job->params.pixels, counter, s);
     uint32_t count = 0;
     e = hipMemcpyAsync(&count, counter, 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);

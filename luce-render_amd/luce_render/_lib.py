@@ -128,6 +128,7 @@ _SIGNATURES = {
     "lr_event_record": (ctypes.c_int, [_P, _P]),
     "lr_event_elapsed": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
     "lr_event_destroy": (None, [_P]),
+    "lr_renderer_reset": (ctypes.c_int, [_P, _P, ctypes.POINTER(Loaded), ctypes.POINTER(RenderSettingsC), _P]),
     "lr_renderer_converged_pixels": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "lr_guide_net_create": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.POINTER(_P)]),
     "lr_guide_net_destroy": (None, [_P]),

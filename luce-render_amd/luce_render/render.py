@@ -359,22 +359,10 @@ class Renderer:
             raise LuceError(INVALID, "a render needs samples and 1 to 1024 bounces")
         r = Renderer()
         r.device = device
-        r.settings = replace(settings, width=frame.width, height=frame.height)
-        r.frame = frame
+        r.scene = scene
         r.owned_scene = gpu_scene is None
         r.gpu_scene = gpu_scene if gpu_scene is not None else GpuScene.upload(device, scene)
-        r.loaded = Loaded(scene, r.settings, frame)
-        ld = _lib.Loaded()
-        ld.lights, ld.light_floats = _lib.ptr(r.loaded.lights), len(r.loaded.lights)
-        ld.constants = _lib.ptr(r.loaded.constants)
-        ld.materials, ld.material_floats = _lib.ptr(r.loaded.materials), len(r.loaded.materials)
-        ld.emitters, ld.emitter_words = _lib.ptr(r.loaded.emitters), len(r.loaded.emitters)
-        ld.light_paths, ld.through = r.loaded.light_paths, r.loaded.through
-        s = r.settings
-        cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, s.pool, s.seed,
-                                  int(s.guiding), 1 if s.profile else 0, 1 if s.use_graph else 0,
-                                  *tuple(s.trace)[:3], s.train_samples, s.train_steps)
-        r.frame16 = frame.pack()
+        ld, cs = r._prepare(scene, replace(settings, width=frame.width, height=frame.height), frame)
         h = ctypes.c_void_p()
         try:
             _lib.check(_lib.lib().lr_renderer_create(device.handle, r.gpu_scene.handle, ctypes.byref(ld),
@@ -385,6 +373,74 @@ class Renderer:
             raise
         r.handle = h
         return r
+
+    def _prepare(self, scene, settings, frame):
+        """The loaded arrays and settings the C ABI takes (kept alive on self)."""
+        self.settings = settings
+        self.frame = frame
+        self.loaded = Loaded(scene, settings, frame)
+        ld = _lib.Loaded()
+        ld.lights, ld.light_floats = _lib.ptr(self.loaded.lights), len(self.loaded.lights)
+        ld.constants = _lib.ptr(self.loaded.constants)
+        ld.materials, ld.material_floats = _lib.ptr(self.loaded.materials), len(self.loaded.materials)
+        ld.emitters, ld.emitter_words = _lib.ptr(self.loaded.emitters), len(self.loaded.emitters)
+        ld.light_paths, ld.through = self.loaded.light_paths, self.loaded.through
+        s = settings
+        cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, s.pool, s.seed,
+                                  int(s.guiding), 1 if s.profile else 0, 1 if s.use_graph else 0,
+                                  *tuple(s.trace)[:3], s.train_samples, s.train_steps)
+        self.frame16 = frame.pack()
+        return ld, cs
+
+    def _reset(self, scene, gpu_scene, settings, frame):
+        old = (self.scene, self.settings, self.frame, self.loaded, self.frame16)
+        ld, cs = self._prepare(scene, settings, frame)
+        try:
+            _lib.check(_lib.lib().lr_renderer_reset(self.handle, gpu_scene.handle if gpu_scene else None,
+                                                    ctypes.byref(ld), ctypes.byref(cs), _lib.ptr(self.frame16)))
+        except Exception:
+            self.scene, self.settings, self.frame, self.loaded, self.frame16 = old
+            raise
+        self.scene = scene
+
+    def restart(self, frame: RayFrame):
+        """Renderer.restart (integrator.lucb:463): start over through `frame`
+        (a moved camera, a resized view), keeping the projection."""
+        if frame.width == 0 or frame.height == 0 or frame.width > 16384 or frame.height > 16384:
+            raise LuceError(INVALID, "an image is 1 to 16384 pixels on a side")
+        if frame.orthographic != self.frame.orthographic:
+            raise LuceError(INVALID, "a restart keeps the view's projection")
+        self._reset(self.scene, None, replace(self.settings, width=frame.width, height=frame.height), frame)
+
+    def reload(self, scene, settings: RenderSettings, frame: RayFrame = None):
+        """Renderer.reload (integrator.lucb:488): the lights, materials and
+        constants of `scene` (the same triangles), `settings` but for the
+        size and the pool; the film starts over, through `frame` when given."""
+        if settings.samples == 0 or settings.max_bounces == 0 or settings.max_bounces > 1024:
+            raise LuceError(INVALID, "a render needs samples and 1 to 1024 bounces")
+        if scene.triangle_count() != self.scene.triangle_count():
+            raise LuceError(INVALID, "a reload keeps the scene's geometry")
+        frame = frame or self.frame
+        chosen = replace(settings, width=frame.width, height=frame.height, pool=self.settings.pool)
+        self._reset(scene, None, chosen, frame)
+
+    def replace_scene(self, scene, settings: RenderSettings, frame: RayFrame):
+        """Renderer.replace_scene (integrator.lucb:542): new geometry in place
+        of this one's (uploaded here), keeping the pool."""
+        if settings.samples == 0 or settings.max_bounces == 0 or settings.max_bounces > 1024:
+            raise LuceError(INVALID, "a render needs samples and 1 to 1024 bounces")
+        if settings.pool != self.settings.pool:
+            raise LuceError(INVALID, "a scene replaced keeps the pool")
+        uploaded = GpuScene.upload(self.device, scene)
+        try:
+            self._reset(scene, uploaded, replace(settings, width=frame.width, height=frame.height), frame)
+        except Exception:
+            uploaded.destroy()
+            raise
+        if self.owned_scene:
+            self.gpu_scene.destroy()
+        self.gpu_scene = uploaded
+        self.owned_scene = True
 
     def advance(self):
         busy = ctypes.c_int(0)

@@ -1,0 +1,100 @@
+"""The interactive lifecycle (integrator.lucb:463-597): restart through a new
+view, reload of lights and materials, replacement of the scene. Each must
+render exactly what a fresh renderer of the same inputs renders (camera paths
+only, so no float-atomic order enters: bit for bit)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _settings(**kw):
+    from luce_render.loading import RenderSettings
+    base = dict(width=48, height=40, samples=6, max_bounces=5, spectral=False, light_tracing=0.0)
+    base.update(kw)
+    return RenderSettings(**base)
+
+
+def _fresh(device, sc, st, frame):
+    from luce_render import render
+    r = render.Renderer.create_viewed(device, sc, st, frame)
+    try:
+        r.finish()
+        return r.read().copy()
+    finally:
+        r.destroy()
+
+
+def test_restart_equals_fresh_render(device):
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.cornell_render_set())
+    st = _settings()
+    first = render.ray_frame(sc.cameras[0], 48, 40)
+    moved = render.look_frame((0.3, 0.2, 2.8), (-0.1, -0.05, -1.0), (0.0, 1.0, 0.0), 0.45, 64, 32)
+    r = render.Renderer.create_viewed(device, sc, st, first)
+    try:
+        r.advance()  # a pass in flight, forgotten by the restart
+        r.restart(moved)
+        assert r.samples_done() == 0
+        r.finish()
+        got = r.read().copy()
+        assert got.shape == (32, 64, 4)
+    finally:
+        r.destroy()
+    ref = _fresh(device, sc, dataclasses.replace(st, width=64, height=32), moved)
+    assert np.array_equal(got, ref)
+
+
+def test_reload_equals_fresh_render(device):
+    from luce_render import render, scenes
+    base = scenes.cornell_render_set()
+    sc = render.compile_scene(base)
+    brighter = scenes.cornell_render_set(light_radiance=40.0)
+    brighter.materials["white"] = {"base_color": (0.5, 0.6, 0.7), "specular_weight": 0.0}
+    sc2 = render.compile_scene(brighter)
+    st = _settings()
+    frame = render.ray_frame(sc.cameras[0], 48, 40)
+    st2 = _settings(samples=4, max_bounces=3, seed=9)
+    r = render.Renderer.create_viewed(device, sc, st, frame)
+    try:
+        r.finish()
+        r.reload(sc2, st2)
+        r.finish()
+        got = r.read().copy()
+    finally:
+        r.destroy()
+    ref = _fresh(device, sc2, st2, frame)
+    assert np.array_equal(got, ref)
+    with pytest.raises(render.LuceError) as e:
+        r2 = render.Renderer.create_viewed(device, sc, st, frame)
+        try:
+            r2.reload(render.compile_scene(scenes.furnace_set()), st)
+        finally:
+            r2.destroy()
+    assert e.value.code == 400
+
+
+def test_replace_scene_equals_fresh_render(device):
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.cornell_render_set())
+    other = render.compile_scene(scenes.furnace_set())
+    st = _settings()
+    frame = render.ray_frame(sc.cameras[0], 48, 40)
+    st2 = _settings(background=(1.0, 1.0, 1.0))
+    frame2 = render.ray_frame(other.cameras[0], 40, 30)
+    r = render.Renderer.create_viewed(device, sc, st, frame)
+    try:
+        r.finish()
+        r.replace_scene(other, st2, frame2)
+        r.finish()
+        got = r.read().copy()
+    finally:
+        r.destroy()
+    ref = _fresh(device, other, dataclasses.replace(st2, width=40, height=30), frame2)
+    assert np.array_equal(got, ref)
