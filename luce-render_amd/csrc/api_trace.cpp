@@ -50,6 +50,8 @@ lr::DeviceScene lr_gpu_scene::device_scene() const {
     DeviceScene sc{};
     int wi = config_width_index();
     sc.nodes = static_cast<const uint8_t *>(wide[wi]);
+    sc.binary = static_cast<const uint8_t *>(wide[0]);
+    sc.ranges = static_cast<const uint2 *>(ranges);
     sc.tris = static_cast<const float4 *>(tris);
     sc.faces = static_cast<const uint32_t *>(faces);
     std::memcpy(sc.root_lo, root_lo, 12);
@@ -67,7 +69,7 @@ const lr::TraceVariant &lr_gpu_scene::variant() const {
 }
 
 lr_gpu_scene::~lr_gpu_scene() {
-    for (void *p : {wide[0], wide[1], wide[2], tris, faces, shading, uvs, spill, (void *)work})
+    for (void *p : {wide[0], wide[1], wide[2], tris, faces, shading, uvs, spill, (void *)work, ranges})
         if (p) (void)hipFree(p);
 }
 
@@ -200,6 +202,8 @@ int lr_scene_upload(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **
         s->info.wide_bytes[wi] = layout.bytes.size();
         s->info.stack_bound[wi] = layout.stack_bound;
         if ((rc = upload(&s->wide[wi], layout.bytes.data(), layout.bytes.size(), "upload wide nodes"))) return rc;
+        if (wi == 0 && (rc = upload(&s->ranges, layout.ranges.data(), layout.ranges.size(), "upload node ranges")))
+            return rc;
     }
     s->info.triangle_bytes = a->triangles * 48;
     if ((rc = upload((void **)&s->work, (const uint32_t *)nullptr, 0, "ray counters"))) return rc;

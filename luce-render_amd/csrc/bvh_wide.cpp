@@ -123,6 +123,23 @@ template <int W> WideLayout collapse(const lr_bvh_node *nodes, size_t node_count
         below[i] = (uint32_t)std::max(0, k - 1) + deepest;
     }
     layout.stack_bound = below.empty() ? 0 : below[0];
+    if (W == 2) {
+        // children follow their parents, so one backward sweep fills the ranges
+        layout.ranges.assign(wide.size() * 2, 0);
+        for (size_t i = wide.size(); i-- > 0;) {
+            const WideNode<W> &w = wide[i];
+            uint32_t lo = 0xffffffffu, hi = 0;
+            for (int c = 0; c < W; c++) {
+                if (w.child[c] == 0xffffffffu && w.count[c] == 0) continue;
+                uint32_t a = w.count[c] ? w.child[c] : layout.ranges[w.child[c] * 2];
+                uint32_t b = w.count[c] ? w.child[c] + w.count[c] - 1u : layout.ranges[w.child[c] * 2 + 1];
+                lo = std::min(lo, a);
+                hi = std::max(hi, b);
+            }
+            layout.ranges[i * 2] = lo;
+            layout.ranges[i * 2 + 1] = hi;
+        }
+    }
     layout.nodes = wide.size();
     layout.bytes.assign(layout.nodes * layout.node_bytes, 0);
     for (size_t i = 0; i < wide.size(); i++) std::memcpy(&layout.bytes[i * layout.node_bytes], &wide[i], sizeof(WideNode<W>));

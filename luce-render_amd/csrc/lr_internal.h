@@ -41,6 +41,9 @@ struct WideLayout {
     size_t nodes = 0;
     std::vector<uint8_t> bytes;
     uint32_t stack_bound = 0;
+    // Width 2 (the binary tree itself): each node's range of leaf-order
+    // triangle slots, lowest and highest, for the wider traversals' ties.
+    std::vector<uint32_t> ranges;
 };
 
 // Collapses the binary tree into width-W nodes (W = 2, 4, 8): each wide node

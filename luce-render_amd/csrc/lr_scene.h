@@ -22,6 +22,7 @@ struct lr_gpu_scene {
     void *spill = nullptr;
     uint64_t spill_entries = 0;
     uint32_t *work = nullptr;  // the persistent launches' two counters
+    void *ranges = nullptr;    // width-2 nodes' triangle slot ranges (uint2 each)
     float root_lo[3] = {0, 0, 0};
     float root_hi[3] = {0, 0, 0};
     lr_scene_info info{};

@@ -11,7 +11,10 @@
 //     (bvh.glsl:55-56);
 //   * closest hit: hit children visited nearest first (ties: lower child
 //     index), the others pushed farthest first (bvh.glsl:73-84); with width 2
-//     this is the reference's visiting order exactly;
+//     this is the reference's visiting order exactly; wider nodes visit the
+//     same boxes in another order, so where two triangles tie at exactly the
+//     same t they settle the tie as the binary order would
+//     (binary_first);
 //   * any hit: children taken in child order, the first triangle hit ends it
 //     (bvh.glsl:93-122).
 // Unlike bvh.glsl:79 no stack entry is ever dropped: entries past the LDS
@@ -38,6 +41,8 @@ struct DeviceScene {
     uint32_t spill_stride;   // threads the launch covers
     uint32_t spill_depth;    // entries per thread the spill area holds
     uint32_t *work;          // persistent launches: {rays handed out, waves finished}, zero between launches
+    const uint8_t *binary;   // the width-2 nodes (the binary tree), for wider traversals' ties
+    const uint2 *ranges;     // their leaf-order triangle slot ranges
 };
 
 struct Hit {
@@ -79,13 +84,13 @@ __device__ __forceinline__ float box_entry(float lox, float loy, float loz, floa
 }
 
 // triangle_hit (bvh.glsl:27-47) on a repacked triangle: whether it is hit
-// before `best`, and where. Every quantity is
+// before `best`, and where; `tie_out`: hit exactly at `best`. Every quantity is
 // computed exactly as the reference computes it; its early exits become one
 // predicate over the same comparisons, so the loads and arithmetic run
 // straight through (no divergent branch, no load held back behind one).
 __device__ __forceinline__ bool triangle_test(float4 a, float4 b, float4 c, float ox, float oy, float oz, float dx,
-                                              float dy, float dz, float best, float &t_out, float &u_out,
-                                              float &v_out) {
+                                              float dy, float dz, float best, float &t_out, float &u_out, float &v_out,
+                                              bool &tie_out) {
     float e1x = a.w, e1y = b.x, e1z = b.y;
     float e2x = b.z, e2y = b.w, e2z = c.x;
     float px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
@@ -99,7 +104,9 @@ __device__ __forceinline__ bool triangle_test(float4 a, float4 b, float4 c, floa
     bool reject = fabsf(det) < 1e-20f;
     reject = reject || u < 0.0f || u > 1.0f;
     reject = reject || v < 0.0f || u + v > 1.0f;
-    reject = reject || t <= 0.0f || t >= best;
+    reject = reject || t <= 0.0f;
+    tie_out = !reject && t == best;
+    reject = reject || t >= best;
     t_out = t;
     u_out = u;
     v_out = v;
@@ -150,6 +157,47 @@ template <int S> struct LaneStack {
     }
 };
 
+// Of two leaf-order triangles the ray hits at the same t, whether `a` is the
+// one the reference's binary traversal finds first (and so keeps): below
+// their lowest common node it visits the child with the nearer box entry
+// first (ties: the left one) and finishes it before the other; within one
+// leaf it tests triangles in order.
+__device__ __forceinline__ bool binary_first(const DeviceScene &sc, uint32_t a, uint32_t b, float ox, float oy, float oz,
+                                          float ix, float iy, float iz, float tmax) {
+    uint32_t node = 0;
+    for (uint32_t depth = 0; depth < 4096u; depth++) {
+        NodeView<2> n;
+        n.load(sc.binary, node);
+        uint32_t lo[2], hi[2];
+        for (int c = 0; c < 2; c++) {
+            uint32_t ch = n.child(c), cnt = n.count(c);
+            if (cnt) {
+                lo[c] = ch;
+                hi[c] = ch + cnt - 1u;
+            } else if (ch != kNone) {
+                uint2 r = sc.ranges[ch];
+                lo[c] = r.x;
+                hi[c] = r.y;
+            } else {
+                lo[c] = 1u;
+                hi[c] = 0u;
+            }
+        }
+        int ca = (a >= lo[0] && a <= hi[0]) ? 0 : 1, cb = (b >= lo[0] && b <= hi[0]) ? 0 : 1;
+        if (ca != cb) {
+            float e0 = box_entry(n.lo_x(0), n.lo_y(0), n.lo_z(0), n.hi_x(0), n.hi_y(0), n.hi_z(0), ox, oy, oz, ix, iy,
+                                 iz, tmax);
+            float e1 = box_entry(n.lo_x(1), n.lo_y(1), n.lo_z(1), n.hi_x(1), n.hi_y(1), n.hi_z(1), ox, oy, oz, ix, iy,
+                                 iz, tmax);
+            bool left_first = e0 >= 0.0f && (e1 < 0.0f || e0 <= e1);
+            return (ca == 0) == left_first;
+        }
+        if (n.count(ca)) return a < b;  // the same leaf
+        node = n.child(ca);
+    }
+    return a < b;
+}
+
 // One ray's closest-hit walk (bvh.glsl:51-90) over width-W nodes, advanced one
 // node at a time, so a persistent wave can hand a finished lane a new ray.
 template <int W, int S> struct ClosestWalk {
@@ -186,7 +234,11 @@ template <int W, int S> struct ClosestWalk {
                 const float4 *np = sc.tris + (size_t)(slot < last ? slot + 1 : slot) * 3;
                 float4 na = np[0], nb = np[1], nc = np[2];
                 float t, u, v;
-                if (triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, hit.t, t, u, v)) hit = Hit{t, slot, u, v};
+                bool tie;
+                bool take = triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, hit.t, t, u, v, tie);
+                if (W > 2 && tie && hit.slot != kNone)
+                    take = binary_first(sc, slot, hit.slot, ox, oy, oz, ix, iy, iz, hit.t);
+                if (take) hit = Hit{t, slot, u, v};
                 if (slot == last) break;
                 a = na, b = nb, c = nc;
             }
@@ -272,7 +324,8 @@ template <int W, int S> struct AnyWalk {
                 const float4 *np = sc.tris + (size_t)(slot < last ? slot + 1 : slot) * 3;
                 float4 na = np[0], nb = np[1], nc = np[2];
                 float t, u, v;
-                if (triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, tmax, t, u, v)) {
+                bool tie;
+                if (triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, tmax, t, u, v, tie)) {
                     blocked = true;
                     return false;
                 }
