@@ -74,6 +74,13 @@ size_t lr_last_error(char *buffer, size_t capacity);
 int lr_bvh_build(const float *positions, size_t points, const uint32_t *indices, size_t triangles,
                  int threads, lr_bvh_node *nodes, size_t *node_count, uint32_t *order);
 
+/* build_bvh (src/render/bvh.lucb:66-217) on the GPU: the same tree, node
+ * numbering and leaf order as lr_bvh_build (node boxes equal as values; a zero
+ * plane that occurs with both signs may keep either sign). Host arrays as in
+ * lr_bvh_build; the device holds the work (about 120 bytes per triangle). */
+int lr_bvh_build_device(lr_device *device, const float *positions, size_t points, const uint32_t *indices,
+                        size_t triangles, lr_bvh_node *nodes, size_t *node_count, uint32_t *order);
+
 /* Replaces GpuScene.upload (src/render/device.lucb:41-89). The binary BVH is
  * repacked on upload into the traversal layouts (width 2, 4 and 8, 64/128/256
  * byte nodes) and a leaf-order triangle array (p0, e1, e2). */

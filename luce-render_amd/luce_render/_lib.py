@@ -88,6 +88,8 @@ _SIGNATURES = {
     "lr_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     "lr_bvh_build": (ctypes.c_int, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.c_int, _P,
                                     ctypes.POINTER(ctypes.c_size_t), _P]),
+    "lr_bvh_build_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P,
+                                           ctypes.POINTER(ctypes.c_size_t), _P]),
     "lr_scene_upload": (ctypes.c_int, [_P, ctypes.POINTER(SceneArrays), ctypes.POINTER(_P)]),
     "lr_scene_destroy": (None, [_P]),
     "lr_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, ctypes.c_size_t]),

@@ -310,16 +310,22 @@ def compile_scene(gset, geometry=True, stopping=None, threads=0):
     return scene
 
 
-def build_bvh(scene, threads=0):
-    """build_bvh (bvh.lucb:66-81) through the native builder, then reorder
-    (bvh.lucb:198-217)."""
+def build_bvh(scene, threads=0, device=None):
+    """build_bvh (bvh.lucb:66-81) through the native builder (on the host's
+    cores, or on `device`'s GPU when one is given: the same tree), then
+    reorder (bvh.lucb:198-217)."""
     n = scene.triangle_count()
     nodes = np.zeros(max(1, 2 * n), NODE_DTYPE)
     order = np.zeros(max(1, n), np.uint32)
     count = _lib.ctypes.c_size_t(0)
-    _lib.check(_lib.lib().lr_bvh_build(_lib.ptr(scene.positions), scene.point_count(),
-                                       _lib.ptr(scene.indices), n, int(threads), _lib.ptr(nodes),
-                                       _lib.ctypes.byref(count), _lib.ptr(order)))
+    if device is not None:
+        _lib.check(_lib.lib().lr_bvh_build_device(device.handle, _lib.ptr(scene.positions), scene.point_count(),
+                                                  _lib.ptr(scene.indices), n, _lib.ptr(nodes),
+                                                  _lib.ctypes.byref(count), _lib.ptr(order)))
+    else:
+        _lib.check(_lib.lib().lr_bvh_build(_lib.ptr(scene.positions), scene.point_count(),
+                                           _lib.ptr(scene.indices), n, int(threads), _lib.ptr(nodes),
+                                           _lib.ctypes.byref(count), _lib.ptr(order)))
     scene.nodes = nodes[:count.value].copy()
     scene.order = order[:n].copy()
     if n:

@@ -1,0 +1,115 @@
+"""lr_bvh_build_device: the GPU build of build_bvh (bvh.lucb:66-217) gives the
+reference's tree node for node (first/count equal, boxes equal as values) and
+its leaf order: against the oracle's sequential restatement on triangle soups,
+degenerate sets (one point, coincident triangles that force halving, a line)
+and the Cornell probe scene; against the host builder on the instanced scene."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _soup(n, seed, spread=1.0):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, (n, 1, 3))
+    p = (c + rng.normal(0, 0.05, (n, 3, 3))).reshape(-1, 3).astype(np.float32)
+    return p, np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+
+
+def _scene(p, idx):
+    from luce_render import render
+    n = len(idx)
+    sc = render.CompiledScene()
+    sc.positions, sc.indices = np.ascontiguousarray(p.reshape(-1, 3)), np.ascontiguousarray(idx)
+    sc.faces = np.arange(n, dtype=np.uint32)
+    sc.shading = np.zeros((n, 4), np.uint32)
+    sc.uvs = np.zeros((n, 6), np.float32)
+    return sc
+
+
+def _same_tree(got, want):
+    assert got.shape == want.shape
+    assert np.array_equal(got["first"], want["first"])
+    assert np.array_equal(got["count"], want["count"])
+    assert np.array_equal(got["lo"], want["lo"]) and np.array_equal(got["hi"], want["hi"])
+
+
+def _device_tree(device, p, idx):
+    from luce_render import render
+    sc = _scene(p, idx)
+    render.build_bvh(sc, device=device)
+    return sc.nodes, sc.order
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 16, 17, 100, 1000, 20000, 300000])
+def test_device_bvh_equals_oracle_on_soups(device, n):
+    p, idx = _soup(n, n + 11)
+    say(f"device build: soup of {n}")
+    nodes, order = _device_tree(device, p, idx)
+    want, want_order, _ = O.bvh_build(p, idx)
+    _same_tree(nodes, want)
+    assert np.array_equal(order, want_order[:n])
+
+
+def _degenerate(kind, n):
+    rng = np.random.default_rng(5)
+    if kind == "point":  # every corner the same point: no split has area, halving throughout
+        p = np.zeros((3 * n, 3), np.float32) + np.float32(0.25)
+    elif kind == "coincident":  # groups of identical triangles: bins cannot separate a group
+        base = rng.uniform(-1, 1, (n // 40 + 1, 3, 3)).astype(np.float32)
+        p = base[np.arange(n) % len(base)].reshape(-1, 3)
+    elif kind == "line":  # centroids on one axis, zero extent on the others
+        x = rng.uniform(-1, 1, n).astype(np.float32)
+        p = np.stack([x, x + 0.01, x + 0.02], 1)[:, :, None] * np.array([1, 0, 0], np.float32)
+        p = p.reshape(-1, 3).astype(np.float32)
+    else:  # signed zeros: planes through the origin with both signs of zero
+        c = rng.integers(-3, 4, (n, 3, 3)).astype(np.float32) * np.float32(0.5)
+        c[rng.random((n, 3, 3)) < 0.3] = -0.0
+        p = c.reshape(-1, 3)
+    return p, np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+
+
+@pytest.mark.parametrize("kind", ["point", "coincident", "line", "zeros"])
+def test_device_bvh_equals_oracle_on_degenerate_sets(device, kind):
+    p, idx = _degenerate(kind, 5000)
+    say(f"device build: degenerate {kind}")
+    nodes, order = _device_tree(device, p, idx)
+    want, want_order, _ = O.bvh_build(p, idx)
+    _same_tree(nodes, want)
+    assert np.array_equal(order, want_order)
+
+
+def test_device_bvh_equals_oracle_on_cornell_probe(device, cornell_probe):
+    sc = cornell_probe
+    inv = np.empty_like(sc.order)
+    inv[sc.order] = np.arange(len(sc.order), dtype=np.uint32)
+    original = sc.indices[inv]
+    nodes, order = _device_tree(device, sc.positions, original)
+    _same_tree(nodes, sc.nodes)
+    assert np.array_equal(order, sc.order)
+
+
+def test_device_bvh_equals_host_build_on_instanced_scene(device):
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.instanced_set(instances=60), geometry=False)
+    p, idx = sc.positions, sc.indices
+    say(f"device build: instanced, {len(idx)} triangles")
+    t = time.perf_counter()
+    host = _scene(p, idx)
+    render.build_bvh(host)
+    t_host = time.perf_counter() - t
+    t = time.perf_counter()
+    nodes, order = _device_tree(device, p, idx)
+    t_dev = time.perf_counter() - t
+    say(f"host {t_host:.3f} s, device {t_dev:.3f} s (host arrays in and out)")
+    _same_tree(nodes, host.nodes)
+    assert np.array_equal(order, host.order)
+    assert O.check_bvh(nodes, p, idx[order]) == 0

@@ -164,3 +164,55 @@ def test_cornell_probe_scene_is_the_committed_one(cornell_probe):
     assert int(sc.faces.max()) + 1 == golden["polygons"] == 10017
     assert hashlib.sha256(np.ascontiguousarray(sc.positions).tobytes()).hexdigest() == golden["positions_sha256"]
     assert hashlib.sha256(np.ascontiguousarray(sc.nodes).tobytes()).hexdigest() == golden["nodes_sha256"]
+
+
+def _swap_partition(items, left):
+    """partition (bvh.lucb:185-196): the reference's in-place swap loop."""
+    a = list(items)
+    low, high = 0, len(a)
+    while low < high:
+        if left(a[low]):
+            low += 1
+        else:
+            high -= 1
+            a[low], a[high] = a[high], a[low]
+    return a, low
+
+
+def _closed_form_partition(items, left):
+    """The device builder's closed form of the same permutation: each element's
+    destination from its rank among the lefts before it and two tables."""
+    n = len(items)
+    f = [left(x) for x in items]
+    L = sum(f)
+    lb = np.concatenate([[0], np.cumsum(f)]).astype(int)
+    S, T = {}, {}
+    for i in range(n):
+        if i < L and not f[i]:
+            S[i - lb[i] + 1] = i
+        if i >= L and f[i]:
+            T[L - lb[i]] = i
+    out = [None] * n
+    for i in range(n):
+        if f[i]:
+            d = i if i < L else S[L - lb[i]]
+        elif i < L:
+            k = i - lb[i] + 1
+            d = (n if k == 1 else T[k - 1]) - 1
+        elif i > L:
+            d = i - 1
+        else:
+            m = L - lb[i]
+            d = (n if m == 0 else T[m]) - 1
+        assert out[d] is None
+        out[d] = items[i]
+    return out, L
+
+
+def test_closed_form_partition_equals_the_swap_loop():
+    rng = np.random.default_rng(4)
+    for _ in range(20000):
+        n = int(rng.integers(0, 14))
+        flags = rng.random(n) < rng.random()
+        items = list(range(n))
+        assert _closed_form_partition(items, lambda x: flags[x]) == _swap_partition(items, lambda x: flags[x])
