@@ -43,9 +43,10 @@ namespace gbvh {
 constexpr int kBins = 12;          // bvh.lucb:29
 constexpr uint32_t kLeafSize = 4;  // bvh.lucb:30
 constexpr uint32_t kLeafLimit = 16;  // bvh.lucb:32
-constexpr uint32_t kChunk = 4096;    // positions per workgroup in the chunked kernels
-constexpr uint32_t kBigNode = 16384; // nodes above this go to the chunked kernels
+constexpr uint32_t kChunk = 2048;    // positions per workgroup in the chunked kernels
+constexpr uint32_t kBigNode = 2048;  // nodes above this go to the chunked kernels
 constexpr uint32_t kSlots = 3 * kBins;  // (axis, bin)
+constexpr uint32_t kReplicas = 16;      // copies of a large node's bounds and bins the chunks merge into
 
 __device__ __forceinline__ float mn(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float mx(float a, float b) { return a > b ? a : b; }
@@ -121,6 +122,7 @@ struct LevelArgs {
     uint32_t *counter;    // temporary ids handed out
     uint32_t begin, end;  // this level's ids
     uint32_t big;         // nodes above this many triangles are done by the chunked kernels
+    uint32_t batch;       // nodes per wave (at most 64)
 };
 
 // The best split over gathered bins (best_split, bvh.lucb:147-182) by one wave:
@@ -207,14 +209,65 @@ __device__ __forceinline__ void table_entry(bool left, uint32_t i, uint32_t lb, 
     if (at < n) tab[at] = i;
 }
 
+// Bins the triangles at positions [i0, i1) of one thread's contiguous run:
+// neighbours in the order tend to share a bin, so each axis keeps a running
+// bin (count and box) and adds it to the LDS bins only when the bin changes,
+// instead of every lane of a wave hitting the same LDS words at once.
+__device__ __forceinline__ void bin_run(const uint32_t *cur, uint32_t i0, uint32_t i1, const float4 *blo,
+                                        const float4 *bhi, const Bounds &b, uint32_t *cnt, uint32_t *bl,
+                                        uint32_t *bh) {
+    int rb[3] = {-1, -1, -1};
+    uint32_t rc[3] = {0u, 0u, 0u};
+    float rlo[3][3], rhi[3][3];
+    auto flush = [&](int ax) {
+        if (!rc[ax]) return;
+        uint32_t slot = (uint32_t)(ax * kBins + rb[ax]);
+        atomicAdd(&cnt[slot], rc[ax]);
+        for (int d = 0; d < 3; d++) {
+            atomicMin(&bl[slot * 3 + d], enc(rlo[ax][d]));
+            atomicMax(&bh[slot * 3 + d], enc(rhi[ax][d]));
+        }
+    };
+    for (uint32_t i = i0; i < i1; i++) {
+        uint32_t p = cur[i];
+        float4 l = blo[p], h = bhi[p];
+        float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
+        for (int ax = 0; ax < 3; ax++) {
+            int bn = bin_of(centroid(lv[ax], hv[ax]), b.ml[ax], b.mh[ax]);
+            if (bn != rb[ax]) {
+                flush(ax);
+                rb[ax] = bn;
+                rc[ax] = 0u;
+                for (int d = 0; d < 3; d++) rlo[ax][d] = 3.0e38f, rhi[ax][d] = -3.0e38f;
+            }
+            rc[ax]++;
+            for (int d = 0; d < 3; d++) {
+                rlo[ax][d] = mn(rlo[ax][d], lv[d]);
+                rhi[ax][d] = mx(rhi[ax][d], hv[d]);
+            }
+        }
+    }
+    for (int ax = 0; ax < 3; ax++) flush(ax);
+}
+
 __global__ __launch_bounds__(64) void level_kernel(LevelArgs a) {
     __shared__ uint32_t cnt[kSlots];
     __shared__ uint32_t bl[kSlots * 3], bh[kSlots * 3];
     const uint32_t lane = threadIdx.x;
-    for (uint32_t id = a.begin + blockIdx.x; id < a.end; id += gridDim.x) {
+    // Batches of up to 64 nodes: lane j keeps node j's split, so the batch's
+    // child pairs take one counter update (one counter for the whole level
+    // would otherwise serialize an atomic per node).
+    for (uint32_t batch = a.begin + blockIdx.x * a.batch; batch < a.end; batch += gridDim.x * a.batch) {
+      const uint32_t nb = min(a.batch, a.end - batch);
+      bool my_split = false;
+      uint32_t my_first = 0, my_count = 0, my_left = 0;
+      float my_lo[3] = {0.0f, 0.0f, 0.0f}, my_hi[3] = {0.0f, 0.0f, 0.0f};
+      for (uint32_t j = 0; j < nb; j++) {
+        const uint32_t id = batch + j;
         const lr_bvh_node pending = a.nodes[id];
         const uint32_t first = pending.first, count = pending.count;
-        if (count > a.big) continue;
+        // count 0: a large node the chunked kernels have split already
+        if (count == 0 || count > a.big) continue;
         const uint32_t *cur = a.cur + first;
         // The node's box and centroid bounds.
         Bounds b;
@@ -246,19 +299,8 @@ __global__ __launch_bounds__(64) void level_kernel(LevelArgs a) {
                 bh[j] = enc(-3.0e38f);
             }
             __syncthreads();
-            for (uint32_t i = lane; i < count; i += 64u) {
-                uint32_t p = cur[i];
-                float4 l = a.blo[p], h = a.bhi[p];
-                float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
-                for (int ax = 0; ax < 3; ax++) {
-                    uint32_t slot = (uint32_t)(ax * kBins + bin_of(centroid(lv[ax], hv[ax]), b.ml[ax], b.mh[ax]));
-                    atomicAdd(&cnt[slot], 1u);
-                    for (int d = 0; d < 3; d++) {
-                        atomicMin(&bl[slot * 3 + d], enc(lv[d]));
-                        atomicMax(&bh[slot * 3 + d], enc(hv[d]));
-                    }
-                }
-            }
+            const uint32_t run = (count + 63u) / 64u, r0 = min(count, lane * run);
+            bin_run(cur, r0, min(count, r0 + run), a.blo, a.bhi, b, cnt, bl, bh);
             __syncthreads();
             best_split(b, cnt, bl, bh, lane, axis, bin, cost);
         }
@@ -311,13 +353,290 @@ __global__ __launch_bounds__(64) void level_kernel(LevelArgs a) {
         if (!split) {
             for (uint32_t i = lane; i < count; i += 64u) a.fin[first + i] = cur[i];
             if (lane == 0) a.nodes[id] = lr_bvh_node{{b.lo[0], b.lo[1], b.lo[2]}, first, {b.hi[0], b.hi[1], b.hi[2]}, count};
-        } else if (lane == 0) {
-            uint32_t pair = atomicAdd(a.counter, 2u);
-            a.nodes[id] = lr_bvh_node{{b.lo[0], b.lo[1], b.lo[2]}, pair, {b.hi[0], b.hi[1], b.hi[2]}, 0u};
-            a.nodes[pair] = lr_bvh_node{{0, 0, 0}, first, {0, 0, 0}, left_count};
-            a.nodes[pair + 1] = lr_bvh_node{{0, 0, 0}, first + left_count, {0, 0, 0}, count - left_count};
+        } else if (lane == j) {
+            my_split = true;
+            my_first = first;
+            my_count = count;
+            my_left = left_count;
+            for (int d = 0; d < 3; d++) my_lo[d] = b.lo[d], my_hi[d] = b.hi[d];
         }
         __syncthreads();  // the LDS bins are reused by the next node
+      }
+      const uint64_t m = __ballot(my_split);
+      if (!m) continue;
+      uint32_t pair = 0;
+      if (lane == 0) pair = atomicAdd(a.counter, 2u * (uint32_t)__popcll(m));
+      pair = __shfl(pair, 0, 64) + 2u * (uint32_t)__popcll(m & lanes_below(lane));
+      if (my_split) {
+          a.nodes[batch + lane] = lr_bvh_node{{my_lo[0], my_lo[1], my_lo[2]}, pair, {my_hi[0], my_hi[1], my_hi[2]}, 0u};
+          a.nodes[pair] = lr_bvh_node{{0, 0, 0}, my_first, {0, 0, 0}, my_left};
+          a.nodes[pair + 1] = lr_bvh_node{{0, 0, 0}, my_first + my_left, {0, 0, 0}, my_count - my_left};
+      }
+    }
+}
+
+// ------------------------------------------------- large nodes: several workgroups each
+
+// A node above kBigNode triangles, its run of chunks, and its gathered state:
+// box and centroid bounds, bins (both as encoded floats) and the split.
+struct BigNode {
+    uint32_t id, first, count, chunk0, chunks;
+};
+struct Chunk {
+    uint32_t big, begin, end;  // positions [begin, end) of big node `big`
+};
+struct BigState {
+    uint32_t box[12];  // lo xyz, hi xyz, centroid lo xyz, centroid hi xyz
+    uint32_t cnt[kSlots];
+    uint32_t bl[kSlots * 3], bh[kSlots * 3];
+    int axis, bin;
+    uint32_t left, halve;
+};
+
+struct BigArgs {
+    const float4 *blo, *bhi;
+    const uint32_t *cur;
+    uint32_t *next, *tab;
+    lr_bvh_node *nodes;
+    uint32_t *counter;
+    const BigNode *bigs;
+    const Chunk *chunks;
+    BigState *st;
+    uint32_t *chunk_left, *chunk_base;
+    BigNode *next_bigs;
+    uint32_t *next_big_count;
+};
+
+__device__ __forceinline__ Bounds big_bounds(const BigState &x) {
+    Bounds b;
+    for (int d = 0; d < 3; d++) {
+        b.lo[d] = dec(x.box[d]);
+        b.hi[d] = dec(x.box[3 + d]);
+        b.ml[d] = dec(x.box[6 + d]);
+        b.mh[d] = dec(x.box[9 + d]);
+    }
+    return b;
+}
+
+// The exclusive rank of `flag` among a 256-thread round's flags, and the
+// round's total.
+__device__ __forceinline__ uint32_t block_prefix(bool flag, uint32_t *waves, uint32_t &total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t m = __ballot(flag);
+    if (lane == 0) waves[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (uint32_t w = 0; w < 4u; w++) {
+        uint32_t c = waves[w];
+        before += w < wave ? c : 0u;
+        total += c;
+    }
+    __syncthreads();
+    return before + (uint32_t)__popcll(m & lanes_below(lane));
+}
+
+// st holds kReplicas states per large node; replica 0 receives the merge.
+__global__ __launch_bounds__(64) void big_init_kernel(BigState *st) {
+    BigState &x = st[blockIdx.x];
+    const uint32_t t = threadIdx.x;
+    if (t < 12u) x.box[t] = enc((t < 3u || (t >= 6u && t < 9u)) ? 3.0e38f : -3.0e38f);
+    for (uint32_t j = t; j < kSlots; j += 64u) x.cnt[j] = 0u;
+    for (uint32_t j = t; j < kSlots * 3u; j += 64u) {
+        x.bl[j] = enc(3.0e38f);
+        x.bh[j] = enc(-3.0e38f);
+    }
+}
+
+__global__ __launch_bounds__(256) void big_bounds_kernel(BigArgs a) {
+    const Chunk ch = a.chunks[blockIdx.x];
+    float v[12];
+    for (int d = 0; d < 3; d++) v[d] = v[6 + d] = 3.0e38f, v[3 + d] = v[9 + d] = -3.0e38f;
+    for (uint32_t i = ch.begin + threadIdx.x; i < ch.end; i += 256u) {
+        uint32_t p = a.cur[i];
+        float4 l = a.blo[p], h = a.bhi[p];
+        float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
+        for (int d = 0; d < 3; d++) {
+            v[d] = mn(v[d], lv[d]);
+            v[3 + d] = mx(v[3 + d], hv[d]);
+            float c = centroid(lv[d], hv[d]);
+            v[6 + d] = mn(v[6 + d], c);
+            v[9 + d] = mx(v[9 + d], c);
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        v[d] = wave_min(v[d]);
+        v[3 + d] = wave_max(v[3 + d]);
+        v[6 + d] = wave_min(v[6 + d]);
+        v[9 + d] = wave_max(v[9 + d]);
+    }
+    __shared__ float waves[4][12];
+    if ((threadIdx.x & 63u) == 0)
+        for (int k = 0; k < 12; k++) waves[threadIdx.x >> 6][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 12u) {
+        const uint32_t k = threadIdx.x;
+        const bool lower = k < 3u || (k >= 6u && k < 9u);
+        float r = waves[0][k];
+        for (int w = 1; w < 4; w++) r = lower ? mn(r, waves[w][k]) : mx(r, waves[w][k]);
+        BigState &x = a.st[ch.big * kReplicas + blockIdx.x % kReplicas];
+        if (lower) atomicMin(&x.box[k], enc(r));
+        else atomicMax(&x.box[k], enc(r));
+    }
+}
+
+__global__ __launch_bounds__(64) void big_merge_bounds_kernel(BigState *st) {
+    const uint32_t k = threadIdx.x;
+    if (k >= 12u) return;
+    BigState *x = st + blockIdx.x * kReplicas;
+    const bool lower = k < 3u || (k >= 6u && k < 9u);
+    uint32_t r = x[0].box[k];
+    for (uint32_t q = 1; q < kReplicas; q++) r = lower ? min(r, x[q].box[k]) : max(r, x[q].box[k]);
+    x[0].box[k] = r;
+}
+
+__global__ __launch_bounds__(256) void big_bins_kernel(BigArgs a) {
+    __shared__ uint32_t cnt[kSlots];
+    __shared__ uint32_t bl[kSlots * 3], bh[kSlots * 3];
+    const Chunk ch = a.chunks[blockIdx.x];
+    const Bounds b = big_bounds(a.st[ch.big * kReplicas]);
+    BigState &x = a.st[ch.big * kReplicas + blockIdx.x % kReplicas];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t j = t; j < kSlots; j += 256u) cnt[j] = 0u;
+    for (uint32_t j = t; j < kSlots * 3u; j += 256u) {
+        bl[j] = enc(3.0e38f);
+        bh[j] = enc(-3.0e38f);
+    }
+    __syncthreads();
+    const uint32_t len = ch.end - ch.begin, run = (len + 255u) / 256u, r0 = ch.begin + min(len, t * run);
+    bin_run(a.cur, r0, min(ch.end, r0 + run), a.blo, a.bhi, b, cnt, bl, bh);
+    __syncthreads();
+    for (uint32_t j = t; j < kSlots; j += 256u)
+        if (cnt[j]) atomicAdd(&x.cnt[j], cnt[j]);
+    for (uint32_t j = t; j < kSlots * 3u; j += 256u)
+        if (cnt[j / 3u]) {
+            atomicMin(&x.bl[j], bl[j]);
+            atomicMax(&x.bh[j], bh[j]);
+        }
+}
+
+// One wave per large node: the split (always: such a node holds more than 16
+// triangles), its child pair, and large children listed for the next level.
+__global__ __launch_bounds__(256) void big_split_kernel(BigArgs a, uint32_t big) {
+    const BigNode bn = a.bigs[blockIdx.x];
+    BigState *reps = a.st + blockIdx.x * kReplicas;
+    BigState &x = reps[0];
+    for (uint32_t k = threadIdx.x; k < kSlots * 7u; k += 256u) {
+        if (k < kSlots) {
+            uint32_t c = 0;
+            for (uint32_t q = 0; q < kReplicas; q++) c += reps[q].cnt[k];
+            x.cnt[k] = c;
+        } else if (k < kSlots * 4u) {
+            uint32_t j = k - kSlots, r = x.bl[j];
+            for (uint32_t q = 1; q < kReplicas; q++) r = min(r, reps[q].bl[j]);
+            x.bl[j] = r;
+        } else {
+            uint32_t j = k - kSlots * 4u, r = x.bh[j];
+            for (uint32_t q = 1; q < kReplicas; q++) r = max(r, reps[q].bh[j]);
+            x.bh[j] = r;
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x >= 64u) return;
+    const Bounds b = big_bounds(x);
+    const uint32_t lane = threadIdx.x;
+    int axis, bin;
+    float cost;
+    best_split(b, x.cnt, x.bl, x.bh, lane, axis, bin, cost);
+    uint32_t L = 0;
+    if (axis >= 0)
+        for (int k = 0; k < bin; k++) L += x.cnt[axis * kBins + k];
+    uint32_t left = (axis < 0 || L == 0 || L == bn.count) ? bn.count / 2u : L;
+    if (lane == 0) {
+        x.axis = axis;
+        x.bin = bin;
+        x.left = L;
+        x.halve = axis < 0 ? 1u : 0u;
+        uint32_t pair = atomicAdd(a.counter, 2u);
+        a.nodes[bn.id] = lr_bvh_node{{b.lo[0], b.lo[1], b.lo[2]}, pair, {b.hi[0], b.hi[1], b.hi[2]}, 0u};
+        const uint32_t f[2] = {bn.first, bn.first + left}, c[2] = {left, bn.count - left};
+        for (int k = 0; k < 2; k++) {
+            a.nodes[pair + k] = lr_bvh_node{{0, 0, 0}, f[k], {0, 0, 0}, c[k]};
+            if (c[k] > big) {
+                uint32_t at = atomicAdd(a.next_big_count, 1u);
+                a.next_bigs[at] = BigNode{pair + (uint32_t)k, f[k], c[k], 0u, 0u};
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ bool goes_left(const BigArgs &a, uint32_t p, const BigState &x, const Bounds &b) {
+    float lo = (&a.blo[p].x)[x.axis], hi = (&a.bhi[p].x)[x.axis];
+    return bin_of(centroid(lo, hi), b.ml[x.axis], b.mh[x.axis]) < x.bin;
+}
+
+__global__ __launch_bounds__(256) void big_count_kernel(BigArgs a) {
+    __shared__ uint32_t waves[4];
+    const Chunk ch = a.chunks[blockIdx.x];
+    const BigState &x = a.st[ch.big * kReplicas];
+    const Bounds b = big_bounds(x);
+    uint32_t sum = 0;
+    for (uint32_t r0 = ch.begin; r0 < ch.end; r0 += 256u) {
+        uint32_t i = r0 + threadIdx.x, total;
+        bool left = !x.halve && i < ch.end && goes_left(a, a.cur[i], x, b);
+        (void)block_prefix(left, waves, total);
+        sum += total;
+    }
+    if (threadIdx.x == 0) a.chunk_left[blockIdx.x] = sum;
+}
+
+// Each large node's chunks: lefts before each chunk (one wave per node).
+__global__ __launch_bounds__(64) void big_scan_kernel(BigArgs a) {
+    const BigNode bn = a.bigs[blockIdx.x];
+    const uint32_t lane = threadIdx.x;
+    uint32_t base = 0;
+    for (uint32_t c0 = 0; c0 < bn.chunks; c0 += 64u) {
+        uint32_t c = c0 + lane;
+        uint32_t v = c < bn.chunks ? a.chunk_left[bn.chunk0 + c] : 0u, inc = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t t = __shfl_up(inc, o, 64);
+            if ((int)lane >= o) inc += t;
+        }
+        if (c < bn.chunks) a.chunk_base[bn.chunk0 + c] = base + inc - v;
+        base += __shfl(inc, 63, 64);
+    }
+}
+
+template <bool SCATTER> __global__ __launch_bounds__(256) void big_partition_kernel(BigArgs a) {
+    __shared__ uint32_t waves[4];
+    const Chunk ch = a.chunks[blockIdx.x];
+    const BigNode bn = a.bigs[ch.big];
+    const BigState &x = a.st[ch.big * kReplicas];
+    const Bounds b = big_bounds(x);
+    if (x.halve) {
+        if (SCATTER)
+            for (uint32_t i = ch.begin + threadIdx.x; i < ch.end; i += 256u) a.next[i] = a.cur[i];
+        return;
+    }
+    const uint32_t L = x.left, n = bn.count;
+    uint32_t *tab = a.tab + bn.first;
+    uint32_t base = a.chunk_base[blockIdx.x];
+    for (uint32_t r0 = ch.begin; r0 < ch.end; r0 += 256u) {
+        uint32_t i = r0 + threadIdx.x, total;
+        uint32_t p = i < ch.end ? a.cur[i] : 0u;
+        bool left = i < ch.end && goes_left(a, p, x, b);
+        uint32_t lb = base + block_prefix(left, waves, total);
+        if (i < ch.end) {
+            uint32_t at = i - bn.first;
+            if (SCATTER) {
+                uint32_t d = destination(left, at, lb, L, n, tab);
+                if (d < n) a.next[bn.first + d] = p;
+            } else {
+                table_entry(left, at, lb, L, n, tab);
+            }
+        }
+        base += total;
     }
 }
 
@@ -383,14 +702,23 @@ int build_bvh_device(lr_device *dev, const float *positions, const uint32_t *ind
         return 0;
     }
     DeviceArray<float4> blo, bhi;
-    DeviceArray<uint32_t> ord[2], tab, counter, inner, rank, index;
+    DeviceArray<uint32_t> ord[2], tab, counter, inner, rank, index, chunk_left, chunk_base, big_count;
     DeviceArray<lr_bvh_node> tmp;
+    DeviceArray<BigNode> bigs_d, next_bigs_d;
+    DeviceArray<Chunk> chunks_d;
+    DeviceArray<BigState> states;
     int rc;
     const size_t cap = 2 * n;
+    // Large nodes of one level are disjoint: at most n / kBigNode of them.
+    const size_t max_big = n / kBigNode + 1, max_chunks = n / kChunk + max_big + 1;
     if ((rc = blo.alloc(n, "hipMalloc(build boxes)")) || (rc = bhi.alloc(n, "hipMalloc(build boxes)")) ||
         (rc = ord[0].alloc(n, "hipMalloc(build order)")) || (rc = ord[1].alloc(n, "hipMalloc(build order)")) ||
         (rc = tab.alloc(n, "hipMalloc(build tables)")) || (rc = counter.alloc(1, "hipMalloc(build counter)")) ||
-        (rc = tmp.alloc(cap, "hipMalloc(build nodes)")))
+        (rc = tmp.alloc(cap, "hipMalloc(build nodes)")) || (rc = bigs_d.alloc(max_big, "hipMalloc(build)")) ||
+        (rc = next_bigs_d.alloc(max_big, "hipMalloc(build)")) ||
+        (rc = states.alloc(max_big * kReplicas, "hipMalloc(build)")) ||
+        (rc = chunks_d.alloc(max_chunks, "hipMalloc(build)")) || (rc = chunk_left.alloc(max_chunks, "hipMalloc(build)")) ||
+        (rc = chunk_base.alloc(max_chunks, "hipMalloc(build)")) || (rc = big_count.alloc(1, "hipMalloc(build)")))
         return rc;
     const uint32_t n32 = (uint32_t)n;
     hipLaunchKernelGGL(prep_kernel, dim3((n32 + 255) / 256), dim3(256), 0, s, positions, indices, n32, blo.p, bhi.p,
@@ -401,17 +729,58 @@ int build_bvh_device(lr_device *dev, const float *positions, const uint32_t *ind
         (e = hipMemcpyAsync(counter.p, &one, 4, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(build root)");
     std::vector<std::pair<uint32_t, uint32_t>> levels{{0u, 1u}};
+    std::vector<BigNode> bigs;
+    std::vector<Chunk> chunks;
+    if (n > kBigNode) bigs.push_back(BigNode{0u, 0u, n32, 0u, 0u});
     int c = 0;
     for (;;) {
         auto [begin, end] = levels.back();
-        LevelArgs a{blo.p, bhi.p, ord[c].p, ord[c ^ 1].p, order, tab.p, tmp.p, counter.p, begin, end, 0xffffffffu};
-        uint32_t blocks = std::min<uint32_t>(end - begin, 65536u);
+        const uint32_t nbig = (uint32_t)bigs.size();
+        if (nbig) {
+            chunks.clear();
+            for (uint32_t j = 0; j < nbig; j++) {
+                BigNode &bn = bigs[j];
+                bn.chunk0 = (uint32_t)chunks.size();
+                bn.chunks = (bn.count + kChunk - 1) / kChunk;
+                for (uint32_t q = 0; q < bn.chunks; q++)
+                    chunks.push_back(Chunk{j, bn.first + q * kChunk, bn.first + std::min(bn.count, (q + 1) * kChunk)});
+            }
+            if (nbig > max_big || chunks.size() > max_chunks)
+                return fail(LR_DEVICE_ERROR, "BVH build: more large nodes than a level can hold");
+            const uint32_t nch = (uint32_t)chunks.size();
+            if ((e = hipMemcpyAsync(bigs_d.p, bigs.data(), nbig * sizeof(BigNode), hipMemcpyHostToDevice, s)) != hipSuccess ||
+                (e = hipMemcpyAsync(chunks_d.p, chunks.data(), nch * sizeof(Chunk), hipMemcpyHostToDevice, s)) != hipSuccess ||
+                (e = hipMemsetAsync(big_count.p, 0, 4, s)) != hipSuccess)
+                return hip_fail(e, "hipMemcpy(build large nodes)");
+            BigArgs ba{blo.p, bhi.p, ord[c].p, ord[c ^ 1].p, tab.p, tmp.p, counter.p, bigs_d.p, chunks_d.p, states.p,
+                       chunk_left.p, chunk_base.p, next_bigs_d.p, big_count.p};
+            hipLaunchKernelGGL(big_init_kernel, dim3(nbig * kReplicas), dim3(64), 0, s, states.p);
+            hipLaunchKernelGGL(big_bounds_kernel, dim3(nch), dim3(256), 0, s, ba);
+            hipLaunchKernelGGL(big_merge_bounds_kernel, dim3(nbig), dim3(64), 0, s, states.p);
+            hipLaunchKernelGGL(big_bins_kernel, dim3(nch), dim3(256), 0, s, ba);
+            hipLaunchKernelGGL(big_split_kernel, dim3(nbig), dim3(256), 0, s, ba, kBigNode);
+            hipLaunchKernelGGL(big_count_kernel, dim3(nch), dim3(256), 0, s, ba);
+            hipLaunchKernelGGL(big_scan_kernel, dim3(nbig), dim3(64), 0, s, ba);
+            hipLaunchKernelGGL(big_partition_kernel<false>, dim3(nch), dim3(256), 0, s, ba);
+            hipLaunchKernelGGL(big_partition_kernel<true>, dim3(nch), dim3(256), 0, s, ba);
+        }
+        // One node per wave while that fills the GPU, up to 64 per wave beyond.
+        const uint32_t batch = std::min<uint32_t>(64u, std::max<uint32_t>(1u, (end - begin) / 16384u));
+        LevelArgs a{blo.p, bhi.p, ord[c].p, ord[c ^ 1].p, order, tab.p, tmp.p, counter.p, begin, end, kBigNode, batch};
+        uint32_t blocks = std::min<uint32_t>((end - begin + batch - 1) / batch, 65536u);
         hipLaunchKernelGGL(level_kernel, dim3(blocks), dim3(64), 0, s, a);
-        uint32_t used = 0;
+        uint32_t used = 0, next_big = 0;
         if ((e = hipMemcpyAsync(&used, counter.p, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (nbig && (e = hipMemcpyAsync(&next_big, big_count.p, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
             (e = hipStreamSynchronize(s)) != hipSuccess)
             return hip_fail(e, "BVH build level");
         if (used > cap) return fail(LR_DEVICE_ERROR, "BVH build handed out more nodes than the tree can hold");
+        if (next_big > max_big) return fail(LR_DEVICE_ERROR, "BVH build: more large nodes than a level can hold");
+        bigs.resize(next_big);
+        if (next_big && ((e = hipMemcpyAsync(bigs.data(), next_bigs_d.p, next_big * sizeof(BigNode), hipMemcpyDeviceToHost,
+                                             s)) != hipSuccess ||
+                         (e = hipStreamSynchronize(s)) != hipSuccess))
+            return hip_fail(e, "hipMemcpy(build large nodes)");
         if (used == end) break;
         if (levels.size() >= 4096) return fail(LR_LIMIT, "BVH deeper than 4096 levels");
         levels.push_back({end, used});
