@@ -85,6 +85,23 @@ int lr_bvh_build_device(lr_device *device, const float *positions, size_t points
  * repacked on upload into the traversal layouts (width 2, 4 and 8, 64/128/256
  * byte nodes) and a leaf-order triangle array (p0, e1, e2). */
 int lr_scene_upload(lr_device *device, const lr_scene_arrays *arrays, lr_gpu_scene **out);
+/* compile_scene's build_bvh and reorder (bvh.lucb:66-217) followed by
+ * GpuScene.upload (device.lucb:41-89), all on the device: `arrays` in the
+ * order the scene was flattened (nodes ignored); the BVH is built on the GPU
+ * (the tree lr_bvh_build gives), the per-triangle arrays are put in leaf order
+ * there, and the traversal layouts collapsed there. When given, `nodes` (room
+ * for max(1, 2 * triangles)), `node_count` and `order` receive the tree and
+ * leaf order for the host's copy of the scene. */
+int lr_scene_build(lr_device *device, const lr_scene_arrays *arrays, lr_gpu_scene **out, lr_bvh_node *nodes,
+                   size_t *node_count, uint32_t *order);
+
+/* The width-`width` traversal nodes as uploaded (`bytes` receives their size;
+ * with `out` NULL only the size), and the host collapse of a binary tree into
+ * the same layout (the repacking of bvh_wide.cpp, for comparison). */
+int lr_scene_read_layout(const lr_gpu_scene *scene, int width, void *out, size_t capacity, size_t *bytes);
+int lr_bvh_collapse(const lr_bvh_node *nodes, size_t node_count, int width, void *out, size_t capacity, size_t *bytes,
+                    uint32_t *stack_bound);
+
 /* GpuScene.destroy (device.lucb:108). */
 void lr_scene_destroy(lr_gpu_scene *scene);
 

@@ -145,13 +145,88 @@ int lr_bvh_build(const float *positions, size_t points, const uint32_t *indices,
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+template <class T> struct Scratch {
+    T *p = nullptr;
+    ~Scratch() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t count, const char *what) {
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+        return e == hipSuccess ? 0 : hip_fail(e, what);
+    }
+    int put(const T *src, size_t count, hipStream_t s, const char *what) {
+        int rc = alloc(count, what);
+        if (rc || !count) return rc;
+        hipError_t e = hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+        return e == hipSuccess ? 0 : hip_fail(e, what);
+    }
+};
+
+// The scene arrays' common checks (device.lucb:41-89 rejects what the
+// kernels could not index).
+int check_arrays(const lr_scene_arrays *a, bool need_nodes) {
+    if (a->triangles && (!a->positions || !a->indices || !a->faces))
+        return fail(LR_INVALID, "a scene with triangles needs positions, indices and faces");
+    if (need_nodes && (!a->nodes || a->node_count == 0)) return fail(LR_INVALID, "a scene needs its BVH (one node at least)");
+    if (a->triangles >= (1ull << 27)) return fail(LR_LIMIT, "a scene holds at most 2^27 triangles");
+    if (a->points > 4294967295ull) return fail(LR_LIMIT, "a scene holds at most 2^32 points");
+    for (size_t i = 0; i < a->triangles * 3; i++)
+        if (a->indices[i] >= a->points) return fail(LR_INVALID, "a triangle names a point the scene lacks");
+    return 0;
+}
+
+// The traversal layouts and triangles of a scene whose leaf-order indices and
+// binary tree are in device memory: triangles packed, widths 2, 4, 8
+// collapsed, all on the device.
+int finish_scene(lr_gpu_scene *s, const float *positions, const uint32_t *indices, const lr_bvh_node *nodes,
+                 size_t node_count) {
+    lr_device *device = s->device;
+    hipStream_t st = device->stream;
+    hipError_t e;
+    const size_t n = s->info.triangles;
+    if ((e = hipMalloc(&s->tris, std::max<size_t>(n, 1) * 48)) != hipSuccess) return hip_fail(e, "hipMalloc(triangles)");
+    pack_triangles_device(device, positions, indices, n, static_cast<float4 *>(s->tris));
+    lr_bvh_node root;
+    if ((e = hipMemcpyAsync(&root, nodes, sizeof(root), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(root)");
+    std::memcpy(s->root_lo, root.lo, 12);
+    std::memcpy(s->root_hi, root.hi, 12);
+    const bool empty = n == 0 || root.lo[0] > root.hi[0];
+    const int widths[3] = {2, 4, 8};
+    for (int wi = 0; wi < 3; wi++) {
+        DeviceLayout layout;
+        int rc = collapse_wide_device(device, nodes, node_count, empty, widths[wi], layout);
+        if (rc) return rc;
+        s->wide[wi] = layout.data;
+        if (wi == 0) s->ranges = layout.ranges;
+        s->info.wide_nodes[wi] = layout.nodes;
+        s->info.wide_bytes[wi] = layout.bytes;
+        s->info.stack_bound[wi] = layout.stack_bound;
+    }
+    s->info.triangle_bytes = n * 48;
+    if ((e = hipMalloc((void **)&s->work, 64)) != hipSuccess || (e = hipMemsetAsync(s->work, 0, 64, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "ray counters");
+    // Default traversal: width 4, 64-wide groups, 24-entry LDS stack, batches
+    // refilled at 32 idle lanes (the fastest on the 30M-triangle scene).
+    s->config = lr_trace_config{4, 64, 24, 32};
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int lr_scene_upload(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **out) {
     if (!device || !a || !out) return fail(LR_INVALID, "lr_scene_upload needs a device, arrays and a handle");
     *out = nullptr;
-    if (a->triangles && (!a->positions || !a->indices || !a->faces))
-        return fail(LR_INVALID, "a scene with triangles needs positions, indices and faces");
-    if (!a->nodes || a->node_count == 0) return fail(LR_INVALID, "a scene needs its BVH (one node at least)");
-    if (a->triangles >= (1ull << 27)) return fail(LR_LIMIT, "a scene holds at most 2^27 triangles");
+    int rc;
+    if ((rc = check_arrays(a, true))) return rc;
     if (a->triangles && a->node_count > 2 * a->triangles) return fail(LR_INVALID, "more BVH nodes than 2 per triangle");
     // The nodes must describe a tree over exactly these triangles.
     for (size_t i = 0; i < a->node_count; i++) {
@@ -163,54 +238,98 @@ int lr_scene_upload(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **
             return fail(LR_INVALID, "a BVH leaf names triangles past the array or more than 16");
         }
     }
-    for (size_t i = 0; i < a->triangles * 3; i++)
-        if (a->indices[i] >= a->points) return fail(LR_INVALID, "a triangle names a point the scene lacks");
     (void)hipSetDevice(device->ordinal);
+    hipStream_t st = device->stream;
     std::unique_ptr<lr_gpu_scene> s(new lr_gpu_scene());
     s->device = device;
     s->info.triangles = a->triangles;
     s->info.binary_nodes = a->node_count;
-    std::memcpy(s->root_lo, a->nodes[0].lo, 12);
-    std::memcpy(s->root_hi, a->nodes[0].hi, 12);
-    // Leaf-order triangles as p0, e1 = p1 - p0, e2 = p2 - p0 (f32, as
-    // triangle_hit computes them, bvh.glsl:28-30).
-    std::vector<float> tri(std::max<size_t>(a->triangles, 1) * 12, 0.0f);
-    for (size_t t = 0; t < a->triangles; t++) {
-        const float *p0 = a->positions + (size_t)a->indices[t * 3] * 3;
-        const float *p1 = a->positions + (size_t)a->indices[t * 3 + 1] * 3;
-        const float *p2 = a->positions + (size_t)a->indices[t * 3 + 2] * 3;
-        float *o = &tri[t * 12];
-        o[0] = p0[0];
-        o[1] = p0[1];
-        o[2] = p0[2];
-        o[3] = p1[0] - p0[0];
-        o[4] = p1[1] - p0[1];
-        o[5] = p1[2] - p0[2];
-        o[6] = p2[0] - p0[0];
-        o[7] = p2[1] - p0[1];
-        o[8] = p2[2] - p0[2];
-    }
-    int rc;
-    if ((rc = upload(&s->tris, tri.data(), a->triangles * 12, "upload triangles"))) return rc;
+    Scratch<float> pos;
+    Scratch<uint32_t> idx;
+    Scratch<lr_bvh_node> nodes;
+    if ((rc = pos.put(a->positions, a->triangles ? a->points * 3 : 0, st, "upload points")) ||
+        (rc = idx.put(a->indices, a->triangles * 3, st, "upload indices")) ||
+        (rc = nodes.put(a->nodes, a->node_count, st, "upload BVH")))
+        return rc;
     if ((rc = upload(&s->faces, a->faces, a->triangles, "upload faces"))) return rc;
     if (a->shading && (rc = upload(&s->shading, a->shading, a->triangles * 4, "upload shading"))) return rc;
     if (a->uvs && (rc = upload(&s->uvs, a->uvs, a->triangles * 6, "upload uvs"))) return rc;
-    const int widths[3] = {2, 4, 8};
-    for (int wi = 0; wi < 3; wi++) {
-        WideLayout layout = collapse_wide(a->nodes, a->node_count, widths[wi]);
-        s->info.wide_nodes[wi] = layout.nodes;
-        s->info.wide_bytes[wi] = layout.bytes.size();
-        s->info.stack_bound[wi] = layout.stack_bound;
-        if ((rc = upload(&s->wide[wi], layout.bytes.data(), layout.bytes.size(), "upload wide nodes"))) return rc;
-        if (wi == 0 && (rc = upload(&s->ranges, layout.ranges.data(), layout.ranges.size(), "upload node ranges")))
-            return rc;
-    }
-    s->info.triangle_bytes = a->triangles * 48;
-    if ((rc = upload((void **)&s->work, (const uint32_t *)nullptr, 0, "ray counters"))) return rc;
-    // Default traversal: width 4, 64-wide groups, 24-entry LDS stack, batches
-    // refilled at 32 idle lanes (the fastest on the 30M-triangle scene).
-    s->config = lr_trace_config{4, 64, 24, 32};
+    if ((rc = finish_scene(s.get(), pos.p, idx.p, nodes.p, a->node_count))) return rc;
     *out = s.release();
+    return 0;
+}
+
+int lr_scene_build(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **out, lr_bvh_node *nodes_out,
+                   size_t *node_count, uint32_t *order_out) {
+    if (!device || !a || !out) return fail(LR_INVALID, "lr_scene_build needs a device, arrays and a handle");
+    *out = nullptr;
+    int rc;
+    if ((rc = check_arrays(a, false))) return rc;
+    (void)hipSetDevice(device->ordinal);
+    hipStream_t st = device->stream;
+    hipError_t e;
+    const size_t n = a->triangles;
+    std::unique_ptr<lr_gpu_scene> s(new lr_gpu_scene());
+    s->device = device;
+    s->info.triangles = n;
+    Scratch<float> pos;
+    Scratch<uint32_t> idx, idx_leaf, order;
+    Scratch<lr_bvh_node> nodes;
+    if ((rc = pos.put(a->positions, n ? a->points * 3 : 0, st, "upload points")) ||
+        (rc = idx.put(a->indices, n * 3, st, "upload indices")) || (rc = idx_leaf.alloc(n * 3, "hipMalloc(indices)")) ||
+        (rc = order.alloc(n, "hipMalloc(order)")) || (rc = nodes.alloc(std::max<size_t>(1, 2 * n), "hipMalloc(BVH)")))
+        return rc;
+    size_t count = 0;
+    if ((rc = build_bvh_device(device, pos.p, idx.p, n, nodes.p, order.p, &count))) return rc;
+    s->info.binary_nodes = count;
+    // Per-triangle arrays into leaf order (bvh.lucb:198-217), on the device.
+    auto reorder = [&](const void *host, uint32_t words, void **dst, const char *what) -> int {
+        Scratch<uint32_t> src;
+        int r = src.put(static_cast<const uint32_t *>(host), n * words, st, what);
+        if (r) return r;
+        hipError_t e2 = hipMalloc(dst, std::max<size_t>(n * words * 4, 64));
+        if (e2 != hipSuccess) return hip_fail(e2, what);
+        gather_rows_device(device, src.p, order.p, n, words, static_cast<uint32_t *>(*dst));
+        e2 = hipStreamSynchronize(st);  // the source scratch goes out of scope
+        return e2 == hipSuccess ? 0 : hip_fail(e2, what);
+    };
+    gather_rows_device(device, idx.p, order.p, n, 3, idx_leaf.p);
+    if ((rc = reorder(a->faces, 1, &s->faces, "upload faces"))) return rc;
+    if (a->shading && (rc = reorder(a->shading, 4, &s->shading, "upload shading"))) return rc;
+    if (a->uvs && (rc = reorder(a->uvs, 6, &s->uvs, "upload uvs"))) return rc;
+    if ((rc = finish_scene(s.get(), pos.p, idx_leaf.p, nodes.p, count))) return rc;
+    if (nodes_out && (e = hipMemcpyAsync(nodes_out, nodes.p, count * sizeof(lr_bvh_node), hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(BVH)");
+    if (order_out && n && (e = hipMemcpyAsync(order_out, order.p, n * 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(order)");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "lr_scene_build");
+    if (node_count) *node_count = count;
+    *out = s.release();
+    return 0;
+}
+
+int lr_scene_read_layout(const lr_gpu_scene *scene, int width, void *out, size_t capacity, size_t *bytes) {
+    if (!scene || !bytes) return fail(LR_INVALID, "lr_scene_read_layout needs a scene and a size");
+    int wi = width == 2 ? 0 : width == 4 ? 1 : width == 8 ? 2 : -1;
+    if (wi < 0) return fail(LR_INVALID, "widths are 2, 4, 8");
+    *bytes = scene->info.wide_bytes[wi];
+    if (!out) return 0;
+    if (capacity < *bytes) return fail(LR_INVALID, "the buffer is smaller than the layout");
+    (void)hipSetDevice(scene->device->ordinal);
+    hipError_t e = hipMemcpy(out, scene->wide[wi], *bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? 0 : hip_fail(e, "hipMemcpy(wide nodes)");
+}
+
+int lr_bvh_collapse(const lr_bvh_node *nodes, size_t node_count, int width, void *out, size_t capacity, size_t *bytes,
+                    uint32_t *stack_bound) {
+    if (!nodes || !bytes || (width != 2 && width != 4 && width != 8))
+        return fail(LR_INVALID, "lr_bvh_collapse needs nodes, a size and a width of 2, 4 or 8");
+    WideLayout layout = collapse_wide(nodes, node_count, width);
+    *bytes = layout.bytes.size();
+    if (stack_bound) *stack_bound = layout.stack_bound;
+    if (!out) return 0;
+    if (capacity < *bytes) return fail(LR_INVALID, "the buffer is smaller than the layout");
+    std::memcpy(out, layout.bytes.data(), *bytes);
     return 0;
 }
 

@@ -52,4 +52,14 @@ struct WideLayout {
 // test is the one the binary traversal makes for that node.
 WideLayout collapse_wide(const lr_bvh_node *nodes, size_t node_count, int width);
 
+// ------------------------------------------------------ device-side repacking
+
+// A wide layout in device memory (collapse_wide's bytes, built on the GPU).
+struct DeviceLayout {
+    void *data = nullptr;    // nodes * node_bytes
+    void *ranges = nullptr;  // width 2: uint2 slot range per node
+    size_t node_bytes = 0, nodes = 0, bytes = 0;
+    uint32_t stack_bound = 0;
+};
+
 }  // namespace lr

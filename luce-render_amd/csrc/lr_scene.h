@@ -38,3 +38,19 @@ struct lr_gpu_scene {
 namespace lr {
 using GpuScene = lr_gpu_scene;
 }
+
+namespace lr {
+// Device BVH build (bvh_build_gpu.hip): `n` triangles whose points and
+// indices are in device memory; `out` (room for max(1, 2n) nodes) and `order`
+// in device memory receive the reference's tree and leaf order.
+int build_bvh_device(lr_device *dev, const float *positions, const uint32_t *indices, size_t n, lr_bvh_node *out,
+                     uint32_t *order, size_t *node_count);
+// Device collapse (bvh_collapse_gpu.hip) of the binary tree in device memory.
+int collapse_wide_device(lr_device *dev, const lr_bvh_node *nodes, size_t node_count, bool empty, int width,
+                         DeviceLayout &layout);
+// Leaf-order triangles (p0, e1, e2 as 3 float4) from device points and indices.
+void pack_triangles_device(lr_device *dev, const float *positions, const uint32_t *indices, size_t n, float4 *tris);
+// dst[t * words + w] = src[order[t] * words + w]: per-triangle rows into leaf order.
+void gather_rows_device(lr_device *dev, const uint32_t *src, const uint32_t *order, size_t n, uint32_t words,
+                        uint32_t *dst);
+}  // namespace lr

@@ -92,6 +92,11 @@ _SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_size_t), _P]),
     "lr_scene_upload": (ctypes.c_int, [_P, ctypes.POINTER(SceneArrays), ctypes.POINTER(_P)]),
     "lr_scene_destroy": (None, [_P]),
+    "lr_scene_build": (ctypes.c_int, [_P, ctypes.POINTER(SceneArrays), ctypes.POINTER(_P), _P,
+                                      ctypes.POINTER(ctypes.c_size_t), _P]),
+    "lr_scene_read_layout": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "lr_bvh_collapse": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int, _P, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_uint32)]),
     "lr_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, ctypes.c_size_t]),
     "lr_scene_set_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
     "lr_scene_get_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import INVALID, GpuUnavailable, LuceError, TraceConfig, SceneInfo
-from .scene import (Camera, CompiledScene, GeometrySet, Instance, Light, Mesh, Stopping, build_bvh,
+from .scene import (NODE_DTYPE, Camera, CompiledScene, GeometrySet, Instance, Light, Mesh, Stopping, build_bvh,
                     compile_scene, octahedral, decode_octahedral)
 
 __all__ = ["Device", "GpuScene", "RayFrame", "ray_frame", "look_frame", "probe", "Probe", "compile_scene",
@@ -52,9 +52,7 @@ class GpuScene:
         self.triangles = triangles
 
     @staticmethod
-    def upload(device, scene: CompiledScene, accelerate=True):
-        """GpuScene.upload (device.lucb:41-89); `accelerate` is accepted for the
-        reference's signature (there is no hardware ray tracing here)."""
+    def _arrays(scene, with_nodes):
         arrays = _lib.SceneArrays()
         arrays.positions = _lib.ptr(scene.positions)
         arrays.points = scene.point_count()
@@ -63,11 +61,52 @@ class GpuScene:
         arrays.shading = _lib.ptr(scene.shading)
         arrays.uvs = _lib.ptr(scene.uvs)
         arrays.triangles = scene.triangle_count()
-        arrays.nodes = _lib.ptr(scene.nodes)
-        arrays.node_count = len(scene.nodes)
+        if with_nodes:
+            arrays.nodes = _lib.ptr(scene.nodes)
+            arrays.node_count = len(scene.nodes)
+        return arrays
+
+    @staticmethod
+    def upload(device, scene: CompiledScene, accelerate=True):
+        """GpuScene.upload (device.lucb:41-89); `accelerate` is accepted for the
+        reference's signature (there is no hardware ray tracing here)."""
+        arrays = GpuScene._arrays(scene, True)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().lr_scene_upload(device.handle, ctypes.byref(arrays), ctypes.byref(h)))
         return GpuScene(h, device, scene.triangle_count())
+
+    @staticmethod
+    def build(device, scene: CompiledScene, keep_tree=True):
+        """build_bvh and GpuScene.upload in one step on the device, for a scene
+        compiled with geometry=False (triangles in flattening order). With
+        `keep_tree` the scene receives its nodes and leaf order and its
+        per-triangle arrays are reordered, as build_bvh leaves them."""
+        n = scene.triangle_count()
+        arrays = GpuScene._arrays(scene, False)
+        h = ctypes.c_void_p()
+        nodes = np.zeros(max(1, 2 * n), NODE_DTYPE) if keep_tree else None
+        order = np.zeros(max(1, n), np.uint32) if keep_tree else None
+        count = ctypes.c_size_t(0)
+        _lib.check(_lib.lib().lr_scene_build(device.handle, ctypes.byref(arrays), ctypes.byref(h),
+                                             _lib.ptr(nodes) if keep_tree else None, ctypes.byref(count),
+                                             _lib.ptr(order) if keep_tree else None))
+        if keep_tree:
+            scene.nodes = nodes[:count.value].copy()
+            scene.order = order[:n].copy()
+            if n:
+                scene.indices = np.ascontiguousarray(scene.indices[scene.order])
+                scene.faces = np.ascontiguousarray(scene.faces[scene.order])
+                scene.shading = np.ascontiguousarray(scene.shading[scene.order])
+                scene.uvs = np.ascontiguousarray(scene.uvs[scene.order])
+        return GpuScene(h, device, n)
+
+    def layout(self, width):
+        """The width-`width` traversal nodes as uploaded (raw bytes)."""
+        size = ctypes.c_size_t(0)
+        _lib.check(_lib.lib().lr_scene_read_layout(self.handle, width, None, 0, ctypes.byref(size)))
+        out = np.zeros(size.value, np.uint8)
+        _lib.check(_lib.lib().lr_scene_read_layout(self.handle, width, _lib.ptr(out), size.value, ctypes.byref(size)))
+        return out
 
     def set_trace_config(self, width=4, group=64, stack=24, refill=32):
         """BVH width, workgroup size, LDS stack entries per lane, and the ray

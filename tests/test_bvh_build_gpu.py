@@ -113,3 +113,70 @@ def test_device_bvh_equals_host_build_on_instanced_scene(device):
     _same_tree(nodes, host.nodes)
     assert np.array_equal(order, host.order)
     assert O.check_bvh(nodes, p, idx[order]) == 0
+
+
+def _host_layout(nodes, width):
+    from luce_render import _lib
+    import ctypes
+    size, bound = ctypes.c_size_t(0), ctypes.c_uint32(0)
+    nodes = np.ascontiguousarray(nodes)
+    _lib.check(_lib.lib().lr_bvh_collapse(_lib.ptr(nodes), len(nodes), width, None, 0, ctypes.byref(size),
+                                          ctypes.byref(bound)))
+    out = np.zeros(size.value, np.uint8)
+    _lib.check(_lib.lib().lr_bvh_collapse(_lib.ptr(nodes), len(nodes), width, _lib.ptr(out), size.value,
+                                          ctypes.byref(size), ctypes.byref(bound)))
+    return out, bound.value
+
+
+@pytest.mark.parametrize("which", ["empty", "one", "soup", "degenerate", "cornell", "instanced"])
+def test_device_collapse_equals_host_collapse(device, cornell_probe, which):
+    """Upload collapses the tree on the device into the host collapse's bytes
+    (bvh_wide.cpp) at widths 2, 4, 8, with the same stack bounds."""
+    from luce_render import render, scenes
+    if which == "cornell":
+        sc = cornell_probe
+    else:
+        if which == "instanced":
+            flat = render.compile_scene(scenes.instanced_set(instances=20), geometry=False)
+            p, idx = flat.positions, flat.indices
+        elif which == "degenerate":
+            p, idx = _degenerate("coincident", 3000)
+        else:
+            p, idx = _soup({"empty": 0, "one": 1, "soup": 50000}[which], 9)
+        sc = _scene(p, idx)
+        render.build_bvh(sc)
+    say(f"device collapse: {which}, {sc.triangle_count()} triangles")
+    g = render.GpuScene.upload(device, sc)
+    info = g.info()
+    for wi, width in enumerate((2, 4, 8)):
+        want, bound = _host_layout(sc.nodes, width)
+        got = g.layout(width)
+        assert np.array_equal(got, want), width
+        assert info["stack_bound"][wi] == bound
+    g.destroy()
+
+
+def test_scene_build_on_device_equals_compile_then_upload(device, cornell_probe):
+    """lr_scene_build (device BVH build, reorder, collapse) gives the scene
+    that compile_scene + upload give: same tree and order, same layouts, the
+    same probe image bit for bit."""
+    from luce_render import render, scenes
+    gset = scenes.cornell_probe_set(random_count=10000)
+    ref = render.compile_scene(gset)
+    flat = render.compile_scene(gset, geometry=False)
+    g = render.GpuScene.build(device, flat)
+    _same_tree(flat.nodes, ref.nodes)
+    assert np.array_equal(flat.order, ref.order)
+    for a in ("indices", "faces", "shading", "uvs"):
+        assert np.array_equal(getattr(flat, a), getattr(ref, a)), a
+    gr = render.GpuScene.upload(device, ref)
+    for width in (2, 4, 8):
+        assert np.array_equal(g.layout(width), gr.layout(width)), width
+    frame = render.ray_frame(ref.cameras[0], 256, 256)
+    f1, t1 = render.probe_arrays(device, g, frame)
+    f2, t2 = render.probe_arrays(device, gr, frame)
+    assert np.array_equal(f1, f2) and np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
+    mf, mt, _, _ = O.probe(ref.nodes, ref.positions, ref.indices, ref.faces, frame.pack(), 256, 256)
+    assert np.array_equal(f1, mf) and np.array_equal(t1.view(np.uint32), mt.view(np.uint32))
+    g.destroy()
+    gr.destroy()
