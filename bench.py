@@ -132,6 +132,8 @@ def main():
                     help="requests of the Poisson serving leg (40 requests/s over the four serving scenes; 0: skip)")
     ap.add_argument("--guiding", default="960x540:1024",
                     help="WxH:reference-spp of the guiding leg on the glossy workload (empty: skip)")
+    ap.add_argument("--build", type=int, default=3,
+                    help="device scene builds of the same scene to time against the host pipeline (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the plain copy kernel over this many MiB (a known-traffic dispatch "
                          "for scaling hardware byte counters)")
@@ -237,6 +239,51 @@ def main():
         dom_bytes, dom_ms, dom_name = bytes_p * n, ms_p, "probe" + suffix
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(f"{dom_name}<{best[0]},{best[2]},{best[1]}>")
+
+    # Scene build on the device (lr_scene_build): the flattened arrays in
+    # their original order (recovered through the leaf order) to a traceable
+    # scene - BVH build, reorder, collapse to widths 2/4/8, triangle packing -
+    # against the host pipeline it replaces: build_bvh with its reorder on all
+    # host cores, then the host collapse of the three widths.
+    build_stats = None
+    if args.build > 0:
+        import ctypes
+        from luce_render import _lib
+        inv = np.empty_like(sc.order)
+        inv[sc.order] = np.arange(len(sc.order), dtype=np.uint32)
+
+        def flat_scene():
+            f = render.CompiledScene()
+            f.positions = sc.positions
+            f.indices, f.faces = np.ascontiguousarray(sc.indices[inv]), np.ascontiguousarray(sc.faces[inv])
+            f.shading, f.uvs = np.ascontiguousarray(sc.shading[inv]), np.ascontiguousarray(sc.uvs[inv])
+            return f
+
+        flat = flat_scene()
+        dev_s = []
+        for _ in range(args.build):
+            t1 = time.perf_counter()
+            g = render.GpuScene.build(device, flat, keep_tree=False)
+            render.synchronize(device)
+            dev_s.append(time.perf_counter() - t1)
+            same = all(np.array_equal(g.layout(w), gpu.layout(w)) for w in (2, 4, 8))
+            g.destroy()
+        host = flat_scene()
+        host_threads = min(16, os.cpu_count() or 1)
+        t1 = time.perf_counter()
+        render.build_bvh(host, threads=host_threads)
+        host_build = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        size = ctypes.c_size_t(0)
+        for w in (2, 4, 8):
+            _lib.check(_lib.lib().lr_bvh_collapse(_lib.ptr(host.nodes), len(host.nodes), w, None, 0,
+                                                  ctypes.byref(size), None))
+        host_collapse = time.perf_counter() - t1
+        build_stats = {"triangles": sc.triangle_count(), "device_s": round(float(np.median(dev_s)), 4),
+                       "host_build_s": round(host_build, 3), "host_collapse_s": round(host_collapse, 3),
+                       "host_threads": host_threads, "layouts_identical": bool(same),
+                       "speedup": round((host_build + host_collapse) / float(np.median(dev_s)), 1)}
+        log("scene build:", build_stats)
 
     # The full wavefront path tracer on the same scene: RGB, 12 bounces,
     # 256 lights in the light tree, light tracing at the reference's default.
@@ -350,6 +397,7 @@ def main():
             "path_tracing": render_stats,
             "serving": serving_stats,
             "guiding": guiding_stats,
+            "scene_build": build_stats,
             "wide_bytes": info["wide_bytes"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
