@@ -279,7 +279,16 @@ def main():
             _lib.check(_lib.lib().lr_bvh_collapse(_lib.ptr(host.nodes), len(host.nodes), w, None, 0,
                                                   ctypes.byref(size), None))
         host_collapse = time.perf_counter() - t1
+        # Refit of the uploaded scene to moved points (same triangles), then
+        # back: the first call also records the layouts' child slots.
+        moved = (sc.positions * np.float32(1.001)).astype(np.float32)
+        refit_s = []
+        for pts in (moved, sc.positions, moved, sc.positions):
+            t1 = time.perf_counter()
+            gpu.refit(pts)
+            refit_s.append(time.perf_counter() - t1)
         build_stats = {"triangles": sc.triangle_count(), "device_s": round(float(np.median(dev_s)), 4),
+                       "refit_first_s": round(refit_s[0], 4), "refit_s": round(float(np.median(refit_s[1:])), 4),
                        "host_build_s": round(host_build, 3), "host_collapse_s": round(host_collapse, 3),
                        "host_threads": host_threads, "layouts_identical": bool(same),
                        "speedup": round((host_build + host_collapse) / float(np.median(dev_s)), 1)}

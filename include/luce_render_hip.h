@@ -95,6 +95,16 @@ int lr_scene_upload(lr_device *device, const lr_scene_arrays *arrays, lr_gpu_sce
 int lr_scene_build(lr_device *device, const lr_scene_arrays *arrays, lr_gpu_scene **out, lr_bvh_node *nodes,
                    size_t *node_count, uint32_t *order);
 
+/* Moves the scene's points (the same number it was built with; triangles,
+ * tree and leaf order unchanged) and refits on the device: every node's box
+ * recomputed bottom-up from the new points, each wide node keeping the
+ * children it was built with, the triangles repacked. No reference
+ * counterpart: the reference recompiles and re-uploads a changed scene; this
+ * serves animated geometry whose topology holds. Renderers on the scene trace
+ * the new geometry from their next pass (restart them with lr_renderer_reset to
+ * drop samples of the old); emitters' records come from the loaded arrays. */
+int lr_scene_refit(lr_gpu_scene *scene, const float *positions, size_t points);
+
 /* The width-`width` traversal nodes as uploaded (`bytes` receives their size;
  * with `out` NULL only the size), and the host collapse of a binary tree into
  * the same layout (the repacking of bvh_wide.cpp, for comparison). */

@@ -69,7 +69,8 @@ const lr::TraceVariant &lr_gpu_scene::variant() const {
 }
 
 lr_gpu_scene::~lr_gpu_scene() {
-    for (void *p : {wide[0], wide[1], wide[2], tris, faces, shading, uvs, spill, (void *)work, ranges})
+    for (void *p : {wide[0], wide[1], wide[2], tris, faces, shading, uvs, spill, (void *)work, ranges, (void *)binary,
+                    (void *)indices, (void *)parent, (void *)arrive, (void *)slots[0], (void *)slots[1], (void *)slots[2]})
         if (p) (void)hipFree(p);
 }
 
@@ -182,12 +183,14 @@ int check_arrays(const lr_scene_arrays *a, bool need_nodes) {
 // The traversal layouts and triangles of a scene whose leaf-order indices and
 // binary tree are in device memory: triangles packed, widths 2, 4, 8
 // collapsed, all on the device.
-int finish_scene(lr_gpu_scene *s, const float *positions, const uint32_t *indices, const lr_bvh_node *nodes,
-                 size_t node_count) {
+int finish_scene(lr_gpu_scene *s, const float *positions, uint32_t *indices, lr_bvh_node *nodes, size_t node_count) {
     lr_device *device = s->device;
     hipStream_t st = device->stream;
     hipError_t e;
     const size_t n = s->info.triangles;
+    // The tree and leaf-order indices stay for refit (the scene frees them).
+    s->binary = nodes;
+    s->indices = indices;
     if ((e = hipMalloc(&s->tris, std::max<size_t>(n, 1) * 48)) != hipSuccess) return hip_fail(e, "hipMalloc(triangles)");
     pack_triangles_device(device, positions, indices, n, static_cast<float4 *>(s->tris));
     lr_bvh_node root;
@@ -254,7 +257,11 @@ int lr_scene_upload(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **
     if ((rc = upload(&s->faces, a->faces, a->triangles, "upload faces"))) return rc;
     if (a->shading && (rc = upload(&s->shading, a->shading, a->triangles * 4, "upload shading"))) return rc;
     if (a->uvs && (rc = upload(&s->uvs, a->uvs, a->triangles * 6, "upload uvs"))) return rc;
-    if ((rc = finish_scene(s.get(), pos.p, idx.p, nodes.p, a->node_count))) return rc;
+    s->points = a->triangles ? a->points : 0;
+    rc = finish_scene(s.get(), pos.p, idx.p, nodes.p, a->node_count);
+    idx.p = nullptr;  // owned by the scene from here
+    nodes.p = nullptr;
+    if (rc) return rc;
     *out = s.release();
     return 0;
 }
@@ -297,8 +304,13 @@ int lr_scene_build(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **o
     if ((rc = reorder(a->faces, 1, &s->faces, "upload faces"))) return rc;
     if (a->shading && (rc = reorder(a->shading, 4, &s->shading, "upload shading"))) return rc;
     if (a->uvs && (rc = reorder(a->uvs, 6, &s->uvs, "upload uvs"))) return rc;
-    if ((rc = finish_scene(s.get(), pos.p, idx_leaf.p, nodes.p, count))) return rc;
-    if (nodes_out && (e = hipMemcpyAsync(nodes_out, nodes.p, count * sizeof(lr_bvh_node), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    s->points = n ? a->points : 0;
+    rc = finish_scene(s.get(), pos.p, idx_leaf.p, nodes.p, count);
+    lr_bvh_node *tree = nodes.p;
+    idx_leaf.p = nullptr;  // owned by the scene from here
+    nodes.p = nullptr;
+    if (rc) return rc;
+    if (nodes_out && (e = hipMemcpyAsync(nodes_out, tree, count * sizeof(lr_bvh_node), hipMemcpyDeviceToHost, st)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(BVH)");
     if (order_out && n && (e = hipMemcpyAsync(order_out, order.p, n * 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(order)");
@@ -306,6 +318,18 @@ int lr_scene_build(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **o
     if (node_count) *node_count = count;
     *out = s.release();
     return 0;
+}
+
+int lr_scene_refit(lr_gpu_scene *scene, const float *positions, size_t points) {
+    if (!scene || (points && !positions)) return fail(LR_INVALID, "lr_scene_refit needs a scene and its points");
+    if (points != scene->points)
+        return fail(LR_INVALID, "a refit moves the scene's points: the same number as it was built with");
+    (void)hipSetDevice(scene->device->ordinal);
+    if (scene->info.triangles == 0) return 0;
+    Scratch<float> pos;
+    int rc = pos.put(positions, points * 3, scene->device->stream, "upload points");
+    if (rc) return rc;
+    return refit_scene_device(scene, pos.p);
 }
 
 int lr_scene_read_layout(const lr_gpu_scene *scene, int width, void *out, size_t capacity, size_t *bytes) {

@@ -160,7 +160,8 @@ __global__ __launch_bounds__(256) void sums_kernel(const lr_bvh_node *nodes, Tem
 
 template <int W>
 __global__ __launch_bounds__(256) void write_kernel(const lr_bvh_node *nodes, Temp<W> t, uint32_t begin, uint32_t end,
-                                                    uint8_t *out, uint32_t node_bytes, uint2 *ranges) {
+                                                    uint8_t *out, uint32_t node_bytes, uint2 *ranges,
+                                                    uint32_t *slots) {
     const uint32_t id = begin + blockIdx.x * 256u + threadIdx.x;
     if (id >= end) return;
     const uint32_t a = t.alloc[id], at = t.index[id];
@@ -204,6 +205,8 @@ __global__ __launch_bounds__(256) void write_kernel(const lr_bvh_node *nodes, Te
     uint32_t *dst = reinterpret_cast<uint32_t *>(out + (size_t)at * node_bytes);
     for (uint32_t q = 0; q < kWords; q++) dst[q] = v.u[q];
     if (W == 2 && ranges) ranges[at] = t.range[id];
+    if (slots)
+        for (int i = 0; i < W; i++) slots[(size_t)at * W + i] = t.kids[(size_t)id * W + i];
 }
 
 // Leaf-order triangles: p0, e1 = p1 - p0, e2 = p2 - p0 (bvh.glsl:28-30).
@@ -240,7 +243,8 @@ template <class T> struct DeviceArray {
 };
 
 template <int W>
-int collapse(hipStream_t s, const lr_bvh_node *nodes, size_t node_count, DeviceLayout &layout, bool with_ranges) {
+int collapse(hipStream_t s, const lr_bvh_node *nodes, size_t node_count, DeviceLayout &layout, bool with_ranges,
+             bool with_slots) {
     layout.node_bytes = (sizeof(WideNode<W>) + 63) / 64 * 64;
     hipError_t e;
     // Interior binary nodes, plus the root: a bound on the wide nodes.
@@ -287,6 +291,8 @@ int collapse(hipStream_t s, const lr_bvh_node *nodes, size_t node_count, DeviceL
         if ((e = hipMalloc(&layout.ranges, (size_t)total * sizeof(uint2))) != hipSuccess)
             return hip_fail(e, "hipMalloc(node ranges)");
     }
+    if (with_slots && (e = hipMalloc(&layout.slots, (size_t)total * W * 4)) != hipSuccess)
+        return hip_fail(e, "hipMalloc(slot map)");
     const uint32_t root[2] = {0u, 1u};  // index 0, first child at 1
     if ((e = hipMemcpyAsync(index.p, &root[0], 4, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(alloc.p, &root[1], 4, hipMemcpyHostToDevice, s)) != hipSuccess)
@@ -294,7 +300,7 @@ int collapse(hipStream_t s, const lr_bvh_node *nodes, size_t node_count, DeviceL
     for (auto [begin, end] : levels)
         hipLaunchKernelGGL(write_kernel<W>, dim3((end - begin + 255) / 256), dim3(256), 0, s, nodes, t, begin, end,
                            static_cast<uint8_t *>(layout.data), (uint32_t)layout.node_bytes,
-                           static_cast<uint2 *>(layout.ranges));
+                           static_cast<uint2 *>(layout.ranges), static_cast<uint32_t *>(layout.slots));
     uint32_t bound = 0;
     if ((e = hipMemcpyAsync(&bound, below.p, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess)
@@ -306,7 +312,7 @@ int collapse(hipStream_t s, const lr_bvh_node *nodes, size_t node_count, DeviceL
 }  // namespace gwide
 
 int collapse_wide_device(lr_device *dev, const lr_bvh_node *nodes, size_t node_count, bool empty, int width,
-                         DeviceLayout &layout) {
+                         DeviceLayout &layout, bool with_slots) {
     hipStream_t s = dev->stream;
     layout = DeviceLayout{};
     if (empty) {
@@ -328,12 +334,12 @@ int collapse_wide_device(lr_device *dev, const lr_bvh_node *nodes, size_t node_c
         }
         return 0;
     }
-    int rc = width == 2 ? gwide::collapse<2>(s, nodes, node_count, layout, true)
-             : width == 4 ? gwide::collapse<4>(s, nodes, node_count, layout, false)
-                          : gwide::collapse<8>(s, nodes, node_count, layout, false);
+    int rc = width == 2 ? gwide::collapse<2>(s, nodes, node_count, layout, true, with_slots)
+             : width == 4 ? gwide::collapse<4>(s, nodes, node_count, layout, false, with_slots)
+                          : gwide::collapse<8>(s, nodes, node_count, layout, false, with_slots);
     if (rc) {
-        if (layout.data) (void)hipFree(layout.data);
-        if (layout.ranges) (void)hipFree(layout.ranges);
+        for (void *p : {layout.data, layout.ranges, layout.slots})
+            if (p) (void)hipFree(p);
         layout = DeviceLayout{};
     }
     return rc;

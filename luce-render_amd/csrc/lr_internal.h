@@ -58,6 +58,7 @@ WideLayout collapse_wide(const lr_bvh_node *nodes, size_t node_count, int width)
 struct DeviceLayout {
     void *data = nullptr;    // nodes * node_bytes
     void *ranges = nullptr;  // width 2: uint2 slot range per node
+    void *slots = nullptr;   // when asked: the binary node in each child slot (width per node)
     size_t node_bytes = 0, nodes = 0, bytes = 0;
     uint32_t stack_bound = 0;
 };

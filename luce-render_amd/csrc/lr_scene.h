@@ -23,6 +23,14 @@ struct lr_gpu_scene {
     uint64_t spill_entries = 0;
     uint32_t *work = nullptr;  // the persistent launches' two counters
     void *ranges = nullptr;    // width-2 nodes' triangle slot ranges (uint2 each)
+    // Kept for refit: the binary tree, leaf-order point indices, the point
+    // count; made at the first refit: parents, arrival counters and each wide
+    // layout's binary node per child slot.
+    lr_bvh_node *binary = nullptr;
+    uint32_t *indices = nullptr;
+    size_t points = 0;
+    uint32_t *parent = nullptr, *arrive = nullptr;
+    uint32_t *slots[3] = {nullptr, nullptr, nullptr};
     float root_lo[3] = {0, 0, 0};
     float root_hi[3] = {0, 0, 0};
     lr_scene_info info{};
@@ -47,7 +55,9 @@ int build_bvh_device(lr_device *dev, const float *positions, const uint32_t *ind
                      uint32_t *order, size_t *node_count);
 // Device collapse (bvh_collapse_gpu.hip) of the binary tree in device memory.
 int collapse_wide_device(lr_device *dev, const lr_bvh_node *nodes, size_t node_count, bool empty, int width,
-                         DeviceLayout &layout);
+                         DeviceLayout &layout, bool with_slots = false);
+// Refit (bvh_refit_gpu.hip): new points in device memory, same triangles.
+int refit_scene_device(lr_gpu_scene *scene, const float *positions);
 // Leaf-order triangles (p0, e1, e2 as 3 float4) from device points and indices.
 void pack_triangles_device(lr_device *dev, const float *positions, const uint32_t *indices, size_t n, float4 *tris);
 // dst[t * words + w] = src[order[t] * words + w]: per-triangle rows into leaf order.

@@ -94,6 +94,7 @@ _SIGNATURES = {
     "lr_scene_destroy": (None, [_P]),
     "lr_scene_build": (ctypes.c_int, [_P, ctypes.POINTER(SceneArrays), ctypes.POINTER(_P), _P,
                                       ctypes.POINTER(ctypes.c_size_t), _P]),
+    "lr_scene_refit": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "lr_scene_read_layout": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "lr_bvh_collapse": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int, _P, ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_uint32)]),

@@ -100,6 +100,12 @@ class GpuScene:
                 scene.uvs = np.ascontiguousarray(scene.uvs[scene.order])
         return GpuScene(h, device, n)
 
+    def refit(self, positions):
+        """Moves the scene's points (same count, same triangles) and refits
+        its boxes on the device; the tree's topology stays."""
+        positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        _lib.check(_lib.lib().lr_scene_refit(self.handle, _lib.ptr(positions), len(positions)))
+
     def layout(self, width):
         """The width-`width` traversal nodes as uploaded (raw bytes)."""
         size = ctypes.c_size_t(0)

@@ -180,3 +180,71 @@ def test_scene_build_on_device_equals_compile_then_upload(device, cornell_probe)
     assert np.array_equal(f1, mf) and np.array_equal(t1.view(np.uint32), mt.view(np.uint32))
     g.destroy()
     gr.destroy()
+
+
+def _refit_tree(nodes, positions, indices):
+    """The refit tree on the CPU: a leaf's box grows over its triangles'
+    corners, an interior node's is the union of its children's (children
+    follow their parent in the reference's numbering: one backward sweep)."""
+    out = nodes.copy()
+    tri = positions[indices]
+    tlo, thi = tri.min(1), tri.max(1)
+    for i in range(len(out) - 1, -1, -1):
+        c, f = int(out["count"][i]), int(out["first"][i])
+        if c:
+            out["lo"][i], out["hi"][i] = tlo[f:f + c].min(0), thi[f:f + c].max(0)
+        else:
+            out["lo"][i] = np.minimum(out["lo"][f], out["lo"][f + 1])
+            out["hi"][i] = np.maximum(out["hi"][f], out["hi"][f + 1])
+    return out
+
+
+def test_refit_moves_boxes_and_keeps_the_tree(device):
+    """lr_scene_refit: unchanged points leave every layout byte for byte; moved
+    points give the width-2 layout of the CPU-refit tree byte for byte, probe
+    images at every width equal to the oracle's mirror over the refit tree,
+    and moving back restores the original layouts."""
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.cornell_probe_set(random_count=10000))
+    g = render.GpuScene.upload(device, sc)
+    before = [g.layout(w) for w in (2, 4, 8)]
+    g.refit(sc.positions)
+    for w, b in zip((2, 4, 8), before):
+        assert np.array_equal(g.layout(w), b), w
+    rng = np.random.default_rng(21)
+    moved = (sc.positions + rng.normal(0, 0.01, sc.positions.shape)).astype(np.float32)
+    moved[:, 1] += np.float32(0.05)
+    g.refit(moved)
+    tree = _refit_tree(sc.nodes, moved, sc.indices)
+    want, _ = _host_layout(tree, 2)
+    assert np.array_equal(g.layout(2), want)
+    frame = render.ray_frame(sc.cameras[0], 128, 128)
+    mf, mt, _, _ = O.probe(tree, moved, sc.indices, sc.faces, frame.pack(), 128, 128)
+    assert (mf != 0xFFFFFFFF).sum() > 128 * 128 // 2
+    for w in (2, 4, 8):
+        g.set_trace_config(w, 64, 24, 0)
+        f, t = render.probe_arrays(device, g, frame)
+        assert np.array_equal(f, mf), w
+        assert np.array_equal(t.view(np.uint32), mt.view(np.uint32)), w
+    g.refit(sc.positions)
+    for w, b in zip((2, 4, 8), before):
+        assert np.array_equal(g.layout(w), b), w
+    with pytest.raises(render.LuceError) as e:
+        g.refit(sc.positions[:-1])
+    assert e.value.code == 400
+    g.destroy()
+
+
+def test_refit_of_a_device_built_scene(device):
+    """A scene built on the device refits like an uploaded one."""
+    from luce_render import render, scenes
+    gset = scenes.cornell_probe_set(random_count=2000)
+    ref = render.compile_scene(gset)
+    flat = render.compile_scene(gset, geometry=False)
+    g = render.GpuScene.build(device, flat, keep_tree=False)
+    moved = (ref.positions * np.float32(0.9)).astype(np.float32)
+    g.refit(moved)
+    tree = _refit_tree(ref.nodes, moved, ref.indices)
+    want, _ = _host_layout(tree, 2)
+    assert np.array_equal(g.layout(2), want)
+    g.destroy()
