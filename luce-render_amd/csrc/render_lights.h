@@ -16,6 +16,28 @@ constexpr uint32_t kEmitterTriangle = 0x80000000u;
 __device__ __forceinline__ float4 light_at(const Params &p, uint32_t light, uint32_t row) {
     return p.lights[light * LIGHT_STRIDE + row];
 }
+
+// Element i of a float4 array that no kernel writes while this one runs, at a
+// wave-uniform address (the analytic-light loops walk every light with a
+// loop counter; a workgroup serves one request): read through the constant
+// address space, so the wave issues one scalar load instead of a vector load
+// of 64 equal addresses. The first active lane's pointer and index are used.
+typedef float UniformFloat4 __attribute__((ext_vector_type(4)));
+typedef const UniformFloat4 __attribute__((address_space(4))) *ConstFloat4;
+__device__ __forceinline__ float4 uniform_at(const float4 *base, uint32_t i) {
+    // readfirstlane returns a signed int: each half goes through uint32_t
+    // before widening, or a low half with bit 31 set would sign-extend.
+    uint64_t a = reinterpret_cast<uint64_t>(base);
+    uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    ConstFloat4 q = (ConstFloat4)((hi << 32) | lo);
+    UniformFloat4 v = q[(uint32_t)__builtin_amdgcn_readfirstlane(i)];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+// light_at for a wave-uniform light index.
+__device__ __forceinline__ float4 uniform_light_at(const Params &p, uint32_t light, uint32_t row) {
+    return uniform_at(p.lights, light * LIGHT_STRIDE + row);
+}
 __device__ __forceinline__ uint32_t light_type(const Params &p, uint32_t light) {
     return (uint32_t)light_at(p, light, 0).w;
 }
@@ -170,16 +192,19 @@ __device__ __forceinline__ float light_cos(const Params &p, uint32_t light, V3 o
 }
 
 // lights.glsl:239-259: the nearer hit on the emitting side, or kInf.
+// UNIFORM: `light` is the same in every active lane (the loops below).
+template <bool UNIFORM = false>
 __device__ __forceinline__ float intersect_light(const Params &p, uint32_t light, V3 o, V3 d, float t_max,
                                                  bool light_flow) {
-    float4 r0 = light_at(p, light, 0);
+    auto at = [&](uint32_t row) { return UNIFORM ? uniform_light_at(p, light, row) : light_at(p, light, row); };
+    float4 r0 = at(0);
     uint32_t type = (uint32_t)r0.w;
     if (type == LIGHT_SPHERE) {
-        float t = sphere_hit(v3(r0), light_at(p, light, 1).w, o, d);
+        float t = sphere_hit(v3(r0), at(1).w, o, d);
         return t < t_max ? t : kInf;
     }
     if (type >= LIGHT_DISTANT) return kInf;
-    V3 u = v3(light_at(p, light, 1)), v = v3(light_at(p, light, 2)), n = v3(light_at(p, light, 3));
+    V3 u = v3(at(1)), v = v3(at(2)), n = v3(at(3));
     float facing = dot(d, n);
     if (light_flow ? facing <= 0.0f : facing >= 0.0f) return kInf;
     float t = dot(v3(r0) - o, n) / facing;
@@ -223,12 +248,12 @@ __device__ __forceinline__ void nearest_light(const Params &p, V3 o, V3 d, bool 
     float dd = dot(d, d);
     uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
     for (uint32_t g = 0; g < groups; g++) {
-        if (!sphere_may_hit(p.light_bounds[g], o, d, dd, t)) continue;
+        if (!sphere_may_hit(uniform_at(p.light_bounds, g), o, d, dd, t)) continue;
         uint32_t end = min(n, (g + 1u) * kLightGroup);
         for (uint32_t l = g * kLightGroup; l < end; l++) {
-            if (!sphere_may_hit(p.light_bounds[groups + l], o, d, dd, t)) continue;
-            if (camera_ray && !light_visible(p, l)) continue;
-            float tl = intersect_light(p, l, o, d, t, light_flow);
+            if (!sphere_may_hit(uniform_at(p.light_bounds, groups + l), o, d, dd, t)) continue;
+            if (camera_ray && !(((uint32_t)uniform_light_at(p, l, 5).w & 1u) != 0u)) continue;
+            float tl = intersect_light<true>(p, l, o, d, t, light_flow);
             if (tl < t) {
                 t = tl;
                 what = kLightHit | l;
@@ -244,11 +269,12 @@ __device__ __forceinline__ bool light_in_the_way(const Params &p, V3 o, V3 d, fl
     float dd = dot(d, d);
     uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
     for (uint32_t g = 0; g < groups; g++) {
-        if (!sphere_may_hit(p.light_bounds[g], o, d, dd, t_max)) continue;
+        if (!sphere_may_hit(uniform_at(p.light_bounds, g), o, d, dd, t_max)) continue;
         uint32_t end = min(n, (g + 1u) * kLightGroup);
         for (uint32_t l = g * kLightGroup; l < end; l++) {
-            if (!sphere_may_hit(p.light_bounds[groups + l], o, d, dd, t_max)) continue;
-            if (light_visible(p, l) && intersect_light(p, l, o, d, t_max, true) < t_max) return true;
+            if (!sphere_may_hit(uniform_at(p.light_bounds, groups + l), o, d, dd, t_max)) continue;
+            if ((((uint32_t)uniform_light_at(p, l, 5).w & 1u) != 0u) && intersect_light<true>(p, l, o, d, t_max, true) < t_max)
+                return true;
         }
     }
     return false;
