@@ -120,6 +120,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     bands = sh.bands;
     rounds = st.max_bounces + 1 + loaded->through;
     adaptive = loaded->constants[44] > 0.0f;  // the noise threshold (loading.lucb:267)
+    cutouts = loaded->constants[3] >= 0.5f;   // K_INFO.w
     int rc = 0;
     rt::Params &p = params;
     std::memcpy(p.frame, frame, 64);
@@ -244,8 +245,9 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     auto light_paths = [](RenderJob *j) -> uint64_t { return j->params.light_paths; };
     auto slots = [](RenderJob *j) -> uint64_t { return j->slots; };
     auto pixels = [](RenderJob *j) -> uint64_t { return j->params.pixels; };
-    bool adaptive = false;
+    bool adaptive = false, cutouts = false;
     for (RenderJob *j : jobs) adaptive = adaptive || j->adaptive;
+    for (RenderJob *j : jobs) cutouts = cutouts || j->cutouts;
     auto mark = [&](int kind) {
         if (!marks) return;
         hipEvent_t e;
@@ -277,7 +279,7 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         mark(KSurface);
         launch_render_stage(Stage::Miss, sp, params, n, 0, blocks(256, slots), s);
         mark(KMiss);
-        tv.shadow(params, n, blocks(tv.group, slots), s);
+        tv.shadow(params, n, blocks(tv.group, slots), cutouts, s);
         mark(KShadow);
     }
     launch_render_stage(Stage::Finish, sp, params, n, 0, blocks(256, band), s);

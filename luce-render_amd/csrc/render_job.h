@@ -32,6 +32,7 @@ struct RenderJob {
     std::unique_ptr<GuideNet> net;  // the learned guide (guiding 2)
     uint32_t train_steps = 0;
     bool adaptive = false;  // a noise threshold is set: passes check convergence
+    bool cutouts = false;   // some material has geometry opacity below one: shadow rays walk through
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);

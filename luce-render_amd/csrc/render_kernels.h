@@ -14,7 +14,8 @@ struct RenderTraceVariant {
     // `p`: a device array of `n` requests' parameters; `blocks` workgroups
     // share their items.
     void (*closest)(const rt::Params *p, uint32_t n, uint32_t current, uint32_t blocks, hipStream_t s);
-    void (*shadow)(const rt::Params *p, uint32_t n, uint32_t blocks, hipStream_t s);
+    // `cutouts`: some request has cut-out materials (the walk through them)
+    void (*shadow)(const rt::Params *p, uint32_t n, uint32_t blocks, bool cutouts, hipStream_t s);
     uint32_t group;
 };
 const RenderTraceVariant *render_trace_variants();

@@ -205,6 +205,20 @@ constexpr uint32_t kStageIterations = 8u;
 
 // --------------------------------------------------------------- camera
 
+// The pixel of the i-th camera path of a sample: the image is walked in 8x8
+// tiles (a row of tiles at a time; the last row and column of tiles may be
+// smaller), so a wave's 64 rays leave one tile together and meet the same
+// nodes. Each pixel still draws its own samples (sample4 keys on the pixel),
+// so images do not depend on the walk; the reference walks rows
+// (camera.comp:14-20).
+__device__ __forceinline__ uint32_t tiled_pixel(uint32_t i, uint32_t width, uint32_t height) {
+    uint32_t tile_row = i / (8u * width), local = i - tile_row * 8u * width;
+    uint32_t rows = min(8u, height - 8u * tile_row);
+    uint32_t tile = local / (8u * rows), within = local - tile * 8u * rows;
+    uint32_t cols = min(8u, width - 8u * tile);
+    return (8u * tile_row + within / cols) * width + 8u * tile + within % cols;
+}
+
 template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(const Params *plist, uint32_t n) {
     __shared__ RowPlan plan;
     uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->band_count; });
@@ -216,7 +230,7 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(co
         const PassState &ps = *p.pass;
         bool push = false;
         if (active) {
-            uint32_t pixel = ps.band_start + path;
+            uint32_t pixel = tiled_pixel(ps.band_start + path, p.width, p.height);
             if (p.film[(size_t)FILM_HALF * p.pixels + pixel].w > 0.5f) {
                 st.set_u(SU_PIXEL, path, kNoneU);
             } else {
@@ -392,7 +406,8 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kerne
 // the miss, surface and light queues, so this kernel needs no LDS beyond the
 // traversal stack and keeps its occupancy. One ray per thread: every
 // persistent schedule measured slower here, its refills stalling on the
-// queue-then-state loads.
+// queue-then-state loads. (Testing the lights in route_kernel instead
+// measured slower: the state is read twice.)
 template <int W, int S, int G>
 __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint32_t n, uint32_t current) {
     __shared__ uint32_t lds[S * G];
@@ -964,7 +979,10 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(cons
 // cut-outs present, a blocked ray walks through the surfaces it meets (at most
 // 16), keeping 1 - opacity of each; an unblocked ray delivers its radiance to
 // the camera path or, for a light path, splats it onto the film.
-template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist, uint32_t n) {
+// CUTOUTS false (no material of the scene has geometry opacity below one)
+// leaves the walk out, and with it a second traversal's registers.
+template <int W, int S, int G, bool CUTOUTS>
+__global__ __launch_bounds__(G) void shadow_kernel(const Params *plist, uint32_t n) {
     __shared__ uint32_t lds[S * G];
     __shared__ RowPlan plan;
     uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[Q_SHADOW]; });
@@ -981,7 +999,7 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
         // spill column: the item's index in its render's queue (a register copy of the scene)
         const DeviceScene sc = p.scene;
         uint32_t nlights = light_count(p);
-        bool cutouts = K(p, K_INFO).w >= 0.5f;
+        bool cutouts = CUTOUTS && K(p, K_INFO).w >= 0.5f;
         uint32_t path = queue_path(p, Q_SHADOW, at);
         float4 o = st.v(SV_SHADOW_ORIGIN, path);
         V3 d = v3(st.v(SV_SHADOW_DIRECTION, path));
@@ -992,7 +1010,7 @@ template <int W, int S, int G> __global__ __launch_bounds__(G) void shadow_kerne
         if (blocked && !cutouts) continue;
         float kept = 1.0f;
         bool deliver = !blocked;
-        if (blocked) {
+        if (CUTOUTS && blocked) {
             V3 from = v3(o);
             float remaining = o.w;
             for (uint32_t hop = 0; hop < 16u; hop++) {
@@ -1298,8 +1316,10 @@ template <int W, int S, int G>
 void launch_closest(const rt::Params *p, uint32_t n, uint32_t cur, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL((rt::closest_kernel_rt<W, S, G>), dim3(blocks), dim3(G), 0, s, p, n, cur);
 }
-template <int W, int S, int G> void launch_shadow(const rt::Params *p, uint32_t n, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((rt::shadow_kernel<W, S, G>), dim3(blocks), dim3(G), 0, s, p, n);
+template <int W, int S, int G> void launch_shadow(const rt::Params *p, uint32_t n, uint32_t blocks, bool cutouts,
+                                                  hipStream_t s) {
+    if (cutouts) hipLaunchKernelGGL((rt::shadow_kernel<W, S, G, true>), dim3(blocks), dim3(G), 0, s, p, n);
+    else hipLaunchKernelGGL((rt::shadow_kernel<W, S, G, false>), dim3(blocks), dim3(G), 0, s, p, n);
 }
 
 template <int W, int S, int G> void fill_trace(RenderTraceVariant &v) {

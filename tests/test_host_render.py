@@ -158,3 +158,21 @@ def test_packed_layouts():
     assert len(L.materials) == 16 * 4 * (1 + len(sc.materials))
     assert L.constants.size == 128 and L.constants[1] == 12
     assert L.light_paths == 64 * 64 // 2
+
+
+def _tiled_pixel(i, w, h):
+    """The camera kernel's walk of the image in 8x8 tiles (render_kernels.hip
+    tiled_pixel), restated."""
+    tile_row, local = divmod(i, 8 * w)
+    rows = min(8, h - 8 * tile_row)
+    tile, within = divmod(local, 8 * rows)
+    cols = min(8, w - 8 * tile)
+    return (8 * tile_row + within // cols) * w + 8 * tile + within % cols
+
+
+def test_tiled_camera_walk_visits_every_pixel_once():
+    for w, h in [(1, 1), (8, 8), (7, 5), (9, 17), (64, 3), (33, 41), (96, 96), (250, 13)]:
+        seen = sorted(_tiled_pixel(i, w, h) for i in range(w * h))
+        assert seen == list(range(w * h)), (w, h)
+    # a wave of 64 consecutive paths stays in one 8x8 tile where tiles are whole
+    assert {(_tiled_pixel(i, 64, 64) % 64) // 8 for i in range(64)} == {0}
