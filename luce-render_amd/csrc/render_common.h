@@ -43,8 +43,8 @@ constexpr int K_GUIDE = 12;       // x on, y cell width per unit distance, z gui
 constexpr int K_GUIDE_SIZE = 13;  // x smallest cell width
 constexpr int K_CAMERA = 14;      // x light paths a sample, y 1/pixel area at unit distance, z lens radius, w focus
 constexpr int K_VIEW = 15;        // camera forward axis
-constexpr int K_DOME = 16;
-constexpr int K_DOME_CDF = 17;
+constexpr int K_DOME = 16;        // x the light whose image this is (-1: none), y its texels' first float4, z width, w height
+constexpr int K_DOME_CDF = 17;    // x its cumulative weights' first float, y the weights' sum
 constexpr int K_BASIS = 18;       // 3 rows
 constexpr int K_EMIT = 21;        // x chance a light path starts afar, y disk radius
 constexpr int K_EMIT_CENTER = 22;

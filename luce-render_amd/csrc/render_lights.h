@@ -107,21 +107,87 @@ __device__ __forceinline__ float sphere_hit(V3 center, float radius, V3 o, V3 d)
     return t > 0.0f ? t : kInf;
 }
 
-// Dome lights: untextured ones shine evenly (the only kind this build packs;
-// a textured dome is outside the measured workloads).
-__device__ __forceinline__ float dome_pdf(const Params &, uint32_t, V3) { return 1.0f / (4.0f * kPi); }
+// Dome lights (lights.glsl:89-175, environment.lucb): a lat-long image around
+// the scene, its center down the light's -Z, its top +Y; a dome with no image
+// (or not the one image K_DOME holds) shines its color evenly. Texels are
+// constant, and sampling picks a row by the rows' cumulative weights, then a
+// texel of it by the row's, so the pdf follows luminance x sin theta.
+__device__ __forceinline__ bool dome_textured(const Params &p, uint32_t light) {
+    return (float)light == K(p, K_DOME).x;
+}
 
-template <bool SPECTRAL> __device__ __forceinline__ Spec dome_radiance(const Params &p, uint32_t light, V3, Spec lambda) {
-    return light_radiance<SPECTRAL>(p, light, lambda);
+// The texel a world direction falls in, and sin theta there.
+__device__ __forceinline__ uint32_t dome_texel(const Params &p, uint32_t light, V3 d, float &sin_theta) {
+    V3 l{dot(d, v3(light_at(p, light, 1))), dot(d, v3(light_at(p, light, 2))), dot(d, v3(light_at(p, light, 3)))};
+    uint32_t width = (uint32_t)K(p, K_DOME).z, height = (uint32_t)K(p, K_DOME).w;
+    float u = 0.5f + atan2f(l.x, -l.z) * (0.5f / kPi);
+    float v = acosf(clampf(l.y, -1.0f, 1.0f)) / kPi;
+    sin_theta = sqrtf(fmaxf(0.0f, 1.0f - l.y * l.y));
+    uint32_t x = min((uint32_t)(u * (float)width), width - 1u);
+    uint32_t y = min((uint32_t)(v * (float)height), height - 1u);
+    return y * width + x;
+}
+
+__device__ __forceinline__ float4 dome_texel_value(const Params &p, uint32_t texel) {
+    return p.lights[(uint32_t)K(p, K_DOME).y + texel];
+}
+
+// The solid-angle pdf of sampling direction d (the light's selection chance
+// not included): a texel's weight over the sum, spread over its solid angle.
+__device__ __forceinline__ float dome_pdf(const Params &p, uint32_t light, V3 d) {
+    if (!dome_textured(p, light)) return 1.0f / (4.0f * kPi);
+    float sin_theta;
+    uint32_t texel = dome_texel(p, light, d, sin_theta);
+    if (sin_theta <= 0.0f) return 0.0f;
+    float weight = dome_texel_value(p, texel).w;
+    return weight / K(p, K_DOME_CDF).y * K(p, K_DOME).z * K(p, K_DOME).w / (2.0f * kPi * kPi * sin_theta);
+}
+
+template <bool SPECTRAL> __device__ __forceinline__ Spec dome_radiance(const Params &p, uint32_t light, V3 d, Spec lambda) {
+    Spec radiance = light_radiance<SPECTRAL>(p, light, lambda);
+    if (!dome_textured(p, light)) return radiance;
+    float sin_theta;
+    uint32_t texel = dome_texel(p, light, d, sin_theta);
+    return radiance * spec_of_texel<SPECTRAL>(p, v3(dome_texel_value(p, texel)), lambda);
+}
+
+__device__ __forceinline__ float dome_cdf(const Params &p, uint32_t at) {
+    return reinterpret_cast<const float *>(p.lights)[(uint32_t)K(p, K_DOME_CDF).x + at];
+}
+
+// The first of `count` cumulative values from `first` that passes u, and u's
+// place within it (0..1).
+__device__ __forceinline__ uint32_t dome_search(const Params &p, uint32_t first, uint32_t count, float u, float &within) {
+    uint32_t low = 0, high = count - 1u;
+    while (low < high) {
+        uint32_t middle = (low + high) / 2u;
+        if (dome_cdf(p, first + middle) > u) high = middle;
+        else low = middle + 1u;
+    }
+    float below = low > 0u ? dome_cdf(p, first + low - 1u) : 0.0f;
+    float above = dome_cdf(p, first + low);
+    within = above > below ? clampf((u - below) / (above - below), 0.0f, 0.99999994f) : 0.5f;
+    return low;
 }
 
 __device__ __forceinline__ bool sample_dome(const Params &p, uint32_t light, float ux, float uy, LightSample &s) {
     s.distance = kInf;
     s.delta = false;
     s.cos_light = 1.0f;
-    float z = 1.0f - 2.0f * ux;
-    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    V3 l{r * cosf(2.0f * kPi * uy), z, r * sinf(2.0f * kPi * uy)};
+    V3 l;
+    if (!dome_textured(p, light)) {
+        float z = 1.0f - 2.0f * ux;
+        float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        l = V3{r * cosf(2.0f * kPi * uy), z, r * sinf(2.0f * kPi * uy)};
+    } else {
+        uint32_t width = (uint32_t)K(p, K_DOME).z, height = (uint32_t)K(p, K_DOME).w;
+        float fy, fx;
+        uint32_t y = dome_search(p, width * height, height, uy, fy);
+        uint32_t x = dome_search(p, y * width, width, ux, fx);
+        float theta = kPi * ((float)y + fy) / (float)height;
+        float phi = 2.0f * kPi * (((float)x + fx) / (float)width - 0.5f);
+        l = V3{sinf(theta) * sinf(phi), cosf(theta), -sinf(theta) * cosf(phi)};
+    }
     s.direction = normalize(v3(light_at(p, light, 1)) * l.x + v3(light_at(p, light, 2)) * l.y +
                             v3(light_at(p, light, 3)) * l.z);
     s.pdf = dome_pdf(p, light, s.direction);

@@ -8,6 +8,7 @@ from dataclasses import dataclass, field, replace
 import numpy as np
 
 from . import spectrum as S
+from ._lib import INVALID, LuceError
 
 LIGHT_STRIDE = 7
 MATERIAL_STRIDE = 16
@@ -84,11 +85,65 @@ def pack_material(m, program=-1.0):
     return out
 
 
-def pack_lights(lights, radius):
+# Linear Rec.709 (sRGB primaries, D65) to ACEScg, Bradford-adapted
+# (environment.lucb:21-25).
+REC709_TO_ACESCG = np.array([[0.613097, 0.339523, 0.047379],
+                             [0.070194, 0.916354, 0.013452],
+                             [0.020616, 0.109570, 0.869815]])
+
+
+@dataclass
+class Environment:
+    """A dome's image ready for importance sampling (environment.lucb:53-79):
+    texels (ACEScg radiance and weight = luminance x sin theta, four floats a
+    texel, rows from the top), each row's cumulative weights (ending at 1)
+    then the rows', the weights' sum and the mean luminance over the sphere."""
+    width: int
+    height: int
+    texels: np.ndarray
+    cdf: np.ndarray
+    total: float
+    average: float
+
+
+def environment_of(pixels):
+    """environment_of (environment.lucb:111-156) from linear Rec.709 pixels
+    (height x width x 3, or height x width for grey)."""
+    px = np.asarray(pixels, np.float64)
+    if px.ndim == 2:
+        px = np.repeat(px[:, :, None], 3, axis=2)
+    if px.ndim != 3 or px.shape[2] < 3 or px.shape[0] < 1 or px.shape[1] < 1:
+        raise LuceError(INVALID, "an environment image needs R, G and B channels, or Y")
+    height, width = px.shape[0], px.shape[1]
+    rgb = np.maximum(0.0, px[:, :, :3] @ REC709_TO_ACESCG.T)
+    lum = S.luminance(np.moveaxis(rgb, 2, 0))
+    sin_theta = np.sin(np.pi * (np.arange(height) + 0.5) / height)
+    weight = lum * sin_theta[:, None]
+    texels = np.concatenate([rgb, weight[:, :, None]], axis=2).astype(np.float32).reshape(-1)
+    rows = np.zeros(height)
+    cdf = np.zeros(width * height + height, np.float32)
+    for y in range(height):
+        running = np.cumsum(weight[y])  # the reference adds in f64, then stores f32
+        row = running[-1]
+        rows[y] = row
+        part = (running.astype(np.float32).astype(np.float64) / row if row > 0.0
+                else (np.arange(width) + 1.0) / width)
+        cdf[y * width:(y + 1) * width] = part
+    total = float(rows.sum())
+    running = np.cumsum(rows)
+    cdf[width * height:] = running / total if total > 0.0 else (np.arange(height) + 1.0) / height
+    solid_angle = float(sin_theta.sum()) * width
+    average = float(weight.sum()) / solid_angle if solid_angle > 0.0 else 0.0
+    return Environment(width, height, texels, cdf, total, average)
+
+
+def pack_lights(lights, radius, environment=None):
     """pack_lights (lights.lucb:29-131): records, the distant lights' total
-    power."""
+    power; and the index of the dome that `environment` belongs to (the first
+    dome with a texture, -1 if none)."""
     out = []
     powers = []
+    textured = -1
     to_acescg = S.film_matrix()
     for light in lights:
         m = np.asarray(light.matrix, np.float64)
@@ -143,7 +198,11 @@ def pack_lights(lights, radius):
             power = max(0.0, S.luminance(rgb)) * (2.0 * math.pi * (1.0 - cos_half) if cos_half < 1.0 - 1e-7 else 1.0) \
                 * math.pi * radius * radius
         if kind == 4.0:
-            power = max(0.0, S.luminance(rgb)) * 1.0 * math.pi * math.pi * radius * radius
+            mean = 1.0
+            if environment is not None and textured < 0 and getattr(light, "texture", None) is not None:
+                textured = len(powers)
+                mean = environment.average
+            power = max(0.0, S.luminance(rgb)) * mean * math.pi * math.pi * radius * radius
         powers.append(power)
         out += [*position, kind, *u, size, *v, cos_half, *emits, area, *fit.row(), *rgb, flags, temperature, 0.0, 0.0, 0.0]
     total = sum(p for p in powers if p >= 0.0)
@@ -160,7 +219,7 @@ def pack_lights(lights, radius):
         running += share
         out[row + 1] = share
         out[row + 2] = running
-    return out, len(powers), total
+    return out, len(powers), total, textured
 
 
 def _cone_union(a_axis, a_theta, b_axis, b_theta):
@@ -379,11 +438,31 @@ class Loaded:
 
     def __init__(self, scene, settings, frame):
         radius = scene_radius(scene)
-        lights, light_count, distant_power = pack_lights(scene.lights, radius)
+        # The first dome with an image lights by it (loading.lucb:55-61).
+        environment = None
+        for light in scene.lights:
+            if light.kind == "dome" and getattr(light, "texture", None) is not None:
+                environment = environment_of(light.texture)
+                break
+        lights, light_count, distant_power, dome = pack_lights(scene.lights, radius, environment)
         library = [settings.material] + [OpenPbr.of(r) for r in scene.materials]
         nodes, links, records, emitter_count, tree_energy = build_light_tree(scene, lights, light_count, library)
         node_offset = len(lights) // 4
         packed = list(lights) + list(nodes)
+        # The dome's texels (a float4 each), then its cumulative weights
+        # (loading.lucb:74-84).
+        dome_row = [-1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+        if dome >= 0:
+            while len(packed) % 4:
+                packed.append(0.0)
+            texel_base = len(packed) // 4
+            packed += environment.texels.tolist()
+            cdf_base = len(packed)
+            packed += environment.cdf.tolist()
+            while len(packed) % 4:
+                packed.append(0.0)
+            dome_row = [float(dome), float(texel_base), float(environment.width), float(environment.height),
+                        float(cdf_base), environment.total, 0.0, 0.0]
         emitters = np.concatenate([links, records]).astype(np.uint32)
         total = distant_power + tree_energy
         if emitter_count > 0:
@@ -406,7 +485,7 @@ class Loaded:
         basis = S.basis_weights()
         for r in range(3):
             rows[72 + r * 4:72 + r * 4 + 3] = basis[r]
-        rows[64] = -1.0  # no textured dome
+        rows[64:72] = dome_row
         used = set(int(m) for m in np.unique(scene.shading[:, 3])) if scene.triangle_count() else set()
         materials = []
         cutouts = False

@@ -61,7 +61,10 @@ class Camera:
 @dataclass
 class Light:
     """A UsdLux light (lights.lucb:29-131): kind rect/disk/sphere/distant/dome,
-    placed by `matrix` (rows: axes, translation)."""
+    placed by `matrix` (rows: axes, translation). A dome may carry `texture`:
+    a lat-long image (height x width x 3, or height x width, linear Rec.709,
+    rows from the top) wrapped around the scene (environment.lucb); the
+    reference reads it from OpenEXR, this harness takes the pixels."""
     kind: str
     matrix: np.ndarray = field(default_factory=lambda: np.eye(4))
     color: tuple = (1.0, 1.0, 1.0)
@@ -74,6 +77,7 @@ class Light:
     angle: float = 0.53
     visible: bool = True
     temperature: float = 0.0
+    texture: Optional[np.ndarray] = None
 
 
 @dataclass

@@ -352,6 +352,44 @@ def furnace_set(albedo=0.8, frequency=16, roughness=0.0):
     return gset
 
 
+def sky_texture(width=64, height=32, seed=11):
+    """A small synthetic lat-long environment (linear Rec.709, rows from the
+    top): a blue-to-white sky gradient over a dark ground, a warm sun patch and
+    seeded cloud noise, so radiance varies strongly across texels."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    sky = np.where(v[:, None] < 0.5, 1.0, 0.05)
+    img = np.zeros((height, width, 3))
+    img[:, :, 0] = sky * (0.4 + 0.6 * v[:, None])
+    img[:, :, 1] = sky * (0.6 + 0.4 * v[:, None])
+    img[:, :, 2] = sky * 1.0
+    img *= 0.5 + rng.random((height, width, 1))
+    sun_v, sun_u = int(height * 0.25), int(width * 0.6)
+    img[sun_v - 1:sun_v + 1, sun_u - 1:sun_u + 1] = (40.0, 30.0, 20.0)
+    return img
+
+
+def dome_set(texture=None, albedo=0.8, frequency=16, intensity=1.0, ground=True, ball=None):
+    """A textured dome (environment.lucb) over an albedo-`albedo` diffuse
+    sphere (radius 1 at the origin; `ball` an OpenPBR dict to use instead)
+    and, with `ground`, a diffuse floor at y = -1; seen from (0, 0, 4) so the
+    environment fills the background."""
+    pts, tris = icosphere(frequency)
+    meshes = [Mesh(pts, tris, np.arange(len(tris)), len(tris), face_materials=["ball"] * len(tris))]
+    if ground:
+        floor = np.array([[-6.0, -1.0, -6.0], [6.0, -1.0, -6.0], [6.0, -1.0, 6.0], [-6.0, -1.0, 6.0]])
+        meshes.append(Mesh(floor, np.array([[0, 2, 1], [0, 3, 2]]), np.array([0, 0]), 1,
+                           face_materials=["floor"]))
+    gset = GeometrySet(meshes=meshes)
+    gset.materials = {"ball": lambert((albedo, albedo, albedo)) if ball is None else ball,
+                      "floor": lambert((0.5, 0.5, 0.5))}
+    tex = sky_texture() if texture is None else texture
+    gset.lights.append(Light("dome", placement(), (1.0, 1.0, 1.0), intensity, texture=tex))
+    gset.cameras.append(Camera(placement(translation=(0.0, 0.0, 4.0)), "perspective", 50.0, 36.0))
+    return gset
+
+
 def many_lights_set(count=40, seed=5):
     """The Cornell render box lit by `count` small analytic lights instead of
     its rect light: rect, disk and sphere lights in turn (0.12 wide, radius

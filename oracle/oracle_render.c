@@ -262,6 +262,69 @@ static spec light_radiance(const oscene *s, uint32_t l, spec lambda) {
     return r;
 }
 
+/* Dome lights (lights.glsl:89-175, environment.lucb): the one textured dome
+ * (constants row 16: light, texels' first float4, width, height; row 17: the
+ * cumulative weights' first float, the weights' sum) looks its radiance up in
+ * a lat-long image and samples it by luminance x sin theta; others are even. */
+static int dome_textured(const oscene *s, uint32_t l) { return (double)l == K(s, 16, 0); }
+static uint32_t dome_texel(const oscene *s, uint32_t l, v3 d, double *sin_theta) {
+    v3 lv = V(dot(d, Lv(s, l, 1)), dot(d, Lv(s, l, 2)), dot(d, Lv(s, l, 3)));
+    uint32_t w = (uint32_t)K(s, 16, 2), h = (uint32_t)K(s, 16, 3);
+    double u = 0.5 + atan2(lv.x, -lv.z) * (0.5 / PI);
+    double v = acos(fmin(1.0, fmax(-1.0, lv.y))) / PI;
+    *sin_theta = sqrt(fmax(0.0, 1.0 - lv.y * lv.y));
+    uint32_t x = (uint32_t)(u * w), y = (uint32_t)(v * h);
+    if (x > w - 1) x = w - 1;
+    if (y > h - 1) y = h - 1;
+    return y * w + x;
+}
+static const float *dome_texel_value(const oscene *s, uint32_t texel) {
+    return &s->lights[((uint32_t)K(s, 16, 1) + texel) * 4];
+}
+static double dome_pdf(const oscene *s, uint32_t l, v3 d) {
+    if (!dome_textured(s, l)) return 1.0 / (4.0 * PI);
+    double st;
+    uint32_t texel = dome_texel(s, l, d, &st);
+    if (st <= 0.0) return 0.0;
+    return dome_texel_value(s, texel)[3] / K(s, 17, 1) * K(s, 16, 2) * K(s, 16, 3) / (2.0 * PI * PI * st);
+}
+static double logistic(double x) { return 1.0 / (1.0 + exp(-x)); }
+/* spectrum.glsl:67-79: a texel's ACEScg as the three basis spectra's sum. */
+static spec spec_of_texel(const oscene *s, const float *rgb, spec lambda) {
+    spec r = S1(0.0);
+    if (s->spectral) {
+        double w[3];
+        for (int k = 0; k < 3; k++) w[k] = K(s, 18 + k, 0) * rgb[0] + K(s, 18 + k, 1) * rgb[1] + K(s, 18 + k, 2) * rgb[2];
+        for (int i = 0; i < 4; i++) {
+            double blue = 1.0 - logistic((lambda.v[i] - 490.0) / 12.0), red = logistic((lambda.v[i] - 595.0) / 12.0);
+            r.v[i] = w[0] * blue + w[1] * (1.0 - blue - red) + w[2] * red;
+        }
+    } else {
+        r.v[0] = rgb[0]; r.v[1] = rgb[1]; r.v[2] = rgb[2];
+    }
+    return r;
+}
+static spec dome_radiance(const oscene *s, uint32_t l, v3 d, spec lambda) {
+    spec r = light_radiance(s, l, lambda);
+    if (!dome_textured(s, l)) return r;
+    double st;
+    spec t = spec_of_texel(s, dome_texel_value(s, dome_texel(s, l, d, &st)), lambda);
+    for (int i = 0; i < 4; i++) r.v[i] *= t.v[i];
+    return r;
+}
+static double dome_cdf(const oscene *s, uint32_t at) { return s->lights[(uint32_t)K(s, 17, 0) + at]; }
+static uint32_t dome_search(const oscene *s, uint32_t first, uint32_t count, double u, double *within) {
+    uint32_t low = 0, high = count - 1;
+    while (low < high) {
+        uint32_t middle = (low + high) / 2;
+        if (dome_cdf(s, first + middle) > u) high = middle;
+        else low = middle + 1;
+    }
+    double below = low > 0 ? dome_cdf(s, first + low - 1) : 0.0, above = dome_cdf(s, first + low);
+    *within = above > below ? fmin(fmax((u - below) / (above - below), 0.0), 0.99999994) : 0.5;
+    return low;
+}
+
 static uint32_t pick_distant(const oscene *s, double u, double *pdf) {
     uint32_t n = light_count(s), chosen = NONE;
     for (uint32_t l = 0; l < n; l++) {
@@ -311,12 +374,22 @@ static int sample_light(const oscene *s, uint32_t l, v3 x, double ux, double uy,
     o->delta = 0;
     o->cos_light = 1.0;
     if (type == 4) {
-        double z = 1 - 2 * ux, r = sqrt(fmax(0, 1 - z * z));
-        v3 lv = V(r * cos(2 * PI * uy), z, r * sin(2 * PI * uy));
+        v3 lv;
+        if (!dome_textured(s, l)) {
+            double z = 1 - 2 * ux, r = sqrt(fmax(0, 1 - z * z));
+            lv = V(r * cos(2 * PI * uy), z, r * sin(2 * PI * uy));
+        } else {
+            uint32_t w = (uint32_t)K(s, 16, 2), h = (uint32_t)K(s, 16, 3);
+            double fy, fx;
+            uint32_t y = dome_search(s, w * h, h, uy, &fy);
+            uint32_t x = dome_search(s, y * w, w, ux, &fx);
+            double theta = PI * (y + fy) / h, phi = 2 * PI * ((x + fx) / w - 0.5);
+            lv = V(sin(theta) * sin(phi), cos(theta), -sin(theta) * cos(phi));
+        }
         o->direction = norm(add(add(mul(Lv(s, l, 1), lv.x), mul(Lv(s, l, 2), lv.y)), mul(Lv(s, l, 3), lv.z)));
         o->distance = INF;
-        o->pdf = 1.0 / (4.0 * PI);
-        return 1;
+        o->pdf = dome_pdf(s, l, o->direction);
+        return o->pdf > 0.0;
     }
     if (type == 0 || type == 1) {
         double ax, ay;
@@ -376,7 +449,7 @@ static double light_hit_pdf(const oscene *s, uint32_t l, v3 o, v3 d, double t) {
         return 1.0 / (2 * PI * fmax(1e-12, 1 - cm));
     }
     if (type == 3) return 1.0 / (2 * PI * fmax(1e-12, 1 - L(s, l, 2, 3)));
-    if (type == 4) return 1.0 / (4 * PI);
+    if (type == 4) return dome_pdf(s, l, d);
     double cl = fmax(1e-12, -dot(d, Lv(s, l, 3)));
     return t * t / (cl * L(s, l, 3, 3));
 }
@@ -829,7 +902,7 @@ static spec miss_radiance(const oscene *s, uint32_t bounce, v3 rd, double ray_pd
         double w = 1.0;
         if (bounce == 0) { if (!light_visible(s, l)) continue; }
         else if (ray_pdf > 0) w = mis_weight(light_hit_pdf(s, l, V(0, 0, 0), rd, INF) * light_select_pdf(s, l) / ray_pdf, 0.0);
-        rad = sadd(rad, sscale(light_radiance(s, l, lambda), w));
+        rad = sadd(rad, sscale(type == 4 ? dome_radiance(s, l, rd, lambda) : light_radiance(s, l, lambda), w));
     }
     return rad;
 }
@@ -958,7 +1031,8 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
             uint32_t light = pick_distant(s, ul[0] / p_distant, &share);
             if (light != NONE && share > 0 && sample_light(s, light, x, ul[1], ul[2], &ls)) {
                 chance = p_distant * share;
-                emitted = light_radiance(s, light, lambda);
+                emitted = light_type(s, light) == 4 ? dome_radiance(s, light, ls.direction, lambda)
+                                                    : light_radiance(s, light, lambda);
                 sampled = 1;
             }
         } else {
