@@ -189,6 +189,16 @@ typedef struct lr_loaded {
     size_t emitter_words;
     uint32_t light_paths;
     uint32_t through;
+    /* Materials' node-graph programs (shader_graph.lucb: u32 words, each
+     * material's entry in its record's row 13 x, -1 none) and the textures
+     * their image nodes read (RGBA f32 texels, rows from the top; per texture
+     * slot the first texel, width, height and a zero). May be empty. */
+    const uint32_t *programs;
+    size_t program_words;
+    const float *textures;
+    size_t texture_floats;
+    const uint32_t *texture_table;
+    size_t texture_count;
 } lr_loaded;
 
 /* RenderSettings (src/render/integrator.lucb:42-85) as far as the device

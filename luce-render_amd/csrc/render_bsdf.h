@@ -172,13 +172,42 @@ __device__ __forceinline__ Surface surface_at(V3 normal, V3 geometric, uint32_t 
     return s;
 }
 
-// openpbr.glsl:264-319: the material's parameters for the side seen.
+// The OpenPBR parameters a material's node graph drives at one hit
+// (openpbr.glsl:52-60): `mask` bit b set means the register group of
+// shader_graph.lucb's surface table, values[] holding them. `viewed`: the
+// program read the view direction (Fresnel or Layer Weight).
+struct Program {
+    uint32_t mask;
+    bool viewed;
+    float values[23];
+    __device__ __forceinline__ bool driven(uint32_t bit) const { return (mask >> bit) & 1u; }
+    __device__ __forceinline__ V3 driven3(uint32_t at) const { return V3{values[at], values[at + 1], values[at + 2]}; }
+};
+
+// openpbr.glsl:264-319: the material's parameters for the side seen; what
+// a program drives (g, may be null) replaces the record's.
 template <bool SPECTRAL>
-__device__ __forceinline__ void read_material(const Params &p, Surface &s, float hero_wavelength) {
+__device__ __forceinline__ void read_material(const Params &p, Surface &s, float hero_wavelength,
+                                              const Program *g = nullptr) {
     uint32_t m = s.material;
     float4 base_rgb = material_at(p, m, 1), specular_rgb = material_at(p, m, 3), coat_rgb = material_at(p, m, 5),
            transmission_rgb = material_at(p, m, 9), scalars = material_at(p, m, 10), indices = material_at(p, m, 11),
            geometry = material_at(p, m, 12);
+    if (g && g->mask) {
+        if (g->driven(0)) base_rgb.x = g->values[0], base_rgb.y = g->values[1], base_rgb.z = g->values[2];
+        if (g->driven(1)) base_rgb.w = g->values[3];
+        if (g->driven(2)) scalars.x = g->values[4];
+        if (g->driven(3)) scalars.y = g->values[5];
+        if (g->driven(4)) specular_rgb.w = g->values[6];
+        if (g->driven(5)) scalars.z = g->values[7];
+        if (g->driven(6)) coat_rgb.w = g->values[8];
+        if (g->driven(7)) indices.y = g->values[9];
+        if (g->driven(8)) transmission_rgb.w = g->values[10];
+        if (g->driven(10)) specular_rgb.x = g->values[14], specular_rgb.y = g->values[15], specular_rgb.z = g->values[16];
+        if (g->driven(11))
+            transmission_rgb.x = g->values[17], transmission_rgb.y = g->values[18], transmission_rgb.z = g->values[19];
+        if (g->driven(12)) coat_rgb.x = g->values[20], coat_rgb.y = g->values[21], coat_rgb.z = g->values[22];
+    }
     s.diffuse_roughness = scalars.y;
     s.metalness = clampf(scalars.x, 0.0f, 1.0f);
     s.specular_weight = fmaxf(specular_rgb.w, 0.0f);
@@ -267,8 +296,30 @@ __device__ __forceinline__ void update_substrate(Surface &s) {
     }
 }
 
-template <bool SPECTRAL> __device__ __forceinline__ void read_colors(const Params &p, Surface &s, Spec lambda) {
+// A color a program drives: a reflectance through the environment basis, at
+// most one (openpbr.glsl:391-401).
+template <bool SPECTRAL> __device__ __forceinline__ Spec driven_color(const Params &p, V3 rgb, Spec lambda) {
+    Spec c = spec_of_texel<SPECTRAL>(p, rgb, lambda);
+    return make_float4(clampf(c.x, 0.0f, 1.0f), clampf(c.y, 0.0f, 1.0f), clampf(c.z, 0.0f, 1.0f), clampf(c.w, 0.0f, 1.0f));
+}
+
+template <bool SPECTRAL>
+__device__ __forceinline__ void read_colors(const Params &p, Surface &s, Spec lambda, const Program *g = nullptr) {
     uint32_t m = s.material;
+    if (g && g->mask) {
+        float weight = g->driven(1) ? g->values[3] : material_at(p, m, 1).w;
+        s.base = (g->driven(0) ? driven_color<SPECTRAL>(p, g->driven3(0), lambda) : material_color<SPECTRAL>(p, m, 0, lambda)) *
+                 weight;
+        s.metal_tint =
+            g->driven(10) ? driven_color<SPECTRAL>(p, g->driven3(14), lambda) : material_color<SPECTRAL>(p, m, 2, lambda);
+        s.specular_tint = s.metal_tint * s.specular_weight;
+        s.transmission_tint =
+            g->driven(11) ? driven_color<SPECTRAL>(p, g->driven3(17), lambda) : material_color<SPECTRAL>(p, m, 8, lambda);
+        s.coat_tint = s.coat > 0.0f ? (g->driven(12) ? driven_color<SPECTRAL>(p, g->driven3(20), lambda)
+                                                     : material_color<SPECTRAL>(p, m, 4, lambda))
+                                    : spec(1.0f);
+        return;
+    }
     float weight = material_at(p, m, 1).w;
     s.base = material_color<SPECTRAL>(p, m, 0, lambda) * weight;
     s.metal_tint = material_color<SPECTRAL>(p, m, 2, lambda);
@@ -278,10 +329,11 @@ template <bool SPECTRAL> __device__ __forceinline__ void read_colors(const Param
 }
 
 template <bool SPECTRAL>
-__device__ __forceinline__ void prepare_surface(const Params &p, Surface &s, V3 wo, Spec lambda, float hero) {
-    read_material<SPECTRAL>(p, s, hero);
+__device__ __forceinline__ void prepare_surface(const Params &p, Surface &s, V3 wo, Spec lambda, float hero,
+                                                const Program *g = nullptr) {
+    read_material<SPECTRAL>(p, s, hero, g);
     orient_lobes(p, s, wo);
-    read_colors<SPECTRAL>(p, s, lambda);
+    read_colors<SPECTRAL>(p, s, lambda, g);
     update_substrate(s);
 }
 

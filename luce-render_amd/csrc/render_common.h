@@ -100,6 +100,14 @@ struct Params {
     const float *tables;
     const uint4 *shading;
     const float *uvs;
+    // Materials' node-graph programs and the textures their image nodes read
+    // (texture_table rows: first texel, width, height, 0; textures: RGBA
+    // texels, rows from the top). Empty when no material has a program.
+    const uint32_t *programs;
+    const float4 *textures;
+    const uint4 *texture_table;
+    uint32_t program_words;
+    uint32_t texture_count;
     float4 *state_v;
     uint32_t *state_u;
     uint32_t *queues;

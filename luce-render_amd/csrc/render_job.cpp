@@ -72,6 +72,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     Shape s = shape_of(loaded, *st);
     size_t b = 0;
     b += loaded->light_floats * 4 + 512 + loaded->material_floats * 4 + loaded->emitter_words * 4;
+    b += loaded->program_words * 4 + loaded->texture_floats * 4 + loaded->texture_count * 16;
     uint32_t nlights = (uint32_t)loaded->constants[0];
     b += ((size_t)(nlights + 7u) / 8u + nlights) * 16;
     b += (size_t)rt::kStateV * s.slots * 16 + (size_t)rt::kStateU * s.slots * 4;
@@ -104,6 +105,30 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     if (!loaded->constants || !loaded->lights || !loaded->materials || !loaded->emitters || loaded->material_floats < 64)
         return fail(LR_INVALID, "a render needs the loaded lights, constants, materials (the default one at least) and emitters");
     if (!sc->shading) return fail(LR_INVALID, "rendering needs the scene's shading attributes");
+    // Node-graph programs: every entry a material names lies in the words,
+    // every texture's texels in the texel array (the kernels also stop at
+    // the words' end).
+    bool with_programs = false;
+    for (size_t m = 0; (m + 1) * 64 <= loaded->material_floats; m++) {
+        float entry = loaded->materials[m * 64 + 52];  // row 13 x
+        if (!(entry >= 0.0f)) continue;
+        if (!loaded->programs || (double)entry >= (double)loaded->program_words)
+            return fail(LR_INVALID, "a material's node-graph program lies outside the program words");
+        with_programs = true;
+    }
+    if (with_programs) {
+        if (!sc->uvs) return fail(LR_INVALID, "node-graph programs need the scene's texture coordinates");
+        if (loaded->program_words > 0xFFFFFFFFull || loaded->texture_count > 0xFFFFFFFFull)
+            return fail(LR_INVALID, "too many program words or textures");
+        if (loaded->texture_count && (!loaded->texture_table || !loaded->textures))
+            return fail(LR_INVALID, "textures need their table and texels");
+        size_t texels = loaded->texture_floats / 4;
+        for (size_t t = 0; t < loaded->texture_count; t++) {
+            const uint32_t *row = loaded->texture_table + t * 4;
+            if ((uint64_t)row[0] + (uint64_t)row[1] * row[2] > texels)
+                return fail(LR_INVALID, "a texture's texels lie outside the texel array");
+        }
+    }
     device = dev;
     scene = sc;
     settings = st;
@@ -167,6 +192,17 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     p.tables = tables;
     p.shading = (const uint4 *)sc->shading;
     p.uvs = (const float *)sc->uvs;
+    programs = with_programs;
+    if (with_programs) {
+        p.programs = (const uint32_t *)upload(loaded->programs, loaded->program_words * 4);
+        p.program_words = (uint32_t)loaded->program_words;
+        p.texture_count = (uint32_t)loaded->texture_count;
+        if (loaded->texture_count) {
+            p.textures = (const float4 *)upload(loaded->textures, loaded->texture_floats / 4 * 16);
+            p.texture_table = (const uint4 *)upload(loaded->texture_table, loaded->texture_count * 16);
+        }
+        if (rc) return rc;
+    }
     p.state_v = (float4 *)alloc((size_t)rt::kStateV * slots * 16, rc);
     p.state_u = (uint32_t *)alloc((size_t)rt::kStateU * slots * 4, rc);
     size_t queue_words = rt::kQueueBase + (size_t)rt::kQueues * slots;
@@ -245,9 +281,10 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     auto light_paths = [](RenderJob *j) -> uint64_t { return j->params.light_paths; };
     auto slots = [](RenderJob *j) -> uint64_t { return j->slots; };
     auto pixels = [](RenderJob *j) -> uint64_t { return j->params.pixels; };
-    bool adaptive = false, cutouts = false;
+    bool adaptive = false, cutouts = false, programs = false;
     for (RenderJob *j : jobs) adaptive = adaptive || j->adaptive;
     for (RenderJob *j : jobs) cutouts = cutouts || j->cutouts;
+    for (RenderJob *j : jobs) programs = programs || j->programs;
     auto mark = [&](int kind) {
         if (!marks) return;
         hipEvent_t e;
@@ -274,8 +311,10 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
             launch_learned_infer(params, jobs[0]->params.band_size, s);
             mark(KRest);
         }
-        launch_render_stage(Stage::Shade, sp, params, n, cur, blocks(128, band), s);
-        if (lights) launch_render_stage(Stage::ShadeLight, sp, params, n, cur, blocks(128, light_paths), s);
+        launch_render_stage(programs ? Stage::ShadeProgram : Stage::Shade, sp, params, n, cur, blocks(128, band), s);
+        if (lights)
+            launch_render_stage(programs ? Stage::ShadeLightProgram : Stage::ShadeLight, sp, params, n, cur,
+                                blocks(128, light_paths), s);
         mark(KSurface);
         launch_render_stage(Stage::Miss, sp, params, n, 0, blocks(256, slots), s);
         mark(KMiss);

@@ -33,6 +33,7 @@ struct RenderJob {
     uint32_t train_steps = 0;
     bool adaptive = false;  // a noise threshold is set: passes check convergence
     bool cutouts = false;   // some material has geometry opacity below one: shadow rays walk through
+    bool programs = false;  // some material has a node-graph program: shading runs it
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);

@@ -7,7 +7,24 @@
 
 namespace lr {
 
-enum class Stage { BeginPass, Camera, LightEmit, Route, Shade, ShadeLight, Miss, Finish, GuideFit, Adaptive, Convert, EndPass };
+// ShadeProgram / ShadeLightProgram: the shading stages that run materials'
+// node-graph programs.
+enum class Stage {
+    BeginPass,
+    Camera,
+    LightEmit,
+    Route,
+    Shade,
+    ShadeLight,
+    ShadeProgram,
+    ShadeLightProgram,
+    Miss,
+    Finish,
+    GuideFit,
+    Adaptive,
+    Convert,
+    EndPass
+};
 
 // The tracing kernels of a pass for one (width, stack, group) variant.
 struct RenderTraceVariant {

@@ -14,6 +14,7 @@
 
 #include "render_bsdf.h"
 #include "render_guide.h"
+#include "render_program.h"
 
 namespace lr {
 namespace rt {
@@ -61,12 +62,13 @@ __device__ __forceinline__ Surface turned(const Surface &s) {
 }
 
 template <bool SPECTRAL>
-__device__ __forceinline__ float light_sampler_pdf(const Params &p, const Surface &s, V3 from, V3 to, float hero) {
+__device__ __forceinline__ float light_sampler_pdf(const Params &p, const Surface &s, V3 from, V3 to, float hero,
+                                                   const Program *g = nullptr) {
     bool same_side = dot(from, s.geometric) >= 0.0f;
     Surface v = same_side ? s : turned(s);
     V3 wo = to_local(v, from), wi = to_local(v, to);
     if ((dot(to, v.geometric) > 0.0f) != (wi.z > 0.0f)) return 0.0f;
-    if (!same_side) read_material<SPECTRAL>(p, v, hero);
+    if (!same_side) read_material<SPECTRAL>(p, v, hero, g);
     light_lobes(v);
     return surface_pdf(v, wi, wo);
 }
@@ -489,8 +491,10 @@ template <bool SPECTRAL> __device__ void dispersive_hit(Spec &throughput, Spec &
     lambda = make_float4(lambda.x, -lambda.y, -lambda.z, -lambda.w);
 }
 
-// shade_surface.comp:48-312
-template <bool SPECTRAL> __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t path) {
+// shade_surface.comp:48-312. PROGRAMS: some material has a node-graph
+// program (shader_program.glsl), run once a hit before the surface is read.
+template <bool SPECTRAL, bool PROGRAMS>
+__device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t path) {
     State st(p);
     Pushes out;
     uint32_t pixel = st.u(SU_PIXEL, path);
@@ -570,9 +574,25 @@ template <bool SPECTRAL> __device__ Pushes shade_camera(const Params &p, const P
     if (total >= (uint32_t)K(p, K_INFO).y) return out;
 
     V3 normal = shading_normal<SPECTRAL>(p, attributes, hit, geometric);
+    // The material's node graph, once for this hit, on the outward normal; it
+    // may bend the normal. A program that reads the view makes the light
+    // tracer's density past the first vertex zero here (`viewed`).
+    Program program;
+    const Program *g = nullptr;
+    bool viewed = false;
+    if constexpr (PROGRAMS) {
+        run_program(p, program, attributes.w, tri, x, inside ? -normal : normal, -rd,
+                    triangle_uv(p, tri, hit.z, hit.w));
+        g = &program;
+        viewed = program.viewed;
+        if (program.driven(9)) {
+            normal = normalize(program.driven3(11));
+            if (dot(normal, geometric) < 0.0f) normal = -normal;
+        }
+    }
     Surface surface = surface_at(normal, geometric, attributes.w, inside);
     V3 wo = to_local(surface, -rd);
-    prepare_surface<SPECTRAL>(p, surface, wo, lambda, hero);
+    prepare_surface<SPECTRAL>(p, surface, wo, lambda, hero, g);
 
     V3 from = offset_origin(x, geometric);
 
@@ -635,14 +655,18 @@ template <bool SPECTRAL> __device__ Pushes shade_camera(const Params &p, const P
                 if (tracing && !s.delta && emitter != kNoneU && density > 0.0f) {
                     float d2 = s.distance * s.distance;
                     float at_last = total == 0u ? 1.0f
-                                                : light_sampler_pdf<SPECTRAL>(p, surface, s.direction, -rd, hero) * back_reach / mis.y;
+                                    : (viewed ? 0.0f
+                                              : light_sampler_pdf<SPECTRAL>(p, surface, s.direction, -rd, hero, g) *
+                                                    back_reach / mis.y);
                     float at_vertex = emission_direction_pdf(p, emitter, s.cos_light) * fabsf(side) / d2 / density;
                     float at_light = emitter_area_pdf(p, emitter) / (bsdf_pdf * s.cos_light / d2);
                     traced = mis.x * at_last * at_vertex * at_light;
                 }
                 if (tracing && afar != kNoneU && emits_from_afar(p, afar) && density > 0.0f) {
                     float at_last = total == 0u ? 1.0f
-                                                : light_sampler_pdf<SPECTRAL>(p, surface, s.direction, -rd, hero) * back_reach / mis.y;
+                                    : (viewed ? 0.0f
+                                              : light_sampler_pdf<SPECTRAL>(p, surface, s.direction, -rd, hero, g) *
+                                                    back_reach / mis.y);
                     float at_vertex = afar_first_density(p, x, s.direction, fabsf(side)) / density;
                     float at_light = afar_emission_pdf(p, afar, s.direction) / bsdf_pdf;
                     traced = mis.x * at_last * at_vertex * at_light;
@@ -718,7 +742,8 @@ template <bool SPECTRAL> __device__ Pushes shade_camera(const Params &p, const P
     st.set_v(SV_ORIGIN, path, f4(next, 0.0f));
     st.set_v(SV_VERTEX, path, f4(next, survival_scale));
     if (tracing) {
-        float at_last = total == 0u ? 1.0f : light_sampler_pdf<SPECTRAL>(p, surface, direction, -rd, hero) * back_reach / mis.y;
+        float at_last = total == 0u ? 1.0f
+                        : (viewed ? 0.0f : light_sampler_pdf<SPECTRAL>(p, surface, direction, -rd, hero, g) * back_reach / mis.y);
         st.set_v(SV_MIS, path, make_float4(mis.x * at_last, density, fabsf(side), 0.0f));
         pdf = bsdf_only;
     }
@@ -749,7 +774,8 @@ __device__ __forceinline__ float light_sample_ratio(const Params &p, uint32_t em
 }
 
 // light_paths.glsl:182-340
-template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t path) {
+template <bool SPECTRAL, bool PROGRAMS>
+__device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t path) {
     State st(p);
     Pushes out;
     uint32_t what = st.u(SU_HIT, path);
@@ -799,11 +825,29 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
     float4 ul = sample4(p.seed, id, ps.sample, group_light(total));
     V3 lens = lens_point(p, ul.x, ul.y);
     bool on_film = camera_pixel(p, x, lens, pixel, to_point, camera_distance);
+    // The material's node graph as camera paths run it (a program reading the
+    // view sees the point from the camera). Such a program has no surface for
+    // directions the camera does not see from: the path connects here, if it
+    // can, and goes no further.
+    Program program;
+    const Program *g = nullptr;
+    bool viewed = false;
+    if constexpr (PROGRAMS) {
+        run_program(p, program, attributes.w, tri, x, back ? -normal : normal, on_film ? -to_point : -rd,
+                    triangle_uv(p, tri, hit.z, hit.w));
+        g = &program;
+        viewed = program.viewed;
+        if (viewed && !on_film) return out;
+        if (program.driven(9)) {
+            normal = normalize(program.driven3(11));
+            if (dot(normal, geometric) < 0.0f) normal = -normal;
+        }
+    }
 
     Surface lit = surface_at(normal, geometric, attributes.w, back);
     V3 toward_light = -rd;
-    read_material<SPECTRAL>(p, lit, hero);
-    read_colors<SPECTRAL>(p, lit, lambda);
+    read_material<SPECTRAL>(p, lit, hero, g);
+    read_colors<SPECTRAL>(p, lit, lambda, g);
     V3 x_last = v3(vertex_row);
     float reach = distance(x_last, x);
     float cos_in = fabsf(dot(rd, geometric));
@@ -825,7 +869,7 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
         V3 wo = to_local(seen, to_camera), wi = to_local(seen, toward_light);
         if ((dot(toward_light, seen.geometric) > 0.0f) == (wi.z > 0.0f) && wo.z > 0.0f) {
             if (side > 0.0f) orient_surface(p, seen, wo);
-            else prepare_surface<SPECTRAL>(p, seen, wo, lambda, hero);
+            else prepare_surface<SPECTRAL>(p, seen, wo, lambda, hero, g);
             float camera_bsdf_pdf;
             Spec f = evaluate_surface(seen, wi, wo, camera_bsdf_pdf);
             float camera_density = camera_pdf(p, to_point) * fabsf(side) / (camera_distance * camera_distance);
@@ -846,7 +890,7 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
         }
     }
 
-    if (total + 1u >= (uint32_t)K(p, K_INFO).y) return out;
+    if (viewed || total + 1u >= (uint32_t)K(p, K_INFO).y) return out;
     float4 u = sample4(p.seed, id, ps.sample, group_bsdf(total));
     Surface choosing = lit;
     light_lobes(choosing);
@@ -862,7 +906,7 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
     V3 wo = to_local(seen, direction), wi = to_local(seen, toward_light);
     if ((dot(toward_light, seen.geometric) > 0.0f) != (wi.z > 0.0f)) return out;
     if (side > 0.0f) orient_surface(p, seen, wo);
-    else prepare_surface<SPECTRAL>(p, seen, wo, lambda, hero);
+    else prepare_surface<SPECTRAL>(p, seen, wo, lambda, hero, g);
     float camera_bsdf_pdf;
     Spec f = evaluate_surface(seen, wi, wo, camera_bsdf_pdf);
     bounces = next_bounce(p, bounces, lobe);
@@ -892,7 +936,7 @@ template <bool SPECTRAL> __device__ Pushes shade_light(const Params &p, const Pa
 // needs fewer registers than the two together, so more waves fit; capped at
 // 168 registers (3 waves a SIMD, a few spilled values): measured faster than
 // the uncapped 215 (2 waves) and than 128 (4 waves, more spills).
-template <bool SPECTRAL, bool LIGHT>
+template <bool SPECTRAL, bool LIGHT, bool PROGRAMS>
 __global__ __launch_bounds__(128, 3) void shade_kernel(const Params *plist, uint32_t n, uint32_t current) {
     __shared__ RowPlan plan;
     const uint32_t queue = LIGHT ? Q_LIGHT : Q_SURFACE;
@@ -909,8 +953,8 @@ __global__ __launch_bounds__(128, 3) void shade_kernel(const Params *plist, uint
         uint32_t path = 0;
         if (active) {
             path = queue_path(p, queue, at);
-            if constexpr (LIGHT) pu = shade_light<SPECTRAL>(p, ps, path);
-            else pu = shade_camera<SPECTRAL>(p, ps, path);
+            if constexpr (LIGHT) pu = shade_light<SPECTRAL, PROGRAMS>(p, ps, path);
+            else pu = shade_camera<SPECTRAL, PROGRAMS>(p, ps, path);
         }
         staged.push(0, path, pu.closest);
         staged.push(1, path, pu.shadow);
@@ -1359,8 +1403,10 @@ static void launch_stage(Stage stage, const rt::Params *p, uint32_t n, uint32_t 
     switch (stage) {
         case Stage::Camera: LR_STAGE((rt::camera_kernel<SPECTRAL>), 256, p, n); break;
         case Stage::LightEmit: LR_STAGE((rt::light_emit_kernel<SPECTRAL>), 256, p, n); break;
-        case Stage::Shade: LR_STAGE((rt::shade_kernel<SPECTRAL, false>), 128, p, n, arg); break;
-        case Stage::ShadeLight: LR_STAGE((rt::shade_kernel<SPECTRAL, true>), 128, p, n, arg); break;
+        case Stage::Shade: LR_STAGE((rt::shade_kernel<SPECTRAL, false, false>), 128, p, n, arg); break;
+        case Stage::ShadeLight: LR_STAGE((rt::shade_kernel<SPECTRAL, true, false>), 128, p, n, arg); break;
+        case Stage::ShadeProgram: LR_STAGE((rt::shade_kernel<SPECTRAL, false, true>), 128, p, n, arg); break;
+        case Stage::ShadeLightProgram: LR_STAGE((rt::shade_kernel<SPECTRAL, true, true>), 128, p, n, arg); break;
         case Stage::Route: LR_STAGE(rt::route_kernel, 256, p, n, arg); break;
         case Stage::Miss: LR_STAGE((rt::miss_kernel<SPECTRAL>), 256, p, n); break;
         case Stage::Finish: LR_STAGE((rt::finish_kernel<SPECTRAL>), 256, p, n); break;

@@ -52,7 +52,10 @@ class SceneInfo(ctypes.Structure):
 class Loaded(ctypes.Structure):
     _fields_ = [("lights", ctypes.c_void_p), ("light_floats", ctypes.c_size_t), ("constants", ctypes.c_void_p),
                 ("materials", ctypes.c_void_p), ("material_floats", ctypes.c_size_t), ("emitters", ctypes.c_void_p),
-                ("emitter_words", ctypes.c_size_t), ("light_paths", ctypes.c_uint32), ("through", ctypes.c_uint32)]
+                ("emitter_words", ctypes.c_size_t), ("light_paths", ctypes.c_uint32), ("through", ctypes.c_uint32),
+                ("programs", ctypes.c_void_p), ("program_words", ctypes.c_size_t), ("textures", ctypes.c_void_p),
+                ("texture_floats", ctypes.c_size_t), ("texture_table", ctypes.c_void_p),
+                ("texture_count", ctypes.c_size_t)]
 
 
 class RenderSettingsC(ctypes.Structure):

@@ -7,6 +7,7 @@ from dataclasses import dataclass, field, replace
 
 import numpy as np
 
+from . import shader_graph as SG
 from . import spectrum as S
 from ._lib import INVALID, LuceError
 
@@ -47,6 +48,11 @@ class OpenPbr:
     # A tabulated reflectance for the base colour (64 bins over 380..720 nm):
     # fitted to a sigmoid spectrum through its ACEScg colour.
     base_spectrum: object = None
+    # A node graph driving some of the parameters (shader_graph.lucb), and the
+    # images its image nodes name: {name: pixels, height x width x 3 or 4,
+    # rows from the top; colour images sRGB-encoded in 0..1, data as stored}.
+    shader_graph: str = ""
+    images: object = None
 
     @staticmethod
     def of(record):
@@ -67,6 +73,23 @@ def cauchy_b(m):
     f, c = 486.1, 656.3
     return (m.specular_ior - 1.0) * m.transmission_dispersion_scale / (
         m.transmission_dispersion_abbe_number * (1.0 / (f * f) - 1.0 / (c * c)))
+
+
+def texture_texels(pixels, color):
+    """An image node's texels (textures.lucb): RGBA f32, rows from the top;
+    a colour image's sRGB-encoded values decoded to linear light (the
+    reference's 8-bit sRGB textures decode before filtering), data as stored,
+    alpha as stored (1 when absent)."""
+    px = np.asarray(pixels, np.float64)
+    if px.ndim == 2:
+        px = np.repeat(px[:, :, None], 3, axis=2)
+    if px.ndim != 3 or px.shape[2] < 3 or px.shape[0] < 1 or px.shape[1] < 1:
+        raise LuceError(INVALID, "an image needs height x width x 3 (or 4) pixels")
+    rgb = px[:, :, :3]
+    if color:
+        rgb = np.where(rgb <= 0.04045, rgb / 12.92, ((np.maximum(rgb, 0.0) + 0.055) / 1.055) ** 2.4)
+    alpha = px[:, :, 3] if px.shape[2] > 3 else np.ones(px.shape[:2])
+    return np.concatenate([rgb, alpha[:, :, None]], axis=2).astype(np.float32)
 
 
 def pack_material(m, program=-1.0):
@@ -487,10 +510,38 @@ class Loaded:
             rows[72 + r * 4:72 + r * 4 + 3] = basis[r]
         rows[64:72] = dome_row
         used = set(int(m) for m in np.unique(scene.shading[:, 3])) if scene.triangle_count() else set()
+        # Node graphs: every material's program in one word array, each at
+        # its entry (-1: none); their images in one texel array with a table
+        # (first texel, width, height) per texture slot (textures.lucb).
+        programs, texels, table, slots = [], [], [], {}
+
+        def program_of(m):
+            if not m.shader_graph:
+                return -1.0
+
+            def resolve(name, color):
+                images = m.images or {}
+                if name not in images:
+                    raise LuceError(INVALID, f"a node graph reads image {name!r} its material does not carry")
+                key = (id(images[name]), bool(color))
+                if key not in slots:
+                    t = texture_texels(images[name], color)
+                    slots[key] = len(table)
+                    table.append([sum(len(x) for x in texels) // 4, t.shape[1], t.shape[0], 0])
+                    texels.append(t.reshape(-1))
+                return slots[key]
+
+            words = SG.compile_graph(m.shader_graph, resolve)
+            if words is None:
+                return -1.0
+            entry = len(programs)
+            programs.extend(words)
+            return float(entry)
+
         materials = []
         cutouts = False
         for index, m in enumerate(library):
-            materials += pack_material(m)
+            materials += pack_material(m, program_of(m))
             cutouts = cutouts or (index in used and m.geometry_opacity < 1.0)
         rows[3] = 1.0 if cutouts else 0.0
         distant = any(lights[i * LIGHT_STRIDE * 4 + 3] == 4.0 or
@@ -520,6 +571,10 @@ class Loaded:
         self.lights = np.array(packed if packed else [0.0] * 4, np.float32)
         self.constants = rows.astype(np.float32)
         self.materials = np.array(materials, np.float32)
+        self.programs = np.array(programs if programs else [0], np.uint32)
+        self.textures = np.concatenate(texels).astype(np.float32) if texels else np.zeros(4, np.float32)
+        self.texture_table = np.array(table if table else [[0, 0, 0, 0]], np.uint32).reshape(-1)
+        self.texture_count = len(table)
         self.emitters = emitters if len(emitters) else np.zeros(2, np.uint32)
         self.light_count = light_count
         self.through = 8 if cutouts else 0

@@ -430,6 +430,9 @@ class Renderer:
         ld.materials, ld.material_floats = _lib.ptr(self.loaded.materials), len(self.loaded.materials)
         ld.emitters, ld.emitter_words = _lib.ptr(self.loaded.emitters), len(self.loaded.emitters)
         ld.light_paths, ld.through = self.loaded.light_paths, self.loaded.through
+        ld.programs, ld.program_words = _lib.ptr(self.loaded.programs), len(self.loaded.programs)
+        ld.textures, ld.texture_floats = _lib.ptr(self.loaded.textures), len(self.loaded.textures)
+        ld.texture_table, ld.texture_count = _lib.ptr(self.loaded.texture_table), self.loaded.texture_count
         s = settings
         cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, s.pool, s.seed,
                                   int(s.guiding), 1 if s.profile else 0, 1 if s.use_graph else 0,

@@ -390,6 +390,175 @@ def dome_set(texture=None, albedo=0.8, frequency=16, intensity=1.0, ground=True,
     return gset
 
 
+GRAPH_FLOOR = """
+node s openpbr_surface
+node c checker
+input c scale 4
+input c color1 0.8 0.7 0.6
+input c color2 0.1 0.1 0.12
+link s base_color c color
+"""
+
+GRAPH_BALL = """
+node s openpbr_surface
+node n noise
+input n scale 3
+input n detail 3.5
+node r ramp
+link r fac n fac
+input r color1 0.1 0.3 0.7
+input r color2 0.9 0.8 0.2
+input r position1 0.3
+input r position2 0.7
+link s base_color r color
+node b bump
+input b strength 0.6
+input b distance 0.5
+link b height n fac
+link s geometry_normal b normal
+node m map_range
+link m value n fac
+input m to_min 0.1
+input m to_max 0.6
+node k clamp
+link k value m value
+input k min 0.2
+input k max 0.5
+link s specular_roughness k value
+node f fresnel
+input f ior 1.45
+link s coat_weight f fac
+"""
+
+GRAPH_WALL = """
+node s openpbr_surface
+node i image wall
+node t image bumps data
+node nm normal_map
+link nm color t color
+input nm strength 0.8
+link s base_color i color
+link s geometry_normal nm normal
+node w math multiply
+link w a i alpha
+input w b 0.5
+link s base_metalness w value
+node x separate
+link x color i color
+node y combine
+link y r x b
+link y g x r
+link y b x g
+node z mix
+input z fac 0.3
+link z a y color
+input z b 1 1 1
+link s specular_color z color
+node lw layer_weight
+input lw blend 0.3
+node pw math power
+link pw a lw facing
+input pw b 2
+link s specular_weight pw value
+"""
+
+
+def graph_images(seed=13):
+    """The wall's images: an 8 x 6 colour image with alpha (sRGB-encoded,
+    rows from the top) and a 4 x 4 tangent-space normal map (data)."""
+    rng = np.random.default_rng(seed)
+    wall = rng.random((6, 8, 4))
+    wall[:, :, 3] = 0.25 + 0.75 * rng.random((6, 8))
+    tilt = rng.normal(0.0, 0.35, (4, 4, 2))
+    n = np.concatenate([tilt, np.ones((4, 4, 1))], axis=2)
+    n /= np.linalg.norm(n, axis=2, keepdims=True)
+    return {"wall": wall, "bumps": 0.5 + 0.5 * n}
+
+
+def graph_set(frequency=24):
+    """Materials driven by node graphs (shader_graph.lucb) under the textured
+    dome: a checker floor by its uvs; a ball whose noise drives its colour
+    (a ramp), bump, roughness (map range and clamp) and, by Fresnel, its coat;
+    a back wall whose image and normal-map textures drive colour, normal and
+    metalness, with separate/combine/mix specular colour and a layer weight."""
+    pts, tris = icosphere(frequency)
+    meshes = [Mesh(pts, tris, np.arange(len(tris)), len(tris), face_materials=["ball"] * len(tris))]
+    floor = np.array([[-6.0, -1.0, -6.0], [6.0, -1.0, -6.0], [6.0, -1.0, 6.0], [-6.0, -1.0, 6.0]])
+    corner_uv = np.array([[0.0, 0.0], [3.0, 0.0], [3.0, 3.0], [0.0, 3.0]])
+    ftris = np.array([[0, 2, 1], [0, 3, 2]])
+    meshes.append(Mesh(floor, ftris, np.array([0, 0]), 1, face_materials=["floor"], uvs=corner_uv[ftris]))
+    wall = np.array([[-3.0, -1.0, -2.5], [3.0, -1.0, -2.5], [3.0, 2.5, -2.5], [-3.0, 2.5, -2.5]])
+    wtris = np.array([[0, 1, 2], [0, 2, 3]])
+    wall_uv = np.array([[0.0, 1.0], [2.0, 1.0], [2.0, 0.0], [0.0, 0.0]])
+    meshes.append(Mesh(wall, wtris, np.array([0, 0]), 1, face_materials=["wall"], uvs=wall_uv[wtris]))
+    gset = GeometrySet(meshes=meshes)
+    gset.materials = {"ball": {"base_color": (0.5, 0.5, 0.5), "coat_color": (0.9, 0.95, 1.0),
+                               "shader_graph": GRAPH_BALL},
+                      "floor": {"base_color": (0.5, 0.5, 0.5), "shader_graph": GRAPH_FLOOR},
+                      "wall": {"base_color": (0.5, 0.5, 0.5), "specular_roughness": 0.35,
+                               "shader_graph": GRAPH_WALL, "images": graph_images()}}
+    gset.lights.append(Light("dome", placement(), (1.0, 1.0, 1.0), 1.0, texture=sky_texture()))
+    gset.cameras.append(Camera(placement(translation=(0.0, 0.3, 4.5)), "perspective", 40.0, 36.0))
+    return gset
+
+
+def grid_mesh(size=20.0, divisions=8, material=None, uv_scale=0.05):
+    """A size x size floor at y = 0 in divisions^2 squares (two triangles
+    each, counter-clockwise from above, normals up), corner uvs
+    uv_scale * (x, z)."""
+    t = np.linspace(-0.5 * size, 0.5 * size, divisions + 1)
+    xs, zs = np.meshgrid(t, t, indexing="xy")
+    pts = np.stack([xs.reshape(-1), np.zeros(xs.size), zs.reshape(-1)], 1)
+    tris = []
+    for j in range(divisions):
+        for i in range(divisions):
+            a = j * (divisions + 1) + i
+            b, c, d = a + 1, a + divisions + 2, a + divisions + 1
+            tris += [[a, d, c], [a, c, b]]
+    tris = np.array(tris)
+    n = np.zeros((len(tris), 3, 3))
+    n[:, :, 1] = 1.0
+    uvs = (pts[:, [0, 2]] * uv_scale)[tris]
+    return Mesh(pts, tris, np.arange(len(tris)) // 2, divisions * divisions, normals=n,
+                face_materials=None if material is None else [material] * (divisions * divisions), uvs=uvs)
+
+
+def _rotation_x(radians):
+    c, s = math.cos(radians), math.sin(radians)
+    return np.array([[1.0, 0.0, 0.0], [0.0, c, s], [0.0, -s, c]])
+
+
+def graph_floor_set(material):
+    """The reference's floor window (tests/gpu/integrator.lucb:754-769): a
+    20 x 20 floor (8 x 8 squares, uvs 0.05 (x, z)) of `material` under a 1 x 1
+    rect light at y = 1 facing down (radiance 4), seen from (0, 0.5, 3)
+    pitched down toward the light's foot through a 200 mm lens."""
+    gset = GeometrySet(meshes=[grid_mesh(material="floor")])
+    gset.materials = {"floor": material}
+    m = placement(rotation=_rotation_x(-0.5 * math.pi), translation=(0.0, 1.0, 0.0))
+    gset.lights.append(Light("rect", m, (1.0, 1.0, 1.0), 4.0, width=1.0, height=1.0))
+    pitch = -math.atan2(0.5, 3.0)
+    gset.cameras.append(Camera(placement(rotation=_rotation_x(pitch), translation=(0.0, 0.5, 3.0)), "perspective",
+                               200.0, 36.0))
+    return gset
+
+
+def viewed_graph_set(material, frequency=16):
+    """The reference's viewed-graph scene (tests/gpu/integrator.lucb:701-730):
+    a radius-0.5 sphere at the origin and a 20 x 20 floor at y = 0, both of
+    `material`, under a 1 x 1 rect light at y = 2 facing down (radiance 8),
+    seen from (0, 0.6, 3) pitched down 0.1 rad through a 50 mm lens."""
+    pts, tris = icosphere(frequency)
+    ball = Mesh(np.asarray(pts) * 0.5, tris, np.arange(len(tris)), len(tris), face_materials=["viewed"] * len(tris))
+    gset = GeometrySet(meshes=[ball, grid_mesh(material="viewed")])
+    gset.materials = {"viewed": material}
+    m = placement(rotation=_rotation_x(-0.5 * math.pi), translation=(0.0, 2.0, 0.0))
+    gset.lights.append(Light("rect", m, (1.0, 1.0, 1.0), 8.0, width=1.0, height=1.0))
+    gset.cameras.append(Camera(placement(rotation=_rotation_x(-0.1), translation=(0.0, 0.6, 3.0)), "perspective", 50.0,
+                               36.0))
+    return gset
+
+
 def many_lights_set(count=40, seed=5):
     """The Cornell render box lit by `count` small analytic lights instead of
     its rect light: rect, disk and sphere lights in turn (0.12 wide, radius

@@ -49,6 +49,9 @@ class Server:
         ld.materials, ld.material_floats = _lib.ptr(loaded.materials), len(loaded.materials)
         ld.emitters, ld.emitter_words = _lib.ptr(loaded.emitters), len(loaded.emitters)
         ld.light_paths, ld.through = loaded.light_paths, loaded.through
+        ld.programs, ld.program_words = _lib.ptr(loaded.programs), len(loaded.programs)
+        ld.textures, ld.texture_floats = _lib.ptr(loaded.textures), len(loaded.textures)
+        ld.texture_table, ld.texture_count = _lib.ptr(loaded.texture_table), loaded.texture_count
         s = settings
         cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, 0, s.seed,
                                   int(s.guiding), 0, 0, 0, 0, 0, 0, 0)

@@ -43,7 +43,10 @@ def lib():
             "or_render": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, _P, _P, _P,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, ctypes.c_size_t,
-                                         ctypes.c_int, _P]),
+                                         ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t]),
+            "or_run_program": (None, [_P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.c_uint32,
+                                      _P, _P, _P, _P, _P, _P, _P]),
+            "or_gradient_noise": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "or_sample4": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, _P]),
             "or_make_tables": (None, [_P]),
             "or_brute_force": (None, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_uint32, ctypes.c_uint32, _P,
@@ -194,5 +197,31 @@ def render(scene, loaded, frame16, width, height, samples, max_bounces, spectral
                     _p(np.ascontiguousarray(scene.nodes)), len(scene.nodes),
                     _p(loaded.lights), _p(loaded.constants), _p(loaded.materials), _p(loaded.emitters),
                     _p(np.ascontiguousarray(frame16, np.float32)), width, height, samples, max_bounces,
-                    1 if spectral else 0, seed, loaded.through, _p(pixels), len(pixels), threads, _p(out))
+                    1 if spectral else 0, seed, loaded.through, _p(pixels), len(pixels), threads, _p(out),
+                    _p(np.ascontiguousarray(scene.uvs, np.float32)), _p(loaded.programs), len(loaded.programs),
+                    _p(loaded.textures), _p(loaded.texture_table), loaded.texture_count)
     return out
+
+
+def run_program(materials, programs, material, x, n, view, uv, corners=None, uvs=None, frame=None, textures=None,
+                texture_table=None, texture_count=0):
+    """One node-graph program run (shader_program.glsl:205-349) at a hit on
+    one triangle: (mask, viewed, values[23])."""
+    materials = np.ascontiguousarray(materials, np.float32)
+    programs = np.ascontiguousarray(programs, np.uint32)
+    corners = np.ascontiguousarray(np.zeros(9) if corners is None else corners, np.float32).reshape(-1)
+    uvs = np.ascontiguousarray(np.zeros(6) if uvs is None else uvs, np.float32).reshape(-1)
+    frame = np.ascontiguousarray(np.zeros(16) if frame is None else frame, np.float32)
+    textures = np.ascontiguousarray(np.zeros(4) if textures is None else textures, np.float32)
+    texture_table = np.ascontiguousarray(np.zeros(4) if texture_table is None else texture_table, np.uint32)
+    d = [np.ascontiguousarray(v, np.float64) for v in (x, n, view, uv)]
+    mask, viewed = ctypes.c_uint32(0), ctypes.c_int(0)
+    values = np.zeros(23, np.float64)
+    lib().or_run_program(_p(materials), _p(programs), len(programs), _p(textures), _p(texture_table), texture_count,
+                         _p(corners), _p(uvs), _p(frame), material, *[_p(v) for v in d], ctypes.byref(mask),
+                         ctypes.byref(viewed), _p(values))
+    return mask.value, bool(viewed.value), values
+
+
+def gradient_noise(x, y, z):
+    return lib().or_gradient_noise(x, y, z)

@@ -18,6 +18,9 @@
  *                    albedo tables by tables.comp's estimator, in f64
  *   miss             shaders/shade_miss.comp:12-51
  *   shadow           shaders/intersect_shadow.comp:28-97
+ *   node graphs      shaders/shader_program.glsl:1-349 (the interpreter),
+ *                    openpbr.glsl:41-70, 264-287, 391-401 (uvs, textures,
+ *                    the parameters a program drives)
  * Ray casts use the scene's binary BVH with an f64 slab test and f64
  * Moller-Trumbore (the brute force's arithmetic, tests/gpu/main.lucb:105-119).
  *
@@ -69,6 +72,10 @@ typedef struct {
     const double *frame;     /* 16 floats of the packed frame, as f64 */
     uint32_t width, height, samples, max_bounces, spectral, seed, through;
     double *tables;          /* f64 albedo tables */
+    /* node graphs: leaf-order uvs (6 a triangle), program words, RGBA texels
+     * and per texture (first texel, width, height, 0) */
+    const float *uvs; const uint32_t *programs; size_t program_words;
+    const float *textures; const uint32_t *texture_table; size_t texture_count;
 } oscene;
 
 static v3 point_of(const oscene *s, uint32_t i) {
@@ -290,7 +297,7 @@ static double dome_pdf(const oscene *s, uint32_t l, v3 d) {
 }
 static double logistic(double x) { return 1.0 / (1.0 + exp(-x)); }
 /* spectrum.glsl:67-79: a texel's ACEScg as the three basis spectra's sum. */
-static spec spec_of_texel(const oscene *s, const float *rgb, spec lambda) {
+static spec spec_of_rgb(const oscene *s, const double *rgb, spec lambda) {
     spec r = S1(0.0);
     if (s->spectral) {
         double w[3];
@@ -303,6 +310,10 @@ static spec spec_of_texel(const oscene *s, const float *rgb, spec lambda) {
         r.v[0] = rgb[0]; r.v[1] = rgb[1]; r.v[2] = rgb[2];
     }
     return r;
+}
+static spec spec_of_texel(const oscene *s, const float *rgb, spec lambda) {
+    double d[3] = {rgb[0], rgb[1], rgb[2]};
+    return spec_of_rgb(s, d, lambda);
 }
 static spec dome_radiance(const oscene *s, uint32_t l, v3 d, spec lambda) {
     spec r = light_radiance(s, l, lambda);
@@ -669,6 +680,349 @@ static spec f82(spec f0, spec tint, double mu) {
 }
 static double lum(double r, double g, double b) { return r * 0.2722287168 + g * 0.6740817658 + b * 0.0536895174; }
 
+/* ------------------------------------------------------- node graphs
+ * shader_program.glsl: a material's program runs once a hit and drives some
+ * OpenPBR parameters (openpbr.glsl:52-60). f64 throughout; word reads stop at
+ * the words' end and stack indices stay in the 64-value stack. */
+
+#define PROGRAM_STACK 64u
+#define SLOT_TAG 0x7FC10000u
+typedef struct { uint32_t mask; int viewed; double values[23]; } oprog;
+
+static int driven(const oprog *g, uint32_t bit) { return g && ((g->mask >> bit) & 1u); }
+static v3 driven3(const oprog *g, uint32_t at) { return V(g->values[at], g->values[at + 1], g->values[at + 2]); }
+
+/* openpbr.glsl:41-50 */
+static void triangle_uv(const oscene *s, uint32_t tri, double b1, double b2, double uv[2]) {
+    const float *t = s->uvs + (size_t)tri * 6;
+    double w = 1.0 - b1 - b2;
+    uv[0] = t[0] * w + t[2] * b1 + t[4] * b2;
+    uv[1] = t[1] * w + t[3] * b1 + t[5] * b2;
+}
+
+/* openpbr.glsl:66-68: linear filtering, repeat addressing, over fract(uv);
+ * texel centres at (i + 0.5) / size, v = 0 the first row. */
+static void texture_at(const oscene *s, uint32_t slot, double u, double v, double out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 1.0;
+    if (slot >= s->texture_count) return;
+    const uint32_t *t = s->texture_table + (size_t)slot * 4;
+    long w = t[1], h = t[2];
+    if (!w || !h) return;
+    double x = fract(u) * w - 0.5, y = fract(v) * h - 0.5;
+    double fx = floor(x), fy = floor(y);
+    if (!(fx >= -1.0 && fx <= (double)w)) fx = x = 0.0;
+    if (!(fy >= -1.0 && fy <= (double)h)) fy = y = 0.0;
+    double ax = x - fx, ay = y - fy;
+    long xs[2] = {(long)fx, (long)fx + 1}, ys[2] = {(long)fy, (long)fy + 1};
+    for (int k = 0; k < 2; k++) {
+        xs[k] = ((xs[k] % w) + w) % w;
+        ys[k] = ((ys[k] % h) + h) % h;
+    }
+    const float *base = s->textures + (size_t)t[0] * 4;
+    for (int c = 0; c < 4; c++) {
+        double a = base[((size_t)ys[0] * w + xs[0]) * 4 + c], b = base[((size_t)ys[0] * w + xs[1]) * 4 + c];
+        double d = base[((size_t)ys[1] * w + xs[0]) * 4 + c], e = base[((size_t)ys[1] * w + xs[1]) * 4 + c];
+        out[c] = mixd(mixd(a, b, ax), mixd(d, e, ax), ay);
+    }
+}
+
+/* shader_program.glsl:27-51: Perlin's improved noise, gradients by a hash of
+ * the lattice point. */
+static v3 gradient_at(int32_t cx, int32_t cy, int32_t cz) {
+    uint32_t h = or_hash_u32((uint32_t)cx * 73856093u ^
+                             or_hash_u32((uint32_t)cy * 19349663u ^ or_hash_u32((uint32_t)cz * 83492791u))) % 12u;
+    double sx = (h & 1u) ? -1.0 : 1.0, sy = (h & 2u) ? -1.0 : 1.0;
+    if (h < 4u) return V(sx, sy, 0);
+    if (h < 8u) return V(sx, 0, sy);
+    return V(0, sx, sy);
+}
+static double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+double or_gradient_noise(double px, double py, double pz) {
+    v3 i = V(floor(px), floor(py), floor(pz)), f = V(px - i.x, py - i.y, pz - i.z);
+    double ux = fade(f.x), uy = fade(f.y), uz = fade(f.z);
+    int32_t cx = (int32_t)i.x, cy = (int32_t)i.y, cz = (int32_t)i.z;
+    double n[8];
+    for (int k = 0; k < 8; k++) {
+        int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+        n[k] = dot(gradient_at(cx + dx, cy + dy, cz + dz), V(f.x - dx, f.y - dy, f.z - dz));
+    }
+    return mixd(mixd(mixd(n[0], n[1], ux), mixd(n[2], n[3], ux), uy), mixd(mixd(n[4], n[5], ux), mixd(n[6], n[7], ux), uy), uz);
+}
+/* shader_program.glsl:53-72 */
+static double fractal_noise(v3 q, double detail, double roughness) {
+    double octaves = clampd(detail, 0.0, 15.0);
+    int whole = (int)floor(octaves);
+    double part = octaves - whole, sum = 0.0, total = 0.0, amplitude = 1.0;
+    for (int octave = 0; octave <= whole; octave++) {
+        sum += or_gradient_noise(q.x, q.y, q.z) * amplitude;
+        total += amplitude;
+        amplitude *= clampd(roughness, 0.0, 1.0);
+        q = mul(q, 2.0);
+    }
+    if (part > 0.0) {
+        sum += or_gradient_noise(q.x, q.y, q.z) * amplitude * part;
+        total += amplitude * part;
+    }
+    return clampd(0.5 + 0.5 * sum / total, 0.0, 1.0);
+}
+/* shader_program.glsl:74-89 */
+static double math_of(uint32_t op, double a, double b) {
+    switch (op) {
+    case 0: return a + b;
+    case 1: return a - b;
+    case 2: return a * b;
+    case 3: return b != 0.0 ? a / b : 0.0;
+    case 4:
+        if (a >= 0.0) return pow(a, b);
+        if (b != floor(b)) return 0.0;
+        return pow(-a, b) * (b - 2.0 * floor(b / 2.0) == 1.0 ? -1.0 : 1.0);
+    case 5: return fmin(a, b);
+    case 6: return fmax(a, b);
+    case 7: return fabs(a);
+    case 8: return sin(a);
+    case 9: return cos(a);
+    case 10: return floor(a);
+    default: return fract(a);
+    }
+}
+/* shader_program.glsl:142-149: Duff et al.'s frame */
+static void bump_tangents(v3 n, v3 *t, v3 *b) {
+    double sg = n.z >= 0.0 ? 1.0 : -1.0, a = -1.0 / (sg + n.z), c = n.x * n.y * a;
+    *t = V(1.0 + sg * n.x * n.x * a, sg * c, -sg * n.x);
+    *b = V(c, sg + n.y * n.y * a, -n.y);
+}
+/* shader_program.glsl:121-130 */
+static double bump_width(const oscene *s, v3 x, double pixels) {
+    const double *f = s->frame;
+    double pixel = len(V(f[8], f[9], f[10])) * (f[3] > 0.5 ? 1.0 : len(sub(x, V(f[0], f[1], f[2]))));
+    return fmax(pixel * fmax(pixels, 0.0), 4e-6 * (1.0 + fmax(fabs(x.x), fmax(fabs(x.y), fabs(x.z)))));
+}
+static void tri_edges(const oscene *s, uint32_t tri, v3 *e1, v3 *e2) {
+    v3 p0 = point_of(s, s->indices[tri * 3]);
+    *e1 = sub(point_of(s, s->indices[tri * 3 + 1]), p0);
+    *e2 = sub(point_of(s, s->indices[tri * 3 + 2]), p0);
+}
+/* shader_program.glsl:160-174 */
+static void uv_step(const oscene *s, uint32_t tri, v3 step, double out[2]) {
+    v3 e1, e2;
+    tri_edges(s, tri, &e1, &e2);
+    double g11 = dot(e1, e1), g12 = dot(e1, e2), g22 = dot(e2, e2), det = g11 * g22 - g12 * g12;
+    out[0] = out[1] = 0.0;
+    if (fabs(det) < 1e-24) return;
+    double a = (g22 * dot(e1, step) - g12 * dot(e2, step)) / det, b = (g11 * dot(e2, step) - g12 * dot(e1, step)) / det;
+    const float *t = s->uvs + (size_t)tri * 6;
+    out[0] = (t[2] - t[0]) * a + (t[4] - t[0]) * b;
+    out[1] = (t[3] - t[1]) * a + (t[5] - t[1]) * b;
+}
+/* shader_program.glsl:91-115 */
+static v3 normal_from_map(const oscene *s, uint32_t tri, v3 n, v3 color, double strength) {
+    v3 e1, e2;
+    tri_edges(s, tri, &e1, &e2);
+    const float *q = s->uvs + (size_t)tri * 6;
+    double d1x = q[2] - q[0], d1y = q[3] - q[1], d2x = q[4] - q[0], d2y = q[5] - q[1];
+    double det = d1x * d2y - d2x * d1y;
+    if (fabs(det) < 1e-12) return n;
+    v3 t = mul(sub(mul(e1, d2y), mul(e2, d1y)), 1.0 / det), b = mul(sub(mul(e2, d1x), mul(e1, d2x)), 1.0 / det);
+    t = sub(t, mul(n, dot(n, t)));
+    if (dot(t, t) < 1e-20) return n;
+    t = norm(t);
+    v3 bt = cross(n, t);
+    if (dot(bt, b) < 0.0) bt = neg(bt);
+    v3 c = V(color.x * 2 - 1, color.y * 2 - 1, color.z * 2 - 1);
+    v3 bent = add(add(mul(t, c.x), mul(bt, c.y)), mul(n, c.z));
+    if (dot(bent, bent) < 1e-12) return n;
+    double k = clampd(strength, 0.0, 1.0);
+    return norm(add(mul(n, 1 - k), mul(norm(bent), k)));
+}
+/* shader_program.glsl:176-190 (Cycles' svm_node_set_bump) */
+static v3 bumped(const oscene *s, v3 x, v3 n, v3 normal, double strength, double dist, double pixels, double hc, double hx,
+                 double hy) {
+    v3 base = dot(normal, normal) > 1e-12 ? norm(normal) : n, t, b;
+    bump_tangents(n, &t, &b);
+    double w = bump_width(s, x, pixels);
+    v3 dx = mul(t, w), dy = mul(b, w), rx = cross(dy, base), ry = cross(base, dx);
+    double det = dot(dx, rx), sg = det > 0 ? 1.0 : (det < 0 ? -1.0 : 0.0);
+    v3 gradient = add(mul(rx, hx - hc), mul(ry, hy - hc));
+    v3 out = sub(mul(base, fabs(det)), mul(gradient, dist * sg));
+    if (dot(out, out) < 1e-30) return base;
+    double k = clampd(strength, 0.0, 1.0);
+    return norm(add(mul(base, 1 - k), mul(norm(out), k)));
+}
+/* shader_program.glsl:192-203 */
+static double facing_cos(v3 n, v3 normal, v3 view) { return dot(view, dot(normal, normal) > 1e-12 ? norm(normal) : n); }
+
+/* shader_program.glsl:205-349 */
+static void run_program(const oscene *s, oprog *g, uint32_t m, uint32_t tri, v3 x, v3 n, v3 view, const double uv[2]) {
+    g->mask = 0;
+    g->viewed = 0;
+    double entry = M(s, m, 13, 0);
+    if (!(entry >= 0.0) || !(entry < (double)s->program_words)) return;
+    double stack[PROGRAM_STACK];
+    memset(stack, 0, sizeof(stack));
+    const uint32_t *code = s->programs;
+    size_t words = s->program_words, pc = (size_t)entry;
+#define WORD() (pc < words ? code[pc++] : 0u)
+#define OPERAND(dst) do { uint32_t w_ = WORD(); float f_; memcpy(&f_, &w_, 4); \
+        dst = (w_ & 0xFFFF0000u) == SLOT_TAG ? stack[w_ & (PROGRAM_STACK - 1u)] : (double)f_; } while (0)
+#define OPERAND3(dst) do { OPERAND(dst.x); OPERAND(dst.y); OPERAND(dst.z); } while (0)
+#define PUT(v) do { double v_ = (v); stack[WORD() & (PROGRAM_STACK - 1u)] = v_; } while (0)
+#define PUT3(v) do { v3 p_ = (v); PUT(p_.x); PUT(p_.y); PUT(p_.z); } while (0)
+    for (uint32_t step = 0; step < 4096u; step++) {
+        if (pc >= words) return;
+        uint32_t op = code[pc++];
+        if (op == 0) return;
+        if (op == 1 || op == 2) {
+            uint32_t axis = WORD();
+            double pixels;
+            OPERAND(pixels);
+            v3 offset = V(0, 0, 0);
+            if (axis != 0) {
+                v3 t, b;
+                bump_tangents(n, &t, &b);
+                offset = mul(axis == 1 ? t : b, bump_width(s, x, pixels));
+            }
+            if (op == 1) {
+                double d[2] = {0, 0};
+                if (axis != 0) uv_step(s, tri, offset, d);
+                PUT3(V(uv[0] + d[0], uv[1] + d[1], 0));
+            } else {
+                PUT3(add(x, offset));
+            }
+        } else if (op == 3) {
+            double v;
+            OPERAND(v);
+            PUT(v);
+        } else if (op == 4 || op == 12 || op == 13) {
+            v3 v;
+            OPERAND3(v);
+            PUT3(v);
+        } else if (op == 5) {
+            uint32_t slot = WORD();
+            int color = WORD() != 0;
+            v3 at;
+            OPERAND3(at);
+            double t[4];
+            texture_at(s, slot, at.x, at.y, t);
+            v3 rgb = V(t[0], t[1], t[2]);
+            if (color)
+                rgb = V(0.613097 * t[0] + 0.339523 * t[1] + 0.047379 * t[2], 0.070194 * t[0] + 0.916354 * t[1] + 0.013452 * t[2],
+                        0.020616 * t[0] + 0.109570 * t[1] + 0.869815 * t[2]);
+            PUT3(rgb);
+            PUT(t[3]);
+        } else if (op == 6) {
+            v3 at, a, b;
+            double scale;
+            OPERAND3(at); OPERAND3(a); OPERAND3(b); OPERAND(scale);
+            long cx = (long)floor(at.x * scale + 1e-5), cy = (long)floor(at.y * scale + 1e-5), cz = (long)floor(at.z * scale + 1e-5);
+            double fac = (double)((cx + cy + cz) & 1);
+            PUT3(fac > 0.5 ? a : b);
+            PUT(fac);
+        } else if (op == 7) {
+            v3 at;
+            double scale, detail, rough;
+            OPERAND3(at); OPERAND(scale); OPERAND(detail); OPERAND(rough);
+            v3 q = mul(at, scale);
+            double fac = fractal_noise(q, detail, rough);
+            PUT(fac);
+            PUT3(V(fac, fractal_noise(add(q, V(19.1, 33.4, 47.2)), detail, rough),
+                   fractal_noise(add(q, V(73.7, 11.3, 29.9)), detail, rough)));
+        } else if (op == 8) {
+            uint32_t operation = WORD();
+            double a, b;
+            OPERAND(a); OPERAND(b);
+            PUT(math_of(operation, a, b));
+        } else if (op == 9) {
+            double fac;
+            v3 a, b;
+            OPERAND(fac); OPERAND3(a); OPERAND3(b);
+            fac = clampd(fac, 0, 1);
+            PUT3(V(mixd(a.x, b.x, fac), mixd(a.y, b.y, fac), mixd(a.z, b.z, fac)));
+        } else if (op == 10) {
+            double v, f0, f1, t0, t1;
+            OPERAND(v); OPERAND(f0); OPERAND(f1); OPERAND(t0); OPERAND(t1);
+            double t = f1 != f0 ? (v - f0) / (f1 - f0) : 0.0;
+            PUT(t0 + t * (t1 - t0));
+        } else if (op == 11) {
+            double v, lo, hi;
+            OPERAND(v); OPERAND(lo); OPERAND(hi);
+            PUT(clampd(v, fmin(lo, hi), fmax(lo, hi)));
+        } else if (op == 14) {
+            double fac, first, last;
+            v3 a, b;
+            OPERAND(fac); OPERAND3(a); OPERAND3(b); OPERAND(first); OPERAND(last);
+            double t = last != first ? clampd((fac - first) / (last - first), 0, 1) : (fac < first ? 0.0 : 1.0);
+            PUT3(V(mixd(a.x, b.x, t), mixd(a.y, b.y, t), mixd(a.z, b.z, t)));
+        } else if (op == 15) {
+            v3 color;
+            double strength;
+            OPERAND3(color); OPERAND(strength);
+            PUT3(normal_from_map(s, tri, n, color, strength));
+        } else if (op == 16) {
+            uint32_t first = WORD(), width = WORD(), bit = WORD();
+            for (uint32_t k = 0; k < width && k < 23u; k++) {
+                double v;
+                OPERAND(v);
+                if (first + k < 23u) g->values[first + k] = v;
+            }
+            if (bit < 32u) g->mask |= 1u << bit;
+        } else if (op == 17) {
+            double strength, dist, pixels, hc, hx, hy;
+            v3 normal;
+            OPERAND(strength); OPERAND(dist); OPERAND(pixels); OPERAND(hc); OPERAND(hx); OPERAND(hy); OPERAND3(normal);
+            PUT3(bumped(s, x, n, normal, strength, dist, pixels, hc, hx, hy));
+        } else if (op == 18 || op == 19) {
+            double parameter;
+            v3 normal;
+            OPERAND(parameter); OPERAND3(normal);
+            g->viewed = 1;
+            double c = facing_cos(n, normal, view);
+            if (op == 18) {
+                double eta = fmax(parameter, 1e-5);
+                PUT(fresnel_dielectric(c, c < 0.0 ? 1.0 / eta : eta));
+            } else {
+                double eta = fmax(1.0 - parameter, 1e-5), facing = fabs(c);
+                double fr = fresnel_dielectric(c, c < 0.0 ? eta : 1.0 / eta);
+                if (parameter != 0.5) {
+                    double b = clampd(parameter, 0.0, 1.0 - 1e-5);
+                    facing = pow(facing, b < 0.5 ? 2.0 * b : 0.5 / (1.0 - b));
+                }
+                PUT(fr);
+                PUT(1.0 - facing);
+            }
+        } else {
+            return;
+        }
+    }
+#undef WORD
+#undef OPERAND
+#undef OPERAND3
+#undef PUT
+#undef PUT3
+}
+
+/* One program run at a given hit, for tests: the material records, program
+ * words and textures as the renderer receives them, the triangle's corners and
+ * uvs, the frame; out: mask, viewed, values[23]. */
+void or_run_program(const float *materials, const uint32_t *programs, size_t program_words, const float *textures,
+                    const uint32_t *texture_table, size_t texture_count, const float *corners, const float *uvs,
+                    const float *frame, uint32_t material, const double *x, const double *n, const double *view,
+                    const double *uv, uint32_t *mask, int *viewed, double *values) {
+    oscene s;
+    memset(&s, 0, sizeof(s));
+    static const uint32_t idx[3] = {0, 1, 2};
+    double fr[16];
+    for (int k = 0; k < 16; k++) fr[k] = frame[k];
+    s.positions = corners; s.indices = idx; s.materials = materials; s.frame = fr; s.uvs = uvs;
+    s.programs = programs; s.program_words = program_words; s.textures = textures; s.texture_table = texture_table;
+    s.texture_count = texture_count;
+    oprog g;
+    memset(&g, 0, sizeof(g));
+    run_program(&s, &g, material, 0, V(x[0], x[1], x[2]), V(n[0], n[1], n[2]), V(view[0], view[1], view[2]), uv);
+    *mask = g.mask;
+    *viewed = g.viewed;
+    for (int k = 0; k < 23; k++) values[k] = g.values[k];
+}
+
 typedef struct {
     v3 n, t, b, g;
     uint32_t m;
@@ -684,8 +1038,22 @@ typedef struct {
 static v3 to_local(const surf *s, v3 v) { return V(dot(v, s->t), dot(v, s->b), dot(v, s->n)); }
 static v3 to_world(const surf *s, v3 v) { return add(add(mul(s->t, v.x), mul(s->b, v.y)), mul(s->n, v.z)); }
 
-static void read_material(const oscene *sc, surf *s, double hero) {
+/* A material row as read_material sees it: the record's, with what the
+ * program drives in its place (openpbr.glsl:264-287). */
+static double Mg(const oscene *sc, const oprog *g, uint32_t m, int row, int c) {
+    static const struct { int row, c, bit, at; } drive[] = {
+        {1, 0, 0, 0}, {1, 1, 0, 1}, {1, 2, 0, 2}, {1, 3, 1, 3}, {10, 0, 2, 4}, {10, 1, 3, 5}, {3, 3, 4, 6},
+        {10, 2, 5, 7}, {5, 3, 6, 8}, {11, 1, 7, 9}, {9, 3, 8, 10}, {3, 0, 10, 14}, {3, 1, 10, 15}, {3, 2, 10, 16},
+        {9, 0, 11, 17}, {9, 1, 11, 18}, {9, 2, 11, 19}, {5, 0, 12, 20}, {5, 1, 12, 21}, {5, 2, 12, 22}};
+    if (g && g->mask)
+        for (size_t k = 0; k < sizeof(drive) / sizeof(drive[0]); k++)
+            if (drive[k].row == row && drive[k].c == c && driven(g, (uint32_t)drive[k].bit)) return g->values[drive[k].at];
+    return M(sc, m, row, c);
+}
+
+static void read_material(const oscene *sc, surf *s, double hero, const oprog *g) {
     uint32_t m = s->m;
+#define M(sc, m, row, c) Mg(sc, g, m, row, c)
     s->droughness = M(sc, m, 10, 1);
     s->metal = clampd(M(sc, m, 10, 0), 0, 1);
     s->sw = fmax(M(sc, m, 3, 3), 0);
@@ -713,6 +1081,7 @@ static void read_material(const oscene *sc, surf *s, double hero) {
     s->lspec = fmax(lum(M(sc, m, 3, 0), M(sc, m, 3, 1), M(sc, m, 3, 2)), 0);
     s->ltrans = fmax(lum(M(sc, m, 9, 0), M(sc, m, 9, 1), M(sc, m, 9, 2)), 0.05);
     s->lcoat = lum(M(sc, m, 5, 0), M(sc, m, 5, 1), M(sc, m, 5, 2));
+#undef M
 }
 static void choose(surf *s, double wc, double wm, double ws, double wt, double wd) {
     if (s->inside) { wm = 0; wd = 0; ws = fmax(ws, 0.05); wt = fmax(wt, s->trans > 0 ? 0.05 : 0); }
@@ -744,12 +1113,21 @@ static void orient(const oscene *sc, surf *s, v3 wo) {
     choose(s, s->coat * s->calb, sub_rgb * s->metal, sub_rgb * di * s->salb * s->lspec,
            sub_rgb * di * s->trans * (1 - s->salb) * s->ltrans, sub_rgb * di * (1 - s->trans) * (1 - s->salb) * s->lbase);
 }
-static void colors(const oscene *sc, surf *s, spec lambda) {
-    s->base = sscale(material_color(sc, s->m, 0, lambda), M(sc, s->m, 1, 3));
-    s->mtint = material_color(sc, s->m, 2, lambda);
+/* A color a program drives: a reflectance through the environment basis,
+ * at most one (openpbr.glsl:391-401). */
+static spec driven_color(const oscene *sc, const oprog *g, uint32_t at, spec lambda) {
+    double rgb[3] = {g->values[at], g->values[at + 1], g->values[at + 2]};
+    spec c = spec_of_rgb(sc, rgb, lambda);
+    for (int k = 0; k < 4; k++) c.v[k] = clampd(c.v[k], 0, 1);
+    return c;
+}
+static void colors(const oscene *sc, surf *s, spec lambda, const oprog *g) {
+    double weight = driven(g, 1) ? g->values[3] : M(sc, s->m, 1, 3);
+    s->base = sscale(driven(g, 0) ? driven_color(sc, g, 0, lambda) : material_color(sc, s->m, 0, lambda), weight);
+    s->mtint = driven(g, 10) ? driven_color(sc, g, 14, lambda) : material_color(sc, s->m, 2, lambda);
     s->stint = sscale(s->mtint, s->sw);
-    s->ttint = material_color(sc, s->m, 8, lambda);
-    s->ctint = s->coat > 0 ? material_color(sc, s->m, 4, lambda) : S1(1);
+    s->ttint = driven(g, 11) ? driven_color(sc, g, 17, lambda) : material_color(sc, s->m, 8, lambda);
+    s->ctint = s->coat > 0 ? (driven(g, 12) ? driven_color(sc, g, 20, lambda) : material_color(sc, s->m, 4, lambda)) : S1(1);
     s->substrate = S1(1);
     if (s->coat > 0)
         for (int k = 0; k < 4; k++) {
@@ -1006,6 +1384,18 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
         double b1 = h.u, b2 = h.v;
         v3 normal = norm(add(add(mul(oct_decode(attr[0]), 1 - b1 - b2), mul(oct_decode(attr[1]), b1)), mul(oct_decode(attr[2]), b2)));
         if (dot(normal, geo) < 0) normal = neg(normal);
+        /* the material's program, on the outward normal (shade_surface.comp:134-145) */
+        oprog prog;
+        memset(&prog, 0, sizeof(prog));
+        if (s->program_words) {
+            double uv[2];
+            triangle_uv(s, tri, b1, b2, uv);
+            run_program(s, &prog, attr[3], tri, x, inside ? neg(normal) : normal, neg(rd), uv);
+            if (driven(&prog, 9)) {
+                normal = norm(driven3(&prog, 11));
+                if (dot(normal, geo) < 0) normal = neg(normal);
+            }
+        }
         surf sf;
         memset(&sf, 0, sizeof(sf));
         sf.n = normal;
@@ -1014,9 +1404,9 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
         sf.m = attr[3];
         sf.back = inside;
         v3 wo = to_local(&sf, neg(rd));
-        read_material(s, &sf, hero);
+        read_material(s, &sf, hero, &prog);
         orient(s, &sf, wo);
-        colors(s, &sf, lambda);
+        colors(s, &sf, lambda, &prog);
         v3 from = offset_origin(x, geo);
         double ul[4];
         sample4(s, pixel, index, 2 + 3 * total, ul);
@@ -1123,7 +1513,8 @@ int or_render(const float *positions, const uint32_t *indices, const uint32_t *s
               const void *nodes, size_t node_count, const float *lights, const float *constants,
               const float *materials, const uint32_t *emitters, const float *frame, uint32_t width, uint32_t height,
               uint32_t samples, uint32_t max_bounces, uint32_t spectral, uint32_t seed, uint32_t through,
-              const uint32_t *pixels, size_t n, int threads, double *out) {
+              const uint32_t *pixels, size_t n, int threads, double *out, const float *uvs, const uint32_t *programs,
+              size_t program_words, const float *textures, const uint32_t *texture_table, size_t texture_count) {
     oscene s;
     double fr[16];
     for (int k = 0; k < 16; k++) fr[k] = frame[k];
@@ -1135,6 +1526,8 @@ int or_render(const float *positions, const uint32_t *indices, const uint32_t *s
     s.lights = lights; s.constants = constants; s.materials = materials; s.emitters = emitters;
     s.frame = fr; s.width = width; s.height = height; s.samples = samples; s.max_bounces = max_bounces;
     s.spectral = spectral; s.seed = seed; s.through = through; s.tables = tables;
+    s.uvs = uvs; s.programs = programs; s.program_words = uvs ? program_words : 0;
+    s.textures = textures; s.texture_table = texture_table; s.texture_count = texture_count;
     if (threads < 1) threads = 1;
     if (threads > 64) threads = 64;
     pthread_t th[64];
