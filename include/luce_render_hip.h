@@ -57,6 +57,16 @@ typedef struct lr_scene_arrays {
     size_t triangles;
     const lr_bvh_node *nodes;
     size_t node_count;
+    /* Gaussian splats (CompiledSplats, src/render/splats.lucb:45-70), already
+     * in their tree's leaf order: 16 words a record, the harmonics words the
+     * records name, and the splats' own binary BVH. May be empty; both upload
+     * and build take them as given. */
+    const uint32_t *splat_records;
+    size_t splats;
+    const uint32_t *splat_harmonics;
+    size_t splat_words;
+    const lr_bvh_node *splat_nodes;
+    size_t splat_node_count;
 } lr_scene_arrays;
 
 /* Replaces gpu.Device.open (luce-gpu, used at tests/gpu/main.lucb:18). */
@@ -73,6 +83,13 @@ size_t lr_last_error(char *buffer, size_t capacity);
  * cores. The caller reorders its per-triangle arrays by `order`. */
 int lr_bvh_build(const float *positions, size_t points, const uint32_t *indices, size_t triangles,
                  int threads, lr_bvh_node *nodes, size_t *node_count, uint32_t *order);
+
+/* build_tree (src/render/bvh.lucb:85-133) over boxes, as build_splat_bvh
+ * (src/render/splats.lucb:193-213) builds the splats' tree: `boxes` six floats
+ * each (low xyz, high xyz); `nodes` room for max(1, 2 * count); the leaf order
+ * in `order`. */
+int lr_bvh_build_boxes(const float *boxes, size_t count, int threads, lr_bvh_node *nodes, size_t *node_count,
+                       uint32_t *order);
 
 /* build_bvh (src/render/bvh.lucb:66-217) on the GPU: the same tree, node
  * numbering and leaf order as lr_bvh_build (node boxes equal as values; a zero

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -70,7 +71,8 @@ const lr::TraceVariant &lr_gpu_scene::variant() const {
 
 lr_gpu_scene::~lr_gpu_scene() {
     for (void *p : {wide[0], wide[1], wide[2], tris, faces, shading, uvs, spill, (void *)work, ranges, (void *)binary,
-                    (void *)indices, (void *)parent, (void *)arrive, (void *)slots[0], (void *)slots[1], (void *)slots[2]})
+                    (void *)indices, (void *)parent, (void *)arrive, (void *)slots[0], (void *)slots[1], (void *)slots[2],
+                    splat_records, splat_words, splat_nodes})
         if (p) (void)hipFree(p);
 }
 
@@ -146,6 +148,17 @@ int lr_bvh_build(const float *positions, size_t points, const uint32_t *indices,
     return 0;
 }
 
+int lr_bvh_build_boxes(const float *boxes, size_t count, int threads, lr_bvh_node *nodes, size_t *node_count,
+                       uint32_t *order) {
+    if (!nodes || !node_count || (count && (!boxes || !order)))
+        return fail(LR_INVALID, "lr_bvh_build_boxes needs boxes, nodes and order");
+    if (count >= (1ull << 27)) return fail(LR_LIMIT, "a tree holds at most 2^27 boxes");
+    for (size_t i = 0; i < count * 6; i++)
+        if (!std::isfinite(boxes[i])) return fail(LR_INVALID, "a box has a coordinate that is not finite");
+    *node_count = build_box_bvh(boxes, count, threads, nodes, order);
+    return 0;
+}
+
 }  // extern "C"
 
 namespace {
@@ -177,6 +190,44 @@ int check_arrays(const lr_scene_arrays *a, bool need_nodes) {
     if (a->points > 4294967295ull) return fail(LR_LIMIT, "a scene holds at most 2^32 points");
     for (size_t i = 0; i < a->triangles * 3; i++)
         if (a->indices[i] >= a->points) return fail(LR_INVALID, "a triangle names a point the scene lacks");
+    return 0;
+}
+
+// The splats as given (device.lucb:54-62): records, harmonics and their tree,
+// checked so that every index the kernels follow stays inside the arrays.
+int upload_splats(lr_gpu_scene *s, const lr_scene_arrays *a) {
+    const size_t n = a->splats;
+    if (n == 0) return 0;
+    if (n > (1ull << 26)) return fail(LR_LIMIT, "a render holds at most 2^26 splats");
+    if (!a->splat_records || !a->splat_nodes || a->splat_node_count == 0 || a->splat_node_count > 2 * n)
+        return fail(LR_INVALID, "splats need their records and their BVH (at most 2 nodes a splat)");
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t *r = a->splat_records + i * 16;
+        uint32_t items = r[15] & 31u;
+        if (items > 15u) return fail(LR_INVALID, "a splat has more than 15 SH items");
+        if (items >= 3u) {
+            uint64_t stride = 2u + (items * 3u + 1u) / 2u;
+            if (!a->splat_harmonics || (uint64_t)r[11] + stride > a->splat_words)
+                return fail(LR_INVALID, "a splat's harmonics lie outside the harmonics words");
+        }
+    }
+    for (size_t i = 0; i < a->splat_node_count; i++) {
+        const lr_bvh_node &nd = a->splat_nodes[i];
+        if (nd.count == 0) {
+            if ((size_t)nd.first + 1 >= a->splat_node_count || nd.first == 0)
+                return fail(LR_INVALID, "a splat BVH node names a child past the array");
+        } else if ((size_t)nd.first + nd.count > n || nd.count > 16) {
+            return fail(LR_INVALID, "a splat BVH leaf names splats past the array or more than 16");
+        }
+    }
+    int rc;
+    if ((rc = upload(&s->splat_records, a->splat_records, n * 16, "upload splat records")) ||
+        (rc = upload(&s->splat_nodes, a->splat_nodes, a->splat_node_count, "upload splat BVH")))
+        return rc;
+    if (a->splat_harmonics && a->splat_words &&
+        (rc = upload(&s->splat_words, a->splat_harmonics, a->splat_words, "upload splat harmonics")))
+        return rc;
+    s->splat_count = n;
     return 0;
 }
 
@@ -257,6 +308,7 @@ int lr_scene_upload(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **
     if ((rc = upload(&s->faces, a->faces, a->triangles, "upload faces"))) return rc;
     if (a->shading && (rc = upload(&s->shading, a->shading, a->triangles * 4, "upload shading"))) return rc;
     if (a->uvs && (rc = upload(&s->uvs, a->uvs, a->triangles * 6, "upload uvs"))) return rc;
+    if ((rc = upload_splats(s.get(), a))) return rc;
     s->points = a->triangles ? a->points : 0;
     rc = finish_scene(s.get(), pos.p, idx.p, nodes.p, a->node_count);
     idx.p = nullptr;  // owned by the scene from here
@@ -304,6 +356,7 @@ int lr_scene_build(lr_device *device, const lr_scene_arrays *a, lr_gpu_scene **o
     if ((rc = reorder(a->faces, 1, &s->faces, "upload faces"))) return rc;
     if (a->shading && (rc = reorder(a->shading, 4, &s->shading, "upload shading"))) return rc;
     if (a->uvs && (rc = reorder(a->uvs, 6, &s->uvs, "upload uvs"))) return rc;
+    if ((rc = upload_splats(s.get(), a))) return rc;
     s->points = n ? a->points : 0;
     rc = finish_scene(s.get(), pos.p, idx_leaf.p, nodes.p, count);
     lr_bvh_node *tree = nodes.p;

@@ -308,8 +308,12 @@ class Builder {
 
 }  // namespace
 
-size_t build_binary_bvh(const float *positions, const uint32_t *indices, size_t n, int threads, lr_bvh_node *nodes,
-                        uint32_t *order) {
+namespace {
+
+// build_tree (bvh.lucb:85-133) over primitive boxes: `box_of(t)` gives
+// primitive t's box; centres are the boxes' midpoints.
+template <class BoxOf>
+size_t build_over_boxes(size_t n, int threads, lr_bvh_node *nodes, uint32_t *order, BoxOf box_of) {
     if (n == 0) {
         nodes[0] = lr_bvh_node{{1.0f, 1.0f, 1.0f}, 0, {-1.0f, -1.0f, -1.0f}, 0};
         return 0 + 1;
@@ -321,8 +325,7 @@ size_t build_binary_bvh(const float *positions, const uint32_t *indices, size_t 
     centers.xyz.resize(n * 3);
     auto prep = [&](size_t b, size_t e) {
         for (size_t t = b; t < e; t++) {
-            Box box = empty_box();
-            for (int k = 0; k < 3; k++) grow_point(box, positions + (size_t)indices[t * 3 + k] * 3);
+            Box box = box_of(t);
             boxes[t] = box;
             for (int k = 0; k < 3; k++) centers.xyz[t * 3 + k] = 0.5f * (box.lo[k] + box.hi[k]);
             order[t] = (uint32_t)t;
@@ -365,6 +368,28 @@ size_t build_binary_bvh(const float *positions, const uint32_t *indices, size_t 
         stack.push_back(Item{t.left, child});
     }
     return next;
+}
+
+}  // namespace
+
+size_t build_binary_bvh(const float *positions, const uint32_t *indices, size_t n, int threads, lr_bvh_node *nodes,
+                        uint32_t *order) {
+    return build_over_boxes(n, threads, nodes, order, [&](size_t t) {
+        Box box = empty_box();
+        for (int k = 0; k < 3; k++) grow_point(box, positions + (size_t)indices[t * 3 + k] * 3);
+        return box;
+    });
+}
+
+size_t build_box_bvh(const float *boxes, size_t n, int threads, lr_bvh_node *nodes, uint32_t *order) {
+    return build_over_boxes(n, threads, nodes, order, [&](size_t t) {
+        Box box;
+        for (int k = 0; k < 3; k++) {
+            box.lo[k] = boxes[t * 6 + k];
+            box.hi[k] = boxes[t * 6 + 3 + k];
+        }
+        return box;
+    });
 }
 
 }  // namespace lr

@@ -22,6 +22,9 @@ int hip_fail(int status, const char *what);
 // the reference's pair-allocation order. Returns the node count.
 size_t build_binary_bvh(const float *positions, const uint32_t *indices, size_t triangles, int threads,
                         lr_bvh_node *nodes, uint32_t *order);
+// The same build over primitive boxes (six floats each: low, high), as
+// build_tree (bvh.lucb:85) takes them for the splats' tree.
+size_t build_box_bvh(const float *boxes, size_t count, int threads, lr_bvh_node *nodes, uint32_t *order);
 
 // --------------------------------------------------------------- wide BVH
 

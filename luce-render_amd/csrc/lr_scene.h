@@ -33,6 +33,12 @@ struct lr_gpu_scene {
     uint32_t *slots[3] = {nullptr, nullptr, nullptr};
     float root_lo[3] = {0, 0, 0};
     float root_hi[3] = {0, 0, 0};
+    // Gaussian splats: records (16 words each, leaf order), harmonics words
+    // and their binary tree (32-byte nodes), as uploaded.
+    void *splat_records = nullptr;
+    void *splat_words = nullptr;
+    void *splat_nodes = nullptr;
+    size_t splat_count = 0;
     lr_scene_info info{};
     lr_trace_config config{4, 64, 24, 32};
 

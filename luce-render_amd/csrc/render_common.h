@@ -108,6 +108,13 @@ struct Params {
     const uint4 *texture_table;
     uint32_t program_words;
     uint32_t texture_count;
+    // Gaussian splats (the scene's): records (4 uint4 each, leaf order),
+    // harmonics words, and their binary tree (two float4 a node: lo + first,
+    // hi + count). splat_count 0: no splats.
+    const uint4 *splat_records;
+    const uint32_t *splat_words;
+    const float4 *splat_nodes;
+    uint32_t splat_count;
     float4 *state_v;
     uint32_t *state_u;
     uint32_t *queues;

@@ -193,6 +193,13 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     p.shading = (const uint4 *)sc->shading;
     p.uvs = (const float *)sc->uvs;
     programs = with_programs;
+    splats = sc->splat_count > 0 && loaded->constants[106] > 0.0f;  // K_SPLAT_TREE z
+    if (splats) {
+        p.splat_records = (const uint4 *)sc->splat_records;
+        p.splat_words = (const uint32_t *)sc->splat_words;
+        p.splat_nodes = (const float4 *)sc->splat_nodes;
+        p.splat_count = (uint32_t)sc->splat_count;
+    }
     if (with_programs) {
         p.programs = (const uint32_t *)upload(loaded->programs, loaded->program_words * 4);
         p.program_words = (uint32_t)loaded->program_words;
@@ -281,10 +288,11 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     auto light_paths = [](RenderJob *j) -> uint64_t { return j->params.light_paths; };
     auto slots = [](RenderJob *j) -> uint64_t { return j->slots; };
     auto pixels = [](RenderJob *j) -> uint64_t { return j->params.pixels; };
-    bool adaptive = false, cutouts = false, programs = false;
+    bool adaptive = false, cutouts = false, programs = false, splats = false;
     for (RenderJob *j : jobs) adaptive = adaptive || j->adaptive;
     for (RenderJob *j : jobs) cutouts = cutouts || j->cutouts;
     for (RenderJob *j : jobs) programs = programs || j->programs;
+    for (RenderJob *j : jobs) splats = splats || j->splats;
     auto mark = [&](int kind) {
         if (!marks) return;
         hipEvent_t e;
@@ -305,6 +313,7 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     for (uint32_t b = 0; b < rounds; b++) {
         uint32_t cur = b % 2u;
         tv.closest(params, n, cur, blocks(tv.group, slots), s);
+        if (splats) launch_render_stage(Stage::Splat, sp, params, n, cur, blocks(128, slots), s);
         launch_render_stage(Stage::Route, sp, params, n, cur, blocks(256, slots), s);
         mark(KClosest);
         if (learned) {
@@ -318,6 +327,7 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         mark(KSurface);
         launch_render_stage(Stage::Miss, sp, params, n, 0, blocks(256, slots), s);
         mark(KMiss);
+        if (splats) launch_render_stage(Stage::SplatShadow, sp, params, n, 0, blocks(128, slots), s);
         tv.shadow(params, n, blocks(tv.group, slots), cutouts, s);
         mark(KShadow);
     }

@@ -34,6 +34,7 @@ struct RenderJob {
     bool adaptive = false;  // a noise threshold is set: passes check convergence
     bool cutouts = false;   // some material has geometry opacity below one: shadow rays walk through
     bool programs = false;  // some material has a node-graph program: shading runs it
+    bool splats = false;    // the scene has Gaussian splats: the splat stages run
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);

@@ -18,6 +18,8 @@ enum class Stage {
     ShadeLight,
     ShadeProgram,
     ShadeLightProgram,
+    Splat,
+    SplatShadow,
     Miss,
     Finish,
     GuideFit,

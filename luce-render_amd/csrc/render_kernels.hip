@@ -15,6 +15,7 @@
 #include "render_bsdf.h"
 #include "render_guide.h"
 #include "render_program.h"
+#include "render_splats.h"
 
 namespace lr {
 namespace rt {
@@ -462,6 +463,67 @@ __global__ __launch_bounds__(256) void route_kernel(const Params *plist, uint32_
         staged.push(1, path, active && what != kNoneU && !light_path);
         staged.push(2, path, active && what != kNoneU && light_path);
         staged.sync(p, row_ends(plan, n, total, g, r));
+    }
+}
+
+// Gaussian splats on this bounce's segments (intersect_closest.comp:27-42,
+// 65-68), between the closest hits and the routing: a camera path gathers the
+// splats' radiance before its hit (or to infinity on a miss) and keeps their
+// transmittance of its throughput; a light path is only attenuated. A path
+// the splats stop (Russian roulette) becomes a miss with no throughput left.
+template <bool SPECTRAL>
+__global__ __launch_bounds__(128) void splat_kernel(const Params *plist, uint32_t n, uint32_t current) {
+    __shared__ uint32_t stack[kSplatStack * 128];
+    __shared__ RowPlan plan;
+    const uint32_t queue = Q_CLOSEST + current;
+    uint32_t total = plan.make(plist, n, [queue](const Params &q) { return q.queues[queue]; });
+    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
+        if (!active) continue;
+        const Params &p = plist[r];
+        if (!splats_present(p)) continue;
+        State st(p);
+        const PassState &ps = *p.pass;
+        uint32_t path = queue_path(p, queue, at);
+        uint32_t what = st.u(SU_HIT, path);
+        float reach = what == kNoneU ? kInf : st.v(SV_HIT, path).x;
+        V3 o = v3(st.v(SV_ORIGIN, path)), d = v3(st.v(SV_DIRECTION, path));
+        uint32_t bounces = st.u(SU_BOUNCE, path);
+        uint32_t seed = splat_seed(p, ps.sample, path, bounces, 0u);
+        Spec throughput = st.v(SV_THROUGHPUT, path);
+        float kept;
+        if (path >= ps.band_count) {
+            kept = splat_transmittance(p, o, d, 0.0f, reach, seed, stack + threadIdx.x, 128u);
+        } else {
+            Spec added;
+            splat_composite<SPECTRAL>(p, o, d, reach, st.v(SV_LAMBDA, path), seed, added, kept, stack + threadIdx.x,
+                                      128u);
+            st.set_v(SV_RADIANCE, path,
+                     st.v(SV_RADIANCE, path) + clamped<SPECTRAL>(p, throughput * added, bounce_total(bounces)));
+        }
+        if (kept == 1.0f) continue;
+        st.set_v(SV_THROUGHPUT, path, throughput * kept);
+        if (!(kept > 0.0f)) st.set_u(SU_HIT, path, kNoneU);
+    }
+}
+
+// The splats' transmittance of this bounce's shadow rays
+// (intersect_shadow.comp:61-67), before the shadow kernel walks them: it
+// scales what they deliver, in any order with the cut-outs' share.
+__global__ __launch_bounds__(128) void splat_shadow_kernel(const Params *plist, uint32_t n) {
+    __shared__ uint32_t stack[kSplatStack * 128];
+    __shared__ RowPlan plan;
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[Q_SHADOW]; });
+    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
+        if (!active) continue;
+        const Params &p = plist[r];
+        if (!splats_present(p)) continue;
+        State st(p);
+        uint32_t path = queue_path(p, Q_SHADOW, at);
+        float4 o = st.v(SV_SHADOW_ORIGIN, path);
+        V3 d = v3(st.v(SV_SHADOW_DIRECTION, path));
+        uint32_t seed = splat_seed(p, p.pass->sample, path, st.u(SU_BOUNCE, path), 0x68e31da4u);
+        float kept = splat_transmittance(p, v3(o), d, 0.0f, o.w, seed, stack + threadIdx.x, 128u);
+        if (kept != 1.0f) st.set_v(SV_SHADOW_RADIANCE, path, st.v(SV_SHADOW_RADIANCE, path) * kept);
     }
 }
 
@@ -1408,6 +1470,8 @@ static void launch_stage(Stage stage, const rt::Params *p, uint32_t n, uint32_t 
         case Stage::ShadeProgram: LR_STAGE((rt::shade_kernel<SPECTRAL, false, true>), 128, p, n, arg); break;
         case Stage::ShadeLightProgram: LR_STAGE((rt::shade_kernel<SPECTRAL, true, true>), 128, p, n, arg); break;
         case Stage::Route: LR_STAGE(rt::route_kernel, 256, p, n, arg); break;
+        case Stage::Splat: LR_STAGE((rt::splat_kernel<SPECTRAL>), 128, p, n, arg); break;
+        case Stage::SplatShadow: LR_STAGE(rt::splat_shadow_kernel, 128, p, n); break;
         case Stage::Miss: LR_STAGE((rt::miss_kernel<SPECTRAL>), 256, p, n); break;
         case Stage::Finish: LR_STAGE((rt::finish_kernel<SPECTRAL>), 256, p, n); break;
         case Stage::Convert: LR_STAGE((rt::film_convert_kernel<SPECTRAL>), 256, p, n, arg); break;

@@ -35,7 +35,9 @@ class SceneArrays(ctypes.Structure):
                 ("indices", ctypes.c_void_p), ("faces", ctypes.c_void_p),
                 ("shading", ctypes.c_void_p), ("uvs", ctypes.c_void_p),
                 ("triangles", ctypes.c_size_t), ("nodes", ctypes.c_void_p),
-                ("node_count", ctypes.c_size_t)]
+                ("node_count", ctypes.c_size_t), ("splat_records", ctypes.c_void_p), ("splats", ctypes.c_size_t),
+                ("splat_harmonics", ctypes.c_void_p), ("splat_words", ctypes.c_size_t),
+                ("splat_nodes", ctypes.c_void_p), ("splat_node_count", ctypes.c_size_t)]
 
 
 class TraceConfig(ctypes.Structure):
@@ -91,6 +93,7 @@ _SIGNATURES = {
     "lr_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     "lr_bvh_build": (ctypes.c_int, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.c_int, _P,
                                     ctypes.POINTER(ctypes.c_size_t), _P]),
+    "lr_bvh_build_boxes": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_size_t), _P]),
     "lr_bvh_build_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P,
                                            ctypes.POINTER(ctypes.c_size_t), _P]),
     "lr_scene_upload": (ctypes.c_int, [_P, ctypes.POINTER(SceneArrays), ctypes.POINTER(_P)]),

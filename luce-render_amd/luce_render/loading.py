@@ -509,6 +509,11 @@ class Loaded:
         for r in range(3):
             rows[72 + r * 4:72 + r * 4 + 3] = basis[r]
         rows[64:72] = dome_row
+        # The splats (loading.lucb:112-118): their count and alpha cull
+        # (K_SPLAT_TREE z, w); the device finds their arrays in the scene.
+        splats = getattr(scene, "splats", None)
+        rows[106] = float(len(splats.records)) if splats is not None and splats.nodes is not None else 0.0
+        rows[107] = 0.00392156862745098
         used = set(int(m) for m in np.unique(scene.shading[:, 3])) if scene.triangle_count() else set()
         # Node graphs: every material's program in one word array, each at
         # its entry (-1: none); their images in one texel array with a table

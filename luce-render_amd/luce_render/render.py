@@ -64,6 +64,11 @@ class GpuScene:
         if with_nodes:
             arrays.nodes = _lib.ptr(scene.nodes)
             arrays.node_count = len(scene.nodes)
+        sp = getattr(scene, "splats", None)
+        if sp is not None and len(sp.records) and sp.nodes is not None:
+            arrays.splat_records, arrays.splats = _lib.ptr(sp.records), len(sp.records)
+            arrays.splat_harmonics, arrays.splat_words = _lib.ptr(sp.harmonics), len(sp.harmonics)
+            arrays.splat_nodes, arrays.splat_node_count = _lib.ptr(sp.nodes), len(sp.nodes)
         return arrays
 
     @staticmethod

@@ -18,6 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import INVALID, LIMIT, STOPPED, LuceError
+from .splats import CompiledSplats
 
 IDENTITY = np.eye(4)
 
@@ -81,6 +82,26 @@ class Light:
 
 
 @dataclass
+class SplatCloud:
+    """A point cloud with luce-geocore's splat attributes (splats.lucb:5-13):
+    points (N,3) about `origin`; orient (N,4) xyzw, scale (N,3) and opacity
+    (N,) make it a splat cloud (without them it is a plain point cloud, not
+    rendered); colour Cd (N,3) linear (0.5 when absent); `sh` (N, items, 3)
+    with items 0, 3, 8 or 15 (degrees 0 to 3); optional restorient (N,4), the
+    orientation the SH were authored at; `color_space` "linear" or "srgb",
+    the encoding the DC colour and SH are authored in."""
+    points: np.ndarray
+    orient: Optional[np.ndarray] = None
+    scale: Optional[np.ndarray] = None
+    opacity: Optional[np.ndarray] = None
+    color: Optional[np.ndarray] = None
+    sh: Optional[np.ndarray] = None
+    restorient: Optional[np.ndarray] = None
+    color_space: str = "linear"
+    origin: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
 class Instance:
     set: "GeometrySet"
     matrix: np.ndarray
@@ -94,6 +115,7 @@ class GeometrySet:
     cameras: List[Camera] = field(default_factory=list)
     lights: List[Light] = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # name -> material record
+    clouds: List[SplatCloud] = field(default_factory=list)
 
 
 class Stopping:
@@ -124,6 +146,7 @@ class CompiledScene:
         self.materials = []      # material records in index order (index + 1 on triangles)
         self.material_names = []
         self.skipped = 0
+        self.splats = CompiledSplats()
 
     def triangle_count(self):
         return len(self.indices)
@@ -183,9 +206,10 @@ def smooth_normals(points, triangles):
 
 
 class _Builder:
-    def __init__(self, scene, stopping):
+    def __init__(self, scene, stopping, splats=True):
         self.scene = scene
         self.stopping = stopping
+        self.splats = splats
         self.positions = []
         self.indices = []
         self.faces = []
@@ -205,6 +229,10 @@ class _Builder:
         for mesh in gset.meshes:
             self.check_stop()
             self.add_mesh(mesh, matrix)
+        if self.splats:
+            for cloud in gset.clouds:
+                self.check_stop()
+                self.scene.splats.add_cloud(cloud, matrix)
         for inst in gset.instances:
             self.check_stop()
             if not inst.visible:
@@ -293,10 +321,12 @@ def _flatten_materials(gset, scene, depth=0):
 
 def compile_scene(gset, geometry=True, stopping=None, threads=0):
     """compile_scene (scene.lucb:111-129): the set's triangles in world space
-    with their BVH, and its cameras, lights and materials."""
+    with their BVH, its splats with theirs, and its cameras, lights and
+    materials. With geometry=False: triangles and materials, no BVH, no
+    splats."""
     scene = CompiledScene()
     _flatten_materials(gset, scene)
-    b = _Builder(scene, stopping)
+    b = _Builder(scene, stopping, splats=geometry)
     b.add_set(gset, IDENTITY, 0)
     if b.points > 4294967295:
         raise LuceError(LIMIT, "a scene holds at most 2^32 points")
@@ -311,6 +341,7 @@ def compile_scene(gset, geometry=True, stopping=None, threads=0):
         b.check_stop()
         build_bvh(scene, threads=threads)
         b.check_stop()
+        scene.splats.finish(threads=threads)
     return scene
 
 

@@ -21,7 +21,7 @@ import math
 
 import numpy as np
 
-from .scene import Camera, GeometrySet, Instance, Light, Mesh
+from .scene import Camera, GeometrySet, Instance, Light, Mesh, SplatCloud
 
 MASK = (1 << 64) - 1
 
@@ -557,6 +557,107 @@ def viewed_graph_set(material, frequency=16):
     gset.cameras.append(Camera(placement(rotation=_rotation_x(-0.1), translation=(0.0, 0.6, 3.0)), "perspective", 50.0,
                                36.0))
     return gset
+
+
+def _about(axis, angle):
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    return np.array([*(a * math.sin(0.5 * angle)), math.cos(0.5 * angle)])
+
+
+def splat_description(centers, scales, opacities, colors, orients=None, sh=None):
+    """Splats as a dict of arrays (centre, orient xyzw, scale, opacity, linear
+    colour, SH items (n, items, 3))."""
+    n = len(centers)
+    d = {"center": np.asarray(centers, np.float64).reshape(n, 3),
+         "orient": np.tile([0.0, 0.0, 0.0, 1.0], (n, 1)) if orients is None else np.asarray(orients, np.float64),
+         "scale": np.asarray(scales, np.float64).reshape(n, 3),
+         "opacity": np.asarray(opacities, np.float64).reshape(n),
+         "color": np.asarray(colors, np.float64).reshape(n, 3)}
+    if sh is not None:
+        d["sh"] = np.asarray(sh, np.float64).reshape(n, -1, 3)
+    return d
+
+
+def splat_cloud(splats, color_space="linear"):
+    return SplatCloud(points=splats["center"], orient=splats["orient"], scale=splats["scale"],
+                      opacity=splats["opacity"], color=splats["color"], sh=splats.get("sh"), color_space=color_space)
+
+
+def splat_primary_splats():
+    """The reference's six overlapping, rotated, anisotropic splats with SH
+    degree 1 (tests/gpu/splats.lucb:146-159): one hides behind the quad."""
+    sh = np.zeros((6, 3, 3))
+    sh[0] = np.array([0.1, 0.0, -0.05, 0.0, 0.08, 0.0, -0.1, 0.0, 0.1]).reshape(3, 3)
+    return splat_description(
+        centers=[(-0.4, 0.1, 0.0), (0.3, -0.1, -0.5), (0.0, 0.0, 0.6), (0.9, 0.6, 0.2), (-0.8, -0.7, -1.0),
+                 (0.5, -0.5, -2.6)],
+        scales=[(0.5, 0.25, 0.3), (0.35, 0.6, 0.2), (0.2, 0.2, 0.2), (0.3, 0.1, 0.3), (0.4, 0.4, 0.4), (0.5, 0.5, 0.5)],
+        opacities=[0.8, 0.7, 0.5, 0.99, 0.35, 0.95],
+        colors=[(0.9, 0.2, 0.1), (0.1, 0.8, 0.2), (0.2, 0.3, 1.0), (1.0, 1.0, 0.3), (0.6, 0.6, 0.6), (1.0, 0.0, 1.0)],
+        orients=[_about((0.0, 0.0, 1.0), 0.6), _about((0.2822, 0.9406, 0.1881), 0.9), [0.0, 0.0, 0.0, 1.0],
+                 _about((1.0, 0.0, 0.0), 1.1), [0.0, 0.0, 0.0, 1.0], [0.0, 0.0, 0.0, 1.0]],
+        sh=sh)
+
+
+SPLAT_QUAD_Z = -2.0
+SPLAT_QUAD_HALF = 1.5
+SPLAT_QUAD_EMISSION = (0.2, 0.4, 0.9)
+
+
+def splat_primary_set(splats=None):
+    """tests/gpu/splats.lucb:143-214: the six splats in front of an emissive
+    quad (z = -2, 3 x 3, emission (0.2, 0.4, 0.9), no reflection) seen from
+    (0, 0, 5) through a 50 mm lens."""
+    splats = splat_primary_splats() if splats is None else splats
+    h, z = SPLAT_QUAD_HALF, SPLAT_QUAD_Z
+    quad = quad_mesh([[-h, -h, z], [h, -h, z], [h, h, z], [-h, h, z]], [0, 0, 1], "quad")
+    gset = GeometrySet(meshes=[quad], clouds=[splat_cloud(splats)])
+    gset.materials = {"quad": {"base_weight": 0.0, "specular_weight": 0.0, "emission_luminance": 1.0,
+                               "emission_color": SPLAT_QUAD_EMISSION}}
+    gset.cameras.append(Camera(placement(translation=(0.0, 0.0, 5.0)), "perspective", 50.0, 36.0))
+    return gset
+
+
+def splat_shadow_set(light_tracing_view=False):
+    """tests/gpu/splats.lucb:226-288: a white floor (8 x 8, albedo 0.8, no
+    specular) under the sun straight down (radiance 3, 0.01 degrees) with a
+    black splat of opacity 1 and sigma 0.25 one unit above it, seen from
+    (0, 2, 4) pitched down 0.6 rad through a 35 mm lens."""
+    splats = splat_description(centers=[(0.0, 1.0, 0.0)], scales=[(0.25, 0.25, 0.25)], opacities=[1.0],
+                               colors=[(0.0, 0.0, 0.0)])
+    gset = GeometrySet(meshes=[grid_mesh(size=8.0, divisions=8, material="floor")], clouds=[splat_cloud(splats)])
+    gset.materials = {"floor": {"base_color": (0.8, 0.8, 0.8), "specular_weight": 0.0}}
+    gset.lights.append(Light("distant", placement(rotation=_rotation_x(-0.5 * math.pi)), (1.0, 1.0, 1.0), 3.0,
+                             angle=0.01))
+    gset.cameras.append(Camera(placement(rotation=_rotation_x(-0.6), translation=(0.0, 2.0, 4.0)), "perspective",
+                               35.0, 36.0))
+    return gset, splats
+
+
+def splat_ball(count=3000, seed=17):
+    """Random splats in a unit ball, many overlapping, a few large: random
+    orientations, scales 0.02..0.12 (one in fifty 0.3..0.45), opacity
+    0.2..0.9, colours in 0..1 and small degree-1 SH."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(count, 3))
+    centers = u / np.linalg.norm(u, axis=1, keepdims=True) * rng.random((count, 1)) ** (1.0 / 3.0)
+    scales = rng.uniform(0.02, 0.12, (count, 3))
+    big = rng.random(count) < 0.02
+    scales[big] = rng.uniform(0.3, 0.45, (int(big.sum()), 3))
+    q = rng.normal(size=(count, 4))
+    return splat_description(centers, scales, rng.uniform(0.2, 0.9, count), rng.random((count, 3)),
+                             orients=q / np.linalg.norm(q, axis=1, keepdims=True),
+                             sh=rng.normal(0.0, 0.1, (count, 3, 3)))
+
+
+def splat_ball_set(splats=None):
+    """A ball of splats and no triangles, against a (0.1, 0.2, 0.3)
+    background, seen from (0, 0, 4) through a 50 mm lens."""
+    splats = splat_ball() if splats is None else splats
+    gset = GeometrySet(clouds=[splat_cloud(splats)])
+    gset.cameras.append(Camera(placement(translation=(0.0, 0.0, 4.0)), "perspective", 50.0, 36.0))
+    return gset, splats
 
 
 def many_lights_set(count=40, seed=5):
