@@ -32,7 +32,7 @@ int hip_fail(int status, const char *what) {
 using namespace lr;
 
 int lr_gpu_scene::ensure_spill(uint64_t threads, uint32_t stack) {
-    uint32_t bound = info.stack_bound[config_width_index()];
+    uint32_t bound = info.stack_bound[bound_index(config.width)];
     uint64_t depth = bound > stack ? bound - stack : 0;
     uint64_t need = depth * threads;
     if (need <= spill_entries) return 0;
@@ -45,7 +45,7 @@ int lr_gpu_scene::ensure_spill(uint64_t threads, uint32_t stack) {
     return 0;
 }
 
-int lr_gpu_scene::config_width_index() const { return config.width == 2 ? 0 : config.width == 4 ? 1 : 2; }
+int lr_gpu_scene::config_width_index() const { return layout_index(config.width); }
 
 lr::DeviceScene lr_gpu_scene::device_scene() const {
     DeviceScene sc{};
@@ -70,7 +70,7 @@ const lr::TraceVariant &lr_gpu_scene::variant() const {
 }
 
 lr_gpu_scene::~lr_gpu_scene() {
-    for (void *p : {wide[0], wide[1], wide[2], tris, faces, shading, uvs, spill, (void *)work, ranges, (void *)binary,
+    for (void *p : {wide[0], wide[1], wide[2], wide[3], wide[4], tris, faces, shading, uvs, spill, (void *)work, ranges, (void *)binary,
                     (void *)indices, (void *)parent, (void *)arrive, (void *)slots[0], (void *)slots[1], (void *)slots[2],
                     splat_records, splat_words, splat_nodes})
         if (p) (void)hipFree(p);
@@ -262,6 +262,10 @@ int finish_scene(lr_gpu_scene *s, const float *positions, uint32_t *indices, lr_
         s->info.wide_bytes[wi] = layout.bytes;
         s->info.stack_bound[wi] = layout.stack_bound;
     }
+    int qrc;
+    if ((qrc = quantize_wide_device(device, s->wide[1], s->info.wide_nodes[1], 4, &s->wide[3])) ||
+        (qrc = quantize_wide_device(device, s->wide[2], s->info.wide_nodes[2], 8, &s->wide[4])))
+        return qrc;
     s->info.triangle_bytes = n * 48;
     if ((e = hipMalloc((void **)&s->work, 64)) != hipSuccess || (e = hipMemsetAsync(s->work, 0, 64, st)) != hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
@@ -387,9 +391,10 @@ int lr_scene_refit(lr_gpu_scene *scene, const float *positions, size_t points) {
 
 int lr_scene_read_layout(const lr_gpu_scene *scene, int width, void *out, size_t capacity, size_t *bytes) {
     if (!scene || !bytes) return fail(LR_INVALID, "lr_scene_read_layout needs a scene and a size");
-    int wi = width == 2 ? 0 : width == 4 ? 1 : width == 8 ? 2 : -1;
-    if (wi < 0) return fail(LR_INVALID, "widths are 2, 4, 8");
-    *bytes = scene->info.wide_bytes[wi];
+    int wi = layout_index((uint32_t)width);
+    if (wi < 0) return fail(LR_INVALID, "layouts are 2, 4, 8 and the quantized 20, 24");
+    *bytes = wi < 3 ? scene->info.wide_bytes[wi]
+                    : std::max<size_t>(scene->info.wide_nodes[wi - 2], 1) * (wi == 3 ? 64 : 128);
     if (!out) return 0;
     if (capacity < *bytes) return fail(LR_INVALID, "the buffer is smaller than the layout");
     (void)hipSetDevice(scene->device->ordinal);
@@ -420,7 +425,7 @@ void lr_scene_destroy(lr_gpu_scene *scene) {
 int lr_scene_set_trace_config(lr_gpu_scene *scene, const lr_trace_config *config) {
     if (!scene || !config) return fail(LR_INVALID, "lr_scene_set_trace_config needs a scene and a config");
     if (variant_index(config->width, config->stack, config->group) < 0)
-        return fail(LR_INVALID, "traversal widths are 2, 4, 8; stacks 24, 32, 48; groups 64, 128, 256");
+        return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 24, 32, 48; groups 64, 128, 256");
     if (config->refill > 64) return fail(LR_INVALID, "refill is 0 (one ray a thread) or 1..64 idle lanes");
     scene->config = *config;
     return 0;

@@ -362,3 +362,102 @@ void gather_rows_device(lr_device *dev, const uint32_t *src, const uint32_t *ord
                        order, (uint32_t)n, words, dst);
 }
 }  // namespace lr
+
+namespace lr {
+namespace gwide {
+
+// The quantized layout of a width-WD float layout (trace_device.h
+// QuantNodeView), one thread a node. Per axis the grid starts at the node's
+// low corner (the union of its children's boxes) with the smallest power-of-
+// two step whose 255th line reaches the high corner; each child plane is
+// rounded outward and then moved outward until origin + q * step, computed in
+// f32 as the traversal decodes it, lies outside the float plane.
+template <int WD> __global__ __launch_bounds__(256) void quantize_kernel(const uint32_t *src, uint32_t count, uint32_t *dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    constexpr int kSrc = WD == 4 ? 32 : 64;  // words a float node
+    constexpr int kDst = WD == 4 ? 16 : 32;  // words a quantized node
+    const uint32_t *n = src + (size_t)i * kSrc;
+    uint32_t out[kDst];
+#pragma unroll
+    for (int k = 0; k < kDst; k++) out[k] = 0u;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < WD; c++) {
+        if (n[6 * WD + c] == kNone) continue;
+        any = true;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], __uint_as_float(n[2 * a * WD + c]));
+            hi[a] = fmaxf(hi[a], __uint_as_float(n[(2 * a + 1) * WD + c]));
+        }
+    }
+    float origin[3] = {0.0f, 0.0f, 0.0f}, step[3] = {1.0f, 1.0f, 1.0f};
+    uint32_t exps = 0u;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int e = 127;
+        if (any) {
+            origin[a] = lo[a];
+            float extent = hi[a] - origin[a];
+            e = 1;
+            if (extent > 0.0f) {
+                int e2 = 0;
+                frexpf(extent / 255.0f, &e2);
+                e = min(max(e2 + 126, 1), 254);
+            }
+            while (e < 254 && origin[a] + 255.0f * __uint_as_float((uint32_t)e << 23) < hi[a]) e++;
+        }
+        step[a] = __uint_as_float((uint32_t)e << 23);
+        exps |= (uint32_t)e << (8 * a);
+        out[a] = __float_as_uint(origin[a]);
+    }
+    out[3] = exps;
+    auto put_byte = [&](int at, uint32_t v) { out[at / 4] |= v << (8 * (at % 4)); };
+#pragma unroll
+    for (int c = 0; c < WD; c++) {
+        uint32_t child = n[6 * WD + c];
+        out[4 + 6 * WD / 4 + c] = child;
+        put_byte(16 + 6 * WD + 4 * WD + c, (n[7 * WD + c / 4] >> (8 * (c % 4))) & 0xffu);
+        if (child == kNone) continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float l = __uint_as_float(n[2 * a * WD + c]), h = __uint_as_float(n[(2 * a + 1) * WD + c]);
+            float ql = fminf(fmaxf(floorf((l - origin[a]) / step[a]), 0.0f), 255.0f);
+            float qh = fminf(fmaxf(ceilf((h - origin[a]) / step[a]), 0.0f), 255.0f);
+            uint32_t q = (uint32_t)ql, r = (uint32_t)qh;
+            while (q > 0u && origin[a] + (float)q * step[a] > l) q--;
+            while (r < 255u && origin[a] + (float)r * step[a] < h) r++;
+            put_byte(16 + 2 * a * WD + c, q);
+            put_byte(16 + (2 * a + 1) * WD + c, r);
+        }
+    }
+    uint32_t *d = dst + (size_t)i * kDst;
+#pragma unroll
+    for (int k = 0; k < kDst; k += 4) *reinterpret_cast<uint4 *>(d + k) = make_uint4(out[k], out[k + 1], out[k + 2], out[k + 3]);
+}
+
+}  // namespace gwide
+
+int quantize_wide_device(lr_device *dev, const void *src, size_t count, int width, void **dst) {
+    size_t node_bytes = width == 4 ? 64 : 128;
+    size_t bytes = std::max<size_t>(count, 1) * node_bytes;
+    hipError_t e;
+    if (!*dst && (e = hipMalloc(dst, bytes)) != hipSuccess) return hip_fail(e, "hipMalloc(quantized nodes)");
+    if (count == 0) {
+        if ((e = hipMemsetAsync(*dst, 0, bytes, dev->stream)) != hipSuccess) return hip_fail(e, "hipMemset(quantized nodes)");
+        return 0;
+    }
+    dim3 grid((uint32_t)((count + 255) / 256));
+    if (width == 4)
+        hipLaunchKernelGGL(gwide::quantize_kernel<4>, grid, dim3(256), 0, dev->stream, static_cast<const uint32_t *>(src),
+                           (uint32_t)count, static_cast<uint32_t *>(*dst));
+    else
+        hipLaunchKernelGGL(gwide::quantize_kernel<8>, grid, dim3(256), 0, dev->stream, static_cast<const uint32_t *>(src),
+                           (uint32_t)count, static_cast<uint32_t *>(*dst));
+    e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "quantize wide nodes");
+}
+
+}  // namespace lr

@@ -156,6 +156,11 @@ int refit_scene_device(lr_gpu_scene *s, const float *positions) {
         if (wi == 1) hipLaunchKernelGGL(grefit::slots_kernel<4>, g, dim3(256), 0, st, s->binary, s->slots[1], w, out, node_bytes);
         if (wi == 2) hipLaunchKernelGGL(grefit::slots_kernel<8>, g, dim3(256), 0, st, s->binary, s->slots[2], w, out, node_bytes);
     }
+    // The quantized layouts follow their float layouts.
+    int rc;
+    if ((rc = quantize_wide_device(dev, s->wide[1], s->info.wide_nodes[1], 4, &s->wide[3])) ||
+        (rc = quantize_wide_device(dev, s->wide[2], s->info.wide_nodes[2], 8, &s->wide[4])))
+        return rc;
     pack_triangles_device(dev, positions, s->indices, s->info.triangles, static_cast<float4 *>(s->tris));
     lr_bvh_node root;
     if ((e = hipMemcpyAsync(&root, s->binary, sizeof(root), hipMemcpyDeviceToHost, st)) != hipSuccess ||

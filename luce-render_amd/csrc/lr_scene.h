@@ -14,7 +14,9 @@ struct lr_device {
 
 struct lr_gpu_scene {
     lr_device *device = nullptr;
-    void *wide[3] = {nullptr, nullptr, nullptr};  // widths 2, 4, 8
+    // Layouts: widths 2, 4, 8 with float planes, then widths 4 and 8
+    // quantized to bytes (traversal layout codes 2, 4, 8, 20, 24).
+    void *wide[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void *tris = nullptr;                         // 48 B per leaf slot
     void *faces = nullptr;
     void *shading = nullptr;
@@ -62,6 +64,18 @@ int build_bvh_device(lr_device *dev, const float *positions, const uint32_t *ind
 // Device collapse (bvh_collapse_gpu.hip) of the binary tree in device memory.
 int collapse_wide_device(lr_device *dev, const lr_bvh_node *nodes, size_t node_count, bool empty, int width,
                          DeviceLayout &layout, bool with_slots = false);
+// The quantized layout (64 B a node at width 4, 128 B at 8) of a float
+// layout of `count` nodes in device memory; allocates *dst when null.
+int quantize_wide_device(lr_device *dev, const void *src, size_t count, int width, void **dst);
+// Layout index of a traversal layout code (2, 4, 8, 20, 24), and the index of
+// its stack bound (the width's), or -1.
+inline int layout_index(uint32_t code) {
+    return code == 2 ? 0 : code == 4 ? 1 : code == 8 ? 2 : code == 20 ? 3 : code == 24 ? 4 : -1;
+}
+inline int bound_index(uint32_t code) {
+    int l = layout_index(code);
+    return l < 0 ? -1 : (l < 3 ? l : l - 2);
+}
 // Refit (bvh_refit_gpu.hip): new points in device memory, same triangles.
 int refit_scene_device(lr_gpu_scene *scene, const float *positions);
 // Leaf-order triangles (p0, e1, e2 as 3 float4) from device points and indices.

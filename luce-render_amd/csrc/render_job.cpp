@@ -36,7 +36,7 @@ Shape shape_of(const lr_loaded *loaded, const lr_render_settings &st) {
     return s;
 }
 
-int width_index(uint32_t w) { return w == 2 ? 0 : w == 4 ? 1 : 2; }
+int width_index(uint32_t w) { return bound_index(w); }
 
 }  // namespace
 
@@ -237,7 +237,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     // The traversal's spill area for the largest grid a pass launches, and
     // the node layout of the chosen width.
     p.scene = sc->device_scene();
-    p.scene.nodes = (const uint8_t *)sc->wide[width_index(w)];
+    p.scene.nodes = (const uint8_t *)sc->wide[layout_index(w)];
     uint32_t bound = sc->info.stack_bound[width_index(w)];
     uint64_t depth = bound > stack ? bound - stack : 0;
     uint64_t threads = (uint64_t)grid_blocks(slots, group) * group;

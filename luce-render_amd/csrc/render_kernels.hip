@@ -1447,12 +1447,14 @@ template <int W> void fill_trace_s(RenderTraceVariant *t, int wi) {
 }  // namespace
 
 const RenderTraceVariant *render_trace_variants() {
-    static RenderTraceVariant table[27];
+    static RenderTraceVariant table[45];
     static bool ready = false;
     if (!ready) {
         fill_trace_s<2>(table, 0);
         fill_trace_s<4>(table, 1);
         fill_trace_s<8>(table, 2);
+        fill_trace_s<kQuantizedLayout + 4>(table, 3);
+        fill_trace_s<kQuantizedLayout + 8>(table, 4);
         ready = true;
     }
     return table;

@@ -113,6 +113,14 @@ __device__ __forceinline__ bool triangle_test(float4 a, float4 b, float4 c, floa
     return !reject;
 }
 
+// Layout codes: 2, 4, 8 the float layouts of those widths; kQuantizedLayout
+// + 4 and + 8 the 8-bit quantized layouts (a node's child boxes as bytes on a
+// per-axis power-of-two grid from the node's own low corner, rounded outward:
+// 64 and 128 bytes a node instead of 128 and 256).
+constexpr int kQuantizedLayout = 16;
+template <int W> constexpr int kWidthOf = W & 15;
+template <int W> constexpr bool kQuantizedOf = (W & kQuantizedLayout) != 0;
+
 template <int W> struct NodeView {
     static constexpr int kQuads = (W == 2) ? 4 : (W == 4 ? 8 : 16);
     float4 raw[kQuads];
@@ -138,6 +146,50 @@ template <int W> struct NodeView {
     __device__ __forceinline__ uint32_t child(int c) const { return u(6 * W + c); }
     __device__ __forceinline__ uint32_t count(int c) const { return (u(7 * W + c / 4) >> (8 * (c % 4))) & 0xffu; }
 };
+
+// A quantized node (width WD): origin xyz and three biased exponent bytes
+// (16 B), then the six planes' bytes per child (lo x, hi x, lo y, hi y, lo z,
+// hi z; WD each), the WD child words and WD count bytes. A plane decodes to
+// origin + q * 2^(e - 127), computed in f32 exactly as the quantizer checked
+// it, so every decoded box holds the child's float box.
+template <int WD> struct QuantNodeView {
+    static constexpr int kQuads = WD == 4 ? 4 : 8;
+    float4 raw[kQuads];
+    float ox, oy, oz, sx, sy, sz;
+    __device__ __forceinline__ void load(const uint8_t *nodes, uint32_t index) {
+        const float4 *p = reinterpret_cast<const float4 *>(nodes + (size_t)index * (kQuads * 16));
+#pragma unroll
+        for (int q = 0; q < kQuads; q++) raw[q] = p[q];
+        ox = raw[0].x, oy = raw[0].y, oz = raw[0].z;
+        uint32_t e = __float_as_uint(raw[0].w);
+        sx = __uint_as_float((e & 0xffu) << 23);
+        sy = __uint_as_float(((e >> 8) & 0xffu) << 23);
+        sz = __uint_as_float(((e >> 16) & 0xffu) << 23);
+    }
+    __device__ __forceinline__ uint32_t u(int i) const {
+        const uint32_t *x = reinterpret_cast<const uint32_t *>(raw);
+        return x[i];
+    }
+    __device__ __forceinline__ float byte(int i) const { return (float)((u(4 + i / 4) >> (8 * (i % 4))) & 0xffu); }
+    __device__ __forceinline__ float lo_x(int c) const { return ox + byte(c) * sx; }
+    __device__ __forceinline__ float hi_x(int c) const { return ox + byte(WD + c) * sx; }
+    __device__ __forceinline__ float lo_y(int c) const { return oy + byte(2 * WD + c) * sy; }
+    __device__ __forceinline__ float hi_y(int c) const { return oy + byte(3 * WD + c) * sy; }
+    __device__ __forceinline__ float lo_z(int c) const { return oz + byte(4 * WD + c) * sz; }
+    __device__ __forceinline__ float hi_z(int c) const { return oz + byte(5 * WD + c) * sz; }
+    __device__ __forceinline__ uint32_t child(int c) const { return u(4 + 6 * WD / 4 + c); }
+    __device__ __forceinline__ uint32_t count(int c) const {
+        return (u(4 + 6 * WD / 4 + WD + c / 4) >> (8 * (c % 4))) & 0xffu;
+    }
+};
+
+template <int W, bool Q = kQuantizedOf<W>> struct NodeOfT {
+    using type = NodeView<W>;
+};
+template <int W> struct NodeOfT<W, true> {
+    using type = QuantNodeView<kWidthOf<W>>;
+};
+template <int W> using NodeOf = typename NodeOfT<W>::type;
 
 // The per-lane stack: S entries in LDS (lane-interleaved, `stride` apart),
 // the rest in the global spill area.
@@ -201,6 +253,7 @@ __device__ __forceinline__ bool binary_first(const DeviceScene &sc, uint32_t a, 
 // One ray's closest-hit walk (bvh.glsl:51-90) over width-W nodes, advanced one
 // node at a time, so a persistent wave can hand a finished lane a new ray.
 template <int W, int S> struct ClosestWalk {
+    static constexpr int WD = kWidthOf<W>;
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
     Hit hit;
     uint32_t cur;
@@ -236,21 +289,21 @@ template <int W, int S> struct ClosestWalk {
                 float t, u, v;
                 bool tie;
                 bool take = triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, hit.t, t, u, v, tie);
-                if (W > 2 && tie && hit.slot != kNone)
+                if (WD > 2 && tie && hit.slot != kNone)
                     take = binary_first(sc, slot, hit.slot, ox, oy, oz, ix, iy, iz, hit.t);
                 if (take) hit = Hit{t, slot, u, v};
                 if (slot == last) break;
                 a = na, b = nb, c = nc;
             }
         } else {
-            NodeView<W> n;
+            NodeOf<W> n;
             n.load(sc.nodes, cur);
-            float dist[W];
-            uint32_t ent[W];
-            int key[W];
+            float dist[WD];
+            uint32_t ent[WD];
+            int key[WD];
             int hits = 0;
 #pragma unroll
-            for (int c = 0; c < W; c++) {
+            for (int c = 0; c < WD; c++) {
                 uint32_t ch = n.child(c);
                 uint32_t cnt = n.count(c);
                 // every slot's planes are tested (an empty slot holds zeros)
@@ -266,9 +319,9 @@ template <int W, int S> struct ClosestWalk {
             if (hits > 0) {
                 // Sort by (entry distance, child index): a stable order.
 #pragma unroll
-                for (int i = 1; i < W; i++) {
+                for (int i = 1; i < WD; i++) {
 #pragma unroll
-                    for (int j = W - 1; j >= i; j--) {
+                    for (int j = WD - 1; j >= i; j--) {
                         bool sw = dist[j] < dist[j - 1] || (dist[j] == dist[j - 1] && key[j] < key[j - 1]);
                         float td = dist[j];
                         uint32_t te = ent[j];
@@ -282,7 +335,7 @@ template <int W, int S> struct ClosestWalk {
                     }
                 }
 #pragma unroll
-                for (int i = W - 1; i >= 1; i--)
+                for (int i = WD - 1; i >= 1; i--)
                     if (i < hits) stack.push(ent[i], stride);
                 cur = ent[0];
                 return true;
@@ -296,6 +349,7 @@ template <int W, int S> struct ClosestWalk {
 
 // One ray's any-hit walk (bvh.glsl:93-122), advanced one node at a time.
 template <int W, int S> struct AnyWalk {
+    static constexpr int WD = kWidthOf<W>;
     float ox, oy, oz, dx, dy, dz, ix, iy, iz, tmax;
     uint32_t cur;
     bool blocked;
@@ -333,13 +387,13 @@ template <int W, int S> struct AnyWalk {
                 a = na, b = nb, c = nc;
             }
         } else {
-            NodeView<W> n;
+            NodeOf<W> n;
             n.load(sc.nodes, cur);
-            uint32_t ent[W];
-            bool take[W];
+            uint32_t ent[WD];
+            bool take[WD];
             int hits = 0;
 #pragma unroll
-            for (int c = 0; c < W; c++) {
+            for (int c = 0; c < WD; c++) {
                 uint32_t ch = n.child(c);
                 uint32_t cnt = n.count(c);
                 float e = box_entry(n.lo_x(c), n.lo_y(c), n.lo_z(c), n.hi_x(c), n.hi_y(c), n.hi_z(c), ox, oy, oz, ix,
@@ -351,7 +405,7 @@ template <int W, int S> struct AnyWalk {
             if (hits > 0) {
                 uint32_t next = kNone;
 #pragma unroll
-                for (int c = W - 1; c >= 0; c--) {
+                for (int c = WD - 1; c >= 0; c--) {
                     if (!take[c]) continue;
                     if (next != kNone) stack.push(next, stride);
                     next = ent[c];

@@ -3,6 +3,8 @@
 // LDS stack depth S and workgroup size G so the autotuner can pick a variant.
 #include "trace_kernels.h"
 
+#include "lr_scene.h"
+
 // Kernels receive the scene by value; each launcher stamps the spill stride
 // (the threads it launches) before launching.
 
@@ -238,19 +240,21 @@ template <int W> void fill_s(TraceVariant *table, int wi) {
 }  // namespace
 
 const TraceVariant *trace_variants() {
-    static TraceVariant table[27];
+    static TraceVariant table[45];
     static bool ready = false;
     if (!ready) {
         fill_s<2>(table, 0);
         fill_s<4>(table, 1);
         fill_s<8>(table, 2);
+        fill_s<kQuantizedLayout + 4>(table, 3);
+        fill_s<kQuantizedLayout + 8>(table, 4);
         ready = true;
     }
     return table;
 }
 
 int variant_index(uint32_t width, uint32_t stack, uint32_t group) {
-    int wi = width == 2 ? 0 : width == 4 ? 1 : width == 8 ? 2 : -1;
+    int wi = layout_index(width);
     int si = stack == 24 ? 0 : stack == 32 ? 1 : stack == 48 ? 2 : -1;
     int gi = group == 64 ? 0 : group == 128 ? 1 : group == 256 ? 2 : -1;
     if (wi < 0 || si < 0 || gi < 0) return -1;

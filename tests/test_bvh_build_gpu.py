@@ -248,3 +248,58 @@ def test_refit_of_a_device_built_scene(device):
     want, _ = _host_layout(tree, 2)
     assert np.array_equal(g.layout(2), want)
     g.destroy()
+
+
+def _decode_quantized(raw, width):
+    """Child boxes (nodes, width, 2, 3) of a quantized layout, decoded in f32
+    as the traversal does, with the child words."""
+    node = 64 if width == 4 else 128
+    q = raw.reshape(-1, node)
+    words = q.view(np.uint32)
+    origin = words[:, 0:3].view(np.float32)
+    e = words[:, 3]
+    step = np.stack([((e >> (8 * a)) & 0xFF) << 23 for a in range(3)], 1).astype(np.uint32).view(np.float32)
+    planes = q[:, 16:16 + 6 * width].reshape(-1, 6, width).astype(np.float32)
+    lo = origin[:, None, :] + np.stack([planes[:, 0], planes[:, 2], planes[:, 4]], 2) * step[:, None, :]
+    hi = origin[:, None, :] + np.stack([planes[:, 1], planes[:, 3], planes[:, 5]], 2) * step[:, None, :]
+    child = words[:, 4 + 6 * width // 4:4 + 6 * width // 4 + width]
+    return lo.astype(np.float32), hi.astype(np.float32), child
+
+
+def _decode_float(raw, width):
+    node = 128 if width == 4 else 256
+    f = raw.reshape(-1, node).view(np.float32)
+    u = raw.reshape(-1, node).view(np.uint32)
+    lo = np.stack([f[:, 0:width], f[:, 2 * width:3 * width], f[:, 4 * width:5 * width]], 2)
+    hi = np.stack([f[:, width:2 * width], f[:, 3 * width:4 * width], f[:, 5 * width:6 * width]], 2)
+    return lo, hi, u[:, 6 * width:7 * width]
+
+
+def _check_quantized(g, tight=True):
+    for width, code in ((4, 20), (8, 24)):
+        flo, fhi, fch = _decode_float(g.layout(width), width)
+        qlo, qhi, qch = _decode_quantized(g.layout(code), width)
+        assert np.array_equal(fch, qch)
+        valid = fch != 0xFFFFFFFF
+        assert np.all(qlo[valid] <= flo[valid]) and np.all(qhi[valid] >= fhi[valid])
+        if tight:  # the grid is at most 255 steps of the node's extent: boxes grow by a step at most per side
+            ext = np.maximum(fhi.max(1) - flo.min(1), 1e-30)[:, None, :]
+            grow = np.maximum(flo - qlo, qhi - fhi)
+            assert np.all((grow / ext)[valid] <= 2.0 / 255.0 + 1e-6)
+
+
+def test_quantized_layouts_hold_the_float_boxes(device, cornell_probe):
+    """Layouts 20 and 24 (widths 4 and 8 on an 8-bit grid): the same children,
+    every decoded child box holding the float layout's and at most a grid step
+    larger on each side; after a refit too."""
+    from luce_render import render, scenes
+    g = render.GpuScene.upload(device, cornell_probe)
+    _check_quantized(g)
+    moved = (cornell_probe.positions * np.float32(1.3) + np.float32(0.2)).astype(np.float32)
+    g.refit(moved)
+    _check_quantized(g)
+    g.destroy()
+    flat = render.compile_scene(scenes.instanced_set(instances=20), geometry=False)
+    g = render.GpuScene.build(device, flat, keep_tree=False)
+    _check_quantized(g)
+    g.destroy()

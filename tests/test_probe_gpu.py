@@ -8,7 +8,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-WIDTHS = (2, 4, 8)
+WIDTHS = (2, 4, 8, 20, 24)  # 20, 24: the quantized 4- and 8-wide layouts
 REFILLS = (0, 1, 16, 64)  # 0: one ray per thread; k: persistent waves refilled at k idle lanes
 
 
