@@ -99,8 +99,10 @@ int lr_bvh_build_device(lr_device *device, const float *positions, size_t points
                         size_t triangles, lr_bvh_node *nodes, size_t *node_count, uint32_t *order);
 
 /* Replaces GpuScene.upload (src/render/device.lucb:41-89). The binary BVH is
- * repacked on upload into the traversal layouts (width 2, 4 and 8, 64/128/256
- * byte nodes) and a leaf-order triangle array (p0, e1, e2). */
+ * repacked on upload into the traversal layouts (width 2, 4 and 8 with float
+ * child boxes, 64/128/256 byte nodes; widths 4 and 8 with child boxes
+ * quantized to bytes, 64/128 byte nodes) and a leaf-order triangle array
+ * (p0, e1, e2). */
 int lr_scene_upload(lr_device *device, const lr_scene_arrays *arrays, lr_gpu_scene **out);
 /* compile_scene's build_bvh and reorder (bvh.lucb:66-217) followed by
  * GpuScene.upload (device.lucb:41-89), all on the device: `arrays` in the
@@ -139,7 +141,9 @@ void lr_scene_destroy(lr_gpu_scene *scene);
 int lr_probe(lr_device *device, const lr_gpu_scene *scene, const float frame[16], uint32_t width,
              uint32_t height, uint32_t *face_out, float *t_out, size_t count);
 
-/* Traversal configuration: BVH width (2, 4, 8), workgroup size (64, 128, 256),
+/* Traversal configuration: the node layout (width 2, 4, 8 with float child
+ * boxes; LR_QUANTIZED | 4 and LR_QUANTIZED | 8 with 8-bit child boxes, each
+ * holding its float box), workgroup size (64, 128, 256),
  * LDS stack entries per lane (24, 32, 48) and the ray schedule of the ray-batch
  * launches: refill 0 gives each thread one ray; refill k (1..64) runs
  * persistent waves that take new rays from a shared counter once k of their
@@ -147,6 +151,7 @@ int lr_probe(lr_device *device, const lr_gpu_scene *scene, const float frame[16]
  * counterpart: the reference fixes width 2, 64-wide groups, a 32-entry stack
  * (bvh.glsl:49) and one ray per invocation. Launches on one scene are ordered
  * (one stream at a time): they share its spill area and ray counter. */
+#define LR_QUANTIZED 16
 typedef struct lr_trace_config {
     uint32_t width;
     uint32_t group;

@@ -270,9 +270,10 @@ int finish_scene(lr_gpu_scene *s, const float *positions, uint32_t *indices, lr_
     if ((e = hipMalloc((void **)&s->work, 64)) != hipSuccess || (e = hipMemsetAsync(s->work, 0, 64, st)) != hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "ray counters");
-    // Default traversal: width 4, 64-wide groups, 24-entry LDS stack, batches
-    // refilled at 32 idle lanes (the fastest on the 30M-triangle scene).
-    s->config = lr_trace_config{4, 64, 24, 32};
+    // Default traversal: the quantized width-4 layout, 64-wide groups,
+    // 24-entry LDS stack, batches refilled at 32 idle lanes (the fastest on
+    // the 30M-triangle scene).
+    s->config = lr_trace_config{20, 64, 24, 32};
     return 0;
 }
 

@@ -42,7 +42,7 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
     rays[1::3, 3] = 2.5  # some finite t_max
     rb = render.Buffer.create(device, rays)
     ref = None
-    for width in (2, 4, 8):
+    for width in (2, 4, 8, 20, 24):
         for refill in (0, 32):
             g.set_trace_config(width, 64, 24, refill)
             say(f"full size: width {width} refill {refill}")
@@ -70,4 +70,4 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
         assert np.array_equal(hits[sel, 1], t.view(np.uint32))
     for b in (hp, hb, ob, rb):
         b.destroy()
-    g.set_trace_config(4, 64, 24, 32)
+    g.set_trace_config(20, 64, 24, 32)
