@@ -53,7 +53,7 @@ lr::DeviceScene lr_gpu_scene::device_scene() const {
     sc.nodes = static_cast<const uint8_t *>(wide[wi]);
     sc.binary = static_cast<const uint8_t *>(wide[0]);
     sc.ranges = static_cast<const uint2 *>(ranges);
-    sc.tris = static_cast<const float4 *>(tris);
+    sc.tris = static_cast<const float *>(tris);
     sc.faces = static_cast<const uint32_t *>(faces);
     std::memcpy(sc.root_lo, root_lo, 12);
     std::memcpy(sc.root_hi, root_hi, 12);
@@ -242,8 +242,8 @@ int finish_scene(lr_gpu_scene *s, const float *positions, uint32_t *indices, lr_
     // The tree and leaf-order indices stay for refit (the scene frees them).
     s->binary = nodes;
     s->indices = indices;
-    if ((e = hipMalloc(&s->tris, std::max<size_t>(n, 1) * 48)) != hipSuccess) return hip_fail(e, "hipMalloc(triangles)");
-    pack_triangles_device(device, positions, indices, n, static_cast<float4 *>(s->tris));
+    if ((e = hipMalloc(&s->tris, std::max<size_t>(n, 1) * 36)) != hipSuccess) return hip_fail(e, "hipMalloc(triangles)");
+    pack_triangles_device(device, positions, indices, n, static_cast<float *>(s->tris));
     lr_bvh_node root;
     if ((e = hipMemcpyAsync(&root, nodes, sizeof(root), hipMemcpyDeviceToHost, st)) != hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
@@ -266,7 +266,7 @@ int finish_scene(lr_gpu_scene *s, const float *positions, uint32_t *indices, lr_
     if ((qrc = quantize_wide_device(device, s->wide[1], s->info.wide_nodes[1], 4, &s->wide[3])) ||
         (qrc = quantize_wide_device(device, s->wide[2], s->info.wide_nodes[2], 8, &s->wide[4])))
         return qrc;
-    s->info.triangle_bytes = n * 48;
+    s->info.triangle_bytes = n * 36;
     if ((e = hipMalloc((void **)&s->work, 64)) != hipSuccess || (e = hipMemsetAsync(s->work, 0, 64, st)) != hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "ray counters");

@@ -209,17 +209,19 @@ __global__ __launch_bounds__(256) void write_kernel(const lr_bvh_node *nodes, Te
         for (int i = 0; i < W; i++) slots[(size_t)at * W + i] = t.kids[(size_t)id * W + i];
 }
 
-// Leaf-order triangles: p0, e1 = p1 - p0, e2 = p2 - p0 (bvh.glsl:28-30).
+// Leaf-order triangles: p0, e1 = p1 - p0, e2 = p2 - p0 (bvh.glsl:28-30), nine
+// floats a slot.
 __global__ __launch_bounds__(256) void pack_kernel(const float *positions, const uint32_t *indices, uint32_t n,
-                                                  float4 *tris) {
+                                                  float *tris) {
     uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= n) return;
     const float *p0 = positions + (size_t)indices[(size_t)t * 3] * 3;
     const float *p1 = positions + (size_t)indices[(size_t)t * 3 + 1] * 3;
     const float *p2 = positions + (size_t)indices[(size_t)t * 3 + 2] * 3;
-    tris[(size_t)t * 3] = make_float4(p0[0], p0[1], p0[2], p1[0] - p0[0]);
-    tris[(size_t)t * 3 + 1] = make_float4(p1[1] - p0[1], p1[2] - p0[2], p2[0] - p0[0], p2[1] - p0[1]);
-    tris[(size_t)t * 3 + 2] = make_float4(p2[2] - p0[2], 0.0f, 0.0f, 0.0f);
+    float *o = tris + (size_t)t * 9;
+    o[0] = p0[0], o[1] = p0[1], o[2] = p0[2];
+    o[3] = p1[0] - p0[0], o[4] = p1[1] - p0[1], o[5] = p1[2] - p0[2];
+    o[6] = p2[0] - p0[0], o[7] = p2[1] - p0[1], o[8] = p2[2] - p0[2];
 }
 
 // Per-triangle rows of `words` 32-bit words gathered by the leaf order.
@@ -345,7 +347,7 @@ int collapse_wide_device(lr_device *dev, const lr_bvh_node *nodes, size_t node_c
     return rc;
 }
 
-void pack_triangles_device(lr_device *dev, const float *positions, const uint32_t *indices, size_t n, float4 *tris) {
+void pack_triangles_device(lr_device *dev, const float *positions, const uint32_t *indices, size_t n, float *tris) {
     if (n == 0) return;
     hipLaunchKernelGGL(gwide::pack_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, dev->stream, positions,
                        indices, (uint32_t)n, tris);

@@ -161,7 +161,7 @@ int refit_scene_device(lr_gpu_scene *s, const float *positions) {
     if ((rc = quantize_wide_device(dev, s->wide[1], s->info.wide_nodes[1], 4, &s->wide[3])) ||
         (rc = quantize_wide_device(dev, s->wide[2], s->info.wide_nodes[2], 8, &s->wide[4])))
         return rc;
-    pack_triangles_device(dev, positions, s->indices, s->info.triangles, static_cast<float4 *>(s->tris));
+    pack_triangles_device(dev, positions, s->indices, s->info.triangles, static_cast<float *>(s->tris));
     lr_bvh_node root;
     if ((e = hipMemcpyAsync(&root, s->binary, sizeof(root), hipMemcpyDeviceToHost, st)) != hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)

@@ -79,7 +79,7 @@ inline int bound_index(uint32_t code) {
 // Refit (bvh_refit_gpu.hip): new points in device memory, same triangles.
 int refit_scene_device(lr_gpu_scene *scene, const float *positions);
 // Leaf-order triangles (p0, e1, e2 as 3 float4) from device points and indices.
-void pack_triangles_device(lr_device *dev, const float *positions, const uint32_t *indices, size_t n, float4 *tris);
+void pack_triangles_device(lr_device *dev, const float *positions, const uint32_t *indices, size_t n, float *tris);
 // dst[t * words + w] = src[order[t] * words + w]: per-triangle rows into leaf order.
 void gather_rows_device(lr_device *dev, const uint32_t *src, const uint32_t *order, size_t n, uint32_t words,
                         uint32_t *dst);

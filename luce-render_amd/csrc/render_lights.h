@@ -466,8 +466,8 @@ __device__ __forceinline__ uint32_t triangle_emitter(const Params &p, uint32_t t
 // A leaf-order triangle's corners from the repacked triangle array (p0, e1, e2
 // with e1 = p1 - p0, e2 = p2 - p0 as f32: the same edges shading computes).
 __device__ __forceinline__ void triangle_edges(const Params &p, uint32_t tri, V3 &p0, V3 &e1, V3 &e2) {
-    const float4 a = p.scene.tris[(size_t)tri * 3], b = p.scene.tris[(size_t)tri * 3 + 1],
-                 c = p.scene.tris[(size_t)tri * 3 + 2];
+    float4 a, b, c;
+    load_triangle(p.scene.tris, tri, a, b, c);
     p0 = V3{a.x, a.y, a.z};
     e1 = V3{a.w, b.x, b.y};
     e2 = V3{b.z, b.w, c.x};

@@ -32,7 +32,7 @@ namespace lr {
 
 struct DeviceScene {
     const uint8_t *nodes;    // wide nodes, node_bytes each
-    const float4 *tris;      // 3 float4 per leaf slot: p0.xyz e1.x | e1.yz e2.xy | e2.z - - -
+    const float *tris;       // 9 floats per leaf slot (36 B): p0.xyz e1.xyz e2.xyz
     const uint32_t *faces;   // polygon per leaf slot
     float root_lo[3];
     float root_hi[3];
@@ -81,6 +81,16 @@ __device__ __forceinline__ float box_entry(float lox, float loy, float loz, floa
     float enter = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
     float leave = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
     return enter <= leave ? enter : -1.0f;
+}
+
+// A leaf slot's repacked triangle as three registers of four (the last holds
+// e2.z alone): 36 bytes a slot, read at 4-byte alignment.
+constexpr uint32_t kTriangleFloats = 9;
+__device__ __forceinline__ void load_triangle(const float *tris, uint32_t slot, float4 &a, float4 &b, float4 &c) {
+    const float *p = tris + (size_t)slot * kTriangleFloats;
+    a = make_float4(p[0], p[1], p[2], p[3]);
+    b = make_float4(p[4], p[5], p[6], p[7]);
+    c = make_float4(p[8], 0.0f, 0.0f, 0.0f);
 }
 
 // triangle_hit (bvh.glsl:27-47) on a repacked triangle: whether it is hit
@@ -281,11 +291,11 @@ template <int W, int S> struct ClosestWalk {
             // The leaf's triangles in order; the next one's loads are issued
             // before the current one is tested.
             uint32_t first = cur & 0x07ffffffu, last = first + ((cur >> 27) & 0xfu);
-            const float4 *tp = sc.tris + (size_t)first * 3;
-            float4 a = tp[0], b = tp[1], c = tp[2];
+            float4 a, b, c;
+            load_triangle(sc.tris, first, a, b, c);
             for (uint32_t slot = first;; slot++) {
-                const float4 *np = sc.tris + (size_t)(slot < last ? slot + 1 : slot) * 3;
-                float4 na = np[0], nb = np[1], nc = np[2];
+                float4 na, nb, nc;
+                load_triangle(sc.tris, slot < last ? slot + 1 : slot, na, nb, nc);
                 float t, u, v;
                 bool tie;
                 bool take = triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, hit.t, t, u, v, tie);
@@ -372,11 +382,11 @@ template <int W, int S> struct AnyWalk {
     __device__ __forceinline__ bool step(const DeviceScene &sc, int stride) {
         if (cur & kLeafFlag) {
             uint32_t first = cur & 0x07ffffffu, last = first + ((cur >> 27) & 0xfu);
-            const float4 *tp = sc.tris + (size_t)first * 3;
-            float4 a = tp[0], b = tp[1], c = tp[2];
+            float4 a, b, c;
+            load_triangle(sc.tris, first, a, b, c);
             for (uint32_t slot = first;; slot++) {
-                const float4 *np = sc.tris + (size_t)(slot < last ? slot + 1 : slot) * 3;
-                float4 na = np[0], nb = np[1], nc = np[2];
+                float4 na, nb, nc;
+                load_triangle(sc.tris, slot < last ? slot + 1 : slot, na, nb, nc);
                 float t, u, v;
                 bool tie;
                 if (triangle_test(a, b, c, ox, oy, oz, dx, dy, dz, tmax, t, u, v, tie)) {
