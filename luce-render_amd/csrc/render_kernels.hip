@@ -161,6 +161,15 @@ template <bool SPECTRAL> __device__ __forceinline__ Spec clamped(const Params &p
 
 // ------------------------------------------------------------ pass control
 
+// Per-bounce grids are sized for every slot; once few paths remain most of
+// their workgroups have no item. With one request such a workgroup leaves
+// after one load of the queue's count, before the row plan (workgroup 0
+// stays: kernels clear counters there). It would have run no iteration.
+template <class F> __device__ __forceinline__ bool idle_block(const Params *plist, uint32_t n, F count_of) {
+    if (n != 1u || blockIdx.x == 0u) return false;
+    return blockIdx.x * blockDim.x >= count_of(plist[0]);
+}
+
 __global__ void begin_pass_kernel(const Params *plist) {
     const Params &p = plist[blockIdx.x];
     PassState *s = p.pass;
@@ -416,7 +425,9 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint
     __shared__ uint32_t lds[S * G];
     __shared__ RowPlan plan;
     const uint32_t queue = Q_CLOSEST + current;
-    uint32_t total = plan.make(plist, n, [queue](const Params &q) { return q.queues[queue]; });
+    auto count_of = [queue](const Params &q) { return q.queues[queue]; };
+    if (idle_block(plist, n, count_of)) return;
+    uint32_t total = plan.make(plist, n, count_of);
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             plist[i].queues[Q_SHADOW] = 0u;  // consumed by the last bounce's shadow kernel
@@ -449,7 +460,9 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint
 __global__ __launch_bounds__(256) void route_kernel(const Params *plist, uint32_t n, uint32_t current) {
     __shared__ RowPlan plan;
     const uint32_t queue = Q_CLOSEST + current;
-    uint32_t total = plan.make(plist, n, [queue](const Params &q) { return q.queues[queue]; });
+    auto count_of = [queue](const Params &q) { return q.queues[queue]; };
+    if (idle_block(plist, n, count_of)) return;
+    uint32_t total = plan.make(plist, n, count_of);
     __shared__ StagedPushes<3, 256 * kStageIterations> staged;
     staged.init({Q_MISS, Q_SURFACE, Q_LIGHT});
     FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
@@ -476,7 +489,9 @@ __global__ __launch_bounds__(128) void splat_kernel(const Params *plist, uint32_
     __shared__ uint32_t stack[kSplatStack * 128];
     __shared__ RowPlan plan;
     const uint32_t queue = Q_CLOSEST + current;
-    uint32_t total = plan.make(plist, n, [queue](const Params &q) { return q.queues[queue]; });
+    auto count_of = [queue](const Params &q) { return q.queues[queue]; };
+    if (idle_block(plist, n, count_of)) return;
+    uint32_t total = plan.make(plist, n, count_of);
     FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
         if (!active) continue;
         const Params &p = plist[r];
@@ -512,7 +527,9 @@ __global__ __launch_bounds__(128) void splat_kernel(const Params *plist, uint32_
 __global__ __launch_bounds__(128) void splat_shadow_kernel(const Params *plist, uint32_t n) {
     __shared__ uint32_t stack[kSplatStack * 128];
     __shared__ RowPlan plan;
-    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[Q_SHADOW]; });
+    auto count_of = [](const Params &q) { return q.queues[Q_SHADOW]; };
+    if (idle_block(plist, n, count_of)) return;
+    uint32_t total = plan.make(plist, n, count_of);
     FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
         if (!active) continue;
         const Params &p = plist[r];
@@ -1002,7 +1019,9 @@ template <bool SPECTRAL, bool LIGHT, bool PROGRAMS>
 __global__ __launch_bounds__(128, 3) void shade_kernel(const Params *plist, uint32_t n, uint32_t current) {
     __shared__ RowPlan plan;
     const uint32_t queue = LIGHT ? Q_LIGHT : Q_SURFACE;
-    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[LIGHT ? Q_LIGHT : Q_SURFACE]; });
+    auto count_of = [](const Params &q) { return q.queues[LIGHT ? Q_LIGHT : Q_SURFACE]; };
+    if (idle_block(plist, n, count_of)) return;
+    uint32_t total = plan.make(plist, n, count_of);
     // consumed by this bounce's closest-hit and routing kernels
     if (!LIGHT && blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) plist[i].queues[Q_CLOSEST + current] = 0u;
@@ -1028,7 +1047,9 @@ __global__ __launch_bounds__(128, 3) void shade_kernel(const Params *plist, uint
 
 template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(const Params *plist, uint32_t n) {
     __shared__ RowPlan plan;
-    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[Q_MISS]; });
+    auto count_of = [](const Params &q) { return q.queues[Q_MISS]; };
+    if (idle_block(plist, n, count_of)) return;
+    uint32_t total = plan.make(plist, n, count_of);
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             plist[i].queues[Q_SURFACE] = 0u;  // consumed by this bounce's shading
@@ -1091,7 +1112,9 @@ template <int W, int S, int G, bool CUTOUTS>
 __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist, uint32_t n) {
     __shared__ uint32_t lds[S * G];
     __shared__ RowPlan plan;
-    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.queues[Q_SHADOW]; });
+    auto count_of = [](const Params &q) { return q.queues[Q_SHADOW]; };
+    if (idle_block(plist, n, count_of)) return;
+    uint32_t total = plan.make(plist, n, count_of);
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             plist[i].queues[Q_MISS] = 0u;  // consumed by this bounce's miss kernel
