@@ -253,6 +253,19 @@ int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *load
     // Passes in flight finish and count for nothing.
     int rc = r->retire(true);
     if (rc) return rc;
+    // Same scene and shape: the buffers are kept (the image and display show
+    // the last frame until the first new pass).
+    if (!scene || scene == r->job->scene) {
+        rc = r->job->restart_in_place(loaded, settings, frame);
+        if (rc != RenderJob::kNotInPlace) {
+            if (rc) return rc;
+            r->drop_graphs();
+            r->settings = r->job->settings;
+            r->sample = r->band = r->completed = 0;
+            for (double &k : r->kernel_ns) k = 0.0;
+            return 0;
+        }
+    }
     // The new state is made before the old goes, so a failure leaves the
     // renderer as it was.
     std::unique_ptr<RenderJob> job(new RenderJob());

@@ -56,6 +56,67 @@ void *RenderJob::alloc(size_t n, int &rc) {
     return p;
 }
 
+static int check_request(const lr_gpu_scene *sc, const lr_loaded *loaded, const lr_render_settings &st,
+                         const float frame[16], bool &with_programs);
+
+int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_settings *settings_in, const float frame[16]) {
+    if (!loaded || !settings_in || !frame) return fail(LR_INVALID, "a render needs its arguments");
+    const lr_render_settings &st = *settings_in;
+    bool with_programs = false;
+    if (int crc = check_request(scene, loaded, st, frame, with_programs)) return crc;
+    uint32_t w = st.trace_width ? st.trace_width : scene->config.width;
+    uint32_t stack = st.trace_stack ? st.trace_stack : scene->config.stack;
+    uint32_t group = st.trace_group ? st.trace_group : scene->config.group;
+    // The same shape: every buffer's size and every launch the same.
+    bool same = st.width == settings.width && st.height == settings.height && st.max_bounces == settings.max_bounces &&
+                st.spectral == settings.spectral && st.pool == settings.pool && st.guiding == settings.guiding &&
+                st.guiding != 2 && w == settings.trace_width && stack == settings.trace_stack &&
+                group == settings.trace_group && loaded->light_floats == light_floats &&
+                loaded->material_floats == material_floats && loaded->emitter_words == emitter_words &&
+                loaded->light_paths == light_paths && loaded->through == through && with_programs == programs &&
+                (!with_programs || (loaded->program_words == program_words && loaded->texture_floats == texture_floats &&
+                                    loaded->texture_count == texture_count)) &&
+                (uint32_t)loaded->constants[0] == analytic_lights;
+    if (!same) return kNotInPlace;
+    rt::Params &p = params;
+    hipStream_t s = device->stream;
+    hipError_t e = hipSuccess;
+    auto put = [&](const void *dst, const void *src, size_t n) {
+        if (e == hipSuccess && n) e = hipMemcpyAsync(const_cast<void *>(dst), src, n, hipMemcpyHostToDevice, s);
+    };
+    put(p.lights, loaded->lights, loaded->light_floats * 4);
+    put(p.constants, loaded->constants, 128 * 4);
+    put(p.materials, loaded->materials, loaded->material_floats * 4);
+    put(p.emitters, loaded->emitters, loaded->emitter_words * 4);
+    if (with_programs) {
+        put(p.programs, loaded->programs, loaded->program_words * 4);
+        if (loaded->texture_count) {
+            put(p.textures, loaded->textures, loaded->texture_floats / 4 * 16);
+            put(p.texture_table, loaded->texture_table, loaded->texture_count * 16);
+        }
+    }
+    if (e != hipSuccess) return hip_fail(e, "restart upload");
+    launch_light_bounds(p.lights, (uint32_t)loaded->constants[0], (float4 *)p.light_bounds, s);
+    settings.samples = st.samples;
+    settings.seed = st.seed;
+    std::memcpy(p.frame, frame, 64);
+    p.seed = st.seed;
+    adaptive = loaded->constants[44] > 0.0f;
+    cutouts = loaded->constants[3] >= 0.5f;
+    splats = scene->splat_count > 0 && loaded->constants[106] > 0.0f;
+    launch_fill((uint32_t *)p.film, (size_t)4 * p.pixels * 4, 0u, s);
+    launch_fill(p.queues, rt::kQueueBase + (size_t)rt::kQueues * slots, 0u, s);
+    launch_fill(p.state_u, (size_t)rt::kStateU * slots, 0u, s);
+    if (p.guide) launch_fill((uint32_t *)p.guide, rt::kGuideBytes / 4, 0u, s);
+    rt::PassState init{};
+    init.bands = bands;
+    e = hipMemcpyAsync(p.pass, &init, sizeof(init), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(params_device, &p, sizeof(p), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "restart");
+    return 0;
+}
+
 size_t RenderJob::keep_image_only() {
     net.reset();
     std::vector<void *> kept;
@@ -90,10 +151,11 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     return b;
 }
 
-int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded, const lr_render_settings *settings_in,
-                      const float frame[16], float *shared_tables) {
-    if (!dev || !sc || !loaded || !settings_in || !frame) return fail(LR_INVALID, "a render needs its arguments");
-    const lr_render_settings &st = *settings_in;
+// A request's checks (integrator.lucb:343-438): sizes, the loaded arrays,
+// the node-graph programs' words and textures. `with_programs`: some material
+// has a program.
+static int check_request(const lr_gpu_scene *sc, const lr_loaded *loaded, const lr_render_settings &st,
+                         const float frame[16], bool &with_programs) {
     // integrator.lucb:344, 376-377
     if (st.width == 0 || st.height == 0 || st.width > 16384 || st.height > 16384)
         return fail(LR_INVALID, "an image is 1 to 16384 pixels on a side");
@@ -108,7 +170,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     // Node-graph programs: every entry a material names lies in the words,
     // every texture's texels in the texel array (the kernels also stop at
     // the words' end).
-    bool with_programs = false;
+    with_programs = false;
     for (size_t m = 0; (m + 1) * 64 <= loaded->material_floats; m++) {
         float entry = loaded->materials[m * 64 + 52];  // row 13 x
         if (!(entry >= 0.0f)) continue;
@@ -129,6 +191,15 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
                 return fail(LR_INVALID, "a texture's texels lie outside the texel array");
         }
     }
+    return 0;
+}
+
+int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded, const lr_render_settings *settings_in,
+                      const float frame[16], float *shared_tables) {
+    if (!dev || !sc || !loaded || !settings_in || !frame) return fail(LR_INVALID, "a render needs its arguments");
+    const lr_render_settings &st = *settings_in;
+    bool with_programs = false;
+    if (int crc = check_request(sc, loaded, st, frame, with_programs)) return crc;
     device = dev;
     scene = sc;
     settings = st;
@@ -170,6 +241,14 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
         }
         return d;
     };
+    light_floats = loaded->light_floats;
+    material_floats = loaded->material_floats;
+    emitter_words = loaded->emitter_words;
+    program_words = loaded->program_words;
+    texture_floats = loaded->texture_floats;
+    texture_count = loaded->texture_count;
+    light_paths = loaded->light_paths;
+    through = loaded->through;
     p.lights = (const float4 *)upload(loaded->lights, loaded->light_floats * 4);
     p.constants = (const float4 *)upload(loaded->constants, 128 * 4);
     p.materials = (const float4 *)upload(loaded->materials, loaded->material_floats * 4);
@@ -179,6 +258,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     // constant), for the culled light loops.
     uint32_t nlights = (uint32_t)loaded->constants[0];
     if ((size_t)nlights * 28u > loaded->light_floats) return fail(LR_INVALID, "the light count exceeds the light records");
+    analytic_lights = nlights;
     size_t bound_count = (nlights + 7u) / 8u + nlights;
     p.light_bounds = (const float4 *)alloc(bound_count * 16, rc);
     if (rc) return rc;

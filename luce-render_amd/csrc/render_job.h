@@ -35,6 +35,10 @@ struct RenderJob {
     bool cutouts = false;   // some material has geometry opacity below one: shadow rays walk through
     bool programs = false;  // some material has a node-graph program: shading runs it
     bool splats = false;    // the scene has Gaussian splats: the splat stages run
+    // The loaded arrays' sizes this job's buffers hold.
+    size_t light_floats = 0, material_floats = 0, emitter_words = 0, program_words = 0, texture_floats = 0,
+           texture_count = 0;
+    uint32_t light_paths = 0, through = 0, analytic_lights = 0;
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);
@@ -42,6 +46,14 @@ struct RenderJob {
     // queues and pass state start cleared.
     int create(lr_device *device, lr_gpu_scene *scene, const lr_loaded *loaded, const lr_render_settings *settings,
                const float frame[16], float *shared_tables);
+    // A new view or new lights and materials over the same scene with the
+    // same shape (integrator.lucb:463-540: the view's buffers are kept when
+    // they fit): the loaded arrays are copied into the buffers already held,
+    // the film, queues and pass state cleared; the image and display keep the
+    // last frame until the first new pass replaces them. kNotInPlace when
+    // the request needs buffers of other sizes (the caller makes a new job).
+    static constexpr int kNotInPlace = -1;
+    int restart_in_place(const lr_loaded *loaded, const lr_render_settings *settings, const float frame[16]);
     // Frees everything but the finished image; returns the bytes still held.
     size_t keep_image_only();
     // Device bytes a render of these settings holds (its admission cost).

@@ -51,6 +51,38 @@ def test_restart_equals_fresh_render(device):
     assert np.array_equal(got, ref)
 
 
+def test_restart_of_the_same_size_keeps_the_buffers(device):
+    """A restart whose view keeps the image size (integrator.lucb:468-480:
+    buffers made again only when the view outgrows them): the image and
+    display stay at their addresses and show the last frame until the first
+    new pass; the new render still equals a fresh one bit for bit (camera
+    paths alone), and moving the view back gives the first image again."""
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.cornell_render_set())
+    st = _settings()
+    first = render.ray_frame(sc.cameras[0], 48, 40)
+    moved = render.look_frame((0.3, 0.2, 2.8), (-0.1, -0.05, -1.0), (0.0, 1.0, 0.0), 0.45, 48, 40)
+    r = render.Renderer.create_viewed(device, sc, st, first)
+    try:
+        r.finish()
+        before = r.read().copy()
+        im = r.image()
+        r.restart(moved)
+        assert r.samples_done() == 0
+        again = r.image()
+        assert (again.rgba, again.display) == (im.rgba, im.display)
+        assert np.array_equal(r.read(), before)  # the last frame until a new pass
+        r.finish()
+        got = r.read().copy()
+        r.restart(first)
+        r.finish()
+        back = r.read().copy()
+    finally:
+        r.destroy()
+    assert np.array_equal(got, _fresh(device, sc, st, moved))
+    assert np.array_equal(back, before)
+
+
 def test_reload_equals_fresh_render(device):
     from luce_render import render, scenes
     base = scenes.cornell_render_set()
