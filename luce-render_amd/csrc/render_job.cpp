@@ -67,13 +67,16 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
     uint32_t w = st.trace_width ? st.trace_width : scene->config.width;
     uint32_t stack = st.trace_stack ? st.trace_stack : scene->config.stack;
     uint32_t group = st.trace_group ? st.trace_group : scene->config.group;
-    // The same shape: every buffer's size and every launch the same.
-    bool same = st.width == settings.width && st.height == settings.height && st.max_bounces == settings.max_bounces &&
+    // Buffers that fit: the image and the path slots within what is held,
+    // the loaded arrays of the same sizes, the same launches but for their
+    // item counts.
+    Shape sh = shape_of(loaded, st);
+    bool same = sh.pixels <= cap_pixels && sh.slots <= cap_slots && st.max_bounces == settings.max_bounces &&
                 st.spectral == settings.spectral && st.pool == settings.pool && st.guiding == settings.guiding &&
                 st.guiding != 2 && w == settings.trace_width && stack == settings.trace_stack &&
                 group == settings.trace_group && loaded->light_floats == light_floats &&
                 loaded->material_floats == material_floats && loaded->emitter_words == emitter_words &&
-                loaded->light_paths == light_paths && loaded->through == through && with_programs == programs &&
+                loaded->through == through && with_programs == programs &&
                 (!with_programs || (loaded->program_words == program_words && loaded->texture_floats == texture_floats &&
                                     loaded->texture_count == texture_count)) &&
                 (uint32_t)loaded->constants[0] == analytic_lights;
@@ -97,10 +100,27 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
     }
     if (e != hipSuccess) return hip_fail(e, "restart upload");
     launch_light_bounds(p.lights, (uint32_t)loaded->constants[0], (float4 *)p.light_bounds, s);
+    bool resized = st.width != settings.width || st.height != settings.height;
     settings.samples = st.samples;
     settings.seed = st.seed;
+    settings.width = st.width;
+    settings.height = st.height;
     std::memcpy(p.frame, frame, 64);
     p.seed = st.seed;
+    slots = sh.slots;
+    bands = sh.bands;
+    light_paths = loaded->light_paths;
+    p.pool = slots;
+    p.pixels = sh.pixels;
+    p.width = st.width;
+    p.height = st.height;
+    p.band_size = sh.band;
+    p.light_paths = sh.light_paths;
+    p.scene.spill_stride = (uint32_t)((uint64_t)grid_blocks(slots, group) * group);
+    if (resized) {
+        launch_fill((uint32_t *)p.image, (size_t)p.pixels * 4, 0u, s);
+        launch_fill((uint32_t *)p.display, (size_t)p.pixels * 4, 0u, s);
+    }
     adaptive = loaded->constants[44] > 0.0f;
     cutouts = loaded->constants[3] >= 0.5f;
     splats = scene->splat_count > 0 && loaded->constants[106] > 0.0f;
@@ -214,6 +234,8 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     Shape sh = shape_of(loaded, st);
     slots = sh.slots;
     bands = sh.bands;
+    cap_pixels = sh.pixels;
+    cap_slots = sh.slots;
     rounds = st.max_bounces + 1 + loaded->through;
     adaptive = loaded->constants[44] > 0.0f;  // the noise threshold (loading.lucb:267)
     cutouts = loaded->constants[3] >= 0.5f;   // K_INFO.w

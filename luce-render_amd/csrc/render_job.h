@@ -39,6 +39,8 @@ struct RenderJob {
     size_t light_floats = 0, material_floats = 0, emitter_words = 0, program_words = 0, texture_floats = 0,
            texture_count = 0;
     uint32_t light_paths = 0, through = 0, analytic_lights = 0;
+    // What the view's buffers hold room for (pixels; path slots).
+    uint32_t cap_pixels = 0, cap_slots = 0;
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);
@@ -46,12 +48,13 @@ struct RenderJob {
     // queues and pass state start cleared.
     int create(lr_device *device, lr_gpu_scene *scene, const lr_loaded *loaded, const lr_render_settings *settings,
                const float frame[16], float *shared_tables);
-    // A new view or new lights and materials over the same scene with the
-    // same shape (integrator.lucb:463-540: the view's buffers are kept when
-    // they fit): the loaded arrays are copied into the buffers already held,
-    // the film, queues and pass state cleared; the image and display keep the
-    // last frame until the first new pass replaces them. kNotInPlace when
-    // the request needs buffers of other sizes (the caller makes a new job).
+    // A new view or new lights and materials over the same scene whose
+    // image and path slots fit the buffers held (integrator.lucb:463-540:
+    // the view's buffers are kept when they fit): the loaded arrays are
+    // copied into the buffers already held, the film, queues and pass state
+    // cleared; at the same size the image and display keep the last frame
+    // until the first new pass replaces them. kNotInPlace when the request
+    // needs other buffers (the caller makes a new job).
     static constexpr int kNotInPlace = -1;
     int restart_in_place(const lr_loaded *loaded, const lr_render_settings *settings, const float frame[16]);
     // Frees everything but the finished image; returns the bytes still held.

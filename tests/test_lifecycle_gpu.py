@@ -52,11 +52,12 @@ def test_restart_equals_fresh_render(device):
 
 
 def test_restart_of_the_same_size_keeps_the_buffers(device):
-    """A restart whose view keeps the image size (integrator.lucb:468-480:
-    buffers made again only when the view outgrows them): the image and
+    """A restart whose view fits the buffers (integrator.lucb:468-480: made
+    again only when the view outgrows them): at the same size the image and
     display stay at their addresses and show the last frame until the first
-    new pass; the new render still equals a fresh one bit for bit (camera
-    paths alone), and moving the view back gives the first image again."""
+    new pass; a smaller view keeps them too; each render still equals a fresh
+    one bit for bit (camera paths alone), and moving the view back gives the
+    first image again."""
     from luce_render import render, scenes
     sc = render.compile_scene(scenes.cornell_render_set())
     st = _settings()
@@ -74,12 +75,21 @@ def test_restart_of_the_same_size_keeps_the_buffers(device):
         assert np.array_equal(r.read(), before)  # the last frame until a new pass
         r.finish()
         got = r.read().copy()
+        # a smaller view fits the buffers too
+        small = render.look_frame((0.3, 0.2, 2.8), (-0.1, -0.05, -1.0), (0.0, 1.0, 0.0), 0.45, 40, 24)
+        r.restart(small)
+        assert r.image().rgba == im.rgba
+        r.finish()
+        got_small = r.read().copy()
+        assert got_small.shape == (24, 40, 4)
         r.restart(first)
+        assert r.image().rgba == im.rgba
         r.finish()
         back = r.read().copy()
     finally:
         r.destroy()
     assert np.array_equal(got, _fresh(device, sc, st, moved))
+    assert np.array_equal(got_small, _fresh(device, sc, dataclasses.replace(st, width=40, height=24), small))
     assert np.array_equal(back, before)
 
 
