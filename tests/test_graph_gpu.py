@@ -125,3 +125,41 @@ def test_viewed_graph_light_tracing_is_unbiased(device, spectral):
         means.append(float(img[:, :, 1].astype(np.float64).sum()))
     say(f"viewed graph (spectral {spectral}): light paths give {means[1] / means[0]} of the camera paths' mean")
     assert abs(means[1] / means[0] - 1.0) < 0.005
+
+
+def test_graph_request_served_beside_plain_requests(device):
+    """The serving loop with a node-graph request in the mix: the shared
+    launches take the program-running shading kernels; the graph request
+    renders the same bits alone and mixed (camera paths), and the plain
+    request matches its own alone render."""
+    from luce_render import render, scenes, serving
+    from luce_render.loading import RenderSettings
+    sets = [render.compile_scene(scenes.graph_set(frequency=12)), render.compile_scene(scenes.cornell_render_set())]
+    gpus = [render.GpuScene.upload(device, sc) for sc in sets]
+
+    def serve(jobs):
+        server = serving.Server(device, budget_bytes=8 << 30)
+        try:
+            ids = [server.submit(gpus[k], sets[k], RenderSettings(w, h, spp, max_bounces=6, spectral=False,
+                                                                  light_tracing=0.0)) for k, w, h, spp in jobs]
+            steps = 0
+            while server.busy() or steps == 0:
+                server.step()
+                steps += 1
+                assert steps < 1000
+            out = [server.read(i, w, h).copy() for i, (k, w, h, spp) in zip(ids, jobs)]
+            for i in ids:
+                server.release(i)
+            return out
+        finally:
+            server.destroy()
+
+    graph_job, plain_job = (0, 64, 48, 8), (1, 48, 48, 6)
+    (alone_graph,) = serve([graph_job])
+    (alone_plain,) = serve([plain_job])
+    mixed = serve([plain_job, graph_job])
+    for g in gpus:
+        g.destroy()
+    assert np.array_equal(mixed[1], alone_graph)
+    assert np.allclose(mixed[0], alone_plain, rtol=1e-6, atol=1e-7)
+    assert float(alone_graph[:, :, :3].mean()) > 0.0
