@@ -271,9 +271,9 @@ int finish_scene(lr_gpu_scene *s, const float *positions, uint32_t *indices, lr_
         (e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "ray counters");
     // Default traversal: the quantized width-4 layout, 64-wide groups,
-    // 24-entry LDS stack, batches refilled at 32 idle lanes (the fastest on
+    // 16-entry LDS stack, batches refilled at 32 idle lanes (the fastest on
     // the 30M-triangle scene).
-    s->config = lr_trace_config{20, 64, 24, 32};
+    s->config = lr_trace_config{20, 64, 16, 32};
     return 0;
 }
 
@@ -426,7 +426,7 @@ void lr_scene_destroy(lr_gpu_scene *scene) {
 int lr_scene_set_trace_config(lr_gpu_scene *scene, const lr_trace_config *config) {
     if (!scene || !config) return fail(LR_INVALID, "lr_scene_set_trace_config needs a scene and a config");
     if (variant_index(config->width, config->stack, config->group) < 0)
-        return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 24, 32, 48; groups 64, 128, 256");
+        return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 16, 24, 32; groups 64, 128, 256");
     if (config->refill > 64) return fail(LR_INVALID, "refill is 0 (one ray a thread) or 1..64 idle lanes");
     scene->config = *config;
     return 0;

@@ -42,7 +42,7 @@ struct lr_gpu_scene {
     void *splat_nodes = nullptr;
     size_t splat_count = 0;
     lr_scene_info info{};
-    lr_trace_config config{20, 64, 24, 32};
+    lr_trace_config config{20, 64, 16, 32};
 
     ~lr_gpu_scene();
     int ensure_spill(uint64_t threads, uint32_t stack);

@@ -1462,9 +1462,9 @@ template <int W, int S> void fill_trace_g(RenderTraceVariant *t, int wi, int si)
     fill_trace<W, S, 256>(t[(wi * 3 + si) * 3 + 2]);
 }
 template <int W> void fill_trace_s(RenderTraceVariant *t, int wi) {
-    fill_trace_g<W, 24>(t, wi, 0);
-    fill_trace_g<W, 32>(t, wi, 1);
-    fill_trace_g<W, 48>(t, wi, 2);
+    fill_trace_g<W, 16>(t, wi, 0);
+    fill_trace_g<W, 24>(t, wi, 1);
+    fill_trace_g<W, 32>(t, wi, 2);
 }
 
 }  // namespace

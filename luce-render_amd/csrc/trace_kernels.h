@@ -19,7 +19,7 @@ struct TraceVariant {
                hipStream_t stream);
 };
 
-// 45 variants: index (layout 2/4/8/20/24, stack 24/32/48, group 64/128/256).
+// 45 variants: index (layout 2/4/8/20/24, stack 16/24/32, group 64/128/256).
 const TraceVariant *trace_variants();
 int variant_index(uint32_t width, uint32_t stack, uint32_t group);
 

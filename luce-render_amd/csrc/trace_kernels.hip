@@ -232,9 +232,9 @@ template <int W, int S> void fill_g(TraceVariant *table, int wi, int si) {
 }
 
 template <int W> void fill_s(TraceVariant *table, int wi) {
-    fill_g<W, 24>(table, wi, 0);
-    fill_g<W, 32>(table, wi, 1);
-    fill_g<W, 48>(table, wi, 2);
+    fill_g<W, 16>(table, wi, 0);
+    fill_g<W, 24>(table, wi, 1);
+    fill_g<W, 32>(table, wi, 2);
 }
 
 }  // namespace
@@ -255,7 +255,7 @@ const TraceVariant *trace_variants() {
 
 int variant_index(uint32_t width, uint32_t stack, uint32_t group) {
     int wi = layout_index(width);
-    int si = stack == 24 ? 0 : stack == 32 ? 1 : stack == 48 ? 2 : -1;
+    int si = stack == 16 ? 0 : stack == 24 ? 1 : stack == 32 ? 2 : -1;
     int gi = group == 64 ? 0 : group == 128 ? 1 : group == 256 ? 2 : -1;
     if (wi < 0 || si < 0 || gi < 0) return -1;
     return (wi * 3 + si) * 3 + gi;

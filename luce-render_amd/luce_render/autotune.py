@@ -12,7 +12,7 @@ from . import render
 
 WIDTHS = (2, 4, 8, 20, 24)  # 20, 24: the quantized 4- and 8-wide layouts
 GROUPS = (64, 128, 256)
-STACKS = (24, 32, 48)
+STACKS = (16, 24, 32)
 REFILLS = (0, 32, 48)  # 0: one ray per thread; k: persistent waves refilled at k idle lanes
 
 

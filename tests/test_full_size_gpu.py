@@ -70,4 +70,4 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
         assert np.array_equal(hits[sel, 1], t.view(np.uint32))
     for b in (hp, hb, ob, rb):
         b.destroy()
-    g.set_trace_config(20, 64, 24, 32)
+    g.set_trace_config(20, 64, 16, 32)
