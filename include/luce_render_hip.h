@@ -294,8 +294,12 @@ int lr_renderer_stats(lr_renderer *renderer, lr_render_stats *out);
  * the renderer's when not NULL (which must then outlive the renderer). Passes
  * in flight finish and count for nothing; the film, sample count and kernel
  * times start over. The projection (perspective or orthographic) is kept.
- * The renderer's device state is made anew (the image and display pointers
- * change); on failure the renderer is left as it was. */
+ * With the same scene, a view (or lights and materials) whose image and path
+ * slots fit the buffers held keeps them, and the image and display pointers
+ * stay; a view with more pixels than they hold makes them again with a
+ * quarter more room each way, rounded up to 64 (integrator.lucb:474-478,
+ * 986-989); otherwise the device state is made anew and the pointers change.
+ * On failure the renderer is left as it was. */
 int lr_renderer_reset(lr_renderer *renderer, lr_gpu_scene *scene, const lr_loaded *loaded,
                       const lr_render_settings *settings, const float frame[16]);
 /* Renderer.converged_pixels (integrator.lucb:759): pixels adaptive sampling

@@ -245,6 +245,14 @@ int lr_renderer_stats(lr_renderer *r, lr_render_stats *out) {
     return 0;
 }
 
+// A side of a view's buffers that grew: a quarter more than wanted, rounded
+// up to 64, never below what is held, at most 16384 (integrator.lucb:986-989).
+static uint32_t roomy_side(uint32_t wanted, uint32_t held) {
+    uint32_t side = (wanted + wanted / 4u + 63u) / 64u * 64u;
+    if (side < held) side = held;
+    return side <= 16384u ? side : 16384u;
+}
+
 int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *loaded, const lr_render_settings *settings,
                       const float frame[16]) {
     if (!r || !loaded || !settings || !frame) return fail(LR_INVALID, "lr_renderer_reset needs its arguments");
@@ -269,8 +277,34 @@ int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *load
     // The new state is made before the old goes, so a failure leaves the
     // renderer as it was.
     std::unique_ptr<RenderJob> job(new RenderJob());
-    rc = job->create(r->device, scene ? scene : r->job->scene, loaded, settings, frame, nullptr);
-    if (rc) return rc;
+    rc = RenderJob::kNotInPlace;
+    const RenderJob &held = *r->job;
+    if ((!scene || scene == held.scene) && (uint64_t)settings->width * settings->height > held.cap_pixels &&
+        settings->guiding != 2) {
+        // A view that outgrows the buffers: made again with a quarter more
+        // room each way (and light paths in proportion), then the view taken
+        // in place, so a view growing a few pixels at a time (a window
+        // resized) makes its buffers again seldom (integrator.lucb:474-478).
+        lr_render_settings room = *settings;
+        room.width = roomy_side(settings->width, held.cap_width);
+        room.height = roomy_side(settings->height, held.cap_height);
+        float room_frame[16];
+        std::memcpy(room_frame, frame, sizeof(room_frame));
+        room_frame[11] = (float)room.width;
+        room_frame[15] = (float)room.height;
+        lr_loaded room_loaded = *loaded;
+        uint64_t pixels = (uint64_t)settings->width * settings->height, room_pixels = (uint64_t)room.width * room.height;
+        room_loaded.light_paths = (uint32_t)(((uint64_t)loaded->light_paths * room_pixels + pixels - 1) / pixels);
+        rc = job->create(r->device, held.scene, &room_loaded, &room, room_frame, nullptr);
+        if (rc) return rc;
+        rc = job->restart_in_place(loaded, settings, frame);
+        if (rc == RenderJob::kNotInPlace) job.reset(new RenderJob());
+        else if (rc) return rc;
+    }
+    if (rc == RenderJob::kNotInPlace) {
+        rc = job->create(r->device, scene ? scene : r->job->scene, loaded, settings, frame, nullptr);
+        if (rc) return rc;
+    }
     r->drop_graphs();
     r->job.swap(job);
     r->settings = r->job->settings;

@@ -236,6 +236,8 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     bands = sh.bands;
     cap_pixels = sh.pixels;
     cap_slots = sh.slots;
+    cap_width = st.width;
+    cap_height = st.height;
     rounds = st.max_bounces + 1 + loaded->through;
     adaptive = loaded->constants[44] > 0.0f;  // the noise threshold (loading.lucb:267)
     cutouts = loaded->constants[3] >= 0.5f;   // K_INFO.w

@@ -41,6 +41,7 @@ struct RenderJob {
     uint32_t light_paths = 0, through = 0, analytic_lights = 0;
     // What the view's buffers hold room for (pixels; path slots).
     uint32_t cap_pixels = 0, cap_slots = 0;
+    uint32_t cap_width = 0, cap_height = 0;  // the view the buffers were made for
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);

@@ -93,6 +93,47 @@ def test_restart_of_the_same_size_keeps_the_buffers(device):
     assert np.array_equal(back, before)
 
 
+def test_a_growing_view_gets_a_quarter_more_room(device):
+    """integrator.lucb:474-478, 986-989: a restart into a view with more
+    pixels than the buffers makes them again with a quarter more room each way
+    (rounded up to 64: 64 x 48 gets 128 x 64), so the next larger view that
+    fits (100 x 60) is taken in place; each render equals a fresh one bit for
+    bit (camera paths alone). With light tracing the roomy buffers hold the
+    larger view's light paths too: the image equals a fresh render's but for
+    the order of the light paths' float additions."""
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.cornell_render_set())
+    for light_tracing in (0.0, 0.5):
+        st = _settings(light_tracing=light_tracing)
+        first = render.ray_frame(sc.cameras[0], 48, 40)
+        grown = render.look_frame((0.3, 0.2, 2.8), (-0.1, -0.05, -1.0), (0.0, 1.0, 0.0), 0.45, 64, 48)
+        more = render.look_frame((0.3, 0.2, 2.8), (-0.1, -0.05, -1.0), (0.0, 1.0, 0.0), 0.45, 100, 60)
+        r = render.Renderer.create_viewed(device, sc, st, first)
+        try:
+            r.finish()
+            im = r.image()
+            r.restart(grown)
+            roomy = r.image()
+            assert roomy.rgba != im.rgba
+            r.finish()
+            got_grown = r.read().copy()
+            r.restart(more)
+            assert r.image().rgba == roomy.rgba
+            r.finish()
+            got_more = r.read().copy()
+        finally:
+            r.destroy()
+        assert got_grown.shape == (48, 64, 4) and got_more.shape == (60, 100, 4)
+        for got, frame in ((got_grown, grown), (got_more, more)):
+            ref = _fresh(device, sc, dataclasses.replace(st, width=frame.width, height=frame.height), frame)
+            say(f"light tracing {light_tracing}, {frame.width}x{frame.height}: mean {got[:, :, :3].mean()} "
+                f"fresh {ref[:, :, :3].mean()}")
+            if light_tracing == 0.0:
+                assert np.array_equal(got, ref)
+            else:
+                assert np.allclose(got, ref, rtol=1e-4, atol=1e-5)
+
+
 def test_reload_equals_fresh_render(device):
     from luce_render import render, scenes
     base = scenes.cornell_render_set()
