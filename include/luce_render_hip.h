@@ -272,6 +272,15 @@ uint32_t lr_renderer_samples_done(lr_renderer *renderer);
 int lr_renderer_image(lr_renderer *renderer, lr_image *out);
 /* Renderer.read (:778): width * height * 4 floats, row by row from the top. */
 int lr_renderer_read(lr_renderer *renderer, float *rgba, size_t count);
+/* The display image (Renderer.display, :743: exposed, linear Rec.709, through
+ * the view, clamped to [0, 1]) read like lr_renderer_read. */
+int lr_renderer_read_display(lr_renderer *renderer, float *rgba, size_t count);
+/* Renderer.set_display (:750-756): show the image at `exposure` stops through
+ * `view` (0 standard, 1 Khronos PBR Neutral tone mapping). The next pass's
+ * conversion applies it, or, once every sample is in, a conversion of its own
+ * that lr_renderer_advance / lr_renderer_finish launch (:656-669). A restart,
+ * reload or scene replacement takes the settings of its loaded constants. */
+int lr_renderer_set_display(lr_renderer *renderer, float exposure, int view);
 /* The film's four planes (sum, half, scratch, splat; common.glsl:64-75),
  * width * height * 16 floats: no reference counterpart, for tests. */
 int lr_renderer_read_film(lr_renderer *renderer, float *planes, size_t count);

@@ -39,6 +39,8 @@ struct lr_renderer {
     // host mirror of the device pass state
     uint32_t sample = 0, band = 0;
     uint32_t completed = 0;
+    // The display settings changed since the display image was last made.
+    bool display_stale = false;
     std::deque<Pass> in_flight;
     double kernel_ns[KKinds] = {0};
     hipGraph_t graph[2] = {nullptr, nullptr};  // without, with the training steps
@@ -161,9 +163,21 @@ int lr_renderer_advance(lr_renderer *r, int *busy) {
         return 0;
     }
     if (r->sample >= r->settings.samples) {
-        if (busy) *busy = r->in_flight.empty() ? 0 : 1;
-        return 0;
+        if (!r->display_stale) {
+            if (busy) *busy = r->in_flight.empty() ? 0 : 1;
+            return 0;
+        }
+        // Only the display changed: the film converted again, after the
+        // passes in flight (integrator.lucb:656-669).
+        r->display_stale = false;
+        uint32_t pixels = r->settings.width * r->settings.height;
+        launch_render_stage(Stage::Convert, r->settings.spectral != 0, r->job->params_device, 1, 1,
+                            (pixels + 255u) / 256u, r->device->stream);
+        hipError_t e = hipGetLastError();
+        if (busy) *busy = 1;
+        return e == hipSuccess ? 0 : hip_fail(e, "display conversion");
     }
+    r->display_stale = false;  // the pass's last band converts with the new settings
     rc = r->launch_pass();
     if (busy) *busy = 1;
     return rc;
@@ -208,6 +222,32 @@ int lr_renderer_read(lr_renderer *r, float *rgba, size_t count) {
     hipError_t e = hipMemcpyAsync(rgba, r->job->params.image, pixels * 16, hipMemcpyDeviceToHost, r->device->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
     if (e != hipSuccess) return hip_fail(e, "image read");
+    return 0;
+}
+
+int lr_renderer_read_display(lr_renderer *r, float *rgba, size_t count) {
+    if (!r || !rgba) return fail(LR_INVALID, "lr_renderer_read_display needs a renderer and an output");
+    size_t pixels = (size_t)r->settings.width * r->settings.height;
+    if (count != pixels * 4) return fail(LR_INVALID, "reading the display image needs four floats a pixel");
+    int rc = r->retire(true);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(rgba, r->job->params.display, pixels * 16, hipMemcpyDeviceToHost, r->device->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
+    if (e != hipSuccess) return hip_fail(e, "display read");
+    return 0;
+}
+
+int lr_renderer_set_display(lr_renderer *r, float exposure, int view) {
+    if (!r) return fail(LR_INVALID, "lr_renderer_set_display needs a renderer");
+    if (view != 0 && view != 1) return fail(LR_INVALID, "the view is 0 (standard) or 1 (neutral tone mapping)");
+    (void)hipSetDevice(r->device->ordinal);
+    const float row[4] = {exposure, (float)view, 0.0f, 0.0f};
+    hipStream_t s = r->device->stream;
+    hipError_t e = hipMemcpyAsync((char *)r->job->params.constants + rt::K_DISPLAY * 16, row, sizeof(row),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "display settings");
+    r->display_stale = true;
     return 0;
 }
 
@@ -270,6 +310,7 @@ int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *load
             r->drop_graphs();
             r->settings = r->job->settings;
             r->sample = r->band = r->completed = 0;
+            r->display_stale = false;
             for (double &k : r->kernel_ns) k = 0.0;
             return 0;
         }
@@ -309,6 +350,7 @@ int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *load
     r->job.swap(job);
     r->settings = r->job->settings;
     r->sample = r->band = r->completed = 0;
+    r->display_stale = false;
     for (double &k : r->kernel_ns) k = 0.0;
     return 0;
 }

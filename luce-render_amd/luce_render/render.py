@@ -520,6 +520,22 @@ class Renderer:
         _lib.check(_lib.lib().lr_renderer_read(self.handle, _lib.ptr(buf), n))
         return buf.reshape(self.settings.height, self.settings.width, 4)
 
+    def read_display(self, out=None):
+        """The display image (Renderer.display, integrator.lucb:743) as rgba32f
+        (height, width, 4): exposed, linear Rec.709, through the view."""
+        n = self.settings.width * self.settings.height * 4
+        if out is not None and out.size != n:
+            raise LuceError(INVALID, "reading the display image needs four floats a pixel")
+        buf = np.empty(n, np.float32) if out is None else out
+        _lib.check(_lib.lib().lr_renderer_read_display(self.handle, _lib.ptr(buf), n))
+        return buf.reshape(self.settings.height, self.settings.width, 4)
+
+    def set_display(self, exposure: float, view: int = 0):
+        """Renderer.set_display (integrator.lucb:750-756): `exposure` stops,
+        `view` 0 standard or 1 neutral tone mapping; the next pass, or once
+        every sample is in a conversion of its own, applies it."""
+        _lib.check(_lib.lib().lr_renderer_set_display(self.handle, float(exposure), int(view)))
+
     def read_film(self):
         n = self.settings.width * self.settings.height * 16
         buf = np.empty(n, np.float32)

@@ -247,3 +247,49 @@ def test_textured_dome_light_tracing_is_unbiased(device):
     say(f"dome camera-only {means[0]} light-traced {means[1]}")
     assert not np.array_equal(means[0], means[1])
     assert np.all(np.abs(means[1] - means[0]) <= 0.01 * means[0])
+
+
+def test_set_display_converts_the_film_again(device):
+    """Renderer.set_display (integrator.lucb:750-756, 656-669): once every
+    sample is in, a change of exposure or view converts the film again without
+    rendering (the image stays); one stop doubles the display below the clamp
+    and saturates the rest; the neutral view changes it; set during the render,
+    the next pass applies it (the same display as set afterwards)."""
+    from luce_render import render, scenes
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(scenes.cornell_render_set())
+    st = RenderSettings(48, 40, 4, max_bounces=5, spectral=False, light_tracing=0.0)
+    frame = render.ray_frame(sc.cameras[0], 48, 40)
+    r = render.Renderer.create_viewed(device, sc, st, frame)
+    try:
+        r.set_display(0.0, 0)
+        r.finish()
+        image = r.read().copy()
+        base = r.read_display()[:, :, :3].copy()
+        r.set_display(1.0, 0)
+        r.finish()
+        assert r.samples_done() == 4 and np.array_equal(r.read(), image)
+        bright = r.read_display()[:, :, :3].copy()
+        r.set_display(0.0, 1)
+        r.finish()
+        neutral = r.read_display()[:, :, :3].copy()
+        with pytest.raises(render.LuceError):
+            r.set_display(0.0, 2)
+    finally:
+        r.destroy()
+    low = base < 0.49
+    say(f"display: {low.sum()} values below the clamp of {base.size}")
+    assert low.sum() > base.size // 4 and (~low).sum() > 0
+    assert np.allclose(bright[low], 2.0 * base[low], rtol=1e-6, atol=0.0)
+    assert np.all(bright[~low] >= 0.98)
+    assert not np.array_equal(neutral, base) and neutral.min() >= 0.0 and neutral.max() <= 1.0
+    r = render.Renderer.create_viewed(device, sc, st, frame)
+    try:
+        r.set_display(0.0, 0)
+        r.advance()
+        r.set_display(1.0, 0)
+        r.finish()
+        during = r.read_display()[:, :, :3].copy()
+    finally:
+        r.destroy()
+    assert np.array_equal(during, bright)
