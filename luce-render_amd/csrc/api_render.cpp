@@ -337,10 +337,13 @@ int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *load
         uint64_t pixels = (uint64_t)settings->width * settings->height, room_pixels = (uint64_t)room.width * room.height;
         room_loaded.light_paths = (uint32_t)(((uint64_t)loaded->light_paths * room_pixels + pixels - 1) / pixels);
         rc = job->create(r->device, held.scene, &room_loaded, &room, room_frame, nullptr);
-        if (rc) return rc;
-        rc = job->restart_in_place(loaded, settings, frame);
-        if (rc == RenderJob::kNotInPlace) job.reset(new RenderJob());
-        else if (rc) return rc;
+        if (rc == 0) rc = job->restart_in_place(loaded, settings, frame);
+        if (rc) {
+            // No room to spare (or none needed): the exact size, as for a new
+            // renderer.
+            job.reset(new RenderJob());
+            rc = RenderJob::kNotInPlace;
+        }
     }
     if (rc == RenderJob::kNotInPlace) {
         rc = job->create(r->device, scene ? scene : r->job->scene, loaded, settings, frame, nullptr);
