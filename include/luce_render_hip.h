@@ -178,6 +178,12 @@ int lr_trace_closest_device(const lr_gpu_scene *scene, const float *rays_device,
 int lr_trace_any_device(const lr_gpu_scene *scene, const float *rays_device, size_t n,
                         uint32_t *occluded_device, void *stream);
 
+/* Workgroups a refilled (persistent) launch of the scene's traversal variant
+ * takes over n rays on its device: the resident workgroups of that variant's
+ * kernel (closest hit, or any hit when `any`), at most what n rays need. No
+ * reference counterpart (the autotuner's grid sizing, reported). */
+int lr_trace_resident_grid(const lr_gpu_scene *scene, size_t n, int any, uint32_t *blocks);
+
 /* Scene statistics for roofline accounting: node counts per width, bytes of
  * each layout, triangle bytes, and the stack bound of each width. */
 typedef struct lr_scene_info {

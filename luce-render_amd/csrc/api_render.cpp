@@ -247,6 +247,7 @@ int lr_renderer_set_display(lr_renderer *r, float exposure, int view) {
                                   hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "display settings");
+    r->job->set_display_row(row);
     r->display_stale = true;
     return 0;
 }
@@ -347,6 +348,10 @@ int lr_renderer_reset(lr_renderer *r, lr_gpu_scene *scene, const lr_loaded *load
     }
     if (rc == RenderJob::kNotInPlace) {
         rc = job->create(r->device, scene ? scene : r->job->scene, loaded, settings, frame, nullptr);
+        if (rc) return rc;
+    }
+    if (job->scene == held.scene) {
+        rc = job->adopt_display(held);
         if (rc) return rc;
     }
     r->drop_graphs();

@@ -30,8 +30,8 @@ struct Request {
     lr_render_settings settings{};
     float frame[16];
     // the loaded arrays, copied at submission (the caller's are borrowed)
-    std::vector<float> lights, constants, materials;
-    std::vector<uint32_t> emitters;
+    std::vector<float> lights, constants, materials, textures;
+    std::vector<uint32_t> emitters, programs, texture_table;
     lr_loaded loaded{};
     size_t footprint = 0;
     std::unique_ptr<RenderJob> job;
@@ -142,11 +142,20 @@ int lr_server_submit(lr_server *s, lr_gpu_scene *scene, const lr_loaded *loaded,
     r->constants.assign(loaded->constants, loaded->constants + 128);
     r->materials.assign(loaded->materials, loaded->materials + loaded->material_floats);
     r->emitters.assign(loaded->emitters, loaded->emitters + loaded->emitter_words);
+    // The programs and textures too: the request uploads them at admission,
+    // a later step, after the caller's arrays may be gone.
+    if (loaded->programs) r->programs.assign(loaded->programs, loaded->programs + loaded->program_words);
+    if (loaded->textures) r->textures.assign(loaded->textures, loaded->textures + loaded->texture_floats);
+    if (loaded->texture_table)
+        r->texture_table.assign(loaded->texture_table, loaded->texture_table + loaded->texture_count * 4);
     r->loaded = *loaded;
     r->loaded.lights = r->lights.data();
     r->loaded.constants = r->constants.data();
     r->loaded.materials = r->materials.data();
     r->loaded.emitters = r->emitters.data();
+    r->loaded.programs = loaded->programs ? r->programs.data() : nullptr;
+    r->loaded.textures = loaded->textures ? r->textures.data() : nullptr;
+    r->loaded.texture_table = loaded->texture_table ? r->texture_table.data() : nullptr;
     r->footprint = RenderJob::footprint(scene, &r->loaded, &r->settings);
     if (r->footprint > s->settings.budget_bytes)
         return fail(LR_LIMIT, "the request alone needs more device memory than the server's budget");

@@ -498,6 +498,13 @@ int lr_trace_closest_device(const lr_gpu_scene *scene_c, const float *rays, size
     return 0;
 }
 
+int lr_trace_resident_grid(const lr_gpu_scene *scene, size_t n, int any, uint32_t *blocks) {
+    if (!scene || !blocks) return fail(LR_INVALID, "lr_trace_resident_grid needs a scene and an output");
+    (void)hipSetDevice(scene->device->ordinal);
+    *blocks = scene->variant().resident((uint64_t)n, any);
+    return 0;
+}
+
 int lr_trace_any_device(const lr_gpu_scene *scene_c, const float *rays, size_t n, uint32_t *occluded, void *stream) {
     lr_gpu_scene *scene = const_cast<lr_gpu_scene *>(scene_c);
     if (!scene || (n && (!rays || !occluded))) return fail(LR_INVALID, "lr_trace_any_device needs its arguments");

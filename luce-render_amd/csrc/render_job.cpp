@@ -19,6 +19,15 @@ uint32_t grid_blocks(uint64_t items, uint32_t group) {
 }
 
 
+// Columns of a render's traversal spill area. The render's tracing kernels
+// take an item's index in its render's queue (below the render's slots) as
+// its spill column, whatever grid the launch takes, so there is a column for
+// every slot: no clamp to a launch's grid.
+uint64_t spill_columns(uint64_t slots, uint32_t group) {
+    uint64_t g = group ? group : 1;
+    return (slots + g - 1) / g * g;
+}
+
 namespace {
 
 struct Shape {
@@ -71,7 +80,8 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
     // the loaded arrays of the same sizes, the same launches but for their
     // item counts.
     Shape sh = shape_of(loaded, st);
-    bool same = sh.pixels <= cap_pixels && sh.slots <= cap_slots && st.max_bounces == settings.max_bounces &&
+    bool with_splats = scene->splat_count > 0 && loaded->constants[106] > 0.0f;
+    bool same = with_splats == splats && sh.pixels <= cap_pixels && sh.slots <= cap_slots && st.max_bounces == settings.max_bounces &&
                 st.spectral == settings.spectral && st.pool == settings.pool && st.guiding == settings.guiding &&
                 st.guiding != 2 && w == settings.trace_width && stack == settings.trace_stack &&
                 group == settings.trace_group && loaded->light_floats == light_floats &&
@@ -87,8 +97,16 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
     auto put = [&](const void *dst, const void *src, size_t n) {
         if (e == hipSuccess && n) e = hipMemcpyAsync(const_cast<void *>(dst), src, n, hipMemcpyHostToDevice, s);
     };
+    // A restart (integrator.lucb:463-466): the same loaded arrays but for the
+    // view rows (12..15) keep the display row set_display wrote (the
+    // reference uploads the view rows alone). New lights or materials (a
+    // reload) take the display row they were packed with.
+    float next[128];
+    std::memcpy(next, loaded->constants, sizeof(next));
+    if (same_but_view(next, constants))
+        for (int k = 0; k < 4; k++) next[rt::K_DISPLAY * 4 + k] = constants[rt::K_DISPLAY * 4 + k];
     put(p.lights, loaded->lights, loaded->light_floats * 4);
-    put(p.constants, loaded->constants, 128 * 4);
+    put(p.constants, next, 128 * 4);
     put(p.materials, loaded->materials, loaded->material_floats * 4);
     put(p.emitters, loaded->emitters, loaded->emitter_words * 4);
     if (with_programs) {
@@ -98,7 +116,9 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
             put(p.texture_table, loaded->texture_table, loaded->texture_count * 16);
         }
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `next` is on this stack
     if (e != hipSuccess) return hip_fail(e, "restart upload");
+    std::memcpy(constants, next, sizeof(constants));
     launch_light_bounds(p.lights, (uint32_t)loaded->constants[0], (float4 *)p.light_bounds, s);
     bool resized = st.width != settings.width || st.height != settings.height;
     settings.samples = st.samples;
@@ -116,14 +136,13 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
     p.height = st.height;
     p.band_size = sh.band;
     p.light_paths = sh.light_paths;
-    p.scene.spill_stride = (uint32_t)((uint64_t)grid_blocks(slots, group) * group);
+    p.scene.spill_stride = (uint32_t)spill_columns(slots, group);
     if (resized) {
         launch_fill((uint32_t *)p.image, (size_t)p.pixels * 4, 0u, s);
         launch_fill((uint32_t *)p.display, (size_t)p.pixels * 4, 0u, s);
     }
     adaptive = loaded->constants[44] > 0.0f;
     cutouts = loaded->constants[3] >= 0.5f;
-    splats = scene->splat_count > 0 && loaded->constants[106] > 0.0f;
     launch_fill((uint32_t *)p.film, (size_t)4 * p.pixels * 4, 0u, s);
     launch_fill(p.queues, rt::kQueueBase + (size_t)rt::kQueues * slots, 0u, s);
     launch_fill(p.state_u, (size_t)rt::kStateU * slots, 0u, s);
@@ -135,6 +154,29 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "restart");
     return 0;
+}
+
+bool RenderJob::same_but_view(const float a[128], const float b[128]) {
+    for (int k = 0; k < 128; k++) {
+        int row = k / 4;
+        if (row == rt::K_DISPLAY || (row >= 12 && row <= 15)) continue;
+        if (std::memcmp(&a[k], &b[k], 4) != 0) return false;
+    }
+    return true;
+}
+
+int RenderJob::adopt_display(const RenderJob &held) {
+    if (!same_but_view(constants, held.constants)) return 0;
+    const float *row = held.constants + rt::K_DISPLAY * 4;
+    set_display_row(row);
+    hipError_t e = hipMemcpyAsync((char *)params.constants + rt::K_DISPLAY * 16, row, 16, hipMemcpyHostToDevice,
+                                  device->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(device->stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "display settings");
+}
+
+void RenderJob::set_display_row(const float row[4]) {
+    for (int k = 0; k < 4; k++) constants[rt::K_DISPLAY * 4 + k] = row[k];
 }
 
 size_t RenderJob::keep_image_only() {
@@ -167,7 +209,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     uint32_t stack = st->trace_stack ? st->trace_stack : scene->config.stack;
     uint32_t group = st->trace_group ? st->trace_group : scene->config.group;
     uint32_t bound = scene->info.stack_bound[width_index(w)];
-    if (bound > stack) b += (size_t)(bound - stack) * grid_blocks(s.slots, group) * group * 4;
+    if (bound > stack) b += (size_t)(bound - stack) * spill_columns(s.slots, group) * 4;
     return b;
 }
 
@@ -273,8 +315,9 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     texture_count = loaded->texture_count;
     light_paths = loaded->light_paths;
     through = loaded->through;
+    std::memcpy(constants, loaded->constants, sizeof(constants));
     p.lights = (const float4 *)upload(loaded->lights, loaded->light_floats * 4);
-    p.constants = (const float4 *)upload(loaded->constants, 128 * 4);
+    p.constants = (const float4 *)upload(constants, 128 * 4);
     p.materials = (const float4 *)upload(loaded->materials, loaded->material_floats * 4);
     p.emitters = (const uint32_t *)upload(loaded->emitters, loaded->emitter_words * 4);
     if (rc) return rc;
@@ -344,7 +387,8 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     p.scene.nodes = (const uint8_t *)sc->wide[layout_index(w)];
     uint32_t bound = sc->info.stack_bound[width_index(w)];
     uint64_t depth = bound > stack ? bound - stack : 0;
-    uint64_t threads = (uint64_t)grid_blocks(slots, group) * group;
+    uint64_t threads = spill_columns(slots, group);
+    if (threads > 0xFFFFFFFFull || threads < slots) return fail(LR_LIMIT, "too many path slots for the spill area");
     p.scene.spill = depth ? (uint32_t *)alloc(depth * threads * 4, rc) : nullptr;
     p.scene.spill_stride = (uint32_t)threads;
     p.scene.spill_depth = (uint32_t)depth;

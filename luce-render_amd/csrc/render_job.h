@@ -42,6 +42,16 @@ struct RenderJob {
     // What the view's buffers hold room for (pixels; path slots).
     uint32_t cap_pixels = 0, cap_slots = 0;
     uint32_t cap_width = 0, cap_height = 0;  // the view the buffers were made for
+    // The constants the device holds (a host copy): set_display writes its
+    // row there too, so a restart of the same loaded arrays keeps it.
+    float constants[128] = {};
+    // The display row (exposure, view) set by set_display.
+    void set_display_row(const float row[4]);
+    // Whether two constant arrays differ in the view rows (12..15) and the
+    // display row at most: a restart of the same loaded arrays.
+    static bool same_but_view(const float a[128], const float b[128]);
+    // A job made again for a restart takes the held job's display row.
+    int adopt_display(const RenderJob &held);
 
     ~RenderJob();
     void *alloc(size_t bytes, int &rc);
@@ -65,6 +75,7 @@ struct RenderJob {
 };
 
 uint32_t grid_blocks(uint64_t items, uint32_t group);
+uint64_t spill_columns(uint64_t slots, uint32_t group);
 
 // Launches one pass of every render in `jobs` (sharing each launch: grid row
 // y is render y), their parameters at `params` (a device array in job

@@ -17,6 +17,9 @@ struct TraceVariant {
                    hipStream_t stream);
     int (*any)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint32_t *out, uint32_t refill,
                hipStream_t stream);
+    // Workgroups a persistent (refilled) launch over `items` rays takes on
+    // the current device: the closest-hit kernel, or the any-hit one.
+    uint32_t (*resident)(uint64_t items, int any);
 };
 
 // 45 variants: index (layout 2/4/8/20/24, stack 16/24/32, group 64/128/256).

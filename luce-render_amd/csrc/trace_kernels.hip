@@ -3,6 +3,10 @@
 // LDS stack depth S and workgroup size G so the autotuner can pick a variant.
 #include "trace_kernels.h"
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "lr_scene.h"
 
 // Kernels receive the scene by value; each launcher stamps the spill stride
@@ -152,19 +156,27 @@ __global__ __launch_bounds__(G) void any_stream_kernel(DeviceScene sc, const flo
 namespace {
 
 // Workgroups of a persistent launch: as many as are resident at once, and no
-// more than the items need.
-template <class K> uint32_t resident_blocks(K kernel, uint32_t group, uint64_t items) {
-    static int per_cu = -1;
-    static int cus = 0;
-    if (per_cu < 0) {
-        int dev = 0, b = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, (int)group, 0) != hipSuccess || b < 1) b = 1;
-        per_cu = b;
+// more than the items need. The residency is asked of the runtime once per
+// kernel (each variant has its own registers and LDS) and device.
+uint32_t resident_blocks(const void *kernel, uint32_t group, uint64_t items) {
+    static std::mutex lock;
+    static std::map<std::pair<int, const void *>, std::pair<int, int>> cache;  // (device, kernel) -> (per CU, CUs)
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::pair<int, int> occ;
+    {
+        std::lock_guard<std::mutex> g(lock);
+        auto it = cache.find({dev, kernel});
+        if (it == cache.end()) {
+            int cus = 0, b = 0;
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, (int)group, 0) != hipSuccess || b < 1) b = 1;
+            it = cache.emplace(std::make_pair(dev, kernel), std::make_pair(b, cus > 0 ? cus : 1)).first;
+        }
+        occ = it->second;
     }
     uint64_t need = (items + group - 1) / group;
-    uint64_t resident = (uint64_t)per_cu * (uint64_t)(cus > 0 ? cus : 1);
+    uint64_t resident = (uint64_t)occ.first * (uint64_t)occ.second;
     return (uint32_t)(need < resident ? need : resident);
 }
 
@@ -188,7 +200,7 @@ template <int W, int S, int G> int launch_closest_t(const DeviceScene &scene, co
                                                      uint2 *out, uint32_t refill, hipStream_t stream) {
     if (n == 0) return 0;
     if (refill && scene.work && n < 0x80000000u) {
-        uint32_t blocks = resident_blocks(closest_stream_kernel<W, S, G>, G, n);
+        uint32_t blocks = resident_blocks((const void *)closest_stream_kernel<W, S, G>, G, n);
         DeviceScene sc = scene;
         sc.spill_stride = blocks * G;
         hipLaunchKernelGGL((closest_stream_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, rays, n, out,
@@ -206,7 +218,7 @@ template <int W, int S, int G> int launch_any_t(const DeviceScene &scene, const 
                                                  uint32_t refill, hipStream_t stream) {
     if (n == 0) return 0;
     if (refill && scene.work && n < 0x80000000u) {
-        uint32_t blocks = resident_blocks(any_stream_kernel<W, S, G>, G, n);
+        uint32_t blocks = resident_blocks((const void *)any_stream_kernel<W, S, G>, G, n);
         DeviceScene sc = scene;
         sc.spill_stride = blocks * G;
         hipLaunchKernelGGL((any_stream_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, rays, n, out, refill);
@@ -219,7 +231,13 @@ template <int W, int S, int G> int launch_any_t(const DeviceScene &scene, const 
     return (int)hipGetLastError();
 }
 
+template <int W, int S, int G> uint32_t resident_t(uint64_t items, int any) {
+    return any ? resident_blocks((const void *)any_stream_kernel<W, S, G>, G, items)
+               : resident_blocks((const void *)closest_stream_kernel<W, S, G>, G, items);
+}
+
 template <int W, int S, int G> void fill(TraceVariant &v) {
+    v.resident = &resident_t<W, S, G>;
     v.probe = &launch_probe_t<W, S, G>;
     v.closest = &launch_closest_t<W, S, G>;
     v.any = &launch_any_t<W, S, G>;

@@ -131,6 +131,15 @@ class GpuScene:
         _lib.check(_lib.lib().lr_scene_get_trace_config(self.handle, ctypes.byref(cfg)))
         return (cfg.width, cfg.group, cfg.stack, cfg.refill)
 
+    def resident_grid(self, rays, any_hit=False):
+        """Workgroups a refilled launch of the current traversal variant takes
+        over `rays` rays: that variant's resident workgroups, at most what the
+        rays need."""
+        blocks = ctypes.c_uint32()
+        _lib.check(_lib.lib().lr_trace_resident_grid(self.handle, int(rays), 1 if any_hit else 0,
+                                                     ctypes.byref(blocks)))
+        return blocks.value
+
     def info(self):
         info = SceneInfo()
         _lib.check(_lib.lib().lr_scene_info_get(self.handle, ctypes.byref(info)))

@@ -158,8 +158,29 @@ def test_graph_request_served_beside_plain_requests(device):
     (alone_graph,) = serve([graph_job])
     (alone_plain,) = serve([plain_job])
     mixed = serve([plain_job, graph_job])
+
+    # The server copies a request's programs and textures at submission: the
+    # caller's arrays are gone (collected and overwritten) before the step
+    # that admits it, and it still renders the same bits.
+    import gc
+    server = serving.Server(device, budget_bytes=8 << 30)
+    try:
+        k, w, h, spp = graph_job
+        rid = server.submit(gpus[k], sets[k], RenderSettings(w, h, spp, max_bounces=6, spectral=False,
+                                                              light_tracing=0.0))
+        gc.collect()
+        churn = [np.full(1 << 16, 7.0e30, np.float32) for _ in range(64)]
+        del churn
+        gc.collect()
+        while server.busy() or server.status(rid)["state"] != 2:
+            server.step()
+        late = server.read(rid, w, h).copy()
+        server.release(rid)
+    finally:
+        server.destroy()
     for g in gpus:
         g.destroy()
+    assert np.array_equal(late, alone_graph)
     assert np.array_equal(mixed[1], alone_graph)
     assert np.allclose(mixed[0], alone_plain, rtol=1e-6, atol=1e-7)
     assert float(alone_graph[:, :, :3].mean()) > 0.0

@@ -181,3 +181,74 @@ def test_replace_scene_equals_fresh_render(device):
         r.destroy()
     ref = _fresh(device, other, dataclasses.replace(st2, width=40, height=30), frame2)
     assert np.array_equal(got, ref)
+
+
+def test_restart_keeps_the_display_settings(device):
+    """A restart keeps what set_display wrote (integrator.lucb:463-466 uploads
+    the view rows alone; :753-756 writes the display row into the loaded
+    constants): after a restart in place, and after one whose view outgrows
+    the buffers, the display equals that of a fresh render with the same
+    exposure set before its first pass."""
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.cornell_render_set())
+    st = _settings(samples=3)
+    first = render.ray_frame(sc.cameras[0], 48, 40)
+    moved = render.look_frame((0.3, 0.2, 2.8), (-0.1, -0.05, -1.0), (0.0, 1.0, 0.0), 0.45, 48, 40)
+    grown = render.look_frame((0.3, 0.2, 2.8), (-0.1, -0.05, -1.0), (0.0, 1.0, 0.0), 0.45, 96, 64)
+
+    def fresh_display(frame):
+        r = render.Renderer.create_viewed(device, sc, dataclasses.replace(st, width=frame.width,
+                                                                           height=frame.height), frame)
+        try:
+            r.set_display(1.0, 1)
+            r.finish()
+            return r.read_display().copy()
+        finally:
+            r.destroy()
+
+    r = render.Renderer.create_viewed(device, sc, st, first)
+    try:
+        r.set_display(1.0, 1)
+        r.finish()
+        r.restart(moved)
+        r.finish()
+        in_place = r.read_display().copy()
+        r.restart(grown)
+        r.finish()
+        regrown = r.read_display().copy()
+    finally:
+        r.destroy()
+    assert np.array_equal(in_place, fresh_display(moved))
+    assert np.array_equal(regrown, fresh_display(grown))
+    plain = _fresh(device, sc, st, moved)
+    assert plain.shape == in_place.shape
+
+
+def test_a_view_of_more_slots_than_one_launch_grid_spans(device):
+    """A 3840x2160 render in one band (8.3M path slots, more than the
+    4.19M threads of 65,535 workgroups of 64) over a tree whose stack bound
+    exceeds the LDS stack, so rays spill: every slot has its own spill column,
+    and the one-band render equals the same render in four bands bit for bit
+    (camera paths only)."""
+    from luce_render import render, scenes
+    sc = render.compile_scene(scenes.cornell_render_set(random_count=10000))
+    gpu = render.GpuScene.upload(device, sc)
+    try:
+        bound = gpu.info()["stack_bound"]
+        say(f"stack bounds {bound}")
+        assert max(bound) > 16
+        frame = render.ray_frame(sc.cameras[0], 3840, 2160)
+        st = _settings(width=3840, height=2160, samples=1, max_bounces=2, trace=(2, 64, 16))
+        images = []
+        for pool in (3840 * 2160, 1 << 21):
+            say(f"3840x2160 pool {pool}")
+            r = render.Renderer.create_viewed(device, sc, dataclasses.replace(st, pool=pool), frame, gpu_scene=gpu)
+            try:
+                r.finish()
+                images.append(r.read().copy())
+            finally:
+                r.destroy()
+    finally:
+        gpu.destroy()
+    assert np.isfinite(images[0]).all() and images[0][..., :3].max() > 0
+    assert np.array_equal(images[0], images[1])

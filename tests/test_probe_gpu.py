@@ -189,3 +189,28 @@ def test_probe_count_must_match(device, probe_scene):
     with pytest.raises(render.LuceError) as e:
         render.probe(device, probe_scene, frame, out=[None] * 10)
     assert e.value.code == 400
+
+
+def test_each_variant_sizes_its_own_persistent_grid(device, cornell_probe):
+    """A refilled launch takes the resident workgroups of its own kernel:
+    variants of different group sizes (asked in either order) get different
+    grids, each at most what the rays need, and asking again gives the
+    same answer."""
+    from luce_render import render
+    gpu = render.GpuScene.upload(device, cornell_probe)
+    try:
+        grids = {}
+        for g in (64, 256, 128, 64):
+            gpu.set_trace_config(20, g, 16, 32)
+            grids.setdefault(g, []).append((gpu.resident_grid(1 << 26), gpu.resident_grid(1 << 26, any_hit=True)))
+            assert gpu.resident_grid(100) == 2 if g == 64 else gpu.resident_grid(100) == 1
+        say(f"resident grids {grids}")
+        assert grids[64][0] == grids[64][1]
+        for any_hit in (0, 1):
+            sizes = {g: grids[g][0][any_hit] for g in (64, 128, 256)}
+            assert len(set(sizes.values())) == 3
+            threads = {g: sizes[g] * g for g in sizes}
+            # no more waves than a compute unit holds (32 of 64 lanes)
+            assert max(threads.values()) <= 2048 * 256
+    finally:
+        gpu.destroy()
