@@ -178,6 +178,18 @@ int lr_trace_closest_device(const lr_gpu_scene *scene, const float *rays_device,
 int lr_trace_any_device(const lr_gpu_scene *scene, const float *rays_device, size_t n,
                         uint32_t *occluded_device, void *stream);
 
+/* lr_trace_closest_device without allocating: no reference counterpart. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same hits, bit
+ * for bit, by persistent refilled waves whose spill area and counters live
+ * in a 32 MiB array the library holds on each device from load to process
+ * end; the grid is cut to the workgroups whose spill columns fit it. Calls
+ * that overlap on one device race on that array (serialize them on one
+ * stream). Fails with LR_LIMIT for n >= 2^31 or when one workgroup's spill
+ * columns exceed the array. */
+int lr_trace_closest_device_noalloc(const lr_gpu_scene *scene, const float *rays_device, size_t n,
+                                    uint32_t *hits_device, void *stream);
+
 /* Workgroups a refilled (persistent) launch of the scene's traversal variant
  * takes over n rays on its device: the resident workgroups of that variant's
  * kernel (closest hit, or any hit when `any`), at most what n rays need. No

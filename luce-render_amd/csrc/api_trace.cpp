@@ -124,6 +124,7 @@ int lr_device_open(int ordinal, lr_device **out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ordinal) == hipSuccess) d->compute_units = prop.multiProcessorCount;
+    scratch_attach(d);
     *out = d;
     return 0;
 }

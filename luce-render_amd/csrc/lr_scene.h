@@ -10,7 +10,17 @@ struct lr_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;
     int compute_units = 0;
+    // This device's copy of the allocation-free entry point's scratch array
+    // (its device address; null, with scratch_error set, when the runtime
+    // could not give it).
+    void *scratch = nullptr;
+    int scratch_error = 0;
 };
+
+namespace lr {
+// Looks up the scratch array's address on `device` (current), once, at open.
+void scratch_attach(lr_device *device);
+}  // namespace lr
 
 struct lr_gpu_scene {
     lr_device *device = nullptr;

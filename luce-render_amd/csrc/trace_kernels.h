@@ -20,6 +20,11 @@ struct TraceVariant {
     // Workgroups a persistent (refilled) launch over `items` rays takes on
     // the current device: the closest-hit kernel, or the any-hit one.
     uint32_t (*resident)(uint64_t items, int any);
+    // Closest hits by persistent waves (refill 1..64) on at most `max_blocks`
+    // workgroups (fewer when fewer are resident or needed); sc.work and
+    // sc.spill (for the grid launched) are the caller's.
+    int (*closest_persistent)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint2 *out, uint32_t refill,
+                              uint32_t max_blocks, hipStream_t stream);
 };
 
 // 45 variants: index (layout 2/4/8/20/24, stack 16/24/32, group 64/128/256).

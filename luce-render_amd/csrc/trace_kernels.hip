@@ -214,6 +214,19 @@ template <int W, int S, int G> int launch_closest_t(const DeviceScene &scene, co
     return (int)hipGetLastError();
 }
 
+template <int W, int S, int G>
+int launch_closest_persistent_t(const DeviceScene &scene, const float4 *rays, uint32_t n, uint2 *out, uint32_t refill,
+                                uint32_t max_blocks, hipStream_t stream) {
+    if (n == 0) return 0;
+    uint32_t blocks = resident_blocks((const void *)closest_stream_kernel<W, S, G>, G, n);
+    if (blocks > max_blocks) blocks = max_blocks;
+    if (blocks == 0) return (int)hipErrorInvalidValue;
+    DeviceScene sc = scene;
+    sc.spill_stride = blocks * G;
+    hipLaunchKernelGGL((closest_stream_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, rays, n, out, refill);
+    return (int)hipGetLastError();
+}
+
 template <int W, int S, int G> int launch_any_t(const DeviceScene &scene, const float4 *rays, uint32_t n, uint32_t *out,
                                                  uint32_t refill, hipStream_t stream) {
     if (n == 0) return 0;
@@ -238,6 +251,7 @@ template <int W, int S, int G> uint32_t resident_t(uint64_t items, int any) {
 
 template <int W, int S, int G> void fill(TraceVariant &v) {
     v.resident = &resident_t<W, S, G>;
+    v.closest_persistent = &launch_closest_persistent_t<W, S, G>;
     v.probe = &launch_probe_t<W, S, G>;
     v.closest = &launch_closest_t<W, S, G>;
     v.any = &launch_any_t<W, S, G>;

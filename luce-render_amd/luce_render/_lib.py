@@ -111,6 +111,7 @@ _SIGNATURES = {
     "lr_trace_closest_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
     "lr_trace_any_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
     "lr_trace_resident_grid": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int, _P]),
+    "lr_trace_closest_device_noalloc": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
     "lr_scene_info_get": (ctypes.c_int, [_P, ctypes.POINTER(SceneInfo)]),
     "lr_renderer_create": (ctypes.c_int, [_P, _P, ctypes.POINTER(Loaded), ctypes.POINTER(RenderSettingsC), _P,
                                           ctypes.POINTER(_P)]),

@@ -343,6 +343,17 @@ def trace_closest_device(scene: GpuScene, rays: Buffer, n, hits: Buffer, stream=
     _lib.check(_lib.lib().lr_trace_closest_device(scene.handle, rays.ptr, int(n), hits.ptr, stream))
 
 
+def trace_closest_device_noalloc(scene: GpuScene, rays: Buffer, n, hits: Buffer, stream=None):
+    """trace_closest_device without allocating device memory: the library's
+    fixed scratch array holds the traversal's spill area and counters (calls
+    on one device must not overlap)."""
+    if rays.nbytes < n * 32 or hits.nbytes < n * 8:
+        raise LuceError(INVALID, "ray or hit buffer too small")
+    if stream is None:
+        stream = device_stream(scene.device)
+    _lib.check(_lib.lib().lr_trace_closest_device_noalloc(scene.handle, rays.ptr, int(n), hits.ptr, stream))
+
+
 def trace_any_device(scene: GpuScene, rays: Buffer, n, occluded: Buffer, stream=None):
     """Any-hit occlusion of n rays (8 floats each) before their t_max."""
     if rays.nbytes < n * 32 or occluded.nbytes < n * 4:
