@@ -50,6 +50,78 @@ def measured_traffic(kernel):
     return None
 
 
+def host_cpu():
+    """The host's CPU model, its logical CPUs and the ones this process may use."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"model": model, "nproc": os.cpu_count() or 1, "usable": usable}
+
+
+def timed_median(fn, runs):
+    """One warm-up call, then the median wall time of `runs` calls (steady clock)."""
+    fn()
+    times = []
+    for _ in range(runs):
+        t1 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t1)
+    return float(np.median(times)), times
+
+
+def cpu_protocol(O, render, scenes, threads, full=False, runs=5):
+    """The reference's CPU path, timed on this host: the f64 brute-force
+    intersector (tests/gpu/main.lucb:82-119) on the 256x256 Cornell probe with
+    its 10,000 random triangles (65,536 x 10,034 ray-triangle tests) on all
+    usable cores (rows split over threads) and on one core, and the
+    single-threaded binned-SAH build (src/render/bvh.lucb:66-217, restated in
+    the oracle) on that scene and on the 2M-unique-triangle geometry of the
+    large scene (its 64 icospheres once each). One warm-up, then the median of
+    `runs`. The one-core brute force runs on the image's first 4,096 rays
+    unless `full` (the rate is per ray: every ray tests every triangle); with
+    `full` the build also runs once on the realized 30.4M triangles."""
+    out = {"cpu": host_cpu(), "threads": threads, "runs": runs}
+    c1 = render.compile_scene(scenes.cornell_probe_set())
+    f64 = O.ray_frame(c1.cameras[0].matrix, False, 36.0, 30.0, 256, 256)
+    tests = 256 * 256 * c1.triangle_count()
+    med, _ = timed_median(lambda: O.brute_force(c1.positions, c1.indices, c1.faces, frame=f64, width=256, height=256,
+                                                threads=threads), runs)
+    out["brute_force_c1_all_cores"] = {"threads": threads, "s": round(med, 4), "mrays_s": round(65536 / med / 1e6, 5),
+                                       "ns_per_test": round(med * 1e9 * threads / tests, 3)}
+    rows = 256 if full else 16
+    d = np.array([O.frame_ray(f64, x + 0.5, y + 0.5)[1] for y in range(rows) for x in range(256)], np.float64)
+    o = np.tile(np.asarray(O.frame_ray(f64, 0.5, 0.5)[0], np.float64), (len(d), 1))
+    r64 = np.concatenate([o, d], 1)
+    med1, _ = timed_median(lambda: O.brute_force(c1.positions, c1.indices, c1.faces, rays=r64, threads=1), runs)
+    out["brute_force_c1_one_core"] = {"rays": len(d), "s": round(med1, 4), "mrays_s": round(len(d) / med1 / 1e6, 6),
+                                      "ns_per_test": round(med1 * 1e9 / (len(d) * c1.triangle_count()), 3),
+                                      "s_for_65536_rays": round(med1 * 65536 / len(d), 2)}
+    inv = np.empty_like(c1.order)
+    inv[c1.order] = np.arange(len(c1.order), dtype=np.uint32)
+    idx1 = np.ascontiguousarray(c1.indices[inv])
+    med, _ = timed_median(lambda: O.bvh_build(c1.positions, idx1), runs)
+    nodes1 = O.bvh_build(c1.positions, idx1)[0]
+    out["bvh_build_c1_one_thread"] = {"triangles": c1.triangle_count(), "s": round(med, 5), "nodes": len(nodes1)}
+    c3 = render.compile_scene(scenes.instanced_set(instances=64), geometry=False)
+    med, _ = timed_median(lambda: O.bvh_build(c3.positions, c3.indices), runs)
+    nodes3 = O.bvh_build(c3.positions, c3.indices)[0]
+    out["bvh_build_c3_unique_one_thread"] = {"triangles": c3.triangle_count(), "s": round(med, 3),
+                                             "nodes": len(nodes3)}
+    if full:
+        big = render.compile_scene(scenes.instanced_set(instances=1000), geometry=False)
+        t1 = time.perf_counter()
+        nodes = O.bvh_build(big.positions, big.indices)[0]
+        out["bvh_build_c3_realized_one_thread"] = {"triangles": big.triangle_count(),
+                                                   "s": round(time.perf_counter() - t1, 2), "nodes": len(nodes)}
+    return out
+
+
 def bounce_rays(sc, frame, faces_t, rng):
     """Cosine-distributed rays from each primary hit (8 floats a ray)."""
     n = WIDTH * HEIGHT
@@ -125,6 +197,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--instances", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU protocol in full: one-core brute force over all 65,536 probe rays and a one-thread "
+                         "build of the realized 30.4M triangles")
     ap.add_argument("--config", default="", help="width,group,stack[,refill] to skip the sweep")
     ap.add_argument("--render-spp", type=int, default=16,
                     help="samples of the full path-traced render of the same scene (0: skip)")
@@ -366,19 +441,25 @@ def main():
             log(f"guiding {name}:", guiding_stats[name])
 
     cpu = None
+    cpu_detail = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        m = 1024
+        host = host_cpu()
+        threads = max(1, min(16, host["usable"]))
+        m = 512
         sel = rng.choice(n, m, replace=False)
         ys2, xs2 = np.divmod(sel, WIDTH)
         d64 = frame.corner[None, :] + (xs2 + 0.5)[:, None] * frame.du[None, :] + (ys2 + 0.5)[:, None] * frame.dv[None, :]
         r64 = np.concatenate([np.tile(frame.position, (m, 1)), d64], 1)
-        t1 = time.time()
+        t1 = time.perf_counter()
         O.brute_force(sc.positions, sc.indices, sc.faces, rays=r64, threads=threads)
-        secs = time.time() - t1
+        secs = time.perf_counter() - t1
         cpu = {"value": m / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"{m} primary rays of the 1920x1080 frame, f64 brute force over all "
-                         f"{sc.triangle_count()} triangles (tests/gpu/main.lucb:82-119), {secs:.1f} s"}
+                         f"{sc.triangle_count()} triangles (tests/gpu/main.lucb:82-119), {secs:.1f} s on "
+                         f"{threads} threads of {host['model']} ({host['nproc']} logical CPUs, {host['usable']} usable)"}
+        log("cpu baseline:", cpu)
+        cpu_detail = cpu_protocol(O, render, scenes, threads, full=args.cpu_full)
+        log("cpu protocol:", cpu_detail)
 
     out = {
         "metric": "Mrays/s closest-hit",
@@ -408,6 +489,7 @@ def main():
             "guiding": guiding_stats,
             "scene_build": build_stats,
             "wide_bytes": info["wide_bytes"],
+            "cpu_protocol": cpu_detail,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,

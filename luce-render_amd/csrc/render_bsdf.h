@@ -58,7 +58,7 @@ __device__ __forceinline__ float fresnel_dielectric_average(float eta) {
     return logf((10893.0f * eta - 1438.2f) / (-774.4f * eta * eta + 10212.0f * eta + 1.0f));
 }
 
-// Albedo tables (microfacet.glsl:185-202): E_ggx(mu, r), its average over mu,
+// Albedo tables (microfacet.glsl:65-84): E_ggx(mu, r), its average over mu,
 // and E_dielectric(mu, r, z) with z = sqrt(|eta - 1| / (eta + 1)).
 constexpr uint32_t TABLE_MU = 32, TABLE_ROUGHNESS = 32, TABLE_ETA = 16;
 constexpr float TABLE_Z_MAX = 0.8f, TABLE_MU_MIN = 0.02f;

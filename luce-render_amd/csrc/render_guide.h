@@ -118,7 +118,7 @@ __device__ __forceinline__ V3 guide_sample(const Params &p, uint32_t cell, float
     return vmf_sample(guide_lobe(p, cell, lobe), ux, uy);
 }
 
-// guide.glsl:183-201: credit the lobe that explains d best.
+// guide.glsl:121-139: credit the lobe that explains d best.
 __device__ __forceinline__ void guide_train(const Params &p, uint32_t cell, V3 d, float value, float pdf) {
     if (cell == kNoneU || !(value > 0.0f) || !(pdf > 0.0f) || isinf(value)) return;
     float4 w = guide_weights(p, cell);

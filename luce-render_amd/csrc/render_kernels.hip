@@ -852,7 +852,7 @@ __device__ __forceinline__ float light_sample_ratio(const Params &p, uint32_t em
     return chance * pdf / bsdf_pdf;
 }
 
-// light_paths.glsl:182-340
+// light_paths.glsl:31-189
 template <bool SPECTRAL, bool PROGRAMS>
 __device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t path) {
     State st(p);

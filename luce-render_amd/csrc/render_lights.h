@@ -383,7 +383,7 @@ __device__ __forceinline__ float afar_first_density(const Params &p, V3 x, V3 w,
 __device__ __forceinline__ float safe_sqrt(float x) { return sqrtf(fmaxf(0.0f, x)); }
 
 // pbrt-v4's importance of tree node `node` for point x with normal n
-// (lighttree.glsl:326-354).
+// (lighttree.glsl:18-47).
 __device__ __forceinline__ float tree_importance(const Params &p, uint32_t node, V3 x, V3 n) {
     uint32_t base = (uint32_t)K(p, K_TREE).y + node * 3u;
     float4 a = p.lights[base], b = p.lights[base + 1], c = p.lights[base + 2];
@@ -481,7 +481,7 @@ template <bool SPECTRAL> __device__ __forceinline__ Spec material_color(const Pa
     return spec_of<SPECTRAL>(material_at(p, m, row), material_at(p, m, row + 1u), lambda);
 }
 
-// An emissive triangle sampled uniformly by area (lighttree.glsl:415-435).
+// An emissive triangle sampled uniformly by area (lighttree.glsl:107-129).
 template <bool SPECTRAL>
 __device__ __forceinline__ bool sample_triangle(const Params &p, uint32_t tri, V3 x, float ux, float uy, Spec lambda,
                                                 LightSample &s, Spec &radiance) {

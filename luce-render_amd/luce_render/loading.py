@@ -67,7 +67,7 @@ class OpenPbr:
 
 
 def cauchy_b(m):
-    """materials.lucb:226-232."""
+    """materials.lucb:95-103."""
     if m.transmission_dispersion_scale <= 0.0 or m.transmission_dispersion_abbe_number <= 0.0 or m.transmission_weight <= 0.0:
         return 0.0
     f, c = 486.1, 656.3
@@ -93,7 +93,7 @@ def texture_texels(pixels, color):
 
 
 def pack_material(m, program=-1.0):
-    """pack_material (materials.lucb:236-250): 16 float4."""
+    """pack_material (materials.lucb:105-119): 16 float4."""
     out = []
     colors = [m.base_color, m.specular_color, m.coat_color,
               tuple(c * m.emission_luminance for c in m.emission_color), m.transmission_color]

@@ -25,7 +25,7 @@ from .scenes import Xoshiro256
 
 
 class Server:
-    def __init__(self, device, budget_bytes=200 << 30, spectral=False, trace=(4, 64, 24), max_active=0):
+    def __init__(self, device, budget_bytes=200 << 30, spectral=False, trace=(20, 64, 16), max_active=0):
         self.device = device
         cfg = _lib.ServerSettings(int(budget_bytes), 1 if spectral else 0, trace[0], trace[1], trace[2], max_active)
         h = ctypes.c_void_p()

@@ -173,7 +173,7 @@ def agrees(got_face, got_t, face, dist, margin):
 
 
 def sample4(seed, pixel, index, group):
-    """sampler.glsl:323-331: the four 24-bit integers and their values."""
+    """sampler.glsl:47-55: the four 24-bit integers and their values."""
     bits = np.zeros(4, np.uint32)
     vals = np.zeros(4, np.float64)
     lib().or_sample4(seed, pixel, index, group, _p(bits), _p(vals))
@@ -225,3 +225,30 @@ def run_program(materials, programs, material, x, n, view, uv, corners=None, uvs
 
 def gradient_noise(x, y, z):
     return lib().or_gradient_noise(x, y, z)
+
+
+def adaptive_error(film_sum, film_half):
+    """adaptive.comp:22-28, phase 0: each pixel's error, |I - A| summed over
+    the colour channels over the weight, relative to 1e-4 + sqrt(brightness),
+    in f32 as the kernel computes it. film_sum / film_half: (..., 4) f32."""
+    s = np.asarray(film_sum, np.float32)
+    h = np.asarray(film_half, np.float32)
+    scale = np.float32(1.0) / np.maximum(s[..., 3], np.float32(1.0))
+    diff = (np.abs(s[..., 0] - h[..., 0]) + np.abs(s[..., 1] - h[..., 1]) + np.abs(s[..., 2] - h[..., 2])) * scale
+    bright = np.maximum(np.float32(0.0), (s[..., 0] + s[..., 1] + s[..., 2]) * scale)
+    return (diff / (np.float32(1e-4) + np.sqrt(bright))).astype(np.float32)
+
+
+def adaptive_verdict(film_sum, film_half, threshold):
+    """adaptive.comp:16-42: (own, converged) per pixel of (H, W, 4) planes. A
+    pixel's own verdict is error < threshold (phase 0); it is converged when
+    its whole 3 x 3 neighbourhood, clipped at the image's edges, is (phase 1)."""
+    own = adaptive_error(film_sum, film_half) < np.float32(threshold)
+    h, w = own.shape
+    padded = np.ones((h + 2, w + 2), bool)
+    padded[1:-1, 1:-1] = own
+    conv = np.ones((h, w), bool)
+    for dy in (0, 1, 2):
+        for dx in (0, 1, 2):
+            conv &= padded[dy:dy + h, dx:dx + w]
+    return own, conv

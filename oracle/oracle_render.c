@@ -10,10 +10,10 @@
  *   wavelengths/film shaders/spectrum.glsl:10-118, finish.comp:17-27,
  *                    film_convert.comp:36-53
  *   camera           shaders/camera.comp:14-50
- *   closest + lights shaders/intersect_closest.comp:102-133, lights.glsl
+ *   closest + lights shaders/intersect_closest.comp:45-87, lights.glsl
  *   surface shading  shaders/shade_surface.comp:48-312 (light tracing and
  *                    guiding off: the two are checked statistically)
- *   light tree       shaders/lighttree.glsl:326-445
+ *   light tree       shaders/lighttree.glsl:18-137
  *   OpenPBR / GGX    shaders/openpbr.glsl:100-556, microfacet.glsl:9-84;
  *                    albedo tables by tables.comp's estimator, in f64
  *   miss             shaders/shade_miss.comp:12-51
@@ -129,7 +129,7 @@ static uint32_t sobol(uint32_t index, uint32_t d) {
     for (uint32_t bit = 0; index; bit++, index >>= 1) if (index & 1u) x ^= SOBOL[d * 32 + bit];
     return x;
 }
-/* sampler.glsl:323-331; the four 24-bit integers and their values. */
+/* sampler.glsl:47-55; the four 24-bit integers and their values. */
 void or_sample4(uint32_t seed0, uint32_t pixel, uint32_t index, uint32_t group, uint32_t bits[4], double out[4]) {
     uint32_t seed = hash_combine(hash_combine(seed0, pixel), group);
     uint32_t shuffled = nus(index, or_hash_u32(seed));
@@ -465,7 +465,7 @@ static double light_hit_pdf(const oscene *s, uint32_t l, v3 o, v3 d, double t) {
     return t * t / (cl * L(s, l, 3, 3));
 }
 
-/* light tree (lighttree.glsl:326-402) */
+/* light tree (lighttree.glsl:18-95) */
 static double tree_importance(const oscene *s, uint32_t node, v3 x, v3 n) {
     uint32_t base = (uint32_t)K(s, 8, 1) + node * 3;
     const float *a = &s->lights[base * 4], *b = &s->lights[(base + 1) * 4], *c = &s->lights[(base + 2) * 4];

@@ -1,7 +1,7 @@
 """Size-independent properties at the benchmark's full size (the instanced
 30.4M-triangle scene, 1920x1080): every traversal width and ray schedule
-returns the same polygon ids and distance bits on all 2,073,600 primary and
-bounce rays; any-hit occlusion equals "a closest hit exists before t_max";
+and LDS stack (16, the headline configuration's, and 24) returns the same
+polygon ids and distance bits on all 2,073,600 primary and bounce rays; any-hit occlusion equals "a closest hit exists before t_max";
 the closest hits of a sampled subset equal the oracle's f32 mirror."""
 import numpy as np
 import pytest
@@ -42,10 +42,10 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
     rays[1::3, 3] = 2.5  # some finite t_max
     rb = render.Buffer.create(device, rays)
     ref = None
-    for width in (2, 4, 8, 20, 24):
+    for width, stack in [(w, s) for w in (2, 4, 8, 20, 24) for s in (16, 24)]:
         for refill in (0, 32):
-            g.set_trace_config(width, 64, 24, refill)
-            say(f"full size: width {width} refill {refill}")
+            g.set_trace_config(width, 64, stack, refill)
+            say(f"full size: width {width} stack {stack} refill {refill}")
             render.probe_device(g, frame, hp)
             render.trace_closest_device(g, rb, n, hb)
             render.trace_any_device(g, rb, n, ob)
@@ -57,8 +57,8 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
             if ref is None:
                 ref = got
             else:
-                assert np.array_equal(got[0], ref[0]), (width, refill)
-                assert np.array_equal(got[1], ref[1]), (width, refill)
+                assert np.array_equal(got[0], ref[0]), (width, stack, refill)
+                assert np.array_equal(got[1], ref[1]), (width, stack, refill)
     # a sample of the bounce rays against the oracle's mirror of bvh.glsl
     rng = np.random.default_rng(3)
     pick = rng.choice(n, 4096, replace=False)

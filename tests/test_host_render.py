@@ -101,12 +101,31 @@ def _sobol_directions():
     return rows
 
 
+def _table_in(path, name):
+    """The 128 words of the table `name` in a C/HIP source of this build."""
+    import re
+    text = path.read_text()
+    body = text[text.index(name):]
+    body = body[body.index("{") + 1:body.index("};")]
+    return [int(w, 16) for w in re.findall(r"0x([0-9a-fA-F]{8})u", body)]
+
+
 def test_sobol_table_is_joe_kuo():
-    """The kernels' and the oracle's direction table (shaders/sobol_table.glsl,
-    a reference-held vector) regenerated from Joe and Kuo's parameters."""
+    """The kernels' and the oracle's direction tables equal, word for word,
+    the reference's committed table (shaders/sobol_table.glsl, kept as the
+    fixture tests/golden/sobol_directions.json), and that table is Joe and
+    Kuo's, regenerated from their parameters."""
+    import json
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    fixture = json.loads((root / "tests" / "golden" / "sobol_directions.json").read_text())
+    ref = [int(w, 16) for w in fixture["directions"]]
+    assert len(ref) == 128
     rows = _sobol_directions()
-    assert rows[1][:4] == [0x80000000, 0xC0000000, 0xA0000000, 0xF0000000]
-    assert rows[3][31] == 0x50050093 and rows[2][31] == 0xC5005555
+    assert [w for d in range(4) for w in rows[d][:32]] == ref
+    kernels = _table_in(root / "luce-render_amd" / "csrc" / "render_common.h", "kSobol[128]")
+    oracle = _table_in(root / "oracle" / "oracle_render.c", "SOBOL[128]")
+    assert kernels == ref and oracle == ref
 
 
 def test_sampler_stratifies_every_dimension():

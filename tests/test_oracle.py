@@ -130,6 +130,12 @@ def test_mirror_agrees_with_brute_force_on_cornell_probe(cornell_probe):
     assert (bf >= 0).sum() > 256 * 256 // 2
     assert differ.sum() == 0
     assert counters[3] == 0  # no stack entry dropped
+    # The stricter bound: every pixel's polygon equal to the brute force's,
+    # and every hit distance within 1e-5 relative of its f64 distance.
+    hit = bf >= 0
+    assert np.array_equal(np.where(face == 0xFFFFFFFF, -1, face.astype(np.int64)), bf)
+    rel = np.abs(t[hit].astype(np.float64) - bd[hit]) / bd[hit]
+    assert rel.max() <= 1e-5, rel.max()
 
 
 def test_mirror_planted_mistake_is_caught(cornell_probe):
@@ -145,3 +151,34 @@ def test_mirror_planted_mistake_is_caught(cornell_probe):
     bf, bd, bm = O.brute_force(sc.positions, sc.indices, sc.faces, frame=f64, width=64, height=64, threads=8)
     _, differ = O.agrees(face, t, bf, bd, bm)
     assert differ.sum() > 0
+
+
+def test_adaptive_verdict_closed_forms():
+    """The oracle's convergence check (adaptive.comp:16-42) on films whose
+    errors are known in closed form: a pixel whose half estimate equals its
+    full one has error 0; one whose channels differ by d at mean brightness b
+    over weight w has error 3d/w / (1e-4 + sqrt(3b/w)); a single unconverged
+    pixel unconverges exactly its 3 x 3 neighbourhood, clipped at the edges."""
+    h, w = 6, 7
+    s = np.zeros((h, w, 4), np.float32)
+    s[..., :3] = 8.0
+    s[..., 3] = 16.0
+    half = s.copy()
+    half[..., 3] = 0.0
+    err = O.adaptive_error(s, half)
+    assert np.all(err == 0.0)
+    half[2, 3, :3] += 0.8  # d = 0.8 on each channel
+    want = (3 * 0.8 / 16.0) / (1e-4 + np.sqrt(3 * 8.0 / 16.0))
+    err = O.adaptive_error(s, half)
+    assert abs(float(err[2, 3]) - want) < 1e-6 and (err > 0).sum() == 1
+    own, conv = O.adaptive_verdict(s, half, want * 0.99)
+    assert own.sum() == h * w - 1
+    expect = np.ones((h, w), bool)
+    expect[1:4, 2:5] = False
+    assert np.array_equal(conv, expect)
+    half[0, 0, :3] += 0.8  # a corner: its clipped neighbourhood is 2 x 2
+    _, conv = O.adaptive_verdict(s, half, want * 0.99)
+    expect[0:2, 0:2] = False
+    assert np.array_equal(conv, expect)
+    _, conv = O.adaptive_verdict(s, half, want * 1.01)
+    assert conv.all()

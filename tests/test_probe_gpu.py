@@ -52,6 +52,13 @@ def test_probe_agrees_with_brute_force(device, cornell_probe, probe_scene):
     same, differ = O.agrees(faces, t, bf, bd, bm)
     assert (bf >= 0).sum() > 256 * 256 // 2
     assert differ.sum() == 0
+    # Hit polygons equal the brute force's on every pixel and distances lie
+    # within 1e-5 relative of its f64 distances.
+    hit = bf >= 0
+    assert np.array_equal(np.where(faces == 0xFFFFFFFF, -1, faces.astype(np.int64)), bf)
+    rel = np.abs(t[hit].astype(np.float64) - bd[hit]) / bd[hit]
+    say(f"probe vs brute force: max relative distance difference {rel.max():.3e}")
+    assert rel.max() <= 1e-5
 
 
 def test_probe_orthographic(device, cornell_probe, probe_scene):

@@ -212,6 +212,43 @@ def test_adaptive_sampling_stops_converged_pixels(device):
     assert np.all(np.abs(img[16:32, 24:40, :3].mean(axis=(0, 1)) - 0.8) < 0.01)
 
 
+def test_adaptive_verdict_matches_the_oracle_on_a_noisy_and_smooth_image(device):
+    """adaptive.comp on the Cornell box (smooth lit walls, noisy shadows and
+    edges) after its one check at 16 samples: each pixel's own verdict (the
+    scratch plane) and its converged flag after the 3 x 3 rule (the half
+    plane's w) equal the oracle's restatement computed from the same film,
+    pixel for pixel; the threshold is the median error, so about half the
+    pixels pass alone and the neighbourhood rule stops fewer of them."""
+    from luce_render import render, scenes
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(scenes.cornell_render_set())
+    films = {}
+    for threshold in (0.0, None):
+        if threshold is None:
+            threshold = float(np.median(O.adaptive_error(films[0.0][0], films[0.0][1])))
+        st = RenderSettings(64, 48, 16, max_bounces=6, spectral=False, light_tracing=0.0,
+                            noise_threshold=threshold, adaptive_minimum=16)
+        r = render.Renderer.create(device, sc, st)
+        try:
+            r.finish()
+            films[threshold] = r.read_film().copy()
+            conv_count = r.converged_pixels()
+        finally:
+            r.destroy()
+    (_, first), (thr, film) = films.items()
+    assert np.array_equal(film[0], first[0]) and np.array_equal(film[1][..., :3], first[1][..., :3])
+    own, conv = O.adaptive_verdict(film[0], film[1], thr)
+    err = O.adaptive_error(film[0], film[1])
+    clear = np.abs(err - np.float32(thr)) > 1e-5 * thr
+    got_own = film[2][..., 0] > 0.5
+    got_conv = film[1][..., 3] > 0.5
+    say(f"adaptive at threshold {thr:.4g}: own {own.sum()}, converged {conv.sum()} (GPU {got_conv.sum()}, "
+        f"counted {conv_count}) of {own.size}")
+    assert 0.3 * own.size < own.sum() < 0.7 * own.size and conv.sum() < own.sum()
+    assert np.array_equal(got_own[clear], own[clear])
+    assert np.array_equal(got_conv[clear], conv[clear]) and conv_count == got_conv.sum()
+
+
 @pytest.mark.parametrize("spectral", [False, True])
 def test_textured_dome_matches_f64_oracle(device, spectral):
     """A textured dome (environment.lucb, lights.glsl:89-175) over a Lambert

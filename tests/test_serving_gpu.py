@@ -119,3 +119,44 @@ def test_poisson_serving_reports_latency(device, scenes):
     rep = serving.run(device, scenes, sched, max_bounces=4, server_kw={"budget_bytes": 8 << 30})
     say(rep.p50_s, rep.p99_s, rep.rays_per_s, rep.peak_active)
     assert rep.requests == 24 and rep.p99_s >= rep.p50_s > 0 and rep.rays > 0
+
+
+@pytest.fixture(scope="module")
+def serving_scenes(device):
+    """The serving workload's four scenes (scenes.serving_sets: the Cornell
+    box with 10,000 random triangles, the plain Cornell render, and the
+    Cornell render with 500,000 and 5,000,000 random triangles), the large
+    ones built on the device."""
+    from luce_render import render, scenes as S
+    out = []
+    for g in S.serving_sets():
+        sc = render.compile_scene(g, geometry=False)
+        gpu = render.GpuScene.build(device, sc)
+        say(f"serving scene: {sc.triangle_count()} triangles")
+        out.append((gpu, sc))
+    yield out
+    for g, _ in out:
+        g.destroy()
+
+
+@pytest.mark.parametrize("lt", [0.0, 0.5])
+def test_serving_scenes_alone_equal_mixed(device, serving_scenes, lt):
+    """The serving workload's scenes at its resolutions, a 1920x1080 request
+    on the 5M-triangle soup among them: each request renders the same alone
+    and beside the others in shared launches (camera paths bit for bit;
+    with light paths, whose splats add by float atomics, within 1e-4 of the
+    image's peak)."""
+    jobs = [(3, 1920, 1080, 2), (2, 512, 512, 4), (0, 256, 256, 6), (1, 1024, 768, 3)]
+    mixed, steps = _serve(device, serving_scenes, jobs, lt, budget=64 << 30)
+    assert steps == 6
+    for k, job in enumerate(jobs):
+        say(f"serving scenes lt {lt}: request {job} alone")
+        (alone,), _ = _serve(device, serving_scenes, [job], lt, budget=64 << 30)
+        got = mixed[k]
+        assert np.isfinite(alone).all() and alone[..., :3].max() > 0.0
+        if lt == 0.0:
+            assert np.array_equal(alone, got), job
+        else:
+            rel = np.abs(alone - got).max() / (np.abs(alone).max() + 1e-12)
+            say(f"  max relative difference {rel:.3g}")
+            assert rel < 1e-4, job
