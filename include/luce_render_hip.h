@@ -302,6 +302,11 @@ int lr_renderer_set_display(lr_renderer *renderer, float exposure, int view);
 /* The film's four planes (sum, half, scratch, splat; common.glsl:64-75),
  * width * height * 16 floats: no reference counterpart, for tests. */
 int lr_renderer_read_film(lr_renderer *renderer, float *planes, size_t count);
+/* The hash-grid guide's buffer (guiding 1; shaders/guide.glsl:12-31's regions:
+ * 2^19 u32 keys, then per cell 4 lobes and their weights as float4, then per
+ * cell 4 x 5 float statistics), after the passes submitted so far. `bytes`
+ * must be its size, 524,288 x 164. */
+int lr_renderer_read_guide(lr_renderer *renderer, void *out, size_t bytes);
 /* Renderer.kernel_times (:873): milliseconds per kernel kind (profile on),
  * in the order camera, intersect closest, shade surface, shade miss,
  * intersect shadow, light emission, finish, film convert, adaptive, rest. */

@@ -17,6 +17,7 @@
 
 #include "lr_internal.h"
 #include "lr_scene.h"
+#include "render_guide.h"
 #include "render_job.h"
 
 using namespace lr;
@@ -261,6 +262,18 @@ int lr_renderer_read_film(lr_renderer *r, float *planes, size_t count) {
     hipError_t e = hipMemcpyAsync(planes, r->job->params.film, pixels * 64, hipMemcpyDeviceToHost, r->device->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
     if (e != hipSuccess) return hip_fail(e, "film read");
+    return 0;
+}
+
+int lr_renderer_read_guide(lr_renderer *r, void *out, size_t bytes) {
+    if (!r || !out) return fail(LR_INVALID, "lr_renderer_read_guide needs a renderer and an output");
+    if (!r->job->params.guide) return fail(LR_INVALID, "the render does not guide with the hash grid");
+    if (bytes != rt::kGuideBytes) return fail(LR_INVALID, "the guide grid's size is lr_renderer_read_guide's bytes");
+    int rc = r->retire(true);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(out, r->job->params.guide, bytes, hipMemcpyDeviceToHost, r->device->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->device->stream);
+    if (e != hipSuccess) return hip_fail(e, "guide read");
     return 0;
 }
 

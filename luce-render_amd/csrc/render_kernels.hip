@@ -1210,7 +1210,13 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(co
                 float4 v = st.v(SV_GUIDE0 + k, path);
                 if (v.y <= 0.0f) continue;
                 V3 dir = octahedral_decode(st.u(SU_GUIDE_DIR0 + k, path));
-                guide_train(p, cell, dir, fmaxf(0.0f, found - v.z) / v.y * v.w, v.x);
+                // The light found past the vertex (finish.comp's found - v.z).
+                // Where nothing came, the two f32 sums differ only by their
+                // rounding; such a difference is no sample (it would count,
+                // and fit a lobe of kappa 1000 to one stray direction).
+                float gain = found - v.z;
+                if (gain <= 1e-5f * fmaxf(found, v.z)) gain = 0.0f;
+                guide_train(p, cell, dir, gain / v.y * v.w, v.x);
             }
         }
     }

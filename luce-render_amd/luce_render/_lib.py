@@ -123,6 +123,7 @@ _SIGNATURES = {
     "lr_renderer_read_display": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "lr_renderer_set_display": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int]),
     "lr_renderer_read_film": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "lr_renderer_read_guide": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "lr_renderer_kernel_times": (ctypes.c_int, [_P, _P]),
     "lr_renderer_destroy": (None, [_P]),
     "lr_renderer_stats": (ctypes.c_int, [_P, ctypes.POINTER(RenderStats)]),

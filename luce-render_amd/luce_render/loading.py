@@ -414,6 +414,9 @@ class RenderSettings:
     guiding: int = 0  # 0 off, 1 the hash grid of vMF mixtures, 2 the learned guide
     train_samples: int = 0  # learned guide: samples that train it (0: 8)
     train_steps: int = 0  # learned guide: Adam steps after each of them (0: 16)
+    # The guide's share of the one-sample mixture (loading.lucb:281: 0.5): of
+    # all BSDF samples for the grid, of the diffuse lobe's for the learned guide.
+    guide_share: float = 0.5
     light_tracing: float = 0.5
     use_graph: bool = True
     trace: tuple = (0, 0, 0)  # traversal width, group, stack (0: the scene's)
@@ -569,7 +572,8 @@ class Loaded:
         # view rows 12..15 (floats 48..63)
         du, dv = np.linalg.norm(frame.du), np.linalg.norm(frame.dv)
         pixel_area = du * dv
-        rows[48:64] = [1.0 if settings.guiding else 0.0, 8.0 * du, 0.5, 16.0, radius * 1e-4, 0.0, 0.0, 0.0,
+        rows[48:64] = [1.0 if settings.guiding else 0.0, 8.0 * du, float(settings.guide_share), 16.0, radius * 1e-4,
+                       0.0, 0.0, 0.0,
                        float(self.light_paths), 1.0 / pixel_area if pixel_area > 0.0 else 0.0, frame.lens_radius,
                        frame.focus_distance, *frame.forward, 0.0]
         self.radius = radius

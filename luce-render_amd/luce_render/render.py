@@ -562,6 +562,20 @@ class Renderer:
         _lib.check(_lib.lib().lr_renderer_read_film(self.handle, _lib.ptr(buf), n))
         return buf.reshape(4, self.settings.height, self.settings.width, 4)
 
+    GUIDE_CELLS = 524288
+
+    def read_guide(self):
+        """The hash-grid guide (guiding 1): keys (cells,) u32, lobes (cells, 5,
+        4) f32 (four lobes' mean and kappa, then their weights) and statistics
+        (cells, 4, 5) f32 (weight, weighted direction, count per lobe)."""
+        n = self.GUIDE_CELLS
+        buf = np.empty(n * 164 // 4, np.uint32)
+        _lib.check(_lib.lib().lr_renderer_read_guide(self.handle, _lib.ptr(buf), buf.nbytes))
+        keys = buf[:n].copy()
+        lobes = buf[n:n + n * 20].view(np.float32).reshape(n, 5, 4).copy()
+        stats = buf[n + n * 20:].view(np.float32).reshape(n, 4, 5).copy()
+        return keys, lobes, stats
+
     def kernel_times(self):
         out = np.zeros(10, np.float64)
         _lib.check(_lib.lib().lr_renderer_kernel_times(self.handle, _lib.ptr(out)))

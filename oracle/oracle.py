@@ -44,6 +44,15 @@ def lib():
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, ctypes.c_size_t,
                                          ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t]),
+            "or_render_guided": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, _P, _P,
+                                                _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P,
+                                                ctypes.c_size_t, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P, _P,
+                                                ctypes.c_size_t, _P, _P, _P]),
+            "or_vmf_pdf": (ctypes.c_double, [_P, ctypes.c_double, _P]),
+            "or_vmf_sample": (None, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P]),
+            "or_guide_fit_cell": (None, [_P, _P]),
+            "or_guide_key": (ctypes.c_uint32, [_P, _P, _P, _P]),
             "or_run_program": (None, [_P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.c_uint32,
                                       _P, _P, _P, _P, _P, _P, _P]),
             "or_gradient_noise": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_double]),
@@ -187,19 +196,33 @@ def tables():
     return t
 
 
-def render(scene, loaded, frame16, width, height, samples, max_bounces, spectral, seed, pixels, threads=8):
+def render(scene, loaded, frame16, width, height, samples, max_bounces, spectral, seed, pixels, threads=8,
+           guided=False, guide_state=None):
     """The f64 camera-path integrator over the listed pixels: ACEScg (n, 3).
-    `loaded` carries the packed lights/constants/materials/emitters."""
+    `loaded` carries the packed lights/constants/materials/emitters. With
+    `guided` the hash-grid guide learns sample by sample over every pixel
+    (the listed ones are returned)."""
     pixels = np.ascontiguousarray(pixels, np.uint32)
     out = np.zeros((len(pixels), 3), np.float64)
-    lib().or_render(_p(np.ascontiguousarray(scene.positions, np.float32)), _p(np.ascontiguousarray(scene.indices, np.uint32)),
+    extra = ()
+    if guided:
+        cells = 524288
+        if guide_state is not None:
+            guide_state["keys"] = np.zeros(cells, np.uint32)
+            guide_state["lobes"] = np.zeros((cells, 5, 4), np.float32)
+            guide_state["stats"] = np.zeros((cells, 4, 5), np.float32)
+            extra = (_p(guide_state["keys"]), _p(guide_state["lobes"]), _p(guide_state["stats"]))
+        else:
+            extra = (None, None, None)
+    fn = lib().or_render_guided if guided else lib().or_render
+    fn(_p(np.ascontiguousarray(scene.positions, np.float32)), _p(np.ascontiguousarray(scene.indices, np.uint32)),
                     _p(np.ascontiguousarray(scene.shading, np.uint32)), scene.triangle_count(),
                     _p(np.ascontiguousarray(scene.nodes)), len(scene.nodes),
                     _p(loaded.lights), _p(loaded.constants), _p(loaded.materials), _p(loaded.emitters),
                     _p(np.ascontiguousarray(frame16, np.float32)), width, height, samples, max_bounces,
                     1 if spectral else 0, seed, loaded.through, _p(pixels), len(pixels), threads, _p(out),
                     _p(np.ascontiguousarray(scene.uvs, np.float32)), _p(loaded.programs), len(loaded.programs),
-                    _p(loaded.textures), _p(loaded.texture_table), loaded.texture_count)
+                    _p(loaded.textures), _p(loaded.texture_table), loaded.texture_count, *extra)
     return out
 
 
@@ -252,3 +275,33 @@ def adaptive_verdict(film_sum, film_half, threshold):
         for dx in (0, 1, 2):
             conv &= padded[dy:dy + h, dx:dx + w]
     return own, conv
+
+
+def vmf_pdf(mean, kappa, d):
+    """guide.glsl:84-88: a vMF lobe's density at unit direction d."""
+    return lib().or_vmf_pdf(_p(np.ascontiguousarray(mean, np.float64)), float(kappa),
+                            _p(np.ascontiguousarray(d, np.float64)))
+
+
+def vmf_sample(mean, kappa, ux, uy):
+    """guide.glsl:90-98: the direction drawn from a lobe by (ux, uy)."""
+    out = np.zeros(3, np.float64)
+    lib().or_vmf_sample(_p(np.ascontiguousarray(mean, np.float64)), float(kappa), float(ux), float(uy), _p(out))
+    return out
+
+
+def guide_fit_cell(stats, lobes=None):
+    """guide_fit.comp:16-35 on one cell: stats (4, 5) f32 (weight, weighted
+    direction, count per lobe) -> (lobes (4, 4): mean, kappa; weights (4,);
+    the stats after the decay)."""
+    st = np.ascontiguousarray(np.asarray(stats, np.float32).reshape(20)).copy()
+    lb = np.zeros(20, np.float32) if lobes is None else np.ascontiguousarray(np.asarray(lobes, np.float32).reshape(20)).copy()
+    lib().or_guide_fit_cell(_p(st), _p(lb))
+    return lb[:16].reshape(4, 4), lb[16:20].copy(), st.reshape(4, 5)
+
+
+def guide_key(constants, frame16, x, n):
+    """guide.glsl:39-61's key of point x with normal n (the cell before its slot)."""
+    return lib().or_guide_key(_p(np.ascontiguousarray(constants, np.float32)),
+                              _p(np.ascontiguousarray(frame16, np.float32)),
+                              _p(np.ascontiguousarray(x, np.float64)), _p(np.ascontiguousarray(n, np.float64)))

@@ -76,6 +76,7 @@ typedef struct {
      * and per texture (first texel, width, height, 0) */
     const float *uvs; const uint32_t *programs; size_t program_words;
     const float *textures; const uint32_t *texture_table; size_t texture_count;
+    struct oguide *guide;    /* the hash grid of vMF mixtures (guiding 1), or NULL */
 } oscene;
 
 static v3 point_of(const oscene *s, uint32_t i) {
@@ -1216,6 +1217,211 @@ static int sample_dir(const surf *s, v3 wo, double ux, double uy, double uz, v3 
     return *pdf > 0;
 }
 
+/* ------------------------------------------------------------ the guide
+ * The hash grid of vMF mixtures (shaders/guide.glsl:39-139,
+ * guide_fit.comp:16-35): cells keyed by a quantised position (power-of-two
+ * levels sized by the distance from the camera) and the normal's octant, in
+ * an open-addressed table of GUIDE_CELLS slots probed linearly; per cell four
+ * lobes (mean direction, kappa), their weights, and five statistics a lobe.
+ * The state is f32 as the reference's buffers are; keys are computed from the
+ * hit point rounded to f32, as the kernels see it. Cells are made when a
+ * training record names them (a cell made earlier in a pass has no samples
+ * yet, so it cannot guide in that pass either). */
+#define GUIDE_CELLS 524288u
+#define GUIDE_LOBES 4u
+#define GUIDE_RING 3u
+#define GUIDE_TRAINING 1u
+#define GUIDE_PROBES 8u
+
+typedef struct oguide {
+    uint32_t *keys;   /* GUIDE_CELLS */
+    float *lobes;     /* (GUIDE_LOBES + 1) x 4 a cell: lobes, then the weights */
+    float *stats;     /* GUIDE_LOBES x 5 a cell */
+} oguide;
+
+/* One training record of a path vertex (shade_surface.comp:288-295): the
+ * luminance the path held once this vertex's light sample was in (`base`)
+ * and what that sample would have added had it got through (`blocked`). */
+typedef struct { uint32_t key; v3 dir; double pdf, through, base, blocked, cosine; } orecord;
+
+static uint32_t guide_key(const oscene *s, v3 xd, v3 nd) {
+    float minimum = (float)K(s, 13, 0);
+    float x = (float)xd.x, y = (float)xd.y, z = (float)xd.z;
+    float cx = (float)s->frame[0], cy = (float)s->frame[1], cz = (float)s->frame[2];
+    float dx = x - cx, dy = y - cy, dz = z - cz;
+    float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    float wanted = fmaxf(minimum, dist * (float)K(s, 12, 1));
+    float lv = ceilf(log2f(wanted / minimum));
+    lv = lv < 0.0f ? 0.0f : (lv > 31.0f ? 31.0f : lv);
+    uint32_t level = (uint32_t)lv;
+    float size = minimum * exp2f((float)level);
+    int32_t qx = (int32_t)floorf(x / size), qy = (int32_t)floorf(y / size), qz = (int32_t)floorf(z / size);
+    uint32_t octant = (nd.x >= 0 ? 1u : 0u) | (nd.y >= 0 ? 2u : 0u) | (nd.z >= 0 ? 4u : 0u);
+    uint32_t key = or_hash_u32((uint32_t)qx * 73856093u ^
+                               or_hash_u32((uint32_t)qy * 19349663u ^ or_hash_u32((uint32_t)qz * 83492791u ^ (octant | (level << 3u)))));
+    return key > 1u ? key : 1u;
+}
+/* The cell holding `key`, NONE if none (make: claim an empty slot). */
+static uint32_t guide_find(oguide *g, uint32_t key, int make) {
+    uint32_t slot = key % GUIDE_CELLS;
+    for (uint32_t probe = 0; probe < GUIDE_PROBES; probe++) {
+        uint32_t at = (slot + probe) % GUIDE_CELLS;
+        if (g->keys[at] == key) return at;
+        if (g->keys[at] != 0u) continue;
+        if (!make) return NONE;
+        g->keys[at] = key;
+        return at;
+    }
+    return NONE;
+}
+static double guide_count(const oguide *g, uint32_t cell) {
+    double c = 0;
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) c += g->stats[(cell * GUIDE_LOBES + k) * 5 + 4];
+    return c;
+}
+/* guide.glsl:71-76: a lobe; before its first fit the tetrahedron's direction, kappa 1. */
+static void guide_lobe(const oguide *g, uint32_t cell, uint32_t k, v3 *mean, double *kappa) {
+    const float *v = &g->lobes[((size_t)cell * (GUIDE_LOBES + 1) + k) * 4];
+    if (v[3] > 0.0f) { *mean = V(v[0], v[1], v[2]); *kappa = v[3]; return; }
+    double c = 1.0 / sqrt(3.0);
+    *mean = V((k == 0 || k == 1) ? c : -c, (k == 0 || k == 2) ? c : -c, (k == 0 || k == 3) ? c : -c);
+    *kappa = 1.0;
+}
+static void guide_weights(const oguide *g, uint32_t cell, double w[4]) {
+    const float *v = &g->lobes[((size_t)cell * (GUIDE_LOBES + 1) + GUIDE_LOBES) * 4];
+    double t = (double)v[0] + v[1] + v[2] + v[3];
+    for (int k = 0; k < 4; k++) w[k] = t > 0 ? v[k] / t : 0.25;
+}
+/* guide.glsl:84-98 */
+static double vmf_pdf(v3 mean, double kappa, v3 d) {
+    if (kappa < 1e-3) return 1.0 / (4.0 * PI);
+    return kappa / (2.0 * PI * (1.0 - exp(-2.0 * kappa))) * exp(kappa * (dot(mean, d) - 1.0));
+}
+static v3 vmf_sample(v3 mean, double kappa, double ux, double uy) {
+    kappa = fmax(kappa, 1e-3);
+    double w = 1.0 + log(ux + (1.0 - ux) * exp(-2.0 * kappa)) / kappa;
+    double sn = sqrt(fmax(0.0, 1.0 - w * w)), phi = 2.0 * PI * uy;
+    v3 t, b;
+    basis(mean, &t, &b);
+    return norm(add(add(mul(t, sn * cos(phi)), mul(b, sn * sin(phi))), mul(mean, w)));
+}
+/* guide.glsl:100-119 */
+static double guide_pdf(const oguide *g, uint32_t cell, v3 d) {
+    double w[4], pdf = 0, kappa;
+    v3 mean;
+    guide_weights(g, cell, w);
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) { guide_lobe(g, cell, k, &mean, &kappa); pdf += w[k] * vmf_pdf(mean, kappa, d); }
+    return pdf;
+}
+static v3 guide_sample(const oguide *g, uint32_t cell, double ux, double uy, double uz) {
+    double w[4], kappa;
+    v3 mean;
+    guide_weights(g, cell, w);
+    uint32_t lobe = GUIDE_LOBES - 1;
+    double pick = uz;
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) {
+        if (pick < w[k]) { lobe = k; break; }
+        pick -= w[k];
+    }
+    guide_lobe(g, cell, lobe, &mean, &kappa);
+    return vmf_sample(mean, kappa, ux, uy);
+}
+/* guide.glsl:121-139: the lobe that explains d best takes the sample. */
+static void guide_train(oguide *g, uint32_t cell, v3 d, double value, double pdf) {
+    if (cell == NONE || !(value > 0) || !(pdf > 0) || isinf(value)) return;
+    double w[4], best_score = -1, kappa;
+    v3 mean;
+    uint32_t best = 0;
+    guide_weights(g, cell, w);
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) {
+        guide_lobe(g, cell, k, &mean, &kappa);
+        double score = (w[k] + 0.05) * vmf_pdf(mean, kappa, d);
+        if (score > best_score) { best_score = score; best = k; }
+    }
+    float x = (float)fmin(value / pdf, 1e4);
+    float *st = &g->stats[((size_t)cell * GUIDE_LOBES + best) * 5];
+    st[0] += x;
+    st[1] += x * (float)d.x;
+    st[2] += x * (float)d.y;
+    st[3] += x * (float)d.z;
+    st[4] += 1.0f;
+}
+/* guide_fit.comp:16-35, in f32 as the buffers hold it: mean and Banerjee's
+ * kappa from each lobe's statistics, weights from their shares; then the
+ * statistics decay by 0.9. */
+static void guide_fit(oguide *g) {
+    for (uint32_t cell = 0; cell < GUIDE_CELLS; cell++) {
+        if (g->keys[cell] == 0u) continue;
+        float w4[4];
+        for (uint32_t k = 0; k < GUIDE_LOBES; k++) {
+            float *st = &g->stats[((size_t)cell * GUIDE_LOBES + k) * 5];
+            float w = st[0], sx = st[1], sy = st[2], sz = st[3];
+            float ss = sx * sx + sy * sy + sz * sz;
+            if (w > 0.0f && ss > 0.0f) {
+                float l = sqrtf(ss);
+                float r = l / w;
+                r = r < 0.0f ? 0.0f : (r > 0.999f ? 0.999f : r);
+                float kappa = r * (3.0f - r * r) / (1.0f - r * r);
+                kappa = kappa < 0.1f ? 0.1f : (kappa > 1000.0f ? 1000.0f : kappa);
+                float *lb = &g->lobes[((size_t)cell * (GUIDE_LOBES + 1) + k) * 4];
+                lb[0] = sx / l; lb[1] = sy / l; lb[2] = sz / l; lb[3] = kappa;
+            }
+            w4[k] = w;
+            for (int c = 0; c < 5; c++) st[c] *= 0.9f;
+        }
+        if (w4[0] + w4[1] + w4[2] + w4[3] > 0.0f) {
+            float *wt = &g->lobes[((size_t)cell * (GUIDE_LOBES + 1) + GUIDE_LOBES) * 4];
+            for (int k = 0; k < 4; k++) wt[k] = w4[k];
+        }
+    }
+}
+/* The guide's pieces alone, for tests: a lobe's density and a direction
+ * drawn from it, the fit of one cell's statistics (20 floats: 4 lobes x
+ * weight, weighted direction, count) into its lobes and weights (20 floats),
+ * the statistics decayed in place, and the key of a point. */
+double or_vmf_pdf(const double mean[3], double kappa, const double d[3]) {
+    return vmf_pdf(V(mean[0], mean[1], mean[2]), kappa, V(d[0], d[1], d[2]));
+}
+void or_vmf_sample(const double mean[3], double kappa, double ux, double uy, double out[3]) {
+    v3 r = vmf_sample(V(mean[0], mean[1], mean[2]), kappa, ux, uy);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void or_guide_fit_cell(float stats[20], float lobes[20]) {
+    static uint32_t key[GUIDE_CELLS];
+    static float lb[(size_t)GUIDE_CELLS * (GUIDE_LOBES + 1) * 4], st[(size_t)GUIDE_CELLS * GUIDE_LOBES * 5];
+    static pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
+    pthread_mutex_lock(&lock);
+    oguide g = {key, lb, st};
+    memset(key, 0, sizeof(key));
+    key[7] = 1u;
+    memcpy(&st[7 * GUIDE_LOBES * 5], stats, 20 * 4);
+    memcpy(&lb[7 * (GUIDE_LOBES + 1) * 4], lobes, 20 * 4);
+    guide_fit(&g);
+    memcpy(stats, &st[7 * GUIDE_LOBES * 5], 20 * 4);
+    memcpy(lobes, &lb[7 * (GUIDE_LOBES + 1) * 4], 20 * 4);
+    pthread_mutex_unlock(&lock);
+}
+uint32_t or_guide_key(const float *constants, const float *frame16, const double x[3], const double n[3]) {
+    oscene s;
+    double fr[16];
+    memset(&s, 0, sizeof(s));
+    for (int k = 0; k < 16; k++) fr[k] = frame16[k];
+    s.constants = constants;
+    s.frame = fr;
+    return guide_key(&s, V(x[0], x[1], x[2]), V(n[0], n[1], n[2]));
+}
+
+/* spec_luminance (spectrum.glsl): Y of the estimate. */
+static double spec_luminance(const oscene *s, spec r, spec lambda) {
+    if (!s->spectral) return r.v[0] * 0.2722287168 + r.v[1] * 0.6740817658 + r.v[2] * 0.0536895174;
+    double sum = 0;
+    for (int i = 0; i < 4; i++) {
+        double pdf = wavelength_pdf(lambda.v[i]);
+        if (pdf > 0) sum += cmf(lambda.v[i]).y * (r.v[i] / pdf);
+    }
+    return sum * 0.25 / K(s, 6, 1);
+}
+
 /* ------------------------------------------------------------ the path */
 
 static double mis_weight(double a, double b) { a = fmin(a, 1e18); b = fmin(b, 1e18); return 1.0 / (1 + a * a + b * b); }
@@ -1231,6 +1437,22 @@ static v3 oct_decode(uint32_t p) {
         n.x = nx; n.y = ny;
     }
     return norm(n);
+}
+/* octahedral_encode (scene.lucb:400-412) in f32, as the kernels store a
+ * guide record's direction. */
+static uint32_t oct_encode(v3 nd) {
+    float nx = (float)nd.x, ny = (float)nd.y, nz = (float)nd.z;
+    float l = fabsf(nx) + fabsf(ny) + fabsf(nz);
+    float px = nx / l, py = ny / l;
+    if (nz < 0.0f) {
+        float qx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+        float qy = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+        px = qx; py = qy;
+    }
+    px = px < -1.0f ? -1.0f : (px > 1.0f ? 1.0f : px);
+    py = py < -1.0f ? -1.0f : (py > 1.0f ? 1.0f : py);
+    int ix = (int)rintf(px * 32767.0f), iy = (int)rintf(py * 32767.0f);
+    return ((uint32_t)ix & 0xffffu) | (((uint32_t)iy & 0xffffu) << 16);
 }
 static spec clamp_ind(const oscene *s, spec c, uint32_t total) {
     double cv = K(s, 0, 2);
@@ -1285,8 +1507,10 @@ static spec miss_radiance(const oscene *s, uint32_t bounce, v3 rd, double ray_pd
     return rad;
 }
 
-/* One camera path of pixel `pixel`, sample `index`: its film value. */
-static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) {
+/* One camera path of pixel `pixel`, sample `index`: its film value. With a
+ * guide, `rec` receives the training records of its first GUIDE_RING
+ * vertices (key 0 ends them) and `found` the luminance it carried home. */
+static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, orecord *rec, double *found_out) {
     double u[4];
     sample4(s, pixel, index, 0, u);
     const double *f = s->frame;
@@ -1315,6 +1539,8 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
     v3 vertex = origin;
     v3 prev_normal = V(0, 0, 1);
     uint32_t rounds = s->max_bounces + 1 + s->through;
+    int guide_on = s->guide && K(s, 12, 0) > 0.5;
+    if (rec) for (uint32_t k = 0; k < GUIDE_RING; k++) rec[k].key = 0;
     for (uint32_t round = 0; round < rounds; round++) {
         uint32_t total = bounces & 0xff;
         ohit h = closest(s, origin, rd, INF);
@@ -1408,6 +1634,22 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
         orient(s, &sf, wo);
         colors(s, &sf, lambda, &prog);
         v3 from = offset_origin(x, geo);
+        /* Path guiding (shade_surface.comp:155-164): in a cell with enough
+         * samples, BSDF sampling becomes a one-sample mix of the guide at
+         * share K_GUIDE.z and the BSDF. */
+        uint32_t key = 0, cell = NONE;
+        double guided = 0;
+        if (guide_on && total < GUIDE_RING) {
+            key = guide_key(s, x, geo);
+            cell = guide_find(s->guide, key, 0);
+            int suited = !sf.inside && sf.pt <= 0 && (sf.pd > 0.05 || fmin(sf.alpha.x, sf.alpha.y) > 0.1);
+            float count = 0.0f;
+            if (cell != NONE)
+                for (uint32_t k = 0; k < GUIDE_LOBES; k++) count += s->guide->stats[((size_t)cell * GUIDE_LOBES + k) * 5 + 4];
+            if (cell != NONE && suited && count >= (float)K(s, 12, 3)) guided = K(s, 12, 2);
+        }
+        int pushed = 0;
+        spec nee = S1(0);
         double ul[4];
         sample4(s, pixel, index, 2 + 3 * total, ul);
         double p_distant = K(s, 8, 2);
@@ -1440,16 +1682,21 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
             if ((side > 0) == (wi.z > 0) && (side > 0 || sf.pt > 0)) {
                 double bp;
                 spec fv = evaluate(&sf, wi, wo, &bp);
+                if (guided > 0) bp = guided * guide_pdf(s->guide, cell, ls.direction) + (1 - guided) * bp;
                 if (bp > 0) {
                     double lp = ls.pdf * chance;
                     double w = ls.delta ? 1.0 : mis_weight(bp / lp, 0);
                     spec c = clamp_ind(s, sscale(smul(smul(throughput, fv), emitted), fabs(wi.z) * (w / lp)), total);
+                    pushed = 1;
+                    nee = c;
                     v3 target = add(x, mul(ls.direction, ls.distance));
                     double margin = 4e-4 * fmax(1.0, fmax(fabs(target.x), fmax(fabs(target.y), fabs(target.z))));
                     double rs = ls.distance >= INF ? INF : fmax(0, ls.distance - margin);
                     double kept;
-                    if (shadow(s, offset_origin(x, side > 0 ? geo : neg(geo)), ls.direction, rs, &kept))
+                    if (shadow(s, offset_origin(x, side > 0 ? geo : neg(geo)), ls.direction, rs, &kept)) {
                         radiance = sadd(radiance, sscale(c, kept));
+                        nee = sscale(c, 1.0 - kept);
+                    }
                 }
             }
         }
@@ -1458,21 +1705,53 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
         v3 wi;
         double pdf;
         uint32_t lobe;
-        if (!sample_dir(&sf, wo, ub[0], ub[1], ub[3], &wi, &pdf, &lobe)) break;
-        double again;
-        spec fv = evaluate(&sf, wi, wo, &again);
-        spec weight = sscale(fv, fabs(wi.z) / pdf);
+        spec weight;
+        double unguided = 1.0;
+        if (uo[1] < guided) {
+            /* a guide sample (shade_surface.comp:252-262) */
+            wi = to_local(&sf, guide_sample(s->guide, cell, uo[2], uo[3], ub[3]));
+            double bp;
+            spec fv = evaluate(&sf, wi, wo, &bp);
+            pdf = guided * guide_pdf(s->guide, cell, to_world(&sf, wi)) + (1 - guided) * bp;
+            if (pdf <= 0 || spec_max(s, fv) <= 0) break;
+            unguided = pdf / fmax(bp, 1e-6 * pdf);
+            weight = sscale(fv, fabs(wi.z) / pdf);
+            lobe = wi.z < 0 ? 2u : (sf.pd > 0.5 ? 0u : 1u);
+        } else {
+            if (!sample_dir(&sf, wo, ub[0], ub[1], ub[3], &wi, &pdf, &lobe)) break;
+            double again;
+            spec fv = evaluate(&sf, wi, wo, &again);
+            weight = sscale(fv, fabs(wi.z) / pdf);
+            if (guided > 0) {
+                double mixed = guided * guide_pdf(s->guide, cell, to_world(&sf, wi)) + (1 - guided) * pdf;
+                unguided = mixed / pdf;
+                weight = sscale(weight, pdf / mixed);
+                pdf = mixed;
+            }
+        }
         v3 dir = to_world(&sf, wi);
         double side = dot(dir, geo);
         if ((side > 0) != (wi.z > 0)) break;
         bounces = next_bounce(s, bounces, lobe);
         if (bounces == NONE) break;
         throughput = smul(throughput, weight);
-        survival_scale = fmin(survival_scale * 1.0, 1e6);
+        survival_scale = fmin(survival_scale * unguided, 1e6);
         if (total + 1 >= 3) {
             double survive = fmin(0.95, spec_max(s, throughput) * survival_scale);
             if (ub[2] >= survive) break;
             throughput = sscale(throughput, 1.0 / survive);
+        }
+        /* the vertex's training record (shade_surface.comp:288-295) */
+        if (guide_on && total < GUIDE_RING && rec) {
+            orecord *r = &rec[total];
+            r->key = key ? key : guide_key(s, x, geo);
+            r->dir = oct_decode(oct_encode(dir));
+            r->pdf = (float)pdf;
+            r->through = (float)spec_luminance(s, throughput, lambda);
+            r->base = spec_luminance(s, radiance, lambda);
+            r->blocked = pushed ? spec_luminance(s, nee, lambda) : 0.0;
+            r->cosine = (float)fabs(dot(dir, normal));
+            if (total + 1 < GUIDE_RING) rec[total + 1].key = 0;
         }
         prev_normal = normal;
         origin = side > 0 ? from : offset_origin(x, neg(geo));
@@ -1480,6 +1759,7 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan) 
         rd = dir;
         ray_pdf = pdf;
     }
+    if (found_out) *found_out = spec_luminance(s, radiance, lambda);
     v3 value = spec_to_film(s, radiance, lambda);
     if (isnan(value.x) || isnan(value.y) || isnan(value.z) || isinf(value.x) || isinf(value.y) || isinf(value.z)) {
         value = V(0, 0, 0);
@@ -1496,7 +1776,7 @@ static void *render_run(void *arg) {
         uint32_t pixel = j->pixels[i];
         v3 sum = V(0, 0, 0);
         int nan = 0;
-        for (uint32_t k = 0; k < j->s->samples; k++) sum = add(sum, trace_path(j->s, pixel, k, &nan));
+        for (uint32_t k = 0; k < j->s->samples; k++) sum = add(sum, trace_path(j->s, pixel, k, &nan, NULL, NULL));
         v3 mean = mul(sum, 1.0 / fmax((double)j->s->samples, 1.0));
         if (j->s->spectral) {
             v3 r0 = Kv(j->s, 3), r1 = Kv(j->s, 4), r2 = Kv(j->s, 5);
@@ -1505,6 +1785,105 @@ static void *render_run(void *arg) {
         j->out[i * 3] = mean.x; j->out[i * 3 + 1] = mean.y; j->out[i * 3 + 2] = mean.z;
     }
     return NULL;
+}
+
+/* A guided render's pass over every pixel for one sample: film values and
+ * each path's training records, the guide read only. */
+typedef struct { const oscene *s; uint32_t sample; size_t begin, end; double *film; orecord *recs; double *found; } gjob;
+
+static void *guided_run(void *arg) {
+    gjob *j = (gjob *)arg;
+    for (size_t px = j->begin; px < j->end; px++) {
+        int nan = 0;
+        v3 v = trace_path(j->s, (uint32_t)px, j->sample, &nan, &j->recs[px * GUIDE_RING], &j->found[px]);
+        j->film[px * 3] += v.x; j->film[px * 3 + 1] += v.y; j->film[px * 3 + 2] += v.z;
+    }
+    return NULL;
+}
+
+static void setup_scene(oscene *s, double fr[16], const float *positions, const uint32_t *indices,
+                        const uint32_t *shading, size_t triangles, const void *nodes, size_t node_count,
+                        const float *lights, const float *constants, const float *materials, const uint32_t *emitters,
+                        const float *frame, uint32_t width, uint32_t height, uint32_t samples, uint32_t max_bounces,
+                        uint32_t spectral, uint32_t seed, uint32_t through, const float *uvs, const uint32_t *programs,
+                        size_t program_words, const float *textures, const uint32_t *texture_table,
+                        size_t texture_count);
+
+/* Renders a guided image (guiding 1, camera paths): sample by sample over
+ * every pixel, each sample's paths then training the grid in pixel order
+ * (finish.comp; the kernels' float atomics add in another order) and the fit
+ * running after it (guide_fit.comp, integrator.lucb:815-817), as one band a
+ * sample does. Returns the listed pixels as or_render does. */
+int or_render_guided(const float *positions, const uint32_t *indices, const uint32_t *shading, size_t triangles,
+                     const void *nodes, size_t node_count, const float *lights, const float *constants,
+                     const float *materials, const uint32_t *emitters, const float *frame, uint32_t width,
+                     uint32_t height, uint32_t samples, uint32_t max_bounces, uint32_t spectral, uint32_t seed,
+                     uint32_t through, const uint32_t *pixels, size_t n, int threads, double *out, const float *uvs,
+                     const uint32_t *programs, size_t program_words, const float *textures,
+                     const uint32_t *texture_table, size_t texture_count, uint32_t *keys_out, float *lobes_out,
+                     float *stats_out) {
+    oscene s;
+    double fr[16];
+    setup_scene(&s, fr, positions, indices, shading, triangles, nodes, node_count, lights, constants, materials,
+                emitters, frame, width, height, samples, max_bounces, spectral, seed, through, uvs, programs,
+                program_words, textures, texture_table, texture_count);
+    oguide g;
+    g.keys = (uint32_t *)calloc(GUIDE_CELLS, 4);
+    g.lobes = (float *)calloc((size_t)GUIDE_CELLS * (GUIDE_LOBES + 1) * 4, 4);
+    g.stats = (float *)calloc((size_t)GUIDE_CELLS * GUIDE_LOBES * 5, 4);
+    size_t pixel_count = (size_t)width * height;
+    double *film = (double *)calloc(pixel_count * 3, sizeof(double));
+    orecord *recs = (orecord *)calloc(pixel_count * GUIDE_RING, sizeof(orecord));
+    double *found = (double *)calloc(pixel_count, sizeof(double));
+    if (!g.keys || !g.lobes || !g.stats || !film || !recs || !found) {
+        free(g.keys); free(g.lobes); free(g.stats); free(film); free(recs); free(found);
+        return -1;
+    }
+    s.guide = &g;
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    pthread_t th[64];
+    gjob jobs[64];
+    for (uint32_t sample = 0; sample < samples; sample++) {
+        for (int k = 0; k < threads; k++) {
+            jobs[k] = (gjob){&s, sample, pixel_count * (size_t)k / (size_t)threads,
+                             pixel_count * (size_t)(k + 1) / (size_t)threads, film, recs, found};
+            pthread_create(&th[k], NULL, guided_run, &jobs[k]);
+        }
+        for (int k = 0; k < threads; k++) pthread_join(th[k], NULL);
+        for (size_t px = 0; px < pixel_count; px++) {
+            if (or_hash_u32((uint32_t)px * 9781u + sample) % GUIDE_TRAINING != 0u) continue;
+            for (uint32_t k = 0; k < GUIDE_RING; k++) {
+                const orecord *r = &recs[px * GUIDE_RING + k];
+                if (r->key == 0) break;
+                uint32_t cell = guide_find(&g, r->key, 1);
+                if (cell == NONE) break;
+                if (r->through <= 0) continue;
+                /* finish.comp's found - v.z: the light past the vertex less
+                 * what its blocked light sample would have added; nothing
+                 * when no light came past it (as the kernels count it) */
+                double gain = (found[px] - r->base) - r->blocked;
+                double at_vertex = found[px] - gain;
+                if (gain <= 1e-5 * fmax(found[px], at_vertex)) gain = 0.0;
+                guide_train(&g, cell, r->dir, (double)(float)(gain / r->through * r->cosine), r->pdf);
+            }
+        }
+        guide_fit(&g);
+    }
+    for (size_t i = 0; i < n; i++) {
+        uint32_t px = pixels[i];
+        v3 mean = mul(V(film[px * 3], film[px * 3 + 1], film[px * 3 + 2]), 1.0 / fmax((double)samples, 1.0));
+        if (s.spectral) {
+            v3 r0 = Kv(&s, 3), r1 = Kv(&s, 4), r2 = Kv(&s, 5);
+            mean = V(dot(r0, mean), dot(r1, mean), dot(r2, mean));
+        }
+        out[i * 3] = mean.x; out[i * 3 + 1] = mean.y; out[i * 3 + 2] = mean.z;
+    }
+    if (keys_out) memcpy(keys_out, g.keys, (size_t)GUIDE_CELLS * 4);
+    if (lobes_out) memcpy(lobes_out, g.lobes, (size_t)GUIDE_CELLS * (GUIDE_LOBES + 1) * 16);
+    if (stats_out) memcpy(stats_out, g.stats, (size_t)GUIDE_CELLS * GUIDE_LOBES * 20);
+    free(g.keys); free(g.lobes); free(g.stats); free(film); free(recs); free(found);
+    return 0;
 }
 
 /* Renders the listed pixels (camera paths, light tracing and guiding off):
@@ -1517,17 +1896,9 @@ int or_render(const float *positions, const uint32_t *indices, const uint32_t *s
               size_t program_words, const float *textures, const uint32_t *texture_table, size_t texture_count) {
     oscene s;
     double fr[16];
-    for (int k = 0; k < 16; k++) fr[k] = frame[k];
-    static double tables[T_SIZE];
-    static int tables_ready = 0;
-    if (!tables_ready) { or_make_tables(tables); tables_ready = 1; }
-    s.positions = positions; s.indices = indices; s.shading = shading; s.triangles = triangles;
-    s.nodes = (const onode *)nodes; s.node_count = node_count;
-    s.lights = lights; s.constants = constants; s.materials = materials; s.emitters = emitters;
-    s.frame = fr; s.width = width; s.height = height; s.samples = samples; s.max_bounces = max_bounces;
-    s.spectral = spectral; s.seed = seed; s.through = through; s.tables = tables;
-    s.uvs = uvs; s.programs = programs; s.program_words = uvs ? program_words : 0;
-    s.textures = textures; s.texture_table = texture_table; s.texture_count = texture_count;
+    setup_scene(&s, fr, positions, indices, shading, triangles, nodes, node_count, lights, constants, materials,
+                emitters, frame, width, height, samples, max_bounces, spectral, seed, through, uvs, programs,
+                program_words, textures, texture_table, texture_count);
     if (threads < 1) threads = 1;
     if (threads > 64) threads = 64;
     pthread_t th[64];
@@ -1538,4 +1909,29 @@ int or_render(const float *positions, const uint32_t *indices, const uint32_t *s
     }
     for (int k = 0; k < threads; k++) pthread_join(th[k], NULL);
     return 0;
+}
+
+static void setup_scene(oscene *s, double fr[16], const float *positions, const uint32_t *indices,
+                        const uint32_t *shading, size_t triangles, const void *nodes, size_t node_count,
+                        const float *lights, const float *constants, const float *materials, const uint32_t *emitters,
+                        const float *frame, uint32_t width, uint32_t height, uint32_t samples, uint32_t max_bounces,
+                        uint32_t spectral, uint32_t seed, uint32_t through, const float *uvs, const uint32_t *programs,
+                        size_t program_words, const float *textures, const uint32_t *texture_table,
+                        size_t texture_count) {
+    static double tables[T_SIZE];
+    static int tables_ready = 0;
+    static pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
+    pthread_mutex_lock(&lock);
+    if (!tables_ready) { or_make_tables(tables); tables_ready = 1; }
+    pthread_mutex_unlock(&lock);
+    for (int k = 0; k < 16; k++) fr[k] = frame[k];
+    memset(s, 0, sizeof(*s));
+    s->positions = positions; s->indices = indices; s->shading = shading; s->triangles = triangles;
+    s->nodes = (const onode *)nodes; s->node_count = node_count;
+    s->lights = lights; s->constants = constants; s->materials = materials; s->emitters = emitters;
+    s->frame = fr; s->width = width; s->height = height; s->samples = samples; s->max_bounces = max_bounces;
+    s->spectral = spectral; s->seed = seed; s->through = through; s->tables = tables;
+    s->uvs = uvs; s->programs = programs; s->program_words = uvs ? program_words : 0;
+    s->textures = textures; s->texture_table = texture_table; s->texture_count = texture_count;
+    s->guide = NULL;
 }
