@@ -49,6 +49,7 @@ def lib():
                                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P,
                                                 ctypes.c_size_t, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P, _P,
                                                 ctypes.c_size_t, _P, _P, _P]),
+            "or_plant": (None, [ctypes.c_int]),
             "or_vmf_pdf": (ctypes.c_double, [_P, ctypes.c_double, _P]),
             "or_vmf_sample": (None, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P]),
             "or_guide_fit_cell": (None, [_P, _P]),
@@ -305,3 +306,10 @@ def guide_key(constants, frame16, x, n):
     return lib().or_guide_key(_p(np.ascontiguousarray(constants, np.float32)),
                               _p(np.ascontiguousarray(frame16, np.float32)),
                               _p(np.ascontiguousarray(x, np.float64)), _p(np.ascontiguousarray(n, np.float64)))
+
+
+def plant(mistake):
+    """Plants a known mistake in the restatement (0: none; 1: the light
+    sample's MIS weight without the light tracer's term), for tests that
+    check that a comparison catches it."""
+    lib().or_plant(int(mistake))

@@ -11,8 +11,13 @@
  *                    film_convert.comp:36-53
  *   camera           shaders/camera.comp:14-50
  *   closest + lights shaders/intersect_closest.comp:45-87, lights.glsl
- *   surface shading  shaders/shade_surface.comp:48-312 (light tracing and
- *                    guiding off: the two are checked statistically)
+ *   surface shading  shaders/shade_surface.comp:48-312, with the camera
+ *                    side of light tracing's MIS and the guide's mixture
+ *   light tracing    light_emit.comp:21-137, light_paths.glsl:15-189,
+ *                    connect.glsl:30-151 (every light path of every sample,
+ *                    splatted in path order)
+ *   guide            guide.glsl:39-139, guide_fit.comp:16-35 (sample by
+ *                    sample over every pixel, trained in pixel order)
  *   light tree       shaders/lighttree.glsl:18-137
  *   OpenPBR / GGX    shaders/openpbr.glsl:100-556, microfacet.glsl:9-84;
  *                    albedo tables by tables.comp's estimator, in f64
@@ -28,8 +33,12 @@
  * renderer receives; the oracle reads them as the kernels do.
  *
  * Pinning: the furnace closed form (rho L), the rect-light irradiance closed
- * form (rho/pi L x the corner-irradiance of a rectangle), f64 brute-force hit
- * agreement of the caster, and the GPU images themselves in the tests.
+ * form (rho/pi L x the corner-irradiance of a rectangle, also reached by light
+ * paths), f64 brute-force hit agreement of the caster; light tracing on and
+ * off giving one image mean (the three techniques' weights sum to one; a
+ * planted mistake in them, or_plant, breaks it); the vMF lobe's normalisation,
+ * its sampler's mean resultant length coth k - 1/k and the Banerjee fit of
+ * samples of a known lobe; and the GPU images themselves in the tests.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1114,6 +1123,16 @@ static void orient(const oscene *sc, surf *s, v3 wo) {
     choose(s, s->coat * s->calb, sub_rgb * s->metal, sub_rgb * di * s->salb * s->lspec,
            sub_rgb * di * s->trans * (1 - s->salb) * s->ltrans, sub_rgb * di * (1 - s->trans) * (1 - s->salb) * s->lbase);
 }
+/* openpbr.glsl:378-387: what reaches the layers under the coat. */
+static void update_substrate(surf *s) {
+    s->substrate = S1(1);
+    if (s->coat > 0)
+        for (int k = 0; k < 4; k++) {
+            double delta = (1 - s->ck) / (1 - s->base.v[k] * s->ck);
+            double under = s->ctint.v[k] * (1 - s->calb) * mixd(1, delta, s->dark);
+            s->substrate.v[k] = mixd(1, under, s->coat);
+        }
+}
 /* A color a program drives: a reflectance through the environment basis,
  * at most one (openpbr.glsl:391-401). */
 static spec driven_color(const oscene *sc, const oprog *g, uint32_t at, spec lambda) {
@@ -1129,13 +1148,7 @@ static void colors(const oscene *sc, surf *s, spec lambda, const oprog *g) {
     s->stint = sscale(s->mtint, s->sw);
     s->ttint = driven(g, 11) ? driven_color(sc, g, 17, lambda) : material_color(sc, s->m, 8, lambda);
     s->ctint = s->coat > 0 ? (driven(g, 12) ? driven_color(sc, g, 20, lambda) : material_color(sc, s->m, 4, lambda)) : S1(1);
-    s->substrate = S1(1);
-    if (s->coat > 0)
-        for (int k = 0; k < 4; k++) {
-            double delta = (1 - s->ck) / (1 - s->base.v[k] * s->ck);
-            double under = s->ctint.v[k] * (1 - s->calb) * mixd(1, delta, s->dark);
-            s->substrate.v[k] = mixd(1, under, s->coat);
-        }
+    update_substrate(s);
 }
 static int refraction_half(v3 wi, v3 wo, double eta, v3 *h) {
     *h = norm(add(wo, mul(wi, eta)));
@@ -1422,6 +1435,171 @@ static double spec_luminance(const oscene *s, spec r, spec lambda) {
     return sum * 0.25 / K(s, 6, 1);
 }
 
+/* ------------------------------------------------------------ light tracing
+ * Light paths and the three-technique MIS (shaders/connect.glsl:30-151,
+ * light_paths.glsl:15-189, light_emit.comp:21-137, common.glsl:113-143,
+ * 238-242): paths start at emitters chosen by power (or from a dome or wide
+ * distant light, on a disk facing it), connect to the camera at every surface
+ * they reach and splat what gets through onto the pixel it projects to;
+ * camera paths weigh their BSDF hits and light samples against them through
+ * the density ratios they carry (SV_MIS). */
+#define AFAR_EMITTER 0x40000000u
+
+/* A mistake planted for tests that check the checks: 1 drops the light
+ * tracer's term from the light sample's MIS weight. */
+static int planted = 0;
+void or_plant(int mistake) { planted = mistake; }
+
+static int tracing_on(const oscene *s) { return K(s, 14, 0) > 0; }
+static double paths_per_sample(const oscene *s) { return K(s, 14, 0); }
+static v3 frame_pos(const oscene *s) { return V(s->frame[0], s->frame[1], s->frame[2]); }
+static v3 frame_du(const oscene *s) { return V(s->frame[8], s->frame[9], s->frame[10]); }
+static v3 frame_dv(const oscene *s) { return V(s->frame[12], s->frame[13], s->frame[14]); }
+
+/* camera.comp:40-48's lens point */
+static v3 lens_point(const oscene *s, double ux, double uy) {
+    double radius = K(s, 14, 2);
+    if (radius <= 0) return frame_pos(s);
+    double lx, ly;
+    concentric(ux, uy, &lx, &ly);
+    return add(add(frame_pos(s), mul(norm(frame_du(s)), lx * radius)), mul(norm(frame_dv(s)), ly * radius));
+}
+/* common.glsl:125-131: the camera's density of direction d (solid angle). */
+static double camera_pdf(const oscene *s, v3 d) {
+    double c = dot(d, Kv(s, 15));
+    return c > 0 ? K(s, 14, 1) / (c * c * c) : 0;
+}
+/* connect.glsl:30-45 */
+static int camera_pixel(const oscene *s, v3 x, v3 lens, uint32_t *pixel, v3 *d, double *dist) {
+    v3 offset = sub(x, lens);
+    *dist = len(offset);
+    *d = mul(offset, 1.0 / fmax(*dist, 1e-20));
+    double c = dot(*d, Kv(s, 15));
+    if (c <= 1e-6 || *dist <= 0) return 0;
+    double focus = K(s, 14, 3);
+    v3 corner = V(s->frame[4], s->frame[5], s->frame[6]);
+    v3 film = sub(K(s, 14, 2) > 0 ? mul(sub(add(lens, mul(*d, focus / c)), frame_pos(s)), 1.0 / focus) : mul(*d, 1.0 / c),
+                  corner);
+    v3 du = frame_du(s), dv = frame_dv(s);
+    double fx = dot(film, du) / dot(du, du), fy = dot(film, dv) / dot(dv, dv);
+    if (!(fx >= 0 && fy >= 0 && fx < s->frame[11] && fy < s->frame[15])) return 0;
+    *pixel = (uint32_t)fy * (uint32_t)s->frame[11] + (uint32_t)fx;
+    return 1;
+}
+/* connect.glsl:66-110: an emitter's chance by power, and the choice. */
+static double emitter_power_pdf(const oscene *s, uint32_t e) {
+    uint32_t record = (uint32_t)K(s, 10, 0) + e * 4;
+    uint32_t trail = s->emitters[record + 1], depth = s->emitters[record + 2];
+    uint32_t base = (uint32_t)K(s, 8, 1), node = 0;
+    double pdf = 1;
+    for (uint32_t level = 0; level < depth; level++) {
+        uint32_t link = s->emitters[node * 2];
+        double left = fmax(s->lights[(base + link * 3) * 4 + 3], 0), right = fmax(s->lights[(base + (link + 1) * 3) * 4 + 3], 0);
+        if (left + right <= 0) return 0;
+        uint32_t bit = (trail >> level) & 1u;
+        pdf *= (bit == 0 ? left : right) / (left + right);
+        node = link + bit;
+    }
+    return pdf;
+}
+static int emitter_by_power(const oscene *s, double u, uint32_t *e, double *pdf) {
+    *pdf = 1;
+    *e = 0;
+    if (K(s, 8, 0) < 1) return 0;
+    uint32_t base = (uint32_t)K(s, 8, 1), node = 0;
+    for (uint32_t depth = 0; depth < 64; depth++) {
+        uint32_t link = s->emitters[node * 2];
+        if (s->emitters[node * 2 + 1] & 0x80000000u) { *e = link; return *pdf > 0; }
+        double left = fmax(s->lights[(base + link * 3) * 4 + 3], 0), right = fmax(s->lights[(base + (link + 1) * 3) * 4 + 3], 0);
+        if (left + right <= 0) return 0;
+        double chance = left / (left + right);
+        if (u < chance) { u = fmin(u / chance, 0.99999994); *pdf *= chance; node = link; }
+        else { u = fmin((u - chance) / (1 - chance), 0.99999994); *pdf *= 1 - chance; node = link + 1; }
+    }
+    return 0;
+}
+static double emitter_area(const oscene *s, uint32_t code) {
+    if (code & 0x80000000u) {
+        uint32_t t = code & ~0x80000000u;
+        v3 a = point_of(s, s->indices[t * 3]);
+        return 0.5 * len(cross(sub(point_of(s, s->indices[t * 3 + 1]), a), sub(point_of(s, s->indices[t * 3 + 2]), a)));
+    }
+    if (light_type(s, code) == 2) { double r = L(s, code, 1, 3); return 4 * PI * r * r; }
+    return L(s, code, 3, 3);
+}
+/* connect.glsl:112-133 */
+static double emitter_area_pdf(const oscene *s, uint32_t e) {
+    double finite = 1 - K(s, 21, 0), area = emitter_area(s, emitter_code(s, e));
+    return area > 0 ? finite * emitter_power_pdf(s, e) / area : 0;
+}
+static double emission_direction_pdf(const oscene *s, uint32_t e, double c) {
+    return fabs(c) / ((emitter_code(s, e) & 0x80000000u) ? 2 * PI : PI);
+}
+/* connect.glsl:140-151 */
+static double camera_hit_ratio(const oscene *s, const double mis[4], uint32_t e, double cos_light, double dist,
+                               double bsdf_pdf) {
+    if (e == NONE || mis[1] <= 0 || bsdf_pdf <= 0) return 0;
+    double d2 = dist * dist;
+    double at_vertex = emission_direction_pdf(s, e, cos_light) * mis[2] / d2 / mis[1];
+    double at_light = emitter_area_pdf(s, e) / (bsdf_pdf * fabs(cos_light) / d2);
+    return mis[0] * at_vertex * at_light;
+}
+/* lights.glsl:290-308: light paths from afar */
+static int emits_from_afar(const oscene *s, uint32_t l) {
+    uint32_t type = light_type(s, l);
+    return type == 4 || (type == 3 && L(s, l, 2, 3) < 1 - 1e-7);
+}
+static double afar_emission_pdf(const oscene *s, uint32_t l, v3 w) {
+    return K(s, 21, 0) * L(s, l, 6, 1) * light_hit_pdf(s, l, V(0, 0, 0), w, INF);
+}
+static double afar_first_density(const oscene *s, v3 x, v3 w, double c) {
+    double radius = K(s, 21, 1);
+    v3 offset = sub(x, Kv(s, 22));
+    v3 across = sub(offset, mul(w, dot(offset, w)));
+    if (dot(across, across) > radius * radius) return 0;
+    return c / (PI * radius * radius);
+}
+/* lights.glsl's cosine at a light toward the ray's origin */
+static double light_cos(const oscene *s, uint32_t l, v3 o, v3 d, double t) {
+    if (light_type(s, l) == 2) return fmax(0, -dot(d, norm(sub(add(o, mul(d, t)), Lv(s, l, 0)))));
+    return fmax(0, -dot(d, Lv(s, l, 3)));
+}
+/* intersect_shadow.comp:51-57: a light the camera sees stands in the way */
+static int camera_sees_light(const oscene *s, v3 o, v3 d, double reach) {
+    for (uint32_t l = 0; l < light_count(s); l++)
+        if (light_visible(s, l) && intersect_light(s, l, o, d, reach, 1) < reach) return 1;
+    return 0;
+}
+/* openpbr.glsl:248-261: a surface at a hit, and the one seen from its other side */
+static void make_surf(surf *sf, v3 normal, v3 geo, uint32_t m, int back) {
+    memset(sf, 0, sizeof(*sf));
+    sf->n = normal;
+    basis(normal, &sf->t, &sf->b);
+    sf->g = geo;
+    sf->m = m;
+    sf->back = back;
+}
+static void turned(const surf *a, surf *out) { make_surf(out, neg(a->n), neg(a->g), a->m, !a->back); }
+/* openpbr.glsl:417-449: light paths' lobe choice, Fresnel at normal incidence */
+static void light_lobes(surf *s) {
+    double f0 = (s->ior - 1) / (s->ior + 1), spec_alb = clampd(s->sw * f0 * f0, 0, 1), coat_alb = 0;
+    if (s->coat > 0) { double c0 = (s->ceta - 1) / (s->ceta + 1); coat_alb = c0 * c0; }
+    double sub_rgb = mixd(1, s->lcoat * (1 - coat_alb), s->coat), di = 1 - s->metal;
+    choose(s, s->coat * coat_alb, sub_rgb * s->metal, sub_rgb * di * spec_alb * s->lspec,
+           sub_rgb * di * s->trans * (1 - spec_alb) * s->ltrans, sub_rgb * di * (1 - s->trans) * (1 - spec_alb) * s->lbase);
+}
+/* connect.glsl:47-63: the light sampler's pdf of `to` having arrived from `from` */
+static double light_sampler_pdf(const oscene *sc, const surf *s, v3 from, v3 to, double hero, const oprog *g) {
+    int same = dot(from, s->g) >= 0;
+    surf v;
+    if (same) v = *s; else turned(s, &v);
+    v3 wo = to_local(&v, from), wi = to_local(&v, to);
+    if ((dot(to, v.g) > 0) != (wi.z > 0)) return 0;
+    if (!same) read_material(sc, &v, hero, g);
+    light_lobes(&v);
+    return surface_pdf(&v, wi, wo);
+}
+
 /* ------------------------------------------------------------ the path */
 
 static double mis_weight(double a, double b) { a = fmin(a, 1e18); b = fmin(b, 1e18); return 1.0 / (1 + a * a + b * b); }
@@ -1490,7 +1668,10 @@ static int shadow(const oscene *s, v3 o, v3 d, double reach, double *kept) {
     return 0;
 }
 
-static spec miss_radiance(const oscene *s, uint32_t bounce, v3 rd, double ray_pdf, spec lambda) {
+/* shade_miss.comp:12-61; `mis` (light tracing) the path's density ratios and
+ * `vertex` its last vertex. */
+static spec miss_radiance(const oscene *s, uint32_t bounce, v3 rd, double ray_pdf, spec lambda, const double *mis,
+                          v3 vertex) {
     spec rad = spec_of(s, &s->constants[4], &s->constants[8], lambda);
     for (uint32_t l = 0; l < light_count(s); l++) {
         uint32_t type = light_type(s, l);
@@ -1501,7 +1682,12 @@ static spec miss_radiance(const oscene *s, uint32_t bounce, v3 rd, double ray_pd
         }
         double w = 1.0;
         if (bounce == 0) { if (!light_visible(s, l)) continue; }
-        else if (ray_pdf > 0) w = mis_weight(light_hit_pdf(s, l, V(0, 0, 0), rd, INF) * light_select_pdf(s, l) / ray_pdf, 0.0);
+        else if (ray_pdf > 0) {
+            double traced = 0;
+            if (mis && emits_from_afar(s, l) && mis[1] > 0)
+                traced = mis[0] * (afar_first_density(s, vertex, rd, mis[2]) / mis[1]) * (afar_emission_pdf(s, l, rd) / ray_pdf);
+            w = mis_weight(light_hit_pdf(s, l, V(0, 0, 0), rd, INF) * light_select_pdf(s, l) / ray_pdf, traced);
+        }
         rad = sadd(rad, sscale(type == 4 ? dome_radiance(s, l, rd, lambda) : light_radiance(s, l, lambda), w));
     }
     return rad;
@@ -1536,6 +1722,10 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
     spec throughput = S1(1), radiance = S1(0);
     uint32_t bounces = 0;
     double ray_pdf = 0.0, survival_scale = 1.0;
+    /* light tracing: the density ratios the path carries (camera.comp:47-49) */
+    int tracing = tracing_on(s);
+    double mis[4] = {0, 0, 0, 0};
+    if (tracing) { mis[0] = paths_per_sample(s); mis[3] = camera_pdf(s, rd); }
     v3 vertex = origin;
     v3 prev_normal = V(0, 0, 1);
     uint32_t rounds = s->max_bounces + 1 + s->through;
@@ -1552,7 +1742,7 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
             if (t < ht) { ht = t; what = 0x80000000u | l; }
         }
         if (what == NONE) {
-            spec c = smul(throughput, miss_radiance(s, total, rd, ray_pdf, lambda));
+            spec c = smul(throughput, miss_radiance(s, total, rd, ray_pdf, lambda, tracing ? mis : NULL, vertex));
             double cv = K(s, 0, 2), m = spec_max(s, c);
             if (total > 0 && cv > 0 && m > cv) c = sscale(c, cv / m);
             radiance = sadd(radiance, c);
@@ -1566,7 +1756,8 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
                 uint32_t em = (uint32_t)L(s, light, 6, 3);
                 double reach = len(sub(vertex, add(origin, mul(rd, ht))));
                 double chance = (1 - K(s, 8, 2)) * tree_pdf(s, em, vertex, prev_normal);
-                w = mis_weight(light_hit_pdf(s, light, vertex, rd, reach) * chance / ray_pdf, 0);
+                double traced = tracing ? camera_hit_ratio(s, mis, em, light_cos(s, light, vertex, rd, reach), reach, ray_pdf) : 0;
+                w = mis_weight(light_hit_pdf(s, light, vertex, rd, reach) * chance / ray_pdf, traced);
             }
             radiance = sadd(radiance, clamp_ind(s, sscale(smul(throughput, light_radiance(s, light, lambda)), w), total));
             break;
@@ -1579,6 +1770,8 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
         const uint32_t *attr = &s->shading[tri * 4];
         v3 x = add(origin, mul(rd, ht));
         double reach = len(sub(vertex, x));
+        double density = (total == 0 ? mis[3] : ray_pdf) * fabs(dot(rd, geo)) / (reach * reach);
+        double back_reach = mis[2] / (reach * reach);
         double uo[4];
         sample4(s, pixel, index, 4 + 3 * total, uo);
         double op = clampd(M(s, attr[3], 12, 0), 0, 1);
@@ -1602,7 +1795,8 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
             uint32_t em = triangle_emitter(s, tri);
             if (total > 0 && ray_pdf > 0 && em != NONE) {
                 double chance = (1 - K(s, 8, 2)) * tree_pdf(s, em, vertex, prev_normal);
-                w = mis_weight(triangle_hit_pdf(s, tri, rd, reach) * chance / ray_pdf, 0);
+                double traced = tracing ? camera_hit_ratio(s, mis, em, fabs(dot(rd, geo)), reach, ray_pdf) : 0;
+                w = mis_weight(triangle_hit_pdf(s, tri, rd, reach) * chance / ray_pdf, traced);
             }
             radiance = sadd(radiance, clamp_ind(s, sscale(smul(throughput, material_color(s, attr[3], 6, lambda)), w), total));
         }
@@ -1657,11 +1851,12 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
         spec emitted = S1(0);
         double chance = 0;
         int sampled = 0;
-        uint32_t emitter = NONE;
+        uint32_t emitter = NONE, afar = NONE;
         if (ul[0] < p_distant) {
             double share;
             uint32_t light = pick_distant(s, ul[0] / p_distant, &share);
             if (light != NONE && share > 0 && sample_light(s, light, x, ul[1], ul[2], &ls)) {
+                afar = light;
                 chance = p_distant * share;
                 emitted = light_type(s, light) == 4 ? dome_radiance(s, light, ls.direction, lambda)
                                                     : light_radiance(s, light, lambda);
@@ -1682,10 +1877,29 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
             if ((side > 0) == (wi.z > 0) && (side > 0 || sf.pt > 0)) {
                 double bp;
                 spec fv = evaluate(&sf, wi, wo, &bp);
-                if (guided > 0) bp = guided * guide_pdf(s->guide, cell, ls.direction) + (1 - guided) * bp;
+                if (guided > 0 && !tracing) bp = guided * guide_pdf(s->guide, cell, ls.direction) + (1 - guided) * bp;
                 if (bp > 0) {
                     double lp = ls.pdf * chance;
-                    double w = ls.delta ? 1.0 : mis_weight(bp / lp, 0);
+                    /* the light tracer's density of this path over the light
+                     * sample's (shade_surface.comp:206-228) */
+                    double traced = 0;
+                    int viewed = prog.viewed;
+                    if (tracing && !ls.delta && emitter != NONE && density > 0) {
+                        double d2 = ls.distance * ls.distance;
+                        double at_last = total == 0 ? 1.0 : (viewed ? 0.0 :
+                            light_sampler_pdf(s, &sf, ls.direction, neg(rd), hero, &prog) * back_reach / mis[1]);
+                        double at_vertex = emission_direction_pdf(s, emitter, ls.cos_light) * fabs(side) / d2 / density;
+                        double at_light = emitter_area_pdf(s, emitter) / (bp * ls.cos_light / d2);
+                        traced = mis[0] * at_last * at_vertex * at_light;
+                    }
+                    if (tracing && afar != NONE && emits_from_afar(s, afar) && density > 0) {
+                        double at_last = total == 0 ? 1.0 : (viewed ? 0.0 :
+                            light_sampler_pdf(s, &sf, ls.direction, neg(rd), hero, &prog) * back_reach / mis[1]);
+                        double at_vertex = afar_first_density(s, x, ls.direction, fabs(side)) / density;
+                        double at_light = afar_emission_pdf(s, afar, ls.direction) / bp;
+                        traced = mis[0] * at_last * at_vertex * at_light;
+                    }
+                    double w = ls.delta ? 1.0 : mis_weight(bp / lp, (planted == 1 ? 0.0 : traced) * bp / lp);
                     spec c = clamp_ind(s, sscale(smul(smul(throughput, fv), emitted), fabs(wi.z) * (w / lp)), total);
                     pushed = 1;
                     nee = c;
@@ -1706,13 +1920,14 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
         double pdf;
         uint32_t lobe;
         spec weight;
-        double unguided = 1.0;
+        double unguided = 1.0, bsdf_only;
         if (uo[1] < guided) {
             /* a guide sample (shade_surface.comp:252-262) */
             wi = to_local(&sf, guide_sample(s->guide, cell, uo[2], uo[3], ub[3]));
             double bp;
             spec fv = evaluate(&sf, wi, wo, &bp);
             pdf = guided * guide_pdf(s->guide, cell, to_world(&sf, wi)) + (1 - guided) * bp;
+            bsdf_only = bp;
             if (pdf <= 0 || spec_max(s, fv) <= 0) break;
             unguided = pdf / fmax(bp, 1e-6 * pdf);
             weight = sscale(fv, fabs(wi.z) / pdf);
@@ -1722,6 +1937,7 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
             double again;
             spec fv = evaluate(&sf, wi, wo, &again);
             weight = sscale(fv, fabs(wi.z) / pdf);
+            bsdf_only = pdf;
             if (guided > 0) {
                 double mixed = guided * guide_pdf(s->guide, cell, to_world(&sf, wi)) + (1 - guided) * pdf;
                 unguided = mixed / pdf;
@@ -1756,6 +1972,16 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
         prev_normal = normal;
         origin = side > 0 ? from : offset_origin(x, neg(geo));
         vertex = origin;
+        if (tracing) {
+            /* settle this vertex's factor (shade_surface.comp:302-309) */
+            double at_last = total == 0 ? 1.0 : (prog.viewed ? 0.0 :
+                light_sampler_pdf(s, &sf, dir, neg(rd), hero, &prog) * back_reach / mis[1]);
+            mis[0] *= at_last;
+            mis[1] = density;
+            mis[2] = fabs(side);
+            mis[3] = 0;
+            pdf = bsdf_only;
+        }
         rd = dir;
         ray_pdf = pdf;
     }
@@ -1768,6 +1994,319 @@ static v3 trace_path(const oscene *s, uint32_t pixel, uint32_t index, int *nan, 
     return value;
 }
 
+/* One light path (light_emit.comp:21-137, then light_paths.glsl:31-189 at each
+ * surface it reaches): what its camera connections deliver, MIS-weighted, is
+ * added to `splat` (3 doubles a pixel, the film's terms). `slot` is the path's
+ * slot after the band's camera paths; `index` the sample. */
+static void trace_light(const oscene *s, uint32_t slot, uint32_t index, double *splat) {
+    uint32_t id = 0x80000000u | slot;
+    double u_point[4], u_pick[4];
+    sample4(s, id, index, 0, u_point);
+    sample4(s, id, index, 1, u_pick);
+    spec lambda = s->spectral ? sample_wavelengths(u_pick[0]) : S1(0);
+    double afar = K(s, 21, 0);
+    v3 origin, dir, vertex;
+    double ray_pdf, survival, mis[4];
+    spec throughput;
+    uint32_t emitter;
+    if (u_pick[1] < afar) {
+        /* from afar: a direction by the light's distribution, a point on the
+         * disk facing it */
+        double share;
+        uint32_t light = pick_distant(s, u_pick[1] / afar, &share);
+        if (light == NONE || share <= 0 || !emits_from_afar(s, light)) return;
+        lsample ls;
+        if (!sample_light(s, light, V(0, 0, 0), u_point[0], u_point[1], &ls)) return;
+        v3 w = ls.direction, t, b;
+        double radius = K(s, 21, 1), dx, dy;
+        basis(w, &t, &b);
+        concentric(u_point[2], u_point[3], &dx, &dy);
+        v3 start = add(add(add(Kv(s, 22), mul(w, 2 * radius)), mul(t, dx * radius)), mul(b, dy * radius));
+        double chance = afar * share * ls.pdf;
+        spec emitted = light_type(s, light) == 4 ? dome_radiance(s, light, w, lambda) : light_radiance(s, light, lambda);
+        throughput = sscale(emitted, PI * radius * radius / chance);
+        double most = spec_max(s, throughput);
+        if (!(most > 0)) return;
+        emitter = AFAR_EMITTER | light;
+        origin = start; dir = neg(w); ray_pdf = 0; vertex = start; survival = 1 / most;
+        mis[0] = 1; mis[1] = chance; mis[2] = 1; mis[3] = -1;
+    } else {
+        double chance;
+        if (!emitter_by_power(s, (u_pick[1] - afar) / fmax(1e-7, 1 - afar), &emitter, &chance)) return;
+        chance *= 1 - afar;
+        uint32_t code = emitter_code(s, emitter);
+        v3 q, n;
+        spec radiance;
+        double area;
+        int two_sided = 0;
+        if (code & 0x80000000u) {
+            uint32_t tri = code & ~0x80000000u;
+            v3 a = point_of(s, s->indices[tri * 3]), b = point_of(s, s->indices[tri * 3 + 1]), c = point_of(s, s->indices[tri * 3 + 2]);
+            double r = sqrt(u_point[0]);
+            q = add(add(mul(a, 1 - r), mul(b, r * (1 - u_point[1]))), mul(c, r * u_point[1]));
+            v3 cab = cross(sub(b, a), sub(c, a));
+            area = 0.5 * len(cab);
+            n = mul(cab, 1.0 / fmax(2 * area, 1e-30));
+            if (u_pick[2] < 0.5) n = neg(n);
+            two_sided = 1;
+            radiance = material_color(s, s->shading[tri * 4 + 3], 6, lambda);
+        } else {
+            uint32_t type = light_type(s, code);
+            if (type == 2) {
+                double r = L(s, code, 1, 3), z = 1 - 2 * u_point[0], sn = sqrt(fmax(0, 1 - z * z)), phi = 2 * PI * u_point[1];
+                n = V(sn * cos(phi), sn * sin(phi), z);
+                q = add(Lv(s, code, 0), mul(n, r));
+                area = 4 * PI * r * r;
+            } else {
+                double ax, ay;
+                if (type == 0) { ax = 2 * u_point[0] - 1; ay = 2 * u_point[1] - 1; }
+                else concentric(u_point[0], u_point[1], &ax, &ay);
+                q = add(add(Lv(s, code, 0), mul(Lv(s, code, 1), ax)), mul(Lv(s, code, 2), ay));
+                n = Lv(s, code, 3);
+                area = L(s, code, 3, 3);
+            }
+            radiance = light_radiance(s, code, lambda);
+        }
+        if (area <= 0) return;
+        double dx, dy;
+        concentric(u_point[2], u_point[3], &dx, &dy);
+        double cos_n = sqrt(fmax(0, 1 - (dx * dx + dy * dy)));
+        if (cos_n <= 0) return;
+        v3 t, b;
+        basis(n, &t, &b);
+        dir = norm(add(add(mul(t, dx), mul(b, dy)), mul(n, cos_n)));
+        double point_pdf = chance / area, direction_pdf = cos_n / (two_sided ? 2 * PI : PI);
+        throughput = sscale(radiance, cos_n / (point_pdf * direction_pdf));
+        double largest = spec_max(s, throughput);
+        if (!(largest > 0)) return;
+        origin = offset_origin(q, n); ray_pdf = direction_pdf; vertex = q; survival = 1 / largest;
+        mis[0] = 1; mis[1] = point_pdf; mis[2] = cos_n; mis[3] = -1;
+    }
+    double lpps = paths_per_sample(s);
+    uint32_t bounces = 0;
+    uint32_t rounds = s->max_bounces + 1 + s->through;
+    for (uint32_t round = 0; round < rounds; round++) {
+        uint32_t total = bounces & 0xff;
+        ohit h = closest(s, origin, dir, INF);
+        uint32_t what = h.tri;
+        double ht = h.t;
+        for (uint32_t l = 0; l < light_count(s); l++) {
+            if (total == 0 && !light_visible(s, l)) continue;
+            double tl = intersect_light(s, l, origin, dir, ht, 1);
+            if (tl < ht) { ht = tl; what = 0x80000000u | l; }
+        }
+        if (what == NONE || (what & 0x80000000u)) return;  /* light paths that leave, or reach a light, end */
+        uint32_t tri = what;
+        v3 p0 = point_of(s, s->indices[tri * 3]);
+        v3 geo = norm(cross(sub(point_of(s, s->indices[tri * 3 + 1]), p0), sub(point_of(s, s->indices[tri * 3 + 2]), p0)));
+        int back = dot(geo, dir) > 0;
+        if (back) geo = neg(geo);
+        const uint32_t *attr = &s->shading[tri * 4];
+        v3 x = add(origin, mul(dir, ht));
+        double uo[4];
+        sample4(s, id, index, 4 + 3 * total, uo);
+        double op = clampd(M(s, attr[3], 12, 0), 0, 1);
+        if (op < 1) {
+            float htf = (float)ht;
+            uint32_t bits;
+            memcpy(&bits, &htf, 4);
+            if (fract(uo[0] + (double)or_hash_u32(bits) * (1.0 / 4294967296.0)) >= op) {
+                origin = offset_origin(x, neg(geo));
+                continue;
+            }
+        }
+        if (total >= (uint32_t)K(s, 0, 1)) return;
+        if (s->spectral && M(s, attr[3], 12, 3) != 0 && lambda.v[1] >= 0) {
+            throughput = (spec){{throughput.v[0] * 4, 0, 0, 0}};
+            lambda = (spec){{lambda.v[0], -lambda.v[1], -lambda.v[2], -lambda.v[3]}};
+        }
+        double hero = lambda.v[1] < 0 ? lambda.v[0] : 0;
+        double b1 = h.u, b2 = h.v;
+        v3 normal = norm(add(add(mul(oct_decode(attr[0]), 1 - b1 - b2), mul(oct_decode(attr[1]), b1)), mul(oct_decode(attr[2]), b2)));
+        if (dot(normal, geo) < 0) normal = neg(normal);
+        double ul[4];
+        sample4(s, id, index, 2 + 3 * total, ul);
+        v3 lens = lens_point(s, ul[0], ul[1]), to_point;
+        uint32_t pixel = 0;
+        double cam_dist;
+        int on_film = camera_pixel(s, x, lens, &pixel, &to_point, &cam_dist);
+        oprog prog;
+        memset(&prog, 0, sizeof(prog));
+        int viewed = 0;
+        if (s->program_words) {
+            double uv[2];
+            triangle_uv(s, tri, b1, b2, uv);
+            run_program(s, &prog, attr[3], tri, x, back ? neg(normal) : normal, on_film ? neg(to_point) : neg(dir), uv);
+            viewed = prog.viewed;
+            if (viewed && !on_film) return;
+            if (driven(&prog, 9)) {
+                normal = norm(driven3(&prog, 11));
+                if (dot(normal, geo) < 0) normal = neg(normal);
+            }
+        }
+        surf lit;
+        make_surf(&lit, normal, geo, attr[3], back);
+        v3 toward_light = neg(dir);
+        read_material(s, &lit, hero, &prog);
+        colors(s, &lit, lambda, &prog);
+        v3 x_last = vertex;
+        double reach = len(sub(x_last, x)), cos_in = fabs(dot(dir, geo));
+        double density = ray_pdf * cos_in / (reach * reach), back_reach = mis[2] / (reach * reach);
+        if (total == 0 && (emitter & AFAR_EMITTER)) {
+            density = afar_first_density(s, x, neg(dir), cos_in);
+            back_reach = 1;
+            if (!(density > 0)) return;
+        }
+        double cos_shading_in = dot(toward_light, normal);
+        if (cos_shading_in <= 0 || cos_in < 1e-6) return;
+        double shading_in = cos_shading_in / cos_in;
+        if (on_film) {
+            /* connect to the camera */
+            v3 to_camera = neg(to_point);
+            double side = dot(to_camera, geo);
+            surf seen;
+            if (side > 0) seen = lit; else turned(&lit, &seen);
+            v3 wo = to_local(&seen, to_camera), wi = to_local(&seen, toward_light);
+            if ((dot(toward_light, seen.g) > 0) == (wi.z > 0) && wo.z > 0) {
+                if (side > 0) { orient(s, &seen, wo); update_substrate(&seen); }
+                else { read_material(s, &seen, hero, &prog); orient(s, &seen, wo); colors(s, &seen, lambda, &prog); }
+                double cbp;
+                spec f = evaluate(&seen, wi, wo, &cbp);
+                double camera_density = camera_pdf(s, to_point) * fabs(side) / (cam_dist * cam_dist);
+                spec contribution = sscale(smul(throughput, f), shading_in * camera_density / lpps);
+                if (spec_max(s, contribution) > 0) {
+                    double last = mis[1] > 0 ? cbp * back_reach / mis[1] : 0;
+                    double ratio = mis[0] * last * camera_density / density;
+                    double sample_ratio = total == 0 ? 0 : fmax(mis[3], 0);
+                    if (total == 0) {
+                        /* the light sample a camera path here would take toward x0 (light_paths.glsl:15-29) */
+                        if (cbp > 0) {
+                            v3 from = offset_origin(x, seen.g);
+                            if (emitter & AFAR_EMITTER) {
+                                uint32_t light = emitter & ~AFAR_EMITTER;
+                                sample_ratio = light_select_pdf(s, light) * light_hit_pdf(s, light, from, toward_light, INF) / cbp;
+                            } else {
+                                v3 off = sub(x_last, from);
+                                double rr = len(off);
+                                v3 dd = mul(off, 1.0 / rr);
+                                double ch = (1 - K(s, 8, 2)) * tree_pdf(s, emitter, from, seen.n);
+                                uint32_t code = emitter_code(s, emitter);
+                                double lpdf = (code & 0x80000000u) ? triangle_hit_pdf(s, code & ~0x80000000u, dd, rr)
+                                                                   : light_hit_pdf(s, code, from, dd, rr);
+                                sample_ratio = ch * lpdf / cbp;
+                            }
+                        }
+                    }
+                    double weight = mis_weight(ratio / lpps, ratio * sample_ratio / lpps);
+                    double margin = 4e-4 * fmax(1.0, fmax(fabs(x.x), fmax(fabs(x.y), fabs(x.z))));
+                    v3 so = offset_origin(x, seen.g);
+                    double rs = fmax(0, cam_dist - margin);
+                    v3 value = spec_to_film(s, sscale(contribution, weight), lambda);
+                    double kept;
+                    if (!camera_sees_light(s, so, to_camera, rs) && shadow(s, so, to_camera, rs, &kept) &&
+                        !isnan(value.x) && !isnan(value.y) && !isnan(value.z) && !isinf(value.x) && !isinf(value.y) &&
+                        !isinf(value.z)) {
+                        splat[(size_t)pixel * 3] += value.x * kept;
+                        splat[(size_t)pixel * 3 + 1] += value.y * kept;
+                        splat[(size_t)pixel * 3 + 2] += value.z * kept;
+                    }
+                }
+            }
+        }
+        /* scatter on */
+        if (viewed || total + 1 >= (uint32_t)K(s, 0, 1)) return;
+        double u[4];
+        sample4(s, id, index, 3 + 3 * total, u);
+        surf choosing = lit;
+        light_lobes(&choosing);
+        v3 wo_light = to_local(&lit, toward_light), wi_light;
+        double light_pdf;
+        uint32_t lobe;
+        if (!sample_dir(&choosing, wo_light, u[0], u[1], u[3], &wi_light, &light_pdf, &lobe)) return;
+        v3 direction = to_world(&lit, wi_light);
+        double side = dot(direction, geo);
+        if ((side > 0) != (wi_light.z > 0)) return;
+        surf seen;
+        if (side > 0) seen = lit; else turned(&lit, &seen);
+        v3 wo = to_local(&seen, direction), wi = to_local(&seen, toward_light);
+        if ((dot(toward_light, seen.g) > 0) != (wi.z > 0)) return;
+        if (side > 0) { orient(s, &seen, wo); update_substrate(&seen); }
+        else { read_material(s, &seen, hero, &prog); orient(s, &seen, wo); colors(s, &seen, lambda, &prog); }
+        double cbp;
+        spec f = evaluate(&seen, wi, wo, &cbp);
+        bounces = next_bounce(s, bounces, lobe);
+        if (bounces == NONE) return;
+        throughput = sscale(smul(throughput, f), shading_in * fabs(side) / light_pdf);
+        if (total + 1 >= 3) {
+            double survive = fmin(0.95, spec_max(s, throughput) * survival);
+            if (u[2] >= survive) return;
+            throughput = sscale(throughput, 1.0 / survive);
+        }
+        if (!(spec_max(s, throughput) > 0)) return;
+        double last = mis[1] > 0 ? cbp * back_reach / mis[1] : 0;
+        double sample_ratio = mis[3];
+        if (total == 0) {
+            sample_ratio = 0;
+            if (cbp > 0) {
+                v3 from = offset_origin(x, seen.g);
+                if (emitter & AFAR_EMITTER) {
+                    uint32_t light = emitter & ~AFAR_EMITTER;
+                    sample_ratio = light_select_pdf(s, light) * light_hit_pdf(s, light, from, toward_light, INF) / cbp;
+                } else {
+                    v3 off = sub(x_last, from);
+                    double rr = len(off);
+                    v3 dd = mul(off, 1.0 / rr);
+                    double ch = (1 - K(s, 8, 2)) * tree_pdf(s, emitter, from, seen.n);
+                    uint32_t code = emitter_code(s, emitter);
+                    double lpdf = (code & 0x80000000u) ? triangle_hit_pdf(s, code & ~0x80000000u, dd, rr)
+                                                       : light_hit_pdf(s, code, from, dd, rr);
+                    sample_ratio = ch * lpdf / cbp;
+                }
+            }
+        }
+        origin = offset_origin(x, side > 0 ? geo : neg(geo));
+        vertex = x;
+        dir = direction;
+        ray_pdf = light_pdf;
+        mis[0] *= last; mis[1] = density; mis[2] = fabs(side); mis[3] = sample_ratio;
+    }
+}
+
+typedef struct { const oscene *s; uint32_t sample, first; size_t begin, end; double *splat; } ljob;
+
+static void *light_run(void *arg) {
+    ljob *j = (ljob *)arg;
+    for (size_t i = j->begin; i < j->end; i++) trace_light(j->s, j->first + (uint32_t)i, j->sample, j->splat);
+    return NULL;
+}
+
+/* Every sample's light paths (light_count = K_CAMERA.x a sample, in the slots
+ * after the one band's camera paths), splatted into `splat` (3 doubles a
+ * pixel); returns the samples that traced them. */
+static uint32_t splat_light_paths(const oscene *s, int threads, double *splat) {
+    uint32_t count = (uint32_t)K(s, 14, 0);
+    if (!tracing_on(s) || count == 0) return 0;
+    size_t pixels = (size_t)s->width * s->height;
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    double *films = (double *)calloc((size_t)threads * pixels * 3, sizeof(double));
+    if (!films) return 0;
+    pthread_t th[64];
+    ljob jobs[64];
+    for (uint32_t sample = 0; sample < s->samples; sample++) {
+        for (int k = 0; k < threads; k++) {
+            jobs[k] = (ljob){s, sample, (uint32_t)pixels, (size_t)count * k / threads, (size_t)count * (k + 1) / threads,
+                             films + (size_t)k * pixels * 3};
+            pthread_create(&th[k], NULL, light_run, &jobs[k]);
+        }
+        for (int k = 0; k < threads; k++) pthread_join(th[k], NULL);
+    }
+    for (int k = 0; k < threads; k++)
+        for (size_t i = 0; i < pixels * 3; i++) splat[i] += films[(size_t)k * pixels * 3 + i];
+    free(films);
+    return s->samples;
+}
+
 typedef struct { const oscene *s; const uint32_t *pixels; size_t n; double *out; size_t begin, end; } rjob;
 
 static void *render_run(void *arg) {
@@ -1778,13 +2317,31 @@ static void *render_run(void *arg) {
         int nan = 0;
         for (uint32_t k = 0; k < j->s->samples; k++) sum = add(sum, trace_path(j->s, pixel, k, &nan, NULL, NULL));
         v3 mean = mul(sum, 1.0 / fmax((double)j->s->samples, 1.0));
-        if (j->s->spectral) {
-            v3 r0 = Kv(j->s, 3), r1 = Kv(j->s, 4), r2 = Kv(j->s, 5);
-            mean = V(dot(r0, mean), dot(r1, mean), dot(r2, mean));
-        }
         j->out[i * 3] = mean.x; j->out[i * 3 + 1] = mean.y; j->out[i * 3 + 2] = mean.z;
     }
     return NULL;
+}
+
+/* film_convert.comp:36-53 for the listed pixels whose camera means are in
+ * `out`: the light paths' splats over the samples that traced them added,
+ * then (spectral) XYZ under E to ACEScg. */
+static void finish_pixels(const oscene *s, int threads, const uint32_t *pixels, size_t n, double *out) {
+    size_t count = (size_t)s->width * s->height;
+    double *splat = tracing_on(s) ? (double *)calloc(count * 3, sizeof(double)) : NULL;
+    uint32_t light_samples = splat ? splat_light_paths(s, threads, splat) : 0;
+    for (size_t i = 0; i < n; i++) {
+        v3 mean = V(out[i * 3], out[i * 3 + 1], out[i * 3 + 2]);
+        if (light_samples) {
+            size_t px = pixels[i];
+            mean = add(mean, mul(V(splat[px * 3], splat[px * 3 + 1], splat[px * 3 + 2]), 1.0 / light_samples));
+        }
+        if (s->spectral) {
+            v3 r0 = Kv(s, 3), r1 = Kv(s, 4), r2 = Kv(s, 5);
+            mean = V(dot(r0, mean), dot(r1, mean), dot(r2, mean));
+        }
+        out[i * 3] = mean.x; out[i * 3 + 1] = mean.y; out[i * 3 + 2] = mean.z;
+    }
+    free(splat);
 }
 
 /* A guided render's pass over every pixel for one sample: film values and
@@ -1873,12 +2430,9 @@ int or_render_guided(const float *positions, const uint32_t *indices, const uint
     for (size_t i = 0; i < n; i++) {
         uint32_t px = pixels[i];
         v3 mean = mul(V(film[px * 3], film[px * 3 + 1], film[px * 3 + 2]), 1.0 / fmax((double)samples, 1.0));
-        if (s.spectral) {
-            v3 r0 = Kv(&s, 3), r1 = Kv(&s, 4), r2 = Kv(&s, 5);
-            mean = V(dot(r0, mean), dot(r1, mean), dot(r2, mean));
-        }
         out[i * 3] = mean.x; out[i * 3 + 1] = mean.y; out[i * 3 + 2] = mean.z;
     }
+    finish_pixels(&s, threads, pixels, n, out);
     if (keys_out) memcpy(keys_out, g.keys, (size_t)GUIDE_CELLS * 4);
     if (lobes_out) memcpy(lobes_out, g.lobes, (size_t)GUIDE_CELLS * (GUIDE_LOBES + 1) * 16);
     if (stats_out) memcpy(stats_out, g.stats, (size_t)GUIDE_CELLS * GUIDE_LOBES * 20);
@@ -1908,6 +2462,7 @@ int or_render(const float *positions, const uint32_t *indices, const uint32_t *s
         pthread_create(&th[k], NULL, render_run, &jobs[k]);
     }
     for (int k = 0; k < threads; k++) pthread_join(th[k], NULL);
+    finish_pixels(&s, threads, pixels, n, out);
     return 0;
 }
 

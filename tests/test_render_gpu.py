@@ -93,6 +93,38 @@ def test_many_lights_match_f64_oracle(device, spectral):
 
 
 @pytest.mark.parametrize("spectral", [False, True])
+@pytest.mark.parametrize("scene", ["cornell", "many_lights", "dome"])
+def test_light_traced_render_matches_f64_oracle(device, scene, spectral):
+    """Light tracing at half a light path a pixel (the reference's default)
+    against the oracle's light paths and three-technique MIS
+    (light_emit.comp, light_paths.glsl, connect.glsl): camera paths with
+    their light-tracing weights plus every light path's splats, per pixel.
+    The dome's light paths start from afar (a disk facing its metal ball). The
+    splats add by float atomics on the GPU, in path order in the oracle. Tolerances as the camera-path comparison's: image mean within
+    0.5%, 99% of pixels within 1e-3; the light paths change the oracle's
+    image (the splat plane is compared)."""
+    from luce_render import scenes
+    metal = {"base_color": [0.9, 0.8, 0.6], "base_metalness": 1.0, "specular_roughness": 0.05}
+    gset = {"cornell": scenes.cornell_render_set, "many_lights": scenes.many_lights_set,
+            "dome": lambda: scenes.dome_set(ball=metal)}[scene]()
+    width, height = {"cornell": (96, 96), "many_lights": (64, 64), "dome": (48, 36)}[scene]
+    sc, st, img = _render(device, gset, width=width, height=height, samples=16, spectral=spectral, max_bounces=8,
+                          light_tracing=0.5)
+    pixels = np.arange(width * height, dtype=np.uint32)
+    got, ref = _oracle_compare(sc, st, img, pixels)
+    import dataclasses
+    _, camera_only = _oracle_compare(sc, dataclasses.replace(st, light_tracing=0.0), img, pixels)
+    per_pixel = (np.abs(got - ref) / (np.abs(ref) + 1e-6)).max(axis=1)
+    close = (per_pixel < 1e-3).mean()
+    moved = (np.abs(ref - camera_only).max(axis=1) > 1e-4 * (np.abs(ref).max(axis=1) + 1e-6)).mean()
+    say(f"{scene} light traced spectral={spectral}: mean gpu {got.mean(0)} oracle {ref.mean(0)}, within 1e-3: "
+        f"{close:.4f}; light paths change {moved:.3f} of the oracle's pixels")
+    assert moved > (0.2 if scene == "dome" else 0.5)  # the dome's light paths reach what its ball reflects
+    assert np.all(np.abs(got.mean(0) - ref.mean(0)) <= 0.005 * ref.mean(0))
+    assert close >= 0.99
+
+
+@pytest.mark.parametrize("spectral", [False, True])
 def test_furnace_matches_f64_oracle(device, spectral):
     from luce_render import scenes
     sc, st, img = _render(device, scenes.furnace_set(), width=48, height=36, samples=8, spectral=spectral,
