@@ -100,6 +100,7 @@ struct lr_renderer {
     int launch_pass() {
         hipStream_t s = device->stream;
         Pass ps;
+        uint32_t carried = job->samples_in_pass(sample);  // as begin_pass_kernel counts
         bool train = job->params.learned && sample < job->params.train_samples;
         if (settings.profile) {
             (void)hipEventCreate(&ps.start);
@@ -126,7 +127,7 @@ struct lr_renderer {
         band += 1;
         if (band == job->bands) {
             band = 0;
-            sample += 1;
+            sample += carried;
         }
         ps.completes = sample;
         (void)hipEventCreateWithFlags(&ps.done, hipEventDisableTiming);

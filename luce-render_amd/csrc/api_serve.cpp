@@ -1,8 +1,9 @@
 // The serving form: many renders (requests) of different scenes, resolutions
 // and sample counts resident on one GPU, advanced together. Each step runs
-// one pass of every active request in shared launches (grid row y of every
-// kernel is request y), so small requests fill the chip together instead of
-// each leaving most of its 256 compute units idle. Requests are admitted in
+// one pass of every active request in shared launches (each request's items
+// laid end to end), so small requests fill the chip together instead of each
+// leaving most of its 256 compute units idle; a small request's pass carries
+// several samples, so it does not wait a step per sample behind large ones. Requests are admitted in
 // arrival order as their device memory fits the budget; a finished request
 // keeps its image until it is read and released. No reference counterpart
 // (the reference renders one view at a time).
@@ -134,7 +135,7 @@ int lr_server_submit(lr_server *s, lr_gpu_scene *scene, const lr_loaded *loaded,
     r->settings.trace_width = s->settings.trace_width;
     r->settings.trace_group = s->settings.trace_group;
     r->settings.trace_stack = s->settings.trace_stack;
-    r->settings.pool = settings->width * settings->height;  // one band: one sample a step
+    r->settings.pool = settings->width * settings->height;  // one band: a step runs one pass, of one or more samples
     r->settings.profile = 0;
     r->settings.use_graph = 0;
     std::memcpy(r->frame, frame, 64);
@@ -175,14 +176,19 @@ int lr_server_step(lr_server *s, uint32_t *active_out, uint32_t *pending_out) {
         rc = s->upload_params();
         if (rc) return rc;
         std::vector<RenderJob *> jobs;
-        for (Request *r : s->active) jobs.push_back(r->job.get());
+        std::vector<uint32_t> carried;
+        for (Request *r : s->active) {
+            jobs.push_back(r->job.get());
+            carried.push_back(r->job->samples_in_pass(r->samples));
+        }
         record_pass(jobs, s->params, s->device->stream, nullptr);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(s->device->stream);
         if (e != hipSuccess) return hip_fail(e, "server step");
         std::vector<Request *> still;
-        for (Request *r : s->active) {
-            r->samples += 1;
+        for (size_t k = 0; k < s->active.size(); k++) {
+            Request *r = s->active[k];
+            r->samples += carried[k];
             if (r->samples >= r->settings.samples) {
                 // Finished: its image stays until read and released; the
                 // rest of its memory goes back to the budget.

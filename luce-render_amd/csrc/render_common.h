@@ -81,6 +81,8 @@ struct PassState {
     uint32_t bands;         // bands a sample
     unsigned long long closest_rays;  // rays the closest-hit kernels have traced
     unsigned long long shadow_rays;   // shadow rays traced
+    uint32_t sub_samples;   // samples this pass carries (one band a sample: 1 to Params::pass_samples)
+    uint32_t unused;
 };
 
 // Launch parameters fixed for a view: pointers, sizes, the frame.
@@ -93,6 +95,11 @@ struct Params {
     uint32_t band_size;   // camera paths a band
     uint32_t light_paths; // light paths a sample (in the first band)
     uint32_t spectral;
+    // Samples a pass may carry when a sample is one band (its slots hold that
+    // many samples' camera paths, then their light paths), and the render's
+    // sample target, which the last pass's samples stop at.
+    uint32_t pass_samples;
+    uint32_t sample_target;
     const float4 *lights;
     const float4 *constants;
     const float4 *materials;
@@ -136,6 +143,33 @@ struct Params {
     PassState *pass;
     DeviceScene scene;
 };
+
+// ------------------------------------------------------ a pass's samples
+// A pass carries sub_samples samples of its band: camera slots j * band + i
+// (sample `sample + j`, the band's i-th camera path), then light slots
+// sub_samples * band + j * lights + i (sample `sample + j`, the i-th light
+// path). Every path draws its numbers by its sample and by the slot it would
+// hold in a pass of one sample, so a pass of several samples renders what
+// that many passes of one render.
+
+__device__ __forceinline__ uint32_t pass_cameras(const PassState &ps) { return ps.band_count * ps.sub_samples; }
+
+struct SlotIdentity {
+    uint32_t sample;  // the path's sample index
+    uint32_t slot;    // its slot in a pass of one sample
+};
+
+__device__ __forceinline__ SlotIdentity slot_identity(const PassState &ps, uint32_t path) {
+    if (ps.sub_samples <= 1u) return {ps.sample, path};
+    uint32_t cameras = pass_cameras(ps);
+    if (path < cameras) {
+        uint32_t j = path / ps.band_count;
+        return {ps.sample + j, path - j * ps.band_count};
+    }
+    uint32_t i = path - cameras;
+    uint32_t j = ps.light_count ? i / ps.light_count : 0u;
+    return {ps.sample + j, ps.band_count + (i - j * ps.light_count)};
+}
 
 // ------------------------------------------------------------- state access
 

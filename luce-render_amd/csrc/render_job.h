@@ -26,6 +26,7 @@ struct RenderJob {
     uint32_t rounds = 0;
     uint32_t slots = 0;
     uint32_t bands = 1;
+    uint32_t pass_samples = 1;  // samples a pass may carry (Params::pass_samples)
     int trace_variant = -1;
     size_t bytes = 0;  // device memory held
     std::vector<void *> buffers;
@@ -68,6 +69,12 @@ struct RenderJob {
     // needs other buffers (the caller makes a new job).
     static constexpr int kNotInPlace = -1;
     int restart_in_place(const lr_loaded *loaded, const lr_render_settings *settings, const float frame[16]);
+    // Samples the next pass carries when `done` samples are in.
+    uint32_t samples_in_pass(uint32_t done) const {
+        if (bands != 1u) return 1u;
+        uint32_t left = settings.samples > done ? settings.samples - done : 1u;
+        return pass_samples < left ? pass_samples : left;
+    }
     // Frees everything but the finished image; returns the bytes still held.
     size_t keep_image_only();
     // Device bytes a render of these settings holds (its admission cost).

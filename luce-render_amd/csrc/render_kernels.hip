@@ -180,7 +180,12 @@ __global__ void begin_pass_kernel(const Params *plist) {
         s->band_start = start;
         s->band_count = remaining < p.band_size ? remaining : p.band_size;
         s->light_count = s->band == 0 ? p.light_paths : 0u;
-        if (s->light_count > 0) s->light_samples += 1u;
+        // Samples this pass: one a band when a sample takes several, else up
+        // to pass_samples, stopping at the target (the host counts alike).
+        uint32_t left = p.sample_target > s->sample ? p.sample_target - s->sample : 1u;
+        uint32_t k = s->bands == 1u ? (p.pass_samples < left ? p.pass_samples : left) : 1u;
+        s->sub_samples = k > 0u ? k : 1u;
+        if (s->light_count > 0) s->light_samples += s->sub_samples;
     }
 }
 
@@ -191,7 +196,7 @@ __global__ void end_pass_kernel(const Params *plist) {
         s->band += 1u;
         if (s->band == s->bands) {
             s->band = 0;
-            s->sample += 1u;
+            s->sample += s->sub_samples;
             s->samples_done = s->sample;
         }
     }
@@ -233,7 +238,7 @@ __device__ __forceinline__ uint32_t tiled_pixel(uint32_t i, uint32_t width, uint
 
 template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(const Params *plist, uint32_t n) {
     __shared__ RowPlan plan;
-    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->band_count; });
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return pass_cameras(*q.pass); });
     __shared__ StagedPushes<1, 256 * kStageIterations> staged;
     staged.init({Q_CLOSEST});
     FOR_ROW_ITEMS(plan, n, total, g, r, path, active) {
@@ -242,12 +247,13 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(co
         const PassState &ps = *p.pass;
         bool push = false;
         if (active) {
-            uint32_t pixel = tiled_pixel(ps.band_start + path, p.width, p.height);
+            SlotIdentity who = slot_identity(ps, path);
+            uint32_t pixel = tiled_pixel(ps.band_start + who.slot, p.width, p.height);
             if (p.film[(size_t)FILM_HALF * p.pixels + pixel].w > 0.5f) {
                 st.set_u(SU_PIXEL, path, kNoneU);
             } else {
                 uint32_t width = p.width;
-                float4 u = sample4(p.seed, pixel, ps.sample, GROUP_CAMERA);
+                float4 u = sample4(p.seed, pixel, who.sample, GROUP_CAMERA);
                 float fx = (float)(pixel % width) + u.x, fy = (float)(pixel / width) + u.y;
                 V3 film = frame_corner(p) + frame_du(p) * fx + frame_dv(p) * fy;
                 bool ortho = p.frame[3] > 0.5f;
@@ -269,7 +275,7 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(co
                     st.set_v(SV_MIS, path, make_float4(light_paths_per_sample(p), 0.0f, 0.0f, camera_pdf(p, direction)));
                 for (uint32_t k = 0; k < 3u; k++) st.set_u(SU_GUIDE_CELL0 + k, path, kNoneU);
                 if constexpr (SPECTRAL)
-                    st.set_v(SV_LAMBDA, path, sample_wavelengths(sample4(p.seed, pixel, ps.sample, GROUP_WAVELENGTH).x));
+                    st.set_v(SV_LAMBDA, path, sample_wavelengths(sample4(p.seed, pixel, who.sample, GROUP_WAVELENGTH).x));
                 else
                     st.set_v(SV_LAMBDA, path, spec(0.0f));
                 push = true;
@@ -284,7 +290,7 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void camera_kernel(co
 
 template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kernel(const Params *plist, uint32_t n) {
     __shared__ RowPlan plan;
-    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->light_count; });
+    uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->light_count * q.pass->sub_samples; });
     __shared__ StagedPushes<1, 256 * kStageIterations> staged;
     staged.init({Q_CLOSEST});
     FOR_ROW_ITEMS(plan, n, total, g, r, i, active) {
@@ -292,11 +298,12 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void light_emit_kerne
         State st(p);
         const PassState &ps = *p.pass;
         bool push = false;
-        uint32_t path = ps.band_count + i;
+        uint32_t path = pass_cameras(ps) + i;
         if (active) {
-            uint32_t id = 0x80000000u | path;
-            float4 u_point = sample4(p.seed, id, ps.sample, GROUP_CAMERA);
-            float4 u_pick = sample4(p.seed, id, ps.sample, GROUP_WAVELENGTH);
+            SlotIdentity who = slot_identity(ps, path);
+            uint32_t id = 0x80000000u | who.slot;
+            float4 u_point = sample4(p.seed, id, who.sample, GROUP_CAMERA);
+            float4 u_pick = sample4(p.seed, id, who.sample, GROUP_WAVELENGTH);
             Spec lambda = SPECTRAL ? sample_wavelengths(u_pick.x) : spec(0.0f);
             float afar = K(p, K_EMIT).x;
             if (u_pick.y < afar) {
@@ -441,7 +448,7 @@ __global__ __launch_bounds__(G) void closest_kernel_rt(const Params *plist, uint
         // the render's slots, for which its spill area is laid out.
         const DeviceScene sc = p.scene;  // a copy: registers, not reloads after every store
         uint32_t nlights = light_count(p);
-        uint32_t cameras = p.pass->band_count;
+        uint32_t cameras = pass_cameras(*p.pass);
         uint32_t path = queue_path(p, queue, at);
         float4 o4 = st.v(SV_ORIGIN, path), d4 = st.v(SV_DIRECTION, path);
         V3 o = v3(o4), d = v3(d4);
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(256) void route_kernel(const Params *plist, uint32_
     FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
         const Params &p = plist[r];
         State st(p);
-        uint32_t cameras = p.pass->band_count;
+        uint32_t cameras = pass_cameras(*p.pass);
         uint32_t path = active ? queue_path(p, queue, at) : 0u;
         uint32_t what = active ? st.u(SU_HIT, path) : kNoneU;
         bool light_path = path >= cameras;
@@ -503,10 +510,11 @@ __global__ __launch_bounds__(128) void splat_kernel(const Params *plist, uint32_
         float reach = what == kNoneU ? kInf : st.v(SV_HIT, path).x;
         V3 o = v3(st.v(SV_ORIGIN, path)), d = v3(st.v(SV_DIRECTION, path));
         uint32_t bounces = st.u(SU_BOUNCE, path);
-        uint32_t seed = splat_seed(p, ps.sample, path, bounces, 0u);
+        SlotIdentity who = slot_identity(ps, path);
+        uint32_t seed = splat_seed(p, who.sample, who.slot, bounces, 0u);
         Spec throughput = st.v(SV_THROUGHPUT, path);
         float kept;
-        if (path >= ps.band_count) {
+        if (path >= pass_cameras(ps)) {
             kept = splat_transmittance(p, o, d, 0.0f, reach, seed, stack + threadIdx.x, 128u);
         } else {
             Spec added;
@@ -538,7 +546,8 @@ __global__ __launch_bounds__(128) void splat_shadow_kernel(const Params *plist, 
         uint32_t path = queue_path(p, Q_SHADOW, at);
         float4 o = st.v(SV_SHADOW_ORIGIN, path);
         V3 d = v3(st.v(SV_SHADOW_DIRECTION, path));
-        uint32_t seed = splat_seed(p, p.pass->sample, path, st.u(SU_BOUNCE, path), 0x68e31da4u);
+        SlotIdentity who = slot_identity(*p.pass, path);
+        uint32_t seed = splat_seed(p, who.sample, who.slot, st.u(SU_BOUNCE, path), 0x68e31da4u);
         float kept = splat_transmittance(p, v3(o), d, 0.0f, o.w, seed, stack + threadIdx.x, 128u);
         if (kept != 1.0f) st.set_v(SV_SHADOW_RADIANCE, path, st.v(SV_SHADOW_RADIANCE, path) * kept);
     }
@@ -619,7 +628,8 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
     float reach = distance(vertex, x);
     float density = (total == 0u ? mis.w : ray.w) * fabsf(dot(rd, geometric)) / (reach * reach);
     float back_reach = mis.z / (reach * reach);
-    float4 u_opacity = sample4(p.seed, pixel, ps.sample, group_opacity(total));
+    const uint32_t sample = slot_identity(ps, path).sample;
+    float4 u_opacity = sample4(p.seed, pixel, sample, group_opacity(total));
 
     float opacity = clampf(material_at(p, attributes.w, 12).x, 0.0f, 1.0f);
     if (opacity < 1.0f &&
@@ -690,7 +700,7 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
         if (suited && ready) guided = K(p, K_GUIDE).z * (p.learned ? surface.p_diffuse : 1.0f);
     }
 
-    float4 u_light = sample4(p.seed, pixel, ps.sample, group_light(total));
+    float4 u_light = sample4(p.seed, pixel, sample, group_light(total));
     float p_distant = K(p, K_TREE).z;
     LightSample s;
     bool pushed_shadow = false;
@@ -764,7 +774,7 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
         }
     }
 
-    float4 u = sample4(p.seed, pixel, ps.sample, group_bsdf(total));
+    float4 u = sample4(p.seed, pixel, sample, group_bsdf(total));
     V3 wi;
     Spec weight;
     float pdf;
@@ -870,7 +880,8 @@ __device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t pat
     Spec lambda = st.v(SV_LAMBDA, path);
     float4 vertex_row = st.v(SV_VERTEX, path);
     float4 mis = st.v(SV_MIS, path);
-    uint32_t id = 0x80000000u | path;
+    const SlotIdentity who = slot_identity(ps, path);
+    const uint32_t id = 0x80000000u | who.slot;
 
     uint32_t tri = what;
     V3 p0, e1, e2;
@@ -880,7 +891,7 @@ __device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t pat
     if (back) geometric = -geometric;
     uint4 attributes = p.shading[tri];
     V3 x = origin + rd * hit.x;
-    float4 u_opacity = sample4(p.seed, id, ps.sample, group_opacity(total));
+    float4 u_opacity = sample4(p.seed, id, who.sample, group_opacity(total));
 
     float opacity = clampf(material_at(p, attributes.w, 12).x, 0.0f, 1.0f);
     if (opacity < 1.0f &&
@@ -901,7 +912,7 @@ __device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t pat
     uint32_t pixel = 0;
     V3 to_point;
     float camera_distance;
-    float4 ul = sample4(p.seed, id, ps.sample, group_light(total));
+    float4 ul = sample4(p.seed, id, who.sample, group_light(total));
     V3 lens = lens_point(p, ul.x, ul.y);
     bool on_film = camera_pixel(p, x, lens, pixel, to_point, camera_distance);
     // The material's node graph as camera paths run it (a program reading the
@@ -970,7 +981,7 @@ __device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t pat
     }
 
     if (viewed || total + 1u >= (uint32_t)K(p, K_INFO).y) return out;
-    float4 u = sample4(p.seed, id, ps.sample, group_bsdf(total));
+    float4 u = sample4(p.seed, id, who.sample, group_bsdf(total));
     Surface choosing = lit;
     light_lobes(choosing);
     V3 wo_light = to_local(lit, toward_light);
@@ -1062,7 +1073,7 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(cons
         const PassState &ps = *p.pass;
         uint32_t nlights = light_count(p);
         uint32_t path = queue_path(p, Q_MISS, at);
-        if (path >= ps.band_count) continue;  // light paths that leave just end
+        if (path >= pass_cameras(ps)) continue;  // light paths that leave just end
         uint32_t bounce = bounce_total(st.u(SU_BOUNCE, path));
         float4 ray = st.v(SV_DIRECTION, path);
         V3 rd = v3(ray);
@@ -1132,7 +1143,7 @@ __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist, uint32_t
         uint32_t path = queue_path(p, Q_SHADOW, at);
         float4 o = st.v(SV_SHADOW_ORIGIN, path);
         V3 d = v3(st.v(SV_SHADOW_DIRECTION, path));
-        bool light_path = path >= ps.band_count;
+        bool light_path = path >= pass_cameras(ps);
         if (light_path && nlights && light_in_the_way(p, v3(o), d, o.w)) continue;
         uint32_t thread = at;
         bool blocked = trace_any<W, S>(sc, o.x, o.y, o.z, d.x, d.y, d.z, o.w, lds + threadIdx.x, G, thread);
@@ -1179,44 +1190,51 @@ __global__ __launch_bounds__(G) void shadow_kernel(const Params *plist, uint32_t
 
 // ------------------------------------------------------------- the film
 
+// One thread a camera path of the band; a pass of several samples adds that
+// path's samples to its pixel one after another, in sample order, as that
+// many passes of one sample would.
 template <bool SPECTRAL> __global__ __launch_bounds__(256) void finish_kernel(const Params *plist, uint32_t n) {
     __shared__ RowPlan plan;
     uint32_t total = plan.make(plist, n, [](const Params &q) { return q.pass->band_count; });
-    FOR_ROW_ITEMS(plan, n, total, g, r, path, active) {
+    FOR_ROW_ITEMS(plan, n, total, g, r, item, active) {
         if (!active) continue;
         const Params &p = plist[r];
         State st(p);
         const PassState &ps = *p.pass;
         bool guide_on = guiding(p);
-        uint32_t pixel = st.u(SU_PIXEL, path);
-        if (pixel == kNoneU) continue;
-        Spec radiance = st.v(SV_RADIANCE, path), lambda = st.v(SV_LAMBDA, path);
-        V3 value = spec_to_film<SPECTRAL>(p, radiance, lambda);
-        if (isnan(value.x) || isnan(value.y) || isnan(value.z) || isinf(value.x) || isinf(value.y) || isinf(value.z))
-            value = V3{0, 0, 0};
-        float4 &sum = p.film[pixel];
-        float4 s = sum;
-        sum = make_float4(s.x + value.x, s.y + value.y, s.z + value.z, s.w + 1.0f);
-        if (ps.sample & 1u) {
-            float4 &half = p.film[(size_t)p.pixels + pixel];
-            float4 h = half;
-            half = make_float4(h.x + 2.0f * value.x, h.y + 2.0f * value.y, h.z + 2.0f * value.z, h.w);
-        }
-        if (guide_on && !p.learned && hash_u32(pixel * 9781u + ps.sample) % GUIDE_TRAINING == 0u) {
-            float found = spec_luminance<SPECTRAL>(p, radiance, lambda);
-            for (uint32_t k = 0; k < GUIDE_RING; k++) {
-                uint32_t cell = st.u(SU_GUIDE_CELL0 + k, path);
-                if (cell == kNoneU) break;
-                float4 v = st.v(SV_GUIDE0 + k, path);
-                if (v.y <= 0.0f) continue;
-                V3 dir = octahedral_decode(st.u(SU_GUIDE_DIR0 + k, path));
-                // The light found past the vertex (finish.comp's found - v.z).
-                // Where nothing came, the two f32 sums differ only by their
-                // rounding; such a difference is no sample (it would count,
-                // and fit a lobe of kappa 1000 to one stray direction).
-                float gain = found - v.z;
-                if (gain <= 1e-5f * fmaxf(found, v.z)) gain = 0.0f;
-                guide_train(p, cell, dir, gain / v.y * v.w, v.x);
+        for (uint32_t j = 0; j < ps.sub_samples; j++) {
+            uint32_t path = j * ps.band_count + item;
+            uint32_t sample = ps.sample + j;
+            uint32_t pixel = st.u(SU_PIXEL, path);
+            if (pixel == kNoneU) continue;
+            Spec radiance = st.v(SV_RADIANCE, path), lambda = st.v(SV_LAMBDA, path);
+            V3 value = spec_to_film<SPECTRAL>(p, radiance, lambda);
+            if (isnan(value.x) || isnan(value.y) || isnan(value.z) || isinf(value.x) || isinf(value.y) || isinf(value.z))
+                value = V3{0, 0, 0};
+            float4 &sum = p.film[pixel];
+            float4 s = sum;
+            sum = make_float4(s.x + value.x, s.y + value.y, s.z + value.z, s.w + 1.0f);
+            if (sample & 1u) {
+                float4 &half = p.film[(size_t)p.pixels + pixel];
+                float4 h = half;
+                half = make_float4(h.x + 2.0f * value.x, h.y + 2.0f * value.y, h.z + 2.0f * value.z, h.w);
+            }
+            if (guide_on && !p.learned && hash_u32(pixel * 9781u + sample) % GUIDE_TRAINING == 0u) {
+                float found = spec_luminance<SPECTRAL>(p, radiance, lambda);
+                for (uint32_t k = 0; k < GUIDE_RING; k++) {
+                    uint32_t cell = st.u(SU_GUIDE_CELL0 + k, path);
+                    if (cell == kNoneU) break;
+                    float4 v = st.v(SV_GUIDE0 + k, path);
+                    if (v.y <= 0.0f) continue;
+                    V3 dir = octahedral_decode(st.u(SU_GUIDE_DIR0 + k, path));
+                    // The light found past the vertex (finish.comp's found - v.z).
+                    // Where nothing came, the two f32 sums differ only by their
+                    // rounding; such a difference is no sample (it would count,
+                    // and fit a lobe of kappa 1000 to one stray direction).
+                    float gain = found - v.z;
+                    if (gain <= 1e-5f * fmaxf(found, v.z)) gain = 0.0f;
+                    guide_train(p, cell, dir, gain / v.y * v.w, v.x);
+                }
             }
         }
     }
@@ -1254,7 +1272,7 @@ __global__ __launch_bounds__(256) void guide_fit_kernel(const Params *plist) {
 // sample from the adaptive minimum on (integrator.lucb:818-826).
 __device__ __forceinline__ bool adaptive_now(const Params &p) {
     const PassState &ps = *p.pass;
-    uint32_t taken = ps.sample + 1u;
+    uint32_t taken = ps.sample + ps.sub_samples;  // passes with a noise threshold carry one sample
     return K(p, K_ADAPTIVE).x > 0.0f && ps.band + 1u == ps.bands && (float)taken >= K(p, K_ADAPTIVE).y &&
            taken % 4u == 0u;
 }

@@ -55,7 +55,7 @@ template <bool SPECTRAL> struct PathRecords {
         State st(p);
         net::Sample s{};
         s.valid = false;
-        uint32_t band = p.pass->band_count;
+        uint32_t band = pass_cameras(*p.pass);
         if (band == 0) return s;
         uint32_t h = hash_u32(i * 0x9E3779B9u ^ salt);
         uint32_t path = h % band, k = hash_u32(h ^ 0x85EBCA6Bu) % GUIDE_RING;

@@ -63,7 +63,7 @@ class Loaded(ctypes.Structure):
 class RenderSettingsC(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("width", "height", "samples", "max_bounces", "spectral", "pool", "seed",
                                                 "guiding", "profile", "use_graph", "trace_width", "trace_group",
-                                                "trace_stack", "train_samples", "train_steps")]
+                                                "trace_stack", "train_samples", "train_steps", "pass_samples")]
 
 
 class Image(ctypes.Structure):

@@ -420,6 +420,9 @@ class RenderSettings:
     light_tracing: float = 0.5
     use_graph: bool = True
     trace: tuple = (0, 0, 0)  # traversal width, group, stack (0: the scene's)
+    # Samples one pass may carry when a sample is one band (0: as many as fit
+    # in about 2^25 path slots, at most 16; 1: the reference's one a pass).
+    pass_samples: int = 0
 
     @staticmethod
     def of(width, height, samples):

@@ -461,7 +461,7 @@ class Renderer:
         s = settings
         cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, s.pool, s.seed,
                                   int(s.guiding), 1 if s.profile else 0, 1 if s.use_graph else 0,
-                                  *tuple(s.trace)[:3], s.train_samples, s.train_steps)
+                                  *tuple(s.trace)[:3], s.train_samples, s.train_steps, int(s.pass_samples))
         self.frame16 = frame.pack()
         return ld, cs
 

@@ -54,7 +54,7 @@ class Server:
         ld.texture_table, ld.texture_count = _lib.ptr(loaded.texture_table), loaded.texture_count
         s = settings
         cs = _lib.RenderSettingsC(s.width, s.height, s.samples, s.max_bounces, 1 if s.spectral else 0, 0, s.seed,
-                                  int(s.guiding), 0, 0, 0, 0, 0, 0, 0)
+                                  int(s.guiding), 0, 0, 0, 0, 0, 0, 0, int(s.pass_samples))
         packed = frame.pack()
         rid = ctypes.c_uint64()
         _lib.check(_lib.lib().lr_server_submit(self.handle, gpu_scene.handle, ctypes.byref(ld), ctypes.byref(cs),

@@ -187,6 +187,39 @@ def test_bands_equal_one_pass(device):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("spectral", [False, True])
+def test_passes_of_several_samples_equal_passes_of_one(device, spectral):
+    """A pass that carries several samples of a one-band image (camera paths
+    of each sample, then their light paths, in one set of launches) renders
+    what one pass a sample renders: camera paths bit for bit, light paths
+    within 1e-4 of the image's peak (their splats add by float atomics). 19
+    samples at 8 a pass (passes of 8, 8 and 3) and at the default (16, 3)."""
+    from luce_render import render, scenes
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(scenes.cornell_render_set())
+    for lt in (0.0, 0.5):
+        images = []
+        for k in (1, 8, 0):
+            st = RenderSettings(96, 96, 19, max_bounces=8, spectral=spectral, light_tracing=lt, pass_samples=k)
+            say(f"passes of {k or 'default'} samples, spectral {spectral}, light tracing {lt}")
+            r = render.Renderer.create(device, sc, st)
+            try:
+                r.finish()
+                assert r.samples_done() == 19
+                stats = r.stats()
+                assert stats["light_samples"] == (19 if lt else 0)
+                images.append(r.read().copy())
+            finally:
+                r.destroy()
+        for img in images[1:]:
+            if lt == 0.0:
+                assert np.array_equal(img, images[0])
+            else:
+                rel = np.abs(img - images[0]).max() / np.abs(images[0]).max()
+                say(f"  max difference {rel:.2e} of the peak")
+                assert rel < 1e-4
+
+
 def test_renderer_lifecycle(device):
     """advance keeps at most two passes in flight and never waits; finish
     completes every sample; kernel_times charges the kinds with profile on."""

@@ -23,8 +23,10 @@ def scenes(device):
         g.destroy()
 
 
-def _serve(device, scenes, jobs, lt, budget=8 << 30):
-    """jobs: [(scene index, w, h, spp)]; returns the images in order."""
+def _serve(device, scenes, jobs, lt, budget=8 << 30, pass_samples=0):
+    """jobs: [(scene index, w, h, spp)]; returns the images in order and the
+    steps taken (a step runs one pass of every active request; a pass of a
+    small request carries up to 16 samples unless pass_samples says)."""
     from luce_render import serving
     from luce_render.loading import RenderSettings
     server = serving.Server(device, budget_bytes=budget)
@@ -32,7 +34,7 @@ def _serve(device, scenes, jobs, lt, budget=8 << 30):
         ids = []
         for (k, w, h, spp) in jobs:
             st = RenderSettings(w, h, spp, max_bounces=6, spectral=False, light_tracing=lt,
-                                background=(0.2, 0.2, 0.2) if k == 2 else (0.0, 0.0, 0.0))
+                                background=(0.2, 0.2, 0.2) if k == 2 else (0.0, 0.0, 0.0), pass_samples=pass_samples)
             ids.append(server.submit(scenes[k][0], scenes[k][1], st))
         steps = 0
         while server.busy() or steps == 0:
@@ -47,15 +49,20 @@ def _serve(device, scenes, jobs, lt, budget=8 << 30):
         server.destroy()
 
 
+@pytest.mark.parametrize("pass_samples", [1, 0])
 @pytest.mark.parametrize("lt", [0.0, 0.5])
-def test_alone_equals_mixed(device, scenes, lt):
+def test_alone_equals_mixed(device, scenes, lt, pass_samples):
+    """A request renders the same alone and beside four others sharing every
+    launch, with a sample a pass (20 steps for the longest request, 20
+    samples) and with small requests' passes carrying up to 16 samples (2
+    steps), alone and mixed alike."""
     target = (0, 96, 64, 12)
     mix = [(1, 128, 128, 5), target, (2, 64, 48, 20), (0, 32, 32, 3), (1, 200, 100, 12)]
-    say(f"serving mix lt {lt}")
-    (alone,), _ = _serve(device, scenes, [target], lt)
-    mixed, steps = _serve(device, scenes, mix, lt)
+    say(f"serving mix lt {lt} pass samples {pass_samples}")
+    (alone,), _ = _serve(device, scenes, [target], lt, pass_samples=pass_samples)
+    mixed, steps = _serve(device, scenes, mix, lt, pass_samples=pass_samples)
     got = mixed[1]
-    assert steps == 20
+    assert steps == (20 if pass_samples == 1 else 2)
     if lt == 0.0:
         assert np.array_equal(alone, got)  # camera paths: the same bits
     else:
@@ -71,15 +78,15 @@ def test_budget_admits_in_order(device, scenes):
     from luce_render import serving
     from luce_render.loading import RenderSettings
     jobs = [(0, 96, 64, 4), (1, 96, 64, 3), (0, 96, 64, 2)]
-    free, _ = _serve(device, scenes, jobs, 0.0)
+    free, _ = _serve(device, scenes, jobs, 0.0, pass_samples=1)
     # find one request's footprint, then a budget of one and a half of it
     server = serving.Server(device, budget_bytes=8 << 30)
     rid = server.submit(scenes[0][0], scenes[0][1], RenderSettings(96, 64, 1, max_bounces=6, spectral=False,
-                                                                     light_tracing=0.0))
+                                                                     light_tracing=0.0, pass_samples=1))
     need = server.status(rid)["bytes"]
     server.release(rid)
     server.destroy()
-    tight, steps = _serve(device, scenes, jobs, 0.0, budget=int(need * 1.5))
+    tight, steps = _serve(device, scenes, jobs, 0.0, budget=int(need * 1.5), pass_samples=1)
     assert steps == 4 + 3 + 2
     for a, b in zip(free, tight):
         assert np.array_equal(a, b)
@@ -148,7 +155,7 @@ def test_serving_scenes_alone_equal_mixed(device, serving_scenes, lt):
     image's peak)."""
     jobs = [(3, 1920, 1080, 2), (2, 512, 512, 4), (0, 256, 256, 6), (1, 1024, 768, 3)]
     mixed, steps = _serve(device, serving_scenes, jobs, lt, budget=64 << 30)
-    assert steps == 6
+    assert steps == 1  # every request's samples fit one pass (the 1080p one's two)
     for k, job in enumerate(jobs):
         say(f"serving scenes lt {lt}: request {job} alone")
         (alone,), _ = _serve(device, serving_scenes, [job], lt, budget=64 << 30)
