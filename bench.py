@@ -155,13 +155,22 @@ def bounce_rays(sc, frame, faces_t, rng):
     return rays
 
 
-def guiding_leg(device, stream, render, gset, width, height, ref_spp, kw, spp=64):
+def guiding_leg(device, stream, render, gset, width, height, ref_spp, kw, spp=64, methods=("plain", "grid", "learned")):
+    """Plain, grid-guided and learned-guided rendering against a long plain
+    reference, measured (no extrapolation): the plain method in the
+    reference's schedule (one sample a pass) at `spp` samples sets the time
+    budget T and the MSE to reach; every method is then rendered batched
+    (passes of several samples) for T of wall time (its sample count from its
+    measured time a sample; the time each took is reported), and at doubling
+    sample counts until its MSE falls below plain's, the time to plain's MSE
+    read between the two measured renders around it (log-log)."""
     from luce_render.loading import RenderSettings
     sc = render.compile_scene(gset)
     gpu = render.GpuScene.upload(device, sc)
 
-    def run(samples, seed, guiding):
-        st = RenderSettings(width, height, samples, spectral=False, guiding=guiding, seed=seed, **kw)
+    def run(samples, seed, guiding, pass_samples=0):
+        st = RenderSettings(width, height, samples, spectral=False, guiding=guiding, seed=seed,
+                            pass_samples=pass_samples, **kw)
         r = render.Renderer.create(device, sc, st, gpu_scene=gpu)
         t = render.Timer()
         render.synchronize(device)
@@ -177,15 +186,41 @@ def guiding_leg(device, stream, render, gset, width, height, ref_spp, kw, spp=64
     _, ref = run(ref_spp, 1, 0)
     out = {"workload": f"{width}x{height}, {sc.triangle_count()} triangles, reference {ref_spp} spp plain",
            "spp": spp}
-    plain_ms, plain_img = run(spp, 2, 0)
+    budget_ms, plain_img = run(spp, 2, 0, pass_samples=1)
     plain_mse = float(((plain_img - ref) ** 2).mean())
-    out["plain"] = {"ms": round(plain_ms, 2), "mse": plain_mse}
-    for name, mode in (("grid", 1), ("learned", 2)):
-        ms, img = run(spp, 2, mode)
-        mse = float(((img - ref) ** 2).mean())
-        needed = ms * (mse / plain_mse)
-        out[name] = {"ms": round(ms, 2), "mse": mse, "equal_time_mse": mse * ms / plain_ms,
-                     "time_to_plain_mse_ratio": round(needed / plain_ms, 3)}
+    out["plain_one_sample_a_pass"] = {"ms": round(budget_ms, 2), "mse": plain_mse}
+    for name, mode in (("plain", 0), ("grid", 1), ("learned", 2)):
+        if name not in methods:
+            continue
+        ms_spp, img = run(spp, 2, mode)
+        mse_spp = float(((img - ref) ** 2).mean())
+        n = max(1, int(spp * budget_ms / ms_spp))
+        ms_eq, img = run(n, 2, mode)
+        eq_mse = float(((img - ref) ** 2).mean())
+        # doubling (or halving) sample counts until two renders lie either
+        # side of plain's MSE
+        points = [(ms_spp, mse_spp)]
+        samples = spp
+        if mse_spp > plain_mse:
+            while points[-1][1] > plain_mse and samples < 16 * spp:
+                samples *= 2
+                ms, img = run(samples, 2, mode)
+                points.append((ms, float(((img - ref) ** 2).mean())))
+        else:
+            while points[0][1] <= plain_mse and samples > 1:
+                samples //= 2
+                ms, img = run(samples, 2, mode)
+                points.insert(0, (ms, float(((img - ref) ** 2).mean())))
+        reach = None
+        for (t0, e0), (t1, e1) in zip(points, points[1:]):
+            if e0 > plain_mse >= e1 and e1 > 0 and e0 > e1:
+                f = (math.log(e0) - math.log(plain_mse)) / (math.log(e0) - math.log(e1))
+                reach = math.exp(math.log(t0) + f * (math.log(t1) - math.log(t0)))
+        out[name] = {"ms_at_spp": round(ms_spp, 2), "mse_at_spp": mse_spp,
+                     "equal_time": {"samples": n, "ms": round(ms_eq, 2), "mse": eq_mse,
+                                    "mse_over_plain": round(eq_mse / plain_mse, 3)},
+                     "time_to_plain_mse_ratio": None if reach is None else round(reach / budget_ms, 3),
+                     "measured": [[round(t, 2), e] for t, e in points]}
     gpu.destroy()
     return out
 
@@ -205,8 +240,11 @@ def main():
                     help="samples of the full path-traced render of the same scene (0: skip)")
     ap.add_argument("--serving", type=int, default=40,
                     help="requests of the Poisson serving leg (40 requests/s over the four serving scenes; 0: skip)")
-    ap.add_argument("--guiding", default="960x540:1024",
+    ap.add_argument("--guiding", default="1920x1080:4096",
                     help="WxH:reference-spp of the guiding leg on the glossy workload (empty: skip)")
+    ap.add_argument("--guide-methods", default="plain,grid,learned",
+                    help="methods of the guiding leg (profiling one alone: learned)")
+    ap.add_argument("--guide-scenes", default="glossy,lamp_room", help="workloads of the guiding leg")
     ap.add_argument("--build", type=int, default=3,
                     help="device scene builds of the same scene to time against the host pipeline (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -433,11 +471,14 @@ def main():
         from luce_render.loading import OpenPbr
         guiding_stats = {}
         for name, gset, w, h, rspp, kw in (
-                ("glossy", scenes.glossy_set(), gw, gh, int(ref_spp), dict(max_bounces=8, light_tracing=0.5)),
+                ("glossy", scenes.glossy_set(soup=0), gw, gh, int(ref_spp), dict(max_bounces=8, light_tracing=0.5)),
                 ("lamp_room", scenes.lamp_room_set(), 960, 720, 1024,
                  dict(max_bounces=2, light_tracing=0.0,
                       material=OpenPbr(base_color=(0.3, 0.3, 0.3), specular_weight=0.0)))):
-            guiding_stats[name] = guiding_leg(device, stream, render, gset, w, h, rspp, kw)
+            if name not in args.guide_scenes.split(","):
+                continue
+            guiding_stats[name] = guiding_leg(device, stream, render, gset, w, h, rspp, kw,
+                                              methods=tuple(args.guide_methods.split(",")))
             log(f"guiding {name}:", guiding_stats[name])
 
     cpu = None

@@ -267,8 +267,9 @@ typedef struct lr_render_settings {
     /* Samples one pass may carry when a sample is one band (no reference
      * counterpart: the reference's passes carry one): 0 picks as many as fit
      * in about 2^25 path slots, at most 16; 1 is the reference's schedule.
-     * Renders that guide or stop pixels by a noise threshold take one. The
-     * image is the same either way (camera paths bit for bit). */
+     * Renders that stop pixels by a noise threshold take one, the learned
+     * guide one while it trains; the hash grid learns after each pass. An
+     * unguided image is the same either way (camera paths bit for bit). */
     uint32_t pass_samples;
 } lr_render_settings;
 

@@ -51,10 +51,12 @@ Shape shape_of(const lr_loaded *loaded, const lr_render_settings &st) {
     s.light_paths = loaded->light_paths < s.band ? loaded->light_paths : s.band;
     uint32_t one = s.band + s.light_paths;
     s.bands = (s.pixels + s.band - 1) / s.band;
-    // Several samples a pass only where a sample is one band and no state
-    // carries from sample to sample inside a pass: the guides learn after
-    // each sample, adaptive sampling checks between them.
-    bool alone = s.bands == 1u && st.guiding == 0u && !(loaded->constants[44] > 0.0f);
+    // Several samples a pass only where a sample is one band and adaptive
+    // sampling (which checks between samples) is off. A guided render's
+    // guide then learns after each pass of samples (the reference's after
+    // each sample: pass_samples 1); the learned guide trains one sample a
+    // pass.
+    bool alone = s.bands == 1u && !(loaded->constants[44] > 0.0f);
     uint64_t k = 1;
     if (alone) {
         k = st.pass_samples ? st.pass_samples : std::max<uint64_t>(1, std::min<uint64_t>(kMostPassSamples, kPassSlots / one));
@@ -494,7 +496,7 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         launch_render_stage(Stage::Route, sp, params, n, cur, blocks(256, slots), s);
         mark(KClosest);
         if (learned) {
-            launch_learned_infer(params, jobs[0]->params.band_size, s);
+            launch_learned_infer(params, jobs[0]->params.band_size * jobs[0]->pass_samples, s);
             mark(KRest);
         }
         launch_render_stage(programs ? Stage::ShadeProgram : Stage::Shade, sp, params, n, cur, blocks(128, band), s);

@@ -71,7 +71,7 @@ struct RenderJob {
     int restart_in_place(const lr_loaded *loaded, const lr_render_settings *settings, const float frame[16]);
     // Samples the next pass carries when `done` samples are in.
     uint32_t samples_in_pass(uint32_t done) const {
-        if (bands != 1u) return 1u;
+        if (bands != 1u || (params.learned && done < params.train_samples)) return 1u;
         uint32_t left = settings.samples > done ? settings.samples - done : 1u;
         return pass_samples < left ? pass_samples : left;
     }

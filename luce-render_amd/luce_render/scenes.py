@@ -678,9 +678,10 @@ def many_lights_set(count=40, seed=5):
     return gset
 
 
-def glossy_set(seed=7, soup=10000, light_radiance=30.0):
-    """The learned-guiding workload: the Cornell box and its two boxes with
-    `soup` random triangles (the probe's geometry), every surface a random
+def glossy_set(seed=7, soup=0, light_radiance=30.0):
+    """The learned-guiding workload: the Cornell box and its two boxes (the
+    plain render's geometry; `soup` random triangles of the probe's kind may
+    be added), every surface a random
     OpenPBR material (base colour U[0.2, 0.9]^3, metalness 1 with chance
     0.3, specular roughness U[0.05, 0.3]; the triangles in 16 material
     groups), lit only by 8 sphere lights of radius 0.02 just under the
