@@ -495,7 +495,11 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         if (splats) launch_render_stage(Stage::Splat, sp, params, n, cur, blocks(128, slots), s);
         launch_render_stage(Stage::Route, sp, params, n, cur, blocks(256, slots), s);
         mark(KClosest);
-        if (learned) {
+        // Shading consults the guide only at the first GUIDE_RING vertices
+        // below the bounce limit; a path's vertex count trails its round by
+        // at most the cut-out pass-throughs, so later rounds infer nothing
+        // anyone reads.
+        if (learned && b < std::min(rt::GUIDE_RING, jobs[0]->settings.max_bounces) + jobs[0]->through) {
             launch_learned_infer(params, jobs[0]->params.band_size * jobs[0]->pass_samples, s);
             mark(KRest);
         }
