@@ -22,6 +22,7 @@ import math
 import os
 import sys
 import time
+from dataclasses import replace
 from pathlib import Path
 
 import numpy as np
@@ -238,6 +239,8 @@ def main():
     ap.add_argument("--config", default="", help="width,group,stack[,refill] to skip the sweep")
     ap.add_argument("--render-spp", type=int, default=16,
                     help="samples of the full path-traced render of the same scene (0: skip)")
+    ap.add_argument("--render-tune", type=int, default=1,
+                    help="tune the render's own traversal variant on a ten-sample pass (0: the batch pick)")
     ap.add_argument("--serving", type=int, default=40,
                     help="requests of the Poisson serving leg (40 requests/s over the four serving scenes; 0: skip)")
     ap.add_argument("--guiding", default="1920x1080:4096",
@@ -416,8 +419,18 @@ def main():
         st.spectral = False
         st.max_bounces = 12
         st.trace = tuple(best[:3])
+        render_tune = None
+        if args.render_tune and not args.config:
+            # the integrator's own tracing kernels, tuned on one pass of ten
+            # samples of this render (every variant renders the same image)
+            from luce_render import autotune
+            pick, table = autotune.tune_render(device, sc, gpu, replace(st, samples=min(10, args.render_spp)),
+                                               widths=(4, 20, 24))
+            st.trace = tuple(pick)
+            render_tune = {",".join(map(str, k)): round(v, 3) for k, v in sorted(table.items(), key=lambda kv: kv[1])}
+            log("render variants (ms a sample):", render_tune)
         renderer = render.Renderer.create(device, sc, st, gpu_scene=gpu)
-        warm = render.Renderer.create(device, sc, RenderSettings(WIDTH, HEIGHT, 1, spectral=False, trace=tuple(best[:3])),
+        warm = render.Renderer.create(device, sc, RenderSettings(WIDTH, HEIGHT, 1, spectral=False, trace=st.trace),
                                       gpu_scene=gpu)
         warm.finish()
         warm.destroy()
@@ -436,7 +449,8 @@ def main():
                         "samples_per_s": round(args.render_spp * WIDTH * HEIGHT / (ms * 1e-3), 1),
                         "mrays_per_s_in_render": round(rays / (ms * 1e-3) / 1e6, 1),
                         "closest_rays": stats["closest_rays"], "shadow_rays": stats["shadow_rays"],
-                        "finite": bool(np.all(np.isfinite(img)))}
+                        "finite": bool(np.all(np.isfinite(img))), "trace_config": list(st.trace),
+                        "variants_ms_per_sample": render_tune}
         log("render:", render_stats)
 
     # The serving form: Poisson arrivals (40 requests/s) over the four serving

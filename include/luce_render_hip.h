@@ -144,7 +144,7 @@ int lr_probe(lr_device *device, const lr_gpu_scene *scene, const float frame[16]
 /* Traversal configuration: the node layout (width 2, 4, 8 with float child
  * boxes; LR_QUANTIZED | 4 and LR_QUANTIZED | 8 with 8-bit child boxes, each
  * holding its float box), workgroup size (64, 128, 256),
- * LDS stack entries per lane (16, 24, 32) and the ray schedule of the ray-batch
+ * LDS stack entries per lane (16, 24, 32, 48) and the ray schedule of the ray-batch
  * launches: refill 0 gives each thread one ray; refill k (1..64) runs
  * persistent waves that take new rays from a shared counter once k of their
  * lanes are idle (the probe keeps one ray per thread). Every configuration returns the same hits. No reference

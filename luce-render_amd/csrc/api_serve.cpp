@@ -107,7 +107,7 @@ int lr_server_create(lr_device *device, const lr_server_settings *settings, lr_s
     if (!device || !settings || !out) return fail(LR_INVALID, "lr_server_create needs its arguments");
     *out = nullptr;
     if (variant_index(settings->trace_width, settings->trace_stack, settings->trace_group) < 0)
-        return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 16, 24, 32; groups 64, 128, 256");
+        return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 16, 24, 32, 48; groups 64, 128, 256");
     (void)hipSetDevice(device->ordinal);
     std::unique_ptr<lr_server> s(new lr_server());
     s->device = device;

@@ -427,7 +427,7 @@ void lr_scene_destroy(lr_gpu_scene *scene) {
 int lr_scene_set_trace_config(lr_gpu_scene *scene, const lr_trace_config *config) {
     if (!scene || !config) return fail(LR_INVALID, "lr_scene_set_trace_config needs a scene and a config");
     if (variant_index(config->width, config->stack, config->group) < 0)
-        return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 16, 24, 32; groups 64, 128, 256");
+        return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 16, 24, 32, 48; groups 64, 128, 256");
     if (config->refill > 64) return fail(LR_INVALID, "refill is 0 (one ray a thread) or 1..64 idle lanes");
     scene->config = *config;
     return 0;

@@ -295,7 +295,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     uint32_t stack = st.trace_stack ? st.trace_stack : sc->config.stack;
     uint32_t group = st.trace_group ? st.trace_group : sc->config.group;
     trace_variant = variant_index(w, stack, group);
-    if (trace_variant < 0) return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 16, 24, 32; groups 64, 128, 256");
+    if (trace_variant < 0) return fail(LR_INVALID, "traversal layouts are 2, 4, 8, 20, 24; stacks 16, 24, 32, 48; groups 64, 128, 256");
     settings.trace_width = w;
     settings.trace_stack = stack;
     settings.trace_group = group;

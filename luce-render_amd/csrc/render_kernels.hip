@@ -16,6 +16,7 @@
 #include "render_guide.h"
 #include "render_program.h"
 #include "render_splats.h"
+#include "trace_kernels.h"
 
 namespace lr {
 namespace rt {
@@ -1482,20 +1483,21 @@ template <int W, int S, int G> void fill_trace(RenderTraceVariant &v) {
     v.group = G;
 }
 template <int W, int S> void fill_trace_g(RenderTraceVariant *t, int wi, int si) {
-    fill_trace<W, S, 64>(t[(wi * 3 + si) * 3 + 0]);
-    fill_trace<W, S, 128>(t[(wi * 3 + si) * 3 + 1]);
-    fill_trace<W, S, 256>(t[(wi * 3 + si) * 3 + 2]);
+    fill_trace<W, S, 64>(t[(wi * kStackSizes + si) * 3 + 0]);
+    fill_trace<W, S, 128>(t[(wi * kStackSizes + si) * 3 + 1]);
+    fill_trace<W, S, 256>(t[(wi * kStackSizes + si) * 3 + 2]);
 }
 template <int W> void fill_trace_s(RenderTraceVariant *t, int wi) {
     fill_trace_g<W, 16>(t, wi, 0);
     fill_trace_g<W, 24>(t, wi, 1);
     fill_trace_g<W, 32>(t, wi, 2);
+    fill_trace_g<W, 48>(t, wi, 3);
 }
 
 }  // namespace
 
 const RenderTraceVariant *render_trace_variants() {
-    static RenderTraceVariant table[45];
+    static RenderTraceVariant table[kTraceVariants];
     static bool ready = false;
     if (!ready) {
         fill_trace_s<2>(table, 0);

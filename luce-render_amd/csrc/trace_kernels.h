@@ -27,7 +27,9 @@ struct TraceVariant {
                               uint32_t max_blocks, hipStream_t stream);
 };
 
-// 45 variants: index (layout 2/4/8/20/24, stack 16/24/32, group 64/128/256).
+// 60 variants: index (layout 2/4/8/20/24, stack 16/24/32/48, group 64/128/256).
+constexpr int kStackSizes = 4;
+constexpr int kTraceVariants = 5 * kStackSizes * 3;
 const TraceVariant *trace_variants();
 int variant_index(uint32_t width, uint32_t stack, uint32_t group);
 

@@ -258,21 +258,22 @@ template <int W, int S, int G> void fill(TraceVariant &v) {
 }
 
 template <int W, int S> void fill_g(TraceVariant *table, int wi, int si) {
-    fill<W, S, 64>(table[(wi * 3 + si) * 3 + 0]);
-    fill<W, S, 128>(table[(wi * 3 + si) * 3 + 1]);
-    fill<W, S, 256>(table[(wi * 3 + si) * 3 + 2]);
+    fill<W, S, 64>(table[(wi * kStackSizes + si) * 3 + 0]);
+    fill<W, S, 128>(table[(wi * kStackSizes + si) * 3 + 1]);
+    fill<W, S, 256>(table[(wi * kStackSizes + si) * 3 + 2]);
 }
 
 template <int W> void fill_s(TraceVariant *table, int wi) {
     fill_g<W, 16>(table, wi, 0);
     fill_g<W, 24>(table, wi, 1);
     fill_g<W, 32>(table, wi, 2);
+    fill_g<W, 48>(table, wi, 3);
 }
 
 }  // namespace
 
 const TraceVariant *trace_variants() {
-    static TraceVariant table[45];
+    static TraceVariant table[kTraceVariants];
     static bool ready = false;
     if (!ready) {
         fill_s<2>(table, 0);
@@ -287,10 +288,10 @@ const TraceVariant *trace_variants() {
 
 int variant_index(uint32_t width, uint32_t stack, uint32_t group) {
     int wi = layout_index(width);
-    int si = stack == 16 ? 0 : stack == 24 ? 1 : stack == 32 ? 2 : -1;
+    int si = stack == 16 ? 0 : stack == 24 ? 1 : stack == 32 ? 2 : stack == 48 ? 3 : -1;
     int gi = group == 64 ? 0 : group == 128 ? 1 : group == 256 ? 2 : -1;
     if (wi < 0 || si < 0 || gi < 0) return -1;
-    return (wi * 3 + si) * 3 + gi;
+    return (wi * kStackSizes + si) * 3 + gi;
 }
 
 }  // namespace lr

@@ -12,7 +12,7 @@ from . import render
 
 WIDTHS = (2, 4, 8, 20, 24)  # 20, 24: the quantized 4- and 8-wide layouts
 GROUPS = (64, 128, 256)
-STACKS = (16, 24, 32)
+STACKS = (16, 24, 32, 48)
 REFILLS = (0, 32, 48)  # 0: one ray per thread; k: persistent waves refilled at k idle lanes
 
 
@@ -74,3 +74,45 @@ def load(path, key=""):
     if d.get("key") != key:
         return None
     return tuple(d["best"])
+
+
+RENDER_GROUPS = (64, 128, 256)
+
+
+def tune_render(device, scene, gpu_scene, settings, widths=WIDTHS, groups=RENDER_GROUPS, stacks=STACKS, repeats=1):
+    """The integrator's traversal variant: renders `settings` (a
+    RenderSettings; its sample count is the timed work, a pass or two is
+    enough) once per (layout, group, stack) and picks the fastest by wall
+    time on the device's stream. The render's tracing kernels take one item
+    per thread, so the ray schedule is not swept. Every variant returns the
+    same hits, so every render is the same image; the choice is by time
+    alone. Returns the fastest (width, group, stack) and {variant: ms a
+    sample}."""
+    import dataclasses
+    stream = render.device_stream(device)
+    table = {}
+    timer = render.Timer()
+    try:
+        for w in widths:
+            for g in groups:
+                for s in stacks:
+                    st = dataclasses.replace(settings, trace=(w, g, s))
+                    r = render.Renderer.create(device, scene, st, gpu_scene=gpu_scene)
+                    try:
+                        r.finish()  # code load, buffers, tables
+                        best = None
+                        for _ in range(repeats):
+                            r.restart(r.frame)  # the same view again: buffers kept, the film cleared
+                            render.synchronize(device)
+                            timer.begin(stream)
+                            r.finish()
+                            timer.end(stream)
+                            ms = timer.elapsed_ms()
+                            best = ms if best is None else min(best, ms)
+                        table[(w, g, s)] = best / max(st.samples, 1)
+                    finally:
+                        r.destroy()
+    finally:
+        timer.destroy()
+    best = min(table, key=table.get)
+    return best, table

@@ -84,7 +84,7 @@ def test_every_variant_agrees(device, cornell_probe, probe_scene):
     mf, mt, _, _ = _mirror(sc, frame)
     for width in WIDTHS:
         for group in (64, 128, 256):
-            for stack in (16, 24, 32):
+            for stack in (16, 24, 32, 48):
                 probe_scene.set_trace_config(width, group, stack)
                 say(f"probe 100x75 variant w{width} g{group} s{stack}")
                 faces, t = render.probe_arrays(device, probe_scene, frame)

@@ -176,6 +176,34 @@ def test_recorded_pass_equals_launched_pass(device):
     assert np.array_equal(a, b)
 
 
+def test_traversal_variants_render_the_same_image(device):
+    """The integrator's closest-hit and shadow kernels at any layout, group
+    and LDS stack (48 entries included) give the same image bit for bit
+    (camera paths only), which is what lets the render autotuner choose by
+    time alone; its table holds every variant it swept and it returns the
+    fastest."""
+    from luce_render import autotune, render, scenes
+    from luce_render.loading import RenderSettings
+    sc = render.compile_scene(scenes.cornell_render_set())
+    gpu = render.GpuScene.upload(device, sc)
+    try:
+        ref = None
+        for trace in ((4, 64, 16), (20, 128, 48), (24, 256, 32), (2, 64, 48), (8, 256, 48)):
+            st = RenderSettings(64, 64, 4, max_bounces=6, spectral=False, light_tracing=0.0, trace=trace)
+            say(f"render variant {trace}")
+            img = render.render(device, sc, st, gpu_scene=gpu)
+            if ref is None:
+                ref = img
+            assert np.array_equal(img, ref), trace
+        st = RenderSettings(64, 64, 2, max_bounces=6, spectral=False, light_tracing=0.0)
+        best, table = autotune.tune_render(device, sc, gpu, st, widths=(4, 20), groups=(64, 128), stacks=(16, 48))
+        say(f"render autotune: {best} of {table}")
+        assert set(table) == {(w, g, s) for w in (4, 20) for g in (64, 128) for s in (16, 48)}
+        assert table[best] == min(table.values())
+    finally:
+        gpu.destroy()
+
+
 def test_bands_equal_one_pass(device):
     """A pool smaller than the image traces it in bands (integrator.lucb:
     788-796): the same image as one band."""
