@@ -91,6 +91,22 @@ int lr_bvh_build(const float *positions, size_t points, const uint32_t *indices,
 int lr_bvh_build_boxes(const float *boxes, size_t count, int threads, lr_bvh_node *nodes, size_t *node_count,
                        uint32_t *order);
 
+/* References for a build over pieces of triangles (no reference counterpart:
+ * for scenes whose triangles overlap each other's boxes by the hundreds,
+ * where whole-triangle boxes make every ray test hundreds of triangles).
+ * Each triangle whose box is longer than `min_extent` on an axis is cut at
+ * that axis's middle, its pieces cut again while long, at most `max_depth`
+ * (0..8) cuts deep; `splittable` (null: all) marks the triangles that may be
+ * cut. Out: per reference six floats (low xyz, high xyz: the piece's box, a
+ * few ulps wider, inside the triangle's box) and its triangle, in triangle
+ * order. Null `boxes` or `tris` gives the count alone in `count`. The
+ * references are built over with lr_bvh_build_boxes; a leaf slot then holds
+ * the triangle of its reference (a triangle may sit in several leaves; hits
+ * are the same). */
+int lr_bvh_split_triangles(const float *positions, size_t points, const uint32_t *indices, size_t triangles,
+                           const uint8_t *splittable, float min_extent, uint32_t max_depth, int threads,
+                           float *boxes, uint32_t *tris, size_t capacity, size_t *count);
+
 /* build_bvh (src/render/bvh.lucb:66-217) on the GPU: the same tree, node
  * numbering and leaf order as lr_bvh_build (node boxes equal as values; a zero
  * plane that occurs with both signs may keep either sign). Host arrays as in

@@ -94,6 +94,9 @@ _SIGNATURES = {
     "lr_bvh_build": (ctypes.c_int, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.c_int, _P,
                                     ctypes.POINTER(ctypes.c_size_t), _P]),
     "lr_bvh_build_boxes": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_size_t), _P]),
+    "lr_bvh_split_triangles": (ctypes.c_int, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_float,
+                                              ctypes.c_uint32, ctypes.c_int, _P, _P, ctypes.c_size_t,
+                                              ctypes.POINTER(ctypes.c_size_t)]),
     "lr_bvh_build_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P,
                                            ctypes.POINTER(ctypes.c_size_t), _P]),
     "lr_scene_upload": (ctypes.c_int, [_P, ctypes.POINTER(SceneArrays), ctypes.POINTER(_P)]),

@@ -132,15 +132,14 @@ def test_poisson_serving_reports_latency(device, scenes):
 def serving_scenes(device):
     """The serving workload's four scenes (scenes.serving_sets: the Cornell
     box with 10,000 random triangles, the plain Cornell render, and the
-    Cornell render with 500,000 and 5,000,000 random triangles), the large
-    ones built on the device."""
-    from luce_render import render, scenes as S
-    out = []
-    for g in S.serving_sets():
-        sc = render.compile_scene(g, geometry=False)
-        gpu = render.GpuScene.build(device, sc)
-        say(f"serving scene: {sc.triangle_count()} triangles")
-        out.append((gpu, sc))
+    Cornell render with 500,000 and 5,000,000 random triangles) as the
+    server takes them (serving.prepare_scenes: the two clouds built over
+    pieces of their triangles, the others from their triangles, all on the
+    device)."""
+    from luce_render import scenes as S, serving
+    out = serving.prepare_scenes(device, S.serving_sets())
+    for _, sc in out:
+        say(f"serving scene: {sc.triangle_count()} leaf slots ({sc.split_pieces} pieces)")
     yield out
     for g, _ in out:
         g.destroy()
