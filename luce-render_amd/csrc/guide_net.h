@@ -2,19 +2,25 @@
 // of 64 ReLU units -> 32 outputs) that maps a surface point and its normal to
 // a mixture of four von Mises-Fisher lobes over incident directions, the same
 // distribution family the grid guide (render_guide.h) stores per cell. The
-// matrix products run on the CDNA matrix cores (v_mfma_f32_16x16x16_f16: FP16
-// operands, FP32 accumulation), 16 points per product, activations kept in
-// registers between layers. No reference counterpart: the reference guides
+// matrix products run on the CDNA4 matrix cores (v_mfma_f32_16x16x32_f16:
+// FP16 operands, FP32 accumulation, K = 32), 16 points per product,
+// activations kept in registers between layers. No reference counterpart: the reference guides
 // with the grid only; this network is trained online from the same path
 // records the grid learns from, and judged against the grid.
 //
-// Fragment layouts of v_mfma_f32_16x16x16_f16 (64 lanes; lane = c + 16 g):
-//   A[i][k] (16x16): lane i + 16 (k / 4), element k % 4
-//   B[k][j] (16x16): lane j + 16 (k / 4), element k % 4
-//   D[i][j] (16x16): lane j + 16 (i / 4), element i % 4
-// A layer computes Z^T = W^T H^T over 16 points (columns): A = W^T tiles,
-// B = H^T, and D (Z^T) is already the next layer's B, so activations never
-// leave their lanes.
+// Fragment layouts (64 lanes; lane = c + 16 g), checked on the device:
+//   v_mfma_f32_16x16x32_f16  A[i][k] (16x32): lane i + 16 (k / 8), element k % 8
+//                            B[k][j] (32x16): lane j + 16 (k / 8), element k % 8
+//   v_mfma_f32_16x16x16_f16  A[i][k] (16x16): lane i + 16 (k / 4), element k % 4
+//                            B[k][j] (16x16): lane j + 16 (k / 4), element k % 4
+//   both                     D[i][j] (16x16): lane j + 16 (i / 4), element i % 4
+// A layer computes Z^T = W^T H^T over 16 points (columns): A = W^T blocks,
+// B = H^T. A D tile holds rows 4g..4g+3 in lane group g, so two D tiles
+// (rows 0..15 and 16..31 of a 32-row block) side by side are a K = 32 B
+// fragment whose element e of lane group g is row 4g + e (e < 4) or
+// 16 + 4g + e - 4: the A blocks are packed in that order of K, and
+// activations never leave their lanes. The weight gradient (16 samples as
+// the inner dimension) keeps the K = 16 product.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,6 +30,7 @@ namespace lr {
 namespace net {
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kIn = 64, kHidden = 64, kOut = 32, kLobes = 4, kLayers = 4;
@@ -33,43 +40,60 @@ constexpr uint32_t kPosBins = 16, kNormalBins = 8;
 constexpr uint32_t kW3 = 3u * kIn * kHidden;
 constexpr uint32_t kBias = kW3 + kHidden * kOut;
 constexpr uint32_t kParams = kBias + 3u * kHidden + kOut;  // 14560
-// Tiles per layer (16x16 FP16 fragments): forward A = W^T (out tiles x in
-// steps), backward A = W (in tiles x out steps).
-constexpr uint32_t kTiles = 3u * 16u + 8u;  // 56
+// 16x32 FP16 blocks (a half8 a lane): forward A = W^T (16-output tiles x
+// 32-input blocks: 8 a hidden layer, 4 the output layer), backward A = W
+// (16-input tiles x 32-output blocks: 8, then 4).
+constexpr uint32_t kBlocks = 3u * 8u + 4u;  // 28
+constexpr size_t kFragmentHalves = (size_t)kBlocks * 64u * 8u;
 __host__ __device__ constexpr uint32_t layer_out(uint32_t l) { return l == 3u ? kOut : kHidden; }
 __host__ __device__ constexpr uint32_t weight_offset(uint32_t l) { return l * kIn * kHidden; }
 __host__ __device__ constexpr uint32_t bias_offset(uint32_t l) { return kBias + l * kHidden; }
-__host__ __device__ constexpr uint32_t tile_base(uint32_t l) { return l * 16u; }
+__host__ __device__ constexpr uint32_t block_base(uint32_t l) { return l * 8u; }
 
 // The network's device memory: master parameters, Adam moments, FP16
 // fragments for the forward and backward products, bounds and step state.
 struct NetBuffers {
     float *params;       // kParams
     float *m, *v;        // Adam moments
-    _Float16 *fwd;       // kTiles * 64 lanes * 4
-    _Float16 *bwd;       // kTiles * 64 lanes * 4
+    _Float16 *fwd;       // kBlocks * 64 lanes * 8
+    _Float16 *bwd;       // kBlocks * 64 lanes * 8
     float *bounds;       // lo xyz, 1 / extent xyz
     float *stats;        // [0] sum of sample weights of the current batch, [1] Adam step
 };
 
-// Index of W_l[in][out] in the forward fragments (A = W^T: row out, column in).
-__host__ __device__ inline uint32_t fwd_index(uint32_t l, uint32_t in, uint32_t out) {
-    uint32_t m = out / 16u, i = out % 16u, kk = in / 16u, g = (in % 16u) / 4u, r = in % 4u;
-    return ((tile_base(l) + m * 4u + kk) * 64u + i + 16u * g) * 4u + r;
+// Lane group and element of position p (0..31) of a K = 32 block: two D
+// tiles side by side.
+__host__ __device__ inline void k_slot(uint32_t p, uint32_t &g, uint32_t &e) {
+    g = (p % 16u) / 4u;
+    e = p % 4u + (p >= 16u ? 4u : 0u);
 }
-// Index of W_l[in][out] in the backward fragments (A = W: row in, column out).
+// Index of W_l[in][out] in the forward fragments (A = W^T: row out, K in).
+__host__ __device__ inline uint32_t fwd_index(uint32_t l, uint32_t in, uint32_t out) {
+    uint32_t m = out / 16u, i = out % 16u, kb = in / 32u, g, e;
+    k_slot(in % 32u, g, e);
+    return ((block_base(l) + m * 2u + kb) * 64u + i + 16u * g) * 8u + e;
+}
+// Index of W_l[in][out] in the backward fragments (A = W: row in, K out).
 __host__ __device__ inline uint32_t bwd_index(uint32_t l, uint32_t in, uint32_t out) {
-    uint32_t steps = layer_out(l) / 16u;
-    uint32_t m = in / 16u, i = in % 16u, kk = out / 16u, g = (out % 16u) / 4u, r = out % 4u;
-    return ((tile_base(l) + m * steps + kk) * 64u + i + 16u * g) * 4u + r;
+    uint32_t blocks = layer_out(l) / 32u;
+    uint32_t m = in / 16u, i = in % 16u, kb = out / 32u, g, e;
+    k_slot(out % 32u, g, e);
+    return ((block_base(l) + m * blocks + kb) * 64u + i + 16u * g) * 8u + e;
 }
 
-__device__ __forceinline__ half4_t load_frag(const _Float16 *frags, uint32_t tile, uint32_t lane) {
-    return *reinterpret_cast<const half4_t *>(frags + ((size_t)tile * 64u + lane) * 4u);
+__device__ __forceinline__ half8_t load_block(const _Float16 *frags, uint32_t block, uint32_t lane) {
+    return *reinterpret_cast<const half8_t *>(frags + ((size_t)block * 64u + lane) * 8u);
+}
+// Two D tiles' rows as one K = 32 B fragment.
+__device__ __forceinline__ half8_t pair(half4_t lo, half4_t hi) {
+    return half8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 }
 
 __device__ __forceinline__ float4_t mfma(half4_t a, half4_t b, float4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float4_t mfma32(half8_t a, half8_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // One-blob encoding (Mueller et al. 2019): a value in [0, 1] spread over
@@ -125,16 +149,17 @@ __device__ __forceinline__ half4_t to_half(float4_t x) {
     return h;
 }
 
-// Z^T = W_l^T H^T + b for 16 points; `in` the B fragments of H (4 steps).
-// Output tiles: layer_out(l) / 16.
+// Z^T = W_l^T H^T + b for 16 points; `in` the four D tiles of H (the
+// 64 inputs, two K = 32 blocks). Output tiles: layer_out(l) / 16.
 __device__ __forceinline__ void layer_forward(const _Float16 *fwd, const float *params, uint32_t l,
                                               const half4_t in[4], uint32_t lane, float4_t out[4]) {
     uint32_t g = lane / 16u;
     uint32_t tiles = layer_out(l) / 16u;
+    const half8_t b0 = pair(in[0], in[1]), b1 = pair(in[2], in[3]);
     for (uint32_t m = 0; m < tiles; m++) {
         float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (uint32_t kk = 0; kk < 4u; kk++) acc = mfma(load_frag(fwd, tile_base(l) + m * 4u + kk, lane), in[kk], acc);
+        acc = mfma32(load_block(fwd, block_base(l) + m * 2u, lane), b0, acc);
+        acc = mfma32(load_block(fwd, block_base(l) + m * 2u + 1u, lane), b1, acc);
         const float *b = params + bias_offset(l) + m * 16u + 4u * g;
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[r] += b[r];

@@ -80,8 +80,8 @@ int GuideNet::create(lr_device *dev, const float lo[3], const float hi[3], uint3
     b.params = (float *)alloc(kParams * 4);
     b.m = (float *)alloc(kParams * 4);
     b.v = (float *)alloc(kParams * 4);
-    b.fwd = (_Float16 *)alloc((size_t)kTiles * 64 * 4 * 2);
-    b.bwd = (_Float16 *)alloc((size_t)kTiles * 64 * 4 * 2);
+    b.fwd = (_Float16 *)alloc(kFragmentHalves * 2);
+    b.bwd = (_Float16 *)alloc(kFragmentHalves * 2);
     b.bounds = (float *)alloc(64);
     b.stats = (float *)alloc(64);
     partials = (float *)alloc((size_t)waves * (kParams + 1) * 4);

@@ -176,16 +176,16 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
                 atomicAdd(&part[bias_offset((uint32_t)l) + lane], sb);
             }
             if (l > 0) {
-                // dH^T = W dZ^T, then through the ReLU of layer l - 1
-                half4_t b[4];
-#pragma unroll
-                for (uint32_t q = 0; q < 4u; q++) b[q] = to_half(dz[q]);
+                // dH^T = W dZ^T (K = 32 blocks of outputs), then through the
+                // ReLU of layer l - 1
+                const uint32_t blocks = outs / 32u;
+                const half8_t b0 = pair(to_half(dz[0]), to_half(dz[1])), b1 = pair(to_half(dz[2]), to_half(dz[3]));
 #pragma unroll
                 for (uint32_t m = 0; m < 4u; m++) {
                     float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                    for (uint32_t q = 0; q < 4u; q++)
-                        if (q < steps) acc = mfma(load_frag(net.bwd, tile_base((uint32_t)l) + m * steps + q, lane), b[q], acc);
+                    acc = mfma32(load_block(net.bwd, block_base((uint32_t)l) + m * blocks, lane), b0, acc);
+                    if (blocks > 1u)
+                        acc = mfma32(load_block(net.bwd, block_base((uint32_t)l) + m * blocks + 1u, lane), b1, acc);
 #pragma unroll
                     for (uint32_t r = 0; r < 4u; r++) dz[m][r] = zs[l - 1][j][16u * m + 4u * g + r] > 0.0f ? acc[r] : 0.0f;
                 }

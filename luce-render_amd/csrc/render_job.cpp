@@ -230,7 +230,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     if (st->guiding == 1) b += rt::kGuideBytes;
     if (st->guiding == 2)
         b += (size_t)5 * s.slots * 16 + (size_t)kNetRows * (net::kParams + 1) * 4 + (size_t)net::kParams * 12 +
-             (size_t)net::kTiles * 64 * 16 + 128;
+             net::kFragmentHalves * 4 + 128;
     uint32_t w = st->trace_width ? st->trace_width : scene->config.width;
     uint32_t stack = st->trace_stack ? st->trace_stack : scene->config.stack;
     uint32_t group = st->trace_group ? st->trace_group : scene->config.group;
