@@ -137,6 +137,24 @@ __device__ __forceinline__ void encode(const float *bounds, float px, float py, 
     }
 }
 
+// The whole encoding of one point into `row` (64 halves, features in the
+// order encode's fragments take them: position x, y, z blobs, then the
+// normal's u and v blobs).
+__device__ __forceinline__ void encode_row(const float *bounds, float px, float py, float pz, float nx, float ny,
+                                           float nz, _Float16 *row) {
+    float u[3] = {(px - bounds[0]) * bounds[3], (py - bounds[1]) * bounds[4], (pz - bounds[2]) * bounds[5]};
+    float ou, ov;
+    octahedral_uv(nx, ny, nz, ou, ov);
+#pragma unroll
+    for (uint32_t kk = 0; kk < 3u; kk++) {
+        float x = fminf(fmaxf(u[kk], 0.0f), 1.0f);
+#pragma unroll
+        for (uint32_t f = 0; f < kPosBins; f++) row[16u * kk + f] = (_Float16)blob(x, f, kPosBins);
+    }
+#pragma unroll
+    for (uint32_t f = 0; f < 16u; f++) row[48u + f] = (_Float16)(f < 8u ? blob(ou, f, kNormalBins) : blob(ov, f - 8u, kNormalBins));
+}
+
 // Activations of one layer for 16 points: 4 tiles of D (= next B), FP32.
 struct Act {
     float4_t t[4];

@@ -243,7 +243,7 @@ int lr_guide_net_mixtures(lr_guide_net *net, const float *points, size_t n, floa
     if (e == hipSuccess) e = hipMalloc(&dout, n * 80);
     if (e == hipSuccess) e = hipMemcpyAsync(dp, points, n * 24, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
-        uint32_t blocks = (uint32_t)std::min<size_t>((n + 15) / 16, 16384);
+        uint32_t blocks = (uint32_t)std::min<size_t>((n + 63) / 64, 16384);
         ArrayPoints pts{(const float *)dp, (float *)dout};
         hipLaunchKernelGGL((infer_kernel<ArrayPoints>), dim3(blocks), dim3(64), 0, s, g.b, pts, (uint32_t)n);
         e = hipGetLastError();

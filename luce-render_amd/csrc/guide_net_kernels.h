@@ -249,25 +249,38 @@ __device__ __forceinline__ void adam_body(NetBuffers &net, float *partials, uint
 }
 
 
-// Mixtures of points, 16 a wave: `dst.point(i)` gives point i's position and
-// normal, `dst.store(i, mixture)` takes the result.
+// Mixtures of points, 64 a wave at a time: each lane gathers one point and
+// encodes it into its row of LDS, the matrix cores take the rows 16 at a
+// time (four forward passes), each pass's outputs replace its rows' encodings
+// (a row holds 64 halves, then 32 floats: one wave owns the buffer and its
+// LDS operations complete in order), and each lane forms its own point's
+// mixture. `dst.point(i)` gives point i's position and normal,
+// `dst.store(i, mixture)` takes the result.
+constexpr uint32_t kRowWords = 36;  // 32 words a row and 4 of padding: rows 16-byte aligned, 16 rows read at one column 2 to a bank
 template <class Points> __device__ __forceinline__ void infer_body(const NetBuffers &net, Points &pts, uint32_t n) {
-    __shared__ float os[16][kOut];
+    __shared__ uint32_t rows[64 * kRowWords];
     const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
-    for (uint32_t base = blockIdx.x * 16u; base < n; base += gridDim.x * 16u) {
+    _Float16 *own = reinterpret_cast<_Float16 *>(rows + lane * kRowWords);
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         float p[3], nn[3];
-        pts.point(base + j < n ? base + j : n - 1u, p, nn);
-        half4_t x[4];
-        encode(net.bounds, p[0], p[1], p[2], nn[0], nn[1], nn[2], g, x);
-        Act z[3];
-        float4_t o[2];
-        forward(net.fwd, net.params, x, lane, z, o);
-#pragma unroll
-        for (uint32_t m = 0; m < 2u; m++)
-#pragma unroll
-            for (uint32_t r = 0; r < 4u; r++) os[j][16u * m + 4u * g + r] = o[m][r];
+        pts.point(base + lane < n ? base + lane : n - 1u, p, nn);
+        encode_row(net.bounds, p[0], p[1], p[2], nn[0], nn[1], nn[2], own);
         __syncthreads();
-        if (g == 0u && base + j < n) pts.store(base + j, mixture(os[j]));
+#pragma unroll 1
+        for (uint32_t q = 0; q < 4u; q++) {
+            const _Float16 *row = reinterpret_cast<const _Float16 *>(rows + (16u * q + j) * kRowWords);
+            half4_t x[4];
+#pragma unroll
+            for (uint32_t kk = 0; kk < 4u; kk++) x[kk] = *reinterpret_cast<const half4_t *>(row + 16u * kk + 4u * g);
+            Act z[3];
+            float4_t o[2];
+            forward(net.fwd, net.params, x, lane, z, o);
+            float *out = reinterpret_cast<float *>(rows + (16u * q + j) * kRowWords);
+#pragma unroll
+            for (uint32_t m = 0; m < 2u; m++) *reinterpret_cast<float4_t *>(out + 16u * m + 4u * g) = o[m];
+        }
+        __syncthreads();
+        if (base + lane < n) pts.store(base + lane, mixture(reinterpret_cast<const float *>(rows + lane * kRowWords)));
         __syncthreads();
     }
 }

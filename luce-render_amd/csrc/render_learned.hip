@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void learned_adam_kernel(const Params *plist) 
 }  // namespace rt
 
 void launch_learned_infer(const rt::Params *p, uint32_t band, hipStream_t s) {
-    uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)band + 15) / 16, 16384);
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)band + 63) / 64, 16384);
     hipLaunchKernelGGL(rt::learned_infer_kernel, dim3(blocks ? blocks : 1), dim3(64), 0, s, p);
 }
 
