@@ -198,6 +198,9 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// Staged ids per queue: up to eight iterations of a workgroup between flushes.
+constexpr uint32_t kStageIterations = 8u;
+
 // Queue pushes of one workgroup staged in LDS: each wave claims LDS slots with
 // one LDS atomic, and a flush claims the workgroup's run of every queue with
 // one global atomic per queue and copies the staged ids out in order. Global
@@ -254,6 +257,15 @@ template <uint32_t Q, uint32_t CAP> struct StagedPushes {
         __syncthreads();
     }
 };
+
+// Per-bounce grids are sized for every slot; once few paths remain most of
+// their workgroups have no item. With one request such a workgroup leaves
+// after one load of the queue's count, before the row plan (workgroup 0
+// stays: kernels clear counters there). It would have run no iteration.
+template <class F> __device__ __forceinline__ bool idle_block(const Params *plist, uint32_t n, F count_of) {
+    if (n != 1u || blockIdx.x == 0u) return false;
+    return blockIdx.x * blockDim.x >= count_of(plist[0]);
+}
 
 // A launch over several renders (rows of a device array of Params): every
 // workgroup first lays the rows' item counts end to end, each padded to whole

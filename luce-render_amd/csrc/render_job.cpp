@@ -503,7 +503,8 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
             launch_learned_infer(params, jobs[0]->params.band_size * jobs[0]->pass_samples, s);
             mark(KRest);
         }
-        launch_render_stage(programs ? Stage::ShadeProgram : Stage::Shade, sp, params, n, cur, blocks(128, band), s);
+        if (learned) launch_shade_learned(sp, programs, params, n, cur, blocks(128, band), s);
+        else launch_render_stage(programs ? Stage::ShadeProgram : Stage::Shade, sp, params, n, cur, blocks(128, band), s);
         if (lights)
             launch_render_stage(programs ? Stage::ShadeLightProgram : Stage::ShadeLight, sp, params, n, cur,
                                 blocks(128, light_paths), s);

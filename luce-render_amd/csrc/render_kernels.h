@@ -45,6 +45,14 @@ const RenderTraceVariant *render_trace_variants();
 // phase (Adaptive) or "convert every pass" (Convert).
 void launch_render_stage(Stage stage, bool spectral, const rt::Params *p, uint32_t n, uint32_t arg, uint32_t blocks,
                          hipStream_t s);
+// Camera-path shading with the learned guide (render_shade_learned.hip): the
+// Shade / ShadeProgram stage with the guide standing in for part of the
+// diffuse lobe only.
+void launch_shade_learned(bool spectral, bool programs, const rt::Params *p, uint32_t n, uint32_t current,
+                          uint32_t blocks, hipStream_t s);
+// Workgroups of a pushing stage asked for `wanted`: capped at about 32 waves a
+// compute unit (the kernels loop).
+uint32_t staged_blocks(uint32_t group, uint32_t wanted);
 void launch_tables(float *tables, hipStream_t s);
 void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s);
 // Adds the render at p[0]'s converged pixels into *out.

@@ -21,155 +21,16 @@
 namespace lr {
 namespace rt {
 
-// ---------------------------------------------------------- light tracing
-// shaders/connect.glsl and common.glsl:113-143.
+}  // namespace rt
+}  // namespace lr
 
-__device__ __forceinline__ bool light_tracing(const Params &p) { return K(p, K_CAMERA).x > 0.0f; }
-__device__ __forceinline__ float light_paths_per_sample(const Params &p) { return K(p, K_CAMERA).x; }
-__device__ __forceinline__ V3 frame_position(const Params &p) { return V3{p.frame[0], p.frame[1], p.frame[2]}; }
-__device__ __forceinline__ V3 frame_corner(const Params &p) { return V3{p.frame[4], p.frame[5], p.frame[6]}; }
-__device__ __forceinline__ V3 frame_du(const Params &p) { return V3{p.frame[8], p.frame[9], p.frame[10]}; }
-__device__ __forceinline__ V3 frame_dv(const Params &p) { return V3{p.frame[12], p.frame[13], p.frame[14]}; }
+#include "render_connect.h"
 
-__device__ __forceinline__ V3 lens_point(const Params &p, float ux, float uy) {
-    float radius = K(p, K_CAMERA).z;
-    if (radius <= 0.0f) return frame_position(p);
-    float2 d = concentric(ux, uy);
-    return frame_position(p) + normalize(frame_du(p)) * (d.x * radius) + normalize(frame_dv(p)) * (d.y * radius);
-}
-
-__device__ __forceinline__ float camera_pdf(const Params &p, V3 d) {
-    float c = dot(d, v3(K(p, K_VIEW)));
-    return c > 0.0f ? K(p, K_CAMERA).y / (c * c * c) : 0.0f;
-}
-
-__device__ __forceinline__ bool camera_pixel(const Params &p, V3 x, V3 lens, uint32_t &pixel, V3 &d, float &dist) {
-    V3 offset = x - lens;
-    dist = length(offset);
-    d = offset / fmaxf(dist, 1e-20f);
-    float c = dot(d, v3(K(p, K_VIEW)));
-    if (c <= 1e-6f || dist <= 0.0f) return false;
-    float focus = K(p, K_CAMERA).w;
-    V3 film = (K(p, K_CAMERA).z > 0.0f ? (lens + d * (focus / c) - frame_position(p)) / focus : d / c) - frame_corner(p);
-    V3 du = frame_du(p), dv = frame_dv(p);
-    float fx = dot(film, du) / dot(du, du);
-    float fy = dot(film, dv) / dot(dv, dv);
-    if (!(fx >= 0.0f && fy >= 0.0f && fx < p.frame[11] && fy < p.frame[15])) return false;
-    pixel = (uint32_t)fy * (uint32_t)p.frame[11] + (uint32_t)fx;
-    return true;
-}
-
-__device__ __forceinline__ Surface turned(const Surface &s) {
-    return surface_at(-s.normal, -s.geometric, s.material, !s.back);
-}
-
-template <bool SPECTRAL>
-__device__ __forceinline__ float light_sampler_pdf(const Params &p, const Surface &s, V3 from, V3 to, float hero,
-                                                   const Program *g = nullptr) {
-    bool same_side = dot(from, s.geometric) >= 0.0f;
-    Surface v = same_side ? s : turned(s);
-    V3 wo = to_local(v, from), wi = to_local(v, to);
-    if ((dot(to, v.geometric) > 0.0f) != (wi.z > 0.0f)) return 0.0f;
-    if (!same_side) read_material<SPECTRAL>(p, v, hero, g);
-    light_lobes(v);
-    return surface_pdf(v, wi, wo);
-}
-
-__device__ __forceinline__ float emitter_power_pdf(const Params &p, uint32_t emitter) {
-    uint32_t record = (uint32_t)K(p, K_EMITTERS).x + emitter * 4u;
-    uint32_t trail = p.emitters[record + 1u], depth = p.emitters[record + 2u];
-    uint32_t base = (uint32_t)K(p, K_TREE).y;
-    uint32_t node = 0;
-    float pdf = 1.0f;
-    for (uint32_t level = 0; level < depth; level++) {
-        uint32_t link = p.emitters[node * 2u];
-        float left = fmaxf(p.lights[base + link * 3u].w, 0.0f);
-        float right = fmaxf(p.lights[base + (link + 1u) * 3u].w, 0.0f);
-        if (left + right <= 0.0f) return 0.0f;
-        uint32_t bit = (trail >> level) & 1u;
-        pdf *= (bit == 0u ? left : right) / (left + right);
-        node = link + bit;
-    }
-    return pdf;
-}
-
-__device__ __forceinline__ bool emitter_by_power(const Params &p, float u, uint32_t &emitter, float &pdf) {
-    pdf = 1.0f;
-    emitter = 0;
-    if (K(p, K_TREE).x < 1.0f) return false;
-    uint32_t base = (uint32_t)K(p, K_TREE).y;
-    uint32_t node = 0;
-    for (uint32_t depth = 0; depth < 64u; depth++) {
-        uint32_t link = p.emitters[node * 2u];
-        if ((p.emitters[node * 2u + 1u] & kLeafNode) != 0u) {
-            emitter = link;
-            return pdf > 0.0f;
-        }
-        float left = fmaxf(p.lights[base + link * 3u].w, 0.0f);
-        float right = fmaxf(p.lights[base + (link + 1u) * 3u].w, 0.0f);
-        if (left + right <= 0.0f) return false;
-        float chance = left / (left + right);
-        if (u < chance) {
-            u = fminf(u / chance, 0.99999994f);
-            pdf *= chance;
-            node = link;
-        } else {
-            u = fminf((u - chance) / (1.0f - chance), 0.99999994f);
-            pdf *= 1.0f - chance;
-            node = link + 1u;
-        }
-    }
-    return false;
-}
-
-__device__ __forceinline__ float emitter_area_pdf(const Params &p, uint32_t emitter) {
-    float finite = 1.0f - K(p, K_EMIT).x;
-    uint32_t code = emitter_code(p, emitter);
-    float area;
-    if (code & kEmitterTriangle) {
-        V3 a, e1, e2;
-        triangle_edges(p, code & ~kEmitterTriangle, a, e1, e2);
-        area = 0.5f * length(cross(e1, e2));
-    } else if (light_type(p, code) == LIGHT_SPHERE) {
-        float r = light_at(p, code, 1).w;
-        area = 4.0f * kPi * r * r;
-    } else {
-        area = light_at(p, code, 3).w;
-    }
-    return area > 0.0f ? finite * emitter_power_pdf(p, emitter) / area : 0.0f;
-}
-
-__device__ __forceinline__ float emission_direction_pdf(const Params &p, uint32_t emitter, float c) {
-    bool two_sided = (emitter_code(p, emitter) & kEmitterTriangle) != 0u;
-    return fabsf(c) / (two_sided ? 2.0f * kPi : kPi);
-}
-
-__device__ __forceinline__ float camera_hit_ratio(const Params &p, float4 mis, uint32_t emitter, float cos_light,
-                                                  float dist, float bsdf_pdf) {
-    if (emitter == kNoneU || mis.y <= 0.0f || bsdf_pdf <= 0.0f) return 0.0f;
-    float d2 = dist * dist;
-    float at_vertex = emission_direction_pdf(p, emitter, cos_light) * mis.z / d2 / mis.y;
-    float at_light = emitter_area_pdf(p, emitter) / (bsdf_pdf * fabsf(cos_light) / d2);
-    return mis.x * at_vertex * at_light;
-}
-
-template <bool SPECTRAL> __device__ __forceinline__ Spec clamped(const Params &p, Spec c, uint32_t total) {
-    float clamp_value = K(p, K_INFO).z;
-    if (total == 0u || clamp_value <= 0.0f) return c;
-    float largest = spec_max<SPECTRAL>(c);
-    return largest > clamp_value ? c * (clamp_value / largest) : c;
-}
+namespace lr {
+namespace rt {
 
 // ------------------------------------------------------------ pass control
 
-// Per-bounce grids are sized for every slot; once few paths remain most of
-// their workgroups have no item. With one request such a workgroup leaves
-// after one load of the queue's count, before the row plan (workgroup 0
-// stays: kernels clear counters there). It would have run no iteration.
-template <class F> __device__ __forceinline__ bool idle_block(const Params *plist, uint32_t n, F count_of) {
-    if (n != 1u || blockIdx.x == 0u) return false;
-    return blockIdx.x * blockDim.x >= count_of(plist[0]);
-}
 
 __global__ void begin_pass_kernel(const Params *plist) {
     const Params &p = plist[blockIdx.x];
@@ -219,8 +80,6 @@ __global__ void end_pass_kernel(const Params *plist) {
         for (uint32_t i = _base + threadIdx.x, _once = 1; _once; _once = 0)                                 \
             for (bool active = i < (count); _once; _once = 0)
 
-// Staged ids per queue: up to eight iterations of a workgroup between flushes.
-constexpr uint32_t kStageIterations = 8u;
 
 // --------------------------------------------------------------- camera
 
@@ -555,506 +414,13 @@ __global__ __launch_bounds__(128) void splat_shadow_kernel(const Params *plist, 
     }
 }
 
-// ---------------------------------------------------------- surface shading
+}  // namespace rt
+}  // namespace lr
 
-struct Pushes {
-    bool closest = false;
-    bool shadow = false;
-};
+#include "render_shade.h"
 
-template <bool SPECTRAL>
-__device__ __forceinline__ V3 shading_normal(const Params &p, uint4 attributes, float4 hit, V3 geometric) {
-    V3 n = normalize(octahedral_decode(attributes.x) * (1.0f - hit.z - hit.w) + octahedral_decode(attributes.y) * hit.z +
-                     octahedral_decode(attributes.z) * hit.w);
-    if (dot(n, geometric) < 0.0f) n = -n;
-    return n;
-}
-
-__device__ __forceinline__ bool disperses(const Params &p, uint32_t m, bool spectral) {
-    return spectral && material_at(p, m, 12).w != 0.0f;
-}
-
-template <bool SPECTRAL> __device__ void dispersive_hit(Spec &throughput, Spec &radiance, Spec &lambda) {
-    if (lambda.y < 0.0f) return;
-    throughput = make_float4(throughput.x * 4.0f, 0.0f, 0.0f, 0.0f);
-    radiance = make_float4(radiance.x * 4.0f, 0.0f, 0.0f, 0.0f);
-    lambda = make_float4(lambda.x, -lambda.y, -lambda.z, -lambda.w);
-}
-
-// shade_surface.comp:48-312. PROGRAMS: some material has a node-graph
-// program (shader_program.glsl), run once a hit before the surface is read.
-template <bool SPECTRAL, bool PROGRAMS>
-__device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t path) {
-    State st(p);
-    Pushes out;
-    uint32_t pixel = st.u(SU_PIXEL, path);
-    uint32_t bounces = st.u(SU_BOUNCE, path);
-    uint32_t total = bounce_total(bounces);
-    float4 hit = st.v(SV_HIT, path);
-    V3 origin = v3(st.v(SV_ORIGIN, path));
-    float4 ray = st.v(SV_DIRECTION, path);
-    V3 rd = v3(ray);
-    Spec throughput = st.v(SV_THROUGHPUT, path);
-    Spec lambda = st.v(SV_LAMBDA, path);
-    uint32_t what = st.u(SU_HIT, path);
-    float4 vertex_row = st.v(SV_VERTEX, path);
-    V3 vertex = v3(vertex_row);
-    float4 mis = light_tracing(p) ? st.v(SV_MIS, path) : make_float4(0, 0, 0, 0);
-    bool tracing = light_tracing(p);
-
-    if (what & kLightHit) {
-        uint32_t light = what & ~kLightHit;
-        float weight = 1.0f;
-        if (total == 0u) {
-            if (!light_visible(p, light)) return out;
-        } else if (ray.w > 0.0f) {
-            uint32_t emitter = light_emitter(p, light);
-            float reach = distance(vertex, origin + rd * hit.x);
-            float chance = (1.0f - K(p, K_TREE).z) * tree_pdf(p, emitter, vertex, octahedral_decode(st.u(SU_NORMAL, path)));
-            float traced = tracing ? camera_hit_ratio(p, mis, emitter, light_cos(p, light, vertex, rd, reach), reach, ray.w) : 0.0f;
-            weight = mis_weight(light_hit_pdf(p, light, vertex, rd, reach) * chance / ray.w, traced);
-        }
-        st.set_v(SV_RADIANCE, path,
-                 st.v(SV_RADIANCE, path) + clamped<SPECTRAL>(p, throughput * light_radiance<SPECTRAL>(p, light, lambda) * weight, total));
-        return out;
-    }
-
-    uint32_t tri = what;
-    V3 p0, e1, e2;
-    triangle_edges(p, tri, p0, e1, e2);
-    V3 geometric = normalize(cross(e1, e2));
-    bool inside = dot(geometric, rd) > 0.0f;
-    if (inside) geometric = -geometric;
-    uint4 attributes = p.shading[tri];
-    V3 x = origin + rd * hit.x;
-    float reach = distance(vertex, x);
-    float density = (total == 0u ? mis.w : ray.w) * fabsf(dot(rd, geometric)) / (reach * reach);
-    float back_reach = mis.z / (reach * reach);
-    const uint32_t sample = slot_identity(ps, path).sample;
-    float4 u_opacity = sample4(p.seed, pixel, sample, group_opacity(total));
-
-    float opacity = clampf(material_at(p, attributes.w, 12).x, 0.0f, 1.0f);
-    if (opacity < 1.0f &&
-        fractf(u_opacity.x + (float)hash_u32(__float_as_uint(hit.x)) * (1.0f / 4294967296.0f)) >= opacity) {
-        st.set_v(SV_ORIGIN, path, f4(offset_origin(x, -geometric), 0.0f));
-        out.closest = true;
-        return out;
-    }
-
-    if (disperses(p, attributes.w, SPECTRAL)) {
-        Spec radiance = st.v(SV_RADIANCE, path);
-        dispersive_hit<SPECTRAL>(throughput, radiance, lambda);
-        st.set_v(SV_RADIANCE, path, radiance);
-        st.set_v(SV_LAMBDA, path, lambda);
-    }
-    float hero = lambda.y < 0.0f ? lambda.x : 0.0f;
-
-    if (material_at(p, attributes.w, 7).w > 0.0f) {
-        float weight = 1.0f;
-        uint32_t emitter = triangle_emitter(p, tri);
-        if (total > 0u && ray.w > 0.0f && emitter != kNoneU) {
-            float chance = (1.0f - K(p, K_TREE).z) * tree_pdf(p, emitter, vertex, octahedral_decode(st.u(SU_NORMAL, path)));
-            float traced = tracing ? camera_hit_ratio(p, mis, emitter, fabsf(dot(rd, geometric)), reach, ray.w) : 0.0f;
-            weight = mis_weight(triangle_hit_pdf(p, tri, rd, reach) * chance / ray.w, traced);
-        }
-        st.set_v(SV_RADIANCE, path,
-                 st.v(SV_RADIANCE, path) +
-                     clamped<SPECTRAL>(p, throughput * material_color<SPECTRAL>(p, attributes.w, 6u, lambda) * weight, total));
-    }
-
-    if (total >= (uint32_t)K(p, K_INFO).y) return out;
-
-    V3 normal = shading_normal<SPECTRAL>(p, attributes, hit, geometric);
-    // The material's node graph, once for this hit, on the outward normal; it
-    // may bend the normal. A program that reads the view makes the light
-    // tracer's density past the first vertex zero here (`viewed`).
-    Program program;
-    const Program *g = nullptr;
-    bool viewed = false;
-    if constexpr (PROGRAMS) {
-        run_program(p, program, attributes.w, tri, x, inside ? -normal : normal, -rd,
-                    triangle_uv(p, tri, hit.z, hit.w));
-        g = &program;
-        viewed = program.viewed;
-        if (program.driven(9)) {
-            normal = normalize(program.driven3(11));
-            if (dot(normal, geometric) < 0.0f) normal = -normal;
-        }
-    }
-    Surface surface = surface_at(normal, geometric, attributes.w, inside);
-    V3 wo = to_local(surface, -rd);
-    prepare_surface<SPECTRAL>(p, surface, wo, lambda, hero, g);
-
-    V3 from = offset_origin(x, geometric);
-
-    uint32_t cell = kNoneU;
-    float guided = 0.0f;
-    bool guide_on = guiding(p);
-    if (guide_on && total < GUIDE_RING) {
-        cell = p.learned ? path : guide_cell(p, x, geometric);
-        bool suited = !surface.inside && surface.p_transmission <= 0.0f &&
-                      (surface.p_diffuse > 0.05f || fminf(surface.alpha.x, surface.alpha.y) > 0.1f);
-        bool ready = cell != kNoneU && (p.learned ? net_ready(p) : guide_count(p, cell) >= K(p, K_GUIDE).w);
-        // The learned guide takes the diffuse lobe's share of the BSDF's
-        // choice (its mixtures model incident light, which a glossy lobe's
-        // own sampling already follows better); the grid a fixed share
-        // (loading.lucb:281).
-        if (suited && ready) guided = K(p, K_GUIDE).z * (p.learned ? surface.p_diffuse : 1.0f);
-    }
-
-    float4 u_light = sample4(p.seed, pixel, sample, group_light(total));
-    float p_distant = K(p, K_TREE).z;
-    LightSample s;
-    bool pushed_shadow = false;
-    Spec emitted = spec(0.0f);
-    float chance = 0.0f;
-    bool sampled = false;
-    uint32_t emitter = kNoneU, afar = kNoneU;
-    if (u_light.x < p_distant) {
-        float share;
-        uint32_t light = pick_distant(p, u_light.x / p_distant, share);
-        if (light != kNoneU && share > 0.0f && sample_light(p, light, x, u_light.y, u_light.z, s)) {
-            afar = light;
-            chance = p_distant * share;
-            emitted = light_type(p, light) == LIGHT_DOME ? dome_radiance<SPECTRAL>(p, light, s.direction, lambda)
-                                                         : light_radiance<SPECTRAL>(p, light, lambda);
-            sampled = true;
-        }
-    } else {
-        float tree_chance;
-        if (tree_sample(p, (u_light.x - p_distant) / fmaxf(1e-7f, 1.0f - p_distant), from, normal, emitter, tree_chance)) {
-            chance = (1.0f - p_distant) * tree_chance;
-            uint32_t code = emitter_code(p, emitter);
-            if (code & kEmitterTriangle) {
-                sampled = sample_triangle<SPECTRAL>(p, code & ~kEmitterTriangle, x, u_light.y, u_light.z, lambda, s, emitted);
-            } else if (sample_light(p, code, x, u_light.y, u_light.z, s)) {
-                emitted = light_radiance<SPECTRAL>(p, code, lambda);
-                sampled = true;
-            }
-        }
-    }
-    if (sampled && chance > 0.0f) {
-        float side = dot(s.direction, geometric);
-        V3 wi = to_local(surface, s.direction);
-        if ((side > 0.0f) == (wi.z > 0.0f) && (side > 0.0f || surface.p_transmission > 0.0f)) {
-            float bsdf_pdf;
-            Spec f = evaluate_surface(surface, wi, wo, bsdf_pdf);
-            if (guided > 0.0f && !tracing) bsdf_pdf = guided * guide_pdf(p, cell, s.direction) + (1.0f - guided) * bsdf_pdf;
-            if (bsdf_pdf > 0.0f) {
-                float light_pdf = s.pdf * chance;
-                float traced = 0.0f;
-                if (tracing && !s.delta && emitter != kNoneU && density > 0.0f) {
-                    float d2 = s.distance * s.distance;
-                    float at_last = total == 0u ? 1.0f
-                                    : (viewed ? 0.0f
-                                              : light_sampler_pdf<SPECTRAL>(p, surface, s.direction, -rd, hero, g) *
-                                                    back_reach / mis.y);
-                    float at_vertex = emission_direction_pdf(p, emitter, s.cos_light) * fabsf(side) / d2 / density;
-                    float at_light = emitter_area_pdf(p, emitter) / (bsdf_pdf * s.cos_light / d2);
-                    traced = mis.x * at_last * at_vertex * at_light;
-                }
-                if (tracing && afar != kNoneU && emits_from_afar(p, afar) && density > 0.0f) {
-                    float at_last = total == 0u ? 1.0f
-                                    : (viewed ? 0.0f
-                                              : light_sampler_pdf<SPECTRAL>(p, surface, s.direction, -rd, hero, g) *
-                                                    back_reach / mis.y);
-                    float at_vertex = afar_first_density(p, x, s.direction, fabsf(side)) / density;
-                    float at_light = afar_emission_pdf(p, afar, s.direction) / bsdf_pdf;
-                    traced = mis.x * at_last * at_vertex * at_light;
-                }
-                float weight = s.delta ? 1.0f : mis_weight(bsdf_pdf / light_pdf, traced * bsdf_pdf / light_pdf);
-                Spec contribution = clamped<SPECTRAL>(p, throughput * f * fabsf(wi.z) * emitted * (weight / light_pdf), total);
-                V3 target = x + s.direction * s.distance;
-                float margin = 4e-4f * fmaxf(1.0f, fmaxf(fabsf(target.x), fmaxf(fabsf(target.y), fabsf(target.z))));
-                float reach_s = s.distance >= kInf ? kInf : fmaxf(0.0f, s.distance - margin);
-                st.set_v(SV_SHADOW_ORIGIN, path, f4(offset_origin(x, side > 0.0f ? geometric : -geometric), reach_s));
-                st.set_v(SV_SHADOW_DIRECTION, path, f4(s.direction, 0.0f));
-                st.set_v(SV_SHADOW_RADIANCE, path, contribution);
-                out.shadow = true;
-                pushed_shadow = true;
-            }
-        }
-    }
-
-    float4 u = sample4(p.seed, pixel, sample, group_bsdf(total));
-    V3 wi;
-    Spec weight;
-    float pdf;
-    uint32_t lobe;
-    float unguided = 1.0f;
-    float bsdf_only;
-    if (u_opacity.y < guided) {
-        wi = to_local(surface, guide_sample(p, cell, u_opacity.z, u_opacity.w, u.w));
-        float bsdf_pdf;
-        Spec f = evaluate_surface(surface, wi, wo, bsdf_pdf);
-        pdf = guided * guide_pdf(p, cell, to_world(surface, wi)) + (1.0f - guided) * bsdf_pdf;
-        bsdf_only = bsdf_pdf;
-        if (pdf <= 0.0f || spec_max<SPECTRAL>(f) <= 0.0f) return out;
-        unguided = pdf / fmaxf(bsdf_pdf, 1e-6f * pdf);
-        weight = f * (fabsf(wi.z) / pdf);
-        lobe = wi.z < 0.0f ? LOBE_TRANSMISSION : (surface.p_diffuse > 0.5f ? LOBE_DIFFUSE : LOBE_GLOSSY);
-    } else {
-        if (!sample_surface(surface, wo, u.x, u.y, u.w, wi, weight, pdf, lobe)) return out;
-        bsdf_only = pdf;
-        if (guided > 0.0f) {
-            float mixed = guided * guide_pdf(p, cell, to_world(surface, wi)) + (1.0f - guided) * pdf;
-            unguided = mixed / pdf;
-            weight = weight * (pdf / mixed);
-            pdf = mixed;
-        }
-    }
-    V3 direction = to_world(surface, wi);
-    float side = dot(direction, geometric);
-    if ((side > 0.0f) != (wi.z > 0.0f)) return out;
-    bounces = next_bounce(p, bounces, lobe);
-    if (bounces == kNoneU) return out;
-    throughput = throughput * weight;
-    float survival_scale = fminf(vertex_row.w * unguided, 1e6f);
-    if (total + 1u >= 3u) {
-        float survive = fminf(0.95f, spec_max<SPECTRAL>(throughput) * survival_scale);
-        if (u.z >= survive) return out;
-        throughput = throughput / survive;
-    }
-    st.set_u(SU_BOUNCE, path, bounces);
-    st.set_u(SU_NORMAL, path, octahedral_encode(normal));
-    if (guide_on && total < GUIDE_RING) {
-        if (cell == kNoneU) cell = p.learned ? path : guide_cell(p, x, geometric);
-        st.set_u(SU_GUIDE_CELL0 + total, path, cell);
-        if (p.learned)
-            st.set_v(SV_GUIDE_POS0 + total, path, make_float4(x.x, x.y, x.z, __uint_as_float(octahedral_encode(geometric))));
-        st.set_u(SU_GUIDE_DIR0 + total, path, octahedral_encode(direction));
-        float found = spec_luminance<SPECTRAL>(p, st.v(SV_RADIANCE, path), lambda) +
-                      (pushed_shadow ? spec_luminance<SPECTRAL>(p, st.v(SV_SHADOW_RADIANCE, path), lambda) : 0.0f);
-        st.set_v(SV_GUIDE0 + total, path,
-                 make_float4(pdf, spec_luminance<SPECTRAL>(p, throughput, lambda), found, fabsf(dot(direction, normal))));
-        if (total + 1u < GUIDE_RING) st.set_u(SU_GUIDE_CELL0 + total + 1u, path, kNoneU);
-    }
-    V3 next = side > 0.0f ? from : offset_origin(x, -geometric);
-    st.set_v(SV_ORIGIN, path, f4(next, 0.0f));
-    st.set_v(SV_VERTEX, path, f4(next, survival_scale));
-    if (tracing) {
-        float at_last = total == 0u ? 1.0f
-                        : (viewed ? 0.0f : light_sampler_pdf<SPECTRAL>(p, surface, direction, -rd, hero, g) * back_reach / mis.y);
-        st.set_v(SV_MIS, path, make_float4(mis.x * at_last, density, fabsf(side), 0.0f));
-        pdf = bsdf_only;
-    }
-    st.set_v(SV_DIRECTION, path, f4(direction, pdf));
-    st.set_v(SV_THROUGHPUT, path, throughput);
-    out.closest = true;
-    return out;
-}
-
-// light_paths.glsl:166-180
-template <bool SPECTRAL>
-__device__ __forceinline__ float light_sample_ratio(const Params &p, uint32_t emitter, const Surface &s, V3 x, V3 x0,
-                                                    V3 toward, float bsdf_pdf) {
-    if (!(bsdf_pdf > 0.0f)) return 0.0f;
-    V3 from = offset_origin(x, s.geometric);
-    if (emitter & kAfarEmitter) {
-        uint32_t light = emitter & ~kAfarEmitter;
-        return light_select_pdf(p, light) * light_hit_pdf(p, light, from, toward, kInf) / bsdf_pdf;
-    }
-    V3 offset = x0 - from;
-    float reach = length(offset);
-    V3 d = offset / reach;
-    float chance = (1.0f - K(p, K_TREE).z) * tree_pdf(p, emitter, from, s.normal);
-    uint32_t code = emitter_code(p, emitter);
-    float pdf = (code & kEmitterTriangle) ? triangle_hit_pdf(p, code & ~kEmitterTriangle, d, reach)
-                                          : light_hit_pdf(p, code, from, d, reach);
-    return chance * pdf / bsdf_pdf;
-}
-
-// light_paths.glsl:31-189
-template <bool SPECTRAL, bool PROGRAMS>
-__device__ Pushes shade_light(const Params &p, const PassState &ps, uint32_t path) {
-    State st(p);
-    Pushes out;
-    uint32_t what = st.u(SU_HIT, path);
-    if (what & kLightHit) return out;
-    uint32_t emitter = st.u(SU_PIXEL, path);
-    uint32_t bounces = st.u(SU_BOUNCE, path);
-    uint32_t total = bounce_total(bounces);
-    float4 hit = st.v(SV_HIT, path);
-    V3 origin = v3(st.v(SV_ORIGIN, path));
-    float4 ray = st.v(SV_DIRECTION, path);
-    V3 rd = v3(ray);
-    Spec throughput = st.v(SV_THROUGHPUT, path);
-    Spec lambda = st.v(SV_LAMBDA, path);
-    float4 vertex_row = st.v(SV_VERTEX, path);
-    float4 mis = st.v(SV_MIS, path);
-    const SlotIdentity who = slot_identity(ps, path);
-    const uint32_t id = 0x80000000u | who.slot;
-
-    uint32_t tri = what;
-    V3 p0, e1, e2;
-    triangle_edges(p, tri, p0, e1, e2);
-    V3 geometric = normalize(cross(e1, e2));
-    bool back = dot(geometric, rd) > 0.0f;
-    if (back) geometric = -geometric;
-    uint4 attributes = p.shading[tri];
-    V3 x = origin + rd * hit.x;
-    float4 u_opacity = sample4(p.seed, id, who.sample, group_opacity(total));
-
-    float opacity = clampf(material_at(p, attributes.w, 12).x, 0.0f, 1.0f);
-    if (opacity < 1.0f &&
-        fractf(u_opacity.x + (float)hash_u32(__float_as_uint(hit.x)) * (1.0f / 4294967296.0f)) >= opacity) {
-        st.set_v(SV_ORIGIN, path, f4(offset_origin(x, -geometric), 0.0f));
-        out.closest = true;
-        return out;
-    }
-    if (total >= (uint32_t)K(p, K_INFO).y) return out;
-    if (disperses(p, attributes.w, SPECTRAL)) {
-        Spec unused = spec(0.0f);
-        dispersive_hit<SPECTRAL>(throughput, unused, lambda);
-        st.set_v(SV_LAMBDA, path, lambda);
-    }
-    float hero = lambda.y < 0.0f ? lambda.x : 0.0f;
-
-    V3 normal = shading_normal<SPECTRAL>(p, attributes, hit, geometric);
-    uint32_t pixel = 0;
-    V3 to_point;
-    float camera_distance;
-    float4 ul = sample4(p.seed, id, who.sample, group_light(total));
-    V3 lens = lens_point(p, ul.x, ul.y);
-    bool on_film = camera_pixel(p, x, lens, pixel, to_point, camera_distance);
-    // The material's node graph as camera paths run it (a program reading the
-    // view sees the point from the camera). Such a program has no surface for
-    // directions the camera does not see from: the path connects here, if it
-    // can, and goes no further.
-    Program program;
-    const Program *g = nullptr;
-    bool viewed = false;
-    if constexpr (PROGRAMS) {
-        run_program(p, program, attributes.w, tri, x, back ? -normal : normal, on_film ? -to_point : -rd,
-                    triangle_uv(p, tri, hit.z, hit.w));
-        g = &program;
-        viewed = program.viewed;
-        if (viewed && !on_film) return out;
-        if (program.driven(9)) {
-            normal = normalize(program.driven3(11));
-            if (dot(normal, geometric) < 0.0f) normal = -normal;
-        }
-    }
-
-    Surface lit = surface_at(normal, geometric, attributes.w, back);
-    V3 toward_light = -rd;
-    read_material<SPECTRAL>(p, lit, hero, g);
-    read_colors<SPECTRAL>(p, lit, lambda, g);
-    V3 x_last = v3(vertex_row);
-    float reach = distance(x_last, x);
-    float cos_in = fabsf(dot(rd, geometric));
-    float density = ray.w * cos_in / (reach * reach);
-    float back_reach = mis.z / (reach * reach);
-    if (total == 0u && (emitter & kAfarEmitter)) {
-        density = afar_first_density(p, x, -rd, cos_in);
-        back_reach = 1.0f;
-        if (!(density > 0.0f)) return out;
-    }
-    float cos_shading_in = dot(toward_light, normal);
-    if (cos_shading_in <= 0.0f || cos_in < 1e-6f) return out;
-    float shading_in = cos_shading_in / cos_in;
-
-    if (on_film) {
-        V3 to_camera = -to_point;
-        float side = dot(to_camera, geometric);
-        Surface seen = side > 0.0f ? lit : turned(lit);
-        V3 wo = to_local(seen, to_camera), wi = to_local(seen, toward_light);
-        if ((dot(toward_light, seen.geometric) > 0.0f) == (wi.z > 0.0f) && wo.z > 0.0f) {
-            if (side > 0.0f) orient_surface(p, seen, wo);
-            else prepare_surface<SPECTRAL>(p, seen, wo, lambda, hero, g);
-            float camera_bsdf_pdf;
-            Spec f = evaluate_surface(seen, wi, wo, camera_bsdf_pdf);
-            float camera_density = camera_pdf(p, to_point) * fabsf(side) / (camera_distance * camera_distance);
-            Spec contribution = throughput * f * (shading_in * camera_density / light_paths_per_sample(p));
-            if (spec_max<SPECTRAL>(contribution) > 0.0f) {
-                float last = mis.y > 0.0f ? camera_bsdf_pdf * back_reach / mis.y : 0.0f;
-                float ratio = mis.x * last * camera_density / density;
-                float sample_ratio = total == 0u ? light_sample_ratio<SPECTRAL>(p, emitter, seen, x, x_last, toward_light, camera_bsdf_pdf)
-                                                 : fmaxf(mis.w, 0.0f);
-                float weight = mis_weight(ratio / light_paths_per_sample(p), ratio * sample_ratio / light_paths_per_sample(p));
-                float margin = 4e-4f * fmaxf(1.0f, fmaxf(fabsf(x.x), fmaxf(fabsf(x.y), fabsf(x.z))));
-                st.set_v(SV_SHADOW_ORIGIN, path, f4(offset_origin(x, seen.geometric), fmaxf(0.0f, camera_distance - margin)));
-                st.set_v(SV_SHADOW_DIRECTION, path, f4(to_camera, 0.0f));
-                st.set_v(SV_SHADOW_RADIANCE, path, f4(spec_to_film<SPECTRAL>(p, contribution * weight, lambda), 0.0f));
-                st.set_u(SU_SPLAT, path, pixel);
-                out.shadow = true;
-            }
-        }
-    }
-
-    if (viewed || total + 1u >= (uint32_t)K(p, K_INFO).y) return out;
-    float4 u = sample4(p.seed, id, who.sample, group_bsdf(total));
-    Surface choosing = lit;
-    light_lobes(choosing);
-    V3 wo_light = to_local(lit, toward_light);
-    V3 wi_light;
-    float light_pdf;
-    uint32_t lobe;
-    if (!sample_direction(choosing, wo_light, u.x, u.y, u.w, wi_light, light_pdf, lobe)) return out;
-    V3 direction = to_world(lit, wi_light);
-    float side = dot(direction, geometric);
-    if ((side > 0.0f) != (wi_light.z > 0.0f)) return out;
-    Surface seen = side > 0.0f ? lit : turned(lit);
-    V3 wo = to_local(seen, direction), wi = to_local(seen, toward_light);
-    if ((dot(toward_light, seen.geometric) > 0.0f) != (wi.z > 0.0f)) return out;
-    if (side > 0.0f) orient_surface(p, seen, wo);
-    else prepare_surface<SPECTRAL>(p, seen, wo, lambda, hero, g);
-    float camera_bsdf_pdf;
-    Spec f = evaluate_surface(seen, wi, wo, camera_bsdf_pdf);
-    bounces = next_bounce(p, bounces, lobe);
-    if (bounces == kNoneU) return out;
-    throughput = throughput * f * (shading_in * fabsf(side) / light_pdf);
-    float survival_scale = vertex_row.w;
-    if (total + 1u >= 3u) {
-        float survive = fminf(0.95f, spec_max<SPECTRAL>(throughput) * survival_scale);
-        if (u.z >= survive) return out;
-        throughput = throughput / survive;
-    }
-    if (!(spec_max<SPECTRAL>(throughput) > 0.0f)) return out;
-    float last = mis.y > 0.0f ? camera_bsdf_pdf * back_reach / mis.y : 0.0f;
-    float sample_ratio = total == 0u ? light_sample_ratio<SPECTRAL>(p, emitter, seen, x, x_last, toward_light, camera_bsdf_pdf)
-                                     : mis.w;
-    st.set_u(SU_BOUNCE, path, bounces);
-    st.set_v(SV_ORIGIN, path, f4(offset_origin(x, side > 0.0f ? geometric : -geometric), 0.0f));
-    st.set_v(SV_VERTEX, path, f4(x, survival_scale));
-    st.set_v(SV_DIRECTION, path, f4(direction, light_pdf));
-    st.set_v(SV_THROUGHPUT, path, throughput);
-    st.set_v(SV_MIS, path, make_float4(mis.x * last, density, fabsf(side), sample_ratio));
-    out.closest = true;
-    return out;
-}
-
-// Camera paths' and light paths' hits are shaded by separate launches: each
-// needs fewer registers than the two together, so more waves fit; capped at
-// 168 registers (3 waves a SIMD, a few spilled values): measured faster than
-// the uncapped 215 (2 waves) and than 128 (4 waves, more spills).
-template <bool SPECTRAL, bool LIGHT, bool PROGRAMS>
-__global__ __launch_bounds__(128, 3) void shade_kernel(const Params *plist, uint32_t n, uint32_t current) {
-    __shared__ RowPlan plan;
-    const uint32_t queue = LIGHT ? Q_LIGHT : Q_SURFACE;
-    auto count_of = [](const Params &q) { return q.queues[LIGHT ? Q_LIGHT : Q_SURFACE]; };
-    if (idle_block(plist, n, count_of)) return;
-    uint32_t total = plan.make(plist, n, count_of);
-    // consumed by this bounce's closest-hit and routing kernels
-    if (!LIGHT && blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) plist[i].queues[Q_CLOSEST + current] = 0u;
-    __shared__ StagedPushes<2, 128 * kStageIterations> staged;
-    staged.init({Q_CLOSEST + 1u - current, Q_SHADOW});
-    FOR_ROW_ITEMS(plan, n, total, g, r, at, active) {
-        const Params &p = plist[r];
-        const PassState &ps = *p.pass;
-        Pushes pu;
-        uint32_t path = 0;
-        if (active) {
-            path = queue_path(p, queue, at);
-            if constexpr (LIGHT) pu = shade_light<SPECTRAL, PROGRAMS>(p, ps, path);
-            else pu = shade_camera<SPECTRAL, PROGRAMS>(p, ps, path);
-        }
-        staged.push(0, path, pu.closest);
-        staged.push(1, path, pu.shadow);
-        staged.sync(p, row_ends(plan, n, total, g, r));
-    }
-}
+namespace lr {
+namespace rt {
 
 // ---------------------------------------------------------------- misses
 
@@ -1450,8 +816,6 @@ __global__ __launch_bounds__(256) void fill_kernel(uint32_t *dst, size_t words, 
 
 // ------------------------------------------------------------------ launchers
 
-namespace {
-
 // Workgroups of the pushing stages: about 32 waves a compute unit, so that
 // each workgroup loops over several iterations between its staged flushes.
 // (The tracing kernels take one item per thread: measured faster than any
@@ -1466,6 +830,8 @@ uint32_t staged_blocks(uint32_t group, uint32_t wanted) {
     uint64_t cap = (uint64_t)cus * 32u / (group / 64u);
     return wanted < cap ? wanted : (uint32_t)cap;
 }
+
+namespace {
 
 template <int W, int S, int G>
 void launch_closest(const rt::Params *p, uint32_t n, uint32_t cur, uint32_t blocks, hipStream_t s) {
