@@ -435,11 +435,35 @@ __device__ __forceinline__ uint32_t owen(uint32_t x, uint32_t seed) {
     x ^= x * 0x8d22f6e6u;
     return __builtin_bitreverse32(x);
 }
-__device__ __forceinline__ uint32_t sobol(uint32_t index, uint32_t dim) {
-    uint32_t x = 0;
-    for (uint32_t bit = 0; index != 0; bit++, index >>= 1)
-        if (index & 1u) x ^= kSobol[dim * 32u + bit];
-    return x;
+// The first four Sobol dimensions of `index` (sampler.glsl:29-35: the XOR of
+// kSobol's columns for the index's set bits), without a per-lane loop over
+// the bits or table loads:
+//  - dimension 0's matrix is the bit reversal;
+//  - dimension 1's is Pascal's triangle mod 2, so by Lucas' theorem output
+//    bit 31-j is the XOR of the index bits b whose bit pattern holds j's
+//    (j & b == j): a superset sum over the five bits of a bit position, five
+//    shift-mask-XOR steps, then the reversal;
+//  - dimensions 2 and 3 take one pass over the 32 bit positions, unrolled,
+//    the columns immediates and each set bit a mask (no branch).
+// Equal to the table loop for every index (any XOR order is the same word).
+__device__ __forceinline__ void sobol4(uint32_t index, uint32_t (&x)[4]) {
+    x[0] = __builtin_bitreverse32(index);
+    uint32_t g = index;
+    g ^= (g >> 1) & 0x55555555u;
+    g ^= (g >> 2) & 0x33333333u;
+    g ^= (g >> 4) & 0x0F0F0F0Fu;
+    g ^= (g >> 8) & 0x00FF00FFu;
+    g ^= (g >> 16) & 0x0000FFFFu;
+    x[1] = __builtin_bitreverse32(g);
+    uint32_t x2 = 0u, x3 = 0u, rest = index;
+#pragma unroll
+    for (uint32_t bit = 0; bit < 32u; bit++, rest >>= 1) {
+        uint32_t m = 0u - (rest & 1u);
+        x2 ^= m & kSobol[64u + bit];
+        x3 ^= m & kSobol[96u + bit];
+    }
+    x[2] = x2;
+    x[3] = x3;
 }
 
 constexpr uint32_t GROUP_CAMERA = 0, GROUP_WAVELENGTH = 1;
@@ -450,9 +474,11 @@ __device__ __forceinline__ uint32_t group_opacity(uint32_t b) { return 4u + 3u *
 __device__ __forceinline__ float4 sample4(uint32_t seed0, uint32_t pixel, uint32_t index, uint32_t group) {
     uint32_t seed = hash_combine(hash_combine(seed0, pixel), group);
     uint32_t shuffled = owen(index, hash_u32(seed));
+    uint32_t x[4];
+    sobol4(shuffled, x);
     float r[4];
 #pragma unroll
-    for (uint32_t d = 0; d < 4; d++) r[d] = (float)(owen(sobol(shuffled, d), hash_combine(seed, d + 1u)) >> 8) * (1.0f / 16777216.0f);
+    for (uint32_t d = 0; d < 4; d++) r[d] = (float)(owen(x[d], hash_combine(seed, d + 1u)) >> 8) * (1.0f / 16777216.0f);
     return make_float4(r[0], r[1], r[2], r[3]);
 }
 
