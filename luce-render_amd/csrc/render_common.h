@@ -86,6 +86,16 @@ struct PassState {
 };
 
 // Launch parameters fixed for a view: pointers, sizes, the frame.
+// A node of the analytic lights' culling tree (pre-order): a box holding
+// its lights (grown outward from their exact bounds), `escape` the node after
+// its subtree, `light` the light of a leaf (kNoneU for an interior node).
+struct LightNode {
+    float lo[3];
+    uint32_t escape;
+    float hi[3];
+    uint32_t light;
+};
+
 struct Params {
     float frame[16];
     uint32_t pool;        // path slots: the stride of every state field
@@ -129,7 +139,10 @@ struct Params {
     float4 *image;
     float4 *display;
     uint8_t *guide;        // guiding grid (keys, lobes, stats); null without guiding
-    const float4 *light_bounds;  // bounding spheres: one per group of kLightGroup lights, then one per light
+    // The analytic lights' culling tree (LightNode, in pre-order) and its
+    // node count; empty when no light can be hit.
+    const LightNode *light_tree;
+    uint32_t light_nodes;
     // The learned guide (guiding 2): the network, the per-slot mixtures the
     // inference writes each bounce (rows of `pool` float4: four lobes, then
     // the lobe weights), the training gradient rows and schedule.

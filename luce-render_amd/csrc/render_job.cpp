@@ -142,7 +142,13 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `next` is on this stack
     if (e != hipSuccess) return hip_fail(e, "restart upload");
     std::memcpy(constants, next, sizeof(constants));
-    launch_light_bounds(p.lights, (uint32_t)loaded->constants[0], (float4 *)p.light_bounds, s);
+    {
+        // The same light count: a tree of at most the nodes held.
+        std::vector<rt::LightNode> tree = light_tree(loaded->lights, analytic_lights);
+        e = upload_light_tree(tree, const_cast<rt::LightNode *>(p.light_tree), s);
+        if (e != hipSuccess) return hip_fail(e, "restart light tree");
+        p.light_nodes = (uint32_t)tree.size();
+    }
     bool resized = st.width != settings.width || st.height != settings.height;
     settings.samples = st.samples;
     settings.seed = st.seed;
@@ -355,10 +361,15 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     uint32_t nlights = (uint32_t)loaded->constants[0];
     if ((size_t)nlights * 28u > loaded->light_floats) return fail(LR_INVALID, "the light count exceeds the light records");
     analytic_lights = nlights;
-    size_t bound_count = (nlights + 7u) / 8u + nlights;
-    p.light_bounds = (const float4 *)alloc(bound_count * 16, rc);
-    if (rc) return rc;
-    launch_light_bounds(p.lights, nlights, (float4 *)p.light_bounds, s);
+    {
+        std::vector<rt::LightNode> tree = light_tree(loaded->lights, nlights);
+        // room for a tree of every light, so a reload keeps the buffer
+        p.light_tree = (const rt::LightNode *)alloc((size_t)std::max(1u, 2u * nlights) * sizeof(rt::LightNode), rc);
+        if (rc) return rc;
+        if (hipError_t e = upload_light_tree(tree, const_cast<rt::LightNode *>(p.light_tree), s))
+            return hip_fail(e, "light tree");
+        p.light_nodes = (uint32_t)tree.size();
+    }
     float *tables = shared_tables;
     if (!tables) {
         tables = (float *)alloc((size_t)rt::TABLE_SIZE * 4, rc);

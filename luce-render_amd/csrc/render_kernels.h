@@ -2,6 +2,7 @@
 #pragma once
 
 #include <algorithm>
+#include <vector>
 
 #include "render_common.h"
 
@@ -57,7 +58,11 @@ void launch_tables(float *tables, hipStream_t s);
 void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s);
 // Adds the render at p[0]'s converged pixels into *out.
 void launch_converged(const rt::Params *p, uint32_t pixels, uint32_t *out, hipStream_t s);
-// Bounding spheres of `n` analytic lights for culling: ceil(n / 8) + n float4.
-void launch_light_bounds(const float4 *lights, uint32_t n, float4 *out, hipStream_t s);
+// The culling tree of `n` analytic lights from their host records (28
+// floats each): rt::LightNode in pre-order, 2h - 1 nodes for h lights that
+// can be hit (none for distant and dome lights).
+std::vector<rt::LightNode> light_tree(const float *lights, uint32_t n);
+// Copies a tree to `out` (room for nodes.size()) and waits for the copy.
+hipError_t upload_light_tree(const std::vector<rt::LightNode> &nodes, rt::LightNode *out, hipStream_t s);
 
 }  // namespace lr

@@ -12,6 +12,11 @@
 // a device PassState that begin_pass / end_pass advance.
 #include "render_kernels.h"
 
+#include <algorithm>
+#include <cmath>
+#include <utility>
+#include <vector>
+
 #include "render_bsdf.h"
 #include "render_guide.h"
 #include "render_program.h"
@@ -767,45 +772,6 @@ __global__ __launch_bounds__(64) void tables_kernel(float *t, uint32_t phase) {
 }
 
 // A fill kernel (graph-safe in place of memset nodes).
-// Bounding spheres of the analytic lights (see nearest_light): group g's
-// sphere at out[g], light l's at out[groups + l]. One thread per group.
-__global__ __launch_bounds__(64) void light_bounds_kernel(const float4 *lights, uint32_t n, float4 *out) {
-    uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= groups) return;
-    uint32_t first = g * kLightGroup, end = min(n, first + kLightGroup);
-    float4 sph[kLightGroup];
-    V3 centre{0, 0, 0};
-    uint32_t hittable = 0;
-    for (uint32_t l = first; l < end; l++) {
-        const float4 *r = lights + (size_t)l * 7u;
-        uint32_t type = (uint32_t)r[0].w;
-        float radius = -1.0f;
-        if (type == LIGHT_SPHERE) radius = r[1].w;
-        else if (type < LIGHT_DISTANT) radius = length(v3(r[1])) + length(v3(r[2]));
-        V3 c = v3(r[0]);
-        if (radius >= 0.0f) {
-            float scale = fmaxf(fabsf(c.x), fmaxf(fabsf(c.y), fabsf(c.z)));
-            radius = radius * 1.001f + 1e-5f * scale + 1e-30f;
-            centre = centre + c;
-            hittable++;
-        }
-        sph[l - first] = f4(c, radius);
-        out[groups + l] = sph[l - first];
-    }
-    float group_radius = -1.0f;
-    if (hittable) {
-        centre = centre * (1.0f / (float)hittable);
-        group_radius = 0.0f;
-        for (uint32_t l = first; l < end; l++) {
-            float4 s = sph[l - first];
-            if (s.w >= 0.0f) group_radius = fmaxf(group_radius, length(v3(s) - centre) + s.w);
-        }
-        float scale = fmaxf(fabsf(centre.x), fmaxf(fabsf(centre.y), fabsf(centre.z)));
-        group_radius = group_radius * 1.001f + 1e-5f * scale + 1e-30f;
-    }
-    out[g] = f4(centre, group_radius);
-}
 
 __global__ __launch_bounds__(256) void fill_kernel(uint32_t *dst, size_t words, uint32_t value) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
@@ -923,10 +889,99 @@ void launch_converged(const rt::Params *p, uint32_t pixels, uint32_t *out, hipSt
     hipLaunchKernelGGL(rt::converged_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, s, p, out);
 }
 
-void launch_light_bounds(const float4 *lights, uint32_t n, float4 *out, hipStream_t s) {
-    uint32_t groups = (n + rt::kLightGroup - 1u) / rt::kLightGroup;
-    if (groups == 0) return;
-    hipLaunchKernelGGL(rt::light_bounds_kernel, dim3((groups + 63) / 64), dim3(64), 0, s, lights, n, out);
+std::vector<rt::LightNode> light_tree(const float *lights, uint32_t n) {
+    // Each hittable light's box (a sphere's, or the rectangle / disk's
+    // c +- (|u| + |v|) per axis) in f64, grown by 1e-4 of the lights' scale
+    // and rounded outward to f32; a light with a non-finite number gets an
+    // infinite box (always visited).
+    struct Item {
+        uint64_t key;
+        uint32_t light;
+        float lo[3], hi[3];
+        double c[3];
+    };
+    std::vector<Item> items;
+    double scale = 1.0;
+    for (uint32_t l = 0; l < n; l++) {
+        const float *r = lights + (size_t)l * 28u;
+        if ((uint32_t)r[3] >= rt::LIGHT_DISTANT) continue;
+        for (int a = 0; a < 3; a++) {
+            double half = (uint32_t)r[3] == rt::LIGHT_SPHERE ? std::fabs((double)r[7])
+                                                             : std::fabs((double)r[4 + a]) + std::fabs((double)r[8 + a]);
+            if (std::isfinite(r[a]) && std::isfinite(half)) scale = std::max(scale, std::fabs((double)r[a]) + half);
+        }
+    }
+    double margin = 1e-4 * scale;
+    double box_lo[3] = {INFINITY, INFINITY, INFINITY}, box_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t l = 0; l < n; l++) {
+        const float *r = lights + (size_t)l * 28u;
+        uint32_t type = (uint32_t)r[3];
+        if (type >= rt::LIGHT_DISTANT) continue;
+        Item it{0, l, {}, {}, {}};
+        for (int a = 0; a < 3; a++) {
+            double half = type == rt::LIGHT_SPHERE ? std::fabs((double)r[7])
+                                                    : std::fabs((double)r[4 + a]) + std::fabs((double)r[8 + a]);
+            double c = r[a], lo = c - half - margin, hi = c + half + margin;
+            bool finite = std::isfinite(lo) && std::isfinite(hi);
+            it.lo[a] = finite ? std::nextafter((float)lo, -INFINITY) : -INFINITY;
+            it.hi[a] = finite ? std::nextafter((float)hi, INFINITY) : INFINITY;
+            it.c[a] = finite ? c : 0.0;
+            box_lo[a] = std::min(box_lo[a], it.c[a]);
+            box_hi[a] = std::max(box_hi[a], it.c[a]);
+        }
+        items.push_back(it);
+    }
+    auto spread = [](uint32_t v) {  // 10 bits to every third of 30
+        uint64_t x = v & 1023u;
+        x = (x | (x << 16)) & 0x030000FFull;
+        x = (x | (x << 8)) & 0x0300F00Full;
+        x = (x | (x << 4)) & 0x030C30C3ull;
+        x = (x | (x << 2)) & 0x09249249ull;
+        return x;
+    };
+    for (Item &it : items)
+        for (int a = 0; a < 3; a++) {
+            double extent = box_hi[a] - box_lo[a];
+            double f = extent > 0.0 ? (it.c[a] - box_lo[a]) / extent : 0.0;
+            it.key |= spread((uint32_t)std::min(1023.0, std::max(0.0, f * 1023.0))) << a;
+        }
+    std::sort(items.begin(), items.end(),
+              [](const Item &x, const Item &y) { return x.key != y.key ? x.key < y.key : x.light < y.light; });
+    // Halves of the Morton order, emitted in pre-order with escape links.
+    std::vector<rt::LightNode> nodes;
+    nodes.reserve(items.empty() ? 0 : 2 * items.size() - 1);
+    struct Build {
+        std::vector<Item> &items;
+        std::vector<rt::LightNode> &nodes;
+        void operator()(size_t a, size_t b) {
+            size_t at = nodes.size();
+            nodes.push_back(rt::LightNode{});
+            rt::LightNode nd{{INFINITY, INFINITY, INFINITY}, 0u, {-INFINITY, -INFINITY, -INFINITY}, rt::kNoneU};
+            for (size_t k = a; k < b; k++)
+                for (int c = 0; c < 3; c++) {
+                    nd.lo[c] = std::min(nd.lo[c], items[k].lo[c]);
+                    nd.hi[c] = std::max(nd.hi[c], items[k].hi[c]);
+                }
+            if (b - a == 1) {
+                nd.light = items[a].light;
+            } else {
+                size_t mid = (a + b) / 2;
+                (*this)(a, mid);
+                (*this)(mid, b);
+            }
+            nd.escape = (uint32_t)nodes.size();
+            nodes[at] = nd;
+        }
+    };
+    if (!items.empty()) Build{items, nodes}(0, items.size());
+    return nodes;
+}
+
+hipError_t upload_light_tree(const std::vector<rt::LightNode> &nodes, rt::LightNode *out, hipStream_t s) {
+    if (nodes.empty()) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(out, nodes.data(), nodes.size() * sizeof(rt::LightNode), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `nodes` is the caller's
+    return e;
 }
 
 void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s) {

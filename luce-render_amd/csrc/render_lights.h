@@ -282,68 +282,73 @@ __device__ __forceinline__ float intersect_light(const Params &p, uint32_t light
     return inside ? t : kInf;
 }
 
-// Culling for the analytic lights a ray may meet: a bounding sphere per light
-// and per group of kLightGroup consecutive lights (radius < 0: nothing to
-// hit), inflated so that f32 rounding never culls a light intersect_light
-// would return. Lights are still tested in index order, so the nearest light
-// and its ties come out exactly as from the plain loop.
-constexpr uint32_t kLightGroup = 8u;
-
-__device__ __forceinline__ bool sphere_may_hit(float4 s, V3 o, V3 d, float dd, float t_max) {
-    if (s.w < 0.0f) return false;
-    V3 w = v3(s) - o;
-    float b = dot(w, d);
-    float ww = dot(w, w);
-    float r2 = s.w * s.w * dd;
-    float slack = 1e-5f * ww * dd;
-    if (ww * dd - b * b > r2 + slack) return false;      // the line passes outside
-    if (b < 0.0f && b * b > r2 + slack) return false;    // wholly behind the origin
-    if (t_max < kInf) {
-        float e = b - t_max * dd;
-        float far = t_max * dd;
-        if (e > 0.0f && e * e > r2 + slack + 1e-5f * far * far) return false;  // wholly beyond t_max
+// Culling for the analytic lights a ray may meet: a tree over the lights'
+// boxes (built on the host: light_tree, a binary split of their centres'
+// Morton order), walked by each lane for its own ray without a stack: nodes
+// in pre-order, a box the ray misses or a leaf done sends the walk to the
+// node after its subtree. A wave-uniform loop over groups of lights would
+// enter a group when any of 64 incoherent rays may meet it, so every wave
+// would test nearly every light; here a lane tests the lights its own
+// segment passes. Boxes are grown well past rounding, so a light
+// intersect_light would return is never culled.
+template <class Limit, class Visit>
+__device__ __forceinline__ void walk_lights(const Params &p, V3 o, V3 d, Limit t_max, Visit visit) {
+    const LightNode *nodes = p.light_tree;
+    uint32_t count = p.light_nodes;
+    auto inverse = [](float x) { return 1.0f / (fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x)); };
+    V3 inv{inverse(d.x), inverse(d.y), inverse(d.z)};
+    uint32_t i = 0;
+    while (i < count) {
+        const float4 *at = reinterpret_cast<const float4 *>(nodes + i);
+        float4 lo = at[0], hi = at[1];
+        float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
+        float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;
+        float tz0 = (lo.z - o.z) * inv.z, tz1 = (hi.z - o.z) * inv.z;
+        float enter = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        float leave = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        uint32_t light = __float_as_uint(hi.w);
+        if (enter <= leave && enter <= t_max()) {
+            if (light == kNoneU) {
+                i++;
+                continue;
+            }
+            if (visit(light)) return;
+        }
+        i = __float_as_uint(lo.w);
     }
-    return true;
 }
 
 // The nearest analytic light before `t` along the ray (lights.glsl:239-259
-// over every light, intersect_closest.comp:58-66): updates t and what.
+// over every light, intersect_closest.comp:58-66): updates t and what. The
+// reference's loop goes in index order with a strict `<`, so a triangle wins
+// a tie with a light and the lowest index a tie between lights; the walk
+// meets lights in tree order, so a light at exactly the distance of a
+// higher-indexed light found before it is also taken (intersect_light
+// searches one ulp past t while a higher index holds it).
 __device__ __forceinline__ void nearest_light(const Params &p, V3 o, V3 d, bool camera_ray, bool light_flow, float &t,
                                               uint32_t &what) {
-    uint32_t n = light_count(p);
-    float dd = dot(d, d);
-    uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
-    for (uint32_t g = 0; g < groups; g++) {
-        if (!sphere_may_hit(uniform_at(p.light_bounds, g), o, d, dd, t)) continue;
-        uint32_t end = min(n, (g + 1u) * kLightGroup);
-        for (uint32_t l = g * kLightGroup; l < end; l++) {
-            if (!sphere_may_hit(uniform_at(p.light_bounds, groups + l), o, d, dd, t)) continue;
-            if (camera_ray && !(((uint32_t)uniform_light_at(p, l, 5).w & 1u) != 0u)) continue;
-            float tl = intersect_light<true>(p, l, o, d, t, light_flow);
-            if (tl < t) {
-                t = tl;
-                what = kLightHit | l;
-            }
+    walk_lights(p, o, d, [&] { return t; }, [&](uint32_t l) {
+        if (camera_ray && !light_visible(p, l)) return false;
+        // a light holds the hit (a miss, kNoneU, has the light bit too)
+        bool later = what != kNoneU && (what & kLightHit) && (what & ~kLightHit) > l;
+        float tl = intersect_light(p, l, o, d, later ? nextafterf(t, kInf) : t, light_flow);
+        if (tl < t || (later && tl == t)) {
+            t = tl;
+            what = kLightHit | l;
         }
-    }
+        return false;
+    });
 }
 
 // Whether a light path's connection to the camera passes through a visible
 // light before t_max (intersect_shadow.comp:55-60).
 __device__ __forceinline__ bool light_in_the_way(const Params &p, V3 o, V3 d, float t_max) {
-    uint32_t n = light_count(p);
-    float dd = dot(d, d);
-    uint32_t groups = (n + kLightGroup - 1u) / kLightGroup;
-    for (uint32_t g = 0; g < groups; g++) {
-        if (!sphere_may_hit(uniform_at(p.light_bounds, g), o, d, dd, t_max)) continue;
-        uint32_t end = min(n, (g + 1u) * kLightGroup);
-        for (uint32_t l = g * kLightGroup; l < end; l++) {
-            if (!sphere_may_hit(uniform_at(p.light_bounds, groups + l), o, d, dd, t_max)) continue;
-            if ((((uint32_t)uniform_light_at(p, l, 5).w & 1u) != 0u) && intersect_light<true>(p, l, o, d, t_max, true) < t_max)
-                return true;
-        }
-    }
-    return false;
+    bool blocked = false;
+    walk_lights(p, o, d, [&] { return t_max; }, [&](uint32_t l) {
+        blocked = light_visible(p, l) && intersect_light(p, l, o, d, t_max, true) < t_max;
+        return blocked;
+    });
+    return blocked;
 }
 
 __device__ __forceinline__ float light_hit_pdf(const Params &p, uint32_t light, V3 o, V3 d, float t) {
