@@ -447,10 +447,23 @@ def main():
         img = renderer.read()
         renderer.destroy()
         rays = stats["closest_rays"] + stats["shadow_rays"]
+        # The same render with the renderer's per-kernel timers (one pass in
+        # flight, events between the stages): the closest-hit stage's own
+        # rate inside the render (BVH walk, analytic lights, routing).
+        prof = render.Renderer.create(device, sc, replace(st, profile=True), gpu_scene=gpu)
+        prof.finish()
+        kt = prof.kernel_times()
+        prof.destroy()
+        closest_ms = kt["intersect closest"]
         render_stats = {"spp": args.render_spp, "ms_per_sample": round(ms / args.render_spp, 3),
                         "samples_per_s": round(args.render_spp * WIDTH * HEIGHT / (ms * 1e-3), 1),
                         "mrays_per_s_in_render": round(rays / (ms * 1e-3) / 1e6, 1),
                         "closest_rays": stats["closest_rays"], "shadow_rays": stats["shadow_rays"],
+                        "closest_stage_ms_per_sample": round(closest_ms / args.render_spp, 3),
+                        "closest_mrays_per_s_in_render": round(stats["closest_rays"] / (closest_ms * 1e-3) / 1e6, 1),
+                        "batch_over_render_closest": round(value / (stats["closest_rays"] / (closest_ms * 1e-3) / 1e6), 3),
+                        "shadow_stage_ms_per_sample": round(kt["intersect shadow"] / args.render_spp, 3),
+                        "shade_stage_ms_per_sample": round(kt["shade surface"] / args.render_spp, 3),
                         "finite": bool(np.all(np.isfinite(img))), "trace_config": list(st.trace),
                         "variants_ms_per_sample": render_tune}
         log("render:", render_stats)

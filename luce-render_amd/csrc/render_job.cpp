@@ -36,11 +36,13 @@ struct Shape {
     uint32_t pass_samples;  // samples a pass carries at most; slots hold that many
 };
 
-// Slots a pass of several samples aims at: about 2^25, 10-14 GB of path state
-// and traversal spill, a few percent of the 288 GB a GPU holds (1080p with
-// light paths: 10 samples a pass, 19.2 -> 11 ms a sample on the 30M-triangle
-// scene; small images: 16).
-constexpr uint64_t kPassSlots = 1ull << 25;
+// Slots a pass of several samples aims at: about 2^26, 20-28 GB of path state
+// and traversal spill, under a tenth of the 288 GB a GPU holds. Each pass
+// pays the tail rounds' floor (the last incoherent rays' chains of node
+// fetches) once, however many samples it carries: 1080p with light paths
+// takes 16 samples a pass (30.4M-triangle scene, 16 samples: 19.2 ms a sample
+// one sample a pass, 10.0 in passes of 10 and 6, 9.5 in one pass of 16).
+constexpr uint64_t kPassSlots = 1ull << 26;
 constexpr uint32_t kMostPassSamples = 16;
 
 Shape shape_of(const lr_loaded *loaded, const lr_render_settings &st) {
