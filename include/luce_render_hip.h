@@ -263,7 +263,8 @@ typedef struct lr_loaded {
  * traversal variant (0: the scene's configuration). guiding: 0 off, 1 the
  * reference's hash grid of vMF mixtures, 2 the learned guide (an MLP on the
  * matrix cores trained online during the first train_samples samples, 0: 8,
- * with train_steps Adam steps after each, 0: 16; single renders only). */
+ * in passes of a quarter of them, each followed by train_steps Adam steps for
+ * each sample it carried, 0: 16; single renders only). */
 typedef struct lr_render_settings {
     uint32_t width;
     uint32_t height;
@@ -282,10 +283,11 @@ typedef struct lr_render_settings {
     uint32_t train_steps;
     /* Samples one pass may carry when a sample is one band (no reference
      * counterpart: the reference's passes carry one): 0 picks as many as fit
-     * in about 2^25 path slots, at most 16; 1 is the reference's schedule.
-     * Renders that stop pixels by a noise threshold take one, the learned
-     * guide one while it trains; the hash grid learns after each pass. An
-     * unguided image is the same either way (camera paths bit for bit). */
+     * in about 2^26 path slots, at most 16; 1 is the reference's schedule.
+     * Renders that stop pixels by a noise threshold take one; the learned
+     * guide's training samples go in passes of a quarter of them; the hash grid
+     * learns after each pass. An unguided image is the same either way
+     * (camera paths bit for bit). */
     uint32_t pass_samples;
 } lr_render_settings;
 

@@ -56,8 +56,8 @@ Shape shape_of(const lr_loaded *loaded, const lr_render_settings &st) {
     // Several samples a pass only where a sample is one band and adaptive
     // sampling (which checks between samples) is off. A guided render's
     // guide then learns after each pass of samples (the reference's after
-    // each sample: pass_samples 1); the learned guide trains one sample a
-    // pass.
+    // each sample: pass_samples 1); the learned guide trains after each pass
+    // of a quarter of its training samples.
     bool alone = s.bands == 1u && !(loaded->constants[44] > 0.0f);
     uint64_t k = 1;
     if (alone) {
@@ -531,7 +531,10 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
     launch_render_stage(Stage::Finish, sp, params, n, 0, blocks(256, band_items), s);
     mark(KFinish);
     if (learned && train) {
-        launch_learned_train(params, sp, jobs[0]->train_steps, s);
+        // train_steps for each training sample the pass carried
+        RenderJob *j = jobs[0];
+        uint32_t carried = std::min(j->pass_samples, std::max(1u, j->params.train_samples / 4u));
+        launch_learned_train(params, sp, j->train_steps * carried, s);
         mark(KRest);
     }
     if (guide) {

@@ -71,9 +71,13 @@ struct RenderJob {
     int restart_in_place(const lr_loaded *loaded, const lr_render_settings *settings, const float frame[16]);
     // Samples the next pass carries when `done` samples are in.
     uint32_t samples_in_pass(uint32_t done) const {
-        if (bands != 1u || (params.learned && done < params.train_samples)) return 1u;
+        if (bands != 1u) return 1u;
         uint32_t left = settings.samples > done ? settings.samples - done : 1u;
-        return pass_samples < left ? pass_samples : left;
+        uint32_t k = pass_samples < left ? pass_samples : left;
+        // the learned guide trains after each pass of a quarter of its training samples
+        if (params.learned && done < params.train_samples)
+            k = std::min(k, std::min(params.train_samples - done, std::max(1u, params.train_samples / 4u)));
+        return k;
     }
     // Frees everything but the finished image; returns the bytes still held.
     size_t keep_image_only();

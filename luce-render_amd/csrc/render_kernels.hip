@@ -51,7 +51,9 @@ __global__ void begin_pass_kernel(const Params *plist) {
         // to pass_samples, stopping at the target (the host counts alike).
         uint32_t left = p.sample_target > s->sample ? p.sample_target - s->sample : 1u;
         uint32_t k = s->bands == 1u ? (p.pass_samples < left ? p.pass_samples : left) : 1u;
-        if (p.learned && s->sample < p.train_samples) k = 1u;  // it trains after each of those samples
+        // the learned guide trains after each pass of a quarter of its training samples
+        if (p.learned && s->sample < p.train_samples)
+            k = min(k, min(p.train_samples - s->sample, max(1u, p.train_samples / 4u)));
         s->sub_samples = k > 0u ? k : 1u;
         if (s->light_count > 0) s->light_samples += s->sub_samples;
     }

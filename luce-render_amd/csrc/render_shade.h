@@ -73,7 +73,8 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
     // before this vertex adds any, counts it only if it arrived. (Counted as
     // arrived, a blocked light sample leaves the direction's light minus that
     // sample, mostly clipped to nothing where the lights are hidden.)
-    if (LOBEWISE && total >= 1u && total <= GUIDE_RING && st.u(SU_GUIDE_CELL0 + total - 1u, path) != kNoneU) {
+    if (LOBEWISE && ps.sample < p.train_samples && total >= 1u && total <= GUIDE_RING &&
+        st.u(SU_GUIDE_CELL0 + total - 1u, path) != kNoneU) {
         float4 record = st.v(SV_GUIDE0 + total - 1u, path);
         record.z = spec_luminance<SPECTRAL>(p, st.v(SV_RADIANCE, path), lambda);
         st.set_v(SV_GUIDE0 + total - 1u, path, record);
@@ -300,7 +301,9 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
     }
     st.set_u(SU_BOUNCE, path, bounces);
     st.set_u(SU_NORMAL, path, octahedral_encode(normal));
-    if (guide_on && total < GUIDE_RING) {
+    // The learned guide reads records only after its training passes (the
+    // grid after every pass).
+    if (guide_on && total < GUIDE_RING && (!LOBEWISE || ps.sample < p.train_samples)) {
         if (cell == kNoneU) cell = p.learned ? path : guide_cell(p, x, geometric);
         st.set_u(SU_GUIDE_CELL0 + total, path, cell);
         if (p.learned)

@@ -413,7 +413,8 @@ class RenderSettings:
     profile: bool = False
     guiding: int = 0  # 0 off, 1 the hash grid of vMF mixtures, 2 the learned guide
     train_samples: int = 0  # learned guide: samples that train it (0: 8)
-    train_steps: int = 0  # learned guide: Adam steps after each of them (0: 16)
+    train_steps: int = 0  # learned guide: Adam steps for each of them, after their pass (0: 16)
+    # (the training samples go in passes of a quarter of them)
     # The guide's share of the one-sample mixture (loading.lucb:281: 0.5): of
     # all BSDF samples for the grid, of the diffuse lobe's for the learned guide.
     guide_share: float = 0.5
@@ -421,7 +422,7 @@ class RenderSettings:
     use_graph: bool = True
     trace: tuple = (0, 0, 0)  # traversal width, group, stack (0: the scene's)
     # Samples one pass may carry when a sample is one band (0: as many as fit
-    # in about 2^25 path slots, at most 16; 1: the reference's one a pass).
+    # in about 2^26 path slots, at most 16; 1: the reference's one a pass).
     pass_samples: int = 0
 
     @staticmethod
