@@ -512,7 +512,10 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         // below the bounce limit; a path's vertex count trails its round by
         // at most the cut-out pass-throughs, so later rounds infer nothing
         // anyone reads.
-        if (learned && b < std::min(rt::GUIDE_RING, jobs[0]->settings.max_bounces) + jobs[0]->through) {
+        // With light paths the guide is not consulted at the first vertex, so
+        // the first round (every path at its first vertex) infers nothing.
+        if (learned && b < std::min(rt::GUIDE_RING, jobs[0]->settings.max_bounces) + jobs[0]->through &&
+            !(b == 0 && lights > 0)) {
             launch_learned_infer(params, jobs[0]->params.band_size * jobs[0]->pass_samples, s);
             mark(KRest);
         }
