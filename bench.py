@@ -241,9 +241,9 @@ def main():
                     help="samples of the full path-traced render of the same scene (0: skip)")
     ap.add_argument("--render-tune", type=int, default=1,
                     help="tune the render's own traversal variant on a ten-sample pass (0: the batch pick)")
-    ap.add_argument("--serving", type=int, default=40,
+    ap.add_argument("--serving", type=int, default=80,
                     help="requests of the Poisson serving leg (40 requests/s over the four serving scenes; 0: skip)")
-    ap.add_argument("--serving-sustained", type=int, default=24,
+    ap.add_argument("--serving-sustained", type=int, default=48,
                     help="requests of a second serving run at 3/4 of the measured capacity (0: skip)")
     ap.add_argument("--guiding", default="1920x1080:4096",
                     help="WxH:reference-spp of the guiding leg on the glossy workload (empty: skip)")

@@ -456,8 +456,10 @@ __device__ __forceinline__ uint32_t owen(uint32_t x, uint32_t seed) {
 //    bit 31-j is the XOR of the index bits b whose bit pattern holds j's
 //    (j & b == j): a superset sum over the five bits of a bit position, five
 //    shift-mask-XOR steps, then the reversal;
-//  - dimensions 2 and 3 take one pass over the 32 bit positions, unrolled,
-//    the columns immediates and each set bit a mask (no branch).
+//  - dimensions 2 and 3 take one pass over the 32 bit positions, each set bit
+//    a mask (no branch), unrolled by 8 with the columns read at wave-uniform
+//    addresses (fully unrolled, its immediates doubled the shading kernels'
+//    spills: 2.38 against 2.12 ms a 1080p sample of the glossy workload).
 // Equal to the table loop for every index (any XOR order is the same word).
 __device__ __forceinline__ void sobol4(uint32_t index, uint32_t (&x)[4]) {
     x[0] = __builtin_bitreverse32(index);
@@ -469,7 +471,7 @@ __device__ __forceinline__ void sobol4(uint32_t index, uint32_t (&x)[4]) {
     g ^= (g >> 16) & 0x0000FFFFu;
     x[1] = __builtin_bitreverse32(g);
     uint32_t x2 = 0u, x3 = 0u, rest = index;
-#pragma unroll
+#pragma unroll 8
     for (uint32_t bit = 0; bit < 32u; bit++, rest >>= 1) {
         uint32_t m = 0u - (rest & 1u);
         x2 ^= m & kSobol[64u + bit];
