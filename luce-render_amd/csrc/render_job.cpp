@@ -231,7 +231,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     b += loaded->light_floats * 4 + 512 + loaded->material_floats * 4 + loaded->emitter_words * 4;
     b += loaded->program_words * 4 + loaded->texture_floats * 4 + loaded->texture_count * 16;
     uint32_t nlights = (uint32_t)loaded->constants[0];
-    b += ((size_t)(nlights + 7u) / 8u + nlights) * 16;
+    b += (size_t)std::max(1u, 2u * nlights) * sizeof(rt::LightNode);  // the lights' culling tree, as create allocates
     b += (size_t)rt::kStateV * s.slots * 16 + (size_t)rt::kStateU * s.slots * 4;
     b += ((size_t)rt::kQueueBase + (size_t)rt::kQueues * s.slots) * 4;
     b += (size_t)6 * s.pixels * 16;  // film (4 planes), image, display
