@@ -143,6 +143,10 @@ struct Params {
     // node count; empty when no light can be hit.
     const LightNode *light_tree;
     uint32_t light_nodes;
+    // The distant and dome lights (nothing to hit, found by direction), in
+    // index order: what the miss shading and the distant pick walk.
+    const uint32_t *far_lights;
+    uint32_t far_count;
     // The learned guide (guiding 2): the network, the per-slot mixtures the
     // inference writes each bounce (rows of `pool` float4: four lobes, then
     // the lobe weights), the training gradient rows and schedule.

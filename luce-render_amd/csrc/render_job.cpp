@@ -150,6 +150,11 @@ int RenderJob::restart_in_place(const lr_loaded *loaded, const lr_render_setting
         e = upload_light_tree(tree, const_cast<rt::LightNode *>(p.light_tree), s);
         if (e != hipSuccess) return hip_fail(e, "restart light tree");
         p.light_nodes = (uint32_t)tree.size();
+        std::vector<uint32_t> far = far_lights(loaded->lights, analytic_lights);
+        if (!far.empty()) e = hipMemcpyAsync(const_cast<uint32_t *>(p.far_lights), far.data(), far.size() * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);  // `far` is on this stack
+        if (e != hipSuccess) return hip_fail(e, "restart far lights");
+        p.far_count = (uint32_t)far.size();
     }
     bool resized = st.width != settings.width || st.height != settings.height;
     settings.samples = st.samples;
@@ -232,6 +237,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     b += loaded->program_words * 4 + loaded->texture_floats * 4 + loaded->texture_count * 16;
     uint32_t nlights = (uint32_t)loaded->constants[0];
     b += (size_t)std::max(1u, 2u * nlights) * sizeof(rt::LightNode);  // the lights' culling tree, as create allocates
+    b += (size_t)std::max(1u, nlights) * 4;                             // and the distant lights' list (at most all)
     b += (size_t)rt::kStateV * s.slots * 16 + (size_t)rt::kStateU * s.slots * 4;
     b += ((size_t)rt::kQueueBase + (size_t)rt::kQueues * s.slots) * 4;
     b += (size_t)6 * s.pixels * 16;  // film (4 planes), image, display
@@ -371,6 +377,16 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
         if (hipError_t e = upload_light_tree(tree, const_cast<rt::LightNode *>(p.light_tree), s))
             return hip_fail(e, "light tree");
         p.light_nodes = (uint32_t)tree.size();
+        std::vector<uint32_t> far = far_lights(loaded->lights, nlights);
+        // room for every light, so a reload keeps the buffer
+        p.far_lights = (const uint32_t *)alloc((size_t)std::max(1u, nlights) * 4, rc);
+        if (rc) return rc;
+        hipError_t e = far.empty() ? hipSuccess
+                                   : hipMemcpyAsync(const_cast<uint32_t *>(p.far_lights), far.data(), far.size() * 4,
+                                                    hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);  // `far` is on this stack
+        if (e != hipSuccess) return hip_fail(e, "far lights");
+        p.far_count = (uint32_t)far.size();
     }
     float *tables = shared_tables;
     if (!tables) {

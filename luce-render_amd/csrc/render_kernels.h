@@ -62,6 +62,8 @@ void launch_converged(const rt::Params *p, uint32_t pixels, uint32_t *out, hipSt
 // floats each): rt::LightNode in pre-order, 2h - 1 nodes for h lights that
 // can be hit (none for distant and dome lights).
 std::vector<rt::LightNode> light_tree(const float *lights, uint32_t n);
+// The distant and dome lights of `n` host records, in index order.
+std::vector<uint32_t> far_lights(const float *lights, uint32_t n);
 // Copies a tree to `out` (room for nodes.size()) and waits for the copy.
 hipError_t upload_light_tree(const std::vector<rt::LightNode> &nodes, rt::LightNode *out, hipStream_t s);
 

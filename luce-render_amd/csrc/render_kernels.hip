@@ -446,7 +446,6 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(cons
         const Params &p = plist[r];
         State st(p);
         const PassState &ps = *p.pass;
-        uint32_t nlights = light_count(p);
         uint32_t path = queue_path(p, Q_MISS, at);
         if (path >= pass_cameras(ps)) continue;  // light paths that leave just end
         uint32_t bounce = bounce_total(st.u(SU_BOUNCE, path));
@@ -455,9 +454,11 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void miss_kernel(cons
         Spec throughput = st.v(SV_THROUGHPUT, path);
         Spec lambda = st.v(SV_LAMBDA, path);
         Spec radiance = spec_of<SPECTRAL>(K(p, K_BACKGROUND_FIT), K(p, K_BACKGROUND_RGB), lambda);
-        for (uint32_t l = 0; l < nlights; l++) {
+        // the distant and dome lights, in index order (the lights that can
+        // be hit add nothing to a miss)
+        for (uint32_t k = 0; k < p.far_count; k++) {
+            uint32_t l = p.far_lights[k];
             uint32_t type = light_type(p, l);
-            if (type < LIGHT_DISTANT) continue;
             if (type == LIGHT_DISTANT) {
                 float cos_half = light_at(p, l, 2).w;
                 if (cos_half >= 1.0f - 1e-7f || dot(rd, -v3(light_at(p, l, 3))) < cos_half) continue;
@@ -977,6 +978,13 @@ std::vector<rt::LightNode> light_tree(const float *lights, uint32_t n) {
     };
     if (!items.empty()) Build{items, nodes}(0, items.size());
     return nodes;
+}
+
+std::vector<uint32_t> far_lights(const float *lights, uint32_t n) {
+    std::vector<uint32_t> out;
+    for (uint32_t l = 0; l < n; l++)
+        if ((uint32_t)lights[(size_t)l * 28u + 3u] >= rt::LIGHT_DISTANT) out.push_back(l);
+    return out;
 }
 
 hipError_t upload_light_tree(const std::vector<rt::LightNode> &nodes, rt::LightNode *out, hipStream_t s) {

@@ -67,9 +67,9 @@ template <bool SPECTRAL> __device__ __forceinline__ Spec light_radiance(const Pa
 
 // A distant or dome light by power; `pdf` its share among them.
 __device__ __forceinline__ uint32_t pick_distant(const Params &p, float u, float &pdf) {
-    uint32_t n = light_count(p), chosen = kNoneU;
-    for (uint32_t l = 0; l < n; l++) {
-        if (light_type(p, l) < LIGHT_DISTANT) continue;
+    uint32_t chosen = kNoneU;
+    for (uint32_t k = 0; k < p.far_count; k++) {
+        uint32_t l = p.far_lights[k];
         chosen = l;
         if (u < light_at(p, l, 6).z) break;
     }
