@@ -92,6 +92,42 @@ def test_many_lights_match_f64_oracle(device, spectral):
     assert close >= 0.99
 
 
+def test_tied_lights_resolve_as_the_reference_loop(device):
+    """Two rect lights in one horizontal plane whose footprints overlap, the
+    lower index at the larger x: a camera ray through the overlap meets both
+    at exactly the same distance (axis-aligned frames, so the plane distance
+    has no rounding between them). The reference tests lights in index order
+    with a strict `<` (intersect_closest.comp:58-66), so the lower index is
+    the one seen; the closest hit's light tree meets the higher index first
+    (its centre has the smaller Morton code) and must still end on the lower
+    one. Camera rays only, pixel by pixel against the f64 oracle's loop; the
+    lights' radiances differ tenfold, so a wrong pick shows."""
+    import dataclasses
+    from luce_render import scenes
+    gset = scenes.cornell_render_set()
+    base = gset.lights[0]
+    lights = []
+    for x, radiance in ((0.15, 5.0), (-0.15, 50.0)):
+        m = np.array(base.matrix, dtype=np.float64).copy()
+        m[3, :3] = (x, 0.6, 0.0)
+        lights.append(dataclasses.replace(base, matrix=m, intensity=radiance, width=0.5, height=0.5))
+    gset.lights = lights
+    sc, st, img = _render(device, gset, width=96, height=96, samples=1, spectral=False, max_bounces=1,
+                          light_tracing=0.0)
+    pixels = np.arange(96 * 96, dtype=np.uint32)
+    got, ref = _oracle_compare(sc, st, img, pixels)
+    lum = ref.mean(axis=1)
+    dim = np.abs(lum - 5.0) < 1e-3 * 5.0  # light 0 seen
+    bright = np.abs(lum - 50.0) < 1e-3 * 50.0
+    say(f"tied lights: light 0 on {dim.sum()} pixels, light 1 on {bright.sum()} pixels")
+    # Light 0 alone spans x in [0.1, 0.4], light 1 alone [-0.4, -0.1], the overlap [-0.1, 0.1] goes to light 0.
+    assert bright.sum() > 10 and dim.sum() > 1.5 * bright.sum()
+    assert np.allclose(got[dim], ref[dim], rtol=1e-5)
+    assert np.allclose(got[bright], ref[bright], rtol=1e-5)
+    per_pixel = (np.abs(got - ref) / (np.abs(ref) + 1e-6)).max(axis=1)
+    assert (per_pixel < 1e-3).mean() >= 0.99
+
+
 @pytest.mark.parametrize("spectral", [False, True])
 @pytest.mark.parametrize("scene", ["cornell", "many_lights", "dome"])
 def test_light_traced_render_matches_f64_oracle(device, scene, spectral):
