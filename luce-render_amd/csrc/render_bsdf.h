@@ -425,10 +425,9 @@ __device__ __forceinline__ Spec evaluate_surface(const Surface &s, V3 wi, V3 wo,
     return f + s.substrate * below;
 }
 
-// openpbr.glsl:522-547
-__device__ __forceinline__ bool sample_direction(const Surface &s, V3 wo, float ux, float uy, float uz, V3 &wi,
-                                                 float &pdf, uint32_t &lobe) {
-    pdf = 0.0f;
+// openpbr.glsl:522-547: the lobe and the direction, without the density.
+__device__ __forceinline__ bool pick_direction(const Surface &s, V3 wo, float ux, float uy, float uz, V3 &wi,
+                                               uint32_t &lobe) {
     if (wo.z <= 0.0f) return false;
     float pick = uz;
     if (pick < s.p_diffuse) {
@@ -449,7 +448,14 @@ __device__ __forceinline__ bool sample_direction(const Surface &s, V3 wo, float 
     } else {
         return false;
     }
-    if (wi.z == 0.0f) return false;
+    return wi.z != 0.0f;
+}
+
+// openpbr.glsl:522-547
+__device__ __forceinline__ bool sample_direction(const Surface &s, V3 wo, float ux, float uy, float uz, V3 &wi,
+                                                 float &pdf, uint32_t &lobe) {
+    pdf = 0.0f;
+    if (!pick_direction(s, wo, ux, uy, uz, wi, lobe)) return false;
     pdf = surface_pdf(s, wi, wo);
     return pdf > 0.0f;
 }

@@ -148,10 +148,11 @@ struct Params {
     const uint32_t *far_lights;
     uint32_t far_count;
     // The learned guide (guiding 2): the network, the per-slot mixtures the
-    // inference writes each bounce (rows of `pool` float4: four lobes, then
-    // the lobe weights), the training gradient rows and schedule.
+    // inference writes each bounce (two rows of `pool` 16-byte words: lobes
+    // 0-1, then 2-3, each an octahedral mean and a (kappa, weight) pair of
+    // halves), the training gradient rows and schedule.
     net::NetBuffers net;
-    float4 *guide_mix;
+    uint4 *guide_mix;
     float *net_partials;
     uint32_t learned;
     uint32_t train_samples;  // samples whose paths train it

@@ -61,10 +61,7 @@ __device__ __forceinline__ float guide_count(const Params &p, uint32_t cell) {
     return c;
 }
 
-// With the learned guide, `cell` is the path's slot and the lobes are the
-// mixture inferred for its current vertex.
 __device__ __forceinline__ float4 guide_lobe(const Params &p, uint32_t cell, uint32_t lobe) {
-    if (p.learned) return p.guide_mix[(size_t)lobe * p.pool + cell];
     float4 v = guide_lobes(p)[cell * (GUIDE_LOBES + 1u) + lobe];
     if (v.w > 0.0f) return v;
     const float k = 0.57735026919f;  // the tetrahedron's directions, normalized
@@ -75,7 +72,6 @@ __device__ __forceinline__ float4 guide_lobe(const Params &p, uint32_t cell, uin
 }
 
 __device__ __forceinline__ float4 guide_weights(const Params &p, uint32_t cell) {
-    if (p.learned) return p.guide_mix[(size_t)GUIDE_LOBES * p.pool + cell];
     float4 w = guide_lobes(p)[cell * (GUIDE_LOBES + 1u) + GUIDE_LOBES];
     float total = w.x + w.y + w.z + w.w;
     return total > 0.0f ? w / total : make_float4(0.25f, 0.25f, 0.25f, 0.25f);
@@ -116,6 +112,69 @@ __device__ __forceinline__ V3 guide_sample(const Params &p, uint32_t cell, float
         pick -= lane(w, (int)k);
     }
     return vmf_sample(guide_lobe(p, cell, lobe), ux, uy);
+}
+
+// The learned guide's mixture of a path's current vertex as inference stores
+// it: two 16-byte words a path slot (lobes 0-1, then 2-3, `pool` words
+// apart), each lobe its mean's octahedral code and a (kappa, weight) pair of
+// halves. Decoding renormalises the weights, so sampling and density read one
+// distribution.
+struct LearnedMix {
+    uint4 lo, hi;
+};
+
+__device__ __forceinline__ uint32_t half_pair(float a, float b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)a) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)b) << 16);
+}
+__device__ __forceinline__ float half_lo(uint32_t x) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(x & 0xffffu)); }
+__device__ __forceinline__ float half_hi(uint32_t x) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(x >> 16)); }
+
+__device__ __forceinline__ void store_learned_mix(const Params &p, uint32_t path, const V3 (&mu)[GUIDE_LOBES],
+                                                  const float (&kappa)[GUIDE_LOBES], const float (&w)[GUIDE_LOBES]) {
+    uint4 *m = p.guide_mix;
+    m[path] = make_uint4(octahedral_encode(mu[0]), half_pair(kappa[0], w[0]), octahedral_encode(mu[1]),
+                         half_pair(kappa[1], w[1]));
+    m[(size_t)p.pool + path] = make_uint4(octahedral_encode(mu[2]), half_pair(kappa[2], w[2]),
+                                          octahedral_encode(mu[3]), half_pair(kappa[3], w[3]));
+}
+__device__ __forceinline__ LearnedMix learned_mix(const Params &p, uint32_t path) {
+    const uint4 *m = p.guide_mix;
+    return LearnedMix{m[path], m[(size_t)p.pool + path]};
+}
+__device__ __forceinline__ uint32_t mix_word(const LearnedMix &m, uint32_t i) {
+    const uint4 &q = i < 4u ? m.lo : m.hi;
+    uint32_t j = i & 3u;
+    return j == 0u ? q.x : j == 1u ? q.y : j == 2u ? q.z : q.w;
+}
+// Lobe k as (mean, kappa) in guide_lobe's float4 form.
+__device__ __forceinline__ float4 mix_lobe(const LearnedMix &m, uint32_t k) {
+    return f4(octahedral_decode(mix_word(m, 2u * k)), half_lo(mix_word(m, 2u * k + 1u)));
+}
+__device__ __forceinline__ float4 mix_weights(const LearnedMix &m) {
+    float4 w = make_float4(half_hi(m.lo.y), half_hi(m.lo.w), half_hi(m.hi.y), half_hi(m.hi.w));
+    float total = w.x + w.y + w.z + w.w;
+    return total > 0.0f ? w / total : make_float4(0.25f, 0.25f, 0.25f, 0.25f);
+}
+__device__ __forceinline__ float learned_pdf(const LearnedMix &m, V3 d) {
+    float4 w = mix_weights(m);
+    float pdf = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) pdf += lane(w, (int)k) * vmf_pdf(mix_lobe(m, k), d);
+    return pdf;
+}
+__device__ __forceinline__ V3 learned_sample(const LearnedMix &m, float ux, float uy, float uz) {
+    float4 w = mix_weights(m);
+    uint32_t lobe = GUIDE_LOBES - 1u;
+    float pick = uz;
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) {
+        if (pick < lane(w, (int)k)) {
+            lobe = k;
+            break;
+        }
+        pick -= lane(w, (int)k);
+    }
+    return vmf_sample(mix_lobe(m, lobe), ux, uy);
 }
 
 // guide.glsl:121-139: credit the lobe that explains d best.

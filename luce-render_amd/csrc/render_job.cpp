@@ -243,7 +243,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     b += (size_t)6 * s.pixels * 16;  // film (4 planes), image, display
     if (st->guiding == 1) b += rt::kGuideBytes;
     if (st->guiding == 2)
-        b += (size_t)5 * s.slots * 16 + (size_t)kNetRows * (net::kParams + 1) * 4 + (size_t)net::kParams * 12 +
+        b += (size_t)2 * s.slots * 16 + (size_t)kNetRows * (net::kParams + 1) * 4 + (size_t)net::kParams * 12 +
              net::kFragmentHalves * 4 + 128;
     uint32_t w = st->trace_width ? st->trace_width : scene->config.width;
     uint32_t stack = st->trace_stack ? st->trace_stack : scene->config.stack;
@@ -430,7 +430,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
         if (nrc) return nrc;
         p.net = net->b;
         p.net_partials = net->partials;
-        p.guide_mix = (float4 *)alloc((size_t)5 * slots * 16, rc);
+        p.guide_mix = (uint4 *)alloc((size_t)2 * slots * 16, rc);
         p.learned = 1;
         p.train_samples = st.train_samples ? st.train_samples : 8u;
         p.train_batch = 8192u;
@@ -457,7 +457,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     launch_fill(p.queues, queue_words, 0u, s);
     launch_fill(p.state_u, (size_t)rt::kStateU * slots, 0u, s);
     if (p.guide) launch_fill((uint32_t *)p.guide, rt::kGuideBytes / 4, 0u, s);
-    if (p.guide_mix) launch_fill((uint32_t *)p.guide_mix, (size_t)5 * slots * 4, 0u, s);
+    if (p.guide_mix) launch_fill((uint32_t *)p.guide_mix, (size_t)2 * slots * 4, 0u, s);
     rt::PassState init{};
     init.bands = bands;
     hipError_t e = hipMemcpyAsync(p.pass, &init, sizeof(init), hipMemcpyHostToDevice, s);

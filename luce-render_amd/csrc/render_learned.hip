@@ -40,9 +40,9 @@ struct SurfacePoints {
     }
     __device__ __forceinline__ void store(uint32_t i, const net::Mixture &m) const {
         uint32_t path = queue_path(p, Q_SURFACE, i);
-        for (uint32_t k = 0; k < net::kLobes; k++)
-            p.guide_mix[(size_t)k * p.pool + path] = make_float4(m.mu[k][0], m.mu[k][1], m.mu[k][2], m.kappa[k]);
-        p.guide_mix[(size_t)net::kLobes * p.pool + path] = make_float4(m.w[0], m.w[1], m.w[2], m.w[3]);
+        V3 mu[GUIDE_LOBES];
+        for (uint32_t k = 0; k < GUIDE_LOBES; k++) mu[k] = V3{m.mu[k][0], m.mu[k][1], m.mu[k][2]};
+        store_learned_mix(p, path, mu, m.kappa, m.w);
     }
 };
 

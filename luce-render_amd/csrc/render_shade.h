@@ -187,6 +187,10 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
         if (suited && ready && (!LOBEWISE || !tracing || total >= 1u))
             guided = K(p, K_GUIDE).z * (p.learned ? surface.p_diffuse : 1.0f);
     }
+    // The learned mixture (32 bytes) is read once for every use below.
+    LearnedMix mix{};
+    if constexpr (LOBEWISE)
+        if (guided > 0.0f) mix = learned_mix(p, path);
 
     float4 u_light = sample4(p.seed, pixel, sample, group_light(total));
     float p_distant = K(p, K_TREE).z;
@@ -225,7 +229,7 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
         if ((side > 0.0f) == (wi.z > 0.0f) && (side > 0.0f || surface.p_transmission > 0.0f)) {
             float bsdf_pdf;
             Spec f = evaluate_surface(surface, wi, wo, bsdf_pdf);
-            if (guided > 0.0f && !tracing) bsdf_pdf = LOBEWISE ? lobewise_pdf(bsdf_pdf, guide_pdf(p, cell, s.direction), wi.z, guided) : guided * guide_pdf(p, cell, s.direction) + (1.0f - guided) * bsdf_pdf;
+            if (guided > 0.0f && !tracing) bsdf_pdf = LOBEWISE ? lobewise_pdf(bsdf_pdf, learned_pdf(mix, s.direction), wi.z, guided) : guided * guide_pdf(p, cell, s.direction) + (1.0f - guided) * bsdf_pdf;
             if (bsdf_pdf > 0.0f) {
                 float light_pdf = s.pdf * chance;
                 float traced = 0.0f;
@@ -269,25 +273,41 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
     uint32_t lobe;
     float unguided = 1.0f;
     float bsdf_only;
-    if (u_opacity.y < guided) {
+    if constexpr (LOBEWISE) {
+        // The guide's direction or the BSDF's lobe choice (the diffuse lobe
+        // keeping p_diffuse - guided of it), then one evaluation of the BSDF
+        // for either: a wave's lanes take both choices, and the two
+        // evaluations would run one after the other.
+        bool from_guide = u_opacity.y < guided;
+        if (from_guide) {
+            wi = to_local(surface, learned_sample(mix, u_opacity.z, u_opacity.w, u.w));
+            lobe = wi.z < 0.0f ? LOBE_TRANSMISSION : LOBE_DIFFUSE;
+        } else if (!pick_direction(surface, wo, u.x, u.y, guided + u.w * (1.0f - guided), wi, lobe)) {
+            return out;
+        }
+        float bsdf_pdf;
+        Spec f = evaluate_surface(surface, wi, wo, bsdf_pdf);
+        if (!from_guide && !(bsdf_pdf > 0.0f)) return out;
+        bsdf_only = bsdf_pdf;
+        pdf = guided > 0.0f ? lobewise_pdf(bsdf_pdf, learned_pdf(mix, to_world(surface, wi)), wi.z, guided) : bsdf_pdf;
+        if (!(pdf > 0.0f) || (from_guide && spec_max<SPECTRAL>(f) <= 0.0f)) return out;
+        unguided = from_guide ? pdf / fmaxf(bsdf_pdf, 1e-6f * pdf) : pdf / bsdf_pdf;
+        weight = f * (fabsf(wi.z) / pdf);
+    } else if (u_opacity.y < guided) {
         wi = to_local(surface, guide_sample(p, cell, u_opacity.z, u_opacity.w, u.w));
         float bsdf_pdf;
         Spec f = evaluate_surface(surface, wi, wo, bsdf_pdf);
-        pdf = LOBEWISE ? lobewise_pdf(bsdf_pdf, guide_pdf(p, cell, to_world(surface, wi)), wi.z, guided) : guided * guide_pdf(p, cell, to_world(surface, wi)) + (1.0f - guided) * bsdf_pdf;
+        pdf = guided * guide_pdf(p, cell, to_world(surface, wi)) + (1.0f - guided) * bsdf_pdf;
         bsdf_only = bsdf_pdf;
         if (pdf <= 0.0f || spec_max<SPECTRAL>(f) <= 0.0f) return out;
         unguided = pdf / fmaxf(bsdf_pdf, 1e-6f * pdf);
         weight = f * (fabsf(wi.z) / pdf);
-        lobe = wi.z < 0.0f ? LOBE_TRANSMISSION : (LOBEWISE ? LOBE_DIFFUSE : (surface.p_diffuse > 0.5f ? LOBE_DIFFUSE : LOBE_GLOSSY));
+        lobe = wi.z < 0.0f ? LOBE_TRANSMISSION : (surface.p_diffuse > 0.5f ? LOBE_DIFFUSE : LOBE_GLOSSY);
     } else {
-        // Lobe-wise, given no guide sample the diffuse lobe keeps
-        // p_diffuse - guided of the lobe choice and every other lobe its own:
-        // the pick is moved past the guide's chance.
-        if (!sample_surface(surface, wo, u.x, u.y, LOBEWISE ? guided + u.w * (1.0f - guided) : u.w, wi, weight, pdf, lobe)) return out;
+        if (!sample_surface(surface, wo, u.x, u.y, u.w, wi, weight, pdf, lobe)) return out;
         bsdf_only = pdf;
         if (guided > 0.0f) {
-            float mixed = LOBEWISE ? lobewise_pdf(pdf, guide_pdf(p, cell, to_world(surface, wi)), wi.z, guided) : guided * guide_pdf(p, cell, to_world(surface, wi)) + (1.0f - guided) * pdf;
-            if (LOBEWISE && !(mixed > 0.0f)) return out;
+            float mixed = guided * guide_pdf(p, cell, to_world(surface, wi)) + (1.0f - guided) * pdf;
             unguided = mixed / pdf;
             weight = weight * (pdf / mixed);
             pdf = mixed;
