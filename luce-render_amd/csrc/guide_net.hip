@@ -53,7 +53,7 @@ struct ArrayPoints {
 
 __global__ __launch_bounds__(256) void adam_kernel(NetBuffers net, float *partials, uint32_t waves, uint32_t samples,
                                                    float rate) {
-    adam_body(net, partials, waves, samples, rate, false);
+    adam_body(net, partials, waves, samples, rate);
 }
 
 __global__ void zero_kernel(float *p, size_t n) {
@@ -125,7 +125,7 @@ int GuideNet::set_params(const float *params) {
 }
 
 void GuideNet::refresh(hipStream_t s) {
-    hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, b, partials, 0u, 0u, 0.0f);
+    hipLaunchKernelGGL(adam_kernel, dim3(kAdamBlocks), dim3(256), 0, s, b, partials, 0u, 0u, 0.0f);
 }
 
 }  // namespace lr
@@ -183,15 +183,13 @@ static int batch(lr_guide_net *net, const float *samples, size_t n, float rate, 
     hipError_t e = hipMalloc(&dev, n * 40);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(batch)");
     e = hipMemcpyAsync(dev, samples, n * 40, hipMemcpyHostToDevice, s);
-    uint32_t waves = (uint32_t)std::min<size_t>(256, (n + 15) / 16);
-    uint32_t rows = std::min(waves, g.waves);
+    // one wave (and partial row) per 32 samples, at most the rows allocated;
+    // every wave of the grid takes at least one pair of tiles
+    uint32_t rows = (uint32_t)std::min<size_t>(g.waves, (n + 31) / 32);
     ArraySamples src{(const float *)dev};
     if (e == hipSuccess) {
-        size_t words = (size_t)rows * (kParams + 1);
-        hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, s, g.partials, words);
         hipLaunchKernelGGL((weights_kernel<ArraySamples>), dim3(64), dim3(256), 0, s, g.b, src, (uint32_t)n);
-        hipLaunchKernelGGL((grad_kernel<ArraySamples>), dim3(waves), dim3(64), 0, s, g.b, src, (uint32_t)n, g.partials,
-                           rows);
+        hipLaunchKernelGGL((grad_kernel<ArraySamples>), dim3(rows), dim3(64), 0, s, g.b, src, (uint32_t)n, g.partials);
         e = hipGetLastError();
     }
     std::vector<float> part;
@@ -200,7 +198,7 @@ static int batch(lr_guide_net *net, const float *samples, size_t n, float rate, 
         e = hipMemcpyAsync(part.data(), g.partials, part.size() * 4, hipMemcpyDeviceToHost, s);
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, g.b, g.partials, rows,
+        hipLaunchKernelGGL(adam_kernel, dim3(kAdamBlocks), dim3(256), 0, s, g.b, g.partials, rows,
                            (uint32_t)n, rate);
         e = hipGetLastError();
     }

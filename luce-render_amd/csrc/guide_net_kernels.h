@@ -79,115 +79,185 @@ __global__ __launch_bounds__(256) void weights_kernel(NetBuffers net, Source src
     weights_body(net, src, n);
 }
 
-// One wave a workgroup. Each wave takes groups of 16 samples (stride: the
-// grid) and adds its gradient sums into row blockIdx.x % rows of `partials`
-// (kParams + 1 floats: the last the loss), which must start at zero. The
-// adds are atomics whose results nobody waits for, so a wave never stalls on
-// them. Pre-activations wait in LDS for the backward pass, keeping the
+// Z^T = W_l^T H^T + b for the two tiles of a pair: layer_forward's products
+// with each weight block loaded once for both tiles.
+__device__ __forceinline__ void pair_forward(const _Float16 *fwd, const float *params, uint32_t l,
+                                             const half4_t (&in)[2][4], uint32_t lane, float4_t (&out)[2][4]) {
+    const uint32_t g = lane / 16u, tiles = layer_out(l) / 16u;
+    half8_t a[4][2];
+#pragma unroll
+    for (uint32_t m = 0; m < 4u; m++)
+        if (m < tiles) {
+            a[m][0] = load_block(fwd, block_base(l) + m * 2u, lane);
+            a[m][1] = load_block(fwd, block_base(l) + m * 2u + 1u, lane);
+        }
+#pragma unroll
+    for (uint32_t m = 0; m < 4u; m++) {
+        if (m >= tiles) continue;
+        const float *b = params + bias_offset(l) + m * 16u + 4u * g;
+#pragma unroll
+        for (uint32_t t = 0; t < 2u; t++) {
+            float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma32(a[m][0], pair(in[t][0], in[t][1]), acc);
+            acc = mfma32(a[m][1], pair(in[t][2], in[t][3]), acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[r] += b[r];
+            out[t][m] = acc;
+        }
+    }
+}
+
+// One wave a workgroup. Each wave takes pairs of 16-sample tiles (stride:
+// the grid) and writes its gradient sums into its own row, blockIdx.x, of
+// `partials` (kParams + 1 floats: the last the loss): plain stores for its
+// first pair, adds for later ones, so rows need no clearing and no wave waits
+// on another (the grid must not exceed the rows). The weight gradient of a
+// layer takes the pair's 32 samples as the inner dimension of one K = 32
+// product. Pre-activations wait in LDS for the backward pass, keeping the
 // registers to one layer's tiles.
 template <class Source>
-__device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t n, float *partials, uint32_t rows) {
-    __shared__ float xs[16][kIn];          // the encodings
-    __shared__ float zs[3][16][kHidden];   // hidden pre-activations
-    __shared__ float ds[16][kHidden];      // dZ of the layer being differentiated
-    __shared__ float os[16][kOut];         // outputs, then their gradients
+__device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t n, float *partials) {
+    constexpr uint32_t kPair = 32u;
+    __shared__ float xs[kPair][kIn];          // the encodings
+    __shared__ float zs[3][kPair][kHidden];   // hidden pre-activations
+    __shared__ float ds[kPair][kHidden];      // dZ of the layer being differentiated
+    __shared__ float os[kPair][kOut];         // outputs, then their gradients
     const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
-    float *part = partials + (size_t)(blockIdx.x % rows) * (kParams + 1u);
+    float *part = partials + (size_t)blockIdx.x * (kParams + 1u);
     float mean = net.stats[0] / (float)(n > 0 ? n : 1);
     float loss = 0.0f;
-    for (uint32_t base = blockIdx.x * 16u; base < n; base += gridDim.x * 16u) {
-        Sample s = src.sample(base + j < n ? base + j : n - 1u);
-        bool valid = base + j < n && s.valid && mean > 0.0f;
-        float c = valid ? fminf(s.weight / mean, kMaxSampleScale) : 0.0f;
-        half4_t h[4];
-        encode(net.bounds, s.p[0], s.p[1], s.p[2], s.n[0], s.n[1], s.n[2], g, h);
+    bool first = true;
+    for (uint32_t base = blockIdx.x * kPair; base < n; base += gridDim.x * kPair, first = false) {
+        // both tiles' samples first (their loads overlap), then the forward
+        // pass of the pair; lane (j, g) keeps sample j of each tile
+        Sample smp[2];
 #pragma unroll
-        for (uint32_t m = 0; m < 4u; m++)
+        for (uint32_t t = 0; t < 2u; t++) {
+            uint32_t i = base + 16u * t + j;
+            smp[t] = src.sample(i < n ? i : n - 1u);
+        }
+        half4_t h[2][4];
+        float cs[2];
 #pragma unroll
-            for (uint32_t r = 0; r < 4u; r++) xs[j][16u * m + 4u * g + r] = (float)h[m][r];
-        // forward, layer by layer
+        for (uint32_t t = 0; t < 2u; t++) {
+            const Sample &s = smp[t];
+            bool valid = base + 16u * t + j < n && s.valid && mean > 0.0f;
+            cs[t] = valid ? fminf(s.weight / mean, kMaxSampleScale) : 0.0f;
+            encode(net.bounds, s.p[0], s.p[1], s.p[2], s.n[0], s.n[1], s.n[2], g, h[t]);
+#pragma unroll
+            for (uint32_t m = 0; m < 4u; m++)
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; r++) xs[16u * t + j][16u * m + 4u * g + r] = (float)h[t][m][r];
+        }
 #pragma unroll 1
         for (uint32_t l = 0; l < 3u; l++) {
-            float4_t z[4];
-            layer_forward(net.fwd, net.params, l, h, lane, z);
+            float4_t z[2][4];
+            pair_forward(net.fwd, net.params, l, h, lane, z);
 #pragma unroll
-            for (uint32_t m = 0; m < 4u; m++) {
-                float4_t a = z[m];
+            for (uint32_t t = 0; t < 2u; t++)
 #pragma unroll
-                for (uint32_t r = 0; r < 4u; r++) {
-                    zs[l][j][16u * m + 4u * g + r] = a[r];
-                    a[r] = fmaxf(a[r], 0.0f);
+                for (uint32_t m = 0; m < 4u; m++) {
+                    float4_t a = z[t][m];
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; r++) {
+                        zs[l][16u * t + j][16u * m + 4u * g + r] = a[r];
+                        a[r] = fmaxf(a[r], 0.0f);
+                    }
+                    h[t][m] = to_half(a);
                 }
-                h[m] = to_half(a);
-            }
         }
         {
-            float4_t o[4];
-            layer_forward(net.fwd, net.params, 3u, h, lane, o);
+            float4_t o[2][4];
+            pair_forward(net.fwd, net.params, 3u, h, lane, o);
+#pragma unroll
+            for (uint32_t t = 0; t < 2u; t++)
+#pragma unroll
+                for (uint32_t m = 0; m < 2u; m++)
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; r++) os[16u * t + j][16u * m + 4u * g + r] = o[t][m][r];
+        }
+        __syncthreads();
+        // one lane per sample forms the loss gradient: lane group t takes tile t
+        if (g < 2u) {
+            float d[3] = {g == 0u ? smp[0].d[0] : smp[1].d[0], g == 0u ? smp[0].d[1] : smp[1].d[1],
+                          g == 0u ? smp[0].d[2] : smp[1].d[2]};
+            loss += sample_loss(os[16u * g + j], d, g == 0u ? cs[0] : cs[1]);
+        }
+        __syncthreads();
+        // dZ of the output layer, per tile as D tiles
+        float4_t dz[2][4];
+#pragma unroll
+        for (uint32_t t = 0; t < 2u; t++) {
 #pragma unroll
             for (uint32_t m = 0; m < 2u; m++)
 #pragma unroll
-                for (uint32_t r = 0; r < 4u; r++) os[j][16u * m + 4u * g + r] = o[m][r];
+                for (uint32_t r = 0; r < 4u; r++) dz[t][m][r] = os[16u * t + j][16u * m + 4u * g + r];
+            dz[t][2] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            dz[t][3] = dz[t][2];
         }
-        __syncthreads();
-        // one lane per sample forms the loss gradient
-        if (g == 0u) loss += sample_loss(os[j], s.d, c);
-        __syncthreads();
-        // dZ of the output layer, as tiles (D layout)
-        float4_t dz[4];
-#pragma unroll
-        for (uint32_t m = 0; m < 2u; m++)
-#pragma unroll
-            for (uint32_t r = 0; r < 4u; r++) dz[m][r] = os[j][16u * m + 4u * g + r];
-        dz[2] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
-        dz[3] = dz[2];
 #pragma unroll 1
         for (int l = 3; l >= 0; l--) {
             const uint32_t outs = layer_out((uint32_t)l), steps = outs / 16u;
 #pragma unroll
-            for (uint32_t m = 0; m < 4u; m++)
-                if (m < steps)
+            for (uint32_t t = 0; t < 2u; t++)
 #pragma unroll
-                    for (uint32_t r = 0; r < 4u; r++) ds[j][16u * m + 4u * g + r] = dz[m][r];
+                for (uint32_t m = 0; m < 4u; m++)
+                    if (m < steps)
+#pragma unroll
+                        for (uint32_t r = 0; r < 4u; r++) ds[16u * t + j][16u * m + 4u * g + r] = dz[t][m][r];
             __syncthreads();
-            // dW = H^T dZ: rows (inputs) by tile m, columns (outputs) by tile q,
-            // the 16 samples as the product's inner dimension; H the layer's
-            // input (the encoding, or the ReLU of the previous pre-activation)
+            // dW = H^T dZ: rows (inputs) by tile m, columns (outputs) by tile
+            // q, the pair's 32 samples as K (A lane (j, g) element e: H of
+            // sample 8g + e; B: its dZ); H the layer's input (the encoding,
+            // or the ReLU of the previous pre-activation)
 #pragma unroll 1
             for (uint32_t m = 0; m < 4u; m++) {
-                half4_t a;
+                half8_t a;
 #pragma unroll
-                for (uint32_t r = 0; r < 4u; r++)
-                    a[r] = (_Float16)(l == 0 ? xs[4u * g + r][16u * m + j] : fmaxf(zs[l - 1][4u * g + r][16u * m + j], 0.0f));
+                for (uint32_t e = 0; e < 8u; e++)
+                    a[e] = (_Float16)(l == 0 ? xs[8u * g + e][16u * m + j] : fmaxf(zs[l - 1][8u * g + e][16u * m + j], 0.0f));
                 for (uint32_t q = 0; q < steps; q++) {
-                    half4_t b;
+                    half8_t b;
 #pragma unroll
-                    for (uint32_t r = 0; r < 4u; r++) b[r] = (_Float16)ds[4u * g + r][16u * q + j];
-                    float4_t w = mfma(a, b, float4_t{0.0f, 0.0f, 0.0f, 0.0f});
+                    for (uint32_t e = 0; e < 8u; e++) b[e] = (_Float16)ds[8u * g + e][16u * q + j];
+                    float4_t w = mfma32(a, b, float4_t{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
-                    for (uint32_t r = 0; r < 4u; r++)
-                        atomicAdd(part + weight_offset((uint32_t)l) + (16u * m + 4u * g + r) * outs + 16u * q + j, w[r]);
+                    for (uint32_t r = 0; r < 4u; r++) {
+                        float *at = part + weight_offset((uint32_t)l) + (16u * m + 4u * g + r) * outs + 16u * q + j;
+                        *at = first ? w[r] : *at + w[r];
+                    }
                 }
             }
             // db = sum of dZ over the samples
             if (lane < outs) {
                 float sb = 0.0f;
-                for (uint32_t k = 0; k < 16u; k++) sb += ds[k][lane];
-                atomicAdd(&part[bias_offset((uint32_t)l) + lane], sb);
+                for (uint32_t k = 0; k < kPair; k++) sb += ds[k][lane];
+                float *at = part + bias_offset((uint32_t)l) + lane;
+                *at = first ? sb : *at + sb;
             }
             if (l > 0) {
                 // dH^T = W dZ^T (K = 32 blocks of outputs), then through the
-                // ReLU of layer l - 1
+                // ReLU of layer l - 1, per tile
                 const uint32_t blocks = outs / 32u;
-                const half8_t b0 = pair(to_half(dz[0]), to_half(dz[1])), b1 = pair(to_half(dz[2]), to_half(dz[3]));
+                half8_t b0[2], b1[2];
+#pragma unroll
+                for (uint32_t t = 0; t < 2u; t++) {
+                    b0[t] = pair(to_half(dz[t][0]), to_half(dz[t][1]));
+                    b1[t] = pair(to_half(dz[t][2]), to_half(dz[t][3]));
+                }
 #pragma unroll
                 for (uint32_t m = 0; m < 4u; m++) {
-                    float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                    acc = mfma32(load_block(net.bwd, block_base((uint32_t)l) + m * blocks, lane), b0, acc);
-                    if (blocks > 1u)
-                        acc = mfma32(load_block(net.bwd, block_base((uint32_t)l) + m * blocks + 1u, lane), b1, acc);
+                    const half8_t a0 = load_block(net.bwd, block_base((uint32_t)l) + m * blocks, lane);
+                    const half8_t a1 = blocks > 1u ? load_block(net.bwd, block_base((uint32_t)l) + m * blocks + 1u, lane) : a0;
 #pragma unroll
-                    for (uint32_t r = 0; r < 4u; r++) dz[m][r] = zs[l - 1][j][16u * m + 4u * g + r] > 0.0f ? acc[r] : 0.0f;
+                    for (uint32_t t = 0; t < 2u; t++) {
+                        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                        acc = mfma32(a0, b0[t], acc);
+                        if (blocks > 1u) acc = mfma32(a1, b1[t], acc);
+#pragma unroll
+                        for (uint32_t r = 0; r < 4u; r++)
+                            dz[t][m][r] = zs[l - 1][16u * t + j][16u * m + 4u * g + r] > 0.0f ? acc[r] : 0.0f;
+                    }
                 }
             }
             __syncthreads();
@@ -195,13 +265,12 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
     }
     // the wave's loss
     for (uint32_t off = 32u; off > 0; off >>= 1) loss += __shfl_xor(loss, (int)off);
-    if (lane == 0) atomicAdd(&part[kParams], loss);
+    if (lane == 0 && blockIdx.x * kPair < n) part[kParams] = loss;
 }
 
 template <class Source>
-__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials,
-                                                  uint32_t rows) {
-    grad_body(net, src, n, partials, rows);
+__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials) {
+    grad_body(net, src, n, partials);
 }
 
 // Writes parameter i (f32) and its FP16 fragments.
@@ -215,26 +284,29 @@ __device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w
     net.bwd[bwd_index(l, in, out)] = (_Float16)w;
 }
 
-// One thread a parameter (kParams + 1 threads when clearing): the gradient
-// is the sum of `waves` partial rows
-// over the samples; an Adam step at rate `rate` (0: only refresh the FP16
+// Workgroups of 256 threads, 64 parameters each (kAdamBlocks of them): four
+// threads a parameter sum every fourth of the first `waves` partial rows
+// (those the gradient kernel wrote), then one takes the gradient over the
+// samples and an Adam step at rate `rate` (0: only refresh the FP16
 // fragments from the master copy). Clears the batch's weight sum.
-// With `clear`, the partial rows read are zeroed for the next batch.
-__device__ __forceinline__ void adam_body(NetBuffers &net, float *partials, uint32_t waves, uint32_t samples,
-                                          float rate, bool clear) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (clear && i == kParams)
-        for (uint32_t k = 0; k < waves; k++) partials[(size_t)k * (kParams + 1u) + kParams] = 0.0f;
-    if (i < kParams) {
+constexpr uint32_t kAdamBlocks = (kParams + 63u) / 64u;
+__device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials, uint32_t waves, uint32_t samples,
+                                          float rate) {
+    __shared__ float sums[4][64];
+    const uint32_t local = threadIdx.x % 64u, part = threadIdx.x / 64u;
+    const uint32_t i = blockIdx.x * 64u + local;
+    float grad = 0.0f;
+    if (i < kParams && rate > 0.0f) {
+#pragma unroll 4
+        for (uint32_t k = part; k < waves; k += 4u) grad += partials[(size_t)k * (kParams + 1u) + i];
+    }
+    sums[part][local] = grad;
+    __syncthreads();
+    if (part == 0u && i < kParams) {
         float w = net.params[i];
         if (rate > 0.0f) {
-            float grad = 0.0f;
-            for (uint32_t k = 0; k < waves; k++) {
-                float *at = partials + (size_t)k * (kParams + 1u) + i;
-                grad += *at;
-                if (clear) *at = 0.0f;
-            }
-            grad /= (float)(samples > 0 ? samples : 1);
+            grad = ((sums[0][local] + sums[1][local]) + (sums[2][local] + sums[3][local])) /
+                   (float)(samples > 0 ? samples : 1);
             float t = net.stats[1];
             float m = kAdamBeta1 * net.m[i] + (1.0f - kAdamBeta1) * grad;
             float v = kAdamBeta2 * net.v[i] + (1.0f - kAdamBeta2) * grad * grad;
@@ -245,9 +317,8 @@ __device__ __forceinline__ void adam_body(NetBuffers &net, float *partials, uint
         }
         store_param(net, i, w);
     }
-    if (i == 0) net.stats[0] = 0.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) net.stats[0] = 0.0f;
 }
-
 
 // Mixtures of points, 64 a wave at a time: each lane gathers one point and
 // encodes it into its row of LDS, the matrix cores take the rows 16 at a

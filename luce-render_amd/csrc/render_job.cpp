@@ -433,7 +433,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
         p.guide_mix = (uint4 *)alloc((size_t)2 * slots * 16, rc);
         p.learned = 1;
         p.train_samples = st.train_samples ? st.train_samples : 8u;
-        p.train_batch = 8192u;
+        p.train_batch = kTrainBatch;
         p.train_rate = 5e-3f;
         train_steps = st.train_steps ? st.train_steps : 16u;
     }

@@ -7,10 +7,11 @@
 
 namespace lr {
 
-// Waves of a training step's gradient kernel, and the partial rows they add
-// into (wave w into row w % kNetRows).
-constexpr uint32_t kNetWaves = 256u;
-constexpr uint32_t kNetRows = 32u;
+// Records a training step draws, the waves of its gradient kernel (32
+// records each, so every wave writes its partial row) and those rows.
+constexpr uint32_t kTrainBatch = 8192u;
+constexpr uint32_t kNetWaves = kTrainBatch / 32u;
+constexpr uint32_t kNetRows = kNetWaves;
 
 // Mixtures for this bounce's surface queue of the render at p[0] (`band`:
 // its camera paths, an upper bound of the queue).

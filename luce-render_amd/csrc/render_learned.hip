@@ -104,14 +104,14 @@ template <bool SPECTRAL> __global__ __launch_bounds__(64) void learned_grad_kern
     if (!training(p)) return;
     net::NetBuffers n = p.net;
     PathRecords<SPECTRAL> src{p, salt(p, step)};
-    net::grad_body(n, src, p.train_batch, p.net_partials, kNetRows);
+    net::grad_body(n, src, p.train_batch, p.net_partials);
 }
 
 __global__ __launch_bounds__(256) void learned_adam_kernel(const Params *plist) {
     const Params &p = plist[0];
     if (!training(p)) return;
     net::NetBuffers n = p.net;
-    net::adam_body(n, p.net_partials, kNetRows, p.train_batch, p.train_rate, true);
+    net::adam_body(n, p.net_partials, kNetWaves, p.train_batch, p.train_rate);
 }
 
 }  // namespace rt
@@ -130,7 +130,7 @@ void launch_learned_train(const rt::Params *p, bool spectral, uint32_t steps, hi
             hipLaunchKernelGGL(rt::learned_weights_kernel<false>, dim3(32), dim3(256), 0, s, p, step);
             hipLaunchKernelGGL(rt::learned_grad_kernel<false>, dim3(kNetWaves), dim3(64), 0, s, p, step);
         }
-        hipLaunchKernelGGL(rt::learned_adam_kernel, dim3((net::kParams + 1 + 255) / 256), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(rt::learned_adam_kernel, dim3(net::kAdamBlocks), dim3(256), 0, s, p);
     }
 }
 
