@@ -155,11 +155,6 @@ __device__ __forceinline__ void encode_row(const float *bounds, float px, float 
     for (uint32_t f = 0; f < 16u; f++) row[48u + f] = (_Float16)(f < 8u ? blob(ou, f, kNormalBins) : blob(ov, f - 8u, kNormalBins));
 }
 
-// Activations of one layer for 16 points: 4 tiles of D (= next B), FP32.
-struct Act {
-    float4_t t[4];
-};
-
 __device__ __forceinline__ half4_t to_half(float4_t x) {
     half4_t h;
 #pragma unroll
@@ -183,29 +178,6 @@ __device__ __forceinline__ void layer_forward(const _Float16 *fwd, const float *
         for (int r = 0; r < 4; r++) acc[r] += b[r];
         out[m] = acc;
     }
-}
-
-// The forward pass of 16 points from their input fragments: every layer's
-// pre-activations (z[0..2] hidden, out: the 32 outputs in 2 tiles).
-__device__ __forceinline__ void forward(const _Float16 *fwd, const float *params, const half4_t x[4], uint32_t lane,
-                                        Act z[3], float4_t out[2]) {
-    half4_t h[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) h[k] = x[k];
-    for (uint32_t l = 0; l < 3u; l++) {
-        layer_forward(fwd, params, l, h, lane, z[l].t);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float4_t a = z[l].t[k];
-#pragma unroll
-            for (int r = 0; r < 4; r++) a[r] = fmaxf(a[r], 0.0f);
-            h[k] = to_half(a);
-        }
-    }
-    float4_t o[4];
-    layer_forward(fwd, params, 3u, h, lane, o);
-    out[0] = o[0];
-    out[1] = o[1];
 }
 
 // ------------------------------------------------------------ the mixture

@@ -337,18 +337,55 @@ template <class Points> __device__ __forceinline__ void infer_body(const NetBuff
         pts.point(base + lane < n ? base + lane : n - 1u, p, nn);
         encode_row(net.bounds, p[0], p[1], p[2], nn[0], nn[1], nn[2], own);
         __syncthreads();
-#pragma unroll 1
+        // The four 16-point tiles' input fragments, then the layers one at a
+        // time over all four, each weight block loaded once for them
+        // (layer_forward's products and ReLU, tile by tile).
+        half4_t h[4][4];
+#pragma unroll
         for (uint32_t q = 0; q < 4u; q++) {
             const _Float16 *row = reinterpret_cast<const _Float16 *>(rows + (16u * q + j) * kRowWords);
-            half4_t x[4];
 #pragma unroll
-            for (uint32_t kk = 0; kk < 4u; kk++) x[kk] = *reinterpret_cast<const half4_t *>(row + 16u * kk + 4u * g);
-            Act z[3];
-            float4_t o[2];
-            forward(net.fwd, net.params, x, lane, z, o);
-            float *out = reinterpret_cast<float *>(rows + (16u * q + j) * kRowWords);
+            for (uint32_t kk = 0; kk < 4u; kk++) h[q][kk] = *reinterpret_cast<const half4_t *>(row + 16u * kk + 4u * g);
+        }
+#pragma unroll 1
+        for (uint32_t l = 0; l < 3u; l++) {
+            half4_t next[4][4];
 #pragma unroll
-            for (uint32_t m = 0; m < 2u; m++) *reinterpret_cast<float4_t *>(out + 16u * m + 4u * g) = o[m];
+            for (uint32_t m = 0; m < 4u; m++) {
+                const half8_t a0 = load_block(net.fwd, block_base(l) + m * 2u, lane);
+                const half8_t a1 = load_block(net.fwd, block_base(l) + m * 2u + 1u, lane);
+                const float *b = net.params + bias_offset(l) + m * 16u + 4u * g;
+#pragma unroll
+                for (uint32_t q = 0; q < 4u; q++) {
+                    float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                    acc = mfma32(a0, pair(h[q][0], h[q][1]), acc);
+                    acc = mfma32(a1, pair(h[q][2], h[q][3]), acc);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[r] = fmaxf(acc[r] + b[r], 0.0f);
+                    next[q][m] = to_half(acc);
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; q++)
+#pragma unroll
+                for (uint32_t m = 0; m < 4u; m++) h[q][m] = next[q][m];
+        }
+        // the output layer's two tiles, into the rows (the inputs are all read)
+#pragma unroll
+        for (uint32_t m = 0; m < 2u; m++) {
+            const half8_t a0 = load_block(net.fwd, block_base(3u) + m * 2u, lane);
+            const half8_t a1 = load_block(net.fwd, block_base(3u) + m * 2u + 1u, lane);
+            const float *b = net.params + bias_offset(3u) + m * 16u + 4u * g;
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; q++) {
+                float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc = mfma32(a0, pair(h[q][0], h[q][1]), acc);
+                acc = mfma32(a1, pair(h[q][2], h[q][3]), acc);
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[r] += b[r];
+                float *out = reinterpret_cast<float *>(rows + (16u * q + j) * kRowWords);
+                *reinterpret_cast<float4_t *>(out + 16u * m + 4u * g) = acc;
+            }
         }
         __syncthreads();
         if (base + lane < n) pts.store(base + lane, mixture(reinterpret_cast<const float *>(rows + lane * kRowWords)));
