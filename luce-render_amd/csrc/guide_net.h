@@ -50,6 +50,12 @@ __host__ __device__ constexpr uint32_t weight_offset(uint32_t l) { return l * kI
 __host__ __device__ constexpr uint32_t bias_offset(uint32_t l) { return kBias + l * kHidden; }
 __host__ __device__ constexpr uint32_t block_base(uint32_t l) { return l * 8u; }
 
+// The stats words: the Adam step at [1], and the current batch's sample
+// weights summed per block of the weight-sum kernel (at most kWeightBlocks
+// blocks, their count at [2]), added in block order where used, so a
+// training run gives the same network every time.
+constexpr uint32_t kWeightSums = 16u, kWeightBlocks = 64u, kStatsWords = kWeightSums + kWeightBlocks;
+
 // The network's device memory: master parameters, Adam moments, FP16
 // fragments for the forward and backward products, bounds and step state.
 struct NetBuffers {
@@ -58,7 +64,7 @@ struct NetBuffers {
     _Float16 *fwd;       // kBlocks * 64 lanes * 8
     _Float16 *bwd;       // kBlocks * 64 lanes * 8
     float *bounds;       // lo xyz, 1 / extent xyz
-    float *stats;        // [0] sum of sample weights of the current batch, [1] Adam step
+    float *stats;        // [1] Adam step, [2] blocks of the weight sum, [16 + b] block b's sum (kStatsWords)
 };
 
 // Lane group and element of position p (0..31) of a K = 32 block: two D

@@ -83,7 +83,7 @@ int GuideNet::create(lr_device *dev, const float lo[3], const float hi[3], uint3
     b.fwd = (_Float16 *)alloc(kFragmentHalves * 2);
     b.bwd = (_Float16 *)alloc(kFragmentHalves * 2);
     b.bounds = (float *)alloc(64);
-    b.stats = (float *)alloc(64);
+    b.stats = (float *)alloc(kStatsWords * 4);
     partials = (float *)alloc((size_t)waves * (kParams + 1) * 4);
     if (!b.params || !b.m || !b.v || !b.fwd || !b.bwd || !b.bounds || !b.stats || !partials)
         return fail(LR_DEVICE_ERROR, "hipMalloc(learned guide)");
@@ -105,10 +105,10 @@ int GuideNet::create(lr_device *dev, const float lo[3], const float hi[3], uint3
         bounds[k] = lo[k];
         bounds[3 + k] = extent > 0.0f ? 1.0f / extent : 1.0f;
     }
-    float stats[16] = {};
+    float stats[kStatsWords] = {};
     hipStream_t s = dev->stream;
     hipError_t e = hipMemcpyAsync(b.bounds, bounds, 64, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(b.stats, stats, 64, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(b.stats, stats, sizeof(stats), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "learned guide set-up");
     hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, s, partials, (size_t)waves * (kParams + 1));
     return set_params(init.data());
@@ -188,7 +188,7 @@ static int batch(lr_guide_net *net, const float *samples, size_t n, float rate, 
     uint32_t rows = (uint32_t)std::min<size_t>(g.waves, (n + 31) / 32);
     ArraySamples src{(const float *)dev};
     if (e == hipSuccess) {
-        hipLaunchKernelGGL((weights_kernel<ArraySamples>), dim3(64), dim3(256), 0, s, g.b, src, (uint32_t)n);
+        hipLaunchKernelGGL((weights_kernel<ArraySamples>), dim3(kWeightBlocks), dim3(256), 0, s, g.b, src, (uint32_t)n);
         hipLaunchKernelGGL((grad_kernel<ArraySamples>), dim3(rows), dim3(64), 0, s, g.b, src, (uint32_t)n, g.partials);
         e = hipGetLastError();
     }

@@ -53,8 +53,9 @@ __device__ inline float sample_loss(float *io, const float *d, float c) {
     return -c * logf(pdf);
 }
 
-// Workgroup-wide (256 threads): adds the valid samples' weights into
-// stats[0] and counts the Adam step in stats[1].
+// Workgroup-wide (256 threads, at most kWeightBlocks workgroups): stores the
+// sum of its valid samples' weights in its stats word (the block count at
+// stats[2]) and counts the Adam step in stats[1].
 template <class Source> __device__ __forceinline__ void weights_body(NetBuffers &net, Source &src, uint32_t n) {
     __shared__ float part[256];
     float sum = 0.0f;
@@ -69,9 +70,20 @@ template <class Source> __device__ __forceinline__ void weights_body(NetBuffers 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        atomicAdd(&net.stats[0], part[0]);
-        if (blockIdx.x == 0) atomicAdd(&net.stats[1], 1.0f);
+        net.stats[kWeightSums + blockIdx.x] = part[0];
+        if (blockIdx.x == 0) {
+            net.stats[2] = (float)gridDim.x;
+            net.stats[1] += 1.0f;
+        }
     }
+}
+
+// The batch's weight sum: the weight-sum kernel's block sums in block order.
+__device__ __forceinline__ float batch_weight(const NetBuffers &net) {
+    const uint32_t blocks = (uint32_t)net.stats[2];
+    float total = 0.0f;
+    for (uint32_t b = 0; b < blocks; b++) total += net.stats[kWeightSums + b];
+    return total;
 }
 
 template <class Source>
@@ -124,7 +136,7 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
     __shared__ float os[kPair][kOut];         // outputs, then their gradients
     const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
     float *part = partials + (size_t)blockIdx.x * (kParams + 1u);
-    float mean = net.stats[0] / (float)(n > 0 ? n : 1);
+    float mean = batch_weight(net) / (float)(n > 0 ? n : 1);
     float loss = 0.0f;
     bool first = true;
     for (uint32_t base = blockIdx.x * kPair; base < n; base += gridDim.x * kPair, first = false) {
@@ -288,7 +300,7 @@ __device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w
 // threads a parameter sum every fourth of the first `waves` partial rows
 // (those the gradient kernel wrote), then one takes the gradient over the
 // samples and an Adam step at rate `rate` (0: only refresh the FP16
-// fragments from the master copy). Clears the batch's weight sum.
+// fragments from the master copy).
 constexpr uint32_t kAdamBlocks = (kParams + 63u) / 64u;
 __device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials, uint32_t waves, uint32_t samples,
                                           float rate) {
@@ -317,7 +329,6 @@ __device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials
         }
         store_param(net, i, w);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) net.stats[0] = 0.0f;
 }
 
 // Mixtures of points, 64 a wave at a time: each lane gathers one point and
