@@ -448,6 +448,11 @@ int lr_guide_net_get_params(lr_guide_net *net, float *params, size_t count);
 int lr_guide_net_gradient(lr_guide_net *net, const float *samples, size_t n, float *grad, size_t count, float *loss);
 /* One Adam step (beta 0.9 / 0.99) on the batch at learning rate `rate`. */
 int lr_guide_net_step(lr_guide_net *net, const float *samples, size_t n, float rate, float *loss);
+/* Mixtures of the points. After steps they come from the moving average of
+ * the trained parameters (weight 0.8 on the average so far, bias-corrected as
+ * Adam's moments are), which the renderer's guide uses too; after
+ * lr_guide_net_set_params, from the parameters set. lr_guide_net_get_params
+ * returns the trained parameters. */
 int lr_guide_net_mixtures(lr_guide_net *net, const float *points, size_t n, float *out);
 
 #ifdef __cplusplus

@@ -59,10 +59,13 @@ constexpr uint32_t kWeightSums = 16u, kWeightBlocks = 64u, kStatsWords = kWeight
 // The network's device memory: master parameters, Adam moments, FP16
 // fragments for the forward and backward products, bounds and step state.
 struct NetBuffers {
-    float *params;       // kParams
+    float *params;       // kParams (trained)
     float *m, *v;        // Adam moments
     _Float16 *fwd;       // kBlocks * 64 lanes * 8
     _Float16 *bwd;       // kBlocks * 64 lanes * 8
+    float *ema;          // kParams: exponential moving average of the trained parameters (not bias-corrected)
+    float *infer;        // kParams: the parameters inference uses (the corrected average)
+    _Float16 *infer_fwd; // their forward fragments
     float *bounds;       // lo xyz, 1 / extent xyz
     float *stats;        // [1] Adam step, [2] blocks of the weight sum, [16 + b] block b's sum (kStatsWords)
 };

@@ -82,10 +82,14 @@ int GuideNet::create(lr_device *dev, const float lo[3], const float hi[3], uint3
     b.v = (float *)alloc(kParams * 4);
     b.fwd = (_Float16 *)alloc(kFragmentHalves * 2);
     b.bwd = (_Float16 *)alloc(kFragmentHalves * 2);
+    b.ema = (float *)alloc(kParams * 4);
+    b.infer = (float *)alloc(kParams * 4);
+    b.infer_fwd = (_Float16 *)alloc(kFragmentHalves * 2);
     b.bounds = (float *)alloc(64);
     b.stats = (float *)alloc(kStatsWords * 4);
     partials = (float *)alloc((size_t)waves * (kParams + 1) * 4);
-    if (!b.params || !b.m || !b.v || !b.fwd || !b.bwd || !b.bounds || !b.stats || !partials)
+    if (!b.params || !b.m || !b.v || !b.fwd || !b.bwd || !b.ema || !b.infer || !b.infer_fwd || !b.bounds || !b.stats ||
+        !partials)
         return fail(LR_DEVICE_ERROR, "hipMalloc(learned guide)");
     // He-uniform initial weights from a fixed integer hash; zero biases.
     std::vector<float> init(kParams, 0.0f);

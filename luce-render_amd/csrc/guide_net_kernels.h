@@ -20,6 +20,10 @@ struct Sample {
 
 constexpr float kMaxSampleScale = 1024.0f;  // per-sample weight over the batch mean, at most
 constexpr float kAdamBeta1 = 0.9f, kAdamBeta2 = 0.99f, kAdamEps = 1e-8f;
+// Inference reads an exponential moving average of the trained parameters
+// (weight kEmaBeta on the average so far, bias-corrected like Adam's
+// moments): one step's noise does not set the guide.
+constexpr float kEmaBeta = 0.8f;
 
 // Loss of one sample, -c log q(d); the 32 outputs in `io` are replaced by
 // the loss's gradient by them (the 12 unused get 0).
@@ -285,22 +289,26 @@ __global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, ui
     grad_body(net, src, n, partials);
 }
 
-// Writes parameter i (f32) and its FP16 fragments.
-__device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w) {
+// Writes parameter i (f32) and its FP16 fragments: `w` the trained value,
+// `used` the one inference uses.
+__device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w, float used) {
     net.params[i] = w;
+    net.infer[i] = used;
     if (i >= kBias) return;
     uint32_t l = i < kW3 ? i / (kIn * kHidden) : 3u;
     uint32_t local = i - weight_offset(l), outs = layer_out(l);
     uint32_t in = local / outs, out = local % outs;
     net.fwd[fwd_index(l, in, out)] = (_Float16)w;
     net.bwd[bwd_index(l, in, out)] = (_Float16)w;
+    net.infer_fwd[fwd_index(l, in, out)] = (_Float16)used;
 }
 
 // Workgroups of 256 threads, 64 parameters each (kAdamBlocks of them): four
 // threads a parameter sum every fourth of the first `waves` partial rows
 // (those the gradient kernel wrote), then one takes the gradient over the
-// samples and an Adam step at rate `rate` (0: only refresh the FP16
-// fragments from the master copy).
+// samples and an Adam step at rate `rate`, then the moving average and the
+// inference copy; rate 0 only refreshes the fragments from the master copy,
+// which inference then uses as it is.
 constexpr uint32_t kAdamBlocks = (kParams + 63u) / 64u;
 __device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials, uint32_t waves, uint32_t samples,
                                           float rate) {
@@ -316,18 +324,25 @@ __device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials
     __syncthreads();
     if (part == 0u && i < kParams) {
         float w = net.params[i];
+        float t = net.stats[1];
+        float used = w;
         if (rate > 0.0f) {
             grad = ((sums[0][local] + sums[1][local]) + (sums[2][local] + sums[3][local])) /
                    (float)(samples > 0 ? samples : 1);
-            float t = net.stats[1];
             float m = kAdamBeta1 * net.m[i] + (1.0f - kAdamBeta1) * grad;
             float v = kAdamBeta2 * net.v[i] + (1.0f - kAdamBeta2) * grad * grad;
             net.m[i] = m;
             net.v[i] = v;
             float mh = m / (1.0f - powf(kAdamBeta1, t)), vh = v / (1.0f - powf(kAdamBeta2, t));
             w -= rate * mh / (sqrtf(vh) + kAdamEps);
+            float e = kEmaBeta * net.ema[i] + (1.0f - kEmaBeta) * w;
+            net.ema[i] = e;
+            used = e / (1.0f - powf(kEmaBeta, t));
+        } else {
+            // the average restarts from w: its corrected value is w now
+            net.ema[i] = w * (1.0f - powf(kEmaBeta, t));
         }
-        store_param(net, i, w);
+        store_param(net, i, w, used);
     }
 }
 
@@ -363,9 +378,9 @@ template <class Points> __device__ __forceinline__ void infer_body(const NetBuff
             half4_t next[4][4];
 #pragma unroll
             for (uint32_t m = 0; m < 4u; m++) {
-                const half8_t a0 = load_block(net.fwd, block_base(l) + m * 2u, lane);
-                const half8_t a1 = load_block(net.fwd, block_base(l) + m * 2u + 1u, lane);
-                const float *b = net.params + bias_offset(l) + m * 16u + 4u * g;
+                const half8_t a0 = load_block(net.infer_fwd, block_base(l) + m * 2u, lane);
+                const half8_t a1 = load_block(net.infer_fwd, block_base(l) + m * 2u + 1u, lane);
+                const float *b = net.infer + bias_offset(l) + m * 16u + 4u * g;
 #pragma unroll
                 for (uint32_t q = 0; q < 4u; q++) {
                     float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -384,9 +399,9 @@ template <class Points> __device__ __forceinline__ void infer_body(const NetBuff
         // the output layer's two tiles, into the rows (the inputs are all read)
 #pragma unroll
         for (uint32_t m = 0; m < 2u; m++) {
-            const half8_t a0 = load_block(net.fwd, block_base(3u) + m * 2u, lane);
-            const half8_t a1 = load_block(net.fwd, block_base(3u) + m * 2u + 1u, lane);
-            const float *b = net.params + bias_offset(3u) + m * 16u + 4u * g;
+            const half8_t a0 = load_block(net.infer_fwd, block_base(3u) + m * 2u, lane);
+            const half8_t a1 = load_block(net.infer_fwd, block_base(3u) + m * 2u + 1u, lane);
+            const float *b = net.infer + bias_offset(3u) + m * 16u + 4u * g;
 #pragma unroll
             for (uint32_t q = 0; q < 4u; q++) {
                 float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
