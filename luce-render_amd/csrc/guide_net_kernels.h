@@ -303,22 +303,23 @@ __device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w
     net.infer_fwd[fwd_index(l, in, out)] = (_Float16)used;
 }
 
-// Workgroups of 256 threads, 64 parameters each (kAdamBlocks of them): four
-// threads a parameter sum every fourth of the first `waves` partial rows
+// Workgroups of 256 threads, 32 parameters each (kAdamBlocks of them): eight
+// threads a parameter sum every eighth of the first `waves` partial rows
 // (those the gradient kernel wrote), then one takes the gradient over the
 // samples and an Adam step at rate `rate`, then the moving average and the
 // inference copy; rate 0 only refreshes the fragments from the master copy,
 // which inference then uses as it is.
-constexpr uint32_t kAdamBlocks = (kParams + 63u) / 64u;
+constexpr uint32_t kAdamParts = 8u, kAdamParams = 256u / kAdamParts;
+constexpr uint32_t kAdamBlocks = (kParams + kAdamParams - 1u) / kAdamParams;
 __device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials, uint32_t waves, uint32_t samples,
                                           float rate) {
-    __shared__ float sums[4][64];
-    const uint32_t local = threadIdx.x % 64u, part = threadIdx.x / 64u;
-    const uint32_t i = blockIdx.x * 64u + local;
+    __shared__ float sums[kAdamParts][kAdamParams];
+    const uint32_t local = threadIdx.x % kAdamParams, part = threadIdx.x / kAdamParams;
+    const uint32_t i = blockIdx.x * kAdamParams + local;
     float grad = 0.0f;
     if (i < kParams && rate > 0.0f) {
 #pragma unroll 4
-        for (uint32_t k = part; k < waves; k += 4u) grad += partials[(size_t)k * (kParams + 1u) + i];
+        for (uint32_t k = part; k < waves; k += kAdamParts) grad += partials[(size_t)k * (kParams + 1u) + i];
     }
     sums[part][local] = grad;
     __syncthreads();
@@ -327,8 +328,10 @@ __device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials
         float t = net.stats[1];
         float used = w;
         if (rate > 0.0f) {
-            grad = ((sums[0][local] + sums[1][local]) + (sums[2][local] + sums[3][local])) /
-                   (float)(samples > 0 ? samples : 1);
+            grad = 0.0f;
+#pragma unroll
+            for (uint32_t k = 0; k < kAdamParts; k++) grad += sums[k][local];
+            grad /= (float)(samples > 0 ? samples : 1);
             float m = kAdamBeta1 * net.m[i] + (1.0f - kAdamBeta1) * grad;
             float v = kAdamBeta2 * net.v[i] + (1.0f - kAdamBeta2) * grad * grad;
             net.m[i] = m;

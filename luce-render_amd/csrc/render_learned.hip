@@ -124,10 +124,10 @@ void launch_learned_infer(const rt::Params *p, uint32_t band, hipStream_t s) {
 void launch_learned_train(const rt::Params *p, bool spectral, uint32_t steps, hipStream_t s) {
     for (uint32_t step = 0; step < steps; step++) {
         if (spectral) {
-            hipLaunchKernelGGL(rt::learned_weights_kernel<true>, dim3(32), dim3(256), 0, s, p, step);
+            hipLaunchKernelGGL(rt::learned_weights_kernel<true>, dim3(net::kWeightBlocks), dim3(256), 0, s, p, step);
             hipLaunchKernelGGL(rt::learned_grad_kernel<true>, dim3(kNetWaves), dim3(64), 0, s, p, step);
         } else {
-            hipLaunchKernelGGL(rt::learned_weights_kernel<false>, dim3(32), dim3(256), 0, s, p, step);
+            hipLaunchKernelGGL(rt::learned_weights_kernel<false>, dim3(net::kWeightBlocks), dim3(256), 0, s, p, step);
             hipLaunchKernelGGL(rt::learned_grad_kernel<false>, dim3(kNetWaves), dim3(64), 0, s, p, step);
         }
         hipLaunchKernelGGL(rt::learned_adam_kernel, dim3(net::kAdamBlocks), dim3(256), 0, s, p);
