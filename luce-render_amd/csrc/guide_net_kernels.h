@@ -350,12 +350,12 @@ __device__ __forceinline__ void adam_body(NetBuffers &net, const float *partials
 }
 
 // Mixtures of points, 64 a wave at a time: each lane gathers one point and
-// encodes it into its row of LDS, the matrix cores take the rows 16 at a
-// time (four forward passes), each pass's outputs replace its rows' encodings
-// (a row holds 64 halves, then 32 floats: one wave owns the buffer and its
-// LDS operations complete in order), and each lane forms its own point's
-// mixture. `dst.point(i)` gives point i's position and normal,
-// `dst.store(i, mixture)` takes the result.
+// encodes it into its row of LDS, the rows are read as four 16-point tiles
+// that go through the layers together, the outputs replace the rows'
+// encodings (a row holds 64 halves, then 32 floats: one wave owns the buffer
+// and its LDS operations complete in order), and each lane forms its own
+// point's mixture from the moving-average parameters. `pts.point(i)` gives
+// point i's position and normal, `pts.store(i, mixture)` takes the result.
 constexpr uint32_t kRowWords = 36;  // 32 words a row and 4 of padding: rows 16-byte aligned, 16 rows read at one column 2 to a bank
 template <class Points> __device__ __forceinline__ void infer_body(const NetBuffers &net, Points &pts, uint32_t n) {
     __shared__ uint32_t rows[64 * kRowWords];
