@@ -1,4 +1,4 @@
-// Kernels of the learned guide (guide_net.h): inference of 16 points per
+// Kernels of the learned guide (guide_net.h): inference of 64 points per
 // wave, and training steps (weight sum, gradients by backpropagation on the
 // matrix cores, Adam). Each is templated on where its points or samples come
 // from: host-provided arrays (the library's test entry points) or the
