@@ -86,12 +86,12 @@ struct PassState {
 };
 
 // Launch parameters fixed for a view: pointers, sizes, the frame.
-// A node of the analytic lights' culling tree (pre-order): a box holding
-// its lights (grown outward from their exact bounds), `escape` the node after
-// its subtree, `light` the light of a leaf (kNoneU for an interior node).
+// A node of the analytic lights' culling tree: a box holding its lights
+// (grown outward from their exact bounds) and `light`, the light of a leaf
+// (kNoneU for an interior node or an empty leaf, whose box holds nothing).
 struct LightNode {
     float lo[3];
-    uint32_t escape;
+    uint32_t unused;
     float hi[3];
     uint32_t light;
 };
@@ -139,8 +139,10 @@ struct Params {
     float4 *image;
     float4 *display;
     uint8_t *guide;        // guiding grid (keys, lobes, stats); null without guiding
-    // The analytic lights' culling tree (LightNode, in pre-order) and its
-    // node count; empty when no light can be hit.
+    // The analytic lights' culling tree: a complete binary tree in
+    // breadth-first order from index 1 (node k's children at 2k and 2k + 1,
+    // so two siblings are one 64-byte read; leaves from light_nodes / 2), and
+    // light_nodes, twice its leaves (0 when no light can be hit).
     const LightNode *light_tree;
     uint32_t light_nodes;
     // The distant and dome lights (nothing to hit, found by direction), in

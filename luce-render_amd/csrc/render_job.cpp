@@ -236,7 +236,7 @@ size_t RenderJob::footprint(const lr_gpu_scene *scene, const lr_loaded *loaded, 
     b += loaded->light_floats * 4 + 512 + loaded->material_floats * 4 + loaded->emitter_words * 4;
     b += loaded->program_words * 4 + loaded->texture_floats * 4 + loaded->texture_count * 16;
     uint32_t nlights = (uint32_t)loaded->constants[0];
-    b += (size_t)std::max(1u, 2u * nlights) * sizeof(rt::LightNode);  // the lights' culling tree, as create allocates
+    b += (size_t)std::max(1u, light_tree_nodes(nlights)) * sizeof(rt::LightNode);  // the lights' culling tree, as create allocates
     b += (size_t)std::max(1u, nlights) * 4;                             // and the distant lights' list (at most all)
     b += (size_t)rt::kStateV * s.slots * 16 + (size_t)rt::kStateU * s.slots * 4;
     b += ((size_t)rt::kQueueBase + (size_t)rt::kQueues * s.slots) * 4;
@@ -364,15 +364,15 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
     p.materials = (const float4 *)upload(loaded->materials, loaded->material_floats * 4);
     p.emitters = (const uint32_t *)upload(loaded->emitters, loaded->emitter_words * 4);
     if (rc) return rc;
-    // Bounding spheres of the analytic lights (their count is the first
-    // constant), for the culled light loops.
+    // The analytic lights (their count is the first constant): the culling
+    // tree of those a ray can hit and the list of the distant ones.
     uint32_t nlights = (uint32_t)loaded->constants[0];
     if ((size_t)nlights * 28u > loaded->light_floats) return fail(LR_INVALID, "the light count exceeds the light records");
     analytic_lights = nlights;
     {
         std::vector<rt::LightNode> tree = light_tree(loaded->lights, nlights);
         // room for a tree of every light, so a reload keeps the buffer
-        p.light_tree = (const rt::LightNode *)alloc((size_t)std::max(1u, 2u * nlights) * sizeof(rt::LightNode), rc);
+        p.light_tree = (const rt::LightNode *)alloc((size_t)std::max(1u, light_tree_nodes(nlights)) * sizeof(rt::LightNode), rc);
         if (rc) return rc;
         if (hipError_t e = upload_light_tree(tree, const_cast<rt::LightNode *>(p.light_tree), s))
             return hip_fail(e, "light tree");

@@ -59,9 +59,13 @@ void launch_fill(uint32_t *dst, size_t words, uint32_t value, hipStream_t s);
 // Adds the render at p[0]'s converged pixels into *out.
 void launch_converged(const rt::Params *p, uint32_t pixels, uint32_t *out, hipStream_t s);
 // The culling tree of `n` analytic lights from their host records (28
-// floats each): rt::LightNode in pre-order, 2h - 1 nodes for h lights that
-// can be hit (none for distant and dome lights).
+// floats each): rt::LightNode breadth-first from index 1 (index 0 unused),
+// light_tree_nodes(h) nodes for the h lights that can be hit (none for
+// distant and dome lights).
 std::vector<rt::LightNode> light_tree(const float *lights, uint32_t n);
+// Nodes of the culling tree over h lights: twice the next power of two (0
+// for none); for h at most n, at most light_tree_nodes(n).
+uint32_t light_tree_nodes(uint32_t h);
 // The distant and dome lights of `n` host records, in index order.
 std::vector<uint32_t> far_lights(const float *lights, uint32_t n);
 // Copies a tree to `out` (room for nodes.size()) and waits for the copy.

@@ -950,34 +950,30 @@ std::vector<rt::LightNode> light_tree(const float *lights, uint32_t n) {
         }
     std::sort(items.begin(), items.end(),
               [](const Item &x, const Item &y) { return x.key != y.key ? x.key < y.key : x.light < y.light; });
-    // Halves of the Morton order, emitted in pre-order with escape links.
-    std::vector<rt::LightNode> nodes;
-    nodes.reserve(items.empty() ? 0 : 2 * items.size() - 1);
-    struct Build {
-        std::vector<Item> &items;
-        std::vector<rt::LightNode> &nodes;
-        void operator()(size_t a, size_t b) {
-            size_t at = nodes.size();
-            nodes.push_back(rt::LightNode{});
-            rt::LightNode nd{{INFINITY, INFINITY, INFINITY}, 0u, {-INFINITY, -INFINITY, -INFINITY}, rt::kNoneU};
-            for (size_t k = a; k < b; k++)
-                for (int c = 0; c < 3; c++) {
-                    nd.lo[c] = std::min(nd.lo[c], items[k].lo[c]);
-                    nd.hi[c] = std::max(nd.hi[c], items[k].hi[c]);
-                }
-            if (b - a == 1) {
-                nd.light = items[a].light;
-            } else {
-                size_t mid = (a + b) / 2;
-                (*this)(a, mid);
-                (*this)(mid, b);
-            }
-            nd.escape = (uint32_t)nodes.size();
-            nodes[at] = nd;
+    // The Morton order as the leaves of a complete binary tree (padded to a
+    // power of two with empty leaves), breadth-first from index 1, each
+    // interior box the union of its children's.
+    std::vector<rt::LightNode> nodes(light_tree_nodes((uint32_t)items.size()));
+    const rt::LightNode empty{{INFINITY, INFINITY, INFINITY}, 0u, {-INFINITY, -INFINITY, -INFINITY}, rt::kNoneU};
+    std::fill(nodes.begin(), nodes.end(), empty);
+    size_t leaves = nodes.size() / 2;
+    for (size_t k = 0; k < items.size(); k++) {
+        rt::LightNode &nd = nodes[leaves + k];
+        for (int a = 0; a < 3; a++) nd.lo[a] = items[k].lo[a], nd.hi[a] = items[k].hi[a];
+        nd.light = items[k].light;
+    }
+    for (size_t k = leaves; k-- > 1;)
+        for (int a = 0; a < 3; a++) {
+            nodes[k].lo[a] = std::min(nodes[2 * k].lo[a], nodes[2 * k + 1].lo[a]);
+            nodes[k].hi[a] = std::max(nodes[2 * k].hi[a], nodes[2 * k + 1].hi[a]);
         }
-    };
-    if (!items.empty()) Build{items, nodes}(0, items.size());
     return nodes;
+}
+
+uint32_t light_tree_nodes(uint32_t n) {
+    uint32_t leaves = 1;
+    while (leaves < n) leaves *= 2;
+    return n ? 2u * leaves : 0u;
 }
 
 std::vector<uint32_t> far_lights(const float *lights, uint32_t n) {

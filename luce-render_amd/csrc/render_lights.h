@@ -283,38 +283,58 @@ __device__ __forceinline__ float intersect_light(const Params &p, uint32_t light
 }
 
 // Culling for the analytic lights a ray may meet: a tree over the lights'
-// boxes (built on the host: light_tree, a binary split of their centres'
-// Morton order), walked by each lane for its own ray without a stack: nodes
-// in pre-order, a box the ray misses or a leaf done sends the walk to the
-// node after its subtree. A wave-uniform loop over groups of lights would
-// enter a group when any of 64 incoherent rays may meet it, so every wave
-// would test nearly every light; here a lane tests the lights its own
-// segment passes. Boxes are grown well past rounding, so a light
+// boxes (built on the host: light_tree, their centres' Morton order as the
+// leaves of a complete binary tree), walked by each lane for its own ray. A
+// node whose box the segment enters has its two children's boxes read
+// together (one 64-byte read); the walk goes on into the first child it
+// enters, and a bit per level (`trail`) marks a second child still to visit,
+// found again by shifting the node index (an ancestor of node k at d levels
+// up is k >> d, a left child is even). A wave-uniform loop over groups of
+// lights would enter a group when any of 64 incoherent rays may meet it, so
+// every wave would test nearly every light; here a lane tests the lights its
+// own segment passes. Boxes are grown well past rounding, so a light
 // intersect_light would return is never culled.
 template <class Limit, class Visit>
 __device__ __forceinline__ void walk_lights(const Params &p, V3 o, V3 d, Limit t_max, Visit visit) {
+    const uint32_t count = p.light_nodes;
+    if (count == 0u) return;
     const LightNode *nodes = p.light_tree;
-    uint32_t count = p.light_nodes;
+    const uint32_t leaves = count / 2u;
     auto inverse = [](float x) { return 1.0f / (fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x)); };
     V3 inv{inverse(d.x), inverse(d.y), inverse(d.z)};
-    uint32_t i = 0;
-    while (i < count) {
-        const float4 *at = reinterpret_cast<const float4 *>(nodes + i);
-        float4 lo = at[0], hi = at[1];
+    auto enters = [&](float4 lo, float4 hi) {
         float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
         float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;
         float tz0 = (lo.z - o.z) * inv.z, tz1 = (hi.z - o.z) * inv.z;
         float enter = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         float leave = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-        uint32_t light = __float_as_uint(hi.w);
-        if (enter <= leave && enter <= t_max()) {
-            if (light == kNoneU) {
-                i++;
+        return enter <= leave && enter <= t_max();
+    };
+    {
+        const float4 *root = reinterpret_cast<const float4 *>(nodes + 1);
+        if (!enters(root[0], root[1])) return;
+    }
+    uint32_t k = 1u, depth = 0u, trail = 0u;
+    for (;;) {
+        if (k >= leaves) {
+            uint32_t light = nodes[k].light;
+            if (light != kNoneU && visit(light)) return;
+        } else {
+            const float4 *pair = reinterpret_cast<const float4 *>(nodes + 2u * k);
+            bool left = enters(pair[0], pair[1]), right = enters(pair[2], pair[3]);
+            if (left || right) {
+                if (left && right) trail |= 1u << depth;
+                k = left ? 2u * k : 2u * k + 1u;
+                depth++;
                 continue;
             }
-            if (visit(light)) return;
         }
-        i = __float_as_uint(lo.w);
+        // the deepest second child still to visit, if any
+        if (trail == 0u) return;
+        uint32_t level = 31u - (uint32_t)__builtin_clz(trail);
+        trail &= ~(1u << level);
+        k = (k >> (depth - level - 1u)) | 1u;
+        depth = level + 1u;
     }
 }
 
