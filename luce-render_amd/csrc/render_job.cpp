@@ -530,9 +530,12 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         // anyone reads.
         // With light paths the guide is not consulted at the first vertex, so
         // the first round (every path at its first vertex) infers nothing.
+        // Without them, the first round infers once a pixel: at the first
+        // sample's hits, which every sample's first vertex reads.
         if (learned && b < std::min(rt::GUIDE_RING, jobs[0]->settings.max_bounces) + jobs[0]->through &&
             !(b == 0 && lights > 0)) {
-            launch_learned_infer(params, jobs[0]->params.band_size * jobs[0]->pass_samples, s);
+            if (b == 0) launch_learned_infer_first(params, jobs[0]->params.band_size, s);
+            else launch_learned_infer(params, jobs[0]->params.band_size * jobs[0]->pass_samples, s);
             mark(KRest);
         }
         if (learned) launch_shade_learned(sp, programs, params, n, cur, blocks(128, band), s);

@@ -16,6 +16,9 @@ constexpr uint32_t kNetRows = kNetWaves;
 // Mixtures for this bounce's surface queue of the render at p[0] (`band`:
 // its camera paths, an upper bound of the queue).
 void launch_learned_infer(const rt::Params *p, uint32_t band, hipStream_t s);
+// Mixtures at the first hits of the pass's first sample (`band`: its camera
+// paths), which every sample's first vertex reads (without light paths).
+void launch_learned_infer_first(const rt::Params *p, uint32_t band, hipStream_t s);
 // `steps` Adam steps on the render's path records; the kernels do nothing
 // once the render is past its training samples.
 void launch_learned_train(const rt::Params *p, bool spectral, uint32_t steps, hipStream_t s);

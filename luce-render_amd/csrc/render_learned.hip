@@ -46,6 +46,35 @@ struct SurfacePoints {
     }
 };
 
+// The first sample's camera paths (slots 0 .. band_count - 1) at their first
+// hit: the first vertex of every sample of a pass takes the mixture of its
+// pixel's first sample (its slot modulo band_count), a point within the same
+// pixel. A path whose first hit is no surface gets a placeholder point; its
+// mixture is still a distribution for the pixel's other samples.
+struct FirstSamplePoints {
+    const Params &p;
+    __device__ __forceinline__ void point(uint32_t path, float *x, float *n) const {
+        State st(p);
+        uint32_t what = st.u(SU_HIT, path);
+        x[0] = x[1] = x[2] = 0.0f;
+        n[0] = 0.0f, n[1] = 0.0f, n[2] = 1.0f;
+        if (st.u(SU_PIXEL, path) == kNoneU || what == kNoneU || (what & kLightHit)) return;
+        float4 hit = st.v(SV_HIT, path);
+        V3 o = v3(st.v(SV_ORIGIN, path)), d = v3(st.v(SV_DIRECTION, path));
+        V3 at = o + d * hit.x, p0, e1, e2;
+        triangle_edges(p, what, p0, e1, e2);
+        V3 g = normalize(cross(e1, e2));
+        if (dot(g, d) > 0.0f) g = -g;
+        x[0] = at.x, x[1] = at.y, x[2] = at.z;
+        n[0] = g.x, n[1] = g.y, n[2] = g.z;
+    }
+    __device__ __forceinline__ void store(uint32_t path, const net::Mixture &m) const {
+        V3 mu[GUIDE_LOBES];
+        for (uint32_t k = 0; k < GUIDE_LOBES; k++) mu[k] = V3{m.mu[k][0], m.mu[k][1], m.mu[k][2]};
+        store_learned_mix(p, path, mu, m.kappa, m.w);
+    }
+};
+
 // Training samples: record k of path `path`, both drawn by a hash of the
 // sample index and the step's salt over the band's camera paths.
 template <bool SPECTRAL> struct PathRecords {
@@ -91,6 +120,13 @@ __global__ __launch_bounds__(64) void learned_infer_kernel(const Params *plist) 
     net::infer_body(p.net, pts, p.queues[Q_SURFACE]);
 }
 
+__global__ __launch_bounds__(64) void learned_infer_first_kernel(const Params *plist) {
+    const Params &p = plist[0];
+    if (!net_ready(p)) return;
+    FirstSamplePoints pts{p};
+    net::infer_body(p.net, pts, p.pass->band_count);
+}
+
 template <bool SPECTRAL> __global__ __launch_bounds__(256) void learned_weights_kernel(const Params *plist, uint32_t step) {
     const Params &p = plist[0];
     if (!training(p)) return;
@@ -119,6 +155,11 @@ __global__ __launch_bounds__(256) void learned_adam_kernel(const Params *plist) 
 void launch_learned_infer(const rt::Params *p, uint32_t band, hipStream_t s) {
     uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)band + 63) / 64, 16384);
     hipLaunchKernelGGL(rt::learned_infer_kernel, dim3(blocks ? blocks : 1), dim3(64), 0, s, p);
+}
+
+void launch_learned_infer_first(const rt::Params *p, uint32_t band, hipStream_t s) {
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)band + 63) / 64, 16384);
+    hipLaunchKernelGGL(rt::learned_infer_first_kernel, dim3(blocks ? blocks : 1), dim3(64), 0, s, p);
 }
 
 void launch_learned_train(const rt::Params *p, bool spectral, uint32_t steps, hipStream_t s) {

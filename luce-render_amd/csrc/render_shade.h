@@ -187,10 +187,12 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
         if (suited && ready && (!LOBEWISE || !tracing || total >= 1u))
             guided = K(p, K_GUIDE).z * (p.learned ? surface.p_diffuse : 1.0f);
     }
-    // The learned mixture (32 bytes) is read once for every use below.
+    // The learned mixture (32 bytes) is read once for every use below; at the
+    // first vertex without light paths, the one inferred at the pixel's first
+    // sample of the pass (its slot modulo the band).
     LearnedMix mix{};
     if constexpr (LOBEWISE)
-        if (guided > 0.0f) mix = learned_mix(p, path);
+        if (guided > 0.0f) mix = learned_mix(p, total == 0u && !tracing ? path % ps.band_count : path);
 
     float4 u_light = sample4(p.seed, pixel, sample, group_light(total));
     float p_distant = K(p, K_TREE).z;
