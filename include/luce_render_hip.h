@@ -264,7 +264,7 @@ typedef struct lr_loaded {
  * reference's hash grid of vMF mixtures, 2 the learned guide (an MLP on the
  * matrix cores trained online during the first train_samples samples, 0: 8,
  * in passes of a quarter of them, each followed by train_steps Adam steps for
- * each sample it carried, 0: 16; single renders only). */
+ * each sample it carried, 0: 8; single renders only). */
 typedef struct lr_render_settings {
     uint32_t width;
     uint32_t height;

@@ -134,8 +134,10 @@ __device__ __forceinline__ void pair_forward(const _Float16 *fwd, const float *p
 template <class Source>
 __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t n, float *partials) {
     constexpr uint32_t kPair = 32u;
-    __shared__ float xs[kPair][kIn];          // the encodings
-    __shared__ float zs[3][kPair][kHidden];   // hidden pre-activations
+    // Encodings and hidden pre-activations as halves (the products take them
+    // so; 28 KB of LDS a wave, so five waves' workgroups share a CU).
+    __shared__ _Float16 xs[kPair][kIn];
+    __shared__ _Float16 zs[3][kPair][kHidden];
     __shared__ float ds[kPair][kHidden];      // dZ of the layer being differentiated
     __shared__ float os[kPair][kOut];         // outputs, then their gradients
     const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
@@ -163,7 +165,7 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
 #pragma unroll
             for (uint32_t m = 0; m < 4u; m++)
 #pragma unroll
-                for (uint32_t r = 0; r < 4u; r++) xs[16u * t + j][16u * m + 4u * g + r] = (float)h[t][m][r];
+                for (uint32_t r = 0; r < 4u; r++) xs[16u * t + j][16u * m + 4u * g + r] = h[t][m][r];
         }
 #pragma unroll 1
         for (uint32_t l = 0; l < 3u; l++) {
@@ -176,7 +178,7 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
                     float4_t a = z[t][m];
 #pragma unroll
                     for (uint32_t r = 0; r < 4u; r++) {
-                        zs[l][16u * t + j][16u * m + 4u * g + r] = a[r];
+                        zs[l][16u * t + j][16u * m + 4u * g + r] = (_Float16)a[r];
                         a[r] = fmaxf(a[r], 0.0f);
                     }
                     h[t][m] = to_half(a);
@@ -231,7 +233,9 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
                 half8_t a;
 #pragma unroll
                 for (uint32_t e = 0; e < 8u; e++)
-                    a[e] = (_Float16)(l == 0 ? xs[8u * g + e][16u * m + j] : fmaxf(zs[l - 1][8u * g + e][16u * m + j], 0.0f));
+                    a[e] = l == 0 ? xs[8u * g + e][16u * m + j]
+                                  : (zs[l - 1][8u * g + e][16u * m + j] > (_Float16)0.0f ? zs[l - 1][8u * g + e][16u * m + j]
+                                                                                         : (_Float16)0.0f);
                 for (uint32_t q = 0; q < steps; q++) {
                     half8_t b;
 #pragma unroll
@@ -272,7 +276,7 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
                         if (blocks > 1u) acc = mfma32(a1, b1[t], acc);
 #pragma unroll
                         for (uint32_t r = 0; r < 4u; r++)
-                            dz[t][m][r] = zs[l - 1][16u * t + j][16u * m + 4u * g + r] > 0.0f ? acc[r] : 0.0f;
+                            dz[t][m][r] = zs[l - 1][16u * t + j][16u * m + 4u * g + r] > (_Float16)0.0f ? acc[r] : 0.0f;
                     }
                 }
             }

@@ -435,7 +435,7 @@ int RenderJob::create(lr_device *dev, lr_gpu_scene *sc, const lr_loaded *loaded,
         p.train_samples = st.train_samples ? st.train_samples : 8u;
         p.train_batch = kTrainBatch;
         p.train_rate = 5e-3f;
-        train_steps = st.train_steps ? st.train_steps : 16u;
+        train_steps = st.train_steps ? st.train_steps : 8u;
     }
     params_device = (rt::Params *)alloc(sizeof(rt::Params), rc);
     if (rc) return rc;

@@ -9,7 +9,7 @@ namespace lr {
 
 // Records a training step draws, the waves of its gradient kernel (32
 // records each, so every wave writes its partial row) and those rows.
-constexpr uint32_t kTrainBatch = 16384u;
+constexpr uint32_t kTrainBatch = 32768u;
 constexpr uint32_t kNetWaves = kTrainBatch / 32u;
 constexpr uint32_t kNetRows = kNetWaves;
 

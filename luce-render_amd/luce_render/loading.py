@@ -413,7 +413,7 @@ class RenderSettings:
     profile: bool = False
     guiding: int = 0  # 0 off, 1 the hash grid of vMF mixtures, 2 the learned guide
     train_samples: int = 0  # learned guide: samples that train it (0: 8)
-    train_steps: int = 0  # learned guide: Adam steps for each of them, after their pass (0: 16)
+    train_steps: int = 0  # learned guide: Adam steps for each of them, after their pass (0: 8)
     # (the training samples go in passes of a quarter of them)
     # The guide's share of the one-sample mixture (loading.lucb:281: 0.5): of
     # all BSDF samples for the grid, of the diffuse lobe's for the learned guide.
