@@ -307,8 +307,9 @@ __device__ __forceinline__ void store_param(NetBuffers &net, uint32_t i, float w
     net.infer_fwd[fwd_index(l, in, out)] = (_Float16)used;
 }
 
-// Workgroups of 256 threads, 32 parameters each (kAdamBlocks of them): eight
-// threads a parameter sum every eighth of the first `waves` partial rows
+// Workgroups of 256 threads, kAdamParams parameters each (kAdamBlocks of
+// them): kAdamParts threads a parameter sum every kAdamParts-th of the first
+// `waves` partial rows
 // (those the gradient kernel wrote), then one takes the gradient over the
 // samples and an Adam step at rate `rate`, then the moving average and the
 // inference copy; rate 0 only refreshes the fragments from the master copy,
