@@ -528,11 +528,12 @@ void record_pass(const std::vector<RenderJob *> &jobs, const rt::Params *params,
         // below the bounce limit; a path's vertex count trails its round by
         // at most the cut-out pass-throughs, so later rounds infer nothing
         // anyone reads.
-        // With light paths the guide is not consulted at the first vertex, so
-        // the first round (every path at its first vertex) infers nothing.
-        // Without them, the first round infers once a pixel: at the first
-        // sample's hits, which every sample's first vertex reads.
-        if (learned && b < std::min(rt::GUIDE_RING, jobs[0]->settings.max_bounces) + jobs[0]->through &&
+        // With light paths the guide is consulted at the second vertex alone,
+        // so only the second round (plus the pass-throughs) infers. Without
+        // them, the first round infers once a pixel: at the first sample's
+        // hits, which every sample's first vertex reads.
+        const uint32_t guided_rounds = lights > 0 ? 2u : rt::GUIDE_RING;
+        if (learned && b < std::min(guided_rounds, jobs[0]->settings.max_bounces) + jobs[0]->through &&
             !(b == 0 && lights > 0)) {
             if (b == 0) launch_learned_infer_first(params, jobs[0]->params.band_size, s);
             else launch_learned_infer(params, jobs[0]->params.band_size * jobs[0]->pass_samples, s);

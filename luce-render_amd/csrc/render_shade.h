@@ -181,10 +181,11 @@ __device__ Pushes shade_camera(const Params &p, const PassState &ps, uint32_t pa
         // every BSDF sample (loading.lucb:281).
         // With light tracing, light paths connect to the camera from the
         // visible surfaces and estimate their diffuse light; the learned
-        // guide then starts at the second vertex (guiding the first as well
+        // guide then takes the second vertex alone (guiding the first as well
         // raised the glossy workload's error: 1.17e-3 against 1.01e-3 at 64
-        // samples, 960x540).
-        if (suited && ready && (!LOBEWISE || !tracing || total >= 1u))
+        // samples, 960x540; guiding the third as well left it the same,
+        // 1.088e-3 against 1.085e-3, for 3% more time).
+        if (suited && ready && (!LOBEWISE || !tracing || total == 1u))
             guided = K(p, K_GUIDE).z * (p.learned ? surface.p_diffuse : 1.0f);
     }
     // The learned mixture (32 bytes) is read once for every use below; at the
