@@ -66,6 +66,23 @@ def test_guided_room_is_closer_to_reference(device, guiding):
     assert errors[1] < errors[0]
 
 
+def test_learned_render_is_repeatable(device):
+    """A learned-guide render trains its network during the render; two
+    renders with one seed must give the same image bit for bit (the batch's
+    weight sum is added in a fixed order, and every gradient wave writes its
+    own row: no float atomics decide the network)."""
+    from luce_render import render, scenes
+    from luce_render.loading import OpenPbr, RenderSettings
+    sc = render.compile_scene(scenes.lamp_room_set())
+    st = RenderSettings(192, 144, 24, max_bounces=2, spectral=False, guiding=2, light_tracing=0.0, seed=3,
+                        material=OpenPbr(base_color=(0.3, 0.3, 0.3), specular_weight=0.0))
+    first = render.render(device, sc, st)
+    for _ in range(2):
+        again = render.render(device, sc, st)
+        assert np.array_equal(first, again), float(np.abs(first - again).max())
+    assert np.all(np.isfinite(first)) and first[:, :, :3].mean() > 0.0
+
+
 def _guided_cornell(device, spectral, samples):
     """The Cornell box at 96x96, camera paths, 8 bounces, grid-guided: the
     GPU image and guide grid, and the oracle's (image, grid state)."""
