@@ -284,7 +284,9 @@ def main():
     rays_b = render.Buffer.create(device, rays_b_host)
 
     def step():
+        gpu.set_trace_config(*best_probe)
         render.probe_device(gpu, frame, hits_p, stream)
+        gpu.set_trace_config(*best)
         render.trace_closest_device(gpu, rays_b, n, hits_b, stream)
 
     def time_steps(k):
@@ -298,16 +300,24 @@ def main():
         timer.destroy()
         return ms / k
 
-    # Traversal variant: the autotuner's sweep over BVH width x workgroup x
-    # LDS stack depth on both batches (every variant returns the same hits).
+    # Traversal variants: the autotuner's sweep over BVH width x workgroup x
+    # LDS stack depth x ray schedule, each batch taking its own fastest
+    # (every variant returns the same hits; the configuration is the scene's
+    # setting, set before each call).
     if args.config:
         best = tuple(int(x) for x in args.config.split(","))
         best = best + (0,) * (4 - len(best))
+        best_probe = best
         sweep = None
     else:
         from luce_render import autotune
-        best, sweep = autotune.tune(device, gpu, rays_b, n, hits=hits_b, frame=frame, repeats=2)
+        split = {}
+        _, sweep = autotune.tune(device, gpu, rays_b, n, hits=hits_b, frame=frame, repeats=2, split=split)
+        best = min(split["trace"], key=split["trace"].get)
+        best_probe = min(split["probe"], key=split["probe"].get)
         log("sweep (ms, probe + bounce):", {f"w{k[0]}g{k[1]}s{k[2]}r{k[3]}": round(v, 3) for k, v in sweep.items()})
+        log("picks: bounce", best, round(split["trace"][best], 3), "primary", best_probe,
+            round(split["probe"][best_probe], 3))
     gpu.set_trace_config(*best)
 
     for _ in range(args.warmup):
@@ -317,9 +327,11 @@ def main():
     tp, tb = render.Timer(), render.Timer()
     kp, kb = [], []
     for _ in range(args.steps):
+        gpu.set_trace_config(*best_probe)
         tp.begin(stream)
         render.probe_device(gpu, frame, hits_p, stream)
         tp.end(stream)
+        gpu.set_trace_config(*best)
         tb.begin(stream)
         render.trace_closest_device(gpu, rays_b, n, hits_b, stream)
         tb.end(stream)
@@ -352,11 +364,11 @@ def main():
     # dominant kernel: the bounce batch (incoherent)
     suffix = "_stream_kernel" if best[3] > 0 else "_kernel"
     if ms_b >= ms_p:
-        dom_bytes, dom_ms, dom_name = bytes_b * n, ms_b, "closest" + suffix
+        dom_bytes, dom_ms, dom_name, dom_config = bytes_b * n, ms_b, "closest" + suffix, best
     else:
-        dom_bytes, dom_ms, dom_name = bytes_p * n, ms_p, "probe" + suffix
+        dom_bytes, dom_ms, dom_name, dom_config = bytes_p * n, ms_p, "probe_kernel", best_probe
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = measured_traffic(f"{dom_name}<{best[0]},{best[2]},{best[1]}>")
+    traffic = measured_traffic(f"{dom_name}<{dom_config[0]},{dom_config[2]},{dom_config[1]}>")
 
     # Scene build on the device (lr_scene_build): the flattened arrays in
     # their original order (recovered through the leaf order) to a traceable
@@ -562,6 +574,7 @@ def main():
                         f"{sc.triangle_count()}-triangle scene (1,000 instances of 64 displaced icospheres)",
             "rays_per_step": total_rays,
             "trace_config": {"width": best[0], "group": best[1], "stack": best[2], "refill": best[3]},
+            "trace_config_primary": {"width": best_probe[0], "group": best_probe[1], "stack": best_probe[2]},
             "autotune_ms": None if sweep is None else {",".join(map(str, k)): round(v, 3) for k, v in sweep.items()},
             "primary_mrays_s": round(n / (ms_p * 1e-3) / 1e6, 2),
             "bounce_mrays_s": round(n / (ms_b * 1e-3) / 1e6, 2),

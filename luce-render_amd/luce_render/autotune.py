@@ -17,10 +17,13 @@ REFILLS = (0, 32, 48)  # 0: one ray per thread; k: persistent waves refilled at 
 
 
 def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks=STACKS, refills=REFILLS, repeats=3,
-         frame=None):
+         frame=None, split=None):
     """Times trace_closest_device over `rays` (a Buffer of n rays), and the
     probe of `frame` too when given, for each variant; returns the fastest
-    (width, group, stack, refill) and {variant: ms per launch}."""
+    (width, group, stack, refill) and {variant: ms per launch}. A dict
+    `split` receives the two calls' own tables under "trace" and "probe"
+    (the probe takes one ray per thread whatever the refill), so a caller
+    can give each its fastest variant."""
     stream = render.device_stream(device)
     own = hits is None
     if own:
@@ -35,18 +38,26 @@ def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks
                     for r in refills:
                         scene.set_trace_config(w, g, s, r)
 
-                        def once():
+                        def trace():
                             render.trace_closest_device(scene, rays, n, hits, stream)
-                            if frame is not None:
-                                render.probe_device(scene, frame, probe_hits, stream)
 
-                        once()  # first launch: spill area and code load
-                        render.synchronize(device)
-                        timer.begin(stream)
-                        for _ in range(repeats):
-                            once()
-                        timer.end(stream)
-                        table[(w, g, s, r)] = timer.elapsed_ms() / repeats
+                        def probe():
+                            render.probe_device(scene, frame, probe_hits, stream)
+
+                        ms = {}
+                        for name, call in (("trace", trace), ("probe", probe)):
+                            if name == "probe" and frame is None:
+                                continue
+                            call()  # first launch: spill area and code load
+                            render.synchronize(device)
+                            timer.begin(stream)
+                            for _ in range(repeats):
+                                call()
+                            timer.end(stream)
+                            ms[name] = timer.elapsed_ms() / repeats
+                            if split is not None:
+                                split.setdefault(name, {})[(w, g, s, r)] = ms[name]
+                        table[(w, g, s, r)] = sum(ms.values())
     finally:
         timer.destroy()
         if own:
