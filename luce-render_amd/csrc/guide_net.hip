@@ -187,13 +187,13 @@ static int batch(lr_guide_net *net, const float *samples, size_t n, float rate, 
     hipError_t e = hipMalloc(&dev, n * 40);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(batch)");
     e = hipMemcpyAsync(dev, samples, n * 40, hipMemcpyHostToDevice, s);
-    // one wave (and partial row) per 32 samples, at most the rows allocated;
-    // every wave of the grid takes at least one pair of tiles
+    // one workgroup (and partial row) per 32 samples, at most the rows
+    // allocated; every workgroup of the grid takes at least one pair of tiles
     uint32_t rows = (uint32_t)std::min<size_t>(g.waves, (n + 31) / 32);
     ArraySamples src{(const float *)dev};
     if (e == hipSuccess) {
         hipLaunchKernelGGL((weights_kernel<ArraySamples>), dim3(kWeightBlocks), dim3(256), 0, s, g.b, src, (uint32_t)n);
-        hipLaunchKernelGGL((grad_kernel<ArraySamples>), dim3(rows), dim3(64), 0, s, g.b, src, (uint32_t)n, g.partials);
+        hipLaunchKernelGGL((grad_kernel<ArraySamples>), dim3(rows), dim3(kGradThreads), 0, s, g.b, src, (uint32_t)n, g.partials);
         e = hipGetLastError();
     }
     std::vector<float> part;

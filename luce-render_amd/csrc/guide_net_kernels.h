@@ -95,151 +95,115 @@ __global__ __launch_bounds__(256) void weights_kernel(NetBuffers net, Source src
     weights_body(net, src, n);
 }
 
-// Z^T = W_l^T H^T + b for the two tiles of a pair: layer_forward's products
-// with each weight block loaded once for both tiles.
-__device__ __forceinline__ void pair_forward(const _Float16 *fwd, const float *params, uint32_t l,
-                                             const half4_t (&in)[2][4], uint32_t lane, float4_t (&out)[2][4]) {
-    const uint32_t g = lane / 16u, tiles = layer_out(l) / 16u;
-    half8_t a[4][2];
-#pragma unroll
-    for (uint32_t m = 0; m < 4u; m++)
-        if (m < tiles) {
-            a[m][0] = load_block(fwd, block_base(l) + m * 2u, lane);
-            a[m][1] = load_block(fwd, block_base(l) + m * 2u + 1u, lane);
-        }
-#pragma unroll
-    for (uint32_t m = 0; m < 4u; m++) {
-        if (m >= tiles) continue;
-        const float *b = params + bias_offset(l) + m * 16u + 4u * g;
-#pragma unroll
-        for (uint32_t t = 0; t < 2u; t++) {
-            float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma32(a[m][0], pair(in[t][0], in[t][1]), acc);
-            acc = mfma32(a[m][1], pair(in[t][2], in[t][3]), acc);
-#pragma unroll
-            for (int r = 0; r < 4; r++) acc[r] += b[r];
-            out[t][m] = acc;
-        }
-    }
-}
-
-// One wave a workgroup. Each wave takes pairs of 16-sample tiles (stride:
-// the grid) and writes its gradient sums into its own row, blockIdx.x, of
-// `partials` (kParams + 1 floats: the last the loss): plain stores for its
-// first pair, adds for later ones, so rows need no clearing and no wave waits
-// on another (the grid must not exceed the rows). The weight gradient of a
-// layer takes the pair's 32 samples as the inner dimension of one K = 32
-// product. Pre-activations wait in LDS for the backward pass, keeping the
-// registers to one layer's tiles.
+// One workgroup of four waves a pair of 16-sample tiles (32 samples;
+// stride: the grid), its gradient sums written into its own row, blockIdx.x,
+// of `partials` (kParams + 1 floats: the last the loss): plain stores for its
+// first pair, adds for later ones, so rows need no clearing and no workgroup
+// waits on another (the grid must not exceed the rows). The waves share the
+// pair through LDS and split every layer by 16-row tiles: wave w forms
+// output tile w of a forward product, and input tile w of dW = H^T dZ (the
+// 32 samples as the inner dimension of one K = 32 product) and of
+// dH^T = W dZ^T; a barrier ends each layer. Fragments are built as the
+// single-tile products take them (two D tiles side by side as a K = 32 B
+// operand), so the sums are the same; encodings and hidden pre-activations
+// are kept as the halves the products take.
+constexpr uint32_t kGradThreads = 256u;
 template <class Source>
 __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t n, float *partials) {
     constexpr uint32_t kPair = 32u;
-    // Encodings and hidden pre-activations as halves (the products take them
-    // so; 28 KB of LDS a wave, so five waves' workgroups share a CU).
-    __shared__ _Float16 xs[kPair][kIn];
-    __shared__ _Float16 zs[3][kPair][kHidden];
-    __shared__ float ds[kPair][kHidden];      // dZ of the layer being differentiated
-    __shared__ float os[kPair][kOut];         // outputs, then their gradients
-    const uint32_t lane = threadIdx.x, j = lane % 16u, g = lane / 16u;
+    __shared__ __attribute__((aligned(16))) _Float16 xs[kPair][kIn];
+    __shared__ __attribute__((aligned(16))) _Float16 zs[3][kPair][kHidden];
+    __shared__ __attribute__((aligned(16))) float ds[2][kPair][kHidden];  // dZ of the layer differentiated, then the one below
+    __shared__ float os[kPair][kOut];        // outputs, then their gradients
+    __shared__ float dir[kPair][3];
+    __shared__ float cw[kPair];
+    const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u, j = lane % 16u, g = lane / 16u;
     float *part = partials + (size_t)blockIdx.x * (kParams + 1u);
-    float mean = batch_weight(net) / (float)(n > 0 ? n : 1);
+    const float mean = batch_weight(net) / (float)(n > 0 ? n : 1);
+    // layer l's input k of pair row `row`: the encoding, or the ReLU of the
+    // previous pre-activation
+    auto input = [&](uint32_t l, uint32_t row, uint32_t k) -> _Float16 {
+        if (l == 0u) return xs[row][k];
+        _Float16 z = zs[l - 1u][row][k];
+        return z > (_Float16)0.0f ? z : (_Float16)0.0f;
+    };
     float loss = 0.0f;
     bool first = true;
     for (uint32_t base = blockIdx.x * kPair; base < n; base += gridDim.x * kPair, first = false) {
-        // both tiles' samples first (their loads overlap), then the forward
-        // pass of the pair; lane (j, g) keeps sample j of each tile
-        Sample smp[2];
-#pragma unroll
-        for (uint32_t t = 0; t < 2u; t++) {
-            uint32_t i = base + 16u * t + j;
-            smp[t] = src.sample(i < n ? i : n - 1u);
-        }
-        half4_t h[2][4];
-        float cs[2];
-#pragma unroll
-        for (uint32_t t = 0; t < 2u; t++) {
-            const Sample &s = smp[t];
-            bool valid = base + 16u * t + j < n && s.valid && mean > 0.0f;
-            cs[t] = valid ? fminf(s.weight / mean, kMaxSampleScale) : 0.0f;
-            encode(net.bounds, s.p[0], s.p[1], s.p[2], s.n[0], s.n[1], s.n[2], g, h[t]);
+        // waves 0 and 1 gather and encode tile `wave` (lane (j, g): sample j)
+        if (wave < 2u) {
+            const uint32_t row = 16u * wave + j, i = base + row;
+            Sample s = src.sample(i < n ? i : n - 1u);
+            bool valid = i < n && s.valid && mean > 0.0f;
+            half4_t h[4];
+            encode(net.bounds, s.p[0], s.p[1], s.p[2], s.n[0], s.n[1], s.n[2], g, h);
 #pragma unroll
             for (uint32_t m = 0; m < 4u; m++)
 #pragma unroll
-                for (uint32_t r = 0; r < 4u; r++) xs[16u * t + j][16u * m + 4u * g + r] = h[t][m][r];
+                for (uint32_t r = 0; r < 4u; r++) xs[row][16u * m + 4u * g + r] = h[m][r];
+            if (g == 0u) {
+                dir[row][0] = s.d[0], dir[row][1] = s.d[1], dir[row][2] = s.d[2];
+                cw[row] = valid ? fminf(s.weight / mean, kMaxSampleScale) : 0.0f;
+            }
         }
+        __syncthreads();
+        // forward: wave w forms output tile w of each layer, for both tiles
 #pragma unroll 1
-        for (uint32_t l = 0; l < 3u; l++) {
-            float4_t z[2][4];
-            pair_forward(net.fwd, net.params, l, h, lane, z);
+        for (uint32_t l = 0; l < 4u; l++) {
+            if (wave < layer_out(l) / 16u) {
+                const uint32_t m = wave;
+                const half8_t a0 = load_block(net.fwd, block_base(l) + m * 2u, lane);
+                const half8_t a1 = load_block(net.fwd, block_base(l) + m * 2u + 1u, lane);
+                const float *b = net.params + bias_offset(l) + m * 16u + 4u * g;
 #pragma unroll
-            for (uint32_t t = 0; t < 2u; t++)
+                for (uint32_t t = 0; t < 2u; t++) {
+                    const uint32_t row = 16u * t + j;
+                    // the four inputs 16c + 4g .. +3 of each 16-input tile c, one 8-byte read each
+                    const _Float16 *in = l == 0u ? &xs[row][0] : &zs[l - 1u][row][0];
+                    half4_t c[4];
 #pragma unroll
-                for (uint32_t m = 0; m < 4u; m++) {
-                    float4_t a = z[t][m];
+                    for (uint32_t k = 0; k < 4u; k++) {
+                        c[k] = *reinterpret_cast<const half4_t *>(in + 16u * k + 4u * g);
+                        if (l != 0u)
+#pragma unroll
+                            for (uint32_t r = 0; r < 4u; r++) c[k][r] = c[k][r] > (_Float16)0.0f ? c[k][r] : (_Float16)0.0f;
+                    }
+                    const half8_t b0 = pair(c[0], c[1]), b1 = pair(c[2], c[3]);
+                    float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                    acc = mfma32(a0, b0, acc);
+                    acc = mfma32(a1, b1, acc);
 #pragma unroll
                     for (uint32_t r = 0; r < 4u; r++) {
-                        zs[l][16u * t + j][16u * m + 4u * g + r] = (_Float16)a[r];
-                        a[r] = fmaxf(a[r], 0.0f);
+                        float z = acc[r] + b[r];
+                        if (l < 3u) zs[l][row][16u * m + 4u * g + r] = (_Float16)z;
+                        else os[row][16u * m + 4u * g + r] = z;
                     }
-                    h[t][m] = to_half(a);
                 }
+            }
+            __syncthreads();
         }
-        {
-            float4_t o[2][4];
-            pair_forward(net.fwd, net.params, 3u, h, lane, o);
-#pragma unroll
-            for (uint32_t t = 0; t < 2u; t++)
-#pragma unroll
-                for (uint32_t m = 0; m < 2u; m++)
-#pragma unroll
-                    for (uint32_t r = 0; r < 4u; r++) os[16u * t + j][16u * m + 4u * g + r] = o[t][m][r];
+        // wave 0: one lane per sample forms the loss gradient in its row
+        if (wave == 0u && g < 2u) {
+            const uint32_t row = 16u * g + j;
+            float d[3] = {dir[row][0], dir[row][1], dir[row][2]};
+            loss += sample_loss(os[row], d, cw[row]);
         }
         __syncthreads();
-        // one lane per sample forms the loss gradient: lane group t takes tile t
-        if (g < 2u) {
-            float d[3] = {g == 0u ? smp[0].d[0] : smp[1].d[0], g == 0u ? smp[0].d[1] : smp[1].d[1],
-                          g == 0u ? smp[0].d[2] : smp[1].d[2]};
-            loss += sample_loss(os[16u * g + j], d, g == 0u ? cs[0] : cs[1]);
-        }
+        for (uint32_t k = threadIdx.x; k < kPair * kOut; k += kGradThreads) ds[0][k / kOut][k % kOut] = os[k / kOut][k % kOut];
         __syncthreads();
-        // dZ of the output layer, per tile as D tiles
-        float4_t dz[2][4];
-#pragma unroll
-        for (uint32_t t = 0; t < 2u; t++) {
-#pragma unroll
-            for (uint32_t m = 0; m < 2u; m++)
-#pragma unroll
-                for (uint32_t r = 0; r < 4u; r++) dz[t][m][r] = os[16u * t + j][16u * m + 4u * g + r];
-            dz[t][2] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
-            dz[t][3] = dz[t][2];
-        }
+        uint32_t cur = 0u;
 #pragma unroll 1
         for (int l = 3; l >= 0; l--) {
-            const uint32_t outs = layer_out((uint32_t)l), steps = outs / 16u;
-#pragma unroll
-            for (uint32_t t = 0; t < 2u; t++)
-#pragma unroll
-                for (uint32_t m = 0; m < 4u; m++)
-                    if (m < steps)
-#pragma unroll
-                        for (uint32_t r = 0; r < 4u; r++) ds[16u * t + j][16u * m + 4u * g + r] = dz[t][m][r];
-            __syncthreads();
-            // dW = H^T dZ: rows (inputs) by tile m, columns (outputs) by tile
-            // q, the pair's 32 samples as K (A lane (j, g) element e: H of
-            // sample 8g + e; B: its dZ); H the layer's input (the encoding,
-            // or the ReLU of the previous pre-activation)
-#pragma unroll 1
-            for (uint32_t m = 0; m < 4u; m++) {
+            const uint32_t outs = layer_out((uint32_t)l), steps = outs / 16u, m = wave;
+            float(*dz)[kHidden] = ds[cur];
+            {
                 half8_t a;
 #pragma unroll
-                for (uint32_t e = 0; e < 8u; e++)
-                    a[e] = l == 0 ? xs[8u * g + e][16u * m + j]
-                                  : (zs[l - 1][8u * g + e][16u * m + j] > (_Float16)0.0f ? zs[l - 1][8u * g + e][16u * m + j]
-                                                                                         : (_Float16)0.0f);
+                for (uint32_t e = 0; e < 8u; e++) a[e] = input((uint32_t)l, 8u * g + e, 16u * m + j);
                 for (uint32_t q = 0; q < steps; q++) {
                     half8_t b;
 #pragma unroll
-                    for (uint32_t e = 0; e < 8u; e++) b[e] = (_Float16)ds[8u * g + e][16u * q + j];
+                    for (uint32_t e = 0; e < 8u; e++) b[e] = (_Float16)dz[8u * g + e][16u * q + j];
                     float4_t w = mfma32(a, b, float4_t{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
                     for (uint32_t r = 0; r < 4u; r++) {
@@ -248,48 +212,50 @@ __device__ __forceinline__ void grad_body(NetBuffers &net, Source &src, uint32_t
                     }
                 }
             }
-            // db = sum of dZ over the samples
-            if (lane < outs) {
+            if (wave == 0u && lane < outs) {
                 float sb = 0.0f;
-                for (uint32_t k = 0; k < kPair; k++) sb += ds[k][lane];
+                for (uint32_t k = 0; k < kPair; k++) sb += dz[k][lane];
                 float *at = part + bias_offset((uint32_t)l) + lane;
                 *at = first ? sb : *at + sb;
             }
             if (l > 0) {
-                // dH^T = W dZ^T (K = 32 blocks of outputs), then through the
-                // ReLU of layer l - 1, per tile
                 const uint32_t blocks = outs / 32u;
-                half8_t b0[2], b1[2];
+                const half8_t a0 = load_block(net.bwd, block_base((uint32_t)l) + m * blocks, lane);
+                const half8_t a1 = blocks > 1u ? load_block(net.bwd, block_base((uint32_t)l) + m * blocks + 1u, lane) : a0;
 #pragma unroll
                 for (uint32_t t = 0; t < 2u; t++) {
-                    b0[t] = pair(to_half(dz[t][0]), to_half(dz[t][1]));
-                    b1[t] = pair(to_half(dz[t][2]), to_half(dz[t][3]));
-                }
+                    const uint32_t row = 16u * t + j;
+                    // dZ's outputs 16c + 4g .. +3, one 16-byte read each
+                    half4_t c[4];
 #pragma unroll
-                for (uint32_t m = 0; m < 4u; m++) {
-                    const half8_t a0 = load_block(net.bwd, block_base((uint32_t)l) + m * blocks, lane);
-                    const half8_t a1 = blocks > 1u ? load_block(net.bwd, block_base((uint32_t)l) + m * blocks + 1u, lane) : a0;
-#pragma unroll
-                    for (uint32_t t = 0; t < 2u; t++) {
-                        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                        acc = mfma32(a0, b0[t], acc);
-                        if (blocks > 1u) acc = mfma32(a1, b1[t], acc);
-#pragma unroll
-                        for (uint32_t r = 0; r < 4u; r++)
-                            dz[t][m][r] = zs[l - 1][16u * t + j][16u * m + 4u * g + r] > (_Float16)0.0f ? acc[r] : 0.0f;
+                    for (uint32_t k = 0; k < 4u; k++) {
+                        float4_t v = k < 2u || blocks > 1u ? *reinterpret_cast<const float4_t *>(&dz[row][16u * k + 4u * g])
+                                                           : float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+                        c[k] = to_half(v);
                     }
+                    const half8_t b0 = pair(c[0], c[1]), b1 = pair(c[2], c[3]);
+                    float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                    acc = mfma32(a0, b0, acc);
+                    if (blocks > 1u) acc = mfma32(a1, b1, acc);
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; r++)
+                        ds[cur ^ 1u][row][16u * m + 4u * g + r] =
+                            zs[l - 1][row][16u * m + 4u * g + r] > (_Float16)0.0f ? acc[r] : 0.0f;
                 }
             }
             __syncthreads();
+            cur ^= 1u;
         }
     }
-    // the wave's loss
-    for (uint32_t off = 32u; off > 0; off >>= 1) loss += __shfl_xor(loss, (int)off);
-    if (lane == 0 && blockIdx.x * kPair < n) part[kParams] = loss;
+    // the workgroup's loss: wave 0's lanes
+    if (wave == 0u) {
+        for (uint32_t off = 32u; off > 0; off >>= 1) loss += __shfl_xor(loss, (int)off);
+        if (lane == 0 && blockIdx.x * kPair < n) part[kParams] = loss;
+    }
 }
 
 template <class Source>
-__global__ __launch_bounds__(64) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials) {
+__global__ __launch_bounds__(kGradThreads) void grad_kernel(NetBuffers net, Source src, uint32_t n, float *partials) {
     grad_body(net, src, n, partials);
 }
 

@@ -7,11 +7,11 @@
 
 namespace lr {
 
-// Records a training step draws, the waves of its gradient kernel (32
-// records each, so every wave writes its partial row) and those rows.
+// Records a training step draws, the workgroups of its gradient kernel (32
+// records each, so every workgroup writes its partial row) and those rows.
 constexpr uint32_t kTrainBatch = 32768u;
-constexpr uint32_t kNetWaves = kTrainBatch / 32u;
-constexpr uint32_t kNetRows = kNetWaves;
+constexpr uint32_t kGradGroups = kTrainBatch / 32u;
+constexpr uint32_t kNetRows = kGradGroups;
 
 // Mixtures for this bounce's surface queue of the render at p[0] (`band`:
 // its camera paths, an upper bound of the queue).

@@ -135,7 +135,8 @@ template <bool SPECTRAL> __global__ __launch_bounds__(256) void learned_weights_
     net::weights_body(n, src, p.train_batch);
 }
 
-template <bool SPECTRAL> __global__ __launch_bounds__(64) void learned_grad_kernel(const Params *plist, uint32_t step) {
+template <bool SPECTRAL>
+__global__ __launch_bounds__(net::kGradThreads) void learned_grad_kernel(const Params *plist, uint32_t step) {
     const Params &p = plist[0];
     if (!training(p)) return;
     net::NetBuffers n = p.net;
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256) void learned_adam_kernel(const Params *plist) 
     const Params &p = plist[0];
     if (!training(p)) return;
     net::NetBuffers n = p.net;
-    net::adam_body(n, p.net_partials, kNetWaves, p.train_batch, p.train_rate);
+    net::adam_body(n, p.net_partials, kGradGroups, p.train_batch, p.train_rate);
 }
 
 }  // namespace rt
@@ -166,10 +167,10 @@ void launch_learned_train(const rt::Params *p, bool spectral, uint32_t steps, hi
     for (uint32_t step = 0; step < steps; step++) {
         if (spectral) {
             hipLaunchKernelGGL(rt::learned_weights_kernel<true>, dim3(net::kWeightBlocks), dim3(256), 0, s, p, step);
-            hipLaunchKernelGGL(rt::learned_grad_kernel<true>, dim3(kNetWaves), dim3(64), 0, s, p, step);
+            hipLaunchKernelGGL(rt::learned_grad_kernel<true>, dim3(kGradGroups), dim3(net::kGradThreads), 0, s, p, step);
         } else {
             hipLaunchKernelGGL(rt::learned_weights_kernel<false>, dim3(net::kWeightBlocks), dim3(256), 0, s, p, step);
-            hipLaunchKernelGGL(rt::learned_grad_kernel<false>, dim3(kNetWaves), dim3(64), 0, s, p, step);
+            hipLaunchKernelGGL(rt::learned_grad_kernel<false>, dim3(kGradGroups), dim3(net::kGradThreads), 0, s, p, step);
         }
         hipLaunchKernelGGL(rt::learned_adam_kernel, dim3(net::kAdamBlocks), dim3(256), 0, s, p);
     }
