@@ -3,24 +3,24 @@
 // a mixture of four von Mises-Fisher lobes over incident directions, the same
 // distribution family the grid guide (render_guide.h) stores per cell. The
 // matrix products run on the CDNA4 matrix cores (v_mfma_f32_16x16x32_f16:
-// FP16 operands, FP32 accumulation, K = 32), 16 points per product,
-// activations kept in registers between layers. No reference counterpart: the reference guides
-// with the grid only; this network is trained online from the same path
-// records the grid learns from, and judged against the grid.
+// FP16 operands, FP32 accumulation, K = 32), 16 points per product. No
+// reference counterpart: the reference guides with the grid only; this
+// network is trained online from the same path records the grid learns
+// from, and judged against the grid.
 //
-// Fragment layouts (64 lanes; lane = c + 16 g), checked on the device:
-//   v_mfma_f32_16x16x32_f16  A[i][k] (16x32): lane i + 16 (k / 8), element k % 8
-//                            B[k][j] (32x16): lane j + 16 (k / 8), element k % 8
-//   v_mfma_f32_16x16x16_f16  A[i][k] (16x16): lane i + 16 (k / 4), element k % 4
-//                            B[k][j] (16x16): lane j + 16 (k / 4), element k % 4
-//   both                     D[i][j] (16x16): lane j + 16 (i / 4), element i % 4
+// Fragment layouts of v_mfma_f32_16x16x32_f16 (64 lanes; lane = c + 16 g),
+// checked on the device:
+//   A[i][k] (16x32): lane i + 16 (k / 8), element k % 8
+//   B[k][j] (32x16): lane j + 16 (k / 8), element k % 8
+//   D[i][j] (16x16): lane j + 16 (i / 4), element i % 4
 // A layer computes Z^T = W^T H^T over 16 points (columns): A = W^T blocks,
 // B = H^T. A D tile holds rows 4g..4g+3 in lane group g, so two D tiles
 // (rows 0..15 and 16..31 of a 32-row block) side by side are a K = 32 B
 // fragment whose element e of lane group g is row 4g + e (e < 4) or
-// 16 + 4g + e - 4: the A blocks are packed in that order of K, and
-// activations never leave their lanes. The weight gradient (16 samples as
-// the inner dimension) keeps the K = 16 product.
+// 16 + 4g + e - 4: the A blocks are packed in that order of K, so in
+// inference activations never leave their lanes (training's waves share
+// them through LDS and build the same operands from it). The weight
+// gradient takes 32 samples as the inner dimension of the K = 32 product.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -98,9 +98,6 @@ __device__ __forceinline__ half8_t pair(half4_t lo, half4_t hi) {
     return half8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 }
 
-__device__ __forceinline__ float4_t mfma(half4_t a, half4_t b, float4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
 __device__ __forceinline__ float4_t mfma32(half8_t a, half8_t b, float4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
