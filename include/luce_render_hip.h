@@ -189,6 +189,17 @@ int lr_probe_device(const lr_gpu_scene *scene, const float frame[16], uint32_t w
 int lr_trace_closest_device(const lr_gpu_scene *scene, const float *rays_device, size_t n,
                             uint32_t *hits_device, void *stream);
 
+/* lr_probe_device of a frame and lr_trace_closest_device of n rays in one
+ * call: no reference counterpart. For callers that trace a frame's primary
+ * rays and a batch together (a renderer's camera and bounce rays): with the
+ * scene's refill above 0, one persistent launch takes the frame's 8x8 tiles
+ * and then the batch's rays from one counter, so the two share one ramp-up
+ * and one drain; with refill 0, the two launches. The same hits as the two
+ * calls, bit for bit. */
+int lr_trace_frame_and_closest_device(const lr_gpu_scene *scene, const float frame[16], uint32_t width,
+                                      uint32_t height, uint32_t *frame_hits_device, const float *rays_device,
+                                      size_t n, uint32_t *hits_device, void *stream);
+
 /* Whether anything blocks each of n device-resident rays before its t_max
  * (trace_any, bvh.glsl:93-122): occluded[i] = 1 or 0. */
 int lr_trace_any_device(const lr_gpu_scene *scene, const float *rays_device, size_t n,

@@ -499,6 +499,25 @@ int lr_trace_closest_device(const lr_gpu_scene *scene_c, const float *rays, size
     return 0;
 }
 
+int lr_trace_frame_and_closest_device(const lr_gpu_scene *scene_c, const float frame[16], uint32_t width,
+                                      uint32_t height, uint32_t *frame_hits, const float *rays, size_t n,
+                                      uint32_t *hits, void *stream) {
+    lr_gpu_scene *scene = const_cast<lr_gpu_scene *>(scene_c);
+    if (!scene || !frame || !frame_hits || (n && (!rays || !hits)))
+        return fail(LR_INVALID, "lr_trace_frame_and_closest_device needs its arguments");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(LR_INVALID, "an image is 1..16384 pixels on a side");
+    if (n > 0xffffff00ull) return fail(LR_LIMIT, "at most 2^32 - 256 rays a call");
+    int rc = scene->ensure_spill(launch_threads(probe_items(width, height) + n, scene->config.group), scene->config.stack);
+    if (rc) return rc;
+    int e = scene->variant().frame_closest(scene->device_scene(), frame, width, height,
+                                           reinterpret_cast<uint2 *>(frame_hits), reinterpret_cast<const float4 *>(rays),
+                                           (uint32_t)n, reinterpret_cast<uint2 *>(hits), scene->config.refill,
+                                           (hipStream_t)stream);
+    if (e) return hip_fail(e, "frame and closest launch");
+    return 0;
+}
+
 int lr_trace_resident_grid(const lr_gpu_scene *scene, size_t n, int any, uint32_t *blocks) {
     if (!scene || !blocks) return fail(LR_INVALID, "lr_trace_resident_grid needs a scene and an output");
     (void)hipSetDevice(scene->device->ordinal);

@@ -15,6 +15,10 @@ struct TraceVariant {
                  uint32_t refill, hipStream_t stream);
     int (*closest)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint2 *out, uint32_t refill,
                    hipStream_t stream);
+    // A frame's primary rays (as probe) and a ray batch (as closest) in one
+    // persistent launch when refill is above 0, else the two launches.
+    int (*frame_closest)(const DeviceScene &sc, const float *frame, uint32_t width, uint32_t height, uint2 *frame_out,
+                         const float4 *rays, uint32_t n, uint2 *out, uint32_t refill, hipStream_t stream);
     int (*any)(const DeviceScene &sc, const float4 *rays, uint32_t n, uint32_t *out, uint32_t refill,
                hipStream_t stream);
     // Workgroups a persistent (refilled) launch over `items` rays takes on

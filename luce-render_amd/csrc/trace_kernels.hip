@@ -142,6 +142,55 @@ __global__ __launch_bounds__(G) void closest_stream_kernel(DeviceScene sc, const
     stream_rays<G>(sc, walk, src, n, refill);
 }
 
+// A frame's primary rays, then a batch of rays, as one stream of items: the
+// first `primary` items (probe_items(width, height)) are the frame's 8x8
+// pixel tiles, 64 items each, in the probe kernel's order (items past the
+// image edge end at once, storing nothing), the rest the batch's rays. One
+// persistent launch traces both, so the two share one ramp-up and one
+// drain; each ray's walk and result are the probe's or the batch's own.
+struct FrameThenBatch {
+    Frame f;
+    uint32_t width, height, tiles_x, primary;
+    uint2 *frame_out;
+    BatchClosest batch;
+    __device__ __forceinline__ bool pixel(uint32_t r, uint32_t &x, uint32_t &y) const {
+        uint32_t tile = r >> 6, lane = r & 63u;
+        x = (tile % tiles_x) * 8u + (lane & 7u);
+        y = (tile / tiles_x) * 8u + (lane >> 3);
+        return x < width && y < height;
+    }
+    template <class Walk> __device__ __forceinline__ bool begin(const DeviceScene &sc, Walk &w, uint32_t r) {
+        if (r >= primary) return batch.begin(sc, w, r - primary);
+        uint32_t x, y;
+        if (!pixel(r, x, y)) return false;
+        float ox, oy, oz, dx, dy, dz;
+        probe_ray(f, x, y, ox, oy, oz, dx, dy, dz);
+        if (w.begin(sc, ox, oy, oz, dx, dy, dz, 3.0e38f)) return true;
+        frame_out[(size_t)y * width + x] = make_uint2(kNone, __float_as_uint(w.hit.t));
+        return false;
+    }
+    template <class Walk> __device__ __forceinline__ void finish(const DeviceScene &sc, Walk &w, uint32_t r) {
+        if (r >= primary) {
+            batch.finish(sc, w, r - primary);
+            return;
+        }
+        uint32_t x, y;
+        pixel(r, x, y);
+        uint32_t face = w.hit.slot == kNone ? kNone : sc.faces[w.hit.slot];
+        frame_out[(size_t)y * width + x] = make_uint2(face, __float_as_uint(w.hit.t));
+    }
+};
+
+template <int W, int S, int G>
+__global__ __launch_bounds__(G) void frame_closest_stream_kernel(DeviceScene sc, FrameThenBatch src, uint32_t n,
+                                                                 uint32_t refill) {
+    __shared__ uint32_t lds[S * G];
+    uint32_t thread = blockIdx.x * G + threadIdx.x;
+    ClosestWalk<W, S> walk;
+    walk.stack = LaneStack<S>{lds + threadIdx.x, sc.spill + thread, sc.spill_stride, 0};
+    stream_rays<G>(sc, walk, src, n, refill);
+}
+
 template <int W, int S, int G>
 __global__ __launch_bounds__(G) void any_stream_kernel(DeviceScene sc, const float4 *rays, uint32_t n, uint32_t *out,
                                                        uint32_t refill) {
@@ -244,6 +293,29 @@ template <int W, int S, int G> int launch_any_t(const DeviceScene &scene, const 
     return (int)hipGetLastError();
 }
 
+// The frame's primary rays and the batch in one persistent launch (refill
+// above 0 and the scene's counters), or the probe then the batch.
+template <int W, int S, int G>
+int launch_frame_closest_t(const DeviceScene &scene, const float *frame, uint32_t w, uint32_t h, uint2 *frame_out,
+                           const float4 *rays, uint32_t n, uint2 *out, uint32_t refill, hipStream_t stream) {
+    uint64_t primary = probe_items(w, h), items = primary + n;
+    if (!refill || !scene.work || items >= 0x80000000ull) {
+        int e = launch_probe_t<W, S, G>(scene, frame, w, h, frame_out, refill, stream);
+        return e ? e : launch_closest_t<W, S, G>(scene, rays, n, out, refill, stream);
+    }
+    FrameThenBatch src;
+    for (int k = 0; k < 16; k++) src.f.v[k] = frame[k];
+    src.width = w, src.height = h, src.tiles_x = (w + 7u) / 8u, src.primary = (uint32_t)primary;
+    src.frame_out = frame_out;
+    src.batch = BatchClosest{rays, out};
+    uint32_t blocks = resident_blocks((const void *)frame_closest_stream_kernel<W, S, G>, G, items);
+    DeviceScene sc = scene;
+    sc.spill_stride = blocks * G;
+    hipLaunchKernelGGL((frame_closest_stream_kernel<W, S, G>), dim3(blocks), dim3(G), 0, stream, sc, src,
+                       (uint32_t)items, refill);
+    return (int)hipGetLastError();
+}
+
 template <int W, int S, int G> uint32_t resident_t(uint64_t items, int any) {
     return any ? resident_blocks((const void *)any_stream_kernel<W, S, G>, G, items)
                : resident_blocks((const void *)closest_stream_kernel<W, S, G>, G, items);
@@ -254,6 +326,7 @@ template <int W, int S, int G> void fill(TraceVariant &v) {
     v.closest_persistent = &launch_closest_persistent_t<W, S, G>;
     v.probe = &launch_probe_t<W, S, G>;
     v.closest = &launch_closest_t<W, S, G>;
+    v.frame_closest = &launch_frame_closest_t<W, S, G>;
     v.any = &launch_any_t<W, S, G>;
 }
 

@@ -112,6 +112,8 @@ _SIGNATURES = {
     "lr_scene_get_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
     "lr_probe_device": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P]),
     "lr_trace_closest_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
+    "lr_trace_frame_and_closest_device": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P,
+                                                         ctypes.c_size_t, _P, _P]),
     "lr_trace_any_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),
     "lr_trace_resident_grid": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int, _P]),
     "lr_trace_closest_device_noalloc": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P, _P]),

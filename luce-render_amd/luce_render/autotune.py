@@ -22,8 +22,9 @@ def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks
     probe of `frame` too when given, for each variant; returns the fastest
     (width, group, stack, refill) and {variant: ms per launch}. A dict
     `split` receives the two calls' own tables under "trace" and "probe"
-    (the probe takes one ray per thread whatever the refill), so a caller
-    can give each its fastest variant."""
+    (the probe takes one ray per thread whatever the refill) and, with a
+    frame, under "fused" the time of both in one call
+    (trace_frame_and_closest_device)."""
     stream = render.device_stream(device)
     own = hits is None
     if own:
@@ -44,9 +45,14 @@ def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks
                         def probe():
                             render.probe_device(scene, frame, probe_hits, stream)
 
+                        def fused():
+                            render.trace_frame_and_closest_device(scene, frame, probe_hits, rays, n, hits, stream)
+
                         ms = {}
-                        for name, call in (("trace", trace), ("probe", probe)):
-                            if name == "probe" and frame is None:
+                        for name, call in (("trace", trace), ("probe", probe), ("fused", fused)):
+                            if name != "trace" and frame is None:
+                                continue
+                            if name == "fused" and split is None:
                                 continue
                             call()  # first launch: spill area and code load
                             render.synchronize(device)
@@ -57,7 +63,7 @@ def tune(device, scene, rays, n, hits=None, widths=WIDTHS, groups=GROUPS, stacks
                             ms[name] = timer.elapsed_ms() / repeats
                             if split is not None:
                                 split.setdefault(name, {})[(w, g, s, r)] = ms[name]
-                        table[(w, g, s, r)] = sum(ms.values())
+                        table[(w, g, s, r)] = ms["trace"] + ms.get("probe", 0.0)
     finally:
         timer.destroy()
         if own:

@@ -343,6 +343,20 @@ def trace_closest_device(scene: GpuScene, rays: Buffer, n, hits: Buffer, stream=
     _lib.check(_lib.lib().lr_trace_closest_device(scene.handle, rays.ptr, int(n), hits.ptr, stream))
 
 
+def trace_frame_and_closest_device(scene: GpuScene, frame: RayFrame, frame_hits: Buffer, rays: Buffer, n,
+                                   hits: Buffer, stream=None):
+    """probe_device of `frame` and trace_closest_device of n rays in one
+    call: with the scene's refill above 0 one persistent launch traces both
+    (one ramp-up, one drain); the same hits as the two calls."""
+    packed = frame.pack()
+    if stream is None:
+        stream = device_stream(scene.device)
+    if frame_hits.nbytes < frame.width * frame.height * 8 or rays.nbytes < n * 32 or hits.nbytes < n * 8:
+        raise LuceError(INVALID, "a frame hit, ray or hit buffer too small")
+    _lib.check(_lib.lib().lr_trace_frame_and_closest_device(scene.handle, _lib.ptr(packed), frame.width, frame.height,
+                                                            frame_hits.ptr, rays.ptr, int(n), hits.ptr, stream))
+
+
 def trace_closest_device_noalloc(scene: GpuScene, rays: Buffer, n, hits: Buffer, stream=None):
     """trace_closest_device without allocating device memory: the library's
     fixed scratch array holds the traversal's spill area and counters (calls

@@ -169,6 +169,47 @@ def test_persistent_launches_back_to_back(device, cornell_probe, probe_scene):
     probe_scene.set_trace_config(4, 128, 32)
 
 
+@pytest.mark.parametrize("size", [(256, 256), (100, 37)])
+def test_frame_and_batch_in_one_launch(device, cornell_probe, probe_scene, size):
+    """lr_trace_frame_and_closest_device: a frame's primary rays and a ray
+    batch from one counter (refilled persistent waves) give the probe's and
+    the batch call's hits bit for bit, for every layout and refill (0: the
+    two launches), a frame whose tiles overhang its edges, and batches down
+    to none; nothing outside the outputs is written."""
+    from luce_render import render
+    sc = cornell_probe
+    w, h = size
+    frame = render.ray_frame(sc.cameras[0], w, h)
+    mf, mt, _, _ = _mirror(sc, frame)
+    n = 7000
+    o, d = _random_rays(n, 33)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7] = o, 3.0e38, d
+    bf, bt, _ = O.trace_rays(sc.nodes, sc.positions, sc.indices, sc.faces, np.concatenate([o, d], 1))
+    rb = render.Buffer.create(device, rays)
+    hb = render.Buffer(device, (n + 16) * 8)
+    fb = render.Buffer(device, (w * h + 16) * 8)
+    for width in WIDTHS:
+        for refill in REFILLS:
+            for count in (n, 1, 0):
+                probe_scene.set_trace_config(width, 64, 16, refill)
+                say(f"frame {w}x{h} and batch of {count}: width {width} refill {refill}")
+                hb.fill(0xAB)
+                fb.fill(0xAB)
+                render.trace_frame_and_closest_device(probe_scene, frame, fb, rb, count, hb)
+                render.synchronize(device)
+                f = fb.read(np.uint32).reshape(-1, 2)
+                assert np.array_equal(f[: w * h, 0], mf), int((f[: w * h, 0] != mf).sum())
+                assert np.array_equal(f[: w * h, 1], mt.view(np.uint32))
+                assert (f[w * h:] == 0xABABABAB).all()
+                r = hb.read(np.uint32).reshape(-1, 2)
+                assert np.array_equal(r[:count, 0], bf[:count]) and np.array_equal(r[:count, 1], bt[:count].view(np.uint32))
+                assert (r[count:] == 0xABABABAB).all()
+    for b in (rb, hb, fb):
+        b.destroy()
+    probe_scene.set_trace_config(4, 128, 32)
+
+
 def test_empty_and_tiny_scenes(device):
     from luce_render import render
     empty = render.compile_scene(render.GeometrySet())
