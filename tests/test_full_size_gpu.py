@@ -1,7 +1,8 @@
 """Size-independent properties at the benchmark's full size (the instanced
 30.4M-triangle scene, 1920x1080): every traversal width and ray schedule
 and LDS stack (16, the headline configuration's, and 24) returns the same
-polygon ids and distance bits on all 2,073,600 primary and bounce rays; any-hit occlusion equals "a closest hit exists before t_max";
+polygon ids and distance bits on all 2,073,600 primary and bounce rays,
+the frame-and-batch launch the same as the two calls; any-hit occlusion equals "a closest hit exists before t_max";
 the closest hits of a sampled subset equal the oracle's f32 mirror; the
 path-traced render at that size against the f64 oracle on sampled pixels."""
 import numpy as np
@@ -35,6 +36,8 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
     hp = render.Buffer(device, n * 8)
     hb = render.Buffer(device, n * 8)
     ob = render.Buffer(device, n * 4)
+    fp = render.Buffer(device, n * 8)
+    fb = render.Buffer(device, n * 8)
     g.set_trace_config(4, 64, 24, 0)
     render.probe_device(g, frame, hp)
     render.synchronize(device)
@@ -50,8 +53,11 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
             render.probe_device(g, frame, hp)
             render.trace_closest_device(g, rb, n, hb)
             render.trace_any_device(g, rb, n, ob)
+            render.trace_frame_and_closest_device(g, frame, fp, rb, n, fb)
             render.synchronize(device)
             got = (hp.read(np.uint32), hb.read(np.uint32))
+            assert np.array_equal(fp.read(np.uint32), got[0]), (width, stack, refill)
+            assert np.array_equal(fb.read(np.uint32), got[1]), (width, stack, refill)
             occluded = ob.read(np.uint32)
             hits = got[1].reshape(n, 2)
             assert np.array_equal(occluded, (hits[:, 0] != 0xFFFFFFFF).astype(np.uint32))
@@ -69,7 +75,7 @@ def test_widths_and_schedules_agree_at_full_size(device, big):
         f, t, _ = O.trace_rays(sc.nodes, sc.positions, sc.indices, sc.faces, rays[sel][:, [0, 1, 2, 4, 5, 6]], tm)
         assert np.array_equal(hits[sel, 0], f)
         assert np.array_equal(hits[sel, 1], t.view(np.uint32))
-    for b in (hp, hb, ob, rb):
+    for b in (hp, hb, ob, rb, fp, fb):
         b.destroy()
     g.set_trace_config(20, 64, 16, 32)
 
